@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Grid Fock build throughput on one MI355X (config 2 of the nAPMO grid hot path).
+
+One step = one grid Fock build with P resident in HBM: rho = diag(Phi P Phi^T) (MFMA),
+Becke-Dickson Poisson (projection, FP64 radial Green solves, synthesis over all
+centres), F = Phi^T diag(w bw v) Phi (MFMA) and the FP64 trace E = 1/2 sum P o F.
+Phi, the Becke weights and the Poisson plan are per-geometry set-up, outside the step.
+
+Workload (synthetic, seeded): 16 atoms {C,H,N,O} in a 12 bohr box, 50 radial x 590
+Lebedev points per atom (N = 472,000), 300 Cartesian s/p/d Gaussians, P = 2 C C^T from
+40 S-orthonormal orbitals, lmax = 12.
+
+Prints one JSON line. ``--gpus N`` with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {'fp64_mfma': 78.6e12, 'fp32_mfma': 157.3e12, 'bf16_mfma': 2.5e15, 'hbm': 8.0e12}
+METRIC = 'grid Fock builds/s + % roofline, mixed vs FP64 on 1 MI355X at |dE|<=1uEh'
+
+
+def gpu_leg(args, log):
+    import napmo_amd
+    from napmo_amd import workloads
+    from napmo_amd.device import DeviceArray
+    from napmo_amd.system import cext
+    cext.require_gpu()
+    atoms, shells, nrad, nang, lmax = workloads.config2()
+    t0 = time.perf_counter()
+    grid = napmo_amd.BeckeGrid(napmo_amd.make_species(atoms), nrad, nang)
+    basis = napmo_amd.make_basis(atoms, shells_per_atom=shells)
+    log('grid N=%d basis n=%d set-up %.1fs' % (grid.size, basis['nbasis'], time.perf_counter() - t0))
+    modes = ['fp64'] + [m for m in args.precision.split(',') if m and m != 'fp64']
+    plans = {}
+    for m in modes:
+        try:
+            plans[m] = napmo_amd.GridFock(grid, basis, lmax, m)
+        except cext.NapmoError as e:
+            log('precision %s unavailable: %s' % (m, e))
+    S = plans['fp64'].overlap()
+    P = workloads.density_matrix(np.random.default_rng(2), S, 40)
+    n = basis['nbasis']
+    dP = DeviceArray.from_host(P)
+    dF = DeviceArray((n, n))
+    dE = DeviceArray((2,))
+    results = {}
+    for m, plan in plans.items():
+        for _ in range(args.warmup):
+            plan.build_device(dP.ptr, dF.ptr, dE.ptr)
+        plan.synchronize()
+        plan.profile(args.steps)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.build_device(dP.ptr, dF.ptr, dE.ptr)
+        plan.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        phases, nb = plan.profile_read()
+        E = dE.to_host()
+        F = dF.to_host()
+        results[m] = {'ms_per_build': dt * 1e3, 'builds_per_s': 1.0 / dt, 'E': float(E[0]),
+                      'E_grid': float(E[1]), 'phases_ms': phases, 'F': F}
+        log('%s: %.3f ms/build  E=%.12f  phases=%s' % (m, dt * 1e3, E[0],
+                                                       {k: round(v, 4) for k, v in phases.items()}))
+    N = grid.size
+    return results, N, n, lmax, grid.ncenter
+
+
+def roofline(res, N, n, L, M):
+    """Roofline of the dominant kernel, algorithmic work per launch / measured time."""
+    ph = res['phases_ms']
+    name = max(ph, key=ph.get)
+    ms = ph[name]
+    if name in ('density', 'fock'):
+        flops = 2.0 * N * n * n
+        peak = PEAK['fp64_mfma']
+        return name, {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
+                      'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / peak, 'traffic': None}
+    if name == 'synth':
+        # 2 flops per (point, centre, lm) of the survey's count; FP64 vector peak
+        flops = 2.0 * N * M * L
+        peak = PEAK['fp64_mfma']
+        return name, {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
+                      'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / peak, 'traffic': None}
+    nbytes = 8.0 * N * 4
+    return name, {'bound': 'hbm', 'achieved': nbytes / (ms * 1e-3) / 1e9, 'peak': PEAK['hbm'] / 1e9,
+                  'unit': 'GB/s', 'frac': nbytes / (ms * 1e-3) / PEAK['hbm'], 'traffic': None}
+
+
+def cpu_baseline(budget_s, log):
+    """The oracle (C/OpenMP restatement of the reference hot path, Python orchestration
+    of the Poisson solves as napmo/grids/poisson_solver.py) timed per stage on a bounded
+    sample of the config-2 build and scaled to one build."""
+    from oracle import cext as ocx
+    from oracle import core
+    from napmo_amd import workloads
+    atoms, shells, nrad, nang, lmax = workloads.config2()
+    threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    grid = core.BeckeGrid(atoms, nrad, nang)
+    basis = core.Basis()
+    for (s, o), sh in zip(atoms, shells):
+        for shell in sh:
+            basis.add_shell(o, shell['l'], shell['prim'], shell['cont'])
+    N, n, M = grid.size, basis.nbasis, grid.ncenter
+    rng = np.random.default_rng(0)
+    ns = 16384
+    idx = np.sort(rng.choice(N, ns, replace=False))
+    pts = grid.points[idx]
+    phi = ocx.basis_compute(basis, pts)
+    P = rng.normal(size=(n, n))
+    P = P + P.T
+    t = time.perf_counter()
+    rho = ocx.density(phi, P)
+    t_density = (time.perf_counter() - t) * N / ns
+    # Poisson for one atom (projection + all (l, m) solves), scaled by the atom count
+    ag = grid.atgrids[0]
+    f = np.abs(rng.normal(size=ag.size))
+    t = time.perf_counter()
+    sph = core.spherical_expansion(ag, f, lmax)
+    U = core.poisson_radial(ag, sph, lmax)
+    t_poisson = (time.perf_counter() - t) * M
+    t = time.perf_counter()
+    out = np.zeros(ns)
+    ocx.eval_decomposition(U, grid.origin[0], pts, out)
+    t_synth = (time.perf_counter() - t) * M * N / ns
+    v = rng.normal(size=ns)
+    t = time.perf_counter()
+    ocx.coupling(np.ascontiguousarray(phi.T), v, grid.becke_weights[idx], grid.weights[idx])
+    t_fock = (time.perf_counter() - t) * N / ns
+    total = t_density + t_poisson + t_synth + t_fock
+    log('cpu baseline: density %.2fs poisson %.2fs synth %.2fs fock %.2fs (per build, %d threads, '
+        'sampled in %.1fs)' % (t_density, t_poisson, t_synth, t_fock, threads, time.perf_counter() - t0))
+    del rho
+    return {'value': 1.0 / total, 'unit': 'builds/s', 'cores': threads, 'kind': 'port',
+            'sample': 'oracle C/OpenMP stages on 16384 of 472000 points (density, synthesis over 1 of '
+                      '16 centres, contraction) and the full Poisson of 1 of 16 atoms, each scaled to '
+                      'one config-2 build; Becke/collocation set-up excluded as on the GPU',
+            'seconds_per_build': total}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--precision', default='fp64,bf16x3,fp32',
+                    help='comma list of modes to time besides fp64')
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--cpu-seconds', type=float, default=30.0)
+    args = ap.parse_args()
+
+    def log(msg):
+        print('[bench] ' + msg, file=sys.stderr, flush=True)
+
+    if args.gpus != 1:
+        print('bench.py: this build runs the grid hot path on one GPU; --gpus %d is not supported '
+              '(see the sharding design in DESIGN.md)' % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    results, N, n, lmax, M = gpu_leg(args, log)
+    L = (lmax + 1) ** 2
+    ref = results['fp64']
+    best, dE = 'fp64', 0.0
+    for m, r in results.items():
+        if m == 'fp64':
+            continue
+        r['dE'] = r['E'] - ref['E']
+        r['dF_max'] = float(np.max(np.abs(r['F'] - ref['F'])))
+        log('%s vs fp64: dE = %.3e Eh, max|dF| = %.3e Eh, speed-up %.2fx' %
+            (m, r['dE'], r['dF_max'], ref['ms_per_build'] / r['ms_per_build']))
+        if abs(r['dE']) <= 1e-6 and r['builds_per_s'] > results[best]['builds_per_s']:
+            best, dE = m, r['dE']
+    head = results[best]
+    kname, roof = roofline(head, N, n, L, M)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.cpu_seconds, log)
+    line = {
+        'metric': METRIC,
+        'value': head['builds_per_s'],
+        'unit': 'builds/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': head['ms_per_build'],
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': {'fp64': 'f64', 'fp32': 'f32+f64', 'bf16x3': 'bf16x3+f32+f64'}[best],
+        'data': 'synthetic (seeded geometry, basis and density; no reference data files)',
+        'config': {'workload': 'config 2: grid Fock build, 16 atoms, 50x590 Becke grid (472,000 points), '
+                               '300 s/p/d Gaussians, lmax 12, P resident in HBM',
+                   'npoint': N, 'nbasis': n, 'ncenter': M, 'lmax': lmax, 'precision': best},
+        'roofline': dict(roof, kernel=kname),
+        'cpu_baseline': cpu,
+        'modes': {m: {'builds_per_s': r['builds_per_s'], 'ms_per_build': r['ms_per_build'],
+                      'E': r['E'], 'dE_vs_fp64': r.get('dE', 0.0), 'phases_ms': r['phases_ms']}
+                  for m, r in results.items()},
+        'dE_vs_fp64': dE,
+        'speedup_vs_fp64': ref['ms_per_build'] / head['ms_per_build'],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,203 @@
+/* napmo_gpu.h -- C ABI of libnapmo_gpu.so, the MI355X (gfx950) grid hot path of nAPMO.
+ *
+ * Two layers:
+ *
+ *  1. Drop-in symbols. Same names and C signatures as the reference library that
+ *     napmo/__init__.py:95-585 binds with ctypes, restricted to the grid hot path
+ *     (grids, radial transforms, splines, ODE assembly, Poisson synthesis, GTO
+ *     collocation, grid contractions). Each cites the reference declaration it
+ *     replaces. Host arrays in and out, as before; the arithmetic runs on the GPU.
+ *     Pointer-returning getters return borrowed views owned by the handle.
+ *     Callee-allocated results (AtomicGrid_spherical_expansion, AtomicGrid_integrate)
+ *     are released with napmo_free().
+ *
+ *  2. Device-resident entry points (napmo_*). Return int status (0 = ok) and never
+ *     throw or exit; napmo_last_error() returns the thread's last message.
+ *
+ * Every array is C-contiguous float64 unless stated. No torch types cross this ABI.
+ */
+#ifndef NAPMO_GPU_H
+#define NAPMO_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors, memory */
+
+/* Thread-local message of the last failed call ("" when none). */
+const char *napmo_last_error(void);
+/* Releases a callee-allocated result (the reference returns new[] buffers that
+ * Python never frees: src/grid/atomic_grid.cpp:68, 116). */
+void napmo_free(void *ptr);
+/* Number of visible GPUs (0 when none); never initialises a context on failure. */
+int napmo_device_count(void);
+
+/* ------------------------------------------------- radial transforms (rtransform.h) */
+/* napmo/__init__.py RTransform block; src/horton/rtransform.cpp:308-341, :257-283 */
+void *ChebyshevRTransform_new(double radii, int npoint);
+double ChebyshevRTransform_get_radii(void *rtf);
+double ChebyshevRTransform_get_z(void *rtf);
+void *PowerRTransform_new(double rmin, double rmax, int npoint);
+double PowerRTransform_get_rmin(void *rtf);
+double PowerRTransform_get_rmax(void *rtf);
+double PowerRTransform_get_power(void *rtf);
+void *IdentityRTransform_new(int npoint);
+int RTransform_get_npoint(void *rtf);
+double RTransform_radius(void *rtf, double t);
+double RTransform_deriv(void *rtf, double t);
+double RTransform_deriv2(void *rtf, double t);
+double RTransform_deriv3(void *rtf, double t);
+double RTransform_inv(void *rtf, double r);
+void RTransform_radius_array(void *rtf, double *t, double *r, int n);
+void RTransform_deriv_array(void *rtf, double *t, double *d, int n);
+void RTransform_deriv2_array(void *rtf, double *t, double *d, int n);
+void RTransform_deriv3_array(void *rtf, double *t, double *d, int n);
+void RTransform_inv_array(void *rtf, double *r, double *t, int n);
+void RTransform_del(void *rtf);
+
+/* ----------------------------------------------------------- grids (src/grid) */
+/* src/grid/angular.cpp:12-37 (Lebedev rule with lorder points) */
+void *AngularGrid_new(int lorder);
+int AngularGrid_get_lorder(void *grid);
+double *AngularGrid_get_points(void *grid);
+double *AngularGrid_get_weights(void *grid);
+double AngularGrid_integrate(void *grid, int nfunc, double *f);
+void AngularGrid_del(void *grid);
+/* src/grid/radial.cpp:14-51 */
+void *RadialGrid_new(void *rtf, double radii);
+int RadialGrid_get_size(void *grid);
+double RadialGrid_get_radii(void *grid);
+double *RadialGrid_get_points(void *grid);
+double *RadialGrid_get_weights(void *grid);
+double RadialGrid_integrate(void *grid, int segments, double *f);
+void RadialGrid_del(void *grid);
+/* src/grid/atomic_grid.cpp:12-136 */
+void *AtomicGrid_new(void *angular, void *radial, double *origin);
+int AtomicGrid_get_size(void *grid);
+double AtomicGrid_get_radii(void *grid);
+double *AtomicGrid_get_origin(void *grid);
+double *AtomicGrid_get_points(void *grid);
+double *AtomicGrid_get_weights(void *grid);
+/* Returns [nrad x (lmax+1)^2], release with napmo_free. GPU projection. */
+double *AtomicGrid_spherical_expansion(void *grid, double *f, int lmax);
+/* Returns [segments], release with napmo_free. */
+double *AtomicGrid_integrate(void *grid, int nfunc, int segments, int size, double *f);
+void AtomicGrid_del(void *grid);
+/* src/grid/becke.cpp:12-54, 79-190. Becke weights computed on the GPU. Atomic grids
+ * may have different sizes (the reference assumes equal sizes, becke.cpp:121). */
+void *BeckeGrid_new(void **atgrids, int ncenter, int ablmax, double abldep);
+int BeckeGrid_get_ncenter(void *grid);
+int BeckeGrid_get_size(void *grid);
+double *BeckeGrid_get_radii(void *grid);
+double *BeckeGrid_get_points(void *grid);
+double *BeckeGrid_get_weights(void *grid);
+double *BeckeGrid_get_origin(void *grid);
+double *BeckeGrid_get_becke_weights(void *grid);
+double BeckeGrid_integrate(void *grid, double *f);
+void BeckeGrid_del(void *grid);
+
+/* ------------------------------------------- splines, extrapolation, ODE (horton) */
+/* src/horton/cubic_spline.cpp:81-103 */
+void solve_cubic_spline_system(double *y, double *dt, int npoint);
+/* src/horton/cubic_spline.cpp:138-181; aliases y and dt (caller keeps them alive) */
+void *CubicSpline_new(double *y, double *dt, void *extrapolation, void *rtf, int n);
+void CubicSpline_eval(void *spline, double *new_x, double *new_y, int new_n);
+double CubicSpline_get_first_x(void *spline);
+double CubicSpline_get_last_x(void *spline);
+void CubicSpline_del(void *spline);
+/* src/horton/extrapolation.cpp:60-183 */
+void *PotentialExtrapolation_new(int l);
+void *ZeroExtrapolation_new(void);
+void *CuspExtrapolation_new(void);
+void *PowerExtrapolation_new(double power);
+void Extrapolation_del(void *extrapolation);
+/* src/horton/ode2.cpp:232-321: coeffs [2n x 2n] (+=, pre-zeroed) and rhs [2n] */
+void build_ode2(double *b, double *a, double *f, double *bcs, double *coeffs, double *rhs,
+                long npoint);
+/* src/horton/cell.cpp:407; only the non-periodic cell (nvec = 0) is supported */
+void *Cell_new(double *rvecs, int nvec);
+void Cell_del(void *cell);
+/* src/horton/evaluate.cpp:129-223: output[npoint] += sum_lm U_lm(d) Y_lm, on the GPU */
+void eval_decomposition_grid(void **splines, double *center, double *output, double *points,
+                             void *cell, long nspline, long npoint);
+
+/* ------------------------------------------------------------ Gaussian basis (gto.h) */
+/* src/system/gto.cpp:17-51, 161-227; src/system/basis_set.cpp:12-82 */
+void *PrimitiveGaussian_new(int *l, double *origin, double zeta, double coeff);
+double PrimitiveGaussian_get_norma(void *p);
+void *ContractedGaussian_new(void **primitives, int nprim);
+double ContractedGaussian_get_norma(void *c);
+void *BasisSet_new(void **contractions, int n);
+void *BasisSet_new_empty(void);
+void BasisSet_update(void *basis, void *other);
+int BasisSet_get_nbasis(void *basis);
+/* out [npoint x nbasis] point-major; GPU collocation */
+void BasisSet_compute(void *basis, double *r, double *out, int npoint);
+
+/* --------------------------------------------- grid contractions (nwavefunction.cpp) */
+/* WaveFunction: the by-reference struct of src/wavefunction/wavefunction.h:20-44
+ * (== PSIB, napmo/wavefunction/psi_base.py:22-45). */
+typedef struct WaveFunction {
+  double *S, *X, *T, *V, *H, *C, *D, *L, *G, *J, *XC, *F, *O;
+  int nbasis, ndim, occupation;
+  double eta, kappa, x_factor, xc_energy, energy, rmsd;
+} WaveFunction;
+/* src/wavefunction/nwavefunction.cpp:15-75: psi->G = J-part + x_factor * K-part.
+ * phi and K are function-major [ndim x N], J [N]. */
+void nwavefunction_compute_2body_matrix(WaveFunction *psi, void *grid, double *phi, double *J,
+                                        double *K);
+/* nwavefunction.cpp:77-107: res[ndim x ndim] = sum_p phi_i phi_j otherJ bw w */
+void nwavefunction_compute_coupling(WaveFunction *psi, void *grid, double *phi, double *other_J,
+                                    double *res);
+/* nwavefunction.cpp:161-197: psi->XC = sum_p phi_i phi_j vrho bw w */
+void nwavefunction_compute_xc_matrix(WaveFunction *psi, void *grid, double *phi, double *xc_vrho);
+
+/* ------------------------------------------------------- device-resident hot path */
+/* Precision of the grid Fock contractions. */
+enum {
+  NAPMO_PREC_FP64 = 0,   /* everything float64 (FP64 MFMA) */
+  NAPMO_PREC_FP32 = 1,   /* Phi and contractions float32 (FP32 MFMA), FP64 Poisson + traces */
+  NAPMO_PREC_BF16X3 = 2  /* 3-term bf16 split on BF16 MFMA, FP32 accumulate, FP64 Poisson */
+};
+
+/* Poisson: dens [N] (grid order) -> potential v [N] (napmo/wavefunction/ntwobody.py:15-46). */
+int napmo_compute_coulomb(void *grid, const double *dens, int lmax, double *v_out);
+/* Poisson, expansion only (napmo/grids/poisson_solver.py:18-118): for every centre A and
+ * lm, the potential spline on A's radial grid: U_y, U_dt [sum_A nrad_A x L] (atom-major,
+ * then radial, then lm). */
+int napmo_poisson_solve(void *grid, const double *dens, int lmax, double *U_y, double *U_dt);
+
+/* Grid Fock plan: device-resident Phi (collocated once), Poisson plan, workspace. */
+int napmo_fock_plan_new(void *grid, void *basis, int lmax, int prec_mode, void **plan_out);
+void napmo_fock_plan_del(void *plan);
+int napmo_fock_plan_info(void *plan, long *npoint, int *nbasis, int *ncenter, int *lmax);
+/* One grid Fock build from host P [n x n]: rho = diag(Phi P Phi^T), v = Poisson(rho),
+ * F = Phi^T diag(w bw v) Phi [n x n], E = 1/2 sum P o F (FP64 trace). Any output may be
+ * NULL. rho_out / v_out [N] on request. */
+int napmo_grid_fock_build(void *plan, const double *P, double *F_out, double *E_out,
+                          double *rho_out, double *v_out);
+/* Same on device pointers, asynchronous on the plan's stream (no host sync). */
+int napmo_grid_fock_build_device(void *plan, const double *dP, double *dF, double *dE);
+/* Grid overlap S = Phi^T diag(w bw) Phi [n x n] from the plan's resident Phi (FP64).
+ * Same contraction as nwavefunction_compute_coupling with other_J = 1. */
+int napmo_grid_overlap(void *plan, double *S_out);
+/* Per-phase HIP-event timing of the next max_builds builds on the plan's stream; read
+ * back the average milliseconds of each phase (napmo_fock_phase_name(k) names them). */
+int napmo_fock_plan_profile(void *plan, int max_builds);
+int napmo_fock_plan_profile_read(void *plan, double *avg_ms, int nphase, int *nbuild);
+const char *napmo_fock_phase_name(int k);
+/* Device memory for callers without a GPU framework (kind 1 H2D, 2 D2H, 3 D2D). */
+void *napmo_device_malloc(size_t bytes);
+void napmo_device_free(void *ptr);
+int napmo_memcpy(void *dst, const void *src, size_t bytes, int kind);
+/* Plan's HIP stream (hipStream_t) and a full device synchronisation of it. */
+void *napmo_fock_plan_stream(void *plan);
+int napmo_fock_plan_sync(void *plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NAPMO_GPU_H */

@@ -1,0 +1,501 @@
+// extern "C" surface of libnapmo_gpu.so (declared in include/napmo_gpu.h).
+#include "napmo_gpu.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+
+#include "common.hpp"
+#include "fock_plan.hpp"
+#include "host.hpp"
+#include "kernels.hpp"
+#include "poisson_plan.hpp"
+
+namespace napmo {
+
+static thread_local std::string g_error;
+void set_error(const std::string &msg) { g_error = msg; }
+void clear_error() { g_error.clear(); }
+
+void require_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    throw Error("napmo_gpu: no HIP device available (this library has no CPU path)");
+}
+
+// Legacy entry points return values, not status codes: report and abort on failure
+// like the reference (whose C++ exceptions terminate), but with a readable message.
+template <class F> auto legacy(const char *name, F &&f) -> decltype(f()) {
+  try {
+    clear_error();
+    return f();
+  } catch (const std::exception &e) {
+    set_error(e.what());
+    std::fprintf(stderr, "napmo_gpu: %s: %s\n", name, e.what());
+    std::fflush(stderr);
+    std::abort();
+  }
+}
+
+// BeckeGrid handle: the device grid plus the atomic grids it was built from.
+struct BeckeHandle {
+  DeviceGrid grid;
+};
+
+static AtomInfo atom_info(const AtomicGrid &g) {
+  AtomInfo a;
+  std::memcpy(a.origin, g.origin, sizeof(a.origin));
+  a.radii = g.rad->radii;
+  a.nrad = (int)g.rad->points.size();
+  a.nang = g.ang->lorder;
+  a.rtf = *g.rad->rtf;
+  a.r = g.rad->points;
+  a.rw = g.rad->weights;
+  a.u = g.ang->points;
+  a.aw = g.ang->weights;
+  return a;
+}
+
+// Host points (AoS [n x 3]) to device SoA.
+struct DevicePoints {
+  DeviceBuffer<double> x, y, z;
+  void upload(const double *pts, long n, hipStream_t s) {
+    std::vector<double> hx(n), hy(n), hz(n);
+    for (long i = 0; i < n; ++i) hx[i] = pts[3 * i], hy[i] = pts[3 * i + 1], hz[i] = pts[3 * i + 2];
+    x.upload(hx, s), y.upload(hy, s), z.upload(hz, s);
+    NAPMO_HIP(hipStreamSynchronize(s));
+  }
+};
+
+// A stream for one-shot legacy calls that are not tied to a grid handle.
+static hipStream_t scratch_stream() {
+  static thread_local hipStream_t s = nullptr;
+  if (!s) {
+    require_device();
+    NAPMO_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  }
+  return s;
+}
+
+// F = Phi^T diag(s) Y for host function-major Phi / Y [n x N] (legacy contractions).
+static void host_contract(const DeviceGrid &g, const double *phi, const double *yfm,
+                          const double *vec, int n, bool upper_mirror, double scale, double *out) {
+  hipStream_t st = g.stream;
+  const long N = g.N, Npad = round_up(N, TILE);
+  const int ld = (int)round_up(n, TILE);
+  auto to_point_major = [&](const double *fm, DeviceBuffer<double> &dst) {
+    std::vector<double> h((size_t)Npad * ld, 0.0);
+    for (int i = 0; i < n; ++i)
+      for (long p = 0; p < N; ++p) h[(size_t)p * ld + i] = fm[(size_t)i * N + p];
+    dst.upload(h, st);
+  };
+  DeviceBuffer<double> X, Y, v, s, part, F;
+  to_point_major(phi, X);
+  if (yfm) to_point_major(yfm, Y);
+  s.resize(Npad);
+  if (vec) {
+    v.upload(vec, N, st);
+    launch_scale3(v.get(), g.wbw.get(), nullptr, N, Npad, s.get(), st);
+  } else {
+    launch_scale3(g.wbw.get(), g.wbw.get(), nullptr, 0, Npad, s.get(), st);  // zeros
+    launch_scale3(g.w.get(), g.bw.get(), nullptr, N, Npad, s.get(), st);
+  }
+  FockShape sh = fock_shape(ld, Npad, yfm == nullptr);
+  part.resize((size_t)sh.nchunk * sh.ntiles * TILE * TILE);
+  F.resize((size_t)n * n);
+  launch_fock_f64(sh, X.get(), yfm ? Y.get() : X.get(), s.get(), part.get(), st);
+  launch_fock_reduce(sh, part.get(), n, upper_mirror ? 1 : 0, scale, 0, F.get(), st);
+  F.download(out, (size_t)n * n, st);
+  NAPMO_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace napmo
+
+using namespace napmo;
+
+extern "C" {
+
+const char *napmo_last_error(void) { return g_error.c_str(); }
+void napmo_free(void *ptr) { std::free(ptr); }
+int napmo_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+// ------------------------------------------------------------- radial transforms
+void *ChebyshevRTransform_new(double radii, int npoint) {
+  return legacy(__func__, [&] { return (void *)make_chebyshev(radii, npoint); });
+}
+double ChebyshevRTransform_get_radii(void *r) { return ((RTransform *)r)->p0; }
+double ChebyshevRTransform_get_z(void *r) { return ((RTransform *)r)->p1; }
+void *PowerRTransform_new(double rmin, double rmax, int npoint) {
+  return legacy(__func__, [&] { return (void *)make_power(rmin, rmax, npoint); });
+}
+double PowerRTransform_get_rmin(void *r) { return ((RTransform *)r)->p0; }
+double PowerRTransform_get_rmax(void *r) { return ((RTransform *)r)->p2; }
+double PowerRTransform_get_power(void *r) { return ((RTransform *)r)->p1; }
+void *IdentityRTransform_new(int npoint) {
+  return legacy(__func__, [&] { return (void *)make_identity(npoint); });
+}
+int RTransform_get_npoint(void *r) { return ((RTransform *)r)->npoint; }
+double RTransform_radius(void *r, double t) { return ((RTransform *)r)->radius(t); }
+double RTransform_deriv(void *r, double t) { return ((RTransform *)r)->deriv(t); }
+double RTransform_deriv2(void *r, double t) { return ((RTransform *)r)->deriv2(t); }
+double RTransform_deriv3(void *r, double t) { return ((RTransform *)r)->deriv3(t); }
+double RTransform_inv(void *r, double x) { return ((RTransform *)r)->inv(x); }
+#define NAPMO_RTF_ARRAY(name, method)                                          \
+  void name(void *r, double *in, double *out, int n) {                         \
+    for (int i = 0; i < n; ++i) out[i] = ((RTransform *)r)->method(in[i]);     \
+  }
+NAPMO_RTF_ARRAY(RTransform_radius_array, radius)
+NAPMO_RTF_ARRAY(RTransform_deriv_array, deriv)
+NAPMO_RTF_ARRAY(RTransform_deriv2_array, deriv2)
+NAPMO_RTF_ARRAY(RTransform_deriv3_array, deriv3)
+NAPMO_RTF_ARRAY(RTransform_inv_array, inv)
+void RTransform_del(void *r) { delete (RTransform *)r; }
+
+// ------------------------------------------------------------------------ grids
+void *AngularGrid_new(int lorder) {
+  return legacy(__func__, [&] { return (void *)make_angular(lorder); });
+}
+int AngularGrid_get_lorder(void *g) { return ((AngularGrid *)g)->lorder; }
+double *AngularGrid_get_points(void *g) { return ((AngularGrid *)g)->points.data(); }
+double *AngularGrid_get_weights(void *g) { return ((AngularGrid *)g)->weights.data(); }
+double AngularGrid_integrate(void *g, int nfunc, double *f) {
+  // src/grid/angular.cpp: 4 pi sum_j prod_k f_k(j) aw_j
+  AngularGrid *a = (AngularGrid *)g;
+  double s = 0.0;
+  for (int j = 0; j < a->lorder; ++j) {
+    double t = f[j];
+    for (int k = 1; k < nfunc; ++k) t *= f[(size_t)k * a->lorder + j];
+    s += t * a->weights[j];
+  }
+  return s * 4.0 * M_PI;
+}
+void AngularGrid_del(void *g) { delete (AngularGrid *)g; }
+
+void *RadialGrid_new(void *rtf, double radii) {
+  return legacy(__func__, [&] { return (void *)make_radial((RTransform *)rtf, radii); });
+}
+int RadialGrid_get_size(void *g) { return (int)((RadialGrid *)g)->points.size(); }
+double RadialGrid_get_radii(void *g) { return ((RadialGrid *)g)->radii; }
+double *RadialGrid_get_points(void *g) { return ((RadialGrid *)g)->points.data(); }
+double *RadialGrid_get_weights(void *g) { return ((RadialGrid *)g)->weights.data(); }
+double RadialGrid_integrate(void *g, int segments, double *f) {
+  // src/grid/radial.cpp:31-51
+  RadialGrid *r = (RadialGrid *)g;
+  const size_t n = r->points.size();
+  double out = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    double t = f[i];
+    for (int k = 1; k < segments; ++k) t *= f[k * n + i];
+    out += t * r->weights[i];
+  }
+  return out;
+}
+void RadialGrid_del(void *g) { delete (RadialGrid *)g; }
+
+void *AtomicGrid_new(void *ang, void *rad, double *origin) {
+  return legacy(__func__,
+                [&] { return (void *)make_atomic((AngularGrid *)ang, (RadialGrid *)rad, origin); });
+}
+int AtomicGrid_get_size(void *g) { return ((AtomicGrid *)g)->size(); }
+double AtomicGrid_get_radii(void *g) { return ((AtomicGrid *)g)->rad->radii; }
+double *AtomicGrid_get_origin(void *g) { return ((AtomicGrid *)g)->origin; }
+double *AtomicGrid_get_points(void *g) { return ((AtomicGrid *)g)->points.data(); }
+double *AtomicGrid_get_weights(void *g) { return ((AtomicGrid *)g)->weights.data(); }
+void AtomicGrid_del(void *g) { delete (AtomicGrid *)g; }
+
+double *AtomicGrid_integrate(void *g, int nfunc, int segments, int size, double *f) {
+  // src/grid/atomic_grid.cpp:53-80: per segment sum of the product of nfunc arrays
+  const long total = (long)size * segments;
+  double *out = (double *)std::calloc(segments, sizeof(double));
+  for (int s = 0; s < segments; ++s)
+    for (int j = 0; j < size; ++j) {
+      const long i = (long)s * size + j;
+      double t = f[i];
+      for (int k = 1; k < nfunc; ++k) t *= f[k * total + i];
+      out[s] += t;
+    }
+  return out;
+}
+
+double *AtomicGrid_spherical_expansion(void *g, double *f, int lmax) {
+  // src/grid/atomic_grid.cpp:82-136 on the GPU (projection kernel on a one-centre grid)
+  return legacy(__func__, [&] {
+    NAPMO_REQUIRE(lmax >= 0, "lmax can not be negative.");
+    AtomicGrid *ag = (AtomicGrid *)g;
+    DeviceGrid dg;
+    dg.build({atom_info(*ag)});
+    DeviceBuffer<double> dens, ones;
+    dens.upload(f, dg.N, dg.stream);
+    // spherical_expansion projects f itself (no Becke weight): use bw == 1
+    std::vector<double> one(dg.N, 1.0);
+    dg.bw.upload(one, dg.stream);
+    PoissonPlan &pl = dg.plan(lmax);
+    AtomTables at = dg.tables(pl);
+    launch_project(at, dens.get(), dg.bw.get(), pl.yhat.get(), lmax, pl.rho_lm.get(), dg.stream);
+    const int L = (lmax + 1) * (lmax + 1);
+    double *out = (double *)std::malloc(sizeof(double) * dg.atoms[0].nrad * L);
+    pl.rho_lm.download(out, (size_t)dg.atoms[0].nrad * L, dg.stream);
+    NAPMO_HIP(hipStreamSynchronize(dg.stream));
+    return out;
+  });
+}
+
+void *BeckeGrid_new(void **atgrids, int ncenter, int ablmax, double abldep) {
+  (void)ablmax, (void)abldep;
+  return legacy(__func__, [&] {
+    std::vector<AtomInfo> atoms;
+    for (int i = 0; i < ncenter; ++i) atoms.push_back(atom_info(*(AtomicGrid *)atgrids[i]));
+    auto *h = new BeckeHandle;
+    h->grid.build(atoms);
+    return (void *)h;
+  });
+}
+int BeckeGrid_get_ncenter(void *g) { return ((BeckeHandle *)g)->grid.M; }
+int BeckeGrid_get_size(void *g) { return (int)((BeckeHandle *)g)->grid.N; }
+double *BeckeGrid_get_radii(void *g) { return ((BeckeHandle *)g)->grid.radii.data(); }
+double *BeckeGrid_get_points(void *g) { return ((BeckeHandle *)g)->grid.points.data(); }
+double *BeckeGrid_get_weights(void *g) { return ((BeckeHandle *)g)->grid.weights.data(); }
+double *BeckeGrid_get_origin(void *g) { return ((BeckeHandle *)g)->grid.origin.data(); }
+double *BeckeGrid_get_becke_weights(void *g) { return ((BeckeHandle *)g)->grid.becke.data(); }
+double BeckeGrid_integrate(void *g, double *f) {
+  // src/grid/becke.cpp:178-184
+  DeviceGrid &d = ((BeckeHandle *)g)->grid;
+  double s = 0.0;
+  for (long p = 0; p < d.N; ++p) s += f[p] * d.becke[p] * d.weights[p];
+  return s;
+}
+void BeckeGrid_del(void *g) { delete (BeckeHandle *)g; }
+
+// ----------------------------------------------------- splines, extrapolation, ODE
+void solve_cubic_spline_system(double *y, double *dt, int npoint) {
+  legacy(__func__, [&] {
+    spline_derivatives(y, dt, npoint);
+    return 0;
+  });
+}
+void *CubicSpline_new(double *y, double *dt, void *ext, void *rtf, int n) {
+  return legacy(__func__, [&] {
+    RTransform *r = (RTransform *)rtf;
+    NAPMO_REQUIRE(r && r->npoint == n, "CubicSpline: transform size mismatch");
+    auto *cs = new CubicSpline{(Extrapolation *)ext, r, r->radius(0), r->radius(n - 1), y, dt, n};
+    cs->ext->prepare(*cs);
+    return (void *)cs;
+  });
+}
+void CubicSpline_eval(void *s, double *x, double *out, int n) { ((CubicSpline *)s)->eval(x, out, n); }
+double CubicSpline_get_first_x(void *s) { return ((CubicSpline *)s)->first_x; }
+double CubicSpline_get_last_x(void *s) { return ((CubicSpline *)s)->last_x; }
+void CubicSpline_del(void *s) { delete (CubicSpline *)s; }
+
+void *PotentialExtrapolation_new(int l) {
+  return legacy(__func__, [&] {
+    NAPMO_REQUIRE(l >= 0, "The argument l cannot be negative.");
+    auto *e = new Extrapolation{EX_POTENTIAL};
+    e->l = l;
+    return (void *)e;
+  });
+}
+void *ZeroExtrapolation_new(void) { return new Extrapolation{EX_ZERO}; }
+void *CuspExtrapolation_new(void) { return new Extrapolation{EX_CUSP}; }
+void *PowerExtrapolation_new(double power) {
+  auto *e = new Extrapolation{EX_POWER};
+  e->power = power;
+  return e;
+}
+void Extrapolation_del(void *e) { delete (Extrapolation *)e; }
+
+void *Cell_new(double *rvecs, int nvec) {
+  (void)rvecs;
+  return legacy(__func__, [&] {
+    NAPMO_REQUIRE(nvec == 0, "only the non-periodic cell (nvec = 0) is supported");
+    return (void *)new int(0);
+  });
+}
+void Cell_del(void *c) { delete (int *)c; }
+
+void build_ode2(double *b, double *a, double *f, double *bcs, double *coeffs, double *rhs,
+                long npoint) {
+  legacy(__func__, [&] {
+    fe_assemble(b, a, f, bcs, coeffs, rhs, npoint);
+    return 0;
+  });
+}
+
+void eval_decomposition_grid(void **splines, double *center, double *output, double *points,
+                             void *cell, long nspline, long npoint) {
+  (void)cell;
+  legacy(__func__, [&] {
+    const long lmax = (long)std::sqrt((double)nspline) - 1;
+    NAPMO_REQUIRE((lmax + 1) * (lmax + 1) == nspline,
+                  "The number of splines does not match a well-defined lmax.");
+    CubicSpline **sp = (CubicSpline **)splines;
+    decomposition_on_points(sp, (int)nspline, (int)lmax, center, points, npoint, output);
+    return 0;
+  });
+}
+
+// ------------------------------------------------------------------- Gaussians
+void *PrimitiveGaussian_new(int *l, double *origin, double zeta, double coeff) {
+  return legacy(__func__, [&] { return (void *)make_primitive(l, origin, zeta, coeff); });
+}
+double PrimitiveGaussian_get_norma(void *p) { return ((Primitive *)p)->norma; }
+void *ContractedGaussian_new(void **prims, int n) {
+  return legacy(__func__, [&] { return (void *)make_contraction((Primitive **)prims, n); });
+}
+double ContractedGaussian_get_norma(void *c) { return ((Contraction *)c)->norma; }
+void *BasisSet_new(void **conts, int n) {
+  auto *b = new BasisSet;
+  for (int i = 0; i < n; ++i) b->cont.push_back(*(Contraction *)conts[i]);
+  return b;
+}
+void *BasisSet_new_empty(void) { return new BasisSet; }
+void BasisSet_update(void *basis, void *other) {
+  auto *b = (BasisSet *)basis;
+  for (auto &c : ((BasisSet *)other)->cont) b->cont.push_back(c);
+}
+int BasisSet_get_nbasis(void *basis) { return (int)((BasisSet *)basis)->cont.size(); }
+
+void BasisSet_compute(void *basis, double *r, double *out, int npoint) {
+  legacy(__func__, [&] {
+    hipStream_t st = scratch_stream();
+    FlatBasis fb = flatten(*(BasisSet *)basis);
+    DevicePoints pts;
+    pts.upload(r, npoint, st);
+    DeviceBasis db;
+    db.upload(fb, st);
+    DeviceBuffer<double> dout((size_t)npoint * fb.nbasis);
+    db.collocate(pts.x.get(), pts.y.get(), pts.z.get(), npoint, npoint, fb.nbasis, dout.get(),
+                 nullptr, st);
+    dout.download(out, (size_t)npoint * fb.nbasis, st);
+    NAPMO_HIP(hipStreamSynchronize(st));
+    return 0;
+  });
+}
+
+// --------------------------------------------------------- grid contractions
+void nwavefunction_compute_2body_matrix(WaveFunction *psi, void *grid, double *phi, double *J,
+                                        double *K) {
+  legacy(__func__, [&] {
+    DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    const int n = psi->ndim;
+    std::vector<double> G((size_t)n * n), E((size_t)n * n);
+    host_contract(g, phi, nullptr, J, n, false, 1.0, G.data());
+    if (psi->x_factor != 0.0) {
+      // exchange(i, j) = sum_p phi_i K_j bw w for i <= j, mirrored (nwavefunction.cpp:50-62)
+      host_contract(g, phi, K, nullptr, n, true, psi->x_factor, E.data());
+      for (size_t e = 0; e < G.size(); ++e) G[e] += E[e];
+    }
+    std::memcpy(psi->G, G.data(), sizeof(double) * n * n);
+    return 0;
+  });
+}
+
+void nwavefunction_compute_coupling(WaveFunction *psi, void *grid, double *phi, double *other_J,
+                                    double *res) {
+  legacy(__func__, [&] {
+    host_contract(((BeckeHandle *)grid)->grid, phi, nullptr, other_J, psi->ndim, false, 1.0, res);
+    return 0;
+  });
+}
+
+void nwavefunction_compute_xc_matrix(WaveFunction *psi, void *grid, double *phi, double *vrho) {
+  legacy(__func__, [&] {
+    host_contract(((BeckeHandle *)grid)->grid, phi, nullptr, vrho, psi->ndim, false, 1.0, psi->XC);
+    return 0;
+  });
+}
+
+// ----------------------------------------------------------- device-resident path
+int napmo_compute_coulomb(void *grid, const double *dens, int lmax, double *v_out) {
+  return guarded([&] {
+    DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    DeviceBuffer<double> d, v(g.N);
+    d.upload(dens, g.N, g.stream);
+    g.poisson_expand(d.get(), lmax);
+    g.poisson_synthesize(lmax, v.get());
+    v.download(v_out, g.N, g.stream);
+    NAPMO_HIP(hipStreamSynchronize(g.stream));
+  });
+}
+
+int napmo_poisson_solve(void *grid, const double *dens, int lmax, double *U_y, double *U_dt) {
+  return guarded([&] {
+    DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    DeviceBuffer<double> d;
+    d.upload(dens, g.N, g.stream);
+    g.poisson_expand(d.get(), lmax);
+    PoissonPlan &pl = g.plan(lmax);
+    pl.Uy.download(U_y, pl.Uy.size(), g.stream);
+    pl.Udt.download(U_dt, pl.Udt.size(), g.stream);
+    NAPMO_HIP(hipStreamSynchronize(g.stream));
+  });
+}
+
+int napmo_fock_plan_new(void *grid, void *basis, int lmax, int prec_mode, void **plan_out) {
+  return guarded([&] {
+    NAPMO_REQUIRE(grid && basis && plan_out, "napmo_fock_plan_new: null argument");
+    auto *p = new FockPlan(((BeckeHandle *)grid)->grid, *(BasisSet *)basis, lmax, prec_mode);
+    *plan_out = p;
+  });
+}
+void napmo_fock_plan_del(void *plan) { delete (FockPlan *)plan; }
+int napmo_fock_plan_info(void *plan, long *npoint, int *nbasis, int *ncenter, int *lmax) {
+  return guarded([&] {
+    FockPlan *p = (FockPlan *)plan;
+    if (npoint) *npoint = p->grid.N;
+    if (nbasis) *nbasis = p->n;
+    if (ncenter) *ncenter = p->grid.M;
+    if (lmax) *lmax = p->lmax;
+  });
+}
+int napmo_grid_fock_build(void *plan, const double *P, double *F_out, double *E_out,
+                          double *rho_out, double *v_out) {
+  return guarded([&] { ((FockPlan *)plan)->build_host(P, F_out, E_out, rho_out, v_out); });
+}
+int napmo_grid_fock_build_device(void *plan, const double *dP, double *dF, double *dE) {
+  return guarded([&] { ((FockPlan *)plan)->build_device(dP, dF, dE); });
+}
+int napmo_grid_overlap(void *plan, double *S_out) {
+  return guarded([&] { ((FockPlan *)plan)->overlap(S_out); });
+}
+int napmo_fock_plan_profile(void *plan, int max_builds) {
+  return guarded([&] { ((FockPlan *)plan)->profile_start(max_builds); });
+}
+int napmo_fock_plan_profile_read(void *plan, double *avg_ms, int nphase, int *nbuild) {
+  return guarded([&] {
+    int n = ((FockPlan *)plan)->profile_read(avg_ms, nphase);
+    if (nbuild) *nbuild = n;
+  });
+}
+const char *napmo_fock_phase_name(int k) { return FockPlan::phase_name(k); }
+void *napmo_device_malloc(size_t bytes) {
+  void *p = nullptr;
+  if (guarded([&] {
+        require_device();
+        NAPMO_HIP(hipMalloc(&p, bytes));
+      }) != 0)
+    return nullptr;
+  return p;
+}
+void napmo_device_free(void *p) { (void)hipFree(p); }
+int napmo_memcpy(void *dst, const void *src, size_t bytes, int kind) {
+  return guarded([&] {
+    NAPMO_REQUIRE(kind >= 1 && kind <= 3, "napmo_memcpy: kind 1 = H2D, 2 = D2H, 3 = D2D");
+    const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                                      : (kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    NAPMO_HIP(hipMemcpy(dst, src, bytes, k));
+  });
+}
+
+void *napmo_fock_plan_stream(void *plan) { return (void *)((FockPlan *)plan)->grid.stream; }
+int napmo_fock_plan_sync(void *plan) {
+  return guarded([&] { NAPMO_HIP(hipStreamSynchronize(((FockPlan *)plan)->grid.stream)); });
+}
+
+}  // extern "C"

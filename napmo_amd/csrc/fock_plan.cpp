@@ -1,0 +1,150 @@
+// Device-resident grid Fock build (see fock_plan.hpp).
+#include "fock_plan.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace napmo {
+
+void DeviceBasis::upload(const FlatBasis &fb, hipStream_t s) {
+  nbasis = fb.nbasis;
+  origin.upload(fb.origin, s);
+  norm.upload(fb.norm, s);
+  zeta.upload(fb.zeta, s);
+  coef.upload(fb.coef, s);
+  lxyz.upload(fb.lxyz, s);
+  first.upload(fb.first, s);
+  nprim.upload(fb.nprim, s);
+}
+
+void DeviceBasis::collocate(const double *x, const double *y, const double *z, long N, long Npad,
+                            int ld, double *out64, float *out32, hipStream_t s) const {
+  launch_collocate(x, y, z, N, Npad, nbasis, ld, origin.get(), lxyz.get(), first.get(),
+                   nprim.get(), norm.get(), zeta.get(), coef.get(), out64, out32, s);
+}
+
+FockPlan::FockPlan(DeviceGrid &g, const BasisSet &b, int lmax_, int prec_mode)
+    : grid(g), lmax(lmax_), prec(prec_mode) {
+  NAPMO_REQUIRE(prec >= 0 && prec <= 2, "unknown precision mode");
+  NAPMO_REQUIRE(prec == 0, "only the FP64 precision mode is available in this build");
+  FlatBasis fb = flatten(b);
+  n = fb.nbasis;
+  NAPMO_REQUIRE(n > 0, "empty basis");
+  ld = (int)round_up(n, TILE);
+  Npad = round_up(grid.N, TILE);
+  hipStream_t st = grid.stream;
+  basis.upload(fb, st);
+  phi64.resize((size_t)Npad * ld);
+  basis.collocate(grid.x.get(), grid.y.get(), grid.z.get(), grid.N, Npad, ld, phi64.get(), nullptr,
+                  st);
+  grid.plan(lmax);  // Poisson plan (Green matrices) built once
+  P.resize((size_t)ld * ld);
+  NAPMO_HIP(hipMemsetAsync(P.get(), 0, sizeof(double) * ld * ld, st));
+  rho_part.resize((size_t)(ld / TILE) * Npad);
+  rho.resize(grid.N);
+  v.resize(grid.N);
+  s.resize(Npad);
+  fsh = fock_shape(ld, Npad, true);
+  F_part.resize((size_t)fsh.nchunk * fsh.ntiles * TILE * TILE);
+  F.resize((size_t)n * n);
+  E.resize(2);
+  NAPMO_HIP(hipStreamSynchronize(st));
+}
+
+FockPlan::~FockPlan() {
+  for (auto e : prof_events) (void)hipEventDestroy(e);
+}
+
+const char *FockPlan::phase_name(int k) {
+  static const char *names[NPHASE] = {"density",      "rho_reduce", "project", "radial",
+                                      "synth_tables", "synth",      "scale",   "fock",
+                                      "fock_reduce",  "traces"};
+  return (k >= 0 && k < NPHASE) ? names[k] : "";
+}
+
+void FockPlan::mark(int slot, int phase) {
+  if (slot < 0) return;
+  NAPMO_HIP(hipEventRecord(prof_events[(size_t)slot * (NPHASE + 1) + phase], grid.stream));
+}
+
+void FockPlan::profile_start(int max_builds) {
+  for (auto e : prof_events) (void)hipEventDestroy(e);
+  prof_events.clear();
+  prof_cap = std::max(0, max_builds);
+  prof_used = 0;
+  prof_events.resize((size_t)prof_cap * (NPHASE + 1));
+  for (auto &e : prof_events) NAPMO_HIP(hipEventCreate(&e));
+}
+
+int FockPlan::profile_read(double *avg_ms, int nphase) {
+  NAPMO_HIP(hipStreamSynchronize(grid.stream));
+  const int k = std::min(nphase, NPHASE);
+  for (int p = 0; p < k; ++p) avg_ms[p] = 0.0;
+  for (int b = 0; b < prof_used; ++b)
+    for (int p = 0; p < k; ++p) {
+      float ms = 0.f;
+      NAPMO_HIP(hipEventElapsedTime(&ms, prof_events[(size_t)b * (NPHASE + 1) + p],
+                                    prof_events[(size_t)b * (NPHASE + 1) + p + 1]));
+      avg_ms[p] += ms;
+    }
+  if (prof_used)
+    for (int p = 0; p < k; ++p) avg_ms[p] /= prof_used;
+  return prof_used;
+}
+
+void FockPlan::build_device(const double *dP, double *dF, double *dE) {
+  hipStream_t st = grid.stream;
+  const int slot = (prof_used < prof_cap) ? prof_used++ : -1;
+  PoissonPlan &pl = grid.plan(lmax);
+  AtomTables at = grid.tables(pl);
+  // P (n x n) into the zero-padded ld x ld operand
+  NAPMO_HIP(hipMemcpy2DAsync(P.get(), sizeof(double) * ld, dP, sizeof(double) * n,
+                             sizeof(double) * n, n, hipMemcpyDeviceToDevice, st));
+  mark(slot, 0);
+  launch_density_f64(phi64.get(), ld, Npad, P.get(), rho_part.get(), st);
+  mark(slot, 1);
+  launch_rho_reduce(rho_part.get(), ld / TILE, Npad, grid.N, rho.get(), st);
+  mark(slot, 2);
+  launch_project(at, rho.get(), grid.bw.get(), pl.yhat.get(), lmax, pl.rho_lm.get(), st);
+  mark(slot, 3);
+  launch_radial(at, pl.rho_lm.get(), pl.green.get(), lmax, pl.Uy.get(), pl.Udt.get(), st);
+  mark(slot, 4);
+  launch_synth_tables(at, pl.Uy.get(), pl.Udt.get(), lmax, pl.T.get(), pl.ampL.get(),
+                      pl.ampR.get(), st);
+  mark(slot, 5);
+  launch_synth(at, grid.x.get(), grid.y.get(), grid.z.get(), grid.N, pl.T.get(), pl.ampL.get(),
+               pl.ampR.get(), lmax, 0, v.get(), st);
+  mark(slot, 6);
+  launch_scale3(v.get(), grid.wbw.get(), nullptr, grid.N, Npad, s.get(), st);
+  mark(slot, 7);
+  launch_fock_f64(fsh, phi64.get(), phi64.get(), s.get(), F_part.get(), st);
+  mark(slot, 8);
+  launch_fock_reduce(fsh, F_part.get(), n, 0, 1.0, 0, dF, st);
+  mark(slot, 9);
+  launch_traces(dP, dF, n, grid.wbw.get(), rho.get(), v.get(), grid.N, dE, st);
+  mark(slot, 10);
+}
+
+void FockPlan::overlap(double *S_out) {
+  hipStream_t st = grid.stream;
+  launch_scale3(grid.w.get(), grid.bw.get(), nullptr, grid.N, Npad, s.get(), st);
+  launch_fock_f64(fsh, phi64.get(), phi64.get(), s.get(), F_part.get(), st);
+  launch_fock_reduce(fsh, F_part.get(), n, 0, 1.0, 0, F.get(), st);
+  F.download(S_out, (size_t)n * n, st);
+  NAPMO_HIP(hipStreamSynchronize(st));
+}
+
+void FockPlan::build_host(const double *hP, double *F_out, double *E_out, double *rho_out,
+                          double *v_out) {
+  hipStream_t st = grid.stream;
+  DeviceBuffer<double> dP;
+  dP.upload(hP, (size_t)n * n, st);
+  build_device(dP.get(), F.get(), E.get());
+  if (F_out) F.download(F_out, (size_t)n * n, st);
+  if (E_out) E.download(E_out, 2, st);
+  if (rho_out) rho.download(rho_out, grid.N, st);
+  if (v_out) v.download(v_out, grid.N, st);
+  NAPMO_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace napmo

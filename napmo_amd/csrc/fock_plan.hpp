@@ -1,0 +1,51 @@
+// Device-resident grid Fock build: Phi collocated once per geometry, then per build
+// rho = diag(Phi P Phi^T), Poisson, F = Phi^T diag(w bw v) Phi, FP64 energy trace.
+#pragma once
+
+#include "napmo_gpu.h"
+#include "common.hpp"
+#include "host.hpp"
+#include "kernels.hpp"
+#include "poisson_plan.hpp"
+
+namespace napmo {
+
+struct DeviceBasis {
+  int nbasis = 0;
+  DeviceBuffer<double> origin, norm, zeta, coef;
+  DeviceBuffer<int> lxyz, first, nprim;
+  void upload(const FlatBasis &fb, hipStream_t s);
+  // Phi into out64 / out32 [Npad x ld] (either may be null).
+  void collocate(const double *x, const double *y, const double *z, long N, long Npad, int ld,
+                 double *out64, float *out32, hipStream_t s) const;
+};
+
+struct FockPlan {
+  DeviceGrid &grid;  // borrowed: the BeckeGrid handle must outlive the plan
+  int n, ld, lmax, prec;
+  long Npad;
+  DeviceBasis basis;
+  DeviceBuffer<double> phi64;          // [Npad x ld]
+  DeviceBuffer<float> phi32;           // [Npad x ld] (FP32 / BF16x3 modes)
+  DeviceBuffer<double> P, rho_part, rho, v, s, F_part, F, E;
+  FockShape fsh;
+
+  // Per-phase HIP-event timing of builds (on the plan's stream), for roofline reports.
+  static constexpr int NPHASE = 10;
+  static const char *phase_name(int k);
+  int prof_cap = 0, prof_used = 0;
+  std::vector<hipEvent_t> prof_events;  // prof_cap x (NPHASE + 1)
+  void mark(int build_slot, int phase);
+
+  FockPlan(DeviceGrid &g, const BasisSet &b, int lmax, int prec_mode);
+  ~FockPlan();
+  void build_device(const double *dP, double *dF, double *dE);
+  void build_host(const double *P, double *F_out, double *E_out, double *rho_out, double *v_out);
+  // S = Phi^T diag(w bw) Phi [n x n] from the resident Phi (FP64).
+  void overlap(double *S_out);
+  void profile_start(int max_builds);
+  // Average milliseconds per phase over the builds recorded since profile_start.
+  int profile_read(double *avg_ms, int nphase);
+};
+
+}  // namespace napmo

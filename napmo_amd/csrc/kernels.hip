@@ -1,0 +1,527 @@
+// gfx950 kernels of the nAPMO grid hot path: Becke partition, GTO collocation,
+// Poisson projection / radial Green solve / synthesis, and the FP64 MFMA grid
+// contractions rho = diag(Phi P Phi^T) and F = Phi^T diag(s) Phi.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace napmo {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+static constexpr double INV_SQRT_4_PI = 0.28209479177387814347;
+
+// ================================================================ Becke partition
+// One thread per point. Pair tables are wave-uniform loads (scalar cache).
+__global__ void k_becke(const double *__restrict__ x, const double *__restrict__ y,
+                        const double *__restrict__ z, long N, const int *__restrict__ owner, int M,
+                        const double *__restrict__ center, const double *__restrict__ R,
+                        const double *__restrict__ As, double *__restrict__ bw) {
+  // Same operation order as the reference loop (no FMA contraction): the weights pass
+  // through 1 - s with s near 1, where contraction changes the last bits visibly.
+#pragma clang fp contract(off)
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const double px = x[p], py = y[p], pz = z[p];
+  const int own = owner[p];
+  double sum = 0.0, mine = 0.0;
+  for (int i = 0; i < M; ++i) {
+    double dx = px - center[3 * i], dy = py - center[3 * i + 1], dz = pz - center[3 * i + 2];
+    const double ri = sqrt(dx * dx + dy * dy + dz * dz);
+    double prod = 1.0;
+    for (int j = 0; j < M; ++j) {
+      if (j == i) continue;
+      double ex = px - center[3 * j], ey = py - center[3 * j + 1], ez = pz - center[3 * j + 2];
+      const double rj = sqrt(ex * ex + ey * ey + ez * ez);
+      const double mu = (ri - rj) / R[i * M + j];
+      double sv = mu + As[i * M + j] * (1.0 - mu * mu);
+      sv = 0.5 * sv * (3 - sv * sv);
+      sv = 0.5 * sv * (3 - sv * sv);
+      sv = 0.5 * sv * (3 - sv * sv);
+      prod *= 0.5 * (1.0 - sv);
+    }
+    if (i == own) mine = prod;
+    sum += prod;
+  }
+  bw[p] = mine / sum;
+}
+
+void launch_becke(const double *x, const double *y, const double *z, long N, const int *owner,
+                  int M, const double *center, const double *R, const double *As, double *bw,
+                  hipStream_t s) {
+  if (N <= 0) return;
+  k_becke<<<div_up(N, 256), 256, 0, s>>>(x, y, z, N, owner, M, center, R, As, bw);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// ==================================================================== collocation
+// Thread per (point, function): lanes run over functions so row writes coalesce.
+__device__ inline double ipow(double v, int e) {
+  double r = 1.0;
+  for (int k = 0; k < e; ++k) r *= v;
+  return r;
+}
+
+__global__ void k_collocate(const double *__restrict__ x, const double *__restrict__ y,
+                            const double *__restrict__ z, long N, long Npad, int n, int ld,
+                            const double *__restrict__ fo, const int *__restrict__ fl,
+                            const int *__restrict__ ff, const int *__restrict__ fnp,
+                            const double *__restrict__ fnorm, const double *__restrict__ pz,
+                            const double *__restrict__ pc, double *__restrict__ out64,
+                            float *__restrict__ out32) {
+  const int mu = blockIdx.x * blockDim.x + threadIdx.x;
+  const long p = (long)blockIdx.y * blockDim.y + threadIdx.y;
+  if (mu >= ld || p >= Npad) return;
+  double val = 0.0;
+  if (mu < n && p < N) {
+    const double dx = x[p] - fo[3 * mu], dy = y[p] - fo[3 * mu + 1], dz = z[p] - fo[3 * mu + 2];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double ang = ipow(dx, fl[3 * mu]) * ipow(dy, fl[3 * mu + 1]) * ipow(dz, fl[3 * mu + 2]);
+    double acc = 0.0;
+    const int f0 = ff[mu], np = fnp[mu];
+    for (int k = 0; k < np; ++k) acc += pc[f0 + k] * ang * exp(-pz[f0 + k] * r2);
+    val = acc * fnorm[mu];
+  }
+  const size_t o = (size_t)p * ld + mu;
+  if (out64) out64[o] = val;
+  if (out32) out32[o] = (float)val;
+}
+
+void launch_collocate(const double *x, const double *y, const double *z, long N, long Npad, int n,
+                      int ld, const double *fo, const int *fl, const int *ff, const int *fnp,
+                      const double *fnorm, const double *pz, const double *pc, double *out64,
+                      float *out32, hipStream_t s) {
+  dim3 block(64, 4);
+  dim3 grid(div_up(ld, 64), div_up(Npad, 4));
+  k_collocate<<<grid, block, 0, s>>>(x, y, z, N, Npad, n, ld, fo, fl, ff, fnp, fnorm, pz, pc, out64,
+                                     out32);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// ========================================================= Poisson: projection
+// One workgroup per (atom, radial shell); threads over lm. f = dens * bw of the
+// shell is staged in LDS; Yhat[j][lm] = aw_j S_lm(u_j) sqrt(4 pi (2l+1)) / sum aw.
+__global__ void k_project(AtomTables at, const double *__restrict__ dens,
+                          const double *__restrict__ bw, const double *__restrict__ yhat, int L,
+                          double *__restrict__ rho_lm) {
+  extern __shared__ double sf[];
+  const int A = blockIdx.y;
+  const int i = blockIdx.x;
+  if (i >= at.nrad[A]) return;
+  const int na = at.nang[A];
+  const long p0 = at.point_off[A] + (long)i * na;
+  for (int j = threadIdx.x; j < na; j += blockDim.x) sf[j] = dens[p0 + j] * bw[p0 + j];
+  __syncthreads();
+  const double *Y = yhat + at.yhat_off[A];
+  double *out = rho_lm + (at.rad_off[A] + i) * (long)L;
+  for (int lm = threadIdx.x; lm < L; lm += blockDim.x) {
+    double acc = 0.0;
+    for (int j = 0; j < na; ++j) acc += sf[j] * Y[(long)j * L + lm];
+    out[lm] = acc;
+  }
+}
+
+void launch_project(const AtomTables &at, const double *dens, const double *bw, const double *yhat,
+                    int lmax, double *rho_lm, hipStream_t s) {
+  const int L = (lmax + 1) * (lmax + 1);
+  int max_nang = 0;
+  (void)max_nang;
+  dim3 grid(at.max_nrad, at.M);
+  // LDS holds one shell of f: at most 5810 doubles (largest Lebedev rule).
+  k_project<<<grid, 256, 5810 * sizeof(double), s>>>(at, dens, bw, yhat, L, rho_lm);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// ====================================================== Poisson: radial solves
+// U[A][i][lm] = sum_j G_{A,l}[i][j] rho_lm[A][j][lm] for y and dt; one workgroup per
+// (atom, l). green block of an atom: for l = 0..lmax: Gy [nr x nr], then Gd [nr x nr].
+__global__ void k_radial(AtomTables at, const double *__restrict__ rho_lm,
+                         const double *__restrict__ green, int L, double *__restrict__ Uy,
+                         double *__restrict__ Udt) {
+  const int A = blockIdx.x, l = blockIdx.y;
+  const int nr = at.nrad[A], nm = 2 * l + 1;
+  const double *G = green + at.green_off[A] + (long)l * 2 * nr * nr;
+  const long roff = at.rad_off[A] * (long)L;
+  const int total = 2 * nr * nm;
+  for (int o = threadIdx.x; o < total; o += blockDim.x) {
+    const int which = o / (nr * nm);
+    const int rem = o - which * nr * nm;
+    const int i = rem / nm, m = rem - (rem / nm) * nm;
+    const double *g = G + (long)which * nr * nr + (long)i * nr;
+    const double *r = rho_lm + roff + l * l + m;
+    double acc = 0.0;
+    for (int j = 0; j < nr; ++j) acc += g[j] * r[(long)j * L];
+    (which ? Udt : Uy)[roff + (long)i * L + l * l + m] = acc;
+  }
+}
+
+void launch_radial(const AtomTables &at, const double *rho_lm, const double *green, int lmax,
+                   double *Uy, double *Udt, hipStream_t s) {
+  const int L = (lmax + 1) * (lmax + 1);
+  dim3 grid(at.M, lmax + 1);
+  k_radial<<<grid, 256, 0, s>>>(at, rho_lm, green, L, Uy, Udt);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// ====================================================== Poisson: synthesis tables
+// Synthesis order q: for m = 0..lmax, for l = m..lmax: C_lm (and S_lm when m > 0).
+// Reference order index of (l, m, cs): l*l + (m ? 2m - 1 + cs : 0).
+// Each coefficient carries sqrt(2l+1)/sqrt(4 pi) (evaluate.cpp:195-205) and the Racah
+// factor sqrt(2 (l-m)!/(l+m)!) of moments.cpp:106-112, so that the kernel multiplies
+// by (x + i y)^m times the PI_l^m(z) polynomial of the unit vector only.
+__device__ inline void q_decode(int q, int lmax, int &l, int &m, int &cs) {
+  int base = 0;
+  for (m = 0; m <= lmax; ++m) {
+    const int cnt = (lmax - m + 1) * (m ? 2 : 1);
+    if (q < base + cnt) {
+      const int r = q - base;
+      if (m == 0) {
+        l = r, cs = 0;
+      } else {
+        l = m + r / 2, cs = r % 2;
+      }
+      return;
+    }
+    base += cnt;
+  }
+  l = m = cs = 0;
+}
+
+__device__ inline double q_factor(int l, int m) {
+  double f = sqrt(2.0 * l + 1.0) * INV_SQRT_4_PI;
+  if (m > 0) {
+    double r = M_SQRT2;
+    for (int k = 1; k <= m; ++k) r /= sqrt((double)((l + k) * (l - k + 1)));
+    f *= r;
+  }
+  return f;
+}
+
+void synth_order(int lmax, std::vector<int> &l, std::vector<int> &m, std::vector<int> &cs,
+                 std::vector<double> &scale) {
+  l.clear(), m.clear(), cs.clear(), scale.clear();
+  for (int mm = 0; mm <= lmax; ++mm)
+    for (int ll = mm; ll <= lmax; ++ll)
+      for (int c = 0; c < (mm ? 2 : 1); ++c) {
+        double f = std::sqrt(2.0 * ll + 1.0) * INV_SQRT_4_PI;
+        if (mm > 0) {
+          double r = M_SQRT2;
+          for (int k = 1; k <= mm; ++k) r /= std::sqrt((double)((ll + k) * (ll - k + 1)));
+          f *= r;
+        }
+        l.push_back(ll), m.push_back(mm), cs.push_back(c), scale.push_back(f);
+      }
+}
+
+__global__ void k_synth_tables(AtomTables at, const double *__restrict__ Uy,
+                               const double *__restrict__ Udt, int lmax, int L,
+                               int write_amps, double *__restrict__ T,
+                               double *__restrict__ ampL, double *__restrict__ ampR) {
+  const int A = blockIdx.y, j = blockIdx.x;
+  const int nr = at.nrad[A];
+  if (j >= nr - 1) return;
+  const long roff = at.rad_off[A];
+  for (int q = threadIdx.x; q < L; q += blockDim.x) {
+    int l, m, cs;
+    q_decode(q, lmax, l, m, cs);
+    const int ref = l * l + (m ? 2 * m - 1 + cs : 0);
+    const double f = q_factor(l, m);
+    const double y0 = Uy[(roff + j) * L + ref], y1 = Uy[(roff + j + 1) * L + ref];
+    const double d0 = Udt[(roff + j) * L + ref], d1 = Udt[(roff + j + 1) * L + ref];
+    const double zz = y1 - y0;
+    double *t = T + (((long)A * (at.max_nrad - 1) + j) * L + q) * 4;
+    t[0] = y0 * f;
+    t[1] = d0 * f;
+    t[2] = (3 * zz - 2 * d0 - d1) * f;
+    t[3] = (-2 * zz + d0 + d1) * f;
+    if (j == 0 && write_amps) {
+      // PotentialExtrapolation::prepare (src/horton/extrapolation.cpp:156-159)
+      const double r0 = at.r[roff], rn = at.r[roff + nr - 1];
+      const double yn = Uy[(roff + nr - 1) * L + ref];
+      ampL[(long)A * L + q] = y0 / pow(r0, (double)l) * f;
+      ampR[(long)A * L + q] = yn * pow(rn, (double)(l + 1)) * f;
+    }
+  }
+}
+
+void launch_synth_tables(const AtomTables &at, const double *Uy, const double *Udt, int lmax,
+                         double *T, double *ampL, double *ampR, hipStream_t s, int write_amps) {
+  const int L = (lmax + 1) * (lmax + 1);
+  dim3 grid(at.max_nrad - 1, at.M);
+  k_synth_tables<<<grid, 256, 0, s>>>(at, Uy, Udt, lmax, L, write_amps, T, ampL, ampR);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// ================================================================ Poisson: synthesis
+// (kernel template in synth_kernel.hpp, instantiated per lmax in synth_inst_*.hip)
+template <int LM>
+static void synth_dispatch(int lmax, dim3 g, dim3 b, hipStream_t s, const AtomTables &at,
+                           const double *x, const double *y, const double *z, long N,
+                           const double *T, const double *aL, const double *aR, int acc,
+                           int ext_mode, double *v) {
+  if constexpr (LM > MAX_LMAX) {
+    throw Error("lmax above the supported maximum " + std::to_string(MAX_LMAX));
+  } else {
+    if (lmax == LM)
+      launch_synth_t<LM>(g, b, s, at, x, y, z, N, T, aL, aR, acc, ext_mode, v);
+    else
+      synth_dispatch<LM + 1>(lmax, g, b, s, at, x, y, z, N, T, aL, aR, acc, ext_mode, v);
+  }
+}
+void launch_synth(const AtomTables &at, const double *x, const double *y, const double *z, long N,
+                  const double *T, const double *ampL, const double *ampR, int lmax, int accumulate,
+                  double *v, hipStream_t s, int ext_mode) {
+  NAPMO_REQUIRE(lmax >= 0 && lmax <= MAX_LMAX, "lmax out of range for synthesis");
+  if (N <= 0) return;
+  synth_dispatch<0>(lmax, dim3(div_up(N, 256)), dim3(256), s, at, x, y, z, N, T, ampL, ampR,
+                    accumulate, ext_mode, v);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// ================================================= contractions: FP64 MFMA density
+// v_mfma_f64_16x16x4f64 operand layout (lane i of 64):
+//   A[16 x 4]:  lane i holds A[i % 16][i / 16]
+//   B[4 x 16]:  lane i holds B[i / 16][i % 16]
+//   D[16 x 16]: lane i holds D[4 (i / 16) + r][i % 16], r = 0..3
+// Tile: 64 points x 64 columns per workgroup (4 waves of 32 x 32 = 2 x 2 MFMA blocks).
+// X = Phi P over the tile's columns; epilogue rho_partial[ct][p] = sum_cols X o Phi.
+constexpr int KS = 16;  // K elements staged in LDS per step
+
+__global__ void __launch_bounds__(256) k_density_f64(const double *__restrict__ phi, int ld,
+                                                     long Npad, const double *__restrict__ P,
+                                                     double *__restrict__ partial) {
+  __shared__ double As[KS][TILE + 2];  // As[k][m] = Phi[p0 + m][k0 + k]
+  __shared__ double Bs[KS][TILE];      // Bs[k][n] = P[k0 + k][c0 + n]
+  __shared__ double red[TILE][2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long p0 = (long)blockIdx.x * TILE;
+  const int c0 = blockIdx.y * TILE;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  d4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = (d4){0, 0, 0, 0};
+  for (int k0 = 0; k0 < ld; k0 += KS) {
+    // stage Phi[p0 .. p0+63][k0 .. k0+15] transposed, and P[k0 .. k0+15][c0 .. c0+63]
+    for (int e = tid; e < TILE * KS; e += 256) {
+      const int m = e / KS, k = e % KS;
+      As[k][m] = phi[(p0 + m) * ld + k0 + k];
+      const int kk = e / TILE, n = e % TILE;
+      Bs[kk][n] = P[(long)(k0 + kk) * ld + c0 + n];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < KS; ks += 4) {
+      const int kr = ks + (lane >> 4), mi = lane & 15;
+      double a0 = As[kr][wr + mi], a1 = As[kr][wr + 16 + mi];
+      double b0 = Bs[kr][wc + mi], b1 = Bs[kr][wc + 16 + mi];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // epilogue: multiply by Phi and sum over this wave's 32 columns
+  const int col16 = lane & 15, rg = lane >> 4;
+  for (int a = 0; a < 2; ++a) {
+    for (int r = 0; r < 4; ++r) {
+      const int m = wr + a * 16 + rg * 4 + r;
+      const double *prow = phi + (p0 + m) * ld + c0 + wc;
+      double sum = acc[a][0][r] * prow[col16] + acc[a][1][r] * prow[16 + col16];
+      sum += __shfl_xor(sum, 8);
+      sum += __shfl_xor(sum, 4);
+      sum += __shfl_xor(sum, 2);
+      sum += __shfl_xor(sum, 1);
+      if (col16 == 0) red[m][wave & 1] = sum;
+    }
+  }
+  __syncthreads();
+  if (tid < TILE) partial[(long)blockIdx.y * Npad + p0 + tid] = red[tid][0] + red[tid][1];
+}
+
+void launch_density_f64(const double *phi, int ld, long Npad, const double *P, double *partial,
+                        hipStream_t s) {
+  NAPMO_REQUIRE(ld % TILE == 0 && Npad % TILE == 0, "density: padding mismatch");
+  dim3 grid(Npad / TILE, ld / TILE);
+  k_density_f64<<<grid, 256, 0, s>>>(phi, ld, Npad, P, partial);
+  NAPMO_HIP(hipGetLastError());
+}
+
+__global__ void k_rho_reduce(const double *__restrict__ partial, int nt, long Npad, long N,
+                             double *__restrict__ rho) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  double s = 0.0;
+  for (int t = 0; t < nt; ++t) s += partial[(long)t * Npad + p];
+  rho[p] = s;
+}
+
+void launch_rho_reduce(const double *partial, int ntile, long Npad, long N, double *rho,
+                       hipStream_t s) {
+  if (N <= 0) return;
+  k_rho_reduce<<<div_up(N, 256), 256, 0, s>>>(partial, ntile, Npad, N, rho);
+  NAPMO_HIP(hipGetLastError());
+}
+
+__global__ void k_scale3(const double *__restrict__ a, const double *__restrict__ b,
+                         const double *__restrict__ c, long N, long Npad, double *__restrict__ out) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= Npad) return;
+  out[p] = (p < N) ? (c ? a[p] * b[p] * c[p] : a[p] * b[p]) : 0.0;
+}
+
+void launch_scale3(const double *a, const double *b, const double *c, long N, long Npad, double *s,
+                   hipStream_t st) {
+  k_scale3<<<div_up(Npad, 256), 256, 0, st>>>(a, b, c, N, Npad, s);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// =================================================== contractions: FP64 MFMA Fock
+// F[mu][nu] = sum_p X[p][mu] s[p] Y[p][nu]: A[m][k] = X[p0+k][mu0+m], B[k][n] =
+// s[p0+k] Y[p0+k][nu0+n]. Split over points (chunks); one workgroup per (tile, chunk).
+FockShape fock_shape(int ld, long Npad, bool symmetric) {
+  FockShape sh;
+  sh.ld = ld;
+  sh.nt = ld / TILE;
+  sh.ntiles = symmetric ? sh.nt * (sh.nt + 1) / 2 : sh.nt * sh.nt;
+  sh.symmetric = symmetric;
+  sh.Npad = Npad;
+  // aim at ~2048 workgroups, chunks a multiple of KS points
+  long want = std::max(1L, 2048L / sh.ntiles);
+  long chunk = round_up(div_up(Npad, want), KS);
+  sh.chunk = std::max<long>(chunk, KS);
+  sh.nchunk = (int)div_up(Npad, sh.chunk);
+  return sh;
+}
+
+__device__ inline void tile_coords(int t, int nt, bool symmetric, int &ti, int &tj) {
+  if (!symmetric) {
+    ti = t / nt, tj = t % nt;
+    return;
+  }
+  int i = 0, rem = t;
+  while (rem >= nt - i) rem -= nt - i, ++i;
+  ti = i, tj = i + rem;
+}
+
+__global__ void __launch_bounds__(256) k_fock_f64(const double *__restrict__ X,
+                                                  const double *__restrict__ Yv,
+                                                  const double *__restrict__ s, int ld, int nt,
+                                                  int symmetric, long Npad, long chunk,
+                                                  double *__restrict__ partial) {
+  __shared__ double As[KS][TILE];
+  __shared__ double Bs[KS][TILE];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int ti, tj;
+  tile_coords(blockIdx.x, nt, symmetric, ti, tj);
+  const int mu0 = ti * TILE, nu0 = tj * TILE;
+  const long pbeg = (long)blockIdx.y * chunk;
+  const long pend = min(Npad, pbeg + chunk);
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  d4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = (d4){0, 0, 0, 0};
+  for (long k0 = pbeg; k0 < pend; k0 += KS) {
+    for (int e = tid; e < TILE * KS; e += 256) {
+      const int k = e / TILE, m = e % TILE;
+      const long p = k0 + k;
+      As[k][m] = X[p * ld + mu0 + m];
+      Bs[k][m] = s[p] * Yv[p * ld + nu0 + m];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < KS; ks += 4) {
+      const int kr = ks + (lane >> 4), mi = lane & 15;
+      double a0 = As[kr][wr + mi], a1 = As[kr][wr + 16 + mi];
+      double b0 = Bs[kr][wc + mi], b1 = Bs[kr][wc + 16 + mi];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  double *out = partial + ((long)blockIdx.y * gridDim.x + blockIdx.x) * TILE * TILE;
+  const int col16 = lane & 15, rg = lane >> 4;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int r = 0; r < 4; ++r)
+        out[(wr + a * 16 + rg * 4 + r) * TILE + wc + b * 16 + col16] = acc[a][b][r];
+}
+
+void launch_fock_f64(const FockShape &sh, const double *X, const double *Y, const double *s,
+                     double *partial, hipStream_t st) {
+  dim3 grid(sh.ntiles, sh.nchunk);
+  k_fock_f64<<<grid, 256, 0, st>>>(X, Y, s, sh.ld, sh.nt, sh.symmetric ? 1 : 0, sh.Npad, sh.chunk,
+                                   partial);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// Sum the chunk partials of one output element in fixed chunk order.
+__global__ void k_fock_reduce(const double *__restrict__ partial, int nt, int ntiles, int nchunk,
+                              int symmetric, int n, int upper_mirror, double scale, int accumulate,
+                              double *__restrict__ F) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  int i = (int)(e / n), j = (int)(e % n);
+  if ((symmetric || upper_mirror) && i > j) {
+    const int t = i;
+    i = j, j = t;
+  }
+  const int ti = i / TILE, tj = j / TILE;
+  int t;
+  if (symmetric) {
+    t = 0;
+    for (int a = 0; a < ti; ++a) t += nt - a;
+    t += tj - ti;
+  } else {
+    t = ti * nt + tj;
+  }
+  const long off = (long)t * TILE * TILE + (i % TILE) * TILE + (j % TILE);
+  double sum = 0.0;
+  for (int c = 0; c < nchunk; ++c) sum += partial[(long)c * ntiles * TILE * TILE + off];
+  sum *= scale;
+  F[e] = accumulate ? F[e] + sum : sum;
+}
+
+void launch_fock_reduce(const FockShape &sh, const double *partial, int n, int upper_mirror,
+                        double scale, int accumulate, double *F, hipStream_t st) {
+  const long total = (long)n * n;
+  k_fock_reduce<<<div_up(total, 256), 256, 0, st>>>(partial, sh.nt, sh.ntiles, sh.nchunk,
+                                                     sh.symmetric ? 1 : 0, n, upper_mirror, scale,
+                                                     accumulate, F);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// ============================================================ FP64 energy traces
+// Single workgroup, fixed reduction order (deterministic).
+__global__ void k_traces(const double *__restrict__ P, const double *__restrict__ F, int n,
+                         const double *__restrict__ wbw, const double *__restrict__ rho,
+                         const double *__restrict__ v, long N, double *__restrict__ E) {
+  __shared__ double sa[1024], sb[1024];
+  const int tid = threadIdx.x;
+  double a = 0.0, b = 0.0;
+  if (P && F)
+    for (long e = tid; e < (long)n * n; e += blockDim.x) a += P[e] * F[e];
+  if (wbw && rho && v)
+    for (long p = tid; p < N; p += blockDim.x) b += wbw[p] * rho[p] * v[p];
+  sa[tid] = a;
+  sb[tid] = b;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (tid < w) sa[tid] += sa[tid + w], sb[tid] += sb[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) E[0] = 0.5 * sa[0], E[1] = 0.5 * sb[0];
+}
+
+void launch_traces(const double *P, const double *F, int n, const double *wbw, const double *rho,
+                   const double *v, long N, double *E, hipStream_t st) {
+  k_traces<<<1, 1024, 0, st>>>(P, F, n, wbw, rho, v, N, E);
+  NAPMO_HIP(hipGetLastError());
+}
+
+}  // namespace napmo

@@ -1,0 +1,102 @@
+// Launchers of the gfx950 kernels (kernels.hip). All pointers are device pointers;
+// every launcher enqueues on `stream` and returns without synchronising.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace napmo {
+
+constexpr int MAX_LMAX = 24;   // synthesis / projection instantiated for lmax <= MAX_LMAX
+constexpr int TILE = 64;       // MFMA output tile edge of the contractions
+
+// Becke partition (src/grid/becke.cpp:79-176) for every point; owner[p] = atom of p.
+// R: [M*M] centre distances; As: [M*M] clamped size adjustments with the sign of the
+// reference evaluation (negative when i < j).
+void launch_becke(const double *x, const double *y, const double *z, long N, const int *owner,
+                  int M, const double *center, const double *R, const double *As, double *bw,
+                  hipStream_t s);
+
+// Collocation Phi[p, mu] (src/system/gto.cpp:41-51, 182-192) into out64 [Npad x ld]
+// (and out32 when non-null). Rows p >= N and columns mu >= n are written as zero.
+void launch_collocate(const double *x, const double *y, const double *z, long N, long Npad, int n,
+                      int ld, const double *fn_origin, const int *fn_l, const int *fn_first,
+                      const int *fn_nprim, const double *fn_norm, const double *prim_zeta,
+                      const double *prim_coef, double *out64, float *out32, hipStream_t s);
+
+// Per-atom geometry tables shared by the Poisson kernels.
+struct AtomTables {
+  int M;
+  const long *point_off;    // [M+1] first point of each atom
+  const int *nrad;          // [M]
+  const int *nang;          // [M]
+  const long *rad_off;      // [M] first radial shell of the atom in concatenated radial arrays
+  const long *yhat_off;     // [M] offset of the atom's Lebedev projection table
+  const long *green_off;    // [M] offset of the atom's Green matrices (per l: y then dt)
+  const int *rkind;         // [M] radial transform kind
+  const double *rp0, *rp1;  // [M] transform parameters
+  const double *first_x, *last_x;  // [M] spline domain
+  const double *center;     // [M*3]
+  const double *r;          // [sum nrad] radial nodes
+  int max_nrad;
+};
+
+// rho_lm[A][i][lm] = sum_j f_Aij Yhat[j][lm], f = dens * bw  (spherical expansion).
+void launch_project(const AtomTables &at, const double *dens, const double *bw, const double *yhat,
+                    int lmax, double *rho_lm, hipStream_t s);
+// U[A] (y and dt at the radial nodes, lm reference order) = G_{A,l} rho_lm[A].
+void launch_radial(const AtomTables &at, const double *rho_lm, const double *green, int lmax,
+                   double *Uy, double *Udt, hipStream_t s);
+// Synthesis tables: T[A][j][q][4] Horner coefficients in synthesis order q, amp_left /
+// amp_right [A][q], all scaled by sqrt(2l+1)/sqrt(4 pi).
+// write_amps = 0 keeps ampL / ampR as provided (non-potential extrapolations).
+void launch_synth_tables(const AtomTables &at, const double *Uy, const double *Udt, int lmax,
+                         double *T, double *ampL, double *ampR, hipStream_t s, int write_amps = 1);
+// Synthesis-order index q -> (l, m, cos/sin) and its coefficient scale (host copy).
+void synth_order(int lmax, std::vector<int> &l, std::vector<int> &m, std::vector<int> &cs,
+                 std::vector<double> &scale);
+// v[p] (+)= sum_A sum_lm U^A_lm(|p - R_A|) S_lm(unit) (src/horton/evaluate.cpp:129-223),
+// for points given as SoA (x, y, z). accumulate = add into v instead of overwriting.
+// ext_mode 0: PotentialExtrapolation (ampL, ampR); 1: no tail, cusp left (a0, b0).
+void launch_synth(const AtomTables &at, const double *x, const double *y, const double *z, long N,
+                  const double *T, const double *ampL, const double *ampR, int lmax, int accumulate,
+                  double *v, hipStream_t s, int ext_mode = 0);
+
+template <int LMAX>
+void launch_synth_t(dim3 g, dim3 b, hipStream_t s, const AtomTables &at, const double *x,
+                    const double *y, const double *z, long N, const double *T, const double *aL,
+                    const double *aR, int acc, int ext_mode, double *v);
+
+// rho[p] = Phi_p^T P Phi_p with FP64 MFMA (P [ld x ld] zero padded). partial [ntile x Npad].
+void launch_density_f64(const double *phi, int ld, long Npad, const double *P, double *partial,
+                        hipStream_t s);
+// rho[p] = sum_t partial[t][p] for p < N.
+void launch_rho_reduce(const double *partial, int ntile, long Npad, long N, double *rho,
+                       hipStream_t s);
+
+// Scale vector s[p] = a[p] * b[p] * c[p] (c may be null), zero for p >= N.
+void launch_scale3(const double *a, const double *b, const double *c, long N, long Npad, double *s,
+                   hipStream_t st);
+
+// F = X^T diag(s) Y over Npad points. symmetric: only upper tiles of F (X == Y).
+// partial buffer: [nchunk x ntiles x TILE x TILE]. Returns nothing; F written by reduce.
+struct FockShape {
+  int ld, nt, ntiles, nchunk;
+  long Npad, chunk;
+  bool symmetric;
+};
+FockShape fock_shape(int ld, long Npad, bool symmetric);
+void launch_fock_f64(const FockShape &sh, const double *X, const double *Y, const double *s,
+                     double *partial, hipStream_t st);
+// F_out [n x n] row-major from the partials; symmetric mirrors upper tiles;
+// upper_mirror: general tiles, then F_ij := F_ji for i > j from the upper triangle
+// (the exchange block of src/wavefunction/nwavefunction.cpp:53-62).
+void launch_fock_reduce(const FockShape &sh, const double *partial, int n, int upper_mirror,
+                        double scale, int accumulate, double *F, hipStream_t st);
+
+// E[0] = 0.5 * sum_ij P_ij F_ij (FP64), E[1] = 0.5 * sum_p w_p rho_p v_p.
+void launch_traces(const double *P, const double *F, int n, const double *wbw, const double *rho,
+                   const double *v, long N, double *E, hipStream_t st);
+
+}  // namespace napmo

@@ -1,0 +1,493 @@
+// Becke grid device mirror and Poisson plans (host-side setup in FP64).
+#include "poisson_plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace napmo {
+
+// ------------------------------------------------------ Hermite element integrals
+// Cubic Hermite shape functions on a unit segment u in [0, 1], integer coefficients of
+// 1, u, u^2, u^3: node at the segment's left end (value, slope), right end (value, slope).
+static const int SHAPE[2][2][4] = {{{1, 0, -3, 2}, {0, 1, -2, 1}},   // left node
+                                   {{0, 0, 3, -2}, {0, 0, -1, 1}}};  // right node
+
+// Integral over [0,1] of a product of shape functions (or their u-derivatives), exact in
+// rationals (denominator 2520 = lcm(1..10) covers degree <= 9) and rounded once.
+static double seg_integral(const int (*f)[3], int nf) {
+  long long poly[10] = {1};
+  int deg = 0;
+  for (int k = 0; k < nf; ++k) {
+    const int node = f[k][0], kind = f[k][1], der = f[k][2];
+    long long q[4] = {0, 0, 0, 0};
+    int qd = 3;
+    for (int c = 0; c < 4; ++c) q[c] = SHAPE[node][kind][c];
+    if (der) {
+      for (int c = 0; c < 3; ++c) q[c] = (c + 1) * q[c + 1];
+      q[3] = 0;
+      qd = 2;
+    }
+    long long out[10] = {0};
+    for (int a = 0; a <= deg; ++a)
+      for (int b = 0; b <= qd; ++b) out[a + b] += poly[a] * q[b];
+    deg += qd;
+    std::memcpy(poly, out, sizeof(out));
+  }
+  long long num = 0;
+  for (int c = 0; c <= deg; ++c) num += poly[c] * (2520 / (c + 1));
+  return (double)num / 2520.0;
+}
+
+// Overlap of basis functions (index i = 2*node + kind, derivative flag) summed over the
+// segments around the first function's node inside [0, xmax]: left segment first.
+static double fe_overlap(long xmax, const long *idx, const int *der, int nf) {
+  const long c0 = idx[0] / 2;
+  for (int k = 1; k < nf; ++k)
+    if (std::labs(idx[k] / 2 - c0) >= 2) return 0.0;
+  double res = 0.0;
+  for (int side = 0; side < 2; ++side) {
+    const long seg = side == 0 ? c0 - 1 : c0;
+    if (side == 0 && !(c0 >= 1 && c0 <= xmax)) continue;
+    if (side == 1 && !(c0 >= 0 && c0 <= xmax - 1)) continue;
+    int f[3][3];
+    bool zero = false;
+    for (int k = 0; k < nf; ++k) {
+      const long rel = idx[k] / 2 - seg;
+      if (rel < 0 || rel > 1) {
+        zero = true;
+        break;
+      }
+      f[k][0] = (int)rel, f[k][1] = (int)(idx[k] % 2), f[k][2] = der[k];
+    }
+    if (!zero) res += seg_integral(f, nf);
+  }
+  return res;
+}
+
+static double fe_node(long x, long i, int deriv) {
+  return ((i % 2) == deriv) ? (double)(x == i / 2) : 0.0;
+}
+
+// Dense LU with partial pivoting, solving A X = B in place (A n x n, B n x m).
+static void lu_solve(std::vector<double> &A, std::vector<double> &B, int n, int m) {
+  for (int k = 0; k < n; ++k) {
+    int piv = k;
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(A[(size_t)i * n + k]) > std::fabs(A[(size_t)piv * n + k])) piv = i;
+    NAPMO_REQUIRE(A[(size_t)piv * n + k] != 0.0, "singular radial Poisson system");
+    if (piv != k) {
+      for (int c = 0; c < n; ++c) std::swap(A[(size_t)k * n + c], A[(size_t)piv * n + c]);
+      for (int c = 0; c < m; ++c) std::swap(B[(size_t)k * m + c], B[(size_t)piv * m + c]);
+    }
+    const double inv = 1.0 / A[(size_t)k * n + k];
+    for (int i = k + 1; i < n; ++i) {
+      const double f = A[(size_t)i * n + k] * inv;
+      if (f == 0.0) continue;
+      A[(size_t)i * n + k] = f;
+      for (int c = k + 1; c < n; ++c) A[(size_t)i * n + c] -= f * A[(size_t)k * n + c];
+      for (int c = 0; c < m; ++c) B[(size_t)i * m + c] -= f * B[(size_t)k * m + c];
+    }
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    for (int c = 0; c < m; ++c) {
+      double s = B[(size_t)k * m + c];
+      for (int j = k + 1; j < n; ++j) s -= A[(size_t)k * n + j] * B[(size_t)j * m + c];
+      B[(size_t)k * m + c] = s / A[(size_t)k * n + k];
+    }
+  }
+}
+
+void fe_assemble(const double *b, const double *a, const double *f, const double *bcs,
+                 double *coeffs, double *rhs, long npoint) {
+  const long nfn = 2 * npoint, xmax = npoint - 1;
+  long skip[2] = {-1, -1};
+  int cnt = 0;
+  const long rows[4] = {0, 1, nfn - 2, nfn - 1};
+  for (int k = 0; k < 4; ++k)
+    if (bcs[k] == bcs[k]) {  // NaN marks a free end
+      coeffs[rows[k] * nfn + rows[k]] = 1;
+      rhs[rows[k]] = bcs[k];
+      if (cnt < 2) skip[cnt] = rows[k];
+      ++cnt;
+    }
+  for (long irow = 0; irow < nfn; ++irow) {
+    if (irow == skip[0] || irow == skip[1]) continue;
+    const long begin = std::max(2 * (irow / 2 - 1), 0L), end = std::min(2 * (irow / 2 + 2), nfn);
+    for (long icol = begin; icol < end; ++icol) {
+      double c = coeffs[irow * nfn + icol];
+      {
+        long idx[2] = {irow, icol};
+        int der[2] = {1, 1};
+        c -= fe_overlap(xmax, idx, der, 2);
+      }
+      for (long k = begin; k < end; ++k) {
+        long idx[3] = {irow, icol, k};
+        if (b[k] != 0) {
+          int der[3] = {0, 1, 0};
+          c += b[k] * fe_overlap(xmax, idx, der, 3);
+        }
+        if (a[k] != 0) {
+          int der[3] = {0, 0, 0};
+          c += a[k] * fe_overlap(xmax, idx, der, 3);
+        }
+      }
+      c += fe_node(xmax, irow, 0) * fe_node(xmax, icol, 1);
+      c -= fe_node(0, irow, 0) * fe_node(0, icol, 1);
+      coeffs[irow * nfn + icol] = c;
+    }
+    double acc = rhs[irow];
+    for (long k = begin; k < end; ++k) {
+      if (f[k] == 0) continue;
+      long idx[2] = {irow, k};
+      int der[2] = {0, 0};
+      acc += f[k] * fe_overlap(xmax, idx, der, 2);
+    }
+    rhs[irow] = acc;
+  }
+}
+
+void radial_green(const RTransform &rtf, const std::vector<double> &rw, int l,
+                  std::vector<double> &Gy, std::vector<double> &Gd) {
+  const int n = rtf.npoint, nfn = 2 * n;
+  const long xmax = n - 1;
+  std::vector<double> r(n), j1(n), j2(n), j3(n);
+  for (int i = 0; i < n; ++i) {
+    r[i] = rtf.radius(i);
+    j1[i] = rtf.deriv(i);
+    j2[i] = rtf.deriv2(i);
+    j3[i] = rtf.deriv3(i);
+  }
+  // b = 2/r (dx -2/r^2), a = -l(l+1)/r^2 (dx 2l(l+1)/r^3) transformed to the t-axis
+  // (napmo/utilities/ode2.py:75-90); merged [value, slope] node coefficients.
+  const double l2l = (double)l * (l + 1);
+  std::vector<double> b(nfn), a(nfn);
+  for (int i = 0; i < n; ++i) {
+    const double r2 = r[i] * r[i], r3 = r2 * r[i], j1sq = j1[i] * j1[i];
+    const double by = 2 / r[i], bd = -2 / r2, ay = -l2l / r2, ad = 2 * l2l / r3;
+    b[2 * i] = j1[i] * by - j2[i] / j1[i];
+    b[2 * i + 1] = j2[i] * by + j1sq * bd + (j2[i] * j2[i] - j1[i] * j3[i]) / j1sq;
+    a[2 * i] = ay * j1sq;
+    a[2 * i + 1] = (ad * j1sq + 2 * ay * j2[i]) * j1[i];
+  }
+  // Galerkin matrix; rows 0 and nfn-2 are the value boundary conditions at rmin, rmax.
+  std::vector<double> A((size_t)nfn * nfn, 0.0), zf(nfn, 0.0), zr(nfn, 0.0);
+  const double bcs[4] = {0.0, NAN, 0.0, NAN};
+  fe_assemble(b.data(), a.data(), zf.data(), bcs, A.data(), zr.data(), n);
+  // B: rho_j -> right-hand side. Spline slopes of rho: dt = Tinv rho (tridiagonal solve),
+  // f = -4 pi (rho, dt/j1), transformed: fy = f j1^2, fd = (fd j1^2 + 2 fy j2) j1.
+  std::vector<double> B((size_t)nfn * n, 0.0);
+  std::vector<double> e(n), dt(n);
+  const double pi4 = 4.0 * M_PI;
+  for (int jc = 0; jc < n; ++jc) {
+    std::fill(e.begin(), e.end(), 0.0);
+    e[jc] = 1.0;
+    spline_derivatives(e.data(), dt.data(), n);
+    std::vector<double> f(nfn);
+    for (int i = 0; i < n; ++i) {
+      const double fy = -pi4 * e[i], fdx = -pi4 * (dt[i] / j1[i]), j1sq = j1[i] * j1[i];
+      f[2 * i] = fy * j1sq;
+      f[2 * i + 1] = (fdx * j1sq + 2 * fy * j2[i]) * j1[i];
+    }
+    for (long irow = 0; irow < nfn; ++irow) {
+      if (irow == 0 || irow == nfn - 2) continue;
+      long begin = std::max(2 * (irow / 2 - 1), 0L), end = std::min(2 * (irow / 2 + 2), (long)nfn);
+      double acc = 0.0;
+      for (long k = begin; k < end; ++k) {
+        if (f[k] == 0) continue;
+        long idx[2] = {irow, k};
+        int der[2] = {0, 0};
+        acc += f[k] * fe_overlap(xmax, idx, der, 2);
+      }
+      B[(size_t)irow * n + jc] = acc;
+    }
+    // boundary values (napmo/grids/poisson_solver.py:92-105)
+    B[jc] = (e[jc] * std::pow(r[jc], -l - 1.0) * rw[jc]) * (std::pow(r[0], (double)l) / (2 * l + 1));
+    B[(size_t)(nfn - 2) * n + jc] =
+        (e[jc] * std::pow(r[jc], (double)l) * rw[jc]) * (1.0 / std::pow(r[n - 1], l + 1.0) / (2 * l + 1));
+  }
+  lu_solve(A, B, nfn, n);
+  Gy.assign((size_t)n * n, 0.0);
+  Gd.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int jc = 0; jc < n; ++jc) {
+      Gy[(size_t)i * n + jc] = B[(size_t)(2 * i) * n + jc];
+      Gd[(size_t)i * n + jc] = B[(size_t)(2 * i + 1) * n + jc];
+    }
+}
+
+std::vector<double> projection_table(const AngularGrid &ang, int lmax) {
+  const int L = (lmax + 1) * (lmax + 1), na = ang.lorder;
+  std::vector<double> t((size_t)na * L), s(L);
+  double sw = 0.0;
+  for (int j = 0; j < na; ++j) sw += ang.weights[j];
+  for (int j = 0; j < na; ++j) {
+    solid_harmonics(ang.points[3 * j], ang.points[3 * j + 1], ang.points[3 * j + 2], lmax, s.data());
+    int lm = 0;
+    for (int l = 0; l <= lmax; ++l) {
+      const double nrm = std::sqrt(4.0 * M_PI * (2.0 * l + 1));
+      for (int m = -l; m <= l; ++m, ++lm) t[(size_t)j * L + lm] = ang.weights[j] * s[lm] * nrm / sw;
+    }
+  }
+  return t;
+}
+
+// ------------------------------------------------------------------ DeviceGrid
+
+DeviceGrid::~DeviceGrid() {
+  plans.clear();
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void DeviceGrid::build(const std::vector<AtomInfo> &in) {
+  require_device();
+  atoms = in;
+  M = (int)atoms.size();
+  NAPMO_REQUIRE(M > 0, "BeckeGrid needs at least one centre");
+  if (!stream) NAPMO_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  offsets.assign(M + 1, 0);
+  for (int a = 0; a < M; ++a) offsets[a + 1] = offsets[a] + (long)atoms[a].nrad * atoms[a].nang;
+  N = offsets[M];
+  points.resize(3 * N);
+  weights.resize(N);
+  origin.resize(3 * M);
+  radii.resize(M);
+  std::vector<double> hx(N), hy(N), hz(N);
+  std::vector<int> own(N);
+  std::vector<long> rad_off(M);
+  std::vector<int> nrad(M), nang(M), rkind(M);
+  std::vector<double> rp0(M), rp1(M), first(M), last(M), allr;
+  max_nrad = 0;
+  for (int a = 0; a < M; ++a) {
+    const AtomInfo &at = atoms[a];
+    for (int c = 0; c < 3; ++c) origin[3 * a + c] = at.origin[c];
+    radii[a] = at.radii;
+    rad_off[a] = (long)allr.size();
+    allr.insert(allr.end(), at.r.begin(), at.r.end());
+    nrad[a] = at.nrad, nang[a] = at.nang, rkind[a] = at.rtf.kind;
+    rp0[a] = at.rtf.p0, rp1[a] = at.rtf.p1;
+    first[a] = at.rtf.radius(0);
+    last[a] = at.rtf.radius(at.nrad - 1);
+    max_nrad = std::max(max_nrad, at.nrad);
+    for (int i = 0; i < at.nrad; ++i)
+      for (int j = 0; j < at.nang; ++j) {
+        const long p = offsets[a] + (long)i * at.nang + j;
+        for (int c = 0; c < 3; ++c) points[3 * p + c] = at.u[3 * j + c] * at.r[i] + at.origin[c];
+        weights[p] = at.aw[j] * at.rw[i];
+        hx[p] = points[3 * p], hy[p] = points[3 * p + 1], hz[p] = points[3 * p + 2];
+        own[p] = a;
+      }
+  }
+  // Becke pair tables (src/grid/becke.cpp:83-115) with the evaluation sign folded in.
+  std::vector<double> R((size_t)M * M, 0.0), As((size_t)M * M, 0.0);
+  for (int i = 0; i < M; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double s = 0.0;
+      for (int c = 0; c < 3; ++c) {
+        const double d = origin[3 * i + c] - origin[3 * j + c];
+        s += d * d;
+      }
+      R[(size_t)i * M + j] = R[(size_t)j * M + i] = std::sqrt(s);
+      double u = (radii[i] - radii[j]) / (radii[i] + radii[j]);
+      u = u / (u * u - 1.0);
+      u = std::min(std::max(u, -0.45), 0.45);
+      As[(size_t)i * M + j] = u;    // i >= j: sign +1
+      As[(size_t)j * M + i] = -u;   // j < i evaluated as (j, i): sign -1
+      if (i == j) As[(size_t)i * M + i] = 0.0;
+    }
+  for (int i = 0; i < M; ++i)
+    for (int j = 0; j < M; ++j)
+      if (i != j) NAPMO_REQUIRE(R[(size_t)i * M + j] > 0.0, "coincident centres in BeckeGrid");
+  x.upload(hx, stream), y.upload(hy, stream), z.upload(hz, stream), w.upload(weights, stream);
+  owner.upload(own, stream);
+  d_point_off.upload(offsets, stream);
+  d_rad_off.upload(rad_off, stream);
+  d_nrad.upload(nrad, stream), d_nang.upload(nang, stream), d_rkind.upload(rkind, stream);
+  d_rp0.upload(rp0, stream), d_rp1.upload(rp1, stream);
+  d_first.upload(first, stream), d_last.upload(last, stream);
+  d_center.upload(origin, stream);
+  d_r.upload(allr, stream);
+  DeviceBuffer<double> dR, dA;
+  dR.upload(R, stream);
+  dA.upload(As, stream);
+  bw.resize(N);
+  wbw.resize(N);
+  launch_becke(x.get(), y.get(), z.get(), N, owner.get(), M, d_center.get(), dR.get(), dA.get(),
+               bw.get(), stream);
+  launch_scale3(w.get(), bw.get(), nullptr, N, N, wbw.get(), stream);
+  becke.resize(N);
+  bw.download(becke.data(), N, stream);
+  NAPMO_HIP(hipStreamSynchronize(stream));
+}
+
+AtomTables DeviceGrid::tables(const PoissonPlan &plan) const {
+  AtomTables t;
+  t.M = M;
+  t.point_off = d_point_off.get();
+  t.nrad = d_nrad.get();
+  t.nang = d_nang.get();
+  t.rad_off = d_rad_off.get();
+  t.yhat_off = plan.yhat_off.get();
+  t.green_off = plan.green_off.get();
+  t.rkind = d_rkind.get();
+  t.rp0 = d_rp0.get();
+  t.rp1 = d_rp1.get();
+  t.first_x = d_first.get();
+  t.last_x = d_last.get();
+  t.center = d_center.get();
+  t.r = d_r.get();
+  t.max_nrad = max_nrad;
+  return t;
+}
+
+PoissonPlan &DeviceGrid::plan(int lmax) {
+  NAPMO_REQUIRE(lmax >= 0 && lmax <= MAX_LMAX,
+                "lmax must be in [0, " + std::to_string(MAX_LMAX) + "]");
+  auto it = plans.find(lmax);
+  if (it != plans.end()) return *it->second;
+  auto p = std::make_unique<PoissonPlan>();
+  p->lmax = lmax;
+  const int L = (lmax + 1) * (lmax + 1);
+  // projection tables, one per distinct Lebedev rule
+  std::map<int, long> yoff;
+  std::vector<double> yh;
+  std::vector<long> yh_off(M);
+  for (int a = 0; a < M; ++a) {
+    const int na = atoms[a].nang;
+    if (!yoff.count(na)) {
+      AngularGrid ag;
+      ag.lorder = na;
+      ag.points = atoms[a].u;
+      ag.weights = atoms[a].aw;
+      yoff[na] = (long)yh.size();
+      auto t = projection_table(ag, lmax);
+      yh.insert(yh.end(), t.begin(), t.end());
+    }
+    yh_off[a] = yoff[na];
+  }
+  // Green matrices, one set per distinct radial grid
+  std::vector<double> gr;
+  std::vector<long> g_off(M);
+  std::vector<std::pair<std::vector<double>, long>> seen;  // (key, offset)
+  for (int a = 0; a < M; ++a) {
+    const AtomInfo &at = atoms[a];
+    std::vector<double> key = {(double)at.rtf.kind, at.rtf.p0, at.rtf.p1, (double)at.nrad};
+    long off = -1;
+    for (auto &s : seen)
+      if (s.first == key) off = s.second;
+    if (off < 0) {
+      off = (long)gr.size();
+      for (int l = 0; l <= lmax; ++l) {
+        std::vector<double> Gy, Gd;
+        radial_green(at.rtf, at.rw, l, Gy, Gd);
+        gr.insert(gr.end(), Gy.begin(), Gy.end());
+        gr.insert(gr.end(), Gd.begin(), Gd.end());
+      }
+      seen.push_back({key, off});
+    }
+    g_off[a] = off;
+  }
+  p->yhat.upload(yh, stream);
+  p->yhat_off.upload(yh_off, stream);
+  p->green.upload(gr, stream);
+  p->green_off.upload(g_off, stream);
+  long nrad_total = 0;
+  for (auto &at : atoms) nrad_total += at.nrad;
+  p->rho_lm.resize((size_t)nrad_total * L);
+  p->Uy.resize((size_t)nrad_total * L);
+  p->Udt.resize((size_t)nrad_total * L);
+  p->T.resize((size_t)M * (max_nrad - 1) * L * 4);
+  p->ampL.resize((size_t)M * L);
+  p->ampR.resize((size_t)M * L);
+  NAPMO_HIP(hipStreamSynchronize(stream));
+  auto &ref = *p;
+  plans[lmax] = std::move(p);
+  return ref;
+}
+
+void DeviceGrid::poisson_expand(const double *d_dens, int lmax) {
+  PoissonPlan &pl = plan(lmax);
+  AtomTables at = tables(pl);
+  launch_project(at, d_dens, bw.get(), pl.yhat.get(), lmax, pl.rho_lm.get(), stream);
+  launch_radial(at, pl.rho_lm.get(), pl.green.get(), lmax, pl.Uy.get(), pl.Udt.get(), stream);
+}
+
+void DeviceGrid::poisson_synthesize(int lmax, double *d_v) {
+  PoissonPlan &pl = plan(lmax);
+  AtomTables at = tables(pl);
+  launch_synth_tables(at, pl.Uy.get(), pl.Udt.get(), lmax, pl.T.get(), pl.ampL.get(),
+                      pl.ampR.get(), stream);
+  launch_synth(at, x.get(), y.get(), z.get(), N, pl.T.get(), pl.ampL.get(), pl.ampR.get(), lmax, 0,
+               d_v, stream);
+}
+
+}  // namespace napmo
+
+namespace napmo {
+
+void decomposition_on_points(CubicSpline **sp, int nspline, int lmax, const double *center,
+                             const double *points, long npoint, double *output) {
+  require_device();
+  NAPMO_REQUIRE(lmax <= MAX_LMAX, "lmax above the supported maximum");
+  const RTransform &rtf = *sp[0]->rtf;
+  const int n = sp[0]->n;
+  for (int s = 1; s < nspline; ++s)
+    NAPMO_REQUIRE(sp[s]->n == n && sp[s]->rtf->kind == rtf.kind && sp[s]->rtf->p0 == rtf.p0 &&
+                      sp[s]->rtf->p1 == rtf.p1,
+                  "eval_decomposition_grid: splines must share one radial transform");
+  const bool tail = sp[0]->ext->has_tail();
+  const int L = nspline;
+  // U arrays in reference lm order [n x L]
+  std::vector<double> Uy((size_t)n * L), Ud((size_t)n * L);
+  for (int s = 0; s < L; ++s)
+    for (int i = 0; i < n; ++i) Uy[(size_t)i * L + s] = sp[s]->y[i], Ud[(size_t)i * L + s] = sp[s]->dt[i];
+  // extrapolation parameters in synthesis order, scaled like the table coefficients
+  std::vector<int> ql, qm, qc;
+  std::vector<double> qs;
+  synth_order(lmax, ql, qm, qc, qs);
+  std::vector<double> aL(L), aR(L);
+  for (int q = 0; q < L; ++q) {
+    const int ref = ql[q] * ql[q] + (qm[q] ? 2 * qm[q] - 1 + qc[q] : 0);
+    const Extrapolation &e = *sp[ref]->ext;
+    if (tail) {
+      aL[q] = e.amp_left * qs[q], aR[q] = e.amp_right * qs[q];
+    } else if (e.kind == EX_CUSP) {
+      aL[q] = e.a0 * qs[q], aR[q] = e.b0;
+    } else {
+      aL[q] = 0.0, aR[q] = 0.0;
+    }
+  }
+  DeviceGrid one;  // single-centre tables (no grid points needed beyond the radial nodes)
+  hipStream_t st;
+  NAPMO_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  one.stream = st;
+  std::vector<long> z0 = {0, 0}, roff = {0};
+  std::vector<int> nr = {n}, na = {1}, rk = {rtf.kind};
+  std::vector<double> p0 = {rtf.p0}, p1 = {rtf.p1}, fx = {sp[0]->first_x}, lx = {sp[0]->last_x};
+  std::vector<double> c = {center[0], center[1], center[2]}, rr(n);
+  for (int i = 0; i < n; ++i) rr[i] = rtf.radius(i);
+  one.d_point_off.upload(z0, st), one.d_rad_off.upload(roff, st);
+  one.d_nrad.upload(nr, st), one.d_nang.upload(na, st), one.d_rkind.upload(rk, st);
+  one.d_rp0.upload(p0, st), one.d_rp1.upload(p1, st), one.d_first.upload(fx, st);
+  one.d_last.upload(lx, st), one.d_center.upload(c, st), one.d_r.upload(rr, st);
+  one.M = 1;
+  one.max_nrad = n;
+  PoissonPlan pl;
+  pl.Uy.upload(Uy, st), pl.Udt.upload(Ud, st);
+  pl.ampL.upload(aL, st), pl.ampR.upload(aR, st);
+  pl.T.resize((size_t)(n - 1) * L * 4);
+  AtomTables at = one.tables(pl);
+  launch_synth_tables(at, pl.Uy.get(), pl.Udt.get(), lmax, pl.T.get(), pl.ampL.get(), pl.ampR.get(),
+                      st, 0);
+  std::vector<double> hx(npoint), hy(npoint), hz(npoint);
+  for (long i = 0; i < npoint; ++i)
+    hx[i] = points[3 * i], hy[i] = points[3 * i + 1], hz[i] = points[3 * i + 2];
+  DeviceBuffer<double> dx, dy, dz, dv;
+  dx.upload(hx, st), dy.upload(hy, st), dz.upload(hz, st), dv.upload(output, npoint, st);
+  launch_synth(at, dx.get(), dy.get(), dz.get(), npoint, pl.T.get(), pl.ampL.get(), pl.ampR.get(),
+               lmax, 1, dv.get(), st, tail ? 0 : 1);
+  dv.download(output, npoint, st);
+  NAPMO_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace napmo

@@ -1,0 +1,77 @@
+// Device mirror of a Becke grid and its Poisson plans.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "host.hpp"
+#include "kernels.hpp"
+
+namespace napmo {
+
+// Hermite finite-element system of the radial Poisson equation
+// V'' + (2/r) V' - l(l+1)/r^2 V = -4 pi rho on the t-axis (napmo/grids/poisson_solver.py
+// :59-116, napmo/utilities/ode2.py:35-118, src/horton/ode2.cpp:232-321). The matrix depends
+// only on (radial transform, l); rho enters linearly through the right-hand side and the
+// two boundary values, so G = A^{-1} B maps rho_lm(r_j) to the potential spline (y, dt) at
+// the nodes. Returns Gy, Gd [n x n] row-major (output node, input node).
+void radial_green(const RTransform &rtf, const std::vector<double> &rw, int l,
+                  std::vector<double> &Gy, std::vector<double> &Gd);
+
+// build_ode2 of src/horton/ode2.cpp:232-321 (same row/column ranges and boundary rows).
+void fe_assemble(const double *b, const double *a, const double *f, const double *bcs,
+                 double *coeffs, double *rhs, long npoint);
+
+// eval_decomposition_grid on the GPU for one centre (src/horton/evaluate.cpp:129-223):
+// output[npoint] += synthesis of the given splines at the given host points.
+void decomposition_on_points(CubicSpline **sp, int nspline, int lmax, const double *center,
+                             const double *points, long npoint, double *output);
+
+// Per-Lebedev-rule projection table Yhat[j][lm] = aw_j S_lm(u_j) sqrt(4 pi (2l+1)) / sum aw.
+std::vector<double> projection_table(const AngularGrid &ang, int lmax);
+
+struct AtomInfo {
+  double origin[3];
+  double radii;
+  int nrad, nang;
+  RTransform rtf;               // copy
+  std::vector<double> r, rw;    // radial nodes and weights
+  std::vector<double> u, aw;    // Lebedev unit vectors and weights
+};
+
+struct PoissonPlan {
+  int lmax = -1;
+  DeviceBuffer<double> yhat, green;
+  DeviceBuffer<long> yhat_off, green_off;
+  DeviceBuffer<double> rho_lm, Uy, Udt, T, ampL, ampR;
+};
+
+struct DeviceGrid {
+  hipStream_t stream = nullptr;
+  long N = 0;
+  int M = 0;
+  std::vector<AtomInfo> atoms;
+  std::vector<long> offsets;  // [M+1]
+  // host mirrors (borrowed views returned by the getters)
+  std::vector<double> points, weights, becke, origin, radii;
+  // device arrays
+  DeviceBuffer<double> x, y, z, w, bw, wbw;
+  DeviceBuffer<int> owner;
+  DeviceBuffer<long> d_point_off, d_rad_off;
+  DeviceBuffer<int> d_nrad, d_nang, d_rkind;
+  DeviceBuffer<double> d_rp0, d_rp1, d_first, d_last, d_center, d_r;
+  int max_nrad = 0;
+  std::map<int, std::unique_ptr<PoissonPlan>> plans;
+
+  ~DeviceGrid();
+  void build(const std::vector<AtomInfo> &atoms);
+  AtomTables tables(const PoissonPlan &plan) const;
+  PoissonPlan &plan(int lmax);
+  // Poisson of dens (device [N]) into the plan's U / tables; then synthesis to v (device).
+  void poisson_expand(const double *d_dens, int lmax);
+  void poisson_synthesize(int lmax, double *d_v);
+};
+
+}  // namespace napmo

@@ -1,0 +1,26 @@
+// Explicit instantiations of the synthesis kernel for lmax 0..6.
+#include "synth_kernel.hpp"
+
+namespace napmo {
+template void launch_synth_t<0>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
+                                  const double *, const double *, long, const double *,
+                                  const double *, const double *, int, int, double *);
+template void launch_synth_t<1>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
+                                  const double *, const double *, long, const double *,
+                                  const double *, const double *, int, int, double *);
+template void launch_synth_t<2>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
+                                  const double *, const double *, long, const double *,
+                                  const double *, const double *, int, int, double *);
+template void launch_synth_t<3>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
+                                  const double *, const double *, long, const double *,
+                                  const double *, const double *, int, int, double *);
+template void launch_synth_t<4>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
+                                  const double *, const double *, long, const double *,
+                                  const double *, const double *, int, int, double *);
+template void launch_synth_t<5>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
+                                  const double *, const double *, long, const double *,
+                                  const double *, const double *, int, int, double *);
+template void launch_synth_t<6>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
+                                  const double *, const double *, long, const double *,
+                                  const double *, const double *, int, int, double *);
+}  // namespace napmo

@@ -1,0 +1,14 @@
+// Explicit instantiations of the synthesis kernel for lmax 12..14.
+#include "synth_kernel.hpp"
+
+namespace napmo {
+template void launch_synth_t<12>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
+                                  const double *, const double *, long, const double *,
+                                  const double *, const double *, int, int, double *);
+template void launch_synth_t<13>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
+                                  const double *, const double *, long, const double *,
+                                  const double *, const double *, int, int, double *);
+template void launch_synth_t<14>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
+                                  const double *, const double *, long, const double *,
+                                  const double *, const double *, int, int, double *);
+}  // namespace napmo

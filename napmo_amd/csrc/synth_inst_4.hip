@@ -1,0 +1,11 @@
+// Explicit instantiations of the synthesis kernel for lmax 18..19.
+#include "synth_kernel.hpp"
+
+namespace napmo {
+template void launch_synth_t<18>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
+                                  const double *, const double *, long, const double *,
+                                  const double *, const double *, int, int, double *);
+template void launch_synth_t<19>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
+                                  const double *, const double *, long, const double *,
+                                  const double *, const double *, int, int, double *);
+}  // namespace napmo

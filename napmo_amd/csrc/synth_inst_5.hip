@@ -1,0 +1,11 @@
+// Explicit instantiations of the synthesis kernel for lmax 20..21.
+#include "synth_kernel.hpp"
+
+namespace napmo {
+template void launch_synth_t<20>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
+                                  const double *, const double *, long, const double *,
+                                  const double *, const double *, int, int, double *);
+template void launch_synth_t<21>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
+                                  const double *, const double *, long, const double *,
+                                  const double *, const double *, int, int, double *);
+}  // namespace napmo

@@ -1,0 +1,83 @@
+"""Device-resident grid Fock build: the new fast entry point of the grid hot path.
+
+``GridFock(grid, basis, lmax, precision)`` collocates Phi once per geometry (on the
+GPU, point-major, padded to the MFMA tile) and builds the Poisson plan; each
+``build(P)`` then runs rho = diag(Phi P Phi^T), the Becke-Dickson Poisson solve,
+F = Phi^T diag(w bw v) Phi and the FP64 trace E = 1/2 sum P o F, all on the GPU.
+This is the contraction of src/wavefunction/nwavefunction.cpp:77-107 applied to the
+potential of napmo/wavefunction/ntwobody.py:15-46 with the density of
+napmo/wavefunction/psi_hybrid.py:144-154.
+"""
+import ctypes
+
+import numpy as np
+
+from napmo_amd.system import cext
+
+PRECISIONS = {'fp64': 0, 'fp32': 1, 'bf16x3': 2}
+
+
+class GridFock(object):
+
+    def __init__(self, grid, basis, lmax, precision='fp64'):
+        cext.require_gpu()
+        self.grid = grid      # keeps the grid handle alive for the plan's lifetime
+        self.basis = basis
+        self.lmax = int(lmax)
+        self.precision = precision
+        plan = ctypes.c_void_p()
+        cext.check(cext.napmo_fock_plan_new(grid._this, basis._this, self.lmax, PRECISIONS[precision],
+                                            ctypes.byref(plan)))
+        self._this = plan.value
+        n, nb, nc, lm = ctypes.c_long(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        cext.check(cext.napmo_fock_plan_info(self._this, ctypes.byref(n), ctypes.byref(nb),
+                                             ctypes.byref(nc), ctypes.byref(lm)))
+        self.npoint, self.nbasis = n.value, nb.value
+
+    def __del__(self):
+        if getattr(self, '_this', None):
+            cext.napmo_fock_plan_del(self._this)
+            self._this = None
+
+    def build(self, P, want_grid=False):
+        """Returns (F [n x n], E_trace, E_grid[, rho, v]) for a host density matrix P."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        n = self.nbasis
+        assert P.shape == (n, n)
+        F = np.zeros((n, n))
+        E = np.zeros(2)
+        rho = np.zeros(self.npoint) if want_grid else None
+        v = np.zeros(self.npoint) if want_grid else None
+        ptr = lambda a: None if a is None else a.ctypes.data  # noqa: E731
+        cext.check(cext.napmo_grid_fock_build(self._this, ptr(P), ptr(F), ptr(E), ptr(rho), ptr(v)))
+        if want_grid:
+            return F, E[0], E[1], rho, v
+        return F, E[0], E[1]
+
+    def build_device(self, dP, dF, dE):
+        """Asynchronous build on device pointers (ints), on the plan's stream."""
+        cext.check(cext.napmo_grid_fock_build_device(self._this, dP, dF, dE))
+
+    def overlap(self):
+        """S = Phi^T diag(w bw) Phi from the resident Phi."""
+        S = np.zeros((self.nbasis, self.nbasis))
+        cext.check(cext.napmo_grid_overlap(self._this, S))
+        return S
+
+    def profile(self, max_builds):
+        cext.check(cext.napmo_fock_plan_profile(self._this, max_builds))
+
+    def profile_read(self):
+        """{phase name: average ms} over the builds recorded since ``profile``."""
+        avg = np.zeros(16)
+        nb = ctypes.c_int()
+        cext.check(cext.napmo_fock_plan_profile_read(self._this, avg, 16, ctypes.byref(nb)))
+        names = [cext.napmo_fock_phase_name(k).decode() for k in range(16)]
+        return {nm: float(a) for nm, a in zip(names, avg) if nm}, nb.value
+
+    @property
+    def stream(self):
+        return cext.napmo_fock_plan_stream(self._this)
+
+    def synchronize(self):
+        cext.check(cext.napmo_fock_plan_sync(self._this))

@@ -1,0 +1,99 @@
+"""Becke multicentre grid (napmo/grids/becke.py:19-164, src/grid/becke.cpp).
+
+The Becke weights are computed on the GPU when the grid is built. Beyond the
+reference, atomic grids may differ in size: pass ``sizes=[(nrad, nang), ...]`` (one
+per centre); with equal sizes the result is the reference's.
+"""
+import ctypes
+
+import numpy as np
+
+from napmo_amd.grids.atomic import AtomicGrid
+from napmo_amd.system import cext
+
+
+class BeckeGrid(object):
+
+    def __init__(self, species, n_radial=40, n_angular=110, rtransform=None, file=None, ablmax=None,
+                 abldep=None, sizes=None):
+        assert isinstance(species, dict)
+        if file is not None:
+            raise NotImplementedError('Lowdin2 grid files are outside the grid hot path')
+        centers = species.get('particles')
+        ncenter = len(centers)
+        self._symbol = species.get('symbol')
+        ablmax = 0 if ablmax is None else ablmax
+        abldep = 1.0e-6 if abldep is None else abldep
+        self._nrad = n_radial
+        self._nang = n_angular
+        self.atgrids = []
+        for k, center in enumerate(centers):
+            nr, na = (n_radial, n_angular) if sizes is None else sizes[k]
+            self.atgrids.append(AtomicGrid(nr, na, center.get('origin'), center.get('symbol'),
+                                           rtransform=rtransform))
+        atgrids_ptr = np.array([g._this for g in self.atgrids], dtype=ctypes.c_void_p)
+        self._this = cext.BeckeGrid_new(atgrids_ptr, ncenter, ablmax, abldep)
+
+    def __del__(self):
+        if getattr(self, '_this', None):
+            cext.BeckeGrid_del(self._this)
+            self._this = None
+
+    def evaluate_decomposition(self, atom, cubic_splines, output, cell=None):
+        """output += sum_lm U_lm(|r - R_atom|) Y_lm at every grid point (GPU synthesis)."""
+        from napmo_amd.utilities.cell import Cell
+        if cell is None:
+            cell = Cell(None)
+        splines = np.array([cs._this for cs in cubic_splines], dtype=ctypes.c_void_p)
+        cext.eval_decomposition_grid(splines, np.ascontiguousarray(self.origin[atom, :]), output,
+                                     np.ascontiguousarray(self.points), cell._this, len(cubic_splines),
+                                     self.size)
+
+    def integrate(self, f):
+        return cext.BeckeGrid_integrate(self._this, np.ascontiguousarray(f, dtype=np.float64))
+
+    def show(self):
+        print('\nGrid Information:', self._symbol)
+        print('-' * (18 + len(str(self._symbol))))
+        print('Centers: ', self.ncenter)
+        print('Size: ', self.size)
+
+    @property
+    def becke_weights(self):
+        return cext.as_array(cext.BeckeGrid_get_becke_weights(self._this), (self.size,))
+
+    @property
+    def points(self):
+        return cext.as_array(cext.BeckeGrid_get_points(self._this), (self.size, 3))
+
+    @property
+    def weights(self):
+        return cext.as_array(cext.BeckeGrid_get_weights(self._this), (self.size,))
+
+    @property
+    def origin(self):
+        return cext.as_array(cext.BeckeGrid_get_origin(self._this), (self.ncenter, 3))
+
+    @property
+    def size(self):
+        return cext.BeckeGrid_get_size(self._this)
+
+    @property
+    def radii(self):
+        return cext.as_array(cext.BeckeGrid_get_radii(self._this), (self.ncenter,))
+
+    @property
+    def ncenter(self):
+        return cext.BeckeGrid_get_ncenter(self._this)
+
+    @property
+    def nrad(self):
+        return self._nrad
+
+    @property
+    def nang(self):
+        return self._nang
+
+    @property
+    def symbol(self):
+        return self._symbol

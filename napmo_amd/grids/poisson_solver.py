@@ -1,0 +1,39 @@
+"""Becke-Dickson Poisson solver (napmo/grids/poisson_solver.py:18-118).
+
+Same call and result as the reference: ``U[atom][lm]`` are CubicSpline objects of the
+potential on each atom's radial grid, in the reference lm order. The projection and
+the radial solves run on the GPU in one pass over all centres: the Hermite
+finite-element operator of each (radial grid, l) is factorised once in FP64 into a
+Green matrix, so each radial solve is a small FP64 matrix product.
+"""
+import numpy as np
+
+from napmo_amd.grids.cubic_spline import CubicSpline
+from napmo_amd.grids.extrapolation import PotentialExtrapolation
+from napmo_amd.system import cext
+
+
+def poisson_solver(grid, _dens, lmax, sph_exp=None):
+    if sph_exp is not None:
+        raise NotImplementedError('precomputed spherical expansions are not supported')
+    dens = np.ascontiguousarray(_dens, dtype=np.float64)
+    L = (lmax + 1) ** 2
+    nrad_total = sum(g.radial_grid.size for g in grid.atgrids)
+    Uy = np.zeros((nrad_total, L))
+    Udt = np.zeros((nrad_total, L))
+    cext.check(cext.napmo_poisson_solve(grid._this, dens, lmax, Uy, Udt))
+    U = []
+    off = 0
+    for atgrid in grid.atgrids:
+        nr = atgrid.radial_grid.size
+        rtf = atgrid.radial_grid.rtransform
+        result = []
+        idx = 0
+        for l in range(lmax + 1):
+            for _m in range(-l, l + 1):
+                result.append(CubicSpline.from_dt(Uy[off:off + nr, idx].copy(), Udt[off:off + nr, idx].copy(),
+                                                  rtf, PotentialExtrapolation(l)))
+                idx += 1
+        off += nr
+        U.append(result)
+    return U

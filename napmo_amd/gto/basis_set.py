@@ -1,0 +1,80 @@
+"""Gaussian basis set (napmo/gto/basis_set.py:19-158): deMon2k loader, Cartesian
+component order xx, xy, xz, yy, yz, zz, and GPU collocation ``compute``."""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+from napmo_amd.gto.basis_io import parse_demon2k
+from napmo_amd.gto.contracted_gaussian import ContractedGaussian
+from napmo_amd.system import cext
+
+_TABLE = None
+
+
+def _basis_table():
+    global _TABLE
+    if _TABLE is None:
+        with open(os.path.join(os.path.dirname(__file__), '..', 'data', 'basis_sets.json')) as f:
+            _TABLE = json.load(f)
+    return _TABLE
+
+
+def cartesian_powers(l):
+    """Component order of napmo/gto/basis_set.py:122-143."""
+    return [(i, k - j, j) for k, i in enumerate(reversed(range(l + 1))) for j in range(k + 1)]
+
+
+class BasisSet(dict):
+
+    def __init__(self, basis_name, particle, origin=None, basis_file=None, shells=None):
+        super(BasisSet, self).__init__()
+        self['name'] = basis_name
+        self['symbol'] = particle
+        self['particle'] = None
+        self['cont'] = []
+        self['nbasis'] = 0
+        if origin is not None:
+            if shells is not None:
+                data = shells
+            elif basis_file:
+                data = self.load_file(particle, basis_file)
+            else:
+                try:
+                    data = _basis_table()[basis_name][particle]
+                except KeyError:
+                    raise ValueError('Basis for particle ' + particle + ' not found!')
+            self._build(particle, data, origin)
+            self._ptrs = np.array([c._this for c in self['cont']], dtype=ctypes.c_void_p)
+            self._this = cext.BasisSet_new(self._ptrs, self['nbasis'])
+        else:
+            self._this = cext.BasisSet_new_empty()
+
+    def update(self, other):
+        self['nbasis'] += other.get('nbasis')
+        self['cont'] += other.get('cont')
+        cext.BasisSet_update(self._this, other._this)
+
+    def load_file(self, particle, basis_file):
+        with open(basis_file) as f:
+            return parse_demon2k(f.read(), particle)
+
+    def _build(self, particle, basis_data, origin):
+        self['cont'] = [ContractedGaussian(np.array(c['prim'], dtype=np.float64),
+                                           np.array(c['cont'], dtype=np.float64),
+                                           np.array(origin, dtype=np.float64),
+                                           np.array(lxyz, dtype=np.int32))
+                        for c in basis_data for lxyz in cartesian_powers(c['l'])]
+        self['nbasis'] = len(self['cont'])
+
+    def compute(self, coord=np.zeros([1, 3], dtype=np.float64)):
+        """Phi [n_coord x nbasis] on the GPU."""
+        coord = np.ascontiguousarray(coord, dtype=np.float64)
+        out = np.empty([coord.shape[0], self.get('nbasis')])
+        cext.BasisSet_compute(self._this, coord, out, coord.shape[0])
+        return out
+
+    @property
+    def lmax(self):
+        return int(max(c.l.sum() for c in self['cont']))

@@ -1,0 +1,31 @@
+"""Contracted Cartesian Gaussian (napmo/gto/contracted_gaussian.py, src/system/gto.cpp:161-227)."""
+import ctypes
+
+import numpy as np
+
+from napmo_amd.gto.primitive_gaussian import PrimitiveGaussian
+from napmo_amd.system import cext
+
+
+class ContractedGaussian(object):
+
+    def __init__(self, exponents=np.array([0.5], dtype=np.float64),
+                 coefficients=np.array([1.0], dtype=np.float64), origin=np.zeros(3, dtype=np.float64),
+                 l=np.zeros(3, dtype=np.int32)):
+        assert isinstance(exponents, (np.ndarray, list))
+        self._primitives = [PrimitiveGaussian(e, c, l, origin) for e, c in zip(exponents, coefficients)]
+        self._ptrs = np.array([p._this for p in self._primitives], dtype=ctypes.c_void_p)
+        self._this = cext.ContractedGaussian_new(self._ptrs, len(self._primitives))
+        self.l = np.ascontiguousarray(l, dtype=np.int32)
+        self.origin = np.ascontiguousarray(origin, dtype=np.float64)
+
+    @property
+    def nprim(self):
+        return len(self._primitives)
+
+    @property
+    def normalization(self):
+        return cext.ContractedGaussian_get_norma(self._this)
+
+    def compute(self, coord):
+        return self.normalization * sum(p.compute(coord) for p in self._primitives)

@@ -1,0 +1,69 @@
+"""Seeded synthetic workloads of the grid hot path (geometry, basis, density matrix),
+shared by the benchmark and the tests. All coordinates in bohr; seeds use
+numpy.random.default_rng (PCG64).
+
+* config 1: H2 at (0, 0, +-0.7), 2 x 100 x 110 Chebyshev/Lebedev, cc-pVDZ (n = 10), lmax 10
+* config 2: 16 atoms {C,H,N,O} in a 12 bohr box (min separation 1.8), 50 x 590 per atom
+  (472,000 points), 300 Cartesian s/p/d functions (30 cycles of s, p, d shells, cycle c on
+  atom c % 16, K ~ U{1,2,3} primitives, zeta log-uniform [0.1, 1e4], c ~ U(0.2, 1)),
+  P = 2 C C^T from 40 orthonormal occupied orbitals, lmax 12
+"""
+import json
+import os
+
+import numpy as np
+
+_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'data')
+with open(os.path.join(_DATA, 'basis_sets.json')) as _f:
+    BASIS = json.load(_f)
+
+H2_ATOMS = [('H', (0.0, 0.0, 0.7)), ('H', (0.0, 0.0, -0.7))]
+
+
+def random_molecule(rng, natom, box, elements=('C', 'H', 'N', 'O'), min_sep=1.8):
+    """Elements uniform over ``elements``; positions uniform in [0, box]^3, redrawn until
+    every pair is at least ``min_sep`` apart."""
+    syms = [elements[k] for k in rng.integers(0, len(elements), natom)]
+    while True:
+        pos = rng.uniform(0.0, box, (natom, 3))
+        d = np.sqrt(((pos[:, None, :] - pos[None, :, :]) ** 2).sum(-1))
+        d[np.diag_indices(natom)] = np.inf
+        if d.min() >= min_sep:
+            return list(zip(syms, [tuple(float(c) for c in p) for p in pos]))
+
+
+def random_shells(rng, natom, ncycle, lmax_shell=2):
+    """``ncycle`` cycles of shells l = 0..lmax_shell; cycle c goes to atom c % natom."""
+    per_atom = [[] for _ in range(natom)]
+    for c in range(ncycle):
+        for l in range(lmax_shell + 1):
+            k = int(rng.integers(1, 4))
+            zeta = np.exp(rng.uniform(np.log(0.1), np.log(1e4), k))
+            coef = rng.uniform(0.2, 1.0, k)
+            per_atom[c % natom].append({'l': l, 'prim': [float(z) for z in zeta],
+                                        'cont': [float(x) for x in coef]})
+    return per_atom
+
+
+def named_shells(atoms, basis_name):
+    return [BASIS[basis_name][s] for s, _ in atoms]
+
+
+def config1():
+    atoms = H2_ATOMS
+    return atoms, named_shells(atoms, 'CC-PVDZ'), 100, 110, 10
+
+
+def config2(seed=2, nrad=50, nang=590, natom=16, ncycle=30):
+    rng = np.random.default_rng(seed)
+    atoms = random_molecule(rng, natom, 12.0)
+    shells = random_shells(rng, natom, ncycle)
+    return atoms, shells, nrad, nang, 12
+
+
+def density_matrix(rng, S, nocc):
+    """P = 2 C C^T, C = C0 (C0^T S C0)^(-1/2), C0 ~ N(0, 1): nocc S-orthonormal orbitals."""
+    C0 = rng.normal(size=(S.shape[0], nocc))
+    w, V = np.linalg.eigh(C0.T @ S @ C0)
+    C = C0 @ (V @ np.diag(w ** -0.5) @ V.T)
+    return 2.0 * C @ C.T

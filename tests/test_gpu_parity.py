@@ -1,0 +1,255 @@
+"""GPU parity: every hot-path kernel through the C-ABI against the CPU oracle on the same
+seeded inputs. FP64 mode: tolerances are rounding-level (stated per test)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import cext as ocx
+from oracle import core
+from tests import systems
+
+pytestmark = pytest.mark.gpu
+
+GOLD = np.load(os.path.join(systems.ROOT, 'tests', 'golden', 'ref_vectors.npz'))
+
+
+def _mol(seed, natom=4, box=4.0):
+    return systems.random_molecule(np.random.default_rng(seed), natom, box, min_sep=1.2)
+
+
+def test_becke_weights_match_oracle():
+    atoms = _mol(11)
+    g, _ = systems.gpu_system(atoms, [[]] * len(atoms), 30, 50)
+    o = core.BeckeGrid(atoms, 30, 50)
+    print('becke N', g.size)
+    assert np.array_equal(g.points, o.points)
+    # FP64 rounding, amplified by the cancellation in 1 - s (s -> 1) of the reference formula
+    assert np.max(np.abs(g.becke_weights - o.becke_weights)) < 1e-12
+
+
+def test_becke_weights_uneven_grids():
+    atoms = _mol(12, natom=3)
+    sizes = [(20, 26), (35, 110), (15, 50)]
+    g, _ = systems.gpu_system(atoms, [[]] * 3, None, None, sizes=sizes)
+    o = core.BeckeGrid(atoms, None, None, sizes=sizes)
+    assert g.size == o.size == sum(a * b for a, b in sizes)
+    assert np.max(np.abs(g.becke_weights - o.becke_weights)) < 1e-12
+    # partition of unity on shared points: sum over owners at one physical point
+    pts = o.points[::53]
+    tot = sum(core.becke_weights(pts, o.origin, o.radii, [0] * (k + 1) + [len(pts)] * (3 - k)) for k in range(3))
+    assert np.allclose(tot, 1.0, atol=1e-13)
+
+
+def test_collocation_matches_oracle():
+    atoms = _mol(13)
+    rng = np.random.default_rng(13)
+    shells = systems.random_shells(rng, len(atoms), 6)
+    g, b = systems.gpu_system(atoms, shells, 12, 26)
+    o, ob = systems.oracle_system(atoms, shells, 12, 26)
+    phi = b.compute(g.points)
+    ref = ocx.basis_compute(ob, o.points)
+    print('collocation', phi.shape)
+    scale = np.abs(ref).max(axis=0)
+    assert np.max(np.abs(phi - ref) / scale) < 1e-13
+
+
+def test_spherical_expansion_matches_oracle_and_closed_forms():
+    import napmo_amd
+    rtf = napmo_amd.PowerRadialTransform(1e-3, 2e1, 100)
+    origin = np.array([0.3, -0.4, 0.2])
+    ag = napmo_amd.AtomicGrid(100, 110, origin, 'H', rtransform=rtf)
+    d = np.sqrt(((ag.points - origin) ** 2).sum(1))
+    r = ag.radial_grid.points
+    sa = ag.spherical_expansion(4, np.exp(-2 * d) / np.pi)
+    assert np.abs(sa[:, 0] - np.exp(-2 * r) / np.pi * np.sqrt(4 * np.pi)).max() < 1e-10
+    assert np.abs(sa[:, 1:]).max() < 1e-10
+    z = ag.points[:, 2] - origin[2]
+    f = np.exp(-d) / (32.0 * np.pi) * z ** 2
+    sa = ag.spherical_expansion(6, f)
+    assert np.abs(sa[:, 4] - np.exp(-r) / (32 * np.pi) * 2 / 15 * r ** 2 * np.sqrt(20 * np.pi)).max() < 1e-10
+    og = core.AtomicGrid(100, 110, origin, 'H', rtransform=core.PowerRTransform(1e-3, 2e1, 100))
+    ref = core.spherical_expansion(og, f, 6)
+    assert np.max(np.abs(sa - ref)) < 1e-13 * np.abs(ref).max()
+
+
+def _h2_system(nrad=100, nang=110):
+    atoms = systems.H2_ATOMS
+    shells = systems.named_shells(atoms, 'CC-PVDZ')
+    return atoms, shells, nrad, nang
+
+
+def test_poisson_solver_matches_oracle():
+    import napmo_amd
+    atoms, shells, nr, na = _h2_system(60, 110)
+    g, b = systems.gpu_system(atoms, shells, nr, na)
+    o, ob = systems.oracle_system(atoms, shells, nr, na)
+    phi = ocx.basis_compute(ob, o.points)
+    rho = (phi[:, 0] + 0.3 * phi[:, 4] + 0.2 * phi[:, 7]) ** 2
+    lmax = 10
+    U = napmo_amd.poisson_solver(g, rho, lmax)
+    Uo = core.poisson_solver(o, rho, lmax)
+    worst = 0.0
+    for a in range(2):
+        for lm in range(len(Uo[a])):
+            ys, yo = U[a][lm].y, Uo[a][lm].y
+            scale = max(np.abs(Uo[a][0].y).max(), 1e-30)
+            worst = max(worst, np.max(np.abs(ys - yo)) / scale, np.max(np.abs(U[a][lm].dt - Uo[a][lm].dt)) / scale)
+    print('poisson worst rel', worst)
+    assert worst < 1e-11
+
+
+def test_compute_coulomb_matches_oracle_and_reference_constant():
+    import napmo_amd
+    sto = systems.BASIS['STO-3G']['H'][0]
+    atoms = [('H', (0.0, 0.0, 0.0))]
+    g, b = systems.gpu_system(atoms, [[sto]], 100, 110)
+    rho = b.compute(g.points)[:, 0] ** 2
+    v = napmo_amd.compute_coulomb(g, rho, 8)
+    integral = g.integrate(rho * v)
+    print('coulomb integral', integral)
+    assert np.allclose(integral, 0.77461)          # napmo/grids/test/test_coulomb.py:56
+    o, ob = systems.oracle_system(atoms, [[sto]], 100, 110)
+    vo = core.compute_coulomb(o, rho, 8)
+    assert np.max(np.abs(v - vo)) < 1e-11 * np.abs(vo).max()
+    # the reference's own composition (poisson_solver + evaluate_decomposition) agrees
+    U = napmo_amd.poisson_solver(g, rho, 8)
+    v2 = np.zeros(g.size)
+    g.evaluate_decomposition(0, U[0], v2)
+    assert np.max(np.abs(v2 - vo)) < 1e-11 * np.abs(vo).max()
+
+
+def test_eval_decomposition_matches_reference_vectors():
+    import ctypes
+    import napmo_amd
+    from napmo_amd.system import cext
+    lmax = int(GOLD['dec_lmax'][0])
+    nrad = GOLD['dec_y'].shape[1]
+    rtf = napmo_amd.ChebyshevRadialTransform(float(GOLD['dec_radii'][0]), nrad)
+    sp = [napmo_amd.CubicSpline.from_dt(GOLD['dec_y'][s], GOLD['dec_dt'][s], rtf,
+                                        napmo_amd.PotentialExtrapolation(int(math.isqrt(s))))
+          for s in range((lmax + 1) ** 2)]
+    pts = np.ascontiguousarray(GOLD['dec_points'])
+    out = np.zeros(len(pts))
+    cext.eval_decomposition_grid(np.array([s._this for s in sp], dtype=ctypes.c_void_p),
+                                 np.ascontiguousarray(GOLD['dec_center']), out, pts, napmo_amd.Cell()._this,
+                                 len(sp), len(pts))
+    ref = GOLD['dec_out']
+    assert np.max(np.abs(out - ref)) < 1e-12 * np.abs(ref).max()
+
+
+def test_decomposition_resynthesis():
+    # napmo/grids/test/test_decomposition.py:20-64 (one O atom, 6-31G; density expanded,
+    # splined with the default cusp extrapolation and re-synthesised on the grid)
+    import napmo_amd
+    atoms = [('O', (0.0, 0.0, -1.0))]
+    shells = [systems.BASIS['6-31G']['O']]
+    for nrad in (110, 170):
+        g, b = systems.gpu_system(atoms, shells, nrad, 110)
+        lmax = systems.lebedev_lmax(110)
+        rho = b.compute(g.points).sum(axis=1) * g.becke_weights
+        p_lm = g.atgrids[0].spherical_expansion(lmax, rho)
+        rtf = g.atgrids[0].radial_grid.rtransform
+        splines = [napmo_amd.CubicSpline(np.array(p_lm[:, k]), rtransform=rtf) for k in range(p_lm.shape[1])]
+        rho_n = np.zeros(g.size)
+        g.evaluate_decomposition(0, splines, rho_n)
+        assert np.allclose(rho_n, rho)
+
+
+def test_contractions_match_oracle():
+    import napmo_amd
+    from napmo_amd.wavefunction import grid_ops
+    atoms, shells, nr, na = _h2_system(40, 50)
+    g, b = systems.gpu_system(atoms, shells, nr, na)
+    o, ob = systems.oracle_system(atoms, shells, nr, na)
+    rng = np.random.default_rng(21)
+    phi = ocx.basis_compute(ob, o.points)                   # [N, n]
+    n = phi.shape[1]
+    J = rng.normal(size=g.size)
+    K = rng.normal(size=(g.size, n))
+    psi = napmo_amd.PSIB(n, eta=2.0, kappa=-1.0)
+    res = grid_ops.compute_coupling(psi, g, np.ascontiguousarray(phi.T), J)
+    ref = core.coupling_matrix(phi, J, o.becke_weights, o.weights)
+    assert np.max(np.abs(res - ref)) < 1e-12 * np.abs(ref).max()
+    G = grid_ops.compute_2body_matrix(psi, g, np.ascontiguousarray(phi.T), J, np.ascontiguousarray(K.T))
+    Gref = core.two_body_matrix(phi, J, K, psi.x_factor, o.becke_weights, o.weights)
+    assert np.max(np.abs(G - Gref)) < 1e-12 * np.abs(Gref).max()
+    XC = grid_ops.compute_xc_matrix(psi, g, np.ascontiguousarray(phi.T), J)
+    assert np.max(np.abs(XC - ref)) < 1e-12 * np.abs(ref).max()
+
+
+def _config1():
+    atoms, shells, nr, na = _h2_system(100, 110)
+    o, ob = systems.oracle_system(atoms, shells, nr, na)
+    phi = ocx.basis_compute(ob, o.points)
+    S = core.coupling_matrix(phi, np.ones(o.size), o.becke_weights, o.weights)
+    P = systems.density_matrix(np.random.default_rng(1), S, 1)
+    return atoms, shells, nr, na, o, phi, P
+
+
+def test_grid_fock_fp64_config1_energy_parity():
+    # H2 (R = 1.4 bohr), 2 x 100 x 110 Chebyshev/Lebedev points, cc-pVDZ (n = 10), lmax 10
+    import napmo_amd
+    atoms, shells, nr, na, o, phi, P = _config1()
+    g, b = systems.gpu_system(atoms, shells, nr, na)
+    fock = napmo_amd.GridFock(g, b, 10, 'fp64')
+    F, E, Egrid, rho, v = fock.build(P, want_grid=True)
+    rho_o, v_o, F_o, E_o = core.grid_fock_build(o, phi, P, 10)
+    print('config1 E_gpu %.15f E_oracle %.15f dE %.3e' % (E, E_o, E - E_o))
+    assert abs(E - E_o) < 1e-8                       # Eh
+    assert np.max(np.abs(F - F_o)) < 1e-8            # Eh
+    assert abs(Egrid - E) < 1e-10
+    assert np.max(np.abs(rho - rho_o)) < 1e-13 * np.abs(rho_o).max()
+    assert np.max(np.abs(v - v_o)) < 1e-11 * np.abs(v_o).max()
+    assert abs(g.integrate(rho) - 2.0) < 1e-12        # int rho = tr(P S) = 2 electrons
+    # repeated builds are deterministic
+    F2, E2, _ = fock.build(P)
+    assert np.array_equal(F, F2) and E == E2
+
+
+def test_grid_fock_linearity_and_symmetry():
+    import napmo_amd
+    atoms, shells, nr, na, o, phi, P = _config1()
+    g, b = systems.gpu_system(atoms, shells, nr, na)
+    fock = napmo_amd.GridFock(g, b, 10, 'fp64')
+    rng = np.random.default_rng(5)
+    Q = rng.normal(size=P.shape)
+    Q = 1e-2 * (Q + Q.T)
+    Fa, _, _ = fock.build(P)
+    Fb, _, _ = fock.build(Q)
+    Fab, _, _ = fock.build(P + 2.0 * Q)
+    assert np.max(np.abs(Fab - (Fa + 2.0 * Fb))) < 1e-12 * np.abs(Fa).max()
+    assert np.array_equal(Fa, Fa.T)
+
+
+def test_grid_fock_config2_properties():
+    # full config-2 size: 16 atoms, 50 x 590 per atom (472,000 points), 300 s/p/d functions,
+    # lmax 12. Size-independent checks: int rho = tr(P S) = 80, symmetry, trace identity.
+    import napmo_amd
+    from napmo_amd.wavefunction import grid_ops
+    atoms, shells, nr, na, lmax = systems.config2()
+    g, b = systems.gpu_system(atoms, shells, nr, na)
+    print('config2 N', g.size, 'n', b['nbasis'])
+    assert g.size == 472000 and b['nbasis'] == 300
+    fock = napmo_amd.GridFock(g, b, lmax, 'fp64')
+    S, _, _ = fock_overlap(fock, b['nbasis'])
+    P = systems.density_matrix(np.random.default_rng(2), S, 40)
+    F, E, Egrid, rho, v = fock.build(P, want_grid=True)
+    print('config2 E %.12f Egrid %.12f' % (E, Egrid))
+    assert abs(g.integrate(rho) - 80.0) < 1e-9
+    assert np.array_equal(F, F.T)
+    assert abs(E - Egrid) < 1e-9 * abs(E)
+    assert E > 0
+
+
+def fock_overlap(fock, n):
+    """S = Phi^T diag(w bw) Phi from the same plan: the F of a unit potential is obtained
+    by the legacy contraction on the plan's grid."""
+    import napmo_amd
+    from napmo_amd.wavefunction import grid_ops
+    g, b = fock.grid, fock.basis
+    phi = b.compute(g.points)
+    psi = napmo_amd.PSIB(n)
+    S = grid_ops.compute_coupling(psi, g, np.ascontiguousarray(phi.T), np.ones(g.size))
+    return S, phi, psi

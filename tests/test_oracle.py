@@ -1,0 +1,248 @@
+"""Pins the CPU oracle (oracle/core.py, oracle/napmo_oracle.c) against the reference's
+own routines (golden vectors from oracle/_ref, and oracle/_ref itself when built),
+the reference tests' constants, and closed forms. CPU only."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import core, cext
+from tests import systems
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), 'golden', 'ref_vectors.npz'))
+
+
+def test_lebedev_matches_reference_generator():
+    for n in [6, 14, 26, 38, 50, 74, 86, 110, 146, 170, 194, 230, 266, 302, 350, 434, 590]:
+        xyz, w = core.lebedev(n)
+        assert np.array_equal(xyz, GOLD['leb_%d_xyz' % n]), n
+        assert np.array_equal(w, GOLD['leb_%d_w' % n]), n
+
+
+@pytest.mark.parametrize('n', [50, 110, 194, 302, 590])
+def test_lebedev_exactness(n):
+    # sum_j aw_j Y_lm Y_l'm' = delta / (4 pi) up to the rule's degree (orthonormal Y)
+    xyz, w = core.lebedev(n)
+    order = core.lebedev_get_order(n)
+    lmax = order // 2
+    S = np.vstack([np.ones(n), core.pure_polynomials(xyz[:, 2], xyz[:, 0], xyz[:, 1], lmax)])
+    lvec = np.repeat(np.arange(lmax + 1), 2 * np.arange(lmax + 1) + 1)
+    Y = S * np.sqrt(2 * lvec + 1)[:, None]          # Racah -> orthonormal up to 1/sqrt(4 pi)
+    G = (Y * w) @ Y.T
+    assert abs(w.sum() - 1.0) < 1e-14
+    assert np.allclose(G, np.eye(len(lvec)), atol=1e-12)
+
+
+def test_lebedev_order_map():
+    # napmo/grids/test/test_lebedev.py:10-18
+    assert core.lebedev_get_order(110) == 17
+    with pytest.raises(KeyError):
+        core.lebedev_get_order(2)
+
+
+def test_radial_transforms_match_reference():
+    t = np.arange(100, dtype=float)
+    cases = {'cheb': core.ChebyshevRTransform(0.5858151015155881, 100),
+             'power': core.PowerRTransform(1e-3, 2e1, 100)}
+    for name, rtf in cases.items():
+        for which in ('radius', 'deriv', 'deriv2', 'deriv3'):
+            ref = GOLD['rtf_%s_%s' % (name, which)]
+            mine = getattr(rtf, which)(t)
+            assert np.max(np.abs(mine - ref) / np.abs(ref)) < 1e-14, (name, which)
+        r = GOLD['rtf_%s_radius' % name]
+        assert np.max(np.abs(rtf.inv(r) - GOLD['rtf_%s_inv' % name])) < 1e-12
+
+
+def test_he_radial_grid_golden():
+    # napmo/wavefunction/test/test_kinetic.py:64-65 (He, 50 Chebyshev points)
+    rdat = np.array([5.0226280914746524E-004, 2.0128708010486813E-003, 4.5433445281719814E-003,
+                     8.1130928327705666E-003, 1.2749730838578131E-002, 1.8489537628088734E-002,
+                     1.5873699524938556, 557.42029873820456])
+    idx = [0, 1, 2, 3, 4, 5, 33, 49]
+    rtf = core.ChebyshevRTransform(core.atomic_radius('He'), 50)
+    r, _ = core.radial_grid(rtf)
+    assert np.allclose(r[idx], rdat, rtol=1e-13)
+
+
+def test_radial_gaussian_integral():
+    # napmo/grids/test/test_radial.py:13-31: 4 pi int r^2 exp(-r^2/2) = (2 pi)^(3/2)
+    for rtf in (core.PowerRTransform(0.0005, 1e1, 100), core.ChebyshevRTransform(core.atomic_radius('--'), 100)):
+        r, rw = core.radial_grid(rtf)
+        assert abs(np.sum(np.exp(-0.5 * r ** 2) * rw) - (2 * np.pi) ** 1.5) < 1e-9
+
+
+def test_spline_and_ode2_match_reference():
+    assert np.array_equal(core.spline_dt(GOLD['spline_y']), GOLD['spline_dt'])
+    args = (GOLD['ode_b'], GOLD['ode_a'], GOLD['ode_f'], GOLD['ode_bcs'], 12)
+    c1, r1 = core.build_ode2(*args)
+    c2, r2 = cext.build_ode2(*args)
+    assert np.array_equal(c1, GOLD['ode_coeffs']) and np.array_equal(r1, GOLD['ode_rhs'])
+    assert np.array_equal(c2, GOLD['ode_coeffs']) and np.array_equal(r2, GOLD['ode_rhs'])
+
+
+def test_solid_harmonics_match_reference():
+    for v, ref in zip(GOLD['pure_vecs'], GOLD['pure_polys']):
+        mine = core.pure_polynomials(v[2], v[0], v[1], 6)
+        assert np.allclose(mine, ref[:48], rtol=1e-14, atol=1e-14)
+
+
+def test_dot_multi_moments_match_reference():
+    pts, f, center = GOLD['mom_points'], GOLD['mom_f'], GOLD['dec_center']
+    d = pts - center
+    r = np.sqrt((d * d).sum(1))
+    S = np.vstack([np.ones(len(r)), core.pure_polynomials(d[:, 2] / r, d[:, 0] / r, d[:, 1] / r, 3)])
+    mine = np.array([(S[:, k * 10:(k + 1) * 10] * f[k * 10:(k + 1) * 10]).sum(1) for k in range(3)])
+    assert np.allclose(mine, GOLD['mom_out'], rtol=1e-13, atol=1e-14)
+
+
+def _golden_splines():
+    lmax = int(GOLD['dec_lmax'][0])
+    rtf = core.ChebyshevRTransform(float(GOLD['dec_radii'][0]), GOLD['dec_y'].shape[1])
+    return [core.Spline(GOLD['dec_y'][s], GOLD['dec_dt'][s], rtf, core.PotentialExtrapolation(int(math.isqrt(s))))
+            for s in range((lmax + 1) ** 2)]
+
+
+def test_eval_decomposition_matches_reference():
+    sp = _golden_splines()
+    mine = core.eval_decomposition(sp, GOLD['dec_center'], GOLD['dec_points'])
+    c = cext.eval_decomposition(sp, GOLD['dec_center'], GOLD['dec_points'], np.zeros(len(GOLD['dec_points'])))
+    ref = GOLD['dec_out']
+    assert np.allclose(mine, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+    assert np.array_equal(c, ref)
+
+
+def test_collocation_matches_reference_and_constants():
+    basis = core.Basis()
+    basis.add_shell((0.1, 0.2, 0.3), 0, [13.01, 1.962, 0.4446], [0.019685, 0.137977, 0.478148])
+    basis.add_shell((0.1, 0.2, 0.3), 1, [0.727], [1.0])
+    basis.add_shell((-0.5, 0.4, 0.0), 2, [0.8, 2.5], [0.6, 0.4])
+    for phi in (core.basis_compute(basis, GOLD['gto_points']), cext.basis_compute(basis, GOLD['gto_points'])):
+        assert np.allclose(phi, GOLD['gto_phi'], rtol=1e-13, atol=1e-15)
+    # napmo/gto/test/test_primitive_gaussian.py:13-24, test_basis_set.py:25-29
+    assert abs(core.primitive_norm(0.5, (0, 0, 0)) - 0.423777208124) < 1e-12
+    assert abs(core.primitive_norm(0.5, (0, 0, 3)) - 0.309483114995) < 1e-12
+    b = core.Basis()
+    b.add_shell((0, 0, 0), 0, [0.5], [1.0])
+    assert abs(core.basis_compute(b, np.zeros((1, 3)))[0, 0] - 0.423777208124) < 1e-12
+    b = core.Basis()
+    b.add_shell((0, 0, 0), 3, [0.5], [1.0])
+    zzz = [k for k, lx in enumerate(core.cartesian_powers(3)) if lx == (0, 0, 3)][0]
+    assert abs(core.basis_compute(b, np.ones((1, 3)))[0, zzz] - 0.069055017012) < 1e-12
+    sto = systems.BASIS['STO-3G']['H'][0]
+    b = core.Basis()
+    b.add_shell((0, 0, 0), 0, sto['prim'], sto['cont'])
+    assert abs(core.basis_compute(b, np.zeros((1, 3)))[0, 0] - 0.62824688) < 1e-8
+
+
+def test_becke_partition_invariants():
+    rng = np.random.default_rng(7)
+    atoms = systems.random_molecule(rng, 4, 4.0, min_sep=1.2)
+    grid = core.BeckeGrid(atoms, 20, 26)
+    # per-point partition of unity: sum over owners of the same point = 1
+    pts = grid.points[::37]
+    total = np.zeros(len(pts))
+    for owner in range(grid.ncenter):
+        offs = np.array([0] * grid.ncenter + [len(pts)])
+        offs[:owner + 1] = 0
+        offs[owner + 1:] = len(pts)
+        total += core.becke_weights(pts, grid.origin, grid.radii, offs)
+    assert np.allclose(total, 1.0, atol=1e-13)
+    # brute force from Becke (1988) at a few points
+    for p in range(0, grid.size, 251):
+        owner = int(np.searchsorted(grid.offsets, p, side='right') - 1)
+        bf = core.becke_weights_pointwise(grid.points[p], owner, grid.origin, grid.radii)
+        assert abs(bf - grid.becke_weights[p]) < 1e-12
+    # C restatement equals numpy
+    assert np.array_equal(cext.becke_weights(grid.points, grid.origin, grid.radii, grid.offsets),
+                          grid.becke_weights)
+    # permutation of centres permutes nothing but the owner blocks
+    perm = [2, 0, 3, 1]
+    g2 = core.BeckeGrid([atoms[k] for k in perm], 20, 26)
+    for new, old in enumerate(perm):
+        a = g2.becke_weights[g2.offsets[new]:g2.offsets[new + 1]]
+        b = grid.becke_weights[grid.offsets[old]:grid.offsets[old + 1]]
+        assert np.allclose(a, b, atol=1e-14)
+
+
+def test_spherical_expansion_closed_forms():
+    # napmo/grids/test/test_spherical_average.py:38-85 (1s, 1pz densities)
+    rtf = core.PowerRTransform(1e-3, 2e1, 100)
+    origin = np.array([0.3, -0.4, 0.2])
+    ag = core.AtomicGrid(100, 110, origin, 'H', rtransform=rtf)
+    d = np.sqrt(((ag.points - origin) ** 2).sum(1))
+    sa = core.spherical_expansion(ag, np.exp(-2 * d) / np.pi, 4)
+    assert np.abs(sa[:, 0] - np.exp(-2 * ag.r) / np.pi * np.sqrt(4 * np.pi)).max() < 1e-10
+    assert np.abs(sa[:, 1:]).max() < 1e-10
+    z = ag.points[:, 2] - origin[2]
+    sa = core.spherical_expansion(ag, np.exp(-d) / (32.0 * np.pi) * z ** 2, 4)
+    r = ag.r
+    assert np.abs(sa[:, 0] - np.exp(-r) / (32 * np.pi) / 3 * r ** 2 * np.sqrt(4 * np.pi)).max() < 1e-10
+    assert np.abs(sa[:, 4] - np.exp(-r) / (32 * np.pi) * 2 / 15 * r ** 2 * np.sqrt(5 * 4 * np.pi)).max() < 1e-10
+
+
+def test_coulomb_self_energy_h_sto3g():
+    # napmo/grids/test/test_coulomb.py:24-56 (0.77461, allclose), and the closed form
+    # (ss|ss) = 2 pi^(5/2) sum c^4 / (p q sqrt(p+q)) for the one-centre 1s.
+    sto = systems.BASIS['STO-3G']['H'][0]
+    grid, basis = systems.oracle_system([('H', (0.0, 0.0, 0.0))], [[sto]], 100, 110)
+    phi = core.basis_compute(basis, grid.points)
+    rho = phi[:, 0] ** 2
+    v = core.compute_coulomb(grid, rho, systems.lebedev_lmax(110))
+    integral = grid.integrate(rho * v)
+    assert np.allclose(integral, 0.77461)
+    z = np.array(sto['prim'])
+    c = np.array(sto['cont']) * np.array([core.primitive_norm(q, (0, 0, 0)) for q in z])
+    c *= core.contraction_norm(sto['prim'], sto['cont'], (0, 0, 0))
+    closed = sum(c[i] * c[j] * c[k] * c[l] * 2 * np.pi ** 2.5 / ((z[i] + z[j]) * (z[k] + z[l]) * np.sqrt(z[i] + z[j] + z[k] + z[l]))
+                 for i in range(3) for j in range(3) for k in range(3) for l in range(3))
+    assert abs(integral - closed) < 1e-5
+
+
+def test_poisson_gaussian_closed_form():
+    # V of a normalised s Gaussian (alpha) is erf(sqrt(alpha) r) / r. The reference's
+    # finite-element scheme converges to it with the radial grid size; a dropped term,
+    # a wrong sign or a wrong boundary value would leave an O(1) error instead.
+    from scipy.special import erf
+    alpha = 1.3
+    errs = []
+    for nr in (80, 150):
+        ag = core.AtomicGrid(nr, 26, np.zeros(3), 'H')
+        d = np.sqrt((ag.points ** 2).sum(1))
+        rho = (alpha / np.pi) ** 1.5 * np.exp(-alpha * d * d)
+        U = core.poisson_radial(ag, core.spherical_expansion(ag, rho, 0), 0)
+        exact = erf(np.sqrt(alpha) * ag.r) / ag.r * np.sqrt(4 * np.pi)   # U_00 = sqrt(4 pi) V
+        mask = ag.r < 20
+        errs.append(np.max(np.abs(U[0].y[mask] - exact[mask]) / exact[mask]))
+    assert errs[0] < 3e-4 and errs[1] < errs[0] / 3
+
+
+def test_contraction_identities():
+    rng = np.random.default_rng(3)
+    atoms = systems.H2_ATOMS
+    grid, basis = systems.oracle_system(atoms, systems.named_shells(atoms, 'CC-PVDZ'), 30, 50)
+    phi = core.basis_compute(basis, grid.points)
+    v = rng.normal(size=grid.size)
+    M = core.coupling_matrix(phi, v, grid.becke_weights, grid.weights)
+    Mc = cext.coupling(np.ascontiguousarray(phi.T), v, grid.becke_weights, grid.weights)
+    direct = np.einsum('pi,pj,p->ij', phi, phi, v * grid.becke_weights * grid.weights)
+    assert np.allclose(M, direct, rtol=1e-12, atol=1e-14) and np.allclose(Mc, direct, rtol=1e-12, atol=1e-14)
+    P = rng.normal(size=(basis.nbasis, basis.nbasis))
+    P = P + P.T
+    rho = core.density_from_dm(phi, P)
+    assert np.allclose(cext.density(phi, P), rho, rtol=1e-12, atol=1e-14)
+    # trace identity: sum P o J(v) = sum_p w bw rho v
+    assert np.isclose(np.sum(P * M), np.sum(grid.weights * grid.becke_weights * rho * v), rtol=1e-12)
+
+
+def test_two_body_exchange_mirrors_upper_triangle():
+    rng = np.random.default_rng(4)
+    N, n = 40, 5
+    phi, K = rng.normal(size=(N, n)), rng.normal(size=(N, n))
+    J, bw, w = rng.normal(size=N), rng.uniform(size=N), rng.uniform(size=N)
+    G = core.two_body_matrix(phi, J, K, -0.5, bw, w)
+    E = np.einsum('pi,pj,p->ij', phi, K, bw * w)
+    expect = core.coupling_matrix(phi, J, bw, w) - 0.5 * (np.triu(E) + np.triu(E, 1).T)
+    assert np.allclose(G, expect)
+    assert np.allclose(G, G.T)
