@@ -27,6 +27,14 @@ PEAK = {'fp64_mfma': 78.6e12, 'fp32_mfma': 157.3e12, 'bf16_mfma': 2.5e15, 'hbm':
 METRIC = 'grid Fock builds/s + % roofline, mixed vs FP64 on 1 MI355X at |dE|<=1uEh'
 
 
+MODES = {  # name: (plan precision, density input)
+    'fp64': ('fp64', 'P'),
+    'fp64-orb': ('fp64', 'C'),
+    'bf16x3-orb': ('bf16x3', 'C'),
+    'bf16x3': ('bf16x3', 'P'),
+}
+
+
 def gpu_leg(args, log):
     import napmo_amd
     from napmo_amd import workloads
@@ -38,37 +46,52 @@ def gpu_leg(args, log):
     grid = napmo_amd.BeckeGrid(napmo_amd.make_species(atoms), nrad, nang)
     basis = napmo_amd.make_basis(atoms, shells_per_atom=shells)
     log('grid N=%d basis n=%d set-up %.1fs' % (grid.size, basis['nbasis'], time.perf_counter() - t0))
-    modes = ['fp64'] + [m for m in args.precision.split(',') if m and m != 'fp64']
+    modes = ['fp64'] + [m for m in args.modes.split(',') if m and m != 'fp64']
     plans = {}
     for m in modes:
-        try:
-            plans[m] = napmo_amd.GridFock(grid, basis, lmax, m)
-        except cext.NapmoError as e:
-            log('precision %s unavailable: %s' % (m, e))
+        prec = MODES[m][0]
+        if prec not in plans:
+            t0 = time.perf_counter()
+            try:
+                plans[prec] = napmo_amd.GridFock(grid, basis, lmax, prec)
+            except cext.NapmoError as e:
+                log('precision %s unavailable: %s' % (prec, e))
+                continue
+            log('plan %s set-up %.2fs' % (prec, time.perf_counter() - t0))
     S = plans['fp64'].overlap()
-    P = workloads.density_matrix(np.random.default_rng(2), S, 40)
+    C = workloads.occupied_orbitals(np.random.default_rng(2), S, 40)
+    P = 2.0 * C @ C.T
     n = basis['nbasis']
     dP = DeviceArray.from_host(P)
+    dC = DeviceArray.from_host(C)
     dF = DeviceArray((n, n))
     dE = DeviceArray((2,))
     results = {}
-    for m, plan in plans.items():
+    for m in modes:
+        prec, form = MODES[m]
+        if prec not in plans:
+            continue
+        plan = plans[prec]
+        if form == 'P':
+            step = lambda: plan.build_device(dP.ptr, dF.ptr, dE.ptr)  # noqa: E731
+        else:
+            step = lambda: plan.build_orbitals_device(dC.ptr, C.shape[1], 2.0, dF.ptr, dE.ptr)  # noqa: E731
         for _ in range(args.warmup):
-            plan.build_device(dP.ptr, dF.ptr, dE.ptr)
+            step()
         plan.synchronize()
         plan.profile(args.steps)
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            plan.build_device(dP.ptr, dF.ptr, dE.ptr)
+            step()
         plan.synchronize()
         dt = (time.perf_counter() - t0) / args.steps
         phases, nb = plan.profile_read()
         E = dE.to_host()
         F = dF.to_host()
         results[m] = {'ms_per_build': dt * 1e3, 'builds_per_s': 1.0 / dt, 'E': float(E[0]),
-                      'E_grid': float(E[1]), 'phases_ms': phases, 'F': F}
-        log('%s: %.3f ms/build  E=%.12f  phases=%s' % (m, dt * 1e3, E[0],
-                                                       {k: round(v, 4) for k, v in phases.items()}))
+                      'E_grid': float(E[1]), 'phases_ms': phases, 'F': F, 'form': form, 'prec': prec}
+        log('%s: %.3f ms/build  E_trace=%.12f E_grid=%.12f phases=%s' % (
+            m, dt * 1e3, E[0], E[1], {k: round(v, 4) for k, v in phases.items()}))
     N = grid.size
     return results, N, n, lmax, grid.ncenter
 
@@ -78,11 +101,21 @@ def roofline(res, N, n, L, M):
     ph = res['phases_ms']
     name = max(ph, key=ph.get)
     ms = ph[name]
-    if name in ('density', 'fock'):
-        flops = 2.0 * N * n * n
+    if name == 'density':
+        # rho from P: 2 N n^2; orbital form: 2 N n nocc (both FP64 MFMA)
+        flops = 2.0 * N * n * (n if res['form'] == 'P' else res['nocc'])
         peak = PEAK['fp64_mfma']
         return name, {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
                       'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / peak, 'traffic': None}
+    if name == 'fock':
+        # algorithmic 2 N n^2; the bf16x3 split runs 3 bf16 products of it
+        flops = 2.0 * N * n * n
+        if res['prec'] == 'bf16x3':
+            peak, work = PEAK['bf16_mfma'], 3.0 * flops
+        else:
+            peak, work = PEAK['fp64_mfma'], flops
+        return name, {'bound': 'mfma', 'achieved': work / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
+                      'unit': 'TFLOP/s', 'frac': work / (ms * 1e-3) / peak, 'traffic': None}
     if name == 'synth':
         # 2 flops per (point, centre, lm) of the survey's count; FP64 vector peak
         flops = 2.0 * N * M * L
@@ -151,15 +184,26 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--precision', default='fp64,bf16x3,fp32',
-                    help='comma list of modes to time besides fp64')
+    ap.add_argument('--modes', default='fp64,fp64-orb,bf16x3-orb',
+                    help='comma list of modes (%s); fp64 is always timed' % ', '.join(MODES))
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--copy-calibration', type=float, default=0.0, metavar='GIB',
+                    help='only run a plain device copy of GIB gibibytes 5 times (counter calibration)')
     ap.add_argument('--cpu-seconds', type=float, default=30.0)
     args = ap.parse_args()
 
     def log(msg):
         print('[bench] ' + msg, file=sys.stderr, flush=True)
 
+    if args.copy_calibration > 0:
+        from napmo_amd.device import DeviceArray
+        from napmo_amd.system import cext
+        n = int(args.copy_calibration * (1 << 30) / 8) // 2 * 2
+        src, dst = DeviceArray((n,)), DeviceArray((n,))
+        for _ in range(5):
+            cext.check(cext.napmo_device_copy(dst.ptr, src.ptr, n))
+        print(json.dumps({'copy_bytes_read': n * 8, 'copy_bytes_written': n * 8, 'launches': 5}))
+        return
     if args.gpus != 1:
         print('bench.py: this build runs the grid hot path on one GPU; --gpus %d is not supported '
               '(see the sharding design in DESIGN.md)' % args.gpus, file=sys.stderr)
@@ -169,12 +213,15 @@ def main():
     ref = results['fp64']
     best, dE = 'fp64', 0.0
     for m, r in results.items():
+        r['nocc'] = 40
         if m == 'fp64':
             continue
-        r['dE'] = r['E'] - ref['E']
+        # energy of the build: the FP64 grid trace 1/2 sum w bw rho v (= 1/2 tr(P F) exactly)
+        r['dE'] = r['E_grid'] - ref['E_grid']
+        r['dE_trace'] = r['E'] - ref['E']
         r['dF_max'] = float(np.max(np.abs(r['F'] - ref['F'])))
-        log('%s vs fp64: dE = %.3e Eh, max|dF| = %.3e Eh, speed-up %.2fx' %
-            (m, r['dE'], r['dF_max'], ref['ms_per_build'] / r['ms_per_build']))
+        log('%s vs fp64: dE_grid = %.3e Eh, dE_trace = %.3e Eh, max|dF| = %.3e Eh, speed-up %.2fx' %
+            (m, r['dE'], r['dE_trace'], r['dF_max'], ref['ms_per_build'] / r['ms_per_build']))
         if abs(r['dE']) <= 1e-6 and r['builds_per_s'] > results[best]['builds_per_s']:
             best, dE = m, r['dE']
     head = results[best]
@@ -193,7 +240,7 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': {'fp64': 'f64', 'fp32': 'f32+f64', 'bf16x3': 'bf16x3+f32+f64'}[best],
+        'dtype': 'bf16x3+f64' if 'bf16' in best else 'f64',
         'data': 'synthetic (seeded geometry, basis and density; no reference data files)',
         'config': {'workload': 'config 2: grid Fock build, 16 atoms, 50x590 Becke grid (472,000 points), '
                                '300 s/p/d Gaussians, lmax 12, P resident in HBM',
@@ -201,7 +248,9 @@ def main():
         'roofline': dict(roof, kernel=kname),
         'cpu_baseline': cpu,
         'modes': {m: {'builds_per_s': r['builds_per_s'], 'ms_per_build': r['ms_per_build'],
-                      'E': r['E'], 'dE_vs_fp64': r.get('dE', 0.0), 'phases_ms': r['phases_ms']}
+                      'E_grid': r['E_grid'], 'E_trace': r['E'], 'dE_grid_vs_fp64': r.get('dE', 0.0),
+                      'dE_trace_vs_fp64': r.get('dE_trace', 0.0), 'dF_max_vs_fp64': r.get('dF_max', 0.0),
+                      'phases_ms': r['phases_ms']}
                   for m, r in results.items()},
         'dE_vs_fp64': dE,
         'speedup_vs_fp64': ref['ms_per_build'] / head['ms_per_build'],

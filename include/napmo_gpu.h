@@ -181,6 +181,21 @@ int napmo_grid_fock_build(void *plan, const double *P, double *F_out, double *E_
                           double *rho_out, double *v_out);
 /* Same on device pointers, asynchronous on the plan's stream (no host sync). */
 int napmo_grid_fock_build_device(void *plan, const double *dP, double *dF, double *dE);
+/* Orbital form of the density (napmo/wavefunction/psi_numeric.py:361-369: Dgrid = eta psi^2
+ * with psi = Phi C): C [n x nocc] row-major, rho = eta sum_i (Phi C)_i^2 in FP64,
+ * P = eta C C^T for the trace. E_out[0] = 1/2 sum P o F, E_out[1] = 1/2 sum_p w bw rho v
+ * (FP64 in every mode; equal in exact arithmetic). */
+int napmo_grid_fock_build_orbitals(void *plan, const double *C, int nocc, double eta,
+                                   double *F_out, double *E_out, double *rho_out, double *v_out);
+int napmo_grid_fock_build_orbitals_device(void *plan, const double *dC, int nocc, double eta,
+                                          double *dF, double *dE);
+/* No reference counterpart: runs one wavefront through an MFMA builtin (which = 0:
+ * f64 16x16x4, 1: bf16 16x16x32) and writes D [16 x 16] with the lane layout the
+ * kernels assume, for the layout test. */
+int napmo_selftest_mfma(int which, double *out);
+/* No reference counterpart: plain device copy of n doubles (n even) with a streaming
+ * kernel, used to calibrate the HBM byte counters (FETCH_SIZE / WRITE_SIZE). */
+int napmo_device_copy(double *dst, const double *src, long n);
 /* Grid overlap S = Phi^T diag(w bw) Phi [n x n] from the plan's resident Phi (FP64).
  * Same contraction as nwavefunction_compute_coupling with other_J = 1. */
 int napmo_grid_overlap(void *plan, double *S_out);

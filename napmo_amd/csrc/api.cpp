@@ -461,6 +461,29 @@ int napmo_grid_fock_build(void *plan, const double *P, double *F_out, double *E_
 int napmo_grid_fock_build_device(void *plan, const double *dP, double *dF, double *dE) {
   return guarded([&] { ((FockPlan *)plan)->build_device(dP, dF, dE); });
 }
+int napmo_grid_fock_build_orbitals(void *plan, const double *C, int nocc, double eta,
+                                   double *F_out, double *E_out, double *rho_out, double *v_out) {
+  return guarded(
+      [&] { ((FockPlan *)plan)->build_orbitals_host(C, nocc, eta, F_out, E_out, rho_out, v_out); });
+}
+int napmo_grid_fock_build_orbitals_device(void *plan, const double *dC, int nocc, double eta,
+                                          double *dF, double *dE) {
+  return guarded([&] { ((FockPlan *)plan)->build_orbitals_device(dC, nocc, eta, dF, dE); });
+}
+int napmo_device_copy(double *dst, const double *src, long n) {
+  return guarded([&] {
+    launch_copy(src, dst, n, nullptr);
+    NAPMO_HIP(hipDeviceSynchronize());
+  });
+}
+int napmo_selftest_mfma(int which, double *out) {
+  return guarded([&] {
+    require_device();
+    DeviceBuffer<double> d(256);
+    launch_selftest_mfma(which, d.get(), nullptr);
+    NAPMO_HIP(hipMemcpy(out, d.get(), 256 * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
 int napmo_grid_overlap(void *plan, double *S_out) {
   return guarded([&] { ((FockPlan *)plan)->overlap(S_out); });
 }

@@ -25,8 +25,8 @@ void DeviceBasis::collocate(const double *x, const double *y, const double *z, l
 
 FockPlan::FockPlan(DeviceGrid &g, const BasisSet &b, int lmax_, int prec_mode)
     : grid(g), lmax(lmax_), prec(prec_mode) {
-  NAPMO_REQUIRE(prec >= 0 && prec <= 2, "unknown precision mode");
-  NAPMO_REQUIRE(prec == 0, "only the FP64 precision mode is available in this build");
+  NAPMO_REQUIRE(prec == NAPMO_PREC_FP64 || prec == NAPMO_PREC_BF16X3,
+                "precision mode not available: use NAPMO_PREC_FP64 (0) or NAPMO_PREC_BF16X3 (2)");
   FlatBasis fb = flatten(b);
   n = fb.nbasis;
   NAPMO_REQUIRE(n > 0, "empty basis");
@@ -37,7 +37,14 @@ FockPlan::FockPlan(DeviceGrid &g, const BasisSet &b, int lmax_, int prec_mode)
   phi64.resize((size_t)Npad * ld);
   basis.collocate(grid.x.get(), grid.y.get(), grid.z.get(), grid.N, Npad, ld, phi64.get(), nullptr,
                   st);
-  grid.plan(lmax);  // Poisson plan (Green matrices) built once
+  if (prec == NAPMO_PREC_BF16X3) {
+    phiT_hi.resize((size_t)ld * Npad);
+    phiT_lo.resize((size_t)ld * Npad);
+    phiT32.resize((size_t)ld * Npad);
+    launch_split_phi(phi64.get(), ld, Npad, phiT_hi.get(), phiT_lo.get(), phiT32.get(), st);
+  }
+  grid.plan(lmax);     // Poisson plan (Green matrices) built once
+  grid.synth_pairs();  // synthesis pair buckets built once per geometry
   P.resize((size_t)ld * ld);
   NAPMO_HIP(hipMemsetAsync(P.get(), 0, sizeof(double) * ld * ld, st));
   rho_part.resize((size_t)(ld / TILE) * Npad);
@@ -48,6 +55,7 @@ FockPlan::FockPlan(DeviceGrid &g, const BasisSet &b, int lmax_, int prec_mode)
   F_part.resize((size_t)fsh.nchunk * fsh.ntiles * TILE * TILE);
   F.resize((size_t)n * n);
   E.resize(2);
+  trace_work.resize(2 * TRACE_WORK);
   NAPMO_HIP(hipStreamSynchronize(st));
 }
 
@@ -95,8 +103,6 @@ int FockPlan::profile_read(double *avg_ms, int nphase) {
 void FockPlan::build_device(const double *dP, double *dF, double *dE) {
   hipStream_t st = grid.stream;
   const int slot = (prof_used < prof_cap) ? prof_used++ : -1;
-  PoissonPlan &pl = grid.plan(lmax);
-  AtomTables at = grid.tables(pl);
   // P (n x n) into the zero-padded ld x ld operand
   NAPMO_HIP(hipMemcpy2DAsync(P.get(), sizeof(double) * ld, dP, sizeof(double) * n,
                              sizeof(double) * n, n, hipMemcpyDeviceToDevice, st));
@@ -104,6 +110,32 @@ void FockPlan::build_device(const double *dP, double *dF, double *dE) {
   launch_density_f64(phi64.get(), ld, Npad, P.get(), rho_part.get(), st);
   mark(slot, 1);
   launch_rho_reduce(rho_part.get(), ld / TILE, Npad, grid.N, rho.get(), st);
+  finish(slot, dP, dF, dE);
+}
+
+void FockPlan::build_orbitals_device(const double *dC, int nocc, double eta, double *dF,
+                                     double *dE) {
+  NAPMO_REQUIRE(nocc > 0 && nocc <= n, "orbital count out of range");
+  hipStream_t st = grid.stream;
+  const int slot = (prof_used < prof_cap) ? prof_used++ : -1;
+  const int ldc = (int)round_up(nocc, TILE);
+  Cpad.resize((size_t)ld * ldc);
+  Pfull.resize((size_t)n * n);
+  if (rho_part.size() < (size_t)(ldc / TILE) * Npad) rho_part.resize((size_t)(ldc / TILE) * Npad);
+  launch_pad2d(dC, n, nocc, Cpad.get(), ld, ldc, st);
+  launch_gram(dC, n, nocc, eta, Pfull.get(), st);
+  mark(slot, 0);
+  launch_density_orb_f64(phi64.get(), ld, Npad, Cpad.get(), ldc, rho_part.get(), st);
+  mark(slot, 1);
+  launch_rho_reduce(rho_part.get(), ldc / TILE, Npad, grid.N, rho.get(), st, eta);
+  finish(slot, Pfull.get(), dF, dE);
+}
+
+void FockPlan::finish(int slot, const double *dP, double *dF, double *dE) {
+  hipStream_t st = grid.stream;
+  PoissonPlan &pl = grid.plan(lmax);
+  AtomTables at = grid.tables(pl);
+  SynthPairs &sp = grid.synth_pairs();
   mark(slot, 2);
   launch_project(at, rho.get(), grid.bw.get(), pl.yhat.get(), lmax, pl.rho_lm.get(), st);
   mark(slot, 3);
@@ -112,17 +144,35 @@ void FockPlan::build_device(const double *dP, double *dF, double *dE) {
   launch_synth_tables(at, pl.Uy.get(), pl.Udt.get(), lmax, pl.T.get(), pl.ampL.get(),
                       pl.ampR.get(), st);
   mark(slot, 5);
-  launch_synth(at, grid.x.get(), grid.y.get(), grid.z.get(), grid.N, pl.T.get(), pl.ampL.get(),
-               pl.ampR.get(), lmax, 0, v.get(), st);
+  launch_synth2(at, grid.x.get(), grid.y.get(), grid.z.get(), grid.N, sp.list.get(),
+                sp.wave_bucket.get(), sp.nwaves, pl.T.get(), pl.ampL.get(), pl.ampR.get(), lmax,
+                sp.contrib.get(), st);
+  launch_contrib_reduce(sp.contrib.get(), grid.M, grid.N, 0, v.get(), st);
   mark(slot, 6);
   launch_scale3(v.get(), grid.wbw.get(), nullptr, grid.N, Npad, s.get(), st);
   mark(slot, 7);
-  launch_fock_f64(fsh, phi64.get(), phi64.get(), s.get(), F_part.get(), st);
+  if (prec == NAPMO_PREC_BF16X3)
+    launch_fock_bf16x3(fsh, phiT_hi.get(), phiT_lo.get(), phiT32.get(), s.get(), F_part.get(), st);
+  else
+    launch_fock_f64(fsh, phi64.get(), phi64.get(), s.get(), F_part.get(), st);
   mark(slot, 8);
   launch_fock_reduce(fsh, F_part.get(), n, 0, 1.0, 0, dF, st);
   mark(slot, 9);
-  launch_traces(dP, dF, n, grid.wbw.get(), rho.get(), v.get(), grid.N, dE, st);
+  launch_traces(dP, dF, n, grid.wbw.get(), rho.get(), v.get(), grid.N, dE, trace_work.get(), st);
   mark(slot, 10);
+}
+
+void FockPlan::build_orbitals_host(const double *C, int nocc, double eta, double *F_out,
+                                   double *E_out, double *rho_out, double *v_out) {
+  hipStream_t st = grid.stream;
+  DeviceBuffer<double> dC;
+  dC.upload(C, (size_t)n * nocc, st);
+  build_orbitals_device(dC.get(), nocc, eta, F.get(), E.get());
+  if (F_out) F.download(F_out, (size_t)n * n, st);
+  if (E_out) E.download(E_out, 2, st);
+  if (rho_out) rho.download(rho_out, grid.N, st);
+  if (v_out) v.download(v_out, grid.N, st);
+  NAPMO_HIP(hipStreamSynchronize(st));
 }
 
 void FockPlan::overlap(double *S_out) {

@@ -26,8 +26,10 @@ struct FockPlan {
   long Npad;
   DeviceBasis basis;
   DeviceBuffer<double> phi64;          // [Npad x ld]
-  DeviceBuffer<float> phi32;           // [Npad x ld] (FP32 / BF16x3 modes)
-  DeviceBuffer<double> P, rho_part, rho, v, s, F_part, F, E;
+  DeviceBuffer<unsigned short> phiT_hi, phiT_lo;  // bf16 bits, function-major [ld x Npad]
+  DeviceBuffer<float> phiT32;          // function-major [ld x Npad]
+  DeviceBuffer<double> Cpad, Pfull;    // orbital-form inputs
+  DeviceBuffer<double> P, rho_part, rho, v, s, F_part, F, E, trace_work;
   FockShape fsh;
 
   // Per-phase HIP-event timing of builds (on the plan's stream), for roofline reports.
@@ -40,6 +42,12 @@ struct FockPlan {
   FockPlan(DeviceGrid &g, const BasisSet &b, int lmax, int prec_mode);
   ~FockPlan();
   void build_device(const double *dP, double *dF, double *dE);
+  // Orbital form: rho = eta sum_i (Phi C)_i^2 (C [n x nocc] device), P = eta C C^T.
+  void build_orbitals_device(const double *dC, int nocc, double eta, double *dF, double *dE);
+  void build_orbitals_host(const double *C, int nocc, double eta, double *F_out, double *E_out,
+                           double *rho_out, double *v_out);
+  // Poisson, contraction and traces after rho is in place (P: unpadded n x n).
+  void finish(int slot, const double *dP, double *dF, double *dE);
   void build_host(const double *P, double *F_out, double *E_out, double *rho_out, double *v_out);
   // S = Phi^T diag(w bw) Phi [n x n] from the resident Phi (FP64).
   void overlap(double *S_out);

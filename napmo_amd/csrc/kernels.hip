@@ -281,11 +281,106 @@ void launch_synth(const AtomTables &at, const double *x, const double *y, const 
   NAPMO_HIP(hipGetLastError());
 }
 
+// ----------------------------------------------------- bucketed synthesis support
+__device__ inline int pair_bucket(const AtomTables &at, int A, double X, double Y, double Z) {
+  const double dx = X - at.center[3 * A], dy = Y - at.center[3 * A + 1], dz = Z - at.center[3 * A + 2];
+  const double d = sqrt(dx * dx + dy * dy + dz * dz);
+  const int nr = at.nrad[A];
+  int loc;
+  if (d < at.first_x[A]) {
+    loc = 0;
+  } else if (d <= at.last_x[A]) {
+    const int kind = at.rkind[A];
+    const double t = kind == 0 ? -(acos((d - at.rp0[A]) / (d + at.rp0[A])) / at.rp1[A]) + nr
+                               : (kind == 1 ? pow(d / at.rp0[A], 1.0 / at.rp1[A]) - 1 : d);
+    int j = (int)floor(t);
+    if (j >= nr - 1) j = nr - 2;
+    if (j < 0) j = 0;
+    loc = 1 + j;
+  } else {
+    loc = nr;
+  }
+  return A * (at.max_nrad + 1) + loc;
+}
+
+__global__ void k_pair_count(AtomTables at, const double *__restrict__ x,
+                             const double *__restrict__ y, const double *__restrict__ z, long N,
+                             int *__restrict__ count) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  atomicAdd(&count[pair_bucket(at, blockIdx.y, x[p], y[p], z[p])], 1);
+}
+
+__global__ void k_pair_fill(AtomTables at, const double *__restrict__ x,
+                            const double *__restrict__ y, const double *__restrict__ z, long N,
+                            const long *__restrict__ off, int *__restrict__ cursor,
+                            int *__restrict__ list) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int gb = pair_bucket(at, blockIdx.y, x[p], y[p], z[p]);
+  const int slot = atomicAdd(&cursor[gb], 1);
+  list[off[gb] + slot] = (int)p;
+}
+
+void launch_pair_count(const AtomTables &at, const double *x, const double *y, const double *z,
+                       long N, int *count, hipStream_t s) {
+  k_pair_count<<<dim3(div_up(N, 256), at.M), 256, 0, s>>>(at, x, y, z, N, count);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_pair_fill(const AtomTables &at, const double *x, const double *y, const double *z,
+                      long N, const long *off, int *cursor, int *list, hipStream_t s) {
+  k_pair_fill<<<dim3(div_up(N, 256), at.M), 256, 0, s>>>(at, x, y, z, N, off, cursor, list);
+  NAPMO_HIP(hipGetLastError());
+}
+
+template <int LM>
+static void synth2_dispatch(int lmax, dim3 g, dim3 b, hipStream_t s, const AtomTables &at,
+                            const double *x, const double *y, const double *z, long N,
+                            const int *list, const int *wb, long nw, const double *T,
+                            const double *aL, const double *aR, double *contrib) {
+  if constexpr (LM > MAX_LMAX) {
+    throw Error("lmax above the supported maximum " + std::to_string(MAX_LMAX));
+  } else {
+    if (lmax == LM)
+      launch_synth2_t<LM>(g, b, s, at, x, y, z, N, list, wb, nw, T, aL, aR, contrib);
+    else
+      synth2_dispatch<LM + 1>(lmax, g, b, s, at, x, y, z, N, list, wb, nw, T, aL, aR, contrib);
+  }
+}
+
+void launch_synth2(const AtomTables &at, const double *x, const double *y, const double *z, long N,
+                   const int *list, const int *wave_bucket, long nwaves, const double *T,
+                   const double *ampL, const double *ampR, int lmax, double *contrib,
+                   hipStream_t s) {
+  NAPMO_REQUIRE(lmax >= 0 && lmax <= MAX_LMAX, "lmax out of range for synthesis");
+  if (nwaves <= 0) return;
+  synth2_dispatch<0>(lmax, dim3(div_up(nwaves, 4)), dim3(256), s, at, x, y, z, N, list,
+                     wave_bucket, nwaves, T, ampL, ampR, contrib);
+  NAPMO_HIP(hipGetLastError());
+}
+
+__global__ void k_contrib_reduce(const double *__restrict__ contrib, int M, long N, int accumulate,
+                                 double *__restrict__ v) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  double s = accumulate ? v[p] : 0.0;
+  for (int A = 0; A < M; ++A) s += contrib[(long)A * N + p];
+  v[p] = s;
+}
+
+void launch_contrib_reduce(const double *contrib, int M, long N, int accumulate, double *v,
+                           hipStream_t s) {
+  if (N <= 0) return;
+  k_contrib_reduce<<<div_up(N, 256), 256, 0, s>>>(contrib, M, N, accumulate, v);
+  NAPMO_HIP(hipGetLastError());
+}
+
 // ================================================= contractions: FP64 MFMA density
-// v_mfma_f64_16x16x4f64 operand layout (lane i of 64):
+// v_mfma_f64_16x16x4f64 operand layout (lane i of 64), as measured on gfx950:
 //   A[16 x 4]:  lane i holds A[i % 16][i / 16]
 //   B[4 x 16]:  lane i holds B[i / 16][i % 16]
-//   D[16 x 16]: lane i holds D[4 (i / 16) + r][i % 16], r = 0..3
+//   D[16 x 16]: lane i holds D[(i / 16) + 4 r][i % 16] in accumulator element r = 0..3
 // Tile: 64 points x 64 columns per workgroup (4 waves of 32 x 32 = 2 x 2 MFMA blocks).
 // X = Phi P over the tile's columns; epilogue rho_partial[ct][p] = sum_cols X o Phi.
 constexpr int KS = 16;  // K elements staged in LDS per step
@@ -328,7 +423,7 @@ __global__ void __launch_bounds__(256) k_density_f64(const double *__restrict__ 
   const int col16 = lane & 15, rg = lane >> 4;
   for (int a = 0; a < 2; ++a) {
     for (int r = 0; r < 4; ++r) {
-      const int m = wr + a * 16 + rg * 4 + r;
+      const int m = wr + a * 16 + rg + 4 * r;
       const double *prow = phi + (p0 + m) * ld + c0 + wc;
       double sum = acc[a][0][r] * prow[col16] + acc[a][1][r] * prow[16 + col16];
       sum += __shfl_xor(sum, 8);
@@ -351,18 +446,18 @@ void launch_density_f64(const double *phi, int ld, long Npad, const double *P, d
 }
 
 __global__ void k_rho_reduce(const double *__restrict__ partial, int nt, long Npad, long N,
-                             double *__restrict__ rho) {
+                             double scale, double *__restrict__ rho) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
   double s = 0.0;
   for (int t = 0; t < nt; ++t) s += partial[(long)t * Npad + p];
-  rho[p] = s;
+  rho[p] = scale * s;
 }
 
 void launch_rho_reduce(const double *partial, int ntile, long Npad, long N, double *rho,
-                       hipStream_t s) {
+                       hipStream_t s, double scale) {
   if (N <= 0) return;
-  k_rho_reduce<<<div_up(N, 256), 256, 0, s>>>(partial, ntile, Npad, N, rho);
+  k_rho_reduce<<<div_up(N, 256), 256, 0, s>>>(partial, ntile, Npad, N, scale, rho);
   NAPMO_HIP(hipGetLastError());
 }
 
@@ -389,10 +484,11 @@ FockShape fock_shape(int ld, long Npad, bool symmetric) {
   sh.ntiles = symmetric ? sh.nt * (sh.nt + 1) / 2 : sh.nt * sh.nt;
   sh.symmetric = symmetric;
   sh.Npad = Npad;
-  // aim at ~2048 workgroups, chunks a multiple of KS points
+  // aim at ~2048 workgroups; chunks a multiple of 256 points (the mixed kernel's FP32
+  // accumulation block)
   long want = std::max(1L, 2048L / sh.ntiles);
-  long chunk = round_up(div_up(Npad, want), KS);
-  sh.chunk = std::max<long>(chunk, KS);
+  long chunk = round_up(div_up(Npad, want), 256);
+  sh.chunk = std::max<long>(chunk, 256);
   sh.nchunk = (int)div_up(Npad, sh.chunk);
   return sh;
 }
@@ -449,7 +545,7 @@ __global__ void __launch_bounds__(256) k_fock_f64(const double *__restrict__ X,
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int r = 0; r < 4; ++r)
-        out[(wr + a * 16 + rg * 4 + r) * TILE + wc + b * 16 + col16] = acc[a][b][r];
+        out[(wr + a * 16 + rg + 4 * r) * TILE + wc + b * 16 + col16] = acc[a][b][r];
 }
 
 void launch_fock_f64(const FockShape &sh, const double *X, const double *Y, const double *s,
@@ -497,21 +593,42 @@ void launch_fock_reduce(const FockShape &sh, const double *partial, int n, int u
 }
 
 // ============================================================ FP64 energy traces
-// Single workgroup, fixed reduction order (deterministic).
-__global__ void k_traces(const double *__restrict__ P, const double *__restrict__ F, int n,
-                         const double *__restrict__ wbw, const double *__restrict__ rho,
-                         const double *__restrict__ v, long N, double *__restrict__ E) {
-  __shared__ double sa[1024], sb[1024];
+// Two passes with a fixed reduction order (deterministic): per-block partial sums of
+// P o F and w bw rho v, then one block adds the partials.
+constexpr int TRACE_BLOCKS = 512;
+
+__global__ void __launch_bounds__(256) k_traces_partial(const double *__restrict__ P,
+                                                        const double *__restrict__ F, long nn,
+                                                        const double *__restrict__ wbw,
+                                                        const double *__restrict__ rho,
+                                                        const double *__restrict__ v, long N,
+                                                        double *__restrict__ part) {
+  __shared__ double sa[256], sb[256];
   const int tid = threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
   double a = 0.0, b = 0.0;
   if (P && F)
-    for (long e = tid; e < (long)n * n; e += blockDim.x) a += P[e] * F[e];
+    for (long e = (long)blockIdx.x * blockDim.x + tid; e < nn; e += stride) a += P[e] * F[e];
   if (wbw && rho && v)
-    for (long p = tid; p < N; p += blockDim.x) b += wbw[p] * rho[p] * v[p];
+    for (long p = (long)blockIdx.x * blockDim.x + tid; p < N; p += stride) b += wbw[p] * rho[p] * v[p];
   sa[tid] = a;
   sb[tid] = b;
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) sa[tid] += sa[tid + w], sb[tid] += sb[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) part[blockIdx.x] = sa[0], part[gridDim.x + blockIdx.x] = sb[0];
+}
+
+__global__ void __launch_bounds__(512) k_traces_final(const double *__restrict__ part, int nb,
+                                                      double *__restrict__ E) {
+  __shared__ double sa[512], sb[512];
+  const int tid = threadIdx.x;
+  sa[tid] = tid < nb ? part[tid] : 0.0;
+  sb[tid] = tid < nb ? part[nb + tid] : 0.0;
+  __syncthreads();
+  for (int w = 256; w > 0; w >>= 1) {
     if (tid < w) sa[tid] += sa[tid + w], sb[tid] += sb[tid + w];
     __syncthreads();
   }
@@ -519,8 +636,9 @@ __global__ void k_traces(const double *__restrict__ P, const double *__restrict_
 }
 
 void launch_traces(const double *P, const double *F, int n, const double *wbw, const double *rho,
-                   const double *v, long N, double *E, hipStream_t st) {
-  k_traces<<<1, 1024, 0, st>>>(P, F, n, wbw, rho, v, N, E);
+                   const double *v, long N, double *E, double *work, hipStream_t st) {
+  k_traces_partial<<<TRACE_BLOCKS, 256, 0, st>>>(P, F, (long)n * n, wbw, rho, v, N, work);
+  k_traces_final<<<1, 512, 0, st>>>(work, TRACE_BLOCKS, E);
   NAPMO_HIP(hipGetLastError());
 }
 

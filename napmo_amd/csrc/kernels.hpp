@@ -68,12 +68,32 @@ void launch_synth_t(dim3 g, dim3 b, hipStream_t s, const AtomTables &at, const d
                     const double *y, const double *z, long N, const double *T, const double *aL,
                     const double *aR, int acc, int ext_mode, double *v);
 
+template <int LMAX>
+void launch_synth2_t(dim3 g, dim3 b, hipStream_t s, const AtomTables &at, const double *x,
+                     const double *y, const double *z, long N, const int *list,
+                     const int *wave_bucket, long nwaves, const double *T, const double *aL,
+                     const double *aR, double *contrib);
+
+// Pair buckets of the bucketed synthesis (per geometry): count[gb] += 1 for every
+// (point, centre); then list[off[gb] + k] = p (slots pre-filled with -1).
+void launch_pair_count(const AtomTables &at, const double *x, const double *y, const double *z,
+                       long N, int *count, hipStream_t s);
+void launch_pair_fill(const AtomTables &at, const double *x, const double *y, const double *z,
+                      long N, const long *off, int *cursor, int *list, hipStream_t s);
+// Bucketed synthesis into contrib [M x N], then v[p] (+)= sum_A contrib[A][p].
+void launch_synth2(const AtomTables &at, const double *x, const double *y, const double *z, long N,
+                   const int *list, const int *wave_bucket, long nwaves, const double *T,
+                   const double *ampL, const double *ampR, int lmax, double *contrib,
+                   hipStream_t s);
+void launch_contrib_reduce(const double *contrib, int M, long N, int accumulate, double *v,
+                           hipStream_t s);
+
 // rho[p] = Phi_p^T P Phi_p with FP64 MFMA (P [ld x ld] zero padded). partial [ntile x Npad].
 void launch_density_f64(const double *phi, int ld, long Npad, const double *P, double *partial,
                         hipStream_t s);
-// rho[p] = sum_t partial[t][p] for p < N.
+// rho[p] = scale * sum_t partial[t][p] for p < N.
 void launch_rho_reduce(const double *partial, int ntile, long Npad, long N, double *rho,
-                       hipStream_t s);
+                       hipStream_t s, double scale = 1.0);
 
 // Scale vector s[p] = a[p] * b[p] * c[p] (c may be null), zero for p >= N.
 void launch_scale3(const double *a, const double *b, const double *c, long N, long Npad, double *s,
@@ -96,7 +116,25 @@ void launch_fock_reduce(const FockShape &sh, const double *partial, int n, int u
                         double scale, int accumulate, double *F, hipStream_t st);
 
 // E[0] = 0.5 * sum_ij P_ij F_ij (FP64), E[1] = 0.5 * sum_p w_p rho_p v_p.
+// work: device scratch of 2 * TRACE_WORK doubles.
+constexpr int TRACE_WORK = 512;
 void launch_traces(const double *P, const double *F, int n, const double *wbw, const double *rho,
-                   const double *v, long N, double *E, hipStream_t st);
+                   const double *v, long N, double *E, double *work, hipStream_t st);
+
+// ---- kernels_mixed.hip
+// phi64 [Npad x ld] -> function-major bf16 hi / lo bits and float32 copies [ld x Npad].
+void launch_split_phi(const double *phi64, int ld, long Npad, unsigned short *hi, unsigned short *lo,
+                      float *f32, hipStream_t s);
+// F partials with the 3-term bf16 split (see kernels_mixed.hip); same shape as the FP64 one.
+void launch_fock_bf16x3(const FockShape &sh, const unsigned short *Ahi, const unsigned short *Alo,
+                        const float *B32, const double *s, double *partial, hipStream_t st);
+// partial[ct][p] = sum over orbital tile ct of (Phi C)_pi^2, C zero-padded [ld x ldc].
+void launch_density_orb_f64(const double *phi, int ld, long Npad, const double *C, int ldc,
+                            double *partial, hipStream_t s);
+void launch_gram(const double *C, int n, int nocc, double eta, double *P, hipStream_t s);
+void launch_pad2d(const double *src, int rows, int cols, double *dst, int drows, int dcols,
+                  hipStream_t s);
+void launch_selftest_mfma(int which, double *out, hipStream_t s);
+void launch_copy(const double *src, double *dst, long n, hipStream_t s);
 
 }  // namespace napmo

@@ -1,0 +1,295 @@
+// Mixed-precision and orbital-form contractions for gfx950:
+//   * Phi split once per geometry into function-major bf16 hi / lo and float32 copies;
+//   * F = Phi^T diag(s) Phi with the 3-term bf16 split on v_mfma_f32_16x16x32_bf16,
+//     FP32 accumulation over blocks of 256 points flushed into FP64 registers;
+//   * rho_p = eta sum_i (Phi C)_pi^2 with FP64 MFMA (orbital form of the density);
+//   * P = eta C C^T; an MFMA lane-layout self test.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace napmo {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+// float -> bf16 bits, round to nearest even (finite inputs)
+__device__ inline unsigned short bf16_bits(float x) {
+  unsigned int u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float bf16_float(unsigned short h) { return __uint_as_float((unsigned int)h << 16); }
+
+// ---------------------------------------------------------- per-geometry splits
+// phi64 [Npad x ld] point-major -> hi, lo (bf16 bits) and f32, function-major [ld x Npad].
+__global__ void k_split_phi(const double *__restrict__ phi64, int ld, long Npad,
+                            unsigned short *__restrict__ hi, unsigned short *__restrict__ lo,
+                            float *__restrict__ f32) {
+  __shared__ double tile[32][33];
+  const long p0 = (long)blockIdx.x * 32;
+  const int m0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) tile[r][tx] = phi64[(p0 + r) * ld + m0 + tx];
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const double v = tile[tx][r];  // point p0 + tx, function m0 + r
+    const float x = (float)v;
+    const unsigned short h = bf16_bits(x);
+    const unsigned short l = bf16_bits(x - bf16_float(h));
+    const size_t o = (size_t)(m0 + r) * Npad + p0 + tx;
+    hi[o] = h;
+    lo[o] = l;
+    f32[o] = x;
+  }
+}
+
+void launch_split_phi(const double *phi64, int ld, long Npad, unsigned short *hi, unsigned short *lo,
+                      float *f32, hipStream_t s) {
+  NAPMO_REQUIRE(ld % 32 == 0 && Npad % 32 == 0, "split_phi: padding mismatch");
+  k_split_phi<<<dim3(Npad / 32, ld / 32), 256, 0, s>>>(phi64, ld, Npad, hi, lo, f32);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------- BF16x3 Fock build
+// v_mfma_f32_16x16x32_bf16 operand layout (lane i of 64):
+//   A[16 x 32]: lane i holds A[i % 16][8 (i / 16) .. 8 (i / 16) + 7]
+//   B[32 x 16]: lane i holds B[8 (i / 16) .. + 7][i % 16]
+//   D[16 x 16]: lane i holds D[4 (i / 16) + r][i % 16], r = 0..3
+// A[m][k] = Phi[p0+k][mu0+m], B[k][n] = s[p0+k] Phi[p0+k][nu0+n]; both staged in LDS
+// k-contiguous ([row][k]) from the function-major copies. Products hi*hi + hi*lo + lo*hi.
+constexpr int KB = 32;         // points per LDS stage (MFMA k-depth)
+constexpr int KPAD = KB + 8;   // LDS row stride in bf16
+constexpr int FLUSH = 256;     // points accumulated in FP32 before the FP64 flush
+
+__global__ void __launch_bounds__(256) k_fock_bf16x3(const unsigned short *__restrict__ Ahi,
+                                                     const unsigned short *__restrict__ Alo,
+                                                     const float *__restrict__ B32,
+                                                     const double *__restrict__ s, int nt,
+                                                     int symmetric, long Npad, long chunk,
+                                                     double *__restrict__ partial) {
+  __shared__ unsigned short sAh[TILE][KPAD], sAl[TILE][KPAD], sBh[TILE][KPAD], sBl[TILE][KPAD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int ti, tj;
+  if (!symmetric) {
+    ti = blockIdx.x / nt, tj = blockIdx.x % nt;
+  } else {
+    int i = 0, rem = blockIdx.x;
+    while (rem >= nt - i) rem -= nt - i, ++i;
+    ti = i, tj = i + rem;
+  }
+  const int mu0 = ti * TILE, nu0 = tj * TILE;
+  const long pbeg = (long)blockIdx.y * chunk;
+  const long pend = min(Npad, pbeg + chunk);
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  f4 acc[2][2];
+  double acc64[2][2][4];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      acc[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+      for (int r = 0; r < 4; ++r) acc64[a][b][r] = 0.0;
+    }
+  // staging: thread t copies 8 consecutive k of row t / 4 (k segment (t % 4) * 8)
+  const int srow = tid >> 2, sk = (tid & 3) * 8;
+  int since_flush = 0;
+  for (long k0 = pbeg; k0 < pend; k0 += KB) {
+    {
+      const size_t ga = (size_t)(mu0 + srow) * Npad + k0 + sk;
+      *(u16x8 *)&sAh[srow][sk] = *(const u16x8 *)&Ahi[ga];
+      *(u16x8 *)&sAl[srow][sk] = *(const u16x8 *)&Alo[ga];
+      const size_t gb = (size_t)(nu0 + srow) * Npad + k0 + sk;
+      u16x8 bh, bl;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const float x = B32[gb + e] * (float)s[k0 + sk + e];
+        const unsigned short h = bf16_bits(x);
+        bh[e] = h;
+        bl[e] = bf16_bits(x - bf16_float(h));
+      }
+      *(u16x8 *)&sBh[srow][sk] = bh;
+      *(u16x8 *)&sBl[srow][sk] = bl;
+    }
+    __syncthreads();
+    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+    bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      ah[a] = __builtin_bit_cast(bf16x8, *(const u16x8 *)&sAh[wr + a * 16 + r16][kq]);
+      al[a] = __builtin_bit_cast(bf16x8, *(const u16x8 *)&sAl[wr + a * 16 + r16][kq]);
+      bh[a] = __builtin_bit_cast(bf16x8, *(const u16x8 *)&sBh[wc + a * 16 + r16][kq]);
+      bl[a] = __builtin_bit_cast(bf16x8, *(const u16x8 *)&sBl[wc + a * 16 + r16][kq]);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+      }
+    __syncthreads();
+    since_flush += KB;
+    if (since_flush >= FLUSH) {
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+          for (int r = 0; r < 4; ++r) acc64[a][b][r] += (double)acc[a][b][r];
+          acc[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+        }
+      since_flush = 0;
+    }
+  }
+  double *out = partial + ((long)blockIdx.y * gridDim.x + blockIdx.x) * TILE * TILE;
+  const int col16 = lane & 15, rg = lane >> 4;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int r = 0; r < 4; ++r)
+        out[(wr + a * 16 + rg * 4 + r) * TILE + wc + b * 16 + col16] =
+            acc64[a][b][r] + (double)acc[a][b][r];
+}
+
+void launch_fock_bf16x3(const FockShape &sh, const unsigned short *Ahi, const unsigned short *Alo,
+                        const float *B32, const double *s, double *partial, hipStream_t st) {
+  NAPMO_REQUIRE(sh.chunk % KB == 0 && sh.Npad % KB == 0, "fock_bf16x3: chunking mismatch");
+  dim3 grid(sh.ntiles, sh.nchunk);
+  k_fock_bf16x3<<<grid, 256, 0, st>>>(Ahi, Alo, B32, s, sh.nt, sh.symmetric ? 1 : 0, sh.Npad,
+                                      sh.chunk, partial);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// -------------------------------------------------------- orbital-form density
+// X = Phi C over a 64 x 64 (points x orbitals) tile with FP64 MFMA; partial[ct][p] =
+// sum over the tile's orbitals of X^2 (C zero-padded to [ld x ldc]).
+__global__ void __launch_bounds__(256) k_density_orb_f64(const double *__restrict__ phi, int ld,
+                                                         long Npad, const double *__restrict__ C,
+                                                         int ldc, double *__restrict__ partial) {
+  __shared__ double As[16][TILE + 2];
+  __shared__ double Bs[16][TILE];
+  __shared__ double red[TILE][2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long p0 = (long)blockIdx.x * TILE;
+  const int c0 = blockIdx.y * TILE;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  d4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = (d4){0, 0, 0, 0};
+  for (int k0 = 0; k0 < ld; k0 += 16) {
+    for (int e = tid; e < TILE * 16; e += 256) {
+      const int m = e / 16, k = e % 16;
+      As[k][m] = phi[(p0 + m) * ld + k0 + k];
+      const int kk = e / TILE, n = e % TILE;
+      Bs[kk][n] = C[(long)(k0 + kk) * ldc + c0 + n];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 16; ks += 4) {
+      const int kr = ks + (lane >> 4), mi = lane & 15;
+      double a0 = As[kr][wr + mi], a1 = As[kr][wr + 16 + mi];
+      double b0 = Bs[kr][wc + mi], b1 = Bs[kr][wc + 16 + mi];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const int col16 = lane & 15, rg = lane >> 4;
+  for (int a = 0; a < 2; ++a) {
+    for (int r = 0; r < 4; ++r) {
+      const int m = wr + a * 16 + rg + 4 * r;  // f64 16x16x4 output row (see kernels.hip)
+      double sum = acc[a][0][r] * acc[a][0][r] + acc[a][1][r] * acc[a][1][r];
+      sum += __shfl_xor(sum, 8);
+      sum += __shfl_xor(sum, 4);
+      sum += __shfl_xor(sum, 2);
+      sum += __shfl_xor(sum, 1);
+      if (col16 == 0) red[m][wave & 1] = sum;
+    }
+  }
+  __syncthreads();
+  if (tid < TILE) partial[(long)blockIdx.y * Npad + p0 + tid] = red[tid][0] + red[tid][1];
+}
+
+void launch_density_orb_f64(const double *phi, int ld, long Npad, const double *C, int ldc,
+                            double *partial, hipStream_t s) {
+  NAPMO_REQUIRE(ld % TILE == 0 && ldc % TILE == 0 && Npad % TILE == 0, "density_orb: padding");
+  k_density_orb_f64<<<dim3(Npad / TILE, ldc / TILE), 256, 0, s>>>(phi, ld, Npad, C, ldc, partial);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// P[i][j] = eta sum_k C[i][k] C[j][k] (C [n x nocc] row-major), P [n x n] row-major.
+__global__ void k_gram(const double *__restrict__ C, int n, int nocc, double eta,
+                       double *__restrict__ P) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  double s = 0.0;
+  for (int k = 0; k < nocc; ++k) s += C[(long)i * nocc + k] * C[(long)j * nocc + k];
+  P[e] = eta * s;
+}
+
+void launch_gram(const double *C, int n, int nocc, double eta, double *P, hipStream_t s) {
+  k_gram<<<div_up((long)n * n, 256), 256, 0, s>>>(C, n, nocc, eta, P);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// C [n x nocc] row-major -> zero-padded [ld x ldc]
+__global__ void k_pad2d(const double *__restrict__ src, int rows, int cols, int ldd,
+                        double *__restrict__ dst, int drows) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)drows * ldd) return;
+  const int i = (int)(e / ldd), j = (int)(e % ldd);
+  dst[e] = (i < rows && j < cols) ? src[(long)i * cols + j] : 0.0;
+}
+
+void launch_pad2d(const double *src, int rows, int cols, double *dst, int drows, int dcols,
+                  hipStream_t s) {
+  k_pad2d<<<div_up((long)drows * dcols, 256), 256, 0, s>>>(src, rows, cols, dcols, dst, drows);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------ calibration copy
+// Plain streaming copy of n doubles (16 B per lane per iteration): the reference kernel
+// whose FETCH_SIZE / WRITE_SIZE readings scale the HBM-traffic counters.
+__global__ void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, long n2) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+void launch_copy(const double *src, double *dst, long n, hipStream_t s) {
+  NAPMO_REQUIRE(n % 2 == 0, "copy: even length required");
+  k_copy<<<4096, 256, 0, s>>>((const double2 *)src, (double2 *)dst, n / 2);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// -------------------------------------------------------------- layout self test
+// One wave computes D = A B for A[m][k] = m + k / 64, B[k][n] = 1 + (k - n) / 128
+// through the MFMA builtin and writes D with the output layout the kernels assume.
+__global__ void k_selftest_mfma(int which, double *out) {
+  const int lane = threadIdx.x;
+  if (which == 0) {  // f64 16x16x4
+    const double a = (lane % 16) + (lane / 16) / 64.0;
+    const double b = 1.0 + ((lane / 16) - (lane % 16)) / 128.0;
+    d4 acc = {0, 0, 0, 0};
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    for (int r = 0; r < 4; ++r) out[((lane / 16) + 4 * r) * 16 + lane % 16] = acc[r];
+  } else {  // bf16 16x16x32 (values exact in bf16)
+    bf16x8 a, b;
+    for (int e = 0; e < 8; ++e) {
+      const int k = 8 * (lane / 16) + e;
+      a[e] = (__bf16)(float)((lane % 16) + k / 64.0);
+      b[e] = (__bf16)(float)(1.0 + (k - (lane % 16)) / 128.0);
+    }
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    for (int r = 0; r < 4; ++r) out[(4 * (lane / 16) + r) * 16 + lane % 16] = acc[r];
+  }
+}
+
+void launch_selftest_mfma(int which, double *out, hipStream_t s) {
+  k_selftest_mfma<<<1, 64, 0, s>>>(which, out);
+  NAPMO_HIP(hipGetLastError());
+}
+
+}  // namespace napmo

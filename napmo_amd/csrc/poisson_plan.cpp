@@ -415,10 +415,63 @@ void DeviceGrid::poisson_expand(const double *d_dens, int lmax) {
 void DeviceGrid::poisson_synthesize(int lmax, double *d_v) {
   PoissonPlan &pl = plan(lmax);
   AtomTables at = tables(pl);
+  SynthPairs &sp = synth_pairs();
   launch_synth_tables(at, pl.Uy.get(), pl.Udt.get(), lmax, pl.T.get(), pl.ampL.get(),
                       pl.ampR.get(), stream);
-  launch_synth(at, x.get(), y.get(), z.get(), N, pl.T.get(), pl.ampL.get(), pl.ampR.get(), lmax, 0,
-               d_v, stream);
+  launch_synth2(at, x.get(), y.get(), z.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves,
+                pl.T.get(), pl.ampL.get(), pl.ampR.get(), lmax, sp.contrib.get(), stream);
+  launch_contrib_reduce(sp.contrib.get(), M, N, 0, d_v, stream);
+}
+
+SynthPairs &DeviceGrid::synth_pairs() {
+  if (pairs) return *pairs;
+  // bucket geometry does not depend on lmax: use any plan's tables for the atom arrays
+  AtomTables at;
+  at.M = M;
+  at.point_off = d_point_off.get();
+  at.nrad = d_nrad.get();
+  at.nang = d_nang.get();
+  at.rad_off = d_rad_off.get();
+  at.yhat_off = nullptr;
+  at.green_off = nullptr;
+  at.rkind = d_rkind.get();
+  at.rp0 = d_rp0.get();
+  at.rp1 = d_rp1.get();
+  at.first_x = d_first.get();
+  at.last_x = d_last.get();
+  at.center = d_center.get();
+  at.r = d_r.get();
+  at.max_nrad = max_nrad;
+  const int nb = M * (max_nrad + 1);
+  DeviceBuffer<int> count(nb), cursor(nb);
+  NAPMO_HIP(hipMemsetAsync(count.get(), 0, sizeof(int) * nb, stream));
+  launch_pair_count(at, x.get(), y.get(), z.get(), N, count.get(), stream);
+  std::vector<int> hc(nb);
+  count.download(hc.data(), nb, stream);
+  NAPMO_HIP(hipStreamSynchronize(stream));
+  std::vector<long> off(nb);
+  std::vector<int> wb;
+  long total = 0;
+  for (int b = 0; b < nb; ++b) {
+    off[b] = total;
+    const long waves = div_up(hc[b], 64);
+    for (long w = 0; w < waves; ++w) wb.push_back(b);
+    total += waves * 64;
+  }
+  auto p = std::make_unique<SynthPairs>();
+  p->nwaves = (long)wb.size();
+  NAPMO_REQUIRE(total < (1L << 31), "too many (point, centre) pairs for the synthesis lists");
+  p->list.resize(std::max<long>(total, 1));
+  NAPMO_HIP(hipMemsetAsync(p->list.get(), 0xFF, sizeof(int) * std::max<long>(total, 1), stream));
+  DeviceBuffer<long> doff;
+  doff.upload(off, stream);
+  NAPMO_HIP(hipMemsetAsync(cursor.get(), 0, sizeof(int) * nb, stream));
+  launch_pair_fill(at, x.get(), y.get(), z.get(), N, doff.get(), cursor.get(), p->list.get(), stream);
+  p->wave_bucket.upload(wb, stream);
+  p->contrib.resize((size_t)M * N);
+  NAPMO_HIP(hipStreamSynchronize(stream));
+  pairs = std::move(p);
+  return *pairs;
 }
 
 }  // namespace napmo

@@ -48,6 +48,13 @@ struct PoissonPlan {
   DeviceBuffer<double> rho_lm, Uy, Udt, T, ampL, ampR;
 };
 
+// Per-geometry pair buckets of the bucketed synthesis (see launch_synth2).
+struct SynthPairs {
+  long nwaves = 0;
+  DeviceBuffer<int> list, wave_bucket;
+  DeviceBuffer<double> contrib;  // [M x N]
+};
+
 struct DeviceGrid {
   hipStream_t stream = nullptr;
   long N = 0;
@@ -64,6 +71,7 @@ struct DeviceGrid {
   DeviceBuffer<double> d_rp0, d_rp1, d_first, d_last, d_center, d_r;
   int max_nrad = 0;
   std::map<int, std::unique_ptr<PoissonPlan>> plans;
+  std::unique_ptr<SynthPairs> pairs;
 
   ~DeviceGrid();
   void build(const std::vector<AtomInfo> &atoms);
@@ -72,6 +80,7 @@ struct DeviceGrid {
   // Poisson of dens (device [N]) into the plan's U / tables; then synthesis to v (device).
   void poisson_expand(const double *d_dens, int lmax);
   void poisson_synthesize(int lmax, double *d_v);
+  SynthPairs &synth_pairs();
 };
 
 }  // namespace napmo

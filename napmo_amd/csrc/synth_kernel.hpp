@@ -82,7 +82,11 @@ __global__ void __launch_bounds__(256) k_synth(AtomTables at, const double *__re
           pi2 = pi1;
           pi1 = pil;
         } else {
-          pil = (uz * (2 * l - 1) * pi1 - (l + m - 1) * pi2) / (l - m);
+          // PI_l^m = ((2l-1) z PI_{l-1}^m - (l+m-1) PI_{l-2}^m) / (l-m) with r^2 = 1; the
+        // coefficients fold to constants (no FP64 division in the loop)
+        const double c1 = (double)(2 * l - 1) / (double)(l - m);
+        const double c2 = (double)(l + m - 1) / (double)(l - m);
+        pil = uz * c1 * pi1 - c2 * pi2;
           pi2 = pi1;
           pi1 = pil;
         }
@@ -109,6 +113,131 @@ __global__ void __launch_bounds__(256) k_synth(AtomTables at, const double *__re
   v[p] = total;
 }
 
+
+// Harmonic sum of one (point, centre) pair: sum_q s_q(u or d) * Re/Im[(x+iy)^m] PI_l^m(z)
+// over the synthesis order q. REG 1: Horner spline rows c[q][0..3] at u; REG 0 / 2:
+// PotentialExtrapolation amplitudes amp[q] * d^l (left) or amp[q] / d^(l+1) (right).
+template <int LMAX, int REG>
+__device__ __forceinline__ double synth_harmonics(const double *c, double u, double ux, double uy,
+                                                  double uz, double d, double inv_d) {
+  const double base = (REG == 0) ? d : inv_d;
+  double acc = 0.0;
+  int q = 0;
+  double Am = 1.0, Bm = 0.0, pimm = 1.0;
+  double pwm = (REG == 2) ? inv_d : 1.0;
+#pragma unroll
+  for (int m = 0; m <= LMAX; ++m) {
+    if (m > 0) {
+      const double a2 = ux * Am - uy * Bm;
+      Bm = ux * Bm + uy * Am;
+      Am = a2;
+      pimm *= (2 * m - 1);
+      if (REG != 1) pwm *= base;
+    }
+    double pi2 = 0.0, pi1 = pimm, pw = pwm;
+#pragma unroll
+    for (int l = m; l <= LMAX; ++l) {
+      double pil;
+      if (l == m) {
+        pil = pimm;
+      } else if (l == m + 1) {
+        pil = uz * (2 * m + 1) * pimm;
+        pi2 = pi1;
+        pi1 = pil;
+      } else {
+        const double c1 = (double)(2 * l - 1) / (double)(l - m);
+        const double c2 = (double)(l + m - 1) / (double)(l - m);
+        pil = uz * c1 * pi1 - c2 * pi2;
+        pi2 = pi1;
+        pi1 = pil;
+      }
+      if (REG != 1 && l > m) pw *= base;
+#pragma unroll
+      for (int cs = 0; cs < (m ? 2 : 1); ++cs) {
+        const double harm = (cs ? Bm : Am) * pil;
+        double sv;
+        if (REG == 1) {
+          const double *cq = c + q * 4;
+          sv = cq[0] + u * (cq[1] + u * (cq[2] + u * cq[3]));
+        } else {
+          sv = c[q] * pw;
+        }
+        acc += sv * harm;
+        ++q;
+      }
+    }
+  }
+  return acc;
+}
+
+// Bucketed variant: the (point, centre) pairs are sorted by (centre, spline interval)
+// once per geometry; each wavefront handles 64 pairs of one bucket, copies that bucket's
+// spline row into its own LDS slice once, and every lane then reads the coefficients as
+// LDS broadcasts; only the Horner variable u differs per lane.
+// bucket = A * (max_nrad + 1) + loc, loc 0 = left extrapolation, 1 + j = interval j,
+// nrad_A = right extrapolation. Writes contrib[A * N + p].
+template <int LMAX> struct Synth2Cfg {
+  static constexpr int L = (LMAX + 1) * (LMAX + 1);
+  static constexpr int WPB = (L * 4 * 8 * 4 <= 65536) ? 4 : 2;  // waves per block (LDS <= 64 KB)
+};
+
+template <int LMAX>
+__global__ void __launch_bounds__(256) k_synth2(AtomTables at, const double *__restrict__ px,
+                                                const double *__restrict__ py,
+                                                const double *__restrict__ pz, long N,
+                                                const int *__restrict__ list,
+                                                const int *__restrict__ wave_bucket, long nwaves,
+                                                const double *__restrict__ T,
+                                                const double *__restrict__ ampL,
+                                                const double *__restrict__ ampR,
+                                                double *__restrict__ contrib) {
+  constexpr int L = Synth2Cfg<LMAX>::L;
+  constexpr int WPB = Synth2Cfg<LMAX>::WPB;
+  __shared__ double srow[WPB][L * 4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long wid = (long)blockIdx.x * WPB + wave;
+  const bool active = wid < nwaves;
+  const int gb = active ? __builtin_amdgcn_readfirstlane(wave_bucket[wid]) : 0;
+  const int A = gb / (at.max_nrad + 1), loc = gb - A * (at.max_nrad + 1);
+  const int nr = at.nrad[A];
+  const int reg = (loc == 0) ? 0 : (loc == nr ? 2 : 1);
+  const int j = (reg == 1) ? loc - 1 : 0;
+  if (active && reg == 1) {
+    const double *row = T + ((long)A * (at.max_nrad - 1) + j) * (long)L * 4;
+    for (int e = lane; e < L * 4; e += 64) srow[wave][e] = row[e];
+  }
+  __syncthreads();
+  if (!active) return;
+  const int p = list[wid * 64 + lane];
+  const long pp = p >= 0 ? p : 0;
+  const double dx = px[pp] - at.center[3 * A], dy = py[pp] - at.center[3 * A + 1],
+               dz = pz[pp] - at.center[3 * A + 2];
+  const double d = sqrt(dx * dx + dy * dy + dz * dz);
+  const double inv_d = 1.0 / d;
+  const double ux = dx * inv_d, uy = dy * inv_d, uz = dz * inv_d;
+  double acc;
+  if (reg == 1) {
+    const double u = rtf_inv(at.rkind[A], at.rp0[A], at.rp1[A], nr, d) - j;
+    acc = synth_harmonics<LMAX, 1>(srow[wave], u, ux, uy, uz, d, inv_d);
+  } else if (reg == 0) {
+    acc = synth_harmonics<LMAX, 0>(ampL + (long)A * L, 0.0, ux, uy, uz, d, inv_d);
+  } else {
+    acc = synth_harmonics<LMAX, 2>(ampR + (long)A * L, 0.0, ux, uy, uz, d, inv_d);
+  }
+  if (p >= 0) contrib[(long)A * N + p] = acc;
+}
+
+template <int LMAX>
+void launch_synth2_t(dim3 g, dim3 b, hipStream_t s, const AtomTables &at, const double *x,
+                     const double *y, const double *z, long N, const int *list,
+                     const int *wave_bucket, long nwaves, const double *T, const double *aL,
+                     const double *aR, double *contrib) {
+  (void)g, (void)b;
+  constexpr int WPB = Synth2Cfg<LMAX>::WPB;
+  k_synth2<LMAX><<<dim3(div_up(nwaves, WPB)), dim3(64 * WPB), 0, s>>>(at, x, y, z, N, list,
+                                                                      wave_bucket, nwaves, T, aL,
+                                                                      aR, contrib);
+}
 
 template <int LMAX>
 void launch_synth_t(dim3 g, dim3 b, hipStream_t s, const AtomTables &at, const double *x,

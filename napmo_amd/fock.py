@@ -54,6 +54,25 @@ class GridFock(object):
             return F, E[0], E[1], rho, v
         return F, E[0], E[1]
 
+    def build_orbitals(self, C, eta=2.0, want_grid=False):
+        """Orbital form (rho = eta sum_i (Phi C)_i^2, P = eta C C^T); C [n x nocc]."""
+        C = np.ascontiguousarray(C, dtype=np.float64)
+        n = self.nbasis
+        assert C.ndim == 2 and C.shape[0] == n
+        F = np.zeros((n, n))
+        E = np.zeros(2)
+        rho = np.zeros(self.npoint) if want_grid else None
+        v = np.zeros(self.npoint) if want_grid else None
+        ptr = lambda a: None if a is None else a.ctypes.data  # noqa: E731
+        cext.check(cext.napmo_grid_fock_build_orbitals(self._this, C.ctypes.data, C.shape[1], eta,
+                                                       ptr(F), ptr(E), ptr(rho), ptr(v)))
+        if want_grid:
+            return F, E[0], E[1], rho, v
+        return F, E[0], E[1]
+
+    def build_orbitals_device(self, dC, nocc, eta, dF, dE):
+        cext.check(cext.napmo_grid_fock_build_orbitals_device(self._this, dC, nocc, eta, dF, dE))
+
     def build_device(self, dP, dF, dE):
         """Asynchronous build on device pointers (ints), on the plan's stream."""
         cext.check(cext.napmo_grid_fock_build_device(self._this, dP, dF, dE))

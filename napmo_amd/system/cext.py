@@ -139,6 +139,10 @@ _SIGNATURES = {
                                      ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
     'napmo_grid_fock_build': (c_int, [vp, vp, vp, vp, vp, vp]),
     'napmo_grid_fock_build_device': (c_int, [vp, vp, vp, vp]),
+    'napmo_grid_fock_build_orbitals': (c_int, [vp, vp, c_int, c_double, vp, vp, vp, vp]),
+    'napmo_grid_fock_build_orbitals_device': (c_int, [vp, vp, c_int, c_double, vp, vp]),
+    'napmo_selftest_mfma': (c_int, [c_int, a1df]),
+    'napmo_device_copy': (c_int, [vp, vp, c_long]),
     'napmo_grid_overlap': (c_int, [vp, a2df]),
     'napmo_fock_plan_profile': (c_int, [vp, c_int]),
     'napmo_fock_plan_profile_read': (c_int, [vp, a1df, c_int, ctypes.POINTER(c_int)]),

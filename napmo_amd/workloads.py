@@ -61,9 +61,14 @@ def config2(seed=2, nrad=50, nang=590, natom=16, ncycle=30):
     return atoms, shells, nrad, nang, 12
 
 
-def density_matrix(rng, S, nocc):
-    """P = 2 C C^T, C = C0 (C0^T S C0)^(-1/2), C0 ~ N(0, 1): nocc S-orthonormal orbitals."""
+def occupied_orbitals(rng, S, nocc):
+    """C = C0 (C0^T S C0)^(-1/2), C0 ~ N(0, 1): nocc S-orthonormal orbitals [n x nocc]."""
     C0 = rng.normal(size=(S.shape[0], nocc))
     w, V = np.linalg.eigh(C0.T @ S @ C0)
-    C = C0 @ (V @ np.diag(w ** -0.5) @ V.T)
+    return C0 @ (V @ np.diag(w ** -0.5) @ V.T)
+
+
+def density_matrix(rng, S, nocc):
+    """P = 2 C C^T from ``occupied_orbitals`` (same random stream)."""
+    C = occupied_orbitals(rng, S, nocc)
     return 2.0 * C @ C.T

@@ -253,3 +253,61 @@ def fock_overlap(fock, n):
     psi = napmo_amd.PSIB(n)
     S = grid_ops.compute_coupling(psi, g, np.ascontiguousarray(phi.T), np.ones(g.size))
     return S, phi, psi
+
+
+def test_mfma_lane_layouts():
+    # the lane layouts the MFMA kernels assume, checked on one wavefront
+    from napmo_amd.system import cext
+    for which, K in ((0, 4), (1, 32)):
+        out = np.zeros(256)
+        cext.check(cext.napmo_selftest_mfma(which, out))
+        A = np.array([[m + k / 64.0 for k in range(K)] for m in range(16)])
+        B = np.array([[1.0 + (k - n) / 128.0 for n in range(16)] for k in range(K)])
+        if which == 1:  # the kernel feeds bf16-rounded operands
+            A, B = _bf16_round(A), _bf16_round(B)
+        ref = A @ B
+        assert np.allclose(out.reshape(16, 16), ref, rtol=1e-6 if which else 1e-14), which
+
+
+def _bf16_round(x):
+    b = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def test_grid_fock_orbital_form_matches_density_form():
+    import napmo_amd
+    atoms, shells, nr, na, o, phi, P = _config1()
+    g, b = systems.gpu_system(atoms, shells, nr, na)
+    fock = napmo_amd.GridFock(g, b, 10, 'fp64')
+    S = fock.overlap()
+    C = systems.density_matrix(np.random.default_rng(1), S, 1)  # P; rebuild C below
+    w, V = np.linalg.eigh(C)
+    Cocc = V[:, -1:] * np.sqrt(w[-1] / 2.0)                    # P = 2 C C^T
+    Fp, Ep, Egp, rhop, vp = fock.build(C, want_grid=True)
+    Fo, Eo, Ego, rhoo, vo = fock.build_orbitals(Cocc, 2.0, want_grid=True)
+    assert np.max(np.abs(rhoo - rhop)) < 1e-12 * np.abs(rhop).max()
+    assert abs(Eo - Ep) < 1e-10 and abs(Ego - Egp) < 1e-10
+    assert np.max(np.abs(Fo - Fp)) < 1e-10
+
+
+def test_grid_fock_bf16x3_mixed_mode():
+    # mixed mode: rho, Poisson and the energy trace stay FP64, only F uses the 3-term
+    # bf16 split. Stated tolerances: |dE_grid| <= 1e-10 Eh, max|dF| <= 2e-5 max|F|.
+    import napmo_amd
+    for atoms, shells, nr, na, lmax in (
+            (systems.H2_ATOMS, systems.named_shells(systems.H2_ATOMS, 'CC-PVDZ'), 100, 110, 10),
+            (_mol(31, natom=5, box=5.0), systems.random_shells(np.random.default_rng(31), 5, 6), 30, 110, 8)):
+        g, b = systems.gpu_system(atoms, shells, nr, na)
+        f64 = napmo_amd.GridFock(g, b, lmax, 'fp64')
+        mix = napmo_amd.GridFock(g, b, lmax, 'bf16x3')
+        S = f64.overlap()
+        P = systems.density_matrix(np.random.default_rng(3), S, 2)
+        F1, E1, Eg1 = f64.build(P)
+        F2, E2, Eg2 = mix.build(P)
+        dF = np.max(np.abs(F2 - F1))
+        print('bf16x3 n=%d max|dF|=%.3e max|F|=%.3e dE_trace=%.3e dE_grid=%.3e' %
+              (b['nbasis'], dF, np.abs(F1).max(), E2 - E1, Eg2 - Eg1))
+        assert abs(Eg2 - Eg1) < 1e-10
+        assert dF < 2e-5 * np.abs(F1).max()
+        assert np.array_equal(F2, F2.T)
