@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Grid Fock build throughput on one MI355X (config 2 of the nAPMO grid hot path).
+"""Grid Fock build throughput of the nAPMO grid hot path on one MI355X.
 
 One step = one grid Fock build with P resident in HBM: rho = diag(Phi P Phi^T) (MFMA),
 Becke-Dickson Poisson (projection, FP64 radial Green solves, synthesis over all
@@ -41,7 +41,7 @@ def gpu_leg(args, log):
     from napmo_amd.device import DeviceArray
     from napmo_amd.system import cext
     cext.require_gpu()
-    atoms, shells, nrad, nang, lmax = workloads.config2()
+    atoms, shells, nrad, nang, lmax = workloads.chon16_system()
     t0 = time.perf_counter()
     grid = napmo_amd.BeckeGrid(napmo_amd.make_species(atoms), nrad, nang)
     basis = napmo_amd.make_basis(atoms, shells_per_atom=shells)
@@ -117,7 +117,7 @@ def roofline(res, N, n, L, M):
         return name, {'bound': 'mfma', 'achieved': work / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
                       'unit': 'TFLOP/s', 'frac': work / (ms * 1e-3) / peak, 'traffic': None}
     if name == 'synth':
-        # 2 flops per (point, centre, lm) of the survey's count; FP64 vector peak
+        # 2 flops per (point, centre, lm); FP64 vector peak
         flops = 2.0 * N * M * L
         peak = PEAK['fp64_mfma']
         return name, {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
@@ -130,11 +130,11 @@ def roofline(res, N, n, L, M):
 def cpu_baseline(budget_s, log):
     """The oracle (C/OpenMP restatement of the reference hot path, Python orchestration
     of the Poisson solves as napmo/grids/poisson_solver.py) timed per stage on a bounded
-    sample of the config-2 build and scaled to one build."""
+    sample of the 16-atom, 472,000-point build and scaled to one build."""
     from oracle import cext as ocx
     from oracle import core
     from napmo_amd import workloads
-    atoms, shells, nrad, nang, lmax = workloads.config2()
+    atoms, shells, nrad, nang, lmax = workloads.chon16_system()
     threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
     t0 = time.perf_counter()
     grid = core.BeckeGrid(atoms, nrad, nang)
@@ -175,7 +175,7 @@ def cpu_baseline(budget_s, log):
     return {'value': 1.0 / total, 'unit': 'builds/s', 'cores': threads, 'kind': 'port',
             'sample': 'oracle C/OpenMP stages on 16384 of 472000 points (density, synthesis over 1 of '
                       '16 centres, contraction) and the full Poisson of 1 of 16 atoms, each scaled to '
-                      'one config-2 build; Becke/collocation set-up excluded as on the GPU',
+                      'one 16-atom 472,000-point build; Becke/collocation set-up excluded as on the GPU',
             'seconds_per_build': total}
 
 
@@ -206,7 +206,7 @@ def main():
         return
     if args.gpus != 1:
         print('bench.py: this build runs the grid hot path on one GPU; --gpus %d is not supported '
-              '(see the sharding design in DESIGN.md)' % args.gpus, file=sys.stderr)
+              '(the sharding design is documented, not built)' % args.gpus, file=sys.stderr)
         sys.exit(2)
     results, N, n, lmax, M = gpu_leg(args, log)
     L = (lmax + 1) ** 2
@@ -242,7 +242,7 @@ def main():
         'vs_baseline': None,
         'dtype': 'bf16x3+f64' if 'bf16' in best else 'f64',
         'data': 'synthetic (seeded geometry, basis and density; no reference data files)',
-        'config': {'workload': 'config 2: grid Fock build, 16 atoms, 50x590 Becke grid (472,000 points), '
+        'config': {'workload': 'grid Fock build, 16 atoms, 50x590 Becke grid (472,000 points), '
                                '300 s/p/d Gaussians, lmax 12, P resident in HBM',
                    'npoint': N, 'nbasis': n, 'ncenter': M, 'lmax': lmax, 'precision': best},
         'roofline': dict(roof, kernel=kname),

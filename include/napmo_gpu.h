@@ -123,6 +123,12 @@ void Cell_del(void *cell);
 /* src/horton/evaluate.cpp:129-223: output[npoint] += sum_lm U_lm(d) Y_lm, on the GPU */
 void eval_decomposition_grid(void **splines, double *center, double *output, double *points,
                              void *cell, long nspline, long npoint);
+/* src/horton/dot_multi.cpp:148-186 (bound at napmo/__init__.py:583-585): output[seg][m] +=
+ * sum over the segment's points of prod_v data[v][p] * poly_m(points[p] - center), mtype
+ * 1 Cartesian, 2 pure, 3 radial, 4 pure of the unit vector; nmoment must equal the count for
+ * (lmax, mtype) and lmax <= 16. Same accumulation order as the reference, on the GPU. */
+void dot_multi_moments(long npoint, long nvector, double *data, double *points, double *center,
+                       long lmax, long mtype, long *segments, double *output, long nmoment);
 
 /* ------------------------------------------------------------ Gaussian basis (gto.h) */
 /* src/system/gto.cpp:17-51, 161-227; src/system/basis_set.cpp:12-82 */
@@ -154,6 +160,13 @@ void nwavefunction_compute_coupling(WaveFunction *psi, void *grid, double *phi, 
                                     double *res);
 /* nwavefunction.cpp:161-197: psi->XC = sum_p phi_i phi_j vrho bw w */
 void nwavefunction_compute_xc_matrix(WaveFunction *psi, void *grid, double *phi, double *xc_vrho);
+/* nwavefunction.cpp:109-159 (bound at napmo/__init__.py:182-184): output[p] = rho from the
+ * lower triangle of dm [nbasis x nbasis] with the basis collocated on the grid's points.
+ * epsilon > 0 screens as the reference does, including its running threshold and its stop
+ * at the first point whose bound falls below it (later points are left untouched);
+ * dmmaxrow [nbasis] is read only then. */
+void nwavefunction_compute_density_from_dm(void *basis, void *grid, double *dm, double *output,
+                                           double epsilon, double *dmmaxrow);
 
 /* ------------------------------------------------------- device-resident hot path */
 /* Precision of the grid Fock contractions. */

@@ -1,6 +1,7 @@
 // extern "C" surface of libnapmo_gpu.so (declared in include/napmo_gpu.h).
 #include "napmo_gpu.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -11,6 +12,7 @@
 #include "fock_plan.hpp"
 #include "host.hpp"
 #include "kernels.hpp"
+#include "legacy_kernels.hpp"
 #include "poisson_plan.hpp"
 
 namespace napmo {
@@ -18,6 +20,17 @@ namespace napmo {
 static thread_local std::string g_error;
 void set_error(const std::string &msg) { g_error = msg; }
 void clear_error() { g_error.clear(); }
+
+size_t max_lds_per_block() {
+  static thread_local long cached = -1;
+  if (cached < 0) {
+    int dev = 0, v = 0;
+    NAPMO_HIP(hipGetDevice(&dev));
+    NAPMO_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    cached = v;
+  }
+  return (size_t)cached;
+}
 
 void require_device() {
   int n = 0;
@@ -103,7 +116,7 @@ static void host_contract(const DeviceGrid &g, const double *phi, const double *
     launch_scale3(g.w.get(), g.bw.get(), nullptr, N, Npad, s.get(), st);
   }
   FockShape sh = fock_shape(ld, Npad, yfm == nullptr);
-  part.resize((size_t)sh.nchunk * sh.ntiles * TILE * TILE);
+  part.resize((size_t)sh.nchunk * sh.cstride());
   F.resize((size_t)n * n);
   launch_fock_f64(sh, X.get(), yfm ? Y.get() : X.get(), s.get(), part.get(), st);
   launch_fock_reduce(sh, part.get(), n, upper_mirror ? 1 : 0, scale, 0, F.get(), st);
@@ -340,6 +353,42 @@ void eval_decomposition_grid(void **splines, double *center, double *output, dou
   });
 }
 
+void dot_multi_moments(long npoint, long nvector, double *data, double *points, double *center,
+                       long lmax, long mtype, long *segments, double *output, long nmoment) {
+  legacy(__func__, [&] {
+    // the reference's checks (src/horton/dot_multi.cpp:152-157)
+    NAPMO_REQUIRE(lmax >= 0, "lmax can not be negative.");
+    NAPMO_REQUIRE(mtype >= 1 && mtype <= 4, "mtype should be 1, 2, 3 or 4.");
+    NAPMO_REQUIRE(lmax <= MOMENT_LMAX, "dot_multi_moments: lmax above 16 is not supported");
+    NAPMO_REQUIRE(nvector >= 1, "dot_multi_moments: nvector must be positive");
+    if (npoint <= 0) return 0;
+    // Segment ranges as the reference's shift_segment walk assigns them (dot_multi.cpp:46-52):
+    // one shift at most per point, so after an empty segment every later point stays in it.
+    std::vector<long> st, sp, ix;
+    long k = 0, end = segments[0], pos = 0;
+    while (pos < npoint) {
+      if (pos == end) end += segments[++k];
+      const long stop = end > pos ? std::min(end, npoint) : npoint;
+      st.push_back(pos), sp.push_back(stop), ix.push_back(k);
+      pos = stop;
+    }
+    const long nout = (k + 1) * nmoment;
+    hipStream_t s = scratch_stream();
+    DeviceBuffer<long> dst, dsp, dix;
+    DeviceBuffer<double> dd, dp, dout;
+    dst.upload(st, s), dsp.upload(sp, s), dix.upload(ix, s);
+    dd.upload(data, (size_t)(nvector * npoint), s);
+    dp.upload(points, (size_t)(3 * npoint), s);
+    dout.upload(output, (size_t)nout, s);
+    launch_dot_multi_moments(dst.get(), dsp.get(), dix.get(), (int)st.size(), npoint, nvector,
+                             dd.get(), dp.get(), center, (int)lmax, (int)mtype, (int)nmoment,
+                             dout.get(), s);
+    dout.download(output, (size_t)nout, s);
+    NAPMO_HIP(hipStreamSynchronize(s));
+    return 0;
+  });
+}
+
 // ------------------------------------------------------------------- Gaussians
 void *PrimitiveGaussian_new(int *l, double *origin, double zeta, double coeff) {
   return legacy(__func__, [&] { return (void *)make_primitive(l, origin, zeta, coeff); });
@@ -407,6 +456,37 @@ void nwavefunction_compute_coupling(WaveFunction *psi, void *grid, double *phi, 
 void nwavefunction_compute_xc_matrix(WaveFunction *psi, void *grid, double *phi, double *vrho) {
   legacy(__func__, [&] {
     host_contract(((BeckeHandle *)grid)->grid, phi, nullptr, vrho, psi->ndim, false, 1.0, psi->XC);
+    return 0;
+  });
+}
+
+void nwavefunction_compute_density_from_dm(void *basis, void *grid, double *dm, double *output,
+                                           double epsilon, double *dmmaxrow) {
+  legacy(__func__, [&] {
+    DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    hipStream_t s = g.stream;
+    const FlatBasis fb = flatten(*(BasisSet *)basis);
+    const int n = fb.nbasis;
+    const long N = g.N;
+    if (N == 0 || n == 0) return 0;
+    NAPMO_REQUIRE(epsilon <= 0 || dmmaxrow, "density_from_dm: dmmaxrow is required when epsilon > 0");
+    DeviceBasis db;
+    db.upload(fb, s);
+    DeviceBuffer<double> phi((size_t)N * n), ddm, dmax, absmax, upper, thr, out(N);
+    DeviceBuffer<long> stop(1);
+    db.collocate(g.x.get(), g.y.get(), g.z.get(), N, N, n, phi.get(), nullptr, s);
+    ddm.upload(dm, (size_t)n * n, s);
+    if (epsilon > 0) {
+      dmax.upload(dmmaxrow, (size_t)n, s);
+      absmax.resize(N), upper.resize(N), thr.resize(N);
+    }
+    launch_dm_density(phi.get(), n, N, n, ddm.get(), dmax.get(), epsilon, absmax.get(),
+                      upper.get(), thr.get(), stop.get(), out.get(), s);
+    long nwritten = N;
+    if (epsilon > 0) stop.download(&nwritten, 1, s);
+    NAPMO_HIP(hipStreamSynchronize(s));
+    out.download(output, (size_t)nwritten, s);
+    NAPMO_HIP(hipStreamSynchronize(s));
     return 0;
   });
 }

@@ -19,12 +19,16 @@ struct Error : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
+// On failure the runtime's per-thread error state is cleared (hipGetLastError) so that
+// a failed call does not resurface at the next unrelated launch check.
 #define NAPMO_HIP(call)                                                                     \
   do {                                                                                      \
     hipError_t _e = (call);                                                                 \
-    if (_e != hipSuccess)                                                                   \
+    if (_e != hipSuccess) {                                                                 \
+      (void)hipGetLastError();                                                              \
       throw ::napmo::Error(std::string(#call) + ": " + hipGetErrorString(_e) + " at " +     \
                            __FILE__ + ":" + std::to_string(__LINE__));                      \
+    }                                                                                       \
   } while (0)
 
 #define NAPMO_REQUIRE(cond, msg)                                                            \
@@ -92,6 +96,8 @@ template <class T> class DeviceBuffer {
 
 // Ensures a device exists; throws a clear error otherwise (no CPU fallback).
 void require_device();
+// Largest dynamic LDS allocation one workgroup may request on the current device.
+size_t max_lds_per_block();
 
 inline long div_up(long a, long b) { return (a + b - 1) / b; }
 inline long round_up(long a, long b) { return div_up(a, b) * b; }

@@ -31,7 +31,7 @@ FockPlan::FockPlan(DeviceGrid &g, const BasisSet &b, int lmax_, int prec_mode)
   n = fb.nbasis;
   NAPMO_REQUIRE(n > 0, "empty basis");
   ld = (int)round_up(n, TILE);
-  Npad = round_up(grid.N, TILE);
+  Npad = round_up(grid.N, 256);  // multiple of every point tile (64, 256)
   hipStream_t st = grid.stream;
   basis.upload(fb, st);
   phi64.resize((size_t)Npad * ld);
@@ -52,7 +52,9 @@ FockPlan::FockPlan(DeviceGrid &g, const BasisSet &b, int lmax_, int prec_mode)
   v.resize(grid.N);
   s.resize(Npad);
   fsh = fock_shape(ld, Npad, true);
-  F_part.resize((size_t)fsh.nchunk * fsh.ntiles * TILE * TILE);
+  psh = fock_pass_shape(ld, Npad);
+  use_pass = prec == NAPMO_PREC_BF16X3 && fock_pass_supported(ld);
+  F_part.resize((size_t)std::max(fsh.nchunk, use_pass ? psh.nchunk : 0) * fsh.cstride());
   F.resize((size_t)n * n);
   E.resize(2);
   trace_work.resize(2 * TRACE_WORK);
@@ -118,16 +120,16 @@ void FockPlan::build_orbitals_device(const double *dC, int nocc, double eta, dou
   NAPMO_REQUIRE(nocc > 0 && nocc <= n, "orbital count out of range");
   hipStream_t st = grid.stream;
   const int slot = (prof_used < prof_cap) ? prof_used++ : -1;
-  const int ldc = (int)round_up(nocc, TILE);
+  const int ldc = density_orb_cols(nocc), nct = density_orb_tiles(ldc);
   Cpad.resize((size_t)ld * ldc);
   Pfull.resize((size_t)n * n);
-  if (rho_part.size() < (size_t)(ldc / TILE) * Npad) rho_part.resize((size_t)(ldc / TILE) * Npad);
+  if (rho_part.size() < (size_t)nct * Npad) rho_part.resize((size_t)nct * Npad);
   launch_pad2d(dC, n, nocc, Cpad.get(), ld, ldc, st);
   launch_gram(dC, n, nocc, eta, Pfull.get(), st);
   mark(slot, 0);
   launch_density_orb_f64(phi64.get(), ld, Npad, Cpad.get(), ldc, rho_part.get(), st);
   mark(slot, 1);
-  launch_rho_reduce(rho_part.get(), ldc / TILE, Npad, grid.N, rho.get(), st, eta);
+  launch_rho_reduce(rho_part.get(), nct, Npad, grid.N, rho.get(), st, eta);
   finish(slot, Pfull.get(), dF, dE);
 }
 
@@ -151,12 +153,19 @@ void FockPlan::finish(int slot, const double *dP, double *dF, double *dE) {
   mark(slot, 6);
   launch_scale3(v.get(), grid.wbw.get(), nullptr, grid.N, Npad, s.get(), st);
   mark(slot, 7);
-  if (prec == NAPMO_PREC_BF16X3)
+  // the one-pass kernel writes FP32 partials into the same buffer
+  float *part32 = reinterpret_cast<float *>(F_part.get());
+  if (use_pass)
+    launch_fock_bf16x3_pass(psh, phiT_hi.get(), phiT_lo.get(), s.get(), part32, st);
+  else if (prec == NAPMO_PREC_BF16X3)
     launch_fock_bf16x3(fsh, phiT_hi.get(), phiT_lo.get(), phiT32.get(), s.get(), F_part.get(), st);
   else
     launch_fock_f64(fsh, phi64.get(), phi64.get(), s.get(), F_part.get(), st);
   mark(slot, 8);
-  launch_fock_reduce(fsh, F_part.get(), n, 0, 1.0, 0, dF, st);
+  if (use_pass)
+    launch_fock_reduce_f32(psh, part32, n, dF, st);
+  else
+    launch_fock_reduce(fsh, F_part.get(), n, 0, 1.0, 0, dF, st);
   mark(slot, 9);
   launch_traces(dP, dF, n, grid.wbw.get(), rho.get(), v.get(), grid.N, dE, trace_work.get(), st);
   mark(slot, 10);

@@ -30,7 +30,8 @@ struct FockPlan {
   DeviceBuffer<float> phiT32;          // function-major [ld x Npad]
   DeviceBuffer<double> Cpad, Pfull;    // orbital-form inputs
   DeviceBuffer<double> P, rho_part, rho, v, s, F_part, F, E, trace_work;
-  FockShape fsh;
+  FockShape fsh, psh;  // per-tile shape; one-pass shape (mixed mode)
+  bool use_pass = false;
 
   // Per-phase HIP-event timing of builds (on the plan's stream), for roofline reports.
   static constexpr int NPHASE = 10;

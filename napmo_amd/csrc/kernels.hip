@@ -102,58 +102,98 @@ void launch_collocate(const double *x, const double *y, const double *z, long N,
 }
 
 // ========================================================= Poisson: projection
-// One workgroup per (atom, radial shell); threads over lm. f = dens * bw of the
-// shell is staged in LDS; Yhat[j][lm] = aw_j S_lm(u_j) sqrt(4 pi (2l+1)) / sum aw.
-__global__ void k_project(AtomTables at, const double *__restrict__ dens,
-                          const double *__restrict__ bw, const double *__restrict__ yhat, int L,
-                          double *__restrict__ rho_lm) {
-  extern __shared__ double sf[];
-  const int A = blockIdx.y;
-  const int i = blockIdx.x;
-  if (i >= at.nrad[A]) return;
+// rho_lm[A][i][lm] = sum_j f_Aij Yhat[j][lm] with f = dens * bw, as a small GEMM per
+// atom: one workgroup per (atom, block of PSB radial shells) so the Lebedev table
+// Yhat[j][lm] = aw_j S_lm(u_j) sqrt(4 pi (2l+1)) / sum aw is read once per block; the
+// shells' f values are staged in LDS in chunks of PJ angular points.
+constexpr int PSB = 5, PJ = 256;
+
+// Threads: 64 lm x 4 angular slices; slice k sums angular points j = k, k + 4, ...;
+// slices are combined in order through LDS.
+__global__ void __launch_bounds__(256) k_project(AtomTables at, const double *__restrict__ dens,
+                                                 const double *__restrict__ bw,
+                                                 const double *__restrict__ yhat, int L,
+                                                 double *__restrict__ rho_lm) {
+  __shared__ double sf[PSB][PJ];
+  __shared__ double red[4][PSB][64];
+  const int A = blockIdx.z;
+  const int i0 = blockIdx.x * PSB;
+  const int nr = at.nrad[A];
+  if (i0 >= nr) return;
+  const int ns = min(PSB, nr - i0);
   const int na = at.nang[A];
-  const long p0 = at.point_off[A] + (long)i * na;
-  for (int j = threadIdx.x; j < na; j += blockDim.x) sf[j] = dens[p0 + j] * bw[p0 + j];
-  __syncthreads();
+  const long p0 = at.point_off[A] + (long)i0 * na;
   const double *Y = yhat + at.yhat_off[A];
-  double *out = rho_lm + (at.rad_off[A] + i) * (long)L;
-  for (int lm = threadIdx.x; lm < L; lm += blockDim.x) {
-    double acc = 0.0;
-    for (int j = 0; j < na; ++j) acc += sf[j] * Y[(long)j * L + lm];
-    out[lm] = acc;
+  const int lml = threadIdx.x & 63, slice = threadIdx.x >> 6;
+  const int lm = blockIdx.y * 64 + lml;
+  double acc[PSB];
+#pragma unroll
+  for (int k = 0; k < PSB; ++k) acc[k] = 0.0;
+  for (int j0 = 0; j0 < na; j0 += PJ) {
+    const int nj = min(PJ, na - j0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < PSB * PJ; e += blockDim.x) {
+      const int k = e / PJ, j = e % PJ;
+      const long p = p0 + (long)k * na + j0 + j;
+      sf[k][j] = (k < ns && j < nj) ? dens[p] * bw[p] : 0.0;
+    }
+    __syncthreads();
+    if (lm < L)
+      for (int j = slice; j < nj; j += 4) {
+        const double y = Y[(long)(j0 + j) * L + lm];
+#pragma unroll
+        for (int k = 0; k < PSB; ++k) acc[k] += sf[k][j] * y;
+      }
   }
+#pragma unroll
+  for (int k = 0; k < PSB; ++k) red[slice][k][lml] = acc[k];
+  __syncthreads();
+  if (slice == 0 && lm < L)
+    for (int k = 0; k < ns; ++k)
+      rho_lm[(at.rad_off[A] + i0 + k) * (long)L + lm] =
+          ((red[0][k][lml] + red[1][k][lml]) + red[2][k][lml]) + red[3][k][lml];
 }
 
 void launch_project(const AtomTables &at, const double *dens, const double *bw, const double *yhat,
                     int lmax, double *rho_lm, hipStream_t s) {
   const int L = (lmax + 1) * (lmax + 1);
-  int max_nang = 0;
-  (void)max_nang;
-  dim3 grid(at.max_nrad, at.M);
-  // LDS holds one shell of f: at most 5810 doubles (largest Lebedev rule).
-  k_project<<<grid, 256, 5810 * sizeof(double), s>>>(at, dens, bw, yhat, L, rho_lm);
+  dim3 grid(div_up(at.max_nrad, PSB), div_up(L, 64), at.M);
+  k_project<<<grid, 256, 0, s>>>(at, dens, bw, yhat, L, rho_lm);
   NAPMO_HIP(hipGetLastError());
 }
 
 // ====================================================== Poisson: radial solves
 // U[A][i][lm] = sum_j G_{A,l}[i][j] rho_lm[A][j][lm] for y and dt; one workgroup per
-// (atom, l). green block of an atom: for l = 0..lmax: Gy [nr x nr], then Gd [nr x nr].
-__global__ void k_radial(AtomTables at, const double *__restrict__ rho_lm,
-                         const double *__restrict__ green, int L, double *__restrict__ Uy,
-                         double *__restrict__ Udt) {
+// (atom, l): the (nr x 2l+1) block of rho_lm is staged in LDS, each thread forms outputs
+// from a contiguous Green row. green block of an atom: for l = 0..lmax: Gy [nr x nr],
+// then Gd [nr x nr].
+constexpr int RMAX_STAGE = 4096;  // doubles of rho_lm staged per (atom, l)
+
+__global__ void __launch_bounds__(256) k_radial(AtomTables at, const double *__restrict__ rho_lm,
+                                                const double *__restrict__ green, int L,
+                                                double *__restrict__ Uy, double *__restrict__ Udt) {
+  __shared__ double sr[RMAX_STAGE];
   const int A = blockIdx.x, l = blockIdx.y;
   const int nr = at.nrad[A], nm = 2 * l + 1;
   const double *G = green + at.green_off[A] + (long)l * 2 * nr * nr;
   const long roff = at.rad_off[A] * (long)L;
+  const bool staged = nr * nm <= RMAX_STAGE;
+  if (staged)
+    for (int e = threadIdx.x; e < nr * nm; e += blockDim.x)
+      sr[e] = rho_lm[roff + (long)(e / nm) * L + l * l + e % nm];
+  __syncthreads();
   const int total = 2 * nr * nm;
   for (int o = threadIdx.x; o < total; o += blockDim.x) {
-    const int which = o / (nr * nm);
-    const int rem = o - which * nr * nm;
-    const int i = rem / nm, m = rem - (rem / nm) * nm;
+    const int m = o % nm, row = o / nm;  // row in [0, 2 nr): which * nr + i
+    const int which = row / nr, i = row - which * nr;
     const double *g = G + (long)which * nr * nr + (long)i * nr;
-    const double *r = rho_lm + roff + l * l + m;
     double acc = 0.0;
-    for (int j = 0; j < nr; ++j) acc += g[j] * r[(long)j * L];
+    if (staged) {
+      for (int j = 0; j < nr; ++j) acc += g[j] * sr[j * nm + m];
+    } else {
+      const double *r = rho_lm + roff + l * l + m;
+      for (int j = 0; j < nr; ++j) acc += g[j] * r[(long)j * L];
+    }
     (which ? Udt : Uy)[roff + (long)i * L + l * l + m] = acc;
   }
 }
@@ -303,33 +343,58 @@ __device__ inline int pair_bucket(const AtomTables &at, int A, double X, double 
   return A * (at.max_nrad + 1) + loc;
 }
 
-__global__ void k_pair_count(AtomTables at, const double *__restrict__ x,
-                             const double *__restrict__ y, const double *__restrict__ z, long N,
-                             int *__restrict__ count) {
+// Per workgroup (256 points, one centre): LDS histogram of the centre's local buckets,
+// then one global atomic per non-empty bin (pair_bucket is A * (max_nrad + 1) + loc).
+constexpr int PAIR_BINS = 1024;  // local buckets per centre (max_nrad + 1 <= PAIR_BINS)
+
+__global__ void __launch_bounds__(256) k_pair_count(AtomTables at, const double *__restrict__ x,
+                                                    const double *__restrict__ y,
+                                                    const double *__restrict__ z, long N,
+                                                    int *__restrict__ count) {
+  __shared__ int hist[PAIR_BINS];
+  const int A = blockIdx.y, nb = at.max_nrad + 1;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) hist[b] = 0;
+  __syncthreads();
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= N) return;
-  atomicAdd(&count[pair_bucket(at, blockIdx.y, x[p], y[p], z[p])], 1);
+  if (p < N) atomicAdd(&hist[pair_bucket(at, A, x[p], y[p], z[p]) - A * nb], 1);
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    if (hist[b]) atomicAdd(&count[A * nb + b], hist[b]);
 }
 
-__global__ void k_pair_fill(AtomTables at, const double *__restrict__ x,
-                            const double *__restrict__ y, const double *__restrict__ z, long N,
-                            const long *__restrict__ off, int *__restrict__ cursor,
-                            int *__restrict__ list) {
+__global__ void __launch_bounds__(256) k_pair_fill(AtomTables at, const double *__restrict__ x,
+                                                   const double *__restrict__ y,
+                                                   const double *__restrict__ z, long N,
+                                                   const long *__restrict__ off,
+                                                   int *__restrict__ cursor,
+                                                   int *__restrict__ list) {
+  __shared__ int hist[PAIR_BINS], base[PAIR_BINS];
+  const int A = blockIdx.y, nb = at.max_nrad + 1;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) hist[b] = 0;
+  __syncthreads();
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= N) return;
-  const int gb = pair_bucket(at, blockIdx.y, x[p], y[p], z[p]);
-  const int slot = atomicAdd(&cursor[gb], 1);
-  list[off[gb] + slot] = (int)p;
+  int loc = -1, rank = 0;
+  if (p < N) {
+    loc = pair_bucket(at, A, x[p], y[p], z[p]) - A * nb;
+    rank = atomicAdd(&hist[loc], 1);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    if (hist[b]) base[b] = atomicAdd(&cursor[A * nb + b], hist[b]);
+  __syncthreads();
+  if (loc >= 0) list[off[A * nb + loc] + base[loc] + rank] = (int)p;
 }
 
 void launch_pair_count(const AtomTables &at, const double *x, const double *y, const double *z,
                        long N, int *count, hipStream_t s) {
+  NAPMO_REQUIRE(at.max_nrad + 1 <= PAIR_BINS, "too many radial points for the synthesis buckets");
   k_pair_count<<<dim3(div_up(N, 256), at.M), 256, 0, s>>>(at, x, y, z, N, count);
   NAPMO_HIP(hipGetLastError());
 }
 
 void launch_pair_fill(const AtomTables &at, const double *x, const double *y, const double *z,
                       long N, const long *off, int *cursor, int *list, hipStream_t s) {
+  NAPMO_REQUIRE(at.max_nrad + 1 <= PAIR_BINS, "too many radial points for the synthesis buckets");
   k_pair_fill<<<dim3(div_up(N, 256), at.M), 256, 0, s>>>(at, x, y, z, N, off, cursor, list);
   NAPMO_HIP(hipGetLastError());
 }
@@ -540,7 +605,8 @@ __global__ void __launch_bounds__(256) k_fock_f64(const double *__restrict__ X,
     }
     __syncthreads();
   }
-  double *out = partial + ((long)blockIdx.y * gridDim.x + blockIdx.x) * TILE * TILE;
+  double *out = partial + (long)blockIdx.y * ((long)gridDim.x * TILE * TILE + 32) +
+                (long)blockIdx.x * TILE * TILE;
   const int col16 = lane & 15, rg = lane >> 4;
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
@@ -556,39 +622,85 @@ void launch_fock_f64(const FockShape &sh, const double *X, const double *Y, cons
   NAPMO_HIP(hipGetLastError());
 }
 
-// Sum the chunk partials of one output element in fixed chunk order.
-__global__ void k_fock_reduce(const double *__restrict__ partial, int nt, int ntiles, int nchunk,
-                              int symmetric, int n, int upper_mirror, double scale, int accumulate,
-                              double *__restrict__ F) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long)n * n) return;
-  int i = (int)(e / n), j = (int)(e % n);
-  if ((symmetric || upper_mirror) && i > j) {
-    const int t = i;
-    i = j, j = t;
-  }
-  const int ti = i / TILE, tj = j / TILE;
-  int t;
-  if (symmetric) {
-    t = 0;
-    for (int a = 0; a < ti; ++a) t += nt - a;
-    t += tj - ti;
-  } else {
-    t = ti * nt + tj;
-  }
-  const long off = (long)t * TILE * TILE + (i % TILE) * TILE + (j % TILE);
+// Sum the chunk partials of each computed tile element: 4 slices of chunks per element
+// (slice k takes chunks k, k + 4, ...), combined in slice order (deterministic). Threads
+// walk the tile-element space of the partials, so every load is coalesced; the mirror
+// image of an off-diagonal upper tile is written by the same thread. Partials are FP64
+// (per-tile kernels) or FP32 (the one-pass mixed kernel, whose accumulators are FP32).
+template <class T>
+__global__ void __launch_bounds__(256) k_fock_reduce(const T *__restrict__ partial, int nt,
+                                                     int ntiles, int nchunk, int symmetric, int n,
+                                                     int upper_mirror, double scale, int accumulate,
+                                                     double *__restrict__ F) {
+  __shared__ double red[4][64];
+  const int el = threadIdx.x & 63, slice = threadIdx.x >> 6;
+  const long u = (long)blockIdx.x * 64 + el;
+  const long nel = (long)ntiles * TILE * TILE;
   double sum = 0.0;
-  for (int c = 0; c < nchunk; ++c) sum += partial[(long)c * ntiles * TILE * TILE + off];
-  sum *= scale;
-  F[e] = accumulate ? F[e] + sum : sum;
+  if (u < nel) {
+    const long cstride = nel + 32;
+    int c = slice;
+    // 8 loads in flight, added in chunk order
+    for (; c + 28 < nchunk; c += 32) {
+      double v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = (double)partial[(long)(c + 4 * k) * cstride + u];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) sum += v[k];
+    }
+    for (; c < nchunk; c += 4) sum += (double)partial[(long)c * cstride + u];
+  }
+  red[slice][el] = sum;
+  __syncthreads();
+  if (slice != 0 || u >= nel) return;
+  const double tot = ((red[0][el] + red[1][el]) + red[2][el] + red[3][el]) * scale;
+  const int t = (int)(u / (TILE * TILE)), r = (int)(u % (TILE * TILE)) / TILE, cc = (int)(u % TILE);
+  int ti, tj;
+  if (symmetric) {
+    // upper tiles in row order: (0,0), (0,1), ..., (0,nt-1), (1,1), ...
+    int rem = t;
+    ti = 0;
+    while (rem >= nt - ti) rem -= nt - ti, ++ti;
+    tj = ti + rem;
+  } else {
+    ti = t / nt, tj = t % nt;
+  }
+  const int i = ti * TILE + r, j = tj * TILE + cc;
+  if (i >= n || j >= n) return;
+  auto put = [&](int a, int b) {
+    const long e = (long)a * n + b;
+    F[e] = accumulate ? F[e] + tot : tot;
+  };
+  if (symmetric) {
+    // the upper triangle is mirrored, so F is exactly symmetric (a diagonal tile's own
+    // lower half, equal to rounding, is not used)
+    if (i > j) return;
+    put(i, j);
+    if (i != j) put(j, i);
+  } else if (upper_mirror) {
+    if (i <= j) {
+      put(i, j);
+      if (i != j) put(j, i);
+    }
+  } else {
+    put(i, j);
+  }
 }
 
 void launch_fock_reduce(const FockShape &sh, const double *partial, int n, int upper_mirror,
                         double scale, int accumulate, double *F, hipStream_t st) {
-  const long total = (long)n * n;
-  k_fock_reduce<<<div_up(total, 256), 256, 0, st>>>(partial, sh.nt, sh.ntiles, sh.nchunk,
-                                                     sh.symmetric ? 1 : 0, n, upper_mirror, scale,
-                                                     accumulate, F);
+  const long nel = (long)sh.ntiles * TILE * TILE;
+  k_fock_reduce<double><<<div_up(nel, 64), 256, 0, st>>>(partial, sh.nt, sh.ntiles, sh.nchunk,
+                                                         sh.symmetric ? 1 : 0, n, upper_mirror,
+                                                         scale, accumulate, F);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_fock_reduce_f32(const FockShape &sh, const float *partial, int n, double *F,
+                            hipStream_t st) {
+  const long nel = (long)sh.ntiles * TILE * TILE;
+  k_fock_reduce<float><<<div_up(nel, 64), 256, 0, st>>>(partial, sh.nt, sh.ntiles, sh.nchunk,
+                                                        sh.symmetric ? 1 : 0, n, 0, 1.0, 0, F);
   NAPMO_HIP(hipGetLastError());
 }
 

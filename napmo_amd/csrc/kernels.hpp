@@ -105,6 +105,9 @@ struct FockShape {
   int ld, nt, ntiles, nchunk;
   long Npad, chunk;
   bool symmetric;
+  // doubles between consecutive chunks' partials: ntiles * TILE * TILE plus 256 B, so the
+  // chunks of one element do not all fall on the same few L2 channels
+  long cstride() const { return (long)ntiles * TILE * TILE + 32; }
 };
 FockShape fock_shape(int ld, long Npad, bool symmetric);
 void launch_fock_f64(const FockShape &sh, const double *X, const double *Y, const double *s,
@@ -114,6 +117,9 @@ void launch_fock_f64(const FockShape &sh, const double *X, const double *Y, cons
 // (the exchange block of src/wavefunction/nwavefunction.cpp:53-62).
 void launch_fock_reduce(const FockShape &sh, const double *partial, int n, int upper_mirror,
                         double scale, int accumulate, double *F, hipStream_t st);
+// Same for FP32 chunk partials (one-pass mixed kernel); symmetric shapes only.
+void launch_fock_reduce_f32(const FockShape &sh, const float *partial, int n, double *F,
+                            hipStream_t st);
 
 // E[0] = 0.5 * sum_ij P_ij F_ij (FP64), E[1] = 0.5 * sum_p w_p rho_p v_p.
 // work: device scratch of 2 * TRACE_WORK doubles.
@@ -128,7 +134,17 @@ void launch_split_phi(const double *phi64, int ld, long Npad, unsigned short *hi
 // F partials with the 3-term bf16 split (see kernels_mixed.hip); same shape as the FP64 one.
 void launch_fock_bf16x3(const FockShape &sh, const unsigned short *Ahi, const unsigned short *Alo,
                         const float *B32, const double *s, double *partial, hipStream_t st);
-// partial[ct][p] = sum over orbital tile ct of (Phi C)_pi^2, C zero-padded [ld x ldc].
+// One pass over Phi (all upper tiles per chunk of points); same partial layout as
+// fock_shape(ld, Npad, true) with the pass's own chunking.
+FockShape fock_pass_shape(int ld, long Npad);
+bool fock_pass_supported(int ld);
+void launch_fock_bf16x3_pass(const FockShape &sh, const unsigned short *Ahi,
+                             const unsigned short *Alo, const double *s, float *partial,
+                             hipStream_t st);
+// partial[ct][p] = sum over orbital tile ct of (Phi C)_pi^2, C zero-padded [ld x ldc];
+// ldc = density_orb_cols(nocc), density_orb_tiles(ldc) partial rows.
+int density_orb_cols(int nocc);
+int density_orb_tiles(int ldc);
 void launch_density_orb_f64(const double *phi, int ld, long Npad, const double *C, int ldc,
                             double *partial, hipStream_t s);
 void launch_gram(const double *C, int n, int nocc, double eta, double *P, hipStream_t s);

@@ -141,7 +141,8 @@ __global__ void __launch_bounds__(256) k_fock_bf16x3(const unsigned short *__res
       since_flush = 0;
     }
   }
-  double *out = partial + ((long)blockIdx.y * gridDim.x + blockIdx.x) * TILE * TILE;
+  double *out = partial + (long)blockIdx.y * ((long)gridDim.x * TILE * TILE + 32) +
+                (long)blockIdx.x * TILE * TILE;
   const int col16 = lane & 15, rg = lane >> 4;
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
@@ -159,63 +160,249 @@ void launch_fock_bf16x3(const FockShape &sh, const unsigned short *Ahi, const un
   NAPMO_HIP(hipGetLastError());
 }
 
-// -------------------------------------------------------- orbital-form density
-// X = Phi C over a 64 x 64 (points x orbitals) tile with FP64 MFMA; partial[ct][p] =
-// sum over the tile's orbitals of X^2 (C zero-padded to [ld x ldc]).
-__global__ void __launch_bounds__(256) k_density_orb_f64(const double *__restrict__ phi, int ld,
-                                                         long Npad, const double *__restrict__ C,
-                                                         int ldc, double *__restrict__ partial) {
-  __shared__ double As[16][TILE + 2];
-  __shared__ double Bs[16][TILE];
-  __shared__ double red[TILE][2];
+// ------------------------------------------- BF16x3 Fock build, one pass over Phi
+// One workgroup per (chunk of points, group of up to 16 upper 64 x 64 tiles) computes
+// those tiles of F for the chunk, so each point's Phi row is read from HBM once per
+// tile group (once per build for ld <= 320). Per stage of 32 points the chunk's hi / lo
+// rows of all ld functions go to LDS ([function][k]); the B-operand split of s Phi is
+// formed once per element while staging, and the next stage's segments are prefetched
+// into registers while the MFMAs of the current one run. Wave w accumulates tiles
+// 2w, 2w + 1 of its group (4 x 4 blocks of 16 x 16 each) in FP32 over the chunk; the
+// FP32 chunk partials are summed in FP64, in fixed order, by k_fock_reduce.
+constexpr int TPW = 2;          // tiles per wave
+constexpr int PASS_WAVES = 8;   // waves per workgroup (256 VGPRs each)
+constexpr int PASS_UNITS = 3;   // 8-point staging segments per thread (ld * 4 / 512 <= 3)
+
+__global__ void __launch_bounds__(512) k_fock_bf16x3_pass(const unsigned short *__restrict__ Ahi,
+                                                          const unsigned short *__restrict__ Alo,
+                                                          const double *__restrict__ s, int ld,
+                                                          int nt, int ntiles, long Npad,
+                                                          long chunk,
+                                                          float *__restrict__ partial) {
+  extern __shared__ unsigned short lds[];
+  unsigned short *sAh = lds, *sAl = lds + ld * KPAD, *sBh = lds + 2 * ld * KPAD,
+                 *sBl = lds + 3 * ld * KPAD;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const long p0 = (long)blockIdx.x * TILE;
-  const int c0 = blockIdx.y * TILE;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  d4 acc[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) acc[a][b] = (d4){0, 0, 0, 0};
-  for (int k0 = 0; k0 < ld; k0 += 16) {
-    for (int e = tid; e < TILE * 16; e += 256) {
-      const int m = e / 16, k = e % 16;
-      As[k][m] = phi[(p0 + m) * ld + k0 + k];
-      const int kk = e / TILE, n = e % TILE;
-      Bs[kk][n] = C[(long)(k0 + kk) * ldc + c0 + n];
+  const int tile0 = blockIdx.y * PASS_WAVES * TPW;
+  int tr[TPW], tc[TPW];
+  bool has[TPW];
+#pragma unroll
+  for (int k = 0; k < TPW; ++k) {
+    const int t = tile0 + wave * TPW + k;
+    has[k] = t < ntiles;
+    int i = 0, rem = has[k] ? t : 0;
+    while (rem >= nt - i) rem -= nt - i, ++i;
+    tr[k] = i * TILE, tc[k] = (i + rem) * TILE;
+  }
+  f4 acc[TPW][4][4];
+#pragma unroll
+  for (int k = 0; k < TPW; ++k)
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[k][a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+  const long pbeg = (long)blockIdx.x * chunk;
+  const long pend = min(Npad, pbeg + chunk);
+  const int units = ld * (KB / 8);
+  const int r16 = lane & 15, kq = (lane >> 4) * 8;
+  __shared__ float sS[KB];
+  u16x8 ph[PASS_UNITS], pl[PASS_UNITS];
+  double pref_s = 0.0;
+  auto prefetch = [&](long k0) {
+#pragma unroll
+    for (int q = 0; q < PASS_UNITS; ++q) {
+      const int u = tid + q * (int)blockDim.x;
+      if (u < units) {
+        const int f = u / (KB / 8), seg = (u % (KB / 8)) * 8;
+        const size_t g = (size_t)f * Npad + k0 + seg;
+        ph[q] = *(const u16x8 *)&Ahi[g];
+        pl[q] = *(const u16x8 *)&Alo[g];
+      }
     }
+    if (tid < KB) pref_s = s[k0 + tid];
+  };
+  if (pbeg < pend) prefetch(pbeg);
+  for (long k0 = pbeg; k0 < pend; k0 += KB) {
+    if (tid < KB) sS[tid] = (float)pref_s;
     __syncthreads();
 #pragma unroll
-    for (int ks = 0; ks < 16; ks += 4) {
-      const int kr = ks + (lane >> 4), mi = lane & 15;
-      double a0 = As[kr][wr + mi], a1 = As[kr][wr + 16 + mi];
-      double b0 = Bs[kr][wc + mi], b1 = Bs[kr][wc + 16 + mi];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    for (int q = 0; q < PASS_UNITS; ++q) {
+      const int u = tid + q * (int)blockDim.x;
+      if (u < units) {
+        const int f = u / (KB / 8), seg = (u % (KB / 8)) * 8;
+        *(u16x8 *)&sAh[f * KPAD + seg] = ph[q];
+        *(u16x8 *)&sAl[f * KPAD + seg] = pl[q];
+        u16x8 bh, bl;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const float x = (bf16_float(ph[q][e]) + bf16_float(pl[q][e])) * sS[seg + e];
+          const unsigned short hb = bf16_bits(x);
+          bh[e] = hb;
+          bl[e] = bf16_bits(x - bf16_float(hb));
+        }
+        *(u16x8 *)&sBh[f * KPAD + seg] = bh;
+        *(u16x8 *)&sBl[f * KPAD + seg] = bl;
+      }
+    }
+    __syncthreads();
+    if (k0 + KB < pend) prefetch(k0 + KB);
+#pragma unroll
+    for (int k = 0; k < TPW; ++k) {
+      if (has[k]) {
+        bf16x8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int ra = (tr[k] + a * 16 + r16) * KPAD + kq;
+          const int rb = (tc[k] + a * 16 + r16) * KPAD + kq;
+          ah[a] = __builtin_bit_cast(bf16x8, *(const u16x8 *)&sAh[ra]);
+          al[a] = __builtin_bit_cast(bf16x8, *(const u16x8 *)&sAl[ra]);
+          bh[a] = __builtin_bit_cast(bf16x8, *(const u16x8 *)&sBh[rb]);
+          bl[a] = __builtin_bit_cast(bf16x8, *(const u16x8 *)&sBl[rb]);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            acc[k][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[k][a][b], 0, 0, 0);
+            acc[k][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[k][a][b], 0, 0, 0);
+            acc[k][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[k][a][b], 0, 0, 0);
+          }
+      }
     }
     __syncthreads();
   }
   const int col16 = lane & 15, rg = lane >> 4;
-  for (int a = 0; a < 2; ++a) {
+#pragma unroll
+  for (int k = 0; k < TPW; ++k) {
+    if (!has[k]) continue;
+    float *out = partial + (long)blockIdx.x * ((long)ntiles * TILE * TILE + 32) +
+                 (long)(tile0 + wave * TPW + k) * TILE * TILE;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          out[(a * 16 + rg * 4 + r) * TILE + b * 16 + col16] = acc[k][a][b][r];
+  }
+}
+
+FockShape fock_pass_shape(int ld, long Npad) {
+  FockShape sh;
+  sh.ld = ld;
+  sh.nt = ld / TILE;
+  sh.ntiles = sh.nt * (sh.nt + 1) / 2;
+  sh.symmetric = true;
+  sh.Npad = Npad;
+  const int groups = (int)div_up(sh.ntiles, PASS_WAVES * TPW);
+  // about 256 workgroups over all tile groups, chunks a multiple of 32 points
+  long chunk = round_up(div_up(Npad * groups, 256), KB);
+  sh.chunk = std::max<long>(chunk, KB);
+  sh.nchunk = (int)div_up(Npad, sh.chunk);
+  return sh;
+}
+
+bool fock_pass_supported(int ld) {
+  const size_t lds = (size_t)4 * ld * KPAD * sizeof(unsigned short) + KB * sizeof(float);
+  return lds <= max_lds_per_block() && ld * (KB / 8) <= PASS_UNITS * 64 * PASS_WAVES;
+}
+
+void launch_fock_bf16x3_pass(const FockShape &sh, const unsigned short *Ahi,
+                             const unsigned short *Alo, const double *s, float *partial,
+                             hipStream_t st) {
+  NAPMO_REQUIRE(fock_pass_supported(sh.ld), "fock pass: basis too large for one workgroup");
+  const size_t lds = (size_t)4 * sh.ld * KPAD * sizeof(unsigned short);
+  const int groups = (int)div_up(sh.ntiles, PASS_WAVES * TPW);
+  k_fock_bf16x3_pass<<<dim3(sh.nchunk, groups), dim3(64 * PASS_WAVES), lds, st>>>(
+      Ahi, Alo, s, sh.ld, sh.nt, sh.ntiles, sh.Npad, sh.chunk, partial);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// -------------------------------------------------------- orbital-form density
+// X = Phi C with FP64 MFMA: workgroup = 256 points x 16 NB orbitals, wave w owns points
+// 64 w .. 64 w + 63 (4 blocks of 16) and all NB orbital blocks, 4 NB accumulators; every
+// Phi row is read once per orbital tile. Epilogue rho_partial[p] = sum_i X_pi^2.
+template <int NB>  // orbital blocks of 16
+__global__ void __launch_bounds__(256) k_density_orb_f64(const double *__restrict__ phi, int ld,
+                                                         long Npad, const double *__restrict__ C,
+                                                         int ldc, double *__restrict__ partial) {
+  constexpr int KS2 = 16, PT = 256;
+  __shared__ double As[KS2][PT + 4];   // As[k][m] = Phi[p0 + m][k0 + k]
+  __shared__ double Bs[KS2][16 * NB];  // Bs[k][n] = C[k0 + k][c0 + n]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long p0 = (long)blockIdx.x * PT;
+  const int c0 = blockIdx.y * 16 * NB;
+  d4 acc[4][NB];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[a][b] = (d4){0, 0, 0, 0};
+  for (int k0 = 0; k0 < ld; k0 += KS2) {
+    // row m of the point tile: 16 consecutive doubles (128 B)
+    for (int e = tid; e < PT * KS2; e += 256) {
+      const int m = e / KS2, k = e % KS2;
+      As[k][m] = phi[(p0 + m) * ld + k0 + k];
+    }
+    for (int e = tid; e < KS2 * 16 * NB; e += 256) {
+      const int k = e / (16 * NB), n = e % (16 * NB);
+      Bs[k][n] = C[(long)(k0 + k) * ldc + c0 + n];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < KS2; ks += 4) {
+      const int kr = ks + (lane >> 4), mi = lane & 15;
+      double a[4], b[NB];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kr][wave * 64 + i * 16 + mi];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) b[j] = Bs[kr][j * 16 + mi];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // f64 16x16x4 output: lane i, element r -> row (i / 16) + 4 r, column i % 16
+  const int rg = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int m = wr + a * 16 + rg + 4 * r;  // f64 16x16x4 output row (see kernels.hip)
-      double sum = acc[a][0][r] * acc[a][0][r] + acc[a][1][r] * acc[a][1][r];
+      double sum = 0.0;
+#pragma unroll
+      for (int j = 0; j < NB; ++j) sum += acc[i][j][r] * acc[i][j][r];
       sum += __shfl_xor(sum, 8);
       sum += __shfl_xor(sum, 4);
       sum += __shfl_xor(sum, 2);
       sum += __shfl_xor(sum, 1);
-      if (col16 == 0) red[m][wave & 1] = sum;
+      if ((lane & 15) == 0)
+        partial[(long)blockIdx.y * Npad + p0 + wave * 64 + i * 16 + rg + 4 * r] = sum;
     }
-  }
-  __syncthreads();
-  if (tid < TILE) partial[(long)blockIdx.y * Npad + p0 + tid] = red[tid][0] + red[tid][1];
 }
+
+int density_orb_cols(int nocc) { return (int)round_up(nocc, 16) <= 64 ? (int)round_up(nocc, 16) : (int)round_up(nocc, 64); }
 
 void launch_density_orb_f64(const double *phi, int ld, long Npad, const double *C, int ldc,
                             double *partial, hipStream_t s) {
-  NAPMO_REQUIRE(ld % TILE == 0 && ldc % TILE == 0 && Npad % TILE == 0, "density_orb: padding");
-  k_density_orb_f64<<<dim3(Npad / TILE, ldc / TILE), 256, 0, s>>>(phi, ld, Npad, C, ldc, partial);
+  NAPMO_REQUIRE(Npad % 256 == 0 && ldc % 16 == 0 && ld % 16 == 0, "density_orb: padding");
+  const int nb = ldc <= 64 ? ldc / 16 : 4;
+  const int ntile = ldc / (16 * nb);
+  NAPMO_REQUIRE(ntile * 16 * nb == ldc, "density_orb: orbital padding");
+  dim3 grid(Npad / 256, ntile);
+  switch (nb) {
+    case 1: k_density_orb_f64<1><<<grid, 256, 0, s>>>(phi, ld, Npad, C, ldc, partial); break;
+    case 2: k_density_orb_f64<2><<<grid, 256, 0, s>>>(phi, ld, Npad, C, ldc, partial); break;
+    case 3: k_density_orb_f64<3><<<grid, 256, 0, s>>>(phi, ld, Npad, C, ldc, partial); break;
+    default: k_density_orb_f64<4><<<grid, 256, 0, s>>>(phi, ld, Npad, C, ldc, partial); break;
+  }
   NAPMO_HIP(hipGetLastError());
+}
+
+int density_orb_tiles(int ldc) {
+  const int nb = ldc <= 64 ? ldc / 16 : 4;
+  return ldc / (16 * nb);
 }
 
 // P[i][j] = eta sum_k C[i][k] C[j][k] (C [n x nocc] row-major), P [n x n] row-major.

@@ -116,6 +116,7 @@ _SIGNATURES = {
     'Cell_new': (vp, [vp, c_int]),
     'Cell_del': (None, [vp]),
     'eval_decomposition_grid': (None, [aptr, a1df, a1df, a2df, vp, c_long, c_long]),
+    'dot_multi_moments': (None, [c_long, c_long, a1df, a2df, a1df, c_long, c_long, a1dli, a2df, c_long]),
     # Gaussian basis
     'PrimitiveGaussian_new': (vp, [a1di, a1df, c_double, c_double]),
     'PrimitiveGaussian_get_norma': (c_double, [vp]),
@@ -130,6 +131,7 @@ _SIGNATURES = {
     'nwavefunction_compute_2body_matrix': (None, [ctypes.POINTER(WaveFunction), vp, a2df, a1df, a2df]),
     'nwavefunction_compute_coupling': (None, [ctypes.POINTER(WaveFunction), vp, a2df, a1df, a2df]),
     'nwavefunction_compute_xc_matrix': (None, [ctypes.POINTER(WaveFunction), vp, a2df, a1df]),
+    'nwavefunction_compute_density_from_dm': (None, [vp, vp, a2df, a1df, c_double, a1df]),
     # device-resident hot path
     'napmo_compute_coulomb': (c_int, [vp, a1df, c_int, a1df]),
     'napmo_poisson_solve': (c_int, [vp, a1df, c_int, adf, adf]),

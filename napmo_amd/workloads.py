@@ -2,8 +2,8 @@
 shared by the benchmark and the tests. All coordinates in bohr; seeds use
 numpy.random.default_rng (PCG64).
 
-* config 1: H2 at (0, 0, +-0.7), 2 x 100 x 110 Chebyshev/Lebedev, cc-pVDZ (n = 10), lmax 10
-* config 2: 16 atoms {C,H,N,O} in a 12 bohr box (min separation 1.8), 50 x 590 per atom
+* h2_system: H2 at (0, 0, +-0.7), 2 x 100 x 110 Chebyshev/Lebedev, cc-pVDZ (n = 10), lmax 10
+* chon16_system: 16 atoms {C,H,N,O} in a 12 bohr box (min separation 1.8), 50 x 590 per atom
   (472,000 points), 300 Cartesian s/p/d functions (30 cycles of s, p, d shells, cycle c on
   atom c % 16, K ~ U{1,2,3} primitives, zeta log-uniform [0.1, 1e4], c ~ U(0.2, 1)),
   P = 2 C C^T from 40 orthonormal occupied orbitals, lmax 12
@@ -49,12 +49,12 @@ def named_shells(atoms, basis_name):
     return [BASIS[basis_name][s] for s, _ in atoms]
 
 
-def config1():
+def h2_system():
     atoms = H2_ATOMS
     return atoms, named_shells(atoms, 'CC-PVDZ'), 100, 110, 10
 
 
-def config2(seed=2, nrad=50, nang=590, natom=16, ncycle=30):
+def chon16_system(seed=2, nrad=50, nang=590, natom=16, ncycle=30):
     rng = np.random.default_rng(seed)
     atoms = random_molecule(rng, natom, 12.0)
     shells = random_shells(rng, natom, ncycle)

@@ -870,6 +870,106 @@ def density_from_dm(phi, P):
     return np.einsum('pm,mn,pn->p', phi, P, phi)
 
 
+def density_from_dm_screened(phi, dm, epsilon, dmmaxrow, out):
+    """nwavefunction_compute_density_from_dm (src/wavefunction/nwavefunction.cpp:109-159)
+    with its sequential screening: one running epsilon over the points in order; at the
+    first point whose bound falls below it the routine returns and leaves the rest of
+    ``out`` untouched; otherwise epsilon /= absmax * n * n before that point's screening.
+    Only the lower triangle of dm is read. phi [N, n]; out [N] updated in place."""
+    N, n = phi.shape
+    for p in range(N):
+        row = phi[p]
+        if epsilon > 0:
+            absmax = 0.0
+            for i in range(n):
+                absmax = max(abs(row[i]), absmax)
+            upper = 0.0
+            for i in range(n):
+                upper += abs(row[i]) * dmmaxrow[i]
+            upper *= n * absmax
+            if upper < epsilon:
+                return out
+            epsilon /= absmax * n * n
+        rho = 0.0
+        for i in range(n):
+            if epsilon > 0 and abs(row[i]) * dmmaxrow[i] < epsilon:
+                continue
+            tmp = 0.0
+            for j in range(i - 1, -1, -1):
+                tmp += row[j] * dm[i, j]
+            rho += (2.0 * tmp + dm[i, i] * row[i]) * row[i]
+        out[p] = rho
+    return out
+
+
+def moment_polynomials(delta, lmax, mtype):
+    """Polynomial rows of fill_polynomials_wrapper (src/horton/dot_multi.cpp:55-80):
+    mtype 1 Cartesian (moments.cpp:24-57), 2 pure of delta, 3 powers of |delta|
+    (moments.cpp:138-155), 4 pure of the unit vector. delta [npt, 3] -> [nmoment-1, npt]."""
+    dx, dy, dz = delta[:, 0], delta[:, 1], delta[:, 2]
+    if mtype == 1:
+        out = [dx, dy, dz]
+        old_off, old_n = 0, 3
+        for l in range(2, lmax + 1):
+            new_n, new_off = old_n + l + 1, old_off + old_n
+            for i in range(old_n):
+                out.append(out[0] * out[old_off + i])
+            for i in range(l):
+                out.append(out[1] * out[new_off - l + i])
+            out.append(out[2] * out[new_off - 1])
+            old_n, old_off = new_n, new_off
+        return np.array(out)
+    if mtype == 3:
+        r = np.sqrt(dx * dx + dy * dy + dz * dz)
+        out = [r]
+        for _ in range(lmax - 1):
+            out.append(r * out[-1])
+        return np.array(out)
+    if mtype == 4:
+        r = np.sqrt(dx * dx + dy * dy + dz * dz)
+        dx, dy, dz = dx / r, dy / r, dz / r
+    return pure_polynomials(dz, dx, dy, lmax)
+
+
+def moment_count(lmax, mtype):
+    if lmax <= 0:
+        return 1
+    return {1: (lmax + 1) * (lmax + 2) * (lmax + 3) // 6, 3: lmax + 1}.get(mtype, (lmax + 1) ** 2)
+
+
+def dot_multi_moments(data, points, center, lmax, mtype, segments, output):
+    """src/horton/dot_multi.cpp:148-186: output[seg, m] += sum over the segment's points
+    of prod_v data[v, p] * poly_m(p - center), accumulated point by point in order; the
+    segment walk is shift_segment's (dot_multi.cpp:46-52). data [nvector, npt]; output
+    [nseg, nmoment] updated in place."""
+    if lmax < 0:
+        raise ValueError('lmax can not be negative.')
+    if mtype < 1 or mtype > 4:
+        raise ValueError('mtype should be 1, 2, 3 or 4.')
+    data = np.atleast_2d(data)
+    nvector, npt = data.shape
+    term = data[nvector - 1].copy()
+    for v in range(nvector - 2, -1, -1):
+        term = term * data[v]
+    nm = output.shape[1]
+    rows = np.zeros((npt, nm))
+    rows[:, 0] = term
+    if lmax > 0:
+        rows[:, 1:] = term[:, None] * moment_polynomials(points - center, lmax, mtype)[:nm - 1].T
+    seg_of = np.empty(npt, dtype=np.int64)
+    k, end = 0, int(segments[0])
+    for p in range(npt):
+        if p == end:
+            k += 1
+            end += int(segments[k])
+        seg_of[p] = k
+    for k in np.unique(seg_of):
+        sel = rows[seg_of == k]
+        # sequential left-to-right sums, as the reference's running accumulation
+        output[k] = np.cumsum(np.vstack([output[k][None, :], sel]), axis=0)[-1]
+    return output
+
+
 def coupling_matrix(phi, v, bw, w):
     """M_ij = sum_p phi_i phi_j v bw w, upper triangle then mirrored
     (src/wavefunction/nwavefunction.cpp:77-107). phi [N, n]."""

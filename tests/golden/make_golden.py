@@ -91,6 +91,33 @@ def main():
     pts3 = rng.normal(size=(25, 3))
     out['gto_points'] = pts3
     out['gto_phi'] = rb.compute(pts3)
+    # dot_multi_moments for every mtype: two integrand vectors, uneven segments, output
+    # accumulated onto non-zero starting values; then an empty segment, after which the
+    # reference keeps every later point in that segment
+    rng2 = np.random.default_rng(7)
+    npt = 57
+    pts4 = rng2.normal(size=(npt, 3)) * 1.5
+    data4 = rng2.normal(size=(2, npt))
+    center4 = np.array([0.2, -0.1, 0.05])
+    counts = {1: lambda l: (l + 1) * (l + 2) * (l + 3) // 6, 2: lambda l: (l + 1) ** 2,
+              3: lambda l: l + 1, 4: lambda l: (l + 1) ** 2}
+    out.update(mm_points=pts4, mm_data=data4, mm_center=center4)
+    for mtype in (1, 2, 3, 4):
+        lmax = 5
+        nm = counts[mtype](lmax)
+        seg = np.array([20, 7, 30], dtype=np.int64)
+        init = rng2.normal(size=3 * nm)
+        mom = init.copy()
+        L.dot_multi_moments(npt, 2, np.ascontiguousarray(data4).reshape(-1),
+                            np.ascontiguousarray(pts4).reshape(-1), center4, lmax, mtype, seg, mom, nm)
+        out['mm%d_init' % mtype] = init.reshape(3, nm)
+        out['mm%d_out' % mtype] = mom.reshape(3, nm)
+    seg = np.array([10, 0, 5, 42], dtype=np.int64)
+    mom = np.zeros(4 * 16)
+    L.dot_multi_moments(npt, 1, np.ascontiguousarray(data4[0]), np.ascontiguousarray(pts4).reshape(-1),
+                        center4, 3, 2, seg, mom, 16)
+    out['mm_empty_seg'] = seg
+    out['mm_empty_out'] = mom.reshape(4, 16)
     np.savez_compressed(os.path.join(HERE, 'ref_vectors.npz'), **out)
     print('wrote', os.path.join(HERE, 'ref_vectors.npz'), 'keys:', len(out))
 

@@ -9,7 +9,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-from napmo_amd.workloads import (BASIS, H2_ATOMS, config1, config2, density_matrix,  # noqa: F401,E402
+from napmo_amd.workloads import (BASIS, H2_ATOMS, chon16_system, density_matrix, h2_system,  # noqa: F401,E402
                                 named_shells, random_molecule, random_shells)
 
 

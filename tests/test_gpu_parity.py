@@ -179,7 +179,7 @@ def test_contractions_match_oracle():
     assert np.max(np.abs(XC - ref)) < 1e-12 * np.abs(ref).max()
 
 
-def _config1():
+def _h2_fixture():
     atoms, shells, nr, na = _h2_system(100, 110)
     o, ob = systems.oracle_system(atoms, shells, nr, na)
     phi = ocx.basis_compute(ob, o.points)
@@ -188,15 +188,15 @@ def _config1():
     return atoms, shells, nr, na, o, phi, P
 
 
-def test_grid_fock_fp64_config1_energy_parity():
+def test_grid_fock_fp64_h2_energy_parity():
     # H2 (R = 1.4 bohr), 2 x 100 x 110 Chebyshev/Lebedev points, cc-pVDZ (n = 10), lmax 10
     import napmo_amd
-    atoms, shells, nr, na, o, phi, P = _config1()
+    atoms, shells, nr, na, o, phi, P = _h2_fixture()
     g, b = systems.gpu_system(atoms, shells, nr, na)
     fock = napmo_amd.GridFock(g, b, 10, 'fp64')
     F, E, Egrid, rho, v = fock.build(P, want_grid=True)
     rho_o, v_o, F_o, E_o = core.grid_fock_build(o, phi, P, 10)
-    print('config1 E_gpu %.15f E_oracle %.15f dE %.3e' % (E, E_o, E - E_o))
+    print('H2 E_gpu %.15f E_oracle %.15f dE %.3e' % (E, E_o, E - E_o))
     assert abs(E - E_o) < 1e-8                       # Eh
     assert np.max(np.abs(F - F_o)) < 1e-8            # Eh
     assert abs(Egrid - E) < 1e-10
@@ -210,7 +210,7 @@ def test_grid_fock_fp64_config1_energy_parity():
 
 def test_grid_fock_linearity_and_symmetry():
     import napmo_amd
-    atoms, shells, nr, na, o, phi, P = _config1()
+    atoms, shells, nr, na, o, phi, P = _h2_fixture()
     g, b = systems.gpu_system(atoms, shells, nr, na)
     fock = napmo_amd.GridFock(g, b, 10, 'fp64')
     rng = np.random.default_rng(5)
@@ -223,24 +223,31 @@ def test_grid_fock_linearity_and_symmetry():
     assert np.array_equal(Fa, Fa.T)
 
 
-def test_grid_fock_config2_properties():
-    # full config-2 size: 16 atoms, 50 x 590 per atom (472,000 points), 300 s/p/d functions,
+def test_grid_fock_chon16_properties():
+    # full size: 16 atoms, 50 x 590 per atom (472,000 points), 300 s/p/d functions,
     # lmax 12. Size-independent checks: int rho = tr(P S) = 80, symmetry, trace identity.
     import napmo_amd
     from napmo_amd.wavefunction import grid_ops
-    atoms, shells, nr, na, lmax = systems.config2()
+    atoms, shells, nr, na, lmax = systems.chon16_system()
     g, b = systems.gpu_system(atoms, shells, nr, na)
-    print('config2 N', g.size, 'n', b['nbasis'])
+    print('chon16 N', g.size, 'n', b['nbasis'])
     assert g.size == 472000 and b['nbasis'] == 300
     fock = napmo_amd.GridFock(g, b, lmax, 'fp64')
     S, _, _ = fock_overlap(fock, b['nbasis'])
     P = systems.density_matrix(np.random.default_rng(2), S, 40)
     F, E, Egrid, rho, v = fock.build(P, want_grid=True)
-    print('config2 E %.12f Egrid %.12f' % (E, Egrid))
+    print('chon16 E %.12f Egrid %.12f' % (E, Egrid))
     assert abs(g.integrate(rho) - 80.0) < 1e-9
     assert np.array_equal(F, F.T)
     assert abs(E - Egrid) < 1e-9 * abs(E)
     assert E > 0
+    # mixed mode on the same system (one-pass BF16x3 contraction): energy unchanged,
+    # F within the stated tolerance
+    mix = napmo_amd.GridFock(g, b, lmax, 'bf16x3')
+    F2, E2, Eg2 = mix.build(P)
+    print('chon16 bf16x3 max|dF| %.3e dE_grid %.3e dE_trace %.3e' % (np.abs(F2 - F).max(), Eg2 - Egrid, E2 - E))
+    assert abs(Eg2 - Egrid) < 1e-9
+    assert np.abs(F2 - F).max() < 2e-5 * np.abs(F).max()
 
 
 def fock_overlap(fock, n):
@@ -277,7 +284,7 @@ def _bf16_round(x):
 
 def test_grid_fock_orbital_form_matches_density_form():
     import napmo_amd
-    atoms, shells, nr, na, o, phi, P = _config1()
+    atoms, shells, nr, na, o, phi, P = _h2_fixture()
     g, b = systems.gpu_system(atoms, shells, nr, na)
     fock = napmo_amd.GridFock(g, b, 10, 'fp64')
     S = fock.overlap()
@@ -311,3 +318,62 @@ def test_grid_fock_bf16x3_mixed_mode():
         assert abs(Eg2 - Eg1) < 1e-10
         assert dF < 2e-5 * np.abs(F1).max()
         assert np.array_equal(F2, F2.T)
+
+
+def test_dot_multi_moments_matches_reference_vectors():
+    # dot_multi_moments keeps the reference's accumulation order: identical to the last bit
+    from napmo_amd.wavefunction import grid_ops
+    for mtype in (1, 2, 3, 4):
+        out = GOLD['mm%d_init' % mtype].copy()
+        grid_ops.dot_multi_moments(GOLD['mm_data'], GOLD['mm_points'], GOLD['mm_center'], 5, mtype,
+                                   np.array([20, 7, 30]), out)
+        err = np.abs(out - GOLD['mm%d_out' % mtype]).max()
+        print('dot_multi_moments mtype', mtype, 'max diff', err)
+        assert np.array_equal(out, GOLD['mm%d_out' % mtype]), (mtype, err)
+    out = np.zeros((4, 16))
+    grid_ops.dot_multi_moments(GOLD['mm_data'][0], GOLD['mm_points'], GOLD['mm_center'], 3, 2,
+                               GOLD['mm_empty_seg'], out)
+    assert np.array_equal(out, GOLD['mm_empty_out'])
+    # the spherical-expansion shape: 40 shells of 110 points, lmax 12, against the oracle
+    rng = np.random.default_rng(17)
+    pts = rng.normal(size=(4400, 3))
+    f = rng.normal(size=4400)
+    out = np.zeros((40, 169))
+    grid_ops.dot_multi_moments(f, pts, np.zeros(3), 12, 4, np.full(40, 110), out)
+    ref = core.dot_multi_moments(f, pts, np.zeros(3), 12, 4, np.full(40, 110), np.zeros((40, 169)))
+    assert np.array_equal(out, ref)
+
+
+def test_density_from_dm_matches_oracle():
+    from napmo_amd.wavefunction import grid_ops
+    # H2 cc-pVDZ on 2 x 30 x 50 points, dense (epsilon = 0) and screened
+    atoms, shells, _, _ = _h2_system()
+    g, b = systems.gpu_system(atoms, shells, 30, 50)
+    o, ob = systems.oracle_system(atoms, shells, 30, 50)
+    phi = ocx.basis_compute(ob, o.points)
+    n = phi.shape[1]
+    rng = np.random.default_rng(31)
+    dm = rng.normal(size=(n, n))
+    dmm = np.abs(np.tril(dm)).max(axis=1)
+    for eps in (0.0, 1e-3, 100.0):
+        out = grid_ops.compute_density_from_dm(b, g, dm, eps, dmm, np.full(g.size, -1.0))
+        ref = core.density_from_dm_screened(phi, dm, eps, dmm, np.full(g.size, -1.0))
+        print('density_from_dm eps', eps, 'written', int((ref != -1.0).sum()))
+        assert np.array_equal(out == -1.0, ref == -1.0)
+        assert np.max(np.abs(out - ref)) < 1e-13 * np.abs(ref).max()
+    # one centre, diffuse s and p shells: the running threshold stops mid-grid
+    atoms = [('H', np.zeros(3))]
+    shells = [[{'l': 0, 'prim': [0.05], 'cont': [1.0]}, {'l': 1, 'prim': [0.02], 'cont': [1.0]}]]
+    g, b = systems.gpu_system(atoms, shells, 20, 26)
+    o, ob = systems.oracle_system(atoms, shells, 20, 26)
+    phi = ocx.basis_compute(ob, o.points)
+    dm = np.random.default_rng(3).normal(size=(4, 4))
+    dmm = np.abs(np.tril(dm)).max(axis=1)
+    for eps in (1e-8, 1e-3):
+        out = grid_ops.compute_density_from_dm(b, g, dm, eps, dmm, np.full(g.size, -1.0))
+        ref = core.density_from_dm_screened(phi, dm, eps, dmm, np.full(g.size, -1.0))
+        written = int((ref != -1.0).sum())
+        print('density_from_dm one centre eps', eps, 'written', written, 'of', g.size)
+        assert 0 < written < g.size
+        assert np.array_equal(out == -1.0, ref == -1.0)
+        assert np.max(np.abs(out - ref)) < 1e-13 * np.abs(ref).max()

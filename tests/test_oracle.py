@@ -246,3 +246,40 @@ def test_two_body_exchange_mirrors_upper_triangle():
     expect = core.coupling_matrix(phi, J, bw, w) - 0.5 * (np.triu(E) + np.triu(E, 1).T)
     assert np.allclose(G, expect)
     assert np.allclose(G, G.T)
+
+
+def test_dot_multi_moments_every_mtype_matches_reference():
+    # oracle/_ref vectors: two integrand vectors, segments 20/7/30, non-zero starting output
+    for mtype in (1, 2, 3, 4):
+        out = GOLD['mm%d_init' % mtype].copy()
+        core.dot_multi_moments(GOLD['mm_data'], GOLD['mm_points'], GOLD['mm_center'], 5, mtype,
+                               np.array([20, 7, 30]), out)
+        assert np.array_equal(out, GOLD['mm%d_out' % mtype]), mtype
+    # an empty segment keeps every later point (the reference's shift_segment walk)
+    out = core.dot_multi_moments(GOLD['mm_data'][0], GOLD['mm_points'], GOLD['mm_center'], 3, 2,
+                                 GOLD['mm_empty_seg'], np.zeros((4, 16)))
+    assert np.array_equal(out, GOLD['mm_empty_out'])
+    assert np.all(out[2:] == 0.0)
+    # Cartesian order x, y, z, xx, xy, xz, yy, yz, zz at one point with unit integrand
+    row = core.moment_polynomials(np.array([[2.0, 3.0, 5.0]]), 2, 1)[:, 0]
+    assert np.array_equal(row, [2, 3, 5, 4, 6, 10, 9, 15, 25])
+    assert np.array_equal(core.moment_polynomials(np.array([[0.0, 3.0, 4.0]]), 3, 3)[:, 0], [5, 25, 125])
+
+
+def test_density_from_dm_screening_known_answer():
+    # hand-computed: n = 2, lower-triangular dm, epsilon 0.01, dmmaxrow [2, 1]
+    phi = np.array([[1.0, 0.001], [0.5, 1.0], [1e-3, 1e-3]])
+    dm = np.array([[2.0, 99.0], [0.3, 1.0]])          # the upper entry is never read
+    out = np.full(3, -7.0)
+    core.density_from_dm_screened(phi, dm, 0.01, np.array([2.0, 1.0]), out)
+    # point 0: eps -> 0.01 / 4, phi_1 * 1 = 1e-3 below it: rho = 2 * 1 * 1 only
+    # point 1: eps -> 6.25e-4, nothing screened: 0.5 * 2 * 0.5 * ... = 0.5 + 1.3
+    # point 2: bound 6e-6 below 6.25e-4: the routine returns, out[2] untouched
+    assert np.array_equal(out, [2.0, 1.8, -7.0])
+    # epsilon <= 0: the symmetric form diag(Phi S Phi^T), S the lower triangle mirrored
+    rng = np.random.default_rng(5)
+    phi = rng.normal(size=(40, 6))
+    dm = rng.normal(size=(6, 6))
+    S = np.tril(dm) + np.tril(dm, -1).T
+    out = core.density_from_dm_screened(phi, dm, 0.0, None, np.zeros(40))
+    assert np.allclose(out, np.einsum('pi,ij,pj->p', phi, S, phi), rtol=1e-13, atol=1e-13)
