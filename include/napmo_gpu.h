@@ -225,6 +225,38 @@ int napmo_memcpy(void *dst, const void *src, size_t bytes, int kind);
 void *napmo_fock_plan_stream(void *plan);
 int napmo_fock_plan_sync(void *plan);
 
+/* Gradients of the build's energy E = 1/2 sum_p w bw rho v (no reference counterpart: the
+ * reference has no grid gradients). Basis centres are the distinct function origins in
+ * order of first appearance; xyz [ncentre x 3] may be NULL. */
+int napmo_fock_plan_centres(void *plan, int *ncentre, double *xyz);
+/* dE/dR_c [ncentre x 3] for the plan's last build, whose density matrix P must be passed
+ * again: the grid stays fixed and every basis function moves with its centre; grad phi is
+ * recomputed on the fly. dE/dP is the build's F itself (F = dE/dP, to the asymmetry of the
+ * discrete Poisson operator). */
+int napmo_grid_fock_gradient(void *plan, const double *P, double *grad_out);
+int napmo_grid_fock_gradient_device(void *plan, const double *dP, double *dG);
+
+/* Multi-species (APMO) Coulomb build: species s (grid s, basis s, charge q_s, lmax s), one
+ * Becke grid per species (two species may pass the same grid). Per build, with
+ * rho_s from P_s:
+ *   v_eff_s = q_s^2 V[rho_s] + q_s sum_{t != s} q_t V[rho_t]  at s's points
+ *   F_s = Phi_s^T diag(w bw v_eff_s) Phi_s  = q_s^2 G_s(Coulomb) + J_s(coupling)
+ * replacing compute_coulomb + nwavefunction_compute_2body_matrix (charge^2) and the
+ * per-pair nwavefunction_compute_coupling loop of napmo/wavefunction/psi_numeric.py:152-241
+ * and psi_hybrid.py:81-110 (the reference shares the electronic grid,
+ * napmo/solver/hf_solver.py:95-98; V[rho_t] here is t's expansion synthesised at s's
+ * points, which equals the reference's when the grids are shared).
+ * E_out [4 nspecies]: 1/2 sum P_s o F_s, 1/2 sum w bw rho_s v_eff_s, the own-Coulomb part
+ * 1/2 q_s^2 sum w bw rho_s V[rho_s], and the coupling part (second minus third). */
+int napmo_apmo_plan_new(int nspecies, void **grids, void **bases, const double *charges,
+                        const int *lmax, int prec_mode, void **plan_out);
+void napmo_apmo_plan_del(void *plan);
+/* Host pointers: P[s] [n_s x n_s]; F_out[s] [n_s x n_s] (entries may be NULL). */
+int napmo_apmo_build(void *plan, const double **P, double **F_out, double *E_out);
+/* Device pointers; asynchronous on the species' streams (napmo_apmo_sync waits). */
+int napmo_apmo_build_device(void *plan, const double **dP, double **dF, double *dE);
+int napmo_apmo_sync(void *plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,7 @@
 #include <memory>
 
 #include "common.hpp"
+#include "apmo_plan.hpp"
 #include "fock_plan.hpp"
 #include "host.hpp"
 #include "kernels.hpp"
@@ -250,10 +251,10 @@ double *AtomicGrid_spherical_expansion(void *g, double *f, int lmax) {
     dg.bw.upload(one, dg.stream);
     PoissonPlan &pl = dg.plan(lmax);
     AtomTables at = dg.tables(pl);
-    launch_project(at, dens.get(), dg.bw.get(), pl.yhat.get(), lmax, pl.rho_lm.get(), dg.stream);
+    launch_project(at, dens.get(), dg.bw.get(), pl.yhat.get(), lmax, pl.work.rho_lm.get(), dg.stream);
     const int L = (lmax + 1) * (lmax + 1);
     double *out = (double *)std::malloc(sizeof(double) * dg.atoms[0].nrad * L);
-    pl.rho_lm.download(out, (size_t)dg.atoms[0].nrad * L, dg.stream);
+    pl.work.rho_lm.download(out, (size_t)dg.atoms[0].nrad * L, dg.stream);
     NAPMO_HIP(hipStreamSynchronize(dg.stream));
     return out;
   });
@@ -511,8 +512,8 @@ int napmo_poisson_solve(void *grid, const double *dens, int lmax, double *U_y, d
     d.upload(dens, g.N, g.stream);
     g.poisson_expand(d.get(), lmax);
     PoissonPlan &pl = g.plan(lmax);
-    pl.Uy.download(U_y, pl.Uy.size(), g.stream);
-    pl.Udt.download(U_dt, pl.Udt.size(), g.stream);
+    pl.work.Uy.download(U_y, pl.work.Uy.size(), g.stream);
+    pl.work.Udt.download(U_dt, pl.work.Udt.size(), g.stream);
     NAPMO_HIP(hipStreamSynchronize(g.stream));
   });
 }
@@ -577,6 +578,57 @@ int napmo_fock_plan_profile_read(void *plan, double *avg_ms, int nphase, int *nb
   });
 }
 const char *napmo_fock_phase_name(int k) { return FockPlan::phase_name(k); }
+
+int napmo_fock_plan_centres(void *plan, int *ncentre, double *xyz) {
+  return guarded([&] {
+    const FockPlan *p = (const FockPlan *)plan;
+    const int nc = (int)p->centres.size() / 3;
+    if (ncentre) *ncentre = nc;
+    if (xyz) std::memcpy(xyz, p->centres.data(), sizeof(double) * 3 * nc);
+  });
+}
+int napmo_grid_fock_gradient(void *plan, const double *P, double *grad_out) {
+  return guarded([&] {
+    FockPlan *p = (FockPlan *)plan;
+    hipStream_t st = p->grid.stream;
+    const int nc = (int)p->centres.size() / 3;
+    DeviceBuffer<double> dP, dG(3 * nc);
+    dP.upload(P, (size_t)p->n * p->n, st);
+    p->gradient_device(dP.get(), dG.get());
+    dG.download(grad_out, 3 * nc, st);
+    NAPMO_HIP(hipStreamSynchronize(st));
+  });
+}
+int napmo_grid_fock_gradient_device(void *plan, const double *dP, double *dG) {
+  return guarded([&] { ((FockPlan *)plan)->gradient_device(dP, dG); });
+}
+
+int napmo_apmo_plan_new(int nspecies, void **grids, void **bases, const double *charges,
+                        const int *lmax, int prec_mode, void **plan_out) {
+  return guarded([&] {
+    require_device();
+    NAPMO_REQUIRE(nspecies >= 1 && grids && bases && charges && lmax && plan_out,
+                  "napmo_apmo_plan_new: bad arguments");
+    std::vector<DeviceGrid *> g(nspecies);
+    std::vector<const BasisSet *> b(nspecies);
+    for (int s = 0; s < nspecies; ++s) {
+      g[s] = &((BeckeHandle *)grids[s])->grid;
+      b[s] = (const BasisSet *)bases[s];
+    }
+    *plan_out = new ApmoPlan(g, b, std::vector<double>(charges, charges + nspecies),
+                             std::vector<int>(lmax, lmax + nspecies), prec_mode);
+  });
+}
+void napmo_apmo_plan_del(void *plan) { delete (ApmoPlan *)plan; }
+int napmo_apmo_build(void *plan, const double **P, double **F_out, double *E_out) {
+  return guarded([&] { ((ApmoPlan *)plan)->build_host(P, F_out, E_out); });
+}
+int napmo_apmo_build_device(void *plan, const double **dP, double **dF, double *dE) {
+  return guarded([&] { ((ApmoPlan *)plan)->build_device(dP, dF, dE); });
+}
+int napmo_apmo_sync(void *plan) {
+  return guarded([&] { ((ApmoPlan *)plan)->synchronize(); });
+}
 void *napmo_device_malloc(size_t bytes) {
   void *p = nullptr;
   if (guarded([&] {

@@ -1,0 +1,168 @@
+// Grid density on FP64 MFMA: X = Phi B over a tile of points, then one of two epilogues.
+//   P form       (B = P [ld x ld]):     rho_partial[ct][p] = sum_{c in tile ct} X_pc Phi_pc
+//   orbital form (B = C [ld x ldc]):    rho_partial[ct][p] = sum_{c in tile ct} X_pc^2
+//   product      (B [ld x ldb]):        X [Npad x ldb] itself (the gradient's Phi P)
+// (napmo/wavefunction/psi_hybrid.py:144-154 and napmo/wavefunction/psi_numeric.py:361-369).
+//
+// Workgroup: 128 points x 16 NB columns, 4 waves; wave w owns points 32 w .. 32 w + 31
+// (two 16-row blocks) and all NB column blocks. K runs in stages of 16: the next stage's
+// Phi rows and B rows are loaded into registers while the MFMAs of the current stage run,
+// then stored to the other LDS buffer (one barrier per stage). LDS rows are padded so each
+// 16 x 4 fragment read of a wave touches every bank once per half.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace napmo {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+namespace {
+
+constexpr int DPT = 128;        // points per workgroup
+constexpr int DKS = 16;         // K per stage
+constexpr int ASTRIDE = DPT + 16;
+constexpr int MODE_P = 0, MODE_ORB = 1, MODE_STORE_X = 2;
+
+constexpr int bstride(int nb) { return (16 * nb) % 32 == 16 ? 16 * nb : 16 * nb + 16; }
+
+template <int NB, int MODE>
+__global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__ phi, int ld,
+                                                      long Npad, const double *__restrict__ B,
+                                                      int ldb, double *__restrict__ partial) {
+  constexpr int BS = bstride(NB);
+  __shared__ double As[2][DKS][ASTRIDE];  // As[.][k][m] = Phi[p0 + m][k0 + k]
+  __shared__ double Bs[2][DKS][BS];       // Bs[.][k][c] = B[k0 + k][c0 + c]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long p0 = (long)blockIdx.x * DPT;
+  const int c0 = blockIdx.y * 16 * NB;
+  // this thread's share of a stage: Phi row (tid / 2), 8 columns from (tid % 2) * 8;
+  // B elements tid + q * 256 (q < NB) of the 16 x 16 NB stage
+  const double *arow = phi + (p0 + (tid >> 1)) * ld + (tid & 1) * 8;
+  d2 ra[4];
+  double rb[NB];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ra[i] = *(const d2 *)(arow + k0 + 2 * i);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int e = tid + q * 256, k = e / (16 * NB), c = e % (16 * NB);
+      rb[q] = B[(long)(k0 + k) * ldb + c0 + c];
+    }
+  };
+  auto store = [&](int buf) {
+    const int m = tid >> 1, kb = (tid & 1) * 8;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      As[buf][kb + 2 * i][m] = ra[i].x;
+      As[buf][kb + 2 * i + 1][m] = ra[i].y;
+    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int e = tid + q * 256, k = e / (16 * NB), c = e % (16 * NB);
+      Bs[buf][k][c] = rb[q];
+    }
+  };
+  d4 acc[2][NB];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[a][b] = (d4){0, 0, 0, 0};
+  const int nstage = ld / DKS;
+  load(0);
+  store(0);
+  __syncthreads();
+  const int kr = lane >> 4, mi = lane & 15;
+  for (int st = 0; st < nstage; ++st) {
+    const int buf = st & 1;
+    if (st + 1 < nstage) load((st + 1) * DKS);
+#pragma unroll
+    for (int ks = 0; ks < DKS; ks += 4) {
+      double a[2], b[NB];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = As[buf][ks + kr][wave * 32 + i * 16 + mi];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) b[j] = Bs[buf][ks + kr][j * 16 + mi];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (st + 1 < nstage) store(buf ^ 1);
+    __syncthreads();
+  }
+  // f64 16x16x4 output: lane i, element r -> row (i / 16) + 4 r, column i % 16
+  if (MODE == MODE_STORE_X) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long p = p0 + wave * 32 + i * 16 + kr + 4 * r;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) partial[p * ldb + c0 + j * 16 + mi] = acc[i][j][r];
+      }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const long p = p0 + wave * 32 + i * 16 + kr + 4 * r;
+      double sum = 0.0;
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const double x = acc[i][j][r];
+        sum += MODE == MODE_ORB ? x * x : x * phi[p * ld + c0 + j * 16 + mi];
+      }
+      sum += __shfl_xor(sum, 8);
+      sum += __shfl_xor(sum, 4);
+      sum += __shfl_xor(sum, 2);
+      sum += __shfl_xor(sum, 1);
+      if (mi == 0) partial[(long)blockIdx.y * Npad + p] = sum;
+    }
+}
+
+template <int MODE>
+void launch_density_nb(int nb, dim3 grid, const double *phi, int ld, long Npad, const double *B,
+                       int ldb, double *partial, hipStream_t s) {
+  switch (nb) {
+    case 1: k_density_mfma<1, MODE><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
+    case 2: k_density_mfma<2, MODE><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
+    case 3: k_density_mfma<3, MODE><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
+    default: k_density_mfma<4, MODE><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
+  }
+  NAPMO_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void launch_density_f64(const double *phi, int ld, long Npad, const double *P, double *partial,
+                        hipStream_t s) {
+  NAPMO_REQUIRE(ld % TILE == 0 && Npad % DPT == 0, "density: padding mismatch");
+  launch_density_nb<MODE_P>(4, dim3(Npad / DPT, ld / TILE), phi, ld, Npad, P, ld, partial, s);
+}
+
+int density_orb_cols(int nocc) {
+  const int c = (int)round_up(nocc, 16);
+  return c <= 64 ? c : (int)round_up(nocc, 64);
+}
+
+int density_orb_tiles(int ldc) { return ldc <= 64 ? 1 : ldc / 64; }
+
+void launch_density_orb_f64(const double *phi, int ld, long Npad, const double *C, int ldc,
+                            double *partial, hipStream_t s) {
+  NAPMO_REQUIRE(Npad % DPT == 0 && ld % DKS == 0 && ldc % 16 == 0, "density_orb: padding");
+  const int nb = ldc <= 64 ? ldc / 16 : 4;
+  NAPMO_REQUIRE(ldc <= 64 || ldc % 64 == 0, "density_orb: orbital padding");
+  launch_density_nb<MODE_ORB>(nb, dim3(Npad / DPT, density_orb_tiles(ldc)), phi, ld, Npad, C, ldc,
+                          partial, s);
+}
+
+void launch_phi_times(const double *phi, int ld, long Npad, const double *B, int ldb, double *X,
+                      hipStream_t s) {
+  NAPMO_REQUIRE(ld % TILE == 0 && ldb % TILE == 0 && Npad % DPT == 0, "phi_times: padding");
+  launch_density_nb<MODE_STORE_X>(4, dim3(Npad / DPT, ldb / TILE), phi, ld, Npad, B, ldb, X, s);
+}
+
+}  // namespace napmo

@@ -30,6 +30,16 @@ FockPlan::FockPlan(DeviceGrid &g, const BasisSet &b, int lmax_, int prec_mode)
   FlatBasis fb = flatten(b);
   n = fb.nbasis;
   NAPMO_REQUIRE(n > 0, "empty basis");
+  std::vector<int> cof(n);
+  for (int mu = 0; mu < n; ++mu) {
+    const double *o = &fb.origin[3 * mu];
+    int c = 0;
+    const int nc = (int)centres.size() / 3;
+    while (c < nc && !(centres[3 * c] == o[0] && centres[3 * c + 1] == o[1] && centres[3 * c + 2] == o[2])) ++c;
+    if (c == nc) centres.insert(centres.end(), o, o + 3);
+    cof[mu] = c;
+  }
+  centre_of.upload(cof, grid.stream);
   ld = (int)round_up(n, TILE);
   Npad = round_up(grid.N, 256);  // multiple of every point tile (64, 256)
   hipStream_t st = grid.stream;
@@ -43,7 +53,8 @@ FockPlan::FockPlan(DeviceGrid &g, const BasisSet &b, int lmax_, int prec_mode)
     phiT32.resize((size_t)ld * Npad);
     launch_split_phi(phi64.get(), ld, Npad, phiT_hi.get(), phiT_lo.get(), phiT32.get(), st);
   }
-  grid.plan(lmax);     // Poisson plan (Green matrices) built once
+  grid.plan(lmax);     // Poisson tables (projection, Green matrices) built once per grid
+  grid.alloc_work(lmax, work);
   grid.synth_pairs();  // synthesis pair buckets built once per geometry
   P.resize((size_t)ld * ld);
   NAPMO_HIP(hipMemsetAsync(P.get(), 0, sizeof(double) * ld * ld, st));
@@ -102,24 +113,19 @@ int FockPlan::profile_read(double *avg_ms, int nphase) {
   return prof_used;
 }
 
-void FockPlan::build_device(const double *dP, double *dF, double *dE) {
+void FockPlan::density_P(int slot, const double *dP) {
   hipStream_t st = grid.stream;
-  const int slot = (prof_used < prof_cap) ? prof_used++ : -1;
-  // P (n x n) into the zero-padded ld x ld operand
-  NAPMO_HIP(hipMemcpy2DAsync(P.get(), sizeof(double) * ld, dP, sizeof(double) * n,
-                             sizeof(double) * n, n, hipMemcpyDeviceToDevice, st));
+  // P (n x n) into the zero-padded ld x ld operand (a kernel, so the build can be captured)
+  launch_pad2d(dP, n, n, P.get(), ld, ld, st);
   mark(slot, 0);
   launch_density_f64(phi64.get(), ld, Npad, P.get(), rho_part.get(), st);
   mark(slot, 1);
   launch_rho_reduce(rho_part.get(), ld / TILE, Npad, grid.N, rho.get(), st);
-  finish(slot, dP, dF, dE);
 }
 
-void FockPlan::build_orbitals_device(const double *dC, int nocc, double eta, double *dF,
-                                     double *dE) {
+void FockPlan::density_C(int slot, const double *dC, int nocc, double eta) {
   NAPMO_REQUIRE(nocc > 0 && nocc <= n, "orbital count out of range");
   hipStream_t st = grid.stream;
-  const int slot = (prof_used < prof_cap) ? prof_used++ : -1;
   const int ldc = density_orb_cols(nocc), nct = density_orb_tiles(ldc);
   Cpad.resize((size_t)ld * ldc);
   Pfull.resize((size_t)n * n);
@@ -130,28 +136,37 @@ void FockPlan::build_orbitals_device(const double *dC, int nocc, double eta, dou
   launch_density_orb_f64(phi64.get(), ld, Npad, Cpad.get(), ldc, rho_part.get(), st);
   mark(slot, 1);
   launch_rho_reduce(rho_part.get(), nct, Npad, grid.N, rho.get(), st, eta);
-  finish(slot, Pfull.get(), dF, dE);
 }
 
-void FockPlan::finish(int slot, const double *dP, double *dF, double *dE) {
+void FockPlan::expand(int slot) {
+  hipStream_t st = grid.stream;
+  PoissonPlan &pl = grid.plan(lmax);
+  AtomTables at = grid.tables(pl);
+  mark(slot, 2);
+  launch_project(at, rho.get(), grid.bw.get(), pl.yhat.get(), lmax, work.rho_lm.get(), st);
+  mark(slot, 3);
+  launch_radial(at, work.rho_lm.get(), pl.green.get(), lmax, work.Uy.get(), work.Udt.get(), st);
+  mark(slot, 4);
+  launch_synth_tables(at, work.Uy.get(), work.Udt.get(), lmax, work.T.get(), work.ampL.get(),
+                      work.ampR.get(), st);
+}
+
+void FockPlan::synth_self(int slot) {
   hipStream_t st = grid.stream;
   PoissonPlan &pl = grid.plan(lmax);
   AtomTables at = grid.tables(pl);
   SynthPairs &sp = grid.synth_pairs();
-  mark(slot, 2);
-  launch_project(at, rho.get(), grid.bw.get(), pl.yhat.get(), lmax, pl.rho_lm.get(), st);
-  mark(slot, 3);
-  launch_radial(at, pl.rho_lm.get(), pl.green.get(), lmax, pl.Uy.get(), pl.Udt.get(), st);
-  mark(slot, 4);
-  launch_synth_tables(at, pl.Uy.get(), pl.Udt.get(), lmax, pl.T.get(), pl.ampL.get(),
-                      pl.ampR.get(), st);
   mark(slot, 5);
   launch_synth2(at, grid.x.get(), grid.y.get(), grid.z.get(), grid.N, sp.list.get(),
-                sp.wave_bucket.get(), sp.nwaves, pl.T.get(), pl.ampL.get(), pl.ampR.get(), lmax,
+                sp.wave_bucket.get(), sp.nwaves, work.T.get(), work.ampL.get(), work.ampR.get(), lmax,
                 sp.contrib.get(), st);
   launch_contrib_reduce(sp.contrib.get(), grid.M, grid.N, 0, v.get(), st);
+}
+
+void FockPlan::contract(int slot, const double *dP, const double *veff, double *dF, double *dE) {
+  hipStream_t st = grid.stream;
   mark(slot, 6);
-  launch_scale3(v.get(), grid.wbw.get(), nullptr, grid.N, Npad, s.get(), st);
+  launch_scale3(veff, grid.wbw.get(), nullptr, grid.N, Npad, s.get(), st);
   mark(slot, 7);
   // the one-pass kernel writes FP32 partials into the same buffer
   float *part32 = reinterpret_cast<float *>(F_part.get());
@@ -167,8 +182,43 @@ void FockPlan::finish(int slot, const double *dP, double *dF, double *dE) {
   else
     launch_fock_reduce(fsh, F_part.get(), n, 0, 1.0, 0, dF, st);
   mark(slot, 9);
-  launch_traces(dP, dF, n, grid.wbw.get(), rho.get(), v.get(), grid.N, dE, trace_work.get(), st);
+  launch_traces(dP, dF, n, grid.wbw.get(), rho.get(), veff, grid.N, dE, trace_work.get(), st);
   mark(slot, 10);
+  built = true;
+}
+
+void FockPlan::gradient_device(const double *dP, double *dG) {
+  NAPMO_REQUIRE(built, "gradient: run a build with this P first");
+  hipStream_t st = grid.stream;
+  launch_pad2d(dP, n, n, P.get(), ld, ld, st);
+  X.resize((size_t)Npad * ld);
+  launch_phi_times(phi64.get(), ld, Npad, P.get(), ld, X.get(), st);
+  const int ncentre = (int)centres.size() / 3;
+  grad_part.resize((size_t)n * grad_chunks(grid.N) * 3);
+  // s = w bw v of the last build (contract)
+  launch_grad_centres(grid.x.get(), grid.y.get(), grid.z.get(), grid.N, s.get(), X.get(), ld, n,
+                      basis.origin.get(), basis.lxyz.get(), basis.first.get(), basis.nprim.get(),
+                      basis.norm.get(), basis.zeta.get(), basis.coef.get(), centre_of.get(),
+                      ncentre, grad_part.get(), dG, st);
+}
+
+int FockPlan::next_slot() { return (prof_used < prof_cap) ? prof_used++ : -1; }
+
+void FockPlan::build_device(const double *dP, double *dF, double *dE) {
+  const int slot = next_slot();
+  density_P(slot, dP);
+  expand(slot);
+  synth_self(slot);
+  contract(slot, dP, v.get(), dF, dE);
+}
+
+void FockPlan::build_orbitals_device(const double *dC, int nocc, double eta, double *dF,
+                                     double *dE) {
+  const int slot = next_slot();
+  density_C(slot, dC, nocc, eta);
+  expand(slot);
+  synth_self(slot);
+  contract(slot, Pfull.get(), v.get(), dF, dE);
 }
 
 void FockPlan::build_orbitals_host(const double *C, int nocc, double eta, double *F_out,

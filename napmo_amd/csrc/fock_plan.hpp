@@ -25,6 +25,7 @@ struct FockPlan {
   int n, ld, lmax, prec;
   long Npad;
   DeviceBasis basis;
+  PoissonWork work;  // this plan's own Poisson workspace (plans may share a grid)
   DeviceBuffer<double> phi64;          // [Npad x ld]
   DeviceBuffer<unsigned short> phiT_hi, phiT_lo;  // bf16 bits, function-major [ld x Npad]
   DeviceBuffer<float> phiT32;          // function-major [ld x Npad]
@@ -47,11 +48,29 @@ struct FockPlan {
   void build_orbitals_device(const double *dC, int nocc, double eta, double *dF, double *dE);
   void build_orbitals_host(const double *C, int nocc, double eta, double *F_out, double *E_out,
                            double *rho_out, double *v_out);
-  // Poisson, contraction and traces after rho is in place (P: unpadded n x n).
-  void finish(int slot, const double *dP, double *dF, double *dE);
+  // Stages of one build (slot: profiling slot or -1), all on the grid's stream:
+  // rho from P or from orbitals (density_C also forms P = eta C C^T for the trace),
+  void density_P(int slot, const double *dP);
+  void density_C(int slot, const double *dC, int nocc, double eta);
+  // Poisson expansion of rho (projection, radial Green solves, synthesis tables),
+  void expand(int slot);
+  // v = synthesis of this grid's own expansion at its own points,
+  void synth_self(int slot);
+  // F = Phi^T diag(w bw veff) Phi and the FP64 traces with veff [N] (P: unpadded n x n).
+  void contract(int slot, const double *dP, const double *veff, double *dF, double *dE);
+  int next_slot();
   void build_host(const double *P, double *F_out, double *E_out, double *rho_out, double *v_out);
   // S = Phi^T diag(w bw) Phi [n x n] from the resident Phi (FP64).
   void overlap(double *S_out);
+  // Basis centres (distinct function origins, in order of first appearance) and the
+  // centre of every function; the gradient is reported per centre.
+  std::vector<double> centres;   // [ncentre x 3]
+  DeviceBuffer<int> centre_of;   // [n]
+  DeviceBuffer<double> X, grad_part;
+  bool built = false;
+  // dE/dR_c of E = 1/2 sum w bw rho v for the last build (same P), grid fixed, each
+  // function moving with its centre: dG [ncentre x 3] (device), on the plan's stream.
+  void gradient_device(const double *dP, double *dG);
   void profile_start(int max_builds);
   // Average milliseconds per phase over the builds recorded since profile_start.
   int profile_read(double *avg_ms, int nphase);

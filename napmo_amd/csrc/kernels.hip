@@ -426,89 +426,41 @@ void launch_synth2(const AtomTables &at, const double *x, const double *y, const
 }
 
 __global__ void k_contrib_reduce(const double *__restrict__ contrib, int M, long N, int accumulate,
-                                 double *__restrict__ v) {
+                                 double scale, double *__restrict__ v) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
-  double s = accumulate ? v[p] : 0.0;
+  double s = 0.0;
   for (int A = 0; A < M; ++A) s += contrib[(long)A * N + p];
-  v[p] = s;
+  v[p] = accumulate ? v[p] + scale * s : scale * s;
 }
 
 void launch_contrib_reduce(const double *contrib, int M, long N, int accumulate, double *v,
-                           hipStream_t s) {
+                           hipStream_t s, double scale) {
   if (N <= 0) return;
-  k_contrib_reduce<<<div_up(N, 256), 256, 0, s>>>(contrib, M, N, accumulate, v);
+  k_contrib_reduce<<<div_up(N, 256), 256, 0, s>>>(contrib, M, N, accumulate, scale, v);
   NAPMO_HIP(hipGetLastError());
 }
 
-// ================================================= contractions: FP64 MFMA density
+// v[p] = a x[p] + b y[p]
+__global__ void k_axpby(double a, const double *__restrict__ x, double b,
+                        const double *__restrict__ y, long N, double *__restrict__ v) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < N) v[p] = a * x[p] + b * y[p];
+}
+
+void launch_axpby(double a, const double *x, double b, const double *y, long N, double *v,
+                  hipStream_t s) {
+  if (N <= 0) return;
+  k_axpby<<<div_up(N, 256), 256, 0, s>>>(a, x, b, y, N, v);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// ============================================ FP64 MFMA layout; density partial sums
 // v_mfma_f64_16x16x4f64 operand layout (lane i of 64), as measured on gfx950:
 //   A[16 x 4]:  lane i holds A[i % 16][i / 16]
 //   B[4 x 16]:  lane i holds B[i / 16][i % 16]
 //   D[16 x 16]: lane i holds D[(i / 16) + 4 r][i % 16] in accumulator element r = 0..3
-// Tile: 64 points x 64 columns per workgroup (4 waves of 32 x 32 = 2 x 2 MFMA blocks).
-// X = Phi P over the tile's columns; epilogue rho_partial[ct][p] = sum_cols X o Phi.
-constexpr int KS = 16;  // K elements staged in LDS per step
-
-__global__ void __launch_bounds__(256) k_density_f64(const double *__restrict__ phi, int ld,
-                                                     long Npad, const double *__restrict__ P,
-                                                     double *__restrict__ partial) {
-  __shared__ double As[KS][TILE + 2];  // As[k][m] = Phi[p0 + m][k0 + k]
-  __shared__ double Bs[KS][TILE];      // Bs[k][n] = P[k0 + k][c0 + n]
-  __shared__ double red[TILE][2];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const long p0 = (long)blockIdx.x * TILE;
-  const int c0 = blockIdx.y * TILE;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  d4 acc[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) acc[a][b] = (d4){0, 0, 0, 0};
-  for (int k0 = 0; k0 < ld; k0 += KS) {
-    // stage Phi[p0 .. p0+63][k0 .. k0+15] transposed, and P[k0 .. k0+15][c0 .. c0+63]
-    for (int e = tid; e < TILE * KS; e += 256) {
-      const int m = e / KS, k = e % KS;
-      As[k][m] = phi[(p0 + m) * ld + k0 + k];
-      const int kk = e / TILE, n = e % TILE;
-      Bs[kk][n] = P[(long)(k0 + kk) * ld + c0 + n];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < KS; ks += 4) {
-      const int kr = ks + (lane >> 4), mi = lane & 15;
-      double a0 = As[kr][wr + mi], a1 = As[kr][wr + 16 + mi];
-      double b0 = Bs[kr][wc + mi], b1 = Bs[kr][wc + 16 + mi];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-    }
-    __syncthreads();
-  }
-  // epilogue: multiply by Phi and sum over this wave's 32 columns
-  const int col16 = lane & 15, rg = lane >> 4;
-  for (int a = 0; a < 2; ++a) {
-    for (int r = 0; r < 4; ++r) {
-      const int m = wr + a * 16 + rg + 4 * r;
-      const double *prow = phi + (p0 + m) * ld + c0 + wc;
-      double sum = acc[a][0][r] * prow[col16] + acc[a][1][r] * prow[16 + col16];
-      sum += __shfl_xor(sum, 8);
-      sum += __shfl_xor(sum, 4);
-      sum += __shfl_xor(sum, 2);
-      sum += __shfl_xor(sum, 1);
-      if (col16 == 0) red[m][wave & 1] = sum;
-    }
-  }
-  __syncthreads();
-  if (tid < TILE) partial[(long)blockIdx.y * Npad + p0 + tid] = red[tid][0] + red[tid][1];
-}
-
-void launch_density_f64(const double *phi, int ld, long Npad, const double *P, double *partial,
-                        hipStream_t s) {
-  NAPMO_REQUIRE(ld % TILE == 0 && Npad % TILE == 0, "density: padding mismatch");
-  dim3 grid(Npad / TILE, ld / TILE);
-  k_density_f64<<<grid, 256, 0, s>>>(phi, ld, Npad, P, partial);
-  NAPMO_HIP(hipGetLastError());
-}
+constexpr int KS = 16;  // K elements staged in LDS per step (FP64 Fock kernel)
 
 __global__ void k_rho_reduce(const double *__restrict__ partial, int nt, long Npad, long N,
                              double scale, double *__restrict__ rho) {

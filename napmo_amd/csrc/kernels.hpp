@@ -80,13 +80,16 @@ void launch_pair_count(const AtomTables &at, const double *x, const double *y, c
                        long N, int *count, hipStream_t s);
 void launch_pair_fill(const AtomTables &at, const double *x, const double *y, const double *z,
                       long N, const long *off, int *cursor, int *list, hipStream_t s);
-// Bucketed synthesis into contrib [M x N], then v[p] (+)= sum_A contrib[A][p].
+// Bucketed synthesis into contrib [M x N], then v[p] (+)= scale * sum_A contrib[A][p].
 void launch_synth2(const AtomTables &at, const double *x, const double *y, const double *z, long N,
                    const int *list, const int *wave_bucket, long nwaves, const double *T,
                    const double *ampL, const double *ampR, int lmax, double *contrib,
                    hipStream_t s);
 void launch_contrib_reduce(const double *contrib, int M, long N, int accumulate, double *v,
-                           hipStream_t s);
+                           hipStream_t s, double scale = 1.0);
+// v = a x + b y over N points.
+void launch_axpby(double a, const double *x, double b, const double *y, long N, double *v,
+                  hipStream_t s);
 
 // rho[p] = Phi_p^T P Phi_p with FP64 MFMA (P [ld x ld] zero padded). partial [ntile x Npad].
 void launch_density_f64(const double *phi, int ld, long Npad, const double *P, double *partial,
@@ -144,6 +147,17 @@ void launch_fock_bf16x3_pass(const FockShape &sh, const unsigned short *Ahi,
 // partial[ct][p] = sum over orbital tile ct of (Phi C)_pi^2, C zero-padded [ld x ldc];
 // ldc = density_orb_cols(nocc), density_orb_tiles(ldc) partial rows.
 int density_orb_cols(int nocc);
+// X [Npad x ldb] = Phi [Npad x ld] B [ld x ldb] on FP64 MFMA (ld, ldb multiples of 64).
+void launch_phi_times(const double *phi, int ld, long Npad, const double *B, int ldb, double *X,
+                      hipStream_t s);
+// Centre gradient of 1/2 sum w bw rho v (gradient.hip): grad [ncentre x 3]; s = w bw v,
+// X = Phi P [N x ldx]; basis arrays as for collocation; partial [n x grad_chunks(N) x 3].
+int grad_chunks(long N);
+void launch_grad_centres(const double *x, const double *y, const double *z, long N, const double *s,
+                         const double *X, int ldx, int n, const double *fo, const int *fl,
+                         const int *ff, const int *fnp, const double *fnorm, const double *pz,
+                         const double *pc, const int *centre_of, int ncentre, double *partial,
+                         double *grad, hipStream_t st);
 int density_orb_tiles(int ldc);
 void launch_density_orb_f64(const double *phi, int ld, long Npad, const double *C, int ldc,
                             double *partial, hipStream_t s);

@@ -318,93 +318,6 @@ void launch_fock_bf16x3_pass(const FockShape &sh, const unsigned short *Ahi,
   NAPMO_HIP(hipGetLastError());
 }
 
-// -------------------------------------------------------- orbital-form density
-// X = Phi C with FP64 MFMA: workgroup = 256 points x 16 NB orbitals, wave w owns points
-// 64 w .. 64 w + 63 (4 blocks of 16) and all NB orbital blocks, 4 NB accumulators; every
-// Phi row is read once per orbital tile. Epilogue rho_partial[p] = sum_i X_pi^2.
-template <int NB>  // orbital blocks of 16
-__global__ void __launch_bounds__(256) k_density_orb_f64(const double *__restrict__ phi, int ld,
-                                                         long Npad, const double *__restrict__ C,
-                                                         int ldc, double *__restrict__ partial) {
-  constexpr int KS2 = 16, PT = 256;
-  __shared__ double As[KS2][PT + 4];   // As[k][m] = Phi[p0 + m][k0 + k]
-  __shared__ double Bs[KS2][16 * NB];  // Bs[k][n] = C[k0 + k][c0 + n]
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const long p0 = (long)blockIdx.x * PT;
-  const int c0 = blockIdx.y * 16 * NB;
-  d4 acc[4][NB];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < NB; ++b) acc[a][b] = (d4){0, 0, 0, 0};
-  for (int k0 = 0; k0 < ld; k0 += KS2) {
-    // row m of the point tile: 16 consecutive doubles (128 B)
-    for (int e = tid; e < PT * KS2; e += 256) {
-      const int m = e / KS2, k = e % KS2;
-      As[k][m] = phi[(p0 + m) * ld + k0 + k];
-    }
-    for (int e = tid; e < KS2 * 16 * NB; e += 256) {
-      const int k = e / (16 * NB), n = e % (16 * NB);
-      Bs[k][n] = C[(long)(k0 + k) * ldc + c0 + n];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < KS2; ks += 4) {
-      const int kr = ks + (lane >> 4), mi = lane & 15;
-      double a[4], b[NB];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[kr][wave * 64 + i * 16 + mi];
-#pragma unroll
-      for (int j = 0; j < NB; ++j) b[j] = Bs[kr][j * 16 + mi];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < NB; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
-    __syncthreads();
-  }
-  // f64 16x16x4 output: lane i, element r -> row (i / 16) + 4 r, column i % 16
-  const int rg = lane >> 4;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      double sum = 0.0;
-#pragma unroll
-      for (int j = 0; j < NB; ++j) sum += acc[i][j][r] * acc[i][j][r];
-      sum += __shfl_xor(sum, 8);
-      sum += __shfl_xor(sum, 4);
-      sum += __shfl_xor(sum, 2);
-      sum += __shfl_xor(sum, 1);
-      if ((lane & 15) == 0)
-        partial[(long)blockIdx.y * Npad + p0 + wave * 64 + i * 16 + rg + 4 * r] = sum;
-    }
-}
-
-int density_orb_cols(int nocc) { return (int)round_up(nocc, 16) <= 64 ? (int)round_up(nocc, 16) : (int)round_up(nocc, 64); }
-
-void launch_density_orb_f64(const double *phi, int ld, long Npad, const double *C, int ldc,
-                            double *partial, hipStream_t s) {
-  NAPMO_REQUIRE(Npad % 256 == 0 && ldc % 16 == 0 && ld % 16 == 0, "density_orb: padding");
-  const int nb = ldc <= 64 ? ldc / 16 : 4;
-  const int ntile = ldc / (16 * nb);
-  NAPMO_REQUIRE(ntile * 16 * nb == ldc, "density_orb: orbital padding");
-  dim3 grid(Npad / 256, ntile);
-  switch (nb) {
-    case 1: k_density_orb_f64<1><<<grid, 256, 0, s>>>(phi, ld, Npad, C, ldc, partial); break;
-    case 2: k_density_orb_f64<2><<<grid, 256, 0, s>>>(phi, ld, Npad, C, ldc, partial); break;
-    case 3: k_density_orb_f64<3><<<grid, 256, 0, s>>>(phi, ld, Npad, C, ldc, partial); break;
-    default: k_density_orb_f64<4><<<grid, 256, 0, s>>>(phi, ld, Npad, C, ldc, partial); break;
-  }
-  NAPMO_HIP(hipGetLastError());
-}
-
-int density_orb_tiles(int ldc) {
-  const int nb = ldc <= 64 ? ldc / 16 : 4;
-  return ldc / (16 * nb);
-}
-
 // P[i][j] = eta sum_k C[i][k] C[j][k] (C [n x nocc] row-major), P [n x n] row-major.
 __global__ void k_gram(const double *__restrict__ C, int n, int nocc, double eta,
                        double *__restrict__ P) {

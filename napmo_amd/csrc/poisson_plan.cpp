@@ -340,6 +340,18 @@ AtomTables DeviceGrid::tables(const PoissonPlan &plan) const {
   return t;
 }
 
+void DeviceGrid::alloc_work(int lmax, PoissonWork &w) const {
+  const int L = (lmax + 1) * (lmax + 1);
+  long nrad_total = 0;
+  for (auto &at : atoms) nrad_total += at.nrad;
+  w.rho_lm.resize((size_t)nrad_total * L);
+  w.Uy.resize((size_t)nrad_total * L);
+  w.Udt.resize((size_t)nrad_total * L);
+  w.T.resize((size_t)M * std::max(max_nrad - 1, 1) * L * 4);
+  w.ampL.resize((size_t)M * L);
+  w.ampR.resize((size_t)M * L);
+}
+
 PoissonPlan &DeviceGrid::plan(int lmax) {
   NAPMO_REQUIRE(lmax >= 0 && lmax <= MAX_LMAX,
                 "lmax must be in [0, " + std::to_string(MAX_LMAX) + "]");
@@ -391,14 +403,7 @@ PoissonPlan &DeviceGrid::plan(int lmax) {
   p->yhat_off.upload(yh_off, stream);
   p->green.upload(gr, stream);
   p->green_off.upload(g_off, stream);
-  long nrad_total = 0;
-  for (auto &at : atoms) nrad_total += at.nrad;
-  p->rho_lm.resize((size_t)nrad_total * L);
-  p->Uy.resize((size_t)nrad_total * L);
-  p->Udt.resize((size_t)nrad_total * L);
-  p->T.resize((size_t)M * (max_nrad - 1) * L * 4);
-  p->ampL.resize((size_t)M * L);
-  p->ampR.resize((size_t)M * L);
+  alloc_work(lmax, p->work);
   NAPMO_HIP(hipStreamSynchronize(stream));
   auto &ref = *p;
   plans[lmax] = std::move(p);
@@ -408,24 +413,22 @@ PoissonPlan &DeviceGrid::plan(int lmax) {
 void DeviceGrid::poisson_expand(const double *d_dens, int lmax) {
   PoissonPlan &pl = plan(lmax);
   AtomTables at = tables(pl);
-  launch_project(at, d_dens, bw.get(), pl.yhat.get(), lmax, pl.rho_lm.get(), stream);
-  launch_radial(at, pl.rho_lm.get(), pl.green.get(), lmax, pl.Uy.get(), pl.Udt.get(), stream);
+  launch_project(at, d_dens, bw.get(), pl.yhat.get(), lmax, pl.work.rho_lm.get(), stream);
+  launch_radial(at, pl.work.rho_lm.get(), pl.green.get(), lmax, pl.work.Uy.get(), pl.work.Udt.get(), stream);
 }
 
 void DeviceGrid::poisson_synthesize(int lmax, double *d_v) {
   PoissonPlan &pl = plan(lmax);
   AtomTables at = tables(pl);
   SynthPairs &sp = synth_pairs();
-  launch_synth_tables(at, pl.Uy.get(), pl.Udt.get(), lmax, pl.T.get(), pl.ampL.get(),
-                      pl.ampR.get(), stream);
+  launch_synth_tables(at, pl.work.Uy.get(), pl.work.Udt.get(), lmax, pl.work.T.get(), pl.work.ampL.get(),
+                      pl.work.ampR.get(), stream);
   launch_synth2(at, x.get(), y.get(), z.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves,
-                pl.T.get(), pl.ampL.get(), pl.ampR.get(), lmax, sp.contrib.get(), stream);
+                pl.work.T.get(), pl.work.ampL.get(), pl.work.ampR.get(), lmax, sp.contrib.get(), stream);
   launch_contrib_reduce(sp.contrib.get(), M, N, 0, d_v, stream);
 }
 
-SynthPairs &DeviceGrid::synth_pairs() {
-  if (pairs) return *pairs;
-  // bucket geometry does not depend on lmax: use any plan's tables for the atom arrays
+AtomTables DeviceGrid::geometry_tables() const {
   AtomTables at;
   at.M = M;
   at.point_off = d_point_off.get();
@@ -442,10 +445,19 @@ SynthPairs &DeviceGrid::synth_pairs() {
   at.center = d_center.get();
   at.r = d_r.get();
   at.max_nrad = max_nrad;
-  const int nb = M * (max_nrad + 1);
+  return at;
+}
+
+std::unique_ptr<SynthPairs> build_synth_pairs(const DeviceGrid &src, const double *x,
+                                              const double *y, const double *z, long N,
+                                              hipStream_t stream) {
+  // bucket geometry does not depend on lmax: only the atom arrays are needed
+  const AtomTables at = src.geometry_tables();
+  const int M = src.M;
+  const int nb = M * (src.max_nrad + 1);
   DeviceBuffer<int> count(nb), cursor(nb);
   NAPMO_HIP(hipMemsetAsync(count.get(), 0, sizeof(int) * nb, stream));
-  launch_pair_count(at, x.get(), y.get(), z.get(), N, count.get(), stream);
+  launch_pair_count(at, x, y, z, N, count.get(), stream);
   std::vector<int> hc(nb);
   count.download(hc.data(), nb, stream);
   NAPMO_HIP(hipStreamSynchronize(stream));
@@ -466,12 +478,26 @@ SynthPairs &DeviceGrid::synth_pairs() {
   DeviceBuffer<long> doff;
   doff.upload(off, stream);
   NAPMO_HIP(hipMemsetAsync(cursor.get(), 0, sizeof(int) * nb, stream));
-  launch_pair_fill(at, x.get(), y.get(), z.get(), N, doff.get(), cursor.get(), p->list.get(), stream);
+  launch_pair_fill(at, x, y, z, N, doff.get(), cursor.get(), p->list.get(), stream);
   p->wave_bucket.upload(wb, stream);
   p->contrib.resize((size_t)M * N);
   NAPMO_HIP(hipStreamSynchronize(stream));
-  pairs = std::move(p);
+  return p;
+}
+
+SynthPairs &DeviceGrid::synth_pairs() {
+  if (!pairs) pairs = build_synth_pairs(*this, x.get(), y.get(), z.get(), N, stream);
   return *pairs;
+}
+
+void DeviceGrid::synthesize_at(int lmax, const PoissonWork &w, const SynthPairs &sp,
+                               const double *tx, const double *ty, const double *tz, long tN,
+                               double scale, int accumulate, double *d_v, hipStream_t st) {
+  PoissonPlan &pl = plan(lmax);
+  AtomTables at = tables(pl);
+  launch_synth2(at, tx, ty, tz, tN, sp.list.get(), sp.wave_bucket.get(), sp.nwaves, w.T.get(),
+                w.ampL.get(), w.ampR.get(), lmax, sp.contrib.get(), st);
+  launch_contrib_reduce(sp.contrib.get(), M, tN, accumulate, d_v, st, scale);
 }
 
 }  // namespace napmo
@@ -526,18 +552,18 @@ void decomposition_on_points(CubicSpline **sp, int nspline, int lmax, const doub
   one.M = 1;
   one.max_nrad = n;
   PoissonPlan pl;
-  pl.Uy.upload(Uy, st), pl.Udt.upload(Ud, st);
-  pl.ampL.upload(aL, st), pl.ampR.upload(aR, st);
-  pl.T.resize((size_t)(n - 1) * L * 4);
+  pl.work.Uy.upload(Uy, st), pl.work.Udt.upload(Ud, st);
+  pl.work.ampL.upload(aL, st), pl.work.ampR.upload(aR, st);
+  pl.work.T.resize((size_t)(n - 1) * L * 4);
   AtomTables at = one.tables(pl);
-  launch_synth_tables(at, pl.Uy.get(), pl.Udt.get(), lmax, pl.T.get(), pl.ampL.get(), pl.ampR.get(),
+  launch_synth_tables(at, pl.work.Uy.get(), pl.work.Udt.get(), lmax, pl.work.T.get(), pl.work.ampL.get(), pl.work.ampR.get(),
                       st, 0);
   std::vector<double> hx(npoint), hy(npoint), hz(npoint);
   for (long i = 0; i < npoint; ++i)
     hx[i] = points[3 * i], hy[i] = points[3 * i + 1], hz[i] = points[3 * i + 2];
   DeviceBuffer<double> dx, dy, dz, dv;
   dx.upload(hx, st), dy.upload(hy, st), dz.upload(hz, st), dv.upload(output, npoint, st);
-  launch_synth(at, dx.get(), dy.get(), dz.get(), npoint, pl.T.get(), pl.ampL.get(), pl.ampR.get(),
+  launch_synth(at, dx.get(), dy.get(), dz.get(), npoint, pl.work.T.get(), pl.work.ampL.get(), pl.work.ampR.get(),
                lmax, 1, dv.get(), st, tail ? 0 : 1);
   dv.download(output, npoint, st);
   NAPMO_HIP(hipStreamSynchronize(st));

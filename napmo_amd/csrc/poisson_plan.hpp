@@ -41,11 +41,18 @@ struct AtomInfo {
   std::vector<double> u, aw;    // Lebedev unit vectors and weights
 };
 
+// Per-build Poisson workspace: rho_lm and U (y, dt) [sum_A nrad_A x L], synthesis tables
+// T [M x (max_nrad - 1) x L x 4] and extrapolation amplitudes [M x L].
+struct PoissonWork {
+  DeviceBuffer<double> rho_lm, Uy, Udt, T, ampL, ampR;
+};
+
+// Per-(grid, lmax) tables shared by every build on the grid, and a default workspace.
 struct PoissonPlan {
   int lmax = -1;
   DeviceBuffer<double> yhat, green;
   DeviceBuffer<long> yhat_off, green_off;
-  DeviceBuffer<double> rho_lm, Uy, Udt, T, ampL, ampR;
+  PoissonWork work;
 };
 
 // Per-geometry pair buckets of the bucketed synthesis (see launch_synth2).
@@ -77,10 +84,25 @@ struct DeviceGrid {
   void build(const std::vector<AtomInfo> &atoms);
   AtomTables tables(const PoissonPlan &plan) const;
   PoissonPlan &plan(int lmax);
+  // A private workspace sized for plan(lmax) (for plans that must not share one).
+  void alloc_work(int lmax, PoissonWork &w) const;
   // Poisson of dens (device [N]) into the plan's U / tables; then synthesis to v (device).
   void poisson_expand(const double *d_dens, int lmax);
   void poisson_synthesize(int lmax, double *d_v);
   SynthPairs &synth_pairs();
+  // Atom arrays without a Poisson plan (enough for bucketing pairs).
+  AtomTables geometry_tables() const;
+  // Synthesis of this grid's expansion (plan(lmax), after poisson_expand / expand) at the
+  // target points of sp: v (+)= scale * sum_A contrib[A], on stream st.
+  void synthesize_at(int lmax, const PoissonWork &w, const SynthPairs &sp, const double *tx,
+                     const double *ty, const double *tz, long tN, double scale, int accumulate,
+                     double *d_v, hipStream_t st);
 };
+
+// (point, centre) pair buckets of src's centres and radial grids at the target points
+// x, y, z [N] (device): the synthesis lists of src's expansion evaluated at those points.
+std::unique_ptr<SynthPairs> build_synth_pairs(const DeviceGrid &src, const double *x,
+                                              const double *y, const double *z, long N,
+                                              hipStream_t stream);
 
 }  // namespace napmo

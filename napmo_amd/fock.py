@@ -83,6 +83,25 @@ class GridFock(object):
         cext.check(cext.napmo_grid_overlap(self._this, S))
         return S
 
+    def centres(self):
+        """Basis centres [ncentre x 3] (distinct function origins, first-appearance order)."""
+        nc = ctypes.c_int()
+        cext.check(cext.napmo_fock_plan_centres(self._this, ctypes.byref(nc), None))
+        xyz = np.zeros((nc.value, 3))
+        cext.check(cext.napmo_fock_plan_centres(self._this, ctypes.byref(nc), xyz.ctypes.data))
+        return xyz
+
+    def gradient(self, P):
+        """dE/dR_c [ncentre x 3] of the last build's E = 1/2 sum w bw rho v (same P),
+        grid fixed, basis functions moving with their centres."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        g = np.zeros((len(self.centres()), 3))
+        cext.check(cext.napmo_grid_fock_gradient(self._this, P.ctypes.data, g.ctypes.data))
+        return g
+
+    def gradient_device(self, dP, dG):
+        cext.check(cext.napmo_grid_fock_gradient_device(self._this, dP, dG))
+
     def profile(self, max_builds):
         cext.check(cext.napmo_fock_plan_profile(self._this, max_builds))
 

@@ -140,6 +140,14 @@ _SIGNATURES = {
     'napmo_fock_plan_info': (c_int, [vp, ctypes.POINTER(c_long), ctypes.POINTER(c_int),
                                      ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
     'napmo_grid_fock_build': (c_int, [vp, vp, vp, vp, vp, vp]),
+    'napmo_fock_plan_centres': (c_int, [vp, ctypes.POINTER(c_int), vp]),
+    'napmo_grid_fock_gradient': (c_int, [vp, vp, vp]),
+    'napmo_grid_fock_gradient_device': (c_int, [vp, vp, vp]),
+    'napmo_apmo_plan_new': (c_int, [c_int, vp, vp, vp, vp, c_int, ctypes.POINTER(vp)]),
+    'napmo_apmo_plan_del': (None, [vp]),
+    'napmo_apmo_build': (c_int, [vp, vp, vp, vp]),
+    'napmo_apmo_build_device': (c_int, [vp, vp, vp, vp]),
+    'napmo_apmo_sync': (c_int, [vp]),
     'napmo_grid_fock_build_device': (c_int, [vp, vp, vp, vp]),
     'napmo_grid_fock_build_orbitals': (c_int, [vp, vp, c_int, c_double, vp, vp, vp, vp]),
     'napmo_grid_fock_build_orbitals_device': (c_int, [vp, vp, c_int, c_double, vp, vp]),

@@ -61,6 +61,41 @@ def chon16_system(seed=2, nrad=50, nang=590, natom=16, ncycle=30):
     return atoms, shells, nrad, nang, 12
 
 
+def even_tempered_shells(nexp=8, zeta0=4.0, ratio=10.0 ** (1.0 / 7.0), lmax_shell=2):
+    """Nuclear basis of one quantum proton: zeta_k = zeta0 ratio^k (k < nexp), one
+    uncontracted shell per (k, l <= lmax_shell); 8 x (1 + 3 + 6) = 80 functions."""
+    return [{'l': l, 'prim': [float(zeta0 * ratio ** k)], 'cont': [1.0]}
+            for k in range(nexp) for l in range(lmax_shell + 1)]
+
+
+def apmo_system(seed=3, nheavy=4, nh=4, box=12.0, ncycle=12):
+    """Electrons + quantum protons: ``nheavy`` atoms from {C, N, O} and ``nh`` H atoms in a
+    ``box`` bohr cube (min separation 1.8). Electrons: one 100 x 302 grid per atom
+    (241,600 points), ``ncycle`` cycles of s/p/d random shells over all atoms (120
+    functions). Protons (species H_1, charge +1, eta 1): one 60 x 194 grid per H (46,560
+    points) and the even-tempered 80-function basis on each (320). lmax = half the
+    Lebedev degree (14 and 11). Returns a dict of both species."""
+    rng = np.random.default_rng(seed)
+    heavy = [('C', 'N', 'O')[k] for k in rng.integers(0, 3, nheavy)]
+    syms = heavy + ['H'] * nh
+    while True:
+        pos = rng.uniform(0.0, box, (len(syms), 3))
+        d = np.sqrt(((pos[:, None, :] - pos[None, :, :]) ** 2).sum(-1))
+        d[np.diag_indices(len(syms))] = np.inf
+        if d.min() >= 1.8:
+            break
+    atoms = list(zip(syms, [tuple(float(c) for c in p) for p in pos]))
+    e_shells = random_shells(rng, len(atoms), ncycle)
+    protons = [a for a in atoms if a[0] == 'H']
+    n_shells = [even_tempered_shells() for _ in protons]
+    nelec = sum({'H': 1, 'C': 6, 'N': 7, 'O': 8}[s] for s in syms)
+    return {'atoms': atoms, 'electron': {'centres': atoms, 'shells': e_shells, 'nrad': 100,
+                                         'nang': 302, 'lmax': 14, 'charge': -1.0, 'eta': 2.0,
+                                         'nocc': nelec // 2},
+            'proton': {'centres': protons, 'shells': n_shells, 'nrad': 60, 'nang': 194, 'lmax': 11,
+                       'charge': 1.0, 'eta': 1.0, 'nocc': len(protons)}}
+
+
 def occupied_orbitals(rng, S, nocc):
     """C = C0 (C0^T S C0)^(-1/2), C0 ~ N(0, 1): nocc S-orthonormal orbitals [n x nocc]."""
     C0 = rng.normal(size=(S.shape[0], nocc))

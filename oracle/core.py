@@ -777,6 +777,49 @@ def compute_coulomb(grid, dens, lmax, linsolve=None):
     return J
 
 
+def coulomb_at(grid, dens, lmax, points, linsolve=None):
+    """Potential of ``dens`` (given on ``grid``) at arbitrary points: the Becke-Poisson
+    expansion of ``grid``'s atoms synthesised at ``points``. The reference evaluates the
+    other species' compute_coulomb on the grid both species share
+    (napmo/wavefunction/psi_hybrid.py:81-110, napmo/solver/hf_solver.py:95-98); with one
+    grid per species the same expansion is synthesised at the other species' points."""
+    U = poisson_solver(grid, dens, lmax, linsolve)
+    J = np.zeros(len(points))
+    for k in range(grid.ncenter):
+        J += eval_decomposition(U[k], grid.origin[k], points)
+    return J
+
+
+def apmo_coulomb_build(species, evaluate=None, linsolve=None):
+    """Coulomb part of the multi-species (APMO) Fock build. ``species``: list of dicts
+    with grid, phi [N_s, n_s], P, charge, lmax. Per species s:
+
+        v_eff = q_s^2 V[rho_s] + q_s sum_{t != s} q_t V[rho_t]   (both at s's points)
+        F_s = Phi_s^T diag(w bw v_eff) Phi_s   (= q_s^2 J_self + the coupling matrix of
+              napmo/wavefunction/psi_numeric.py:217-241 / psi_hybrid.py:81-110)
+
+    Returns per species (F_s, E_trace = 1/2 sum P o F, E_grid = 1/2 sum w bw rho v_eff,
+    E_self = 1/2 q^2 sum w bw rho V[rho_s]). ``evaluate(grid, dens, lmax, points)`` may
+    replace coulomb_at (e.g. with the C oracle)."""
+    evaluate = evaluate or (lambda g, d, l, pts: coulomb_at(g, d, l, pts, linsolve))
+    rhos = [density_from_dm(s['phi'], s['P']) for s in species]
+    out = []
+    for a, sa in enumerate(species):
+        g = sa['grid']
+        q = sa['charge']
+        v_self = evaluate(g, rhos[a], sa['lmax'], g.points)
+        v_cross = np.zeros(g.size)
+        for b, sb in enumerate(species):
+            if b != a:
+                v_cross += sb['charge'] * evaluate(sb['grid'], rhos[b], sb['lmax'], g.points)
+        v_eff = q * q * v_self + q * v_cross
+        F = coupling_matrix(sa['phi'], v_eff, g.becke_weights, g.weights)
+        wbw = g.becke_weights * g.weights
+        out.append((F, 0.5 * float(np.sum(sa['P'] * F)), 0.5 * float(np.sum(wbw * rhos[a] * v_eff)),
+                    0.5 * q * q * float(np.sum(wbw * rhos[a] * v_self))))
+    return out
+
+
 # ----------------------------------------------------------------------------------
 # Gaussian basis (src/system/gto.cpp, src/ints/overlap.cpp, napmo/gto/basis_set.py)
 # ----------------------------------------------------------------------------------
@@ -859,6 +902,51 @@ def basis_compute(basis, points):
             acc += c * ang * np.exp(-z * r2)
         out[:, mu] = acc * norm
     return out
+
+
+def basis_gradient(basis, points):
+    """grad_r Phi [3, N, n]: for phi = N_c x^a y^b z^c sum_k c_k e^{-zeta_k r^2},
+    d phi/dx = N_c y^b z^c (a x^{a-1} g + x^{a+1} h), g = sum_k c_k e_k,
+    h = -2 sum_k c_k zeta_k e_k (differentiation of src/system/gto.cpp:41-51)."""
+    out = np.empty((3, points.shape[0], basis.nbasis))
+    for mu, (o, lxyz, zetas, cn, norm) in enumerate(basis.functions):
+        d = points - o
+        r2 = (d * d).sum(1)
+        g = np.zeros(len(d))
+        h = np.zeros(len(d))
+        for z, c in zip(zetas, cn):
+            e = c * np.exp(-z * r2)
+            g += e
+            h -= 2.0 * z * e
+        pw = [np.power(d[:, k], lxyz[k]) for k in range(3)]
+        for k in range(3):
+            a = lxyz[k]
+            dk = (a * np.power(d[:, k], a - 1) if a else 0.0) * g + pw[k] * d[:, k] * h
+            others = pw[(k + 1) % 3] * pw[(k + 2) % 3]
+            out[k, :, mu] = norm * dk * others
+    return out
+
+
+def centre_gradient(basis, points, P, s):
+    """dE/dR_c [ncentre, 3] of E = 1/2 sum_p s_p rho_p / ... with s = w bw v held fixed:
+    sum_p s_p d rho_p/dR_c = -2 sum_p s_p sum_{mu on c} grad phi_mu(p) (Phi P)_p mu; centres
+    are the distinct function origins in first-appearance order."""
+    centres, cof = [], []
+    for o, *_ in basis.functions:
+        for c, x in enumerate(centres):
+            if np.array_equal(x, o):
+                cof.append(c)
+                break
+        else:
+            centres.append(o)
+            cof.append(len(centres) - 1)
+    X = basis_compute(basis, points) @ P
+    dphi = basis_gradient(basis, points)
+    per_fn = np.einsum('p,kpm,pm->mk', s, dphi, X)
+    grad = np.zeros((len(centres), 3))
+    for mu, c in enumerate(cof):
+        grad[c] += per_fn[mu]
+    return -2.0 * grad, np.array(centres)
 
 
 # ----------------------------------------------------------------------------------

@@ -10,7 +10,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 from napmo_amd.workloads import (BASIS, H2_ATOMS, chon16_system, density_matrix, h2_system,  # noqa: F401,E402
-                                named_shells, random_molecule, random_shells)
+                                named_shells, occupied_orbitals, random_molecule, random_shells)
 
 
 def oracle_system(atoms, shells, nrad, nang, sizes=None):
