@@ -236,6 +236,29 @@ int napmo_fock_plan_centres(void *plan, int *ncentre, double *xyz);
 int napmo_grid_fock_gradient(void *plan, const double *P, double *grad_out);
 int napmo_grid_fock_gradient_device(void *plan, const double *dP, double *dG);
 
+/* Launch-shape knobs of a plan, for the autotuner (no reference counterpart). Set them
+ * before a batch is captured; not while a build is running. */
+enum {
+  NAPMO_TUNE_FOCK_WG = 0,  /* workgroups of the per-tile Fock kernels (chunks x tiles); 0 = default */
+  NAPMO_TUNE_PASS_WG = 1,  /* workgroups of the one-pass BF16x3 Fock kernel; 0 = default */
+  NAPMO_TUNE_USE_PASS = 2  /* 1: one-pass BF16x3 kernel, 0: per-tile BF16x3 kernel */
+};
+int napmo_fock_plan_set_tuning(void *plan, int key, long value);
+int napmo_fock_plan_get_tuning(void *plan, int key, long *value);
+
+/* Batched builds of independent molecules (no reference counterpart; the reference
+ * runs molecules one at a time). Each plan keeps its own grid, basis and buffers; the
+ * batch spreads the plans over nlanes HIP streams forked from and joined to one origin
+ * stream, and can capture one build of every plan into a HIP graph: after
+ * napmo_batch_capture, napmo_batch_replay re-runs all builds on the same device pointers
+ * (write the next P into them between replays). The plans must outlive the batch. */
+int napmo_batch_plan_new(int nplan, void **plans, int nlanes, void **batch_out);
+void napmo_batch_plan_del(void *batch);
+int napmo_batch_build(void *batch, const double **dP, double **dF, double **dE);
+int napmo_batch_capture(void *batch, const double **dP, double **dF, double **dE);
+int napmo_batch_replay(void *batch);
+int napmo_batch_sync(void *batch);
+
 /* Multi-species (APMO) Coulomb build: species s (grid s, basis s, charge q_s, lmax s), one
  * Becke grid per species (two species may pass the same grid). Per build, with
  * rho_s from P_s:

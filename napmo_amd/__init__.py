@@ -19,3 +19,4 @@ from napmo_amd.system.elements import AtomicElementsDatabase  # noqa: F401
 from napmo_amd.system.species import make_basis, make_species  # noqa: F401
 from napmo_amd.fock import GridFock  # noqa: F401
 from napmo_amd.apmo import APMOFock  # noqa: F401
+from napmo_amd.batch import BatchFock  # noqa: F401

@@ -1,0 +1,106 @@
+"""Launch-shape autotuner for grid Fock plans.
+
+For a plan (one molecule's grid, basis and precision) it times every candidate launch
+shape of the Fock contraction on the device (the per-tile FP64 / BF16x3 kernels' workgroup
+count, the one-pass BF16x3 kernel's workgroup count, and the choice between the two
+BF16x3 kernels), keeps the fastest, and stores it in a JSON cache keyed by the shape class
+(precision, padded basis size, point count rounded to a power of two), so later plans of
+the same class start from the cached choice without timing. The cache lives at
+$NAPMO_TUNING_CACHE or ~/.cache/napmo_amd/tuning.json.
+"""
+import ctypes
+import json
+import math
+import os
+import time
+
+from napmo_amd.system import cext
+
+TUNE_FOCK_WG, TUNE_PASS_WG, TUNE_USE_PASS = 0, 1, 2
+_KEYS = {'fock_wg': TUNE_FOCK_WG, 'pass_wg': TUNE_PASS_WG, 'use_pass': TUNE_USE_PASS}
+DEFAULTS = {'fock_wg': 0, 'pass_wg': 0}   # 0: the library's default shape
+
+
+def cache_path():
+    return os.environ.get('NAPMO_TUNING_CACHE') or os.path.join(
+        os.path.expanduser('~'), '.cache', 'napmo_amd', 'tuning.json')
+
+
+def shape_key(fock):
+    ld = -(-fock.nbasis // 64) * 64
+    nb = 1 << max(0, int(round(math.log2(max(fock.npoint, 1)))))
+    return '%s:ld%d:N%d' % (fock.precision, ld, nb)
+
+
+def candidates(fock):
+    if fock.precision == 'bf16x3':
+        out = [{'use_pass': 1, 'pass_wg': w} for w in (128, 256, 512, 1024)]
+        out += [{'use_pass': 0, 'fock_wg': w} for w in (1024, 2048, 4096)]
+        return out
+    return [{'fock_wg': w} for w in (512, 1024, 2048, 4096, 8192)]
+
+
+def apply(fock, setting):
+    for k, v in setting.items():
+        if k in _KEYS:
+            cext.check(cext.napmo_fock_plan_set_tuning(fock._this, _KEYS[k], int(v)))
+
+
+def current(fock):
+    out = {}
+    for k, code in _KEYS.items():
+        v = ctypes.c_long()
+        cext.check(cext.napmo_fock_plan_get_tuning(fock._this, code, ctypes.byref(v)))
+        out[k] = v.value
+    return out
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save(path, cache):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    tmp = path + '.tmp'
+    with open(tmp, 'w') as f:
+        json.dump(cache, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def tune(fock, dP, dF, dE, reps=3, path=None, force=False):
+    """Choose and apply the fastest launch shape for ``fock`` (device buffers dP, dF, dE
+    of the right sizes are used for the timing builds). Returns (setting, source) with
+    source 'cache' or 'timed'."""
+    path = path or cache_path()
+    cache = _load(path)
+    key = shape_key(fock)
+    if key in cache and not force:
+        setting = {k: v for k, v in cache[key].items() if k in _KEYS}
+        try:
+            apply(fock, dict(DEFAULTS, **setting))
+            return setting, 'cache'
+        except cext.NapmoError:
+            pass   # not applicable to this plan: time again
+    best = None
+    for cand in candidates(fock):
+        try:
+            apply(fock, dict(DEFAULTS, **cand))
+        except cext.NapmoError:
+            continue
+        fock.build_device(dP, dF, dE)
+        fock.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fock.build_device(dP, dF, dE)
+        fock.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        if best is None or t < best[1]:
+            best = (cand, t)
+    apply(fock, dict(DEFAULTS, **best[0]))
+    cache[key] = dict(best[0], ms=best[1] * 1e3)
+    _save(path, cache)
+    return best[0], 'timed'

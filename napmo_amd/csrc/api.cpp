@@ -10,6 +10,7 @@
 
 #include "common.hpp"
 #include "apmo_plan.hpp"
+#include "batch_plan.hpp"
 #include "fock_plan.hpp"
 #include "host.hpp"
 #include "kernels.hpp"
@@ -590,7 +591,7 @@ int napmo_fock_plan_centres(void *plan, int *ncentre, double *xyz) {
 int napmo_grid_fock_gradient(void *plan, const double *P, double *grad_out) {
   return guarded([&] {
     FockPlan *p = (FockPlan *)plan;
-    hipStream_t st = p->grid.stream;
+    hipStream_t st = p->stream;
     const int nc = (int)p->centres.size() / 3;
     DeviceBuffer<double> dP, dG(3 * nc);
     dP.upload(P, (size_t)p->n * p->n, st);
@@ -601,6 +602,35 @@ int napmo_grid_fock_gradient(void *plan, const double *P, double *grad_out) {
 }
 int napmo_grid_fock_gradient_device(void *plan, const double *dP, double *dG) {
   return guarded([&] { ((FockPlan *)plan)->gradient_device(dP, dG); });
+}
+
+int napmo_fock_plan_set_tuning(void *plan, int key, long value) {
+  return guarded([&] { ((FockPlan *)plan)->set_tuning(key, value); });
+}
+int napmo_fock_plan_get_tuning(void *plan, int key, long *value) {
+  return guarded([&] { *value = ((FockPlan *)plan)->get_tuning(key); });
+}
+
+int napmo_batch_plan_new(int nplan, void **plans, int nlanes, void **batch_out) {
+  return guarded([&] {
+    NAPMO_REQUIRE(nplan >= 1 && plans && batch_out, "napmo_batch_plan_new: bad arguments");
+    std::vector<FockPlan *> v(nplan);
+    for (int m = 0; m < nplan; ++m) v[m] = (FockPlan *)plans[m];
+    *batch_out = new BatchPlan(v, nlanes);
+  });
+}
+void napmo_batch_plan_del(void *batch) { delete (BatchPlan *)batch; }
+int napmo_batch_build(void *batch, const double **dP, double **dF, double **dE) {
+  return guarded([&] { ((BatchPlan *)batch)->build(dP, dF, dE); });
+}
+int napmo_batch_capture(void *batch, const double **dP, double **dF, double **dE) {
+  return guarded([&] { ((BatchPlan *)batch)->capture(dP, dF, dE); });
+}
+int napmo_batch_replay(void *batch) {
+  return guarded([&] { ((BatchPlan *)batch)->replay(); });
+}
+int napmo_batch_sync(void *batch) {
+  return guarded([&] { ((BatchPlan *)batch)->synchronize(); });
 }
 
 int napmo_apmo_plan_new(int nspecies, void **grids, void **bases, const double *charges,
@@ -648,9 +678,9 @@ int napmo_memcpy(void *dst, const void *src, size_t bytes, int kind) {
   });
 }
 
-void *napmo_fock_plan_stream(void *plan) { return (void *)((FockPlan *)plan)->grid.stream; }
+void *napmo_fock_plan_stream(void *plan) { return (void *)((FockPlan *)plan)->stream; }
 int napmo_fock_plan_sync(void *plan) {
-  return guarded([&] { NAPMO_HIP(hipStreamSynchronize(((FockPlan *)plan)->grid.stream)); });
+  return guarded([&] { NAPMO_HIP(hipStreamSynchronize(((FockPlan *)plan)->stream)); });
 }
 
 }  // extern "C"

@@ -57,12 +57,12 @@ void ApmoPlan::build_device(const double *const *dP, double *const *dF, double *
     p.density_P(-1, dP[s]);
     p.expand(-1);
     p.synth_self(-1);
-    NAPMO_HIP(hipEventRecord(sp[s].expanded, p.grid.stream));
+    NAPMO_HIP(hipEventRecord(sp[s].expanded, p.stream));
   }
   for (int s = 0; s < ns; ++s) {
     FockPlan &p = *sp[s].plan;
     DeviceGrid &gs = p.grid;
-    hipStream_t st = gs.stream;
+    hipStream_t st = p.stream;
     const double q = sp[s].charge;
     // v_cross = sum_{t != s} q_t V[rho_t] at s's points
     int first = 1;
@@ -95,7 +95,7 @@ void ApmoPlan::build_device(const double *const *dP, double *const *dF, double *
   // potential or expansion another species is still reading
   for (int s = 0; s < ns; ++s)
     for (int t = 0; t < ns; ++t)
-      if (t != s) NAPMO_HIP(hipStreamWaitEvent(sp[s].plan->grid.stream, sp[t].done, 0));
+      if (t != s) NAPMO_HIP(hipStreamWaitEvent(sp[s].plan->stream, sp[t].done, 0));
 }
 
 void ApmoPlan::build_host(const double *const *P, double *const *F_out, double *E_out) {
@@ -105,7 +105,7 @@ void ApmoPlan::build_host(const double *const *P, double *const *F_out, double *
   std::vector<double *> pF(ns);
   for (int s = 0; s < ns; ++s) {
     FockPlan &p = *sp[s].plan;
-    dP[s].upload(P[s], (size_t)p.n * p.n, p.grid.stream);
+    dP[s].upload(P[s], (size_t)p.n * p.n, p.stream);
     pP[s] = dP[s].get();
     pF[s] = p.F.get();
   }
@@ -113,14 +113,14 @@ void ApmoPlan::build_host(const double *const *P, double *const *F_out, double *
   build_device(pP.data(), pF.data(), E.get());
   for (int s = 0; s < ns; ++s) {
     FockPlan &p = *sp[s].plan;
-    if (F_out && F_out[s]) p.F.download(F_out[s], (size_t)p.n * p.n, p.grid.stream);
+    if (F_out && F_out[s]) p.F.download(F_out[s], (size_t)p.n * p.n, p.stream);
   }
-  if (E_out) E.download(E_out, 4 * ns, sp[ns - 1].plan->grid.stream);
+  if (E_out) E.download(E_out, 4 * ns, sp[ns - 1].plan->stream);
   synchronize();
 }
 
 void ApmoPlan::synchronize() {
-  for (auto &S : sp) NAPMO_HIP(hipStreamSynchronize(S.plan->grid.stream));
+  for (auto &S : sp) NAPMO_HIP(hipStreamSynchronize(S.plan->stream));
 }
 
 }  // namespace napmo

@@ -1,0 +1,33 @@
+// Many independent molecules (one FockPlan each) built together: the plans' launches are
+// spread over a few HIP streams (fork / join with events on one origin stream) and the
+// whole batch can be captured once into a HIP graph and replayed per SCF iteration.
+#pragma once
+
+#include <vector>
+
+#include "fock_plan.hpp"
+
+namespace napmo {
+
+struct BatchPlan {
+  std::vector<FockPlan *> plans;       // borrowed; must outlive the batch
+  std::vector<hipStream_t> lanes;      // plan m runs on lanes[m % nlanes]
+  hipStream_t origin = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<hipStream_t> saved;      // the plans' own streams, restored on destruction
+
+  BatchPlan(const std::vector<FockPlan *> &plans, int nlanes);
+  ~BatchPlan();
+  // One build of every plan: dP[m], dF[m] [n_m x n_m], dE[m] [2] (device pointers);
+  // asynchronous on the origin stream.
+  void build(const double *const *dP, double *const *dF, double *const *dE);
+  // Records the same batch into a graph (pointers baked in); replay() launches it.
+  void capture(const double *const *dP, double *const *dF, double *const *dE);
+  void replay();
+  void synchronize();
+};
+
+}  // namespace napmo

@@ -43,6 +43,7 @@ FockPlan::FockPlan(DeviceGrid &g, const BasisSet &b, int lmax_, int prec_mode)
   ld = (int)round_up(n, TILE);
   Npad = round_up(grid.N, 256);  // multiple of every point tile (64, 256)
   hipStream_t st = grid.stream;
+  stream = st;
   basis.upload(fb, st);
   phi64.resize((size_t)Npad * ld);
   basis.collocate(grid.x.get(), grid.y.get(), grid.z.get(), grid.N, Npad, ld, phi64.get(), nullptr,
@@ -62,14 +63,43 @@ FockPlan::FockPlan(DeviceGrid &g, const BasisSet &b, int lmax_, int prec_mode)
   rho.resize(grid.N);
   v.resize(grid.N);
   s.resize(Npad);
-  fsh = fock_shape(ld, Npad, true);
-  psh = fock_pass_shape(ld, Npad);
   use_pass = prec == NAPMO_PREC_BF16X3 && fock_pass_supported(ld);
-  F_part.resize((size_t)std::max(fsh.nchunk, use_pass ? psh.nchunk : 0) * fsh.cstride());
+  reshape();
   F.resize((size_t)n * n);
   E.resize(2);
   trace_work.resize(2 * TRACE_WORK);
   NAPMO_HIP(hipStreamSynchronize(st));
+}
+
+void FockPlan::reshape() {
+  fsh = fock_shape(ld, Npad, true, fock_wg);
+  psh = fock_pass_shape(ld, Npad, pass_wg);
+  F_part.resize((size_t)std::max(fsh.nchunk, use_pass ? psh.nchunk : 0) * fsh.cstride());
+}
+
+void FockPlan::set_tuning(int key, long value) {
+  NAPMO_REQUIRE(value >= 0, "tuning value must be non-negative");
+  switch (key) {
+    case NAPMO_TUNE_FOCK_WG: fock_wg = value; break;
+    case NAPMO_TUNE_PASS_WG: pass_wg = value; break;
+    case NAPMO_TUNE_USE_PASS:
+      NAPMO_REQUIRE(!value || (prec == NAPMO_PREC_BF16X3 && fock_pass_supported(ld)),
+                    "one-pass kernel not available for this plan");
+      use_pass = value != 0;
+      break;
+    default: throw Error("unknown tuning key " + std::to_string(key));
+  }
+  NAPMO_HIP(hipStreamSynchronize(stream));
+  reshape();
+}
+
+long FockPlan::get_tuning(int key) const {
+  switch (key) {
+    case NAPMO_TUNE_FOCK_WG: return (long)fsh.nchunk * fsh.ntiles;
+    case NAPMO_TUNE_PASS_WG: return (long)psh.nchunk;
+    case NAPMO_TUNE_USE_PASS: return use_pass ? 1 : 0;
+    default: throw Error("unknown tuning key " + std::to_string(key));
+  }
 }
 
 FockPlan::~FockPlan() {
@@ -85,7 +115,7 @@ const char *FockPlan::phase_name(int k) {
 
 void FockPlan::mark(int slot, int phase) {
   if (slot < 0) return;
-  NAPMO_HIP(hipEventRecord(prof_events[(size_t)slot * (NPHASE + 1) + phase], grid.stream));
+  NAPMO_HIP(hipEventRecord(prof_events[(size_t)slot * (NPHASE + 1) + phase], stream));
 }
 
 void FockPlan::profile_start(int max_builds) {
@@ -98,7 +128,7 @@ void FockPlan::profile_start(int max_builds) {
 }
 
 int FockPlan::profile_read(double *avg_ms, int nphase) {
-  NAPMO_HIP(hipStreamSynchronize(grid.stream));
+  NAPMO_HIP(hipStreamSynchronize(stream));
   const int k = std::min(nphase, NPHASE);
   for (int p = 0; p < k; ++p) avg_ms[p] = 0.0;
   for (int b = 0; b < prof_used; ++b)
@@ -114,7 +144,7 @@ int FockPlan::profile_read(double *avg_ms, int nphase) {
 }
 
 void FockPlan::density_P(int slot, const double *dP) {
-  hipStream_t st = grid.stream;
+  hipStream_t st = stream;
   // P (n x n) into the zero-padded ld x ld operand (a kernel, so the build can be captured)
   launch_pad2d(dP, n, n, P.get(), ld, ld, st);
   mark(slot, 0);
@@ -125,7 +155,7 @@ void FockPlan::density_P(int slot, const double *dP) {
 
 void FockPlan::density_C(int slot, const double *dC, int nocc, double eta) {
   NAPMO_REQUIRE(nocc > 0 && nocc <= n, "orbital count out of range");
-  hipStream_t st = grid.stream;
+  hipStream_t st = stream;
   const int ldc = density_orb_cols(nocc), nct = density_orb_tiles(ldc);
   Cpad.resize((size_t)ld * ldc);
   Pfull.resize((size_t)n * n);
@@ -139,7 +169,7 @@ void FockPlan::density_C(int slot, const double *dC, int nocc, double eta) {
 }
 
 void FockPlan::expand(int slot) {
-  hipStream_t st = grid.stream;
+  hipStream_t st = stream;
   PoissonPlan &pl = grid.plan(lmax);
   AtomTables at = grid.tables(pl);
   mark(slot, 2);
@@ -152,7 +182,7 @@ void FockPlan::expand(int slot) {
 }
 
 void FockPlan::synth_self(int slot) {
-  hipStream_t st = grid.stream;
+  hipStream_t st = stream;
   PoissonPlan &pl = grid.plan(lmax);
   AtomTables at = grid.tables(pl);
   SynthPairs &sp = grid.synth_pairs();
@@ -164,7 +194,7 @@ void FockPlan::synth_self(int slot) {
 }
 
 void FockPlan::contract(int slot, const double *dP, const double *veff, double *dF, double *dE) {
-  hipStream_t st = grid.stream;
+  hipStream_t st = stream;
   mark(slot, 6);
   launch_scale3(veff, grid.wbw.get(), nullptr, grid.N, Npad, s.get(), st);
   mark(slot, 7);
@@ -189,7 +219,7 @@ void FockPlan::contract(int slot, const double *dP, const double *veff, double *
 
 void FockPlan::gradient_device(const double *dP, double *dG) {
   NAPMO_REQUIRE(built, "gradient: run a build with this P first");
-  hipStream_t st = grid.stream;
+  hipStream_t st = stream;
   launch_pad2d(dP, n, n, P.get(), ld, ld, st);
   X.resize((size_t)Npad * ld);
   launch_phi_times(phi64.get(), ld, Npad, P.get(), ld, X.get(), st);
@@ -223,7 +253,7 @@ void FockPlan::build_orbitals_device(const double *dC, int nocc, double eta, dou
 
 void FockPlan::build_orbitals_host(const double *C, int nocc, double eta, double *F_out,
                                    double *E_out, double *rho_out, double *v_out) {
-  hipStream_t st = grid.stream;
+  hipStream_t st = stream;
   DeviceBuffer<double> dC;
   dC.upload(C, (size_t)n * nocc, st);
   build_orbitals_device(dC.get(), nocc, eta, F.get(), E.get());
@@ -235,7 +265,7 @@ void FockPlan::build_orbitals_host(const double *C, int nocc, double eta, double
 }
 
 void FockPlan::overlap(double *S_out) {
-  hipStream_t st = grid.stream;
+  hipStream_t st = stream;
   launch_scale3(grid.w.get(), grid.bw.get(), nullptr, grid.N, Npad, s.get(), st);
   launch_fock_f64(fsh, phi64.get(), phi64.get(), s.get(), F_part.get(), st);
   launch_fock_reduce(fsh, F_part.get(), n, 0, 1.0, 0, F.get(), st);
@@ -245,7 +275,7 @@ void FockPlan::overlap(double *S_out) {
 
 void FockPlan::build_host(const double *hP, double *F_out, double *E_out, double *rho_out,
                           double *v_out) {
-  hipStream_t st = grid.stream;
+  hipStream_t st = stream;
   DeviceBuffer<double> dP;
   dP.upload(hP, (size_t)n * n, st);
   build_device(dP.get(), F.get(), E.get());

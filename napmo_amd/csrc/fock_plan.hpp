@@ -33,6 +33,12 @@ struct FockPlan {
   DeviceBuffer<double> P, rho_part, rho, v, s, F_part, F, E, trace_work;
   FockShape fsh, psh;  // per-tile shape; one-pass shape (mixed mode)
   bool use_pass = false;
+  long fock_wg = 0, pass_wg = 0;  // workgroup targets of the two shapes (0 = default)
+  hipStream_t stream = nullptr;   // every build launch goes here (default: the grid's)
+  void reshape();
+  // Launch-shape knobs (NAPMO_TUNE_*), for the autotuner; not while a build is running.
+  void set_tuning(int key, long value);
+  long get_tuning(int key) const;
 
   // Per-phase HIP-event timing of builds (on the plan's stream), for roofline reports.
   static constexpr int NPHASE = 10;

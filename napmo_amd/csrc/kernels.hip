@@ -494,16 +494,16 @@ void launch_scale3(const double *a, const double *b, const double *c, long N, lo
 // =================================================== contractions: FP64 MFMA Fock
 // F[mu][nu] = sum_p X[p][mu] s[p] Y[p][nu]: A[m][k] = X[p0+k][mu0+m], B[k][n] =
 // s[p0+k] Y[p0+k][nu0+n]. Split over points (chunks); one workgroup per (tile, chunk).
-FockShape fock_shape(int ld, long Npad, bool symmetric) {
+FockShape fock_shape(int ld, long Npad, bool symmetric, long wg_target) {
   FockShape sh;
   sh.ld = ld;
   sh.nt = ld / TILE;
   sh.ntiles = symmetric ? sh.nt * (sh.nt + 1) / 2 : sh.nt * sh.nt;
   sh.symmetric = symmetric;
   sh.Npad = Npad;
-  // aim at ~2048 workgroups; chunks a multiple of 256 points (the mixed kernel's FP32
-  // accumulation block)
-  long want = std::max(1L, 2048L / sh.ntiles);
+  // aim at ~2048 workgroups (or wg_target); chunks a multiple of 256 points (the mixed
+  // kernel's FP32 accumulation block)
+  long want = std::max(1L, (wg_target > 0 ? wg_target : 2048L) / sh.ntiles);
   long chunk = round_up(div_up(Npad, want), 256);
   sh.chunk = std::max<long>(chunk, 256);
   sh.nchunk = (int)div_up(Npad, sh.chunk);
@@ -520,42 +520,71 @@ __device__ inline void tile_coords(int t, int nt, bool symmetric, int &ti, int &
   ti = i, tj = i + rem;
 }
 
+// Points run in stages of KS: the next stage's rows (4 consecutive doubles of one point
+// per thread for each operand) are loaded into registers while the current stage's MFMAs
+// run, then stored to the other LDS buffer (one barrier per stage). LDS rows hold 64
+// functions plus 16 doubles of padding, so a 16 x 4 fragment read touches each bank once
+// per half-wave.
+constexpr int FSTRIDE = TILE + 16;
+
 __global__ void __launch_bounds__(256) k_fock_f64(const double *__restrict__ X,
                                                   const double *__restrict__ Yv,
                                                   const double *__restrict__ s, int ld, int nt,
                                                   int symmetric, long Npad, long chunk,
                                                   double *__restrict__ partial) {
-  __shared__ double As[KS][TILE];
-  __shared__ double Bs[KS][TILE];
+  __shared__ double As[2][KS][FSTRIDE];  // As[.][k][m] = X[p0 + k][mu0 + m]
+  __shared__ double Bs[2][KS][FSTRIDE];  // Bs[.][k][n] = s[p0 + k] Y[p0 + k][nu0 + n]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int ti, tj;
   tile_coords(blockIdx.x, nt, symmetric, ti, tj);
   const int mu0 = ti * TILE, nu0 = tj * TILE;
+  const bool same = (X == Yv) && (ti == tj);
   const long pbeg = (long)blockIdx.y * chunk;
   const long pend = min(Npad, pbeg + chunk);
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int lr = tid >> 4, lc = (tid & 15) * 4;  // this thread's stage row and 4 columns
+  d4 ra, rb;
+  double rs = 0.0;
+  auto load = [&](long k0) {
+    const long p = k0 + lr;
+    ra = *(const d4 *)&X[p * ld + mu0 + lc];
+    if (!same) rb = *(const d4 *)&Yv[p * ld + nu0 + lc];
+    rs = s[p];
+  };
+  auto store = [&](int buf) {
+    const d4 b = same ? ra : rb;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      As[buf][lr][lc + e] = ra[e];
+      Bs[buf][lr][lc + e] = rs * b[e];
+    }
+  };
   d4 acc[2][2];
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) acc[a][b] = (d4){0, 0, 0, 0};
+  if (pbeg < pend) {
+    load(pbeg);
+    store(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  const int kr0 = lane >> 4, mi = lane & 15;
   for (long k0 = pbeg; k0 < pend; k0 += KS) {
-    for (int e = tid; e < TILE * KS; e += 256) {
-      const int k = e / TILE, m = e % TILE;
-      const long p = k0 + k;
-      As[k][m] = X[p * ld + mu0 + m];
-      Bs[k][m] = s[p] * Yv[p * ld + nu0 + m];
-    }
-    __syncthreads();
+    const bool more = k0 + KS < pend;
+    if (more) load(k0 + KS);
 #pragma unroll
     for (int ks = 0; ks < KS; ks += 4) {
-      const int kr = ks + (lane >> 4), mi = lane & 15;
-      double a0 = As[kr][wr + mi], a1 = As[kr][wr + 16 + mi];
-      double b0 = Bs[kr][wc + mi], b1 = Bs[kr][wc + 16 + mi];
+      const int kr = ks + kr0;
+      const double a0 = As[buf][kr][wr + mi], a1 = As[buf][kr][wr + 16 + mi];
+      const double b0 = Bs[buf][kr][wc + mi], b1 = Bs[buf][kr][wc + 16 + mi];
       acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
       acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
       acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
       acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
+    if (more) store(buf ^ 1);
     __syncthreads();
+    buf ^= 1;
   }
   double *out = partial + (long)blockIdx.y * ((long)gridDim.x * TILE * TILE + 32) +
                 (long)blockIdx.x * TILE * TILE;

@@ -112,7 +112,7 @@ struct FockShape {
   // chunks of one element do not all fall on the same few L2 channels
   long cstride() const { return (long)ntiles * TILE * TILE + 32; }
 };
-FockShape fock_shape(int ld, long Npad, bool symmetric);
+FockShape fock_shape(int ld, long Npad, bool symmetric, long wg_target = 0);
 void launch_fock_f64(const FockShape &sh, const double *X, const double *Y, const double *s,
                      double *partial, hipStream_t st);
 // F_out [n x n] row-major from the partials; symmetric mirrors upper tiles;
@@ -139,7 +139,7 @@ void launch_fock_bf16x3(const FockShape &sh, const unsigned short *Ahi, const un
                         const float *B32, const double *s, double *partial, hipStream_t st);
 // One pass over Phi (all upper tiles per chunk of points); same partial layout as
 // fock_shape(ld, Npad, true) with the pass's own chunking.
-FockShape fock_pass_shape(int ld, long Npad);
+FockShape fock_pass_shape(int ld, long Npad, long wg_target = 0);
 bool fock_pass_supported(int ld);
 void launch_fock_bf16x3_pass(const FockShape &sh, const unsigned short *Ahi,
                              const unsigned short *Alo, const double *s, float *partial,

@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(512) k_fock_bf16x3_pass(const unsigned short *
   }
 }
 
-FockShape fock_pass_shape(int ld, long Npad) {
+FockShape fock_pass_shape(int ld, long Npad, long wg_target) {
   FockShape sh;
   sh.ld = ld;
   sh.nt = ld / TILE;
@@ -295,8 +295,8 @@ FockShape fock_pass_shape(int ld, long Npad) {
   sh.symmetric = true;
   sh.Npad = Npad;
   const int groups = (int)div_up(sh.ntiles, PASS_WAVES * TPW);
-  // about 256 workgroups over all tile groups, chunks a multiple of 32 points
-  long chunk = round_up(div_up(Npad * groups, 256), KB);
+  // about 256 workgroups (or wg_target) over all tile groups, chunks a multiple of 32 points
+  long chunk = round_up(div_up(Npad * groups, wg_target > 0 ? wg_target : 256), KB);
   sh.chunk = std::max<long>(chunk, KB);
   sh.nchunk = (int)div_up(Npad, sh.chunk);
   return sh;

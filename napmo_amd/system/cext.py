@@ -143,6 +143,14 @@ _SIGNATURES = {
     'napmo_fock_plan_centres': (c_int, [vp, ctypes.POINTER(c_int), vp]),
     'napmo_grid_fock_gradient': (c_int, [vp, vp, vp]),
     'napmo_grid_fock_gradient_device': (c_int, [vp, vp, vp]),
+    'napmo_fock_plan_set_tuning': (c_int, [vp, c_int, c_long]),
+    'napmo_fock_plan_get_tuning': (c_int, [vp, c_int, ctypes.POINTER(c_long)]),
+    'napmo_batch_plan_new': (c_int, [c_int, vp, c_int, ctypes.POINTER(vp)]),
+    'napmo_batch_plan_del': (None, [vp]),
+    'napmo_batch_build': (c_int, [vp, vp, vp, vp]),
+    'napmo_batch_capture': (c_int, [vp, vp, vp, vp]),
+    'napmo_batch_replay': (c_int, [vp]),
+    'napmo_batch_sync': (c_int, [vp]),
     'napmo_apmo_plan_new': (c_int, [c_int, vp, vp, vp, vp, c_int, ctypes.POINTER(vp)]),
     'napmo_apmo_plan_del': (None, [vp]),
     'napmo_apmo_build': (c_int, [vp, vp, vp, vp]),

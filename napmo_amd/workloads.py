@@ -96,6 +96,33 @@ def apmo_system(seed=3, nheavy=4, nh=4, box=12.0, ncycle=12):
                        'charge': 1.0, 'eta': 1.0, 'nocc': len(protons)}}
 
 
+LEBEDEV_CHOICES = (110, 146, 170, 194, 230, 266, 302, 350, 434, 590)
+ZNUC = {'H': 1, 'C': 6, 'N': 7, 'O': 8}
+
+
+def batch_molecules(seed=5, count=64):
+    """``count`` small molecules: n_atoms ~ U{2..12} of {C,H,N,O} in a box of side
+    2.5 n^(1/3) + 2 bohr (min separation 1.8); per atom nr ~ U{35..110} Chebyshev points
+    and na uniform over LEBEDEV_CHOICES; U{2..20} cycles of s/p/d shells (n = 20..200);
+    nocc = ceil(n_elec / 2) clipped to n; lmax = min(12, Lebedev degree of the smallest
+    angular grid / 2)."""
+    from napmo_amd.grids.lebedev import lebedev_get_order
+    rng = np.random.default_rng(seed)
+    mols = []
+    for _ in range(count):
+        natom = int(rng.integers(2, 13))
+        atoms = random_molecule(rng, natom, 2.5 * natom ** (1.0 / 3.0) + 2.0)
+        sizes = [(int(rng.integers(35, 111)), int(rng.choice(LEBEDEV_CHOICES))) for _ in range(natom)]
+        ncycle = int(rng.integers(2, 21))
+        shells = random_shells(rng, natom, ncycle)
+        n = 10 * ncycle
+        nelec = sum(ZNUC[s] for s, _ in atoms)
+        lmax = min(12, lebedev_get_order(min(na for _, na in sizes)) // 2)
+        mols.append({'atoms': atoms, 'sizes': sizes, 'shells': shells, 'nbasis': n,
+                     'nocc': min((nelec + 1) // 2, n), 'lmax': lmax})
+    return mols
+
+
 def occupied_orbitals(rng, S, nocc):
     """C = C0 (C0^T S C0)^(-1/2), C0 ~ N(0, 1): nocc S-orthonormal orbitals [n x nocc]."""
     C0 = rng.normal(size=(S.shape[0], nocc))

@@ -1,0 +1,113 @@
+"""GPU batched builds of independent molecules: batch = single builds, graph replays =
+eager batch on every replay, FP64 parity with the oracle, and the launch-shape
+autotuner."""
+import numpy as np
+import pytest
+
+from oracle import cext as ocx
+from oracle import core
+from tests import systems
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(mols, precision='fp64'):
+    import napmo_amd
+    grids, bases, plans, Ps = [], [], [], []
+    for k, m in enumerate(mols):
+        g = napmo_amd.BeckeGrid(napmo_amd.make_species(m['atoms']), None, None, sizes=m['sizes'])
+        b = napmo_amd.make_basis(m['atoms'], shells_per_atom=m['shells'])
+        f = napmo_amd.GridFock(g, b, m['lmax'], precision)
+        S = f.overlap()
+        Ps.append(systems.density_matrix(np.random.default_rng(100 + k), S, m['nocc']))
+        grids.append(g), bases.append(b), plans.append(f)
+    return grids, bases, plans, Ps
+
+
+def _device(Ps):
+    from napmo_amd.device import DeviceArray
+    dP = [DeviceArray.from_host(P) for P in Ps]
+    dF = [DeviceArray(P.shape) for P in Ps]
+    dE = [DeviceArray((2,)) for _ in Ps]
+    return dP, dF, dE
+
+
+def test_batch_matches_single_builds_and_oracle():
+    import napmo_amd
+    from napmo_amd import workloads
+    mols = workloads.batch_molecules(count=6)
+    grids, bases, plans, Ps = _setup(mols)
+    single = [f.build(P) for f, P in zip(plans, Ps)]
+    dP, dF, dE = _device(Ps)
+    batch = napmo_amd.BatchFock(plans, nlanes=3)
+    batch.build([a.ptr for a in dP], [a.ptr for a in dF], [a.ptr for a in dE])
+    batch.synchronize()
+    for k, (F, E, Eg) in enumerate(single):
+        Fb, Eb = dF[k].to_host(), dE[k].to_host()
+        print('molecule %d N=%d n=%d E=%.12f' % (k, grids[k].size, F.shape[0], Eg))
+        assert np.array_equal(Fb, F) and Eb[0] == E and Eb[1] == Eg
+    # FP64 parity with the oracle on the two smallest molecules
+    order = sorted(range(len(mols)), key=lambda k: grids[k].size * mols[k]['nbasis'])
+    for k in order[:2]:
+        m = mols[k]
+        o, ob = systems.oracle_system(m['atoms'], m['shells'], None, None, sizes=m['sizes'])
+        phi = ocx.basis_compute(ob, o.points)
+        rho = core.density_from_dm(phi, Ps[k])
+        v = core.compute_coulomb(o, rho, m['lmax'])
+        Fo = core.coupling_matrix(phi, v, o.becke_weights, o.weights)
+        Eo = 0.5 * float(np.sum(o.becke_weights * o.weights * rho * v))
+        F, E, Eg = single[k]
+        print('oracle molecule %d: dE %.3e max|dF| %.3e' % (k, Eg - Eo, np.abs(F - Fo).max()))
+        assert abs(Eg - Eo) < 1e-10 * abs(Eo)
+        assert np.abs(F - Fo).max() < 1e-10 * np.abs(Fo).max()
+
+
+def test_batch_graph_replays_match_eager():
+    # every replay checked against an eager batch build on the same densities, with eager
+    # work and host synchronisation between replays
+    import napmo_amd
+    from napmo_amd import workloads
+    from napmo_amd.system import cext
+    mols = workloads.batch_molecules(count=5, seed=6)
+    for prec in ('fp64', 'bf16x3'):
+        grids, bases, plans, Ps = _setup(mols, prec)
+        dP, dF, dE = _device(Ps)
+        batch = napmo_amd.BatchFock(plans, nlanes=2)
+        ptr = lambda xs: [a.ptr for a in xs]  # noqa: E731
+        batch.capture(ptr(dP), ptr(dF), ptr(dE))
+        for it in range(3):
+            scale = 1.0 + 0.1 * it
+            for a, P in zip(dP, Ps):
+                Pi = np.ascontiguousarray(P * scale)
+                cext.check(cext.napmo_memcpy(a.ptr, Pi.ctypes.data, Pi.nbytes, 1))
+            batch.replay()
+            batch.synchronize()
+            got = [(f.to_host(), e.to_host()) for f, e in zip(dF, dE)]
+            batch.build(ptr(dP), ptr(dF), ptr(dE))
+            batch.synchronize()
+            for k, (Fg, Eg) in enumerate(got):
+                assert np.array_equal(Fg, dF[k].to_host()), (prec, it, k)
+                assert np.array_equal(Eg, dE[k].to_host()), (prec, it, k)
+            print(prec, 'replay', it, 'E', [float(e[1]) for _, e in got[:3]])
+        del batch
+
+
+def test_autotuner_choice_and_cache(tmp_path):
+    import napmo_amd
+    from napmo_amd import autotune, workloads
+    mols = workloads.batch_molecules(count=1, seed=8)
+    for prec in ('fp64', 'bf16x3'):
+        grids, bases, plans, Ps = _setup(mols, prec)
+        f = plans[0]
+        F0, E0, Eg0 = f.build(Ps[0])
+        dP, dF, dE = _device(Ps)
+        path = str(tmp_path / 'tuning.json')
+        setting, src = autotune.tune(f, dP[0].ptr, dF[0].ptr, dE[0].ptr, path=path)
+        print(prec, 'tuned', setting, src, autotune.current(f))
+        assert src == 'timed'
+        F1, E1, Eg1 = f.build(Ps[0])
+        assert abs(Eg1 - Eg0) < 1e-12 * abs(Eg0)           # rho and v do not depend on it
+        tol = 1e-12 if prec == 'fp64' else 1e-6
+        assert np.abs(F1 - F0).max() < tol * np.abs(F0).max()
+        setting2, src2 = autotune.tune(f, dP[0].ptr, dF[0].ptr, dE[0].ptr, path=path)
+        assert src2 == 'cache' and setting2 == setting
