@@ -282,12 +282,13 @@ def roofline(res, N, n, L, M):
 def cpu_baseline(budget_s, log):
     """The oracle (C/OpenMP restatement of the reference hot path, Python orchestration
     of the Poisson solves as napmo/grids/poisson_solver.py) timed per stage on a bounded
-    sample of the 16-atom, 472,000-point build and scaled to one build."""
+    sample of the 16-atom, 472,000-point build and scaled to one build, with all host
+    threads and with one thread."""
     from oracle import cext as ocx
     from oracle import core
     from napmo_amd import workloads
     atoms, shells, nrad, nang, lmax = workloads.chon16_system()
-    threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+    threads = ocx.max_threads()
     t0 = time.perf_counter()
     grid = core.BeckeGrid(atoms, nrad, nang)
     basis = core.Basis()
@@ -296,39 +297,50 @@ def cpu_baseline(budget_s, log):
             basis.add_shell(o, shell['l'], shell['prim'], shell['cont'])
     N, n, M = grid.size, basis.nbasis, grid.ncenter
     rng = np.random.default_rng(0)
-    ns = 16384
-    idx = np.sort(rng.choice(N, ns, replace=False))
-    pts = grid.points[idx]
-    phi = ocx.basis_compute(basis, pts)
     P = rng.normal(size=(n, n))
     P = P + P.T
-    t = time.perf_counter()
-    rho = ocx.density(phi, P)
-    t_density = (time.perf_counter() - t) * N / ns
-    # Poisson for one atom (projection + all (l, m) solves), scaled by the atom count
+    # Poisson for one atom (projection + all (l, m) solves, single-threaded Python and
+    # SciPy as in the reference), scaled by the atom count
     ag = grid.atgrids[0]
     f = np.abs(rng.normal(size=ag.size))
     t = time.perf_counter()
     sph = core.spherical_expansion(ag, f, lmax)
     U = core.poisson_radial(ag, sph, lmax)
     t_poisson = (time.perf_counter() - t) * M
-    t = time.perf_counter()
-    out = np.zeros(ns)
-    ocx.eval_decomposition(U, grid.origin[0], pts, out)
-    t_synth = (time.perf_counter() - t) * M * N / ns
-    v = rng.normal(size=ns)
-    t = time.perf_counter()
-    ocx.coupling(np.ascontiguousarray(phi.T), v, grid.becke_weights[idx], grid.weights[idx])
-    t_fock = (time.perf_counter() - t) * N / ns
-    total = t_density + t_poisson + t_synth + t_fock
-    log('cpu baseline: density %.2fs poisson %.2fs synth %.2fs fock %.2fs (per build, %d threads, '
-        'sampled in %.1fs)' % (t_density, t_poisson, t_synth, t_fock, threads, time.perf_counter() - t0))
-    del rho
+
+    def stages(ns, nthreads):
+        ocx.set_threads(nthreads)
+        idx = np.sort(rng.choice(N, ns, replace=False))
+        pts = grid.points[idx]
+        phi = ocx.basis_compute(basis, pts)
+        t = time.perf_counter()
+        ocx.density(phi, P)
+        t_density = (time.perf_counter() - t) * N / ns
+        t = time.perf_counter()
+        out = np.zeros(ns)
+        ocx.eval_decomposition(U, grid.origin[0], pts, out)
+        t_synth = (time.perf_counter() - t) * M * N / ns
+        v = rng.normal(size=ns)
+        t = time.perf_counter()
+        ocx.coupling(np.ascontiguousarray(phi.T), v, grid.becke_weights[idx], grid.weights[idx])
+        t_fock = (time.perf_counter() - t) * N / ns
+        return t_density, t_synth, t_fock
+
+    d_all, s_all, f_all = stages(16384, threads)
+    d_one, s_one, f_one = stages(2048, 1)
+    ocx.set_threads(threads)
+    total = d_all + t_poisson + s_all + f_all
+    total_one = d_one + t_poisson + s_one + f_one
+    log('cpu baseline (%d threads): density %.2fs poisson %.2fs synth %.2fs fock %.2fs per build; '
+        '1 thread: %.1fs per build (sampled in %.1fs)' % (threads, d_all, t_poisson, s_all, f_all,
+                                                          total_one, time.perf_counter() - t0))
     return {'value': 1.0 / total, 'unit': 'builds/s', 'cores': threads, 'kind': 'port',
             'sample': 'oracle C/OpenMP stages on 16384 of 472000 points (density, synthesis over 1 of '
                       '16 centres, contraction) and the full Poisson of 1 of 16 atoms, each scaled to '
                       'one 16-atom 472,000-point build; Becke/collocation set-up excluded as on the GPU',
-            'seconds_per_build': total}
+            'seconds_per_build': total,
+            'one_thread': {'value': 1.0 / total_one, 'seconds_per_build': total_one,
+                           'sample': 'same stages on 2048 points, 1 thread'}}
 
 
 def main():

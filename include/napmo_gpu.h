@@ -73,6 +73,9 @@ double RadialGrid_get_radii(void *grid);
 double *RadialGrid_get_points(void *grid);
 double *RadialGrid_get_weights(void *grid);
 double RadialGrid_integrate(void *grid, int segments, double *f);
+/* src/grid/radial.cpp:53-192: (2/r) f' + f'' by finite differences (Chebyshev grids,
+ * size >= 8); returns [size], release with napmo_free. */
+double *RadialGrid_deriv2(void *grid, double *f);
 void RadialGrid_del(void *grid);
 /* src/grid/atomic_grid.cpp:12-136 */
 void *AtomicGrid_new(void *angular, void *radial, double *origin);
@@ -182,6 +185,15 @@ int napmo_compute_coulomb(void *grid, const double *dens, int lmax, double *v_ou
  * lm, the potential spline on A's radial grid: U_y, U_dt [sum_A nrad_A x L] (atom-major,
  * then radial, then lm). */
 int napmo_poisson_solve(void *grid, const double *dens, int lmax, double *U_y, double *U_dt);
+/* Kinetic operator on the grid (napmo/wavefunction/nkinetic.py:15-73): out [N] =
+ * -1/2 laplacian of psi [N] from the Becke partition of psi, per (atom, l, m)
+ * deriv2 - l(l+1)/r^2 of its spherical expansion, splined with PotentialExtrapolation(l)
+ * and synthesised at every point. Chebyshev radial grids of >= 8 points. */
+int napmo_compute_kinetic(void *grid, const double *psi, int lmax, double *out);
+/* Point-charge potential on the grid (napmo/wavefunction/nnuclear.py:15-37): V_out [N] =
+ * sum_A charges[A] / |r - origins[A]|, origins [ncharge x 3]. */
+int napmo_compute_nuclear(void *grid, int ncharge, const double *charges, const double *origins,
+                          double *V_out);
 
 /* Grid Fock plan: device-resident Phi (collocated once), Poisson plan, workspace. */
 int napmo_fock_plan_new(void *grid, void *basis, int lmax, int prec_mode, void **plan_out);

@@ -199,6 +199,16 @@ int RadialGrid_get_size(void *g) { return (int)((RadialGrid *)g)->points.size();
 double RadialGrid_get_radii(void *g) { return ((RadialGrid *)g)->radii; }
 double *RadialGrid_get_points(void *g) { return ((RadialGrid *)g)->points.data(); }
 double *RadialGrid_get_weights(void *g) { return ((RadialGrid *)g)->weights.data(); }
+double *RadialGrid_deriv2(void *g, double *f) {
+  return legacy(__func__, [&] {
+    RadialGrid *rg = (RadialGrid *)g;
+    const int n = (int)rg->points.size();
+    double *out = (double *)std::malloc(sizeof(double) * std::max(n, 1));
+    NAPMO_REQUIRE(out != nullptr, "RadialGrid_deriv2: out of host memory");
+    radial_deriv2(rg->radii, rg->points.data(), f, out, n);
+    return out;
+  });
+}
 double RadialGrid_integrate(void *g, int segments, double *f) {
   // src/grid/radial.cpp:31-51
   RadialGrid *r = (RadialGrid *)g;
@@ -502,6 +512,31 @@ int napmo_compute_coulomb(void *grid, const double *dens, int lmax, double *v_ou
     g.poisson_expand(d.get(), lmax);
     g.poisson_synthesize(lmax, v.get());
     v.download(v_out, g.N, g.stream);
+    NAPMO_HIP(hipStreamSynchronize(g.stream));
+  });
+}
+
+int napmo_compute_kinetic(void *grid, const double *psi, int lmax, double *out) {
+  return guarded([&] {
+    DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    DeviceBuffer<double> d, o(g.N);
+    d.upload(psi, g.N, g.stream);
+    g.kinetic(d.get(), lmax, o.get());
+    o.download(out, g.N, g.stream);
+    NAPMO_HIP(hipStreamSynchronize(g.stream));
+  });
+}
+
+int napmo_compute_nuclear(void *grid, int ncharge, const double *charges, const double *origins,
+                          double *V_out) {
+  return guarded([&] {
+    DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    NAPMO_REQUIRE(ncharge >= 0, "napmo_compute_nuclear: negative charge count");
+    DeviceBuffer<double> q, o, V(g.N);
+    q.upload(charges, ncharge, g.stream);
+    o.upload(origins, 3 * (size_t)ncharge, g.stream);
+    launch_nuclear(g.x.get(), g.y.get(), g.z.get(), g.N, q.get(), o.get(), ncharge, V.get(), g.stream);
+    V.download(V_out, g.N, g.stream);
     NAPMO_HIP(hipStreamSynchronize(g.stream));
   });
 }

@@ -50,6 +50,10 @@ struct RadialGrid {
   std::vector<double> points, weights;  // r_i, 4 pi r_i^2 dr/dt
 };
 RadialGrid *make_radial(RTransform *rtf, double radii);
+// RadialGrid::deriv2 (src/grid/radial.cpp:53-192): (2/r) f' + f'' on the Chebyshev grid of
+// radius parameter radii, by 7-point (ends) and 9-point finite differences in
+// omega = (i + 1) pi / (n + 1); points r [n]; n >= 8.
+void radial_deriv2(double radii, const double *points, const double *f, double *out, int n);
 
 struct AtomicGrid {
   AngularGrid *ang;  // borrowed

@@ -441,6 +441,27 @@ void launch_contrib_reduce(const double *contrib, int M, long N, int accumulate,
   NAPMO_HIP(hipGetLastError());
 }
 
+// V[p] = sum_A q_A / |p - R_A|, charges in list order (napmo/wavefunction/nnuclear.py:15-37).
+__global__ void k_nuclear(const double *__restrict__ x, const double *__restrict__ y,
+                          const double *__restrict__ z, long N, const double *__restrict__ q,
+                          const double *__restrict__ o, int nq, double *__restrict__ V) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  double s = 0.0;
+  for (int a = 0; a < nq; ++a) {
+    const double dx = o[3 * a] - x[p], dy = o[3 * a + 1] - y[p], dz = o[3 * a + 2] - z[p];
+    s += q[a] / sqrt(dx * dx + dy * dy + dz * dz);
+  }
+  V[p] = s;
+}
+
+void launch_nuclear(const double *x, const double *y, const double *z, long N, const double *q,
+                    const double *o, int nq, double *V, hipStream_t s) {
+  if (N <= 0) return;
+  k_nuclear<<<div_up(N, 256), 256, 0, s>>>(x, y, z, N, q, o, nq, V);
+  NAPMO_HIP(hipGetLastError());
+}
+
 // v[p] = a x[p] + b y[p]
 __global__ void k_axpby(double a, const double *__restrict__ x, double b,
                         const double *__restrict__ y, long N, double *__restrict__ v) {

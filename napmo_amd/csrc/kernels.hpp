@@ -87,6 +87,9 @@ void launch_synth2(const AtomTables &at, const double *x, const double *y, const
                    hipStream_t s);
 void launch_contrib_reduce(const double *contrib, int M, long N, int accumulate, double *v,
                            hipStream_t s, double scale = 1.0);
+// Point-charge potential V[p] = sum_A q_A / |p - R_A| (o: [nq x 3]).
+void launch_nuclear(const double *x, const double *y, const double *z, long N, const double *q,
+                    const double *o, int nq, double *V, hipStream_t s);
 // v = a x + b y over N points.
 void launch_axpby(double a, const double *x, double b, const double *y, long N, double *v,
                   hipStream_t s);

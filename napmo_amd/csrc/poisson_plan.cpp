@@ -216,6 +216,27 @@ void radial_green(const RTransform &rtf, const std::vector<double> &rw, int l,
     }
 }
 
+void kinetic_green(const RTransform &rtf, double radii, const std::vector<double> &r, int l,
+                   std::vector<double> &Gy, std::vector<double> &Gd) {
+  const int n = rtf.npoint;
+  NAPMO_REQUIRE(rtf.kind == 0, "kinetic operator: the radial grids must be Chebyshev");
+  Gy.assign((size_t)n * n, 0.0);
+  Gd.assign((size_t)n * n, 0.0);
+  std::vector<double> e(n), y(n), dt(n);
+  const double l2l = (double)l * (l + 1);
+  for (int jc = 0; jc < n; ++jc) {
+    std::fill(e.begin(), e.end(), 0.0);
+    e[jc] = 1.0;
+    radial_deriv2(radii, r.data(), e.data(), y.data(), n);
+    y[jc] -= l2l / (r[jc] * r[jc]);
+    spline_derivatives(y.data(), dt.data(), n);
+    for (int i = 0; i < n; ++i) {
+      Gy[(size_t)i * n + jc] = y[i];
+      Gd[(size_t)i * n + jc] = dt[i];
+    }
+  }
+}
+
 std::vector<double> projection_table(const AngularGrid &ang, int lmax) {
   const int L = (lmax + 1) * (lmax + 1), na = ang.lorder;
   std::vector<double> t((size_t)na * L), s(L);
@@ -352,11 +373,15 @@ void DeviceGrid::alloc_work(int lmax, PoissonWork &w) const {
   w.ampR.resize((size_t)M * L);
 }
 
-PoissonPlan &DeviceGrid::plan(int lmax) {
+PoissonPlan &DeviceGrid::plan(int lmax) { return build_plan(lmax, false); }
+PoissonPlan &DeviceGrid::kinetic_plan(int lmax) { return build_plan(lmax, true); }
+
+PoissonPlan &DeviceGrid::build_plan(int lmax, bool kinetic) {
   NAPMO_REQUIRE(lmax >= 0 && lmax <= MAX_LMAX,
                 "lmax must be in [0, " + std::to_string(MAX_LMAX) + "]");
-  auto it = plans.find(lmax);
-  if (it != plans.end()) return *it->second;
+  auto &cache = kinetic ? kplans : plans;
+  auto it = cache.find(lmax);
+  if (it != cache.end()) return *it->second;
   auto p = std::make_unique<PoissonPlan>();
   p->lmax = lmax;
   const int L = (lmax + 1) * (lmax + 1);
@@ -391,7 +416,10 @@ PoissonPlan &DeviceGrid::plan(int lmax) {
       off = (long)gr.size();
       for (int l = 0; l <= lmax; ++l) {
         std::vector<double> Gy, Gd;
-        radial_green(at.rtf, at.rw, l, Gy, Gd);
+        if (kinetic)
+          kinetic_green(at.rtf, at.radii, at.r, l, Gy, Gd);
+        else
+          radial_green(at.rtf, at.rw, l, Gy, Gd);
         gr.insert(gr.end(), Gy.begin(), Gy.end());
         gr.insert(gr.end(), Gd.begin(), Gd.end());
       }
@@ -406,8 +434,22 @@ PoissonPlan &DeviceGrid::plan(int lmax) {
   alloc_work(lmax, p->work);
   NAPMO_HIP(hipStreamSynchronize(stream));
   auto &ref = *p;
-  plans[lmax] = std::move(p);
+  cache[lmax] = std::move(p);
   return ref;
+}
+
+void DeviceGrid::kinetic(const double *d_psi, int lmax, double *d_out) {
+  PoissonPlan &pl = kinetic_plan(lmax);
+  AtomTables at = tables(pl);
+  PoissonWork &w = pl.work;
+  SynthPairs &sp = synth_pairs();
+  launch_project(at, d_psi, bw.get(), pl.yhat.get(), lmax, w.rho_lm.get(), stream);
+  launch_radial(at, w.rho_lm.get(), pl.green.get(), lmax, w.Uy.get(), w.Udt.get(), stream);
+  launch_synth_tables(at, w.Uy.get(), w.Udt.get(), lmax, w.T.get(), w.ampL.get(), w.ampR.get(),
+                      stream);
+  launch_synth2(at, x.get(), y.get(), z.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves,
+                w.T.get(), w.ampL.get(), w.ampR.get(), lmax, sp.contrib.get(), stream);
+  launch_contrib_reduce(sp.contrib.get(), M, N, 0, d_out, stream, -0.5);
 }
 
 void DeviceGrid::poisson_expand(const double *d_dens, int lmax) {

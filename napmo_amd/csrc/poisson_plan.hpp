@@ -29,6 +29,11 @@ void fe_assemble(const double *b, const double *a, const double *f, const double
 void decomposition_on_points(CubicSpline **sp, int nspline, int lmax, const double *center,
                              const double *points, long npoint, double *output);
 
+// Kinetic radial matrices for one l: Gy = deriv2 - l(l+1)/r^2 (on unit vectors), Gd = the
+// cubic-spline slopes of those columns; same layout as radial_green.
+void kinetic_green(const RTransform &rtf, double radii, const std::vector<double> &r, int l,
+                   std::vector<double> &Gy, std::vector<double> &Gd);
+
 // Per-Lebedev-rule projection table Yhat[j][lm] = aw_j S_lm(u_j) sqrt(4 pi (2l+1)) / sum aw.
 std::vector<double> projection_table(const AngularGrid &ang, int lmax);
 
@@ -77,13 +82,19 @@ struct DeviceGrid {
   DeviceBuffer<int> d_nrad, d_nang, d_rkind;
   DeviceBuffer<double> d_rp0, d_rp1, d_first, d_last, d_center, d_r;
   int max_nrad = 0;
-  std::map<int, std::unique_ptr<PoissonPlan>> plans;
+  std::map<int, std::unique_ptr<PoissonPlan>> plans, kplans;  // Poisson / kinetic, per lmax
   std::unique_ptr<SynthPairs> pairs;
 
   ~DeviceGrid();
   void build(const std::vector<AtomInfo> &atoms);
   AtomTables tables(const PoissonPlan &plan) const;
   PoissonPlan &plan(int lmax);
+  // Same projection tables; radial matrices of the kinetic operator: U_lm = the spline
+  // (y, dt) of deriv2(f_lm) - l(l+1)/r^2 f_lm (napmo/wavefunction/nkinetic.py:15-73).
+  PoissonPlan &kinetic_plan(int lmax);
+  PoissonPlan &build_plan(int lmax, bool kinetic);
+  // d_out [N] = -1/2 laplacian of d_psi [N] (grid order), on the grid's stream.
+  void kinetic(const double *d_psi, int lmax, double *d_out);
   // A private workspace sized for plan(lmax) (for plans that must not share one).
   void alloc_work(int lmax, PoissonWork &w) const;
   // Poisson of dens (device [N]) into the plan's U / tables; then synthesis to v (device).

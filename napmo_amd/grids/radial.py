@@ -25,6 +25,11 @@ class RadialGrid(object):
     def integrate(self, f, segments):
         return cext.RadialGrid_integrate(self._this, segments, np.ascontiguousarray(f, dtype=np.float64))
 
+    def deriv2(self, f):
+        """(2/r) f' + f'' by finite differences (src/grid/radial.cpp:53-192)."""
+        ptr = cext.RadialGrid_deriv2(self._this, np.ascontiguousarray(f, dtype=np.float64))
+        return cext.take_array(ptr, (self.size,))
+
     @property
     def size(self):
         return cext.RadialGrid_get_size(self._this)

@@ -80,6 +80,7 @@ _SIGNATURES = {
     'RadialGrid_get_points': (dptr, [vp]),
     'RadialGrid_get_weights': (dptr, [vp]),
     'RadialGrid_integrate': (c_double, [vp, c_int, a1df]),
+    'RadialGrid_deriv2': (dptr, [vp, a1df]),
     'RadialGrid_del': (None, [vp]),
     'AtomicGrid_new': (vp, [vp, vp, a1df]),
     'AtomicGrid_get_size': (c_int, [vp]),
@@ -135,6 +136,8 @@ _SIGNATURES = {
     # device-resident hot path
     'napmo_compute_coulomb': (c_int, [vp, a1df, c_int, a1df]),
     'napmo_poisson_solve': (c_int, [vp, a1df, c_int, adf, adf]),
+    'napmo_compute_kinetic': (c_int, [vp, a1df, c_int, a1df]),
+    'napmo_compute_nuclear': (c_int, [vp, c_int, a1df, a2df, a1df]),
     'napmo_fock_plan_new': (c_int, [vp, vp, c_int, c_int, ctypes.POINTER(vp)]),
     'napmo_fock_plan_del': (None, [vp]),
     'napmo_fock_plan_info': (c_int, [vp, ctypes.POINTER(c_long), ctypes.POINTER(c_int),

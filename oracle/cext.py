@@ -36,6 +36,9 @@ def lib():
                                                 C.c_double, C.c_double, C.c_double, d1, d1, C.c_long, d1]
         L.oracle_coupling.argtypes = [d1, d1, d1, d1, C.c_long, C.c_int, d1]
         L.oracle_density.argtypes = [d1, d1, C.c_long, C.c_int, d1]
+        L.oracle_set_threads.argtypes = [C.c_int]
+        L.oracle_set_threads.restype = None
+        L.oracle_max_threads.restype = C.c_int
         for fn in ('oracle_build_ode2', 'oracle_becke_weights', 'oracle_basis_compute',
                    'oracle_eval_decomposition', 'oracle_coupling', 'oracle_density'):
             getattr(L, fn).restype = None
@@ -139,3 +142,11 @@ def density(phi, P):
     lib().oracle_density(np.ascontiguousarray(phi).reshape(-1), np.ascontiguousarray(P).reshape(-1),
                          phi.shape[0], phi.shape[1], out)
     return out
+
+
+def set_threads(n):
+    lib().oracle_set_threads(int(n))
+
+
+def max_threads():
+    return int(lib().oracle_max_threads())

@@ -777,6 +777,100 @@ def compute_coulomb(grid, dens, lmax, linsolve=None):
     return J
 
 
+# Finite-difference stencils of RadialGrid::deriv2 (src/grid/radial.cpp:53-192) on the
+# uniform omega = (i + 1) pi / (n + 1) axis. Rows 0..3 act on FF(size-1), ..., FF(size-7);
+# rows size-4..size-1 on FF(6), ..., FF(0) (both in the order the reference adds them);
+# interior rows on FF(idx+4), ..., FF(idx-4), idx = size - i - 1.
+_D1_BOT = [(-1764, 4320, -5400, 4800, -2700, 864, -120), (-120, -924, 1800, -1200, 600, -180, 24),
+           (24, -288, -420, 960, -360, 96, -12), (-12, 108, -540, 0, 540, -108, 12)]
+_D1_TOP = [(-12, 108, -540, 0, 540, -108, 12), (12, -96, 360, -960, 420, 288, -24),
+           (-24, 180, -600, 1200, -1800, 924, 120), (120, -864, 2700, -4800, 5400, -4320, 1764)]
+_D2_BOT = [(1624, -6264, 10530, -10160, 5940, -1944, 274), (274, -294, -510, 940, -570, 186, -26),
+           (-26, 456, -840, 400, 30, -24, 4), (4, -54, 540, -980, 540, -54, 4)]
+_D2_TOP = [(4, -54, 540, -980, 540, -54, 4), (4, -24, 30, 400, -840, 456, -26),
+           (-26, 186, -570, 940, -510, -294, 274), (274, -1944, 5940, -10160, 10530, -6264, 1624)]
+_D1_MID = (144, -1536, 8064, -32256, 0, 32256, -8064, 1536, -144)
+_D2_MID = (-36, 512, -4032, 32256, -57400, 32256, -4032, 512, -36)
+
+
+def _stencil(coeffs, vals):
+    acc = 0.0
+    for c, v in zip(coeffs, vals):
+        if c:
+            acc += c * v
+    return acc
+
+
+def radial_deriv2(radii, points, f):
+    """RadialGrid::deriv2 (src/grid/radial.cpp:53-192) for the Chebyshev grid
+    r = R (1 + x) / (1 - x), x = cos omega: (2/r) f' + f'' from 7-point (ends) and 9-point
+    finite differences in omega. Needs at least 8 points."""
+    f = np.asarray(f, dtype=np.float64)
+    n = len(f)
+    if n < 8:
+        raise ValueError('deriv2 needs at least 8 radial points')
+    H = math.pi / (n + 1.0)
+    FF = lambda k: f[k]  # noqa: E731
+    F1 = np.zeros(n)
+    F2 = np.zeros(n)
+    bot = [FF(n - 1 - k) for k in range(7)]
+    top = [FF(6 - k) for k in range(7)]
+    for i in range(4):
+        F1[i] = _stencil(_D1_BOT[i], bot) / (720.0 * H)
+        F2[i] = _stencil(_D2_BOT[i], bot) / (360.0 * H * H)
+        F1[n - 4 + i] = _stencil(_D1_TOP[i], top) / (720.0 * H)
+        F2[n - 4 + i] = _stencil(_D2_TOP[i], top) / (360.0 * H * H)
+    for i in range(4, n - 4):
+        idx = n - i - 1
+        vals = [FF(idx + 4 - k) for k in range(9)]
+        F1[i] = _stencil(_D1_MID, vals) / (40320.0 * H)
+        F2[i] = _stencil(_D2_MID, vals) / (20160.0 * H * H)
+    D = np.zeros(n)
+    R2 = radii * radii
+    for i in range(n):
+        idx = n - i - 1
+        om = (i + 1) * H
+        X, Y = math.cos(om), math.sin(om)
+        Y2 = Y * Y * R2
+        Y3 = Y2 * Y
+        pre2 = (1.0 - X) * (1.0 - X)
+        pre3 = pre2 * (1.0 - X)
+        pre4 = pre3 * (1.0 - X)
+        D[idx] = ((F1[i] * ((0.5 * pre3 / (R2 * Y)) - (0.25 * pre4 * X / Y3)))
+                  - (F1[i] * pre2 / (radii * points[idx] * Y)) + (0.25 * F2[i] * pre4 / Y2))
+    return D
+
+
+def compute_kinetic(grid, psi, lmax):
+    """napmo/wavefunction/nkinetic.py:15-73: -1/2 laplacian of psi, per atom from the
+    spherical expansion of psi * bw, (deriv2 - l(l+1)/r^2) per (l, m), a cubic spline with
+    PotentialExtrapolation(l), synthesised at every grid point."""
+    U = np.zeros(grid.size)
+    for k, ag in enumerate(grid.atgrids):
+        f = psi[grid.offsets[k]:grid.offsets[k + 1]] * grid.becke_weights[grid.offsets[k]:grid.offsets[k + 1]]
+        sph = spherical_expansion(ag, f, lmax)
+        r = ag.r
+        splines = []
+        idx = 0
+        for l in range(lmax + 1):
+            pref = l * (l + 1) / (r * r)
+            for _m in range(-l, l + 1):
+                aux = np.array(sph[:, idx])
+                y = radial_deriv2(ag.rtf.radii, r, aux) - pref * aux
+                splines.append(Spline(y, spline_dt(y), ag.rtf, PotentialExtrapolation(l)))
+                idx += 1
+        U += eval_decomposition(splines, grid.origin[k], grid.points)
+    return -0.5 * U
+
+
+def compute_nuclear(points, charges):
+    """napmo/wavefunction/nnuclear.py:15-37: V(p) = sum_A Z_A / |p - R_A|."""
+    V = np.zeros(len(points))
+    for z, o in charges:
+        V += z / np.sqrt(((np.asarray(o) - points) ** 2).sum(1))
+    return V
+
+
 def coulomb_at(grid, dens, lmax, points, linsolve=None):
     """Potential of ``dens`` (given on ``grid``) at arbitrary points: the Becke-Poisson
     expansion of ``grid``'s atoms synthesised at ``points``. The reference evaluates the

@@ -8,6 +8,7 @@
  * function cites the reference file:line it restates.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -325,3 +326,7 @@ void oracle_density(const double *phi, const double *P, long npoint, int n, doub
     out[p] = rho;
   }
 }
+
+/* Thread count of the OpenMP loops above (the CPU baseline reports 1 and all cores). */
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int oracle_max_threads(void) { return omp_get_max_threads(); }

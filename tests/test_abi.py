@@ -124,3 +124,16 @@ def test_gto_normalisation_constants():
     assert abs(c.compute(np.ones([1, 3]))[0] - 0.069055017012) < 1e-12
     with pytest.raises(ValueError):
         napmo_amd.BasisSet('STO-3G', 'HH', origin=[0.0, 0.0, 0.0])
+
+
+def test_radial_deriv2_matches_oracle():
+    # RadialGrid_deriv2 is host code (legacy export): same restatement as the oracle
+    import napmo_amd
+    from oracle import core
+    rt = napmo_amd.ChebyshevRadialTransform(0.8, 40)
+    rg = napmo_amd.RadialGrid(40, 'H', rt)
+    r = np.array(rg.points)
+    f = np.exp(-0.9 * r * r) + 0.1 * r * np.exp(-0.2 * r)
+    mine = rg.deriv2(f)
+    ref = core.radial_deriv2(rg.radii, r, f)
+    assert np.max(np.abs(mine - ref)) <= 1e-13 * np.abs(ref).max()

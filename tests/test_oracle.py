@@ -283,3 +283,37 @@ def test_density_from_dm_screening_known_answer():
     S = np.tril(dm) + np.tril(dm, -1).T
     out = core.density_from_dm_screened(phi, dm, 0.0, None, np.zeros(40))
     assert np.allclose(out, np.einsum('pi,ij,pj->p', phi, S, phi), rtol=1e-13, atol=1e-13)
+
+
+def test_radial_deriv2_closed_form_convergence():
+    # (2/r) f' + f'' of exp(-a r^2) is (4 a^2 r^2 - 6 a) exp(-a r^2); fourth-order-like
+    # convergence in the number of Chebyshev points pins the stencils and the mapping
+    errs = []
+    for n in (50, 100, 200):
+        rtf = core.ChebyshevRTransform(1.0, n)
+        r = rtf.radius(np.arange(n, dtype=np.float64))
+        f = np.exp(-0.7 * r * r)
+        d = core.radial_deriv2(1.0, r, f)
+        ex = (4 * 0.49 * r * r - 4.2) * f
+        errs.append(np.abs(d - ex)[r < 6].max())
+    assert errs[2] < 3e-5 and errs[1] < errs[0] / 8 and errs[2] < errs[1] / 8
+
+
+def test_kinetic_closed_form_gaussian():
+    # one centre: -1/2 laplacian of exp(-a r^2) = -1/2 (4 a^2 r^2 - 6 a) exp(-a r^2); the
+    # error (largest at the nucleus) falls about 16x per doubling of the radial grid
+    errs = []
+    for nr in (60, 120):
+        grid = core.BeckeGrid([('H', (0.0, 0.0, 0.0))], nr, 50)
+        r = np.sqrt((grid.points ** 2).sum(1))
+        psi = np.exp(-0.8 * r * r)
+        T = core.compute_kinetic(grid, psi, 2)
+        ex = -0.5 * (4 * 0.64 * r * r - 4.8) * psi
+        errs.append(np.max(np.abs(T - ex)))
+    assert errs[1] < 2e-4 and errs[1] < errs[0] / 8
+
+
+def test_nuclear_potential_matches_direct_sum():
+    pts = np.array([[0.0, 0.0, 1.0], [1.0, 2.0, 2.0]])
+    V = core.compute_nuclear(pts, [(1.0, (0.0, 0.0, 0.0)), (8.0, (1.0, 2.0, 0.0))])
+    assert np.allclose(V, [1.0 + 8.0 / np.sqrt(1 + 4 + 1), 1.0 / 3.0 + 8.0 / 2.0], rtol=1e-15)
