@@ -252,6 +252,7 @@ void synth_order(int lmax, std::vector<int> &l, std::vector<int> &m, std::vector
           for (int k = 1; k <= mm; ++k) r /= std::sqrt((double)((ll + k) * (ll - k + 1)));
           f *= r;
         }
+        f *= pi_sigma(ll, mm);
         l.push_back(ll), m.push_back(mm), cs.push_back(c), scale.push_back(f);
       }
 }
@@ -268,7 +269,7 @@ __global__ void k_synth_tables(AtomTables at, const double *__restrict__ Uy,
     int l, m, cs;
     q_decode(q, lmax, l, m, cs);
     const int ref = l * l + (m ? 2 * m - 1 + cs : 0);
-    const double f = q_factor(l, m);
+    const double f = q_factor(l, m) * pi_sigma(l, m);
     const double y0 = Uy[(roff + j) * L + ref], y1 = Uy[(roff + j + 1) * L + ref];
     const double d0 = Udt[(roff + j) * L + ref], d1 = Udt[(roff + j + 1) * L + ref];
     const double zz = y1 - y0;

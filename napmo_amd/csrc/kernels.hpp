@@ -9,6 +9,23 @@
 namespace napmo {
 
 constexpr int MAX_LMAX = 24;   // synthesis / projection instantiated for lmax <= MAX_LMAX
+
+// Scaled PI_l^m recurrence of the synthesis: PI_l^m(z) = sigma(l, m) Pt_l^m(z) with
+// Pt_m^m = 1, Pt_{m+1}^m = z, Pt_l^m = z Pt_{l-1}^m - beta(l, m) Pt_{l-2}^m (r = 1), which
+// needs one multiply and one FMA per step; sigma(l, m) is folded into the spline tables.
+// From PI_l^m = ((2l-1) z PI_{l-1}^m - (l+m-1) PI_{l-2}^m) / (l-m), PI_m^m = (2m-1)!!,
+// PI_{m+1}^m = (2m+1) z PI_m^m.
+constexpr double pi_sigma(int l, int m) {
+  double s = 1.0;
+  for (int k = 1; k <= m; ++k) s *= (2 * k - 1);
+  if (l > m) s *= (2 * m + 1);
+  for (int k = m + 2; k <= l; ++k) s *= (double)(2 * k - 1) / (double)(k - m);
+  return s;
+}
+constexpr double pi_beta(int l, int m) {
+  return (double)(l + m - 1) / (double)(l - m) * pi_sigma(l - 2, m) / pi_sigma(l, m);
+}
+
 constexpr int TILE = 64;       // MFMA output tile edge of the contractions
 
 // Becke partition (src/grid/becke.cpp:79-176) for every point; owner[p] = atom of p.

@@ -9,7 +9,7 @@
 namespace napmo {
 
 // One thread per point, loop over centres. Per (point, centre): one inverse radial
-// transform, then the m-major recurrence of the PI_l^m(z) polynomials of the unit
+// transform, then the m-major recurrence of the scaled PI_l^m(z) polynomials of the unit
 // vector (r^2 = 1) and (x + i y)^m, each harmonic contracted with its Horner spline.
 static __device__ inline double rtf_inv(int kind, double p0, double p1, int n, double r) {
   if (kind == 0) return -(acos((r - p0) / (r + p0)) / p1) + n;
@@ -60,7 +60,6 @@ __global__ void __launch_bounds__(256) k_synth(AtomTables at, const double *__re
     double acc = 0.0;
     int q = 0;
     double Am = 1.0, Bm = 0.0;   // Re, Im of (ux + i uy)^m
-    double pimm = 1.0;           // PI_m^m = (2m-1)!!
     double pwm = (reg == 2) ? inv_d : 1.0;  // d^m (left) or d^-(m+1) (right)
 #pragma unroll
     for (int m = 0; m <= LMAX; ++m) {
@@ -68,25 +67,21 @@ __global__ void __launch_bounds__(256) k_synth(AtomTables at, const double *__re
         const double a2 = ux * Am - uy * Bm;
         Bm = ux * Bm + uy * Am;
         Am = a2;
-        pimm *= (2 * m - 1);
         pwm *= base;
       }
-      double pi2 = 0.0, pi1 = pimm, pw = pwm;
+      double pi2 = 0.0, pi1 = 1.0, pw = pwm;
 #pragma unroll
       for (int l = m; l <= LMAX; ++l) {
+        // scaled recurrence (pi_sigma folded into the tables)
         double pil;
         if (l == m) {
-          pil = pimm;
+          pil = 1.0;
         } else if (l == m + 1) {
-          pil = uz * (2 * m + 1) * pimm;
+          pil = uz;
           pi2 = pi1;
           pi1 = pil;
         } else {
-          // PI_l^m = ((2l-1) z PI_{l-1}^m - (l+m-1) PI_{l-2}^m) / (l-m) with r^2 = 1; the
-        // coefficients fold to constants (no FP64 division in the loop)
-        const double c1 = (double)(2 * l - 1) / (double)(l - m);
-        const double c2 = (double)(l + m - 1) / (double)(l - m);
-        pil = uz * c1 * pi1 - c2 * pi2;
+          pil = uz * pi1 - pi_beta(l, m) * pi2;
           pi2 = pi1;
           pi1 = pil;
         }
@@ -120,10 +115,13 @@ __global__ void __launch_bounds__(256) k_synth(AtomTables at, const double *__re
 template <int LMAX, int REG>
 __device__ __forceinline__ double synth_harmonics(const double *c, double u, double ux, double uy,
                                                   double uz, double d, double inv_d) {
+  // For each m the cos and sin parts are summed over l with the PI_l^m(z) factor only
+  // (sum_l s_lm PI_l^m), and multiplied by Re / Im (x + i y)^m once per m: one FMA per
+  // (l, m, cos|sin) besides the spline value.
   const double base = (REG == 0) ? d : inv_d;
   double acc = 0.0;
   int q = 0;
-  double Am = 1.0, Bm = 0.0, pimm = 1.0;
+  double Am = 1.0, Bm = 0.0;
   double pwm = (REG == 2) ? inv_d : 1.0;
 #pragma unroll
   for (int m = 0; m <= LMAX; ++m) {
@@ -131,30 +129,28 @@ __device__ __forceinline__ double synth_harmonics(const double *c, double u, dou
       const double a2 = ux * Am - uy * Bm;
       Bm = ux * Bm + uy * Am;
       Am = a2;
-      pimm *= (2 * m - 1);
       if (REG != 1) pwm *= base;
     }
-    double pi2 = 0.0, pi1 = pimm, pw = pwm;
+    double pi2 = 0.0, pi1 = 1.0, pw = pwm;
+    double sc = 0.0, ss = 0.0;
 #pragma unroll
     for (int l = m; l <= LMAX; ++l) {
+      // scaled recurrence Pt_l^m (pi_sigma(l, m) is folded into the tables)
       double pil;
       if (l == m) {
-        pil = pimm;
+        pil = 1.0;
       } else if (l == m + 1) {
-        pil = uz * (2 * m + 1) * pimm;
+        pil = uz;
         pi2 = pi1;
         pi1 = pil;
       } else {
-        const double c1 = (double)(2 * l - 1) / (double)(l - m);
-        const double c2 = (double)(l + m - 1) / (double)(l - m);
-        pil = uz * c1 * pi1 - c2 * pi2;
+        pil = uz * pi1 - pi_beta(l, m) * pi2;
         pi2 = pi1;
         pi1 = pil;
       }
       if (REG != 1 && l > m) pw *= base;
 #pragma unroll
       for (int cs = 0; cs < (m ? 2 : 1); ++cs) {
-        const double harm = (cs ? Bm : Am) * pil;
         double sv;
         if (REG == 1) {
           const double *cq = c + q * 4;
@@ -162,10 +158,14 @@ __device__ __forceinline__ double synth_harmonics(const double *c, double u, dou
         } else {
           sv = c[q] * pw;
         }
-        acc += sv * harm;
+        if (cs)
+          ss += sv * pil;
+        else
+          sc += sv * pil;
         ++q;
       }
     }
+    acc += Am * sc + Bm * ss;
   }
   return acc;
 }
