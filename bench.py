@@ -248,16 +248,52 @@ def batch_leg(args, log):
     }
 
 
+PHASE_KERNELS = {'density': 'k_density_mfma', 'fock': ('k_fock_bf16x3_pass', 'k_fock_f64'),
+                 'synth': 'k_synth2<', 'project': 'k_project', 'radial': 'k_radial',
+                 'fock_reduce': 'k_fock_reduce'}
+
+
+def measured_traffic(res, phase):
+    """HBM bytes per launch (read + write) of the phase's kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_chon16.json), or None."""
+    path = os.path.join(ROOT, 'profiles', 'traffic_chon16.json')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        kern = json.load(f).get('kernels', {})
+    want = PHASE_KERNELS.get(phase)
+    if want is None:
+        return None
+    if isinstance(want, tuple):
+        want = want[0] if res['prec'] == 'bf16x3' else want[1]
+    names = [k for k in kern if want in k]
+    if phase == 'density':
+        tag = ', true>' if res['form'] == 'C' else ', false>'
+        tag_int = ', 1>' if res['form'] == 'C' else ', 0>'
+        names = [k for k in names if k.endswith(tag) or k.endswith(tag_int)] or names
+    for k in names:
+        r, w = kern[k].get('read_bytes'), kern[k].get('write_bytes')
+        if r is not None and w is not None:
+            return r + w
+    return None
+
+
 def roofline(res, N, n, L, M):
     """Roofline of the dominant kernel, algorithmic work per launch / measured time."""
     ph = res['phases_ms']
     name = max(ph, key=ph.get)
     ms = ph[name]
+    roof = _roofline(res, name, ms, N, n, L, M)
+    roof['traffic'] = measured_traffic(res, name)
+    return name, roof
+
+
+def _roofline(res, name, ms, N, n, L, M):
     if name == 'density':
         # rho from P: 2 N n^2; orbital form: 2 N n nocc (both FP64 MFMA)
         flops = 2.0 * N * n * (n if res['form'] == 'P' else res['nocc'])
         peak = PEAK['fp64_mfma']
-        return name, {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
+        return {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
                       'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / peak, 'traffic': None}
     if name == 'fock':
         # algorithmic 2 N n^2; the bf16x3 split runs 3 bf16 products of it
@@ -266,16 +302,16 @@ def roofline(res, N, n, L, M):
             peak, work = PEAK['bf16_mfma'], 3.0 * flops
         else:
             peak, work = PEAK['fp64_mfma'], flops
-        return name, {'bound': 'mfma', 'achieved': work / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
+        return {'bound': 'mfma', 'achieved': work / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
                       'unit': 'TFLOP/s', 'frac': work / (ms * 1e-3) / peak, 'traffic': None}
     if name == 'synth':
         # 2 flops per (point, centre, lm); FP64 vector peak
         flops = 2.0 * N * M * L
         peak = PEAK['fp64_mfma']
-        return name, {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
+        return {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
                       'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / peak, 'traffic': None}
     nbytes = 8.0 * N * 4
-    return name, {'bound': 'hbm', 'achieved': nbytes / (ms * 1e-3) / 1e9, 'peak': PEAK['hbm'] / 1e9,
+    return {'bound': 'hbm', 'achieved': nbytes / (ms * 1e-3) / 1e9, 'peak': PEAK['hbm'] / 1e9,
                   'unit': 'GB/s', 'frac': nbytes / (ms * 1e-3) / PEAK['hbm'], 'traffic': None}
 
 

@@ -190,12 +190,12 @@ void FockPlan::synth_self(int slot) {
   launch_synth2(at, grid.x.get(), grid.y.get(), grid.z.get(), grid.N, sp.list.get(),
                 sp.wave_bucket.get(), sp.nwaves, work.T.get(), work.ampL.get(), work.ampR.get(), lmax,
                 sp.contrib.get(), st);
+  mark(slot, 6);  // the "scale" phase: centre sum, then w bw v
   launch_contrib_reduce(sp.contrib.get(), grid.M, grid.N, 0, v.get(), st);
 }
 
 void FockPlan::contract(int slot, const double *dP, const double *veff, double *dF, double *dE) {
   hipStream_t st = stream;
-  mark(slot, 6);
   launch_scale3(veff, grid.wbw.get(), nullptr, grid.N, Npad, s.get(), st);
   mark(slot, 7);
   // the one-pass kernel writes FP32 partials into the same buffer
