@@ -527,6 +527,19 @@ int napmo_compute_kinetic(void *grid, const double *psi, int lmax, double *out) 
   });
 }
 
+int napmo_kinetic_expand(void *grid, const double *psi, int lmax, double *U_y, double *U_dt) {
+  return guarded([&] {
+    DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    DeviceBuffer<double> d;
+    d.upload(psi, g.N, g.stream);
+    g.kinetic_expand(d.get(), lmax);
+    PoissonPlan &pl = g.kinetic_plan(lmax);
+    pl.work.Uy.download(U_y, pl.work.Uy.size(), g.stream);
+    pl.work.Udt.download(U_dt, pl.work.Udt.size(), g.stream);
+    NAPMO_HIP(hipStreamSynchronize(g.stream));
+  });
+}
+
 int napmo_compute_nuclear(void *grid, int ncharge, const double *charges, const double *origins,
                           double *V_out) {
   return guarded([&] {

@@ -438,13 +438,20 @@ PoissonPlan &DeviceGrid::build_plan(int lmax, bool kinetic) {
   return ref;
 }
 
+void DeviceGrid::kinetic_expand(const double *d_psi, int lmax) {
+  PoissonPlan &pl = kinetic_plan(lmax);
+  AtomTables at = tables(pl);
+  PoissonWork &w = pl.work;
+  launch_project(at, d_psi, bw.get(), pl.yhat.get(), lmax, w.rho_lm.get(), stream);
+  launch_radial(at, w.rho_lm.get(), pl.green.get(), lmax, w.Uy.get(), w.Udt.get(), stream);
+}
+
 void DeviceGrid::kinetic(const double *d_psi, int lmax, double *d_out) {
+  kinetic_expand(d_psi, lmax);
   PoissonPlan &pl = kinetic_plan(lmax);
   AtomTables at = tables(pl);
   PoissonWork &w = pl.work;
   SynthPairs &sp = synth_pairs();
-  launch_project(at, d_psi, bw.get(), pl.yhat.get(), lmax, w.rho_lm.get(), stream);
-  launch_radial(at, w.rho_lm.get(), pl.green.get(), lmax, w.Uy.get(), w.Udt.get(), stream);
   launch_synth_tables(at, w.Uy.get(), w.Udt.get(), lmax, w.T.get(), w.ampL.get(), w.ampR.get(),
                       stream);
   launch_synth2(at, x.get(), y.get(), z.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves,

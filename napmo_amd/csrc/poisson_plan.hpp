@@ -95,6 +95,8 @@ struct DeviceGrid {
   PoissonPlan &build_plan(int lmax, bool kinetic);
   // d_out [N] = -1/2 laplacian of d_psi [N] (grid order), on the grid's stream.
   void kinetic(const double *d_psi, int lmax, double *d_out);
+  // Its first half: the (l, m) splines (y, dt) into kinetic_plan(lmax).work.Uy / Udt.
+  void kinetic_expand(const double *d_psi, int lmax);
   // A private workspace sized for plan(lmax) (for plans that must not share one).
   void alloc_work(int lmax, PoissonWork &w) const;
   // Poisson of dens (device [N]) into the plan's U / tables; then synthesis to v (device).
