@@ -66,6 +66,14 @@ def gpu_leg(args, log):
     dC = DeviceArray.from_host(C)
     dF = DeviceArray((n, n))
     dE = DeviceArray((2,))
+    if not args.no_tune:
+        from napmo_amd import autotune
+        for prec, plan in plans.items():
+            t0 = time.perf_counter()
+            setting, src = autotune.tune(
+                plan, dP.ptr, dF.ptr, dE.ptr, tag='orb',
+                build=lambda plan=plan: plan.build_orbitals_device(dC.ptr, C.shape[1], 2.0, dF.ptr, dE.ptr))
+            log('autotune %s: %s (%s, %.1fs)' % (prec, setting, src, time.perf_counter() - t0))
     results = {}
     for m in modes:
         prec, form = MODES[m]
@@ -268,9 +276,8 @@ def measured_traffic(res, phase):
         want = want[0] if res['prec'] == 'bf16x3' else want[1]
     names = [k for k in kern if want in k]
     if phase == 'density':
-        tag = ', true>' if res['form'] == 'C' else ', false>'
-        tag_int = ', 1>' if res['form'] == 'C' else ', 0>'
-        names = [k for k in names if k.endswith(tag) or k.endswith(tag_int)] or names
+        tag = ', 1, ' if res['form'] == 'C' else ', 0, '
+        names = [k for k in names if tag in k] or names
     for k in names:
         r, w = kern[k].get('read_bytes'), kern[k].get('write_bytes')
         if r is not None and w is not None:
@@ -387,6 +394,7 @@ def main():
     ap.add_argument('--modes', default='fp64,fp64-orb,bf16x3-orb',
                     help='comma list of modes (%s); fp64 is always timed' % ', '.join(MODES))
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-tune', action='store_true', help='keep the default launch shapes')
     ap.add_argument('--copy-calibration', type=float, default=0.0, metavar='GIB',
                     help='only run a plain device copy of GIB gibibytes 5 times (counter calibration)')
     ap.add_argument('--cpu-seconds', type=float, default=30.0)

@@ -256,7 +256,8 @@ int napmo_grid_fock_gradient_device(void *plan, const double *dP, double *dG);
 enum {
   NAPMO_TUNE_FOCK_WG = 0,  /* workgroups of the per-tile Fock kernels (chunks x tiles); 0 = default */
   NAPMO_TUNE_PASS_WG = 1,  /* workgroups of the one-pass BF16x3 Fock kernel; 0 = default */
-  NAPMO_TUNE_USE_PASS = 2  /* 1: one-pass BF16x3 kernel, 0: per-tile BF16x3 kernel */
+  NAPMO_TUNE_USE_PASS = 2, /* 1: one-pass BF16x3 kernel, 0: per-tile BF16x3 kernel */
+  NAPMO_TUNE_DENSITY_KS = 3 /* K per LDS stage of the density kernels: 16 or 32; 0 = 16 */
 };
 int napmo_fock_plan_set_tuning(void *plan, int key, long value);
 int napmo_fock_plan_get_tuning(void *plan, int key, long *value);

@@ -3,7 +3,8 @@
 For a plan (one molecule's grid, basis and precision) it times every candidate launch
 shape of the Fock contraction on the device (the per-tile FP64 / BF16x3 kernels' workgroup
 count, the one-pass BF16x3 kernel's workgroup count, and the choice between the two
-BF16x3 kernels), keeps the fastest, and stores it in a JSON cache keyed by the shape class
+BF16x3 kernels) together with the K depth of the density kernels' LDS stages, keeps the
+fastest, and stores it in a JSON cache keyed by the shape class
 (precision, padded basis size, point count rounded to a power of two), so later plans of
 the same class start from the cached choice without timing. The cache lives at
 $NAPMO_TUNING_CACHE or ~/.cache/napmo_amd/tuning.json.
@@ -16,9 +17,10 @@ import time
 
 from napmo_amd.system import cext
 
-TUNE_FOCK_WG, TUNE_PASS_WG, TUNE_USE_PASS = 0, 1, 2
-_KEYS = {'fock_wg': TUNE_FOCK_WG, 'pass_wg': TUNE_PASS_WG, 'use_pass': TUNE_USE_PASS}
-DEFAULTS = {'fock_wg': 0, 'pass_wg': 0}   # 0: the library's default shape
+TUNE_FOCK_WG, TUNE_PASS_WG, TUNE_USE_PASS, TUNE_DENSITY_KS = 0, 1, 2, 3
+_KEYS = {'fock_wg': TUNE_FOCK_WG, 'pass_wg': TUNE_PASS_WG, 'use_pass': TUNE_USE_PASS,
+         'density_ks': TUNE_DENSITY_KS}
+DEFAULTS = {'fock_wg': 0, 'pass_wg': 0, 'density_ks': 16}   # 0: the library's default shape
 
 
 def cache_path():
@@ -33,11 +35,14 @@ def shape_key(fock):
 
 
 def candidates(fock):
+    """Launch shapes of the Fock contraction, each with both density stage depths."""
     if fock.precision == 'bf16x3':
         out = [{'use_pass': 1, 'pass_wg': w} for w in (128, 256, 512, 1024)]
         out += [{'use_pass': 0, 'fock_wg': w} for w in (1024, 2048, 4096)]
-        return out
-    return [{'fock_wg': w} for w in (512, 1024, 2048, 4096, 8192)]
+    else:
+        out = [{'fock_wg': w} for w in (512, 1024, 2048, 4096, 8192)]
+    ks = [16, 32] if (-(-fock.nbasis // 64) * 64) % 32 == 0 else [16]
+    return [dict(c, density_ks=k) for c in out for k in ks]
 
 
 def apply(fock, setting):
@@ -71,13 +76,15 @@ def _save(path, cache):
     os.replace(tmp, path)
 
 
-def tune(fock, dP, dF, dE, reps=3, path=None, force=False):
+def tune(fock, dP, dF, dE, reps=3, path=None, force=False, build=None, tag=''):
     """Choose and apply the fastest launch shape for ``fock`` (device buffers dP, dF, dE
-    of the right sizes are used for the timing builds). Returns (setting, source) with
-    source 'cache' or 'timed'."""
+    of the right sizes are used for the timing builds, or ``build()`` when given, e.g. an
+    orbital-form build; ``tag`` then names that build in the cache key). Returns
+    (setting, source) with source 'cache' or 'timed'."""
     path = path or cache_path()
     cache = _load(path)
-    key = shape_key(fock)
+    key = shape_key(fock) + (':' + tag if tag else '')
+    run = build or (lambda: fock.build_device(dP, dF, dE))
     if key in cache and not force:
         setting = {k: v for k, v in cache[key].items() if k in _KEYS}
         try:
@@ -91,11 +98,11 @@ def tune(fock, dP, dF, dE, reps=3, path=None, force=False):
             apply(fock, dict(DEFAULTS, **cand))
         except cext.NapmoError:
             continue
-        fock.build_device(dP, dF, dE)
+        run()
         fock.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
-            fock.build_device(dP, dF, dE)
+            run()
         fock.synchronize()
         t = (time.perf_counter() - t0) / reps
         if best is None or t < best[1]:

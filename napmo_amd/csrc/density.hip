@@ -5,7 +5,7 @@
 // (napmo/wavefunction/psi_hybrid.py:144-154 and napmo/wavefunction/psi_numeric.py:361-369).
 //
 // Workgroup: 128 points x 16 NB columns, 4 waves; wave w owns points 32 w .. 32 w + 31
-// (two 16-row blocks) and all NB column blocks. K runs in stages of 16: the next stage's
+// (two 16-row blocks) and all NB column blocks. K runs in stages of DKS (16 or 32): the next stage's
 // Phi rows and B rows are loaded into registers while the MFMAs of the current stage run,
 // then stored to the other LDS buffer (one barrier per stage). LDS rows are padded so each
 // 16 x 4 fragment read of a wave touches every bank once per half.
@@ -20,13 +20,12 @@ typedef double d2 __attribute__((ext_vector_type(2)));
 namespace {
 
 constexpr int DPT = 128;        // points per workgroup
-constexpr int DKS = 16;         // K per stage
 constexpr int ASTRIDE = DPT + 16;
 constexpr int MODE_P = 0, MODE_ORB = 1, MODE_STORE_X = 2;
 
 constexpr int bstride(int nb) { return (16 * nb) % 32 == 16 ? 16 * nb : 16 * nb + 16; }
 
-template <int NB, int MODE>
+template <int NB, int MODE, int DKS>
 __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__ phi, int ld,
                                                       long Npad, const double *__restrict__ B,
                                                       int ldb, double *__restrict__ partial) {
@@ -38,27 +37,28 @@ __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__
   const int c0 = blockIdx.y * 16 * NB;
   // this thread's share of a stage: Phi row (tid / 2), 8 columns from (tid % 2) * 8;
   // B elements tid + q * 256 (q < NB) of the 16 x 16 NB stage
-  const double *arow = phi + (p0 + (tid >> 1)) * ld + (tid & 1) * 8;
-  d2 ra[4];
-  double rb[NB];
+  constexpr int HALF = DKS / 2, NA = HALF / 2, NBQ = NB * DKS / 16;
+  const double *arow = phi + (p0 + (tid >> 1)) * ld + (tid & 1) * HALF;
+  d2 ra[NA];
+  double rb[NBQ];
   auto load = [&](int k0) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) ra[i] = *(const d2 *)(arow + k0 + 2 * i);
+    for (int i = 0; i < NA; ++i) ra[i] = *(const d2 *)(arow + k0 + 2 * i);
 #pragma unroll
-    for (int q = 0; q < NB; ++q) {
+    for (int q = 0; q < NBQ; ++q) {
       const int e = tid + q * 256, k = e / (16 * NB), c = e % (16 * NB);
       rb[q] = B[(long)(k0 + k) * ldb + c0 + c];
     }
   };
   auto store = [&](int buf) {
-    const int m = tid >> 1, kb = (tid & 1) * 8;
+    const int m = tid >> 1, kb = (tid & 1) * HALF;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < NA; ++i) {
       As[buf][kb + 2 * i][m] = ra[i].x;
       As[buf][kb + 2 * i + 1][m] = ra[i].y;
     }
 #pragma unroll
-    for (int q = 0; q < NB; ++q) {
+    for (int q = 0; q < NBQ; ++q) {
       const int e = tid + q * 256, k = e / (16 * NB), c = e % (16 * NB);
       Bs[buf][k][c] = rb[q];
     }
@@ -123,24 +123,34 @@ __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__
     }
 }
 
-template <int MODE>
-void launch_density_nb(int nb, dim3 grid, const double *phi, int ld, long Npad, const double *B,
+template <int MODE, int DKS>
+void launch_density_ks(int nb, dim3 grid, const double *phi, int ld, long Npad, const double *B,
                        int ldb, double *partial, hipStream_t s) {
   switch (nb) {
-    case 1: k_density_mfma<1, MODE><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
-    case 2: k_density_mfma<2, MODE><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
-    case 3: k_density_mfma<3, MODE><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
-    default: k_density_mfma<4, MODE><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
+    case 1: k_density_mfma<1, MODE, DKS><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
+    case 2: k_density_mfma<2, MODE, DKS><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
+    case 3: k_density_mfma<3, MODE, DKS><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
+    default: k_density_mfma<4, MODE, DKS><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
   }
   NAPMO_HIP(hipGetLastError());
+}
+
+// ks: K per stage, 16 or 32 (32 needs ld % 32 == 0)
+template <int MODE>
+void launch_density_nb(int nb, dim3 grid, const double *phi, int ld, long Npad, const double *B,
+                       int ldb, double *partial, hipStream_t s, int ks = 16) {
+  if (ks == 32 && ld % 32 == 0)
+    launch_density_ks<MODE, 32>(nb, grid, phi, ld, Npad, B, ldb, partial, s);
+  else
+    launch_density_ks<MODE, 16>(nb, grid, phi, ld, Npad, B, ldb, partial, s);
 }
 
 }  // namespace
 
 void launch_density_f64(const double *phi, int ld, long Npad, const double *P, double *partial,
-                        hipStream_t s) {
+                        hipStream_t s, int ks) {
   NAPMO_REQUIRE(ld % TILE == 0 && Npad % DPT == 0, "density: padding mismatch");
-  launch_density_nb<MODE_P>(4, dim3(Npad / DPT, ld / TILE), phi, ld, Npad, P, ld, partial, s);
+  launch_density_nb<MODE_P>(4, dim3(Npad / DPT, ld / TILE), phi, ld, Npad, P, ld, partial, s, ks);
 }
 
 int density_orb_cols(int nocc) {
@@ -151,12 +161,12 @@ int density_orb_cols(int nocc) {
 int density_orb_tiles(int ldc) { return ldc <= 64 ? 1 : ldc / 64; }
 
 void launch_density_orb_f64(const double *phi, int ld, long Npad, const double *C, int ldc,
-                            double *partial, hipStream_t s) {
-  NAPMO_REQUIRE(Npad % DPT == 0 && ld % DKS == 0 && ldc % 16 == 0, "density_orb: padding");
+                            double *partial, hipStream_t s, int ks) {
+  NAPMO_REQUIRE(Npad % DPT == 0 && ld % 16 == 0 && ldc % 16 == 0, "density_orb: padding");
   const int nb = ldc <= 64 ? ldc / 16 : 4;
   NAPMO_REQUIRE(ldc <= 64 || ldc % 64 == 0, "density_orb: orbital padding");
   launch_density_nb<MODE_ORB>(nb, dim3(Npad / DPT, density_orb_tiles(ldc)), phi, ld, Npad, C, ldc,
-                          partial, s);
+                              partial, s, ks);
 }
 
 void launch_phi_times(const double *phi, int ld, long Npad, const double *B, int ldb, double *X,

@@ -34,6 +34,7 @@ struct FockPlan {
   FockShape fsh, psh;  // per-tile shape; one-pass shape (mixed mode)
   bool use_pass = false;
   long fock_wg = 0, pass_wg = 0;  // workgroup targets of the two shapes (0 = default)
+  int density_ks = 16;            // K per LDS stage of the density kernels (16 or 32)
   hipStream_t stream = nullptr;   // every build launch goes here (default: the grid's)
   void reshape();
   // Launch-shape knobs (NAPMO_TUNE_*), for the autotuner; not while a build is running.

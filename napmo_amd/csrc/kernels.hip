@@ -106,16 +106,16 @@ void launch_collocate(const double *x, const double *y, const double *z, long N,
 // atom: one workgroup per (atom, block of PSB radial shells) so the Lebedev table
 // Yhat[j][lm] = aw_j S_lm(u_j) sqrt(4 pi (2l+1)) / sum aw is read once per block; the
 // shells' f values are staged in LDS in chunks of PJ angular points.
-constexpr int PSB = 5, PJ = 256;
+constexpr int PSB = 5, PJ = 256, PSL = 8;
 
-// Threads: 64 lm x 4 angular slices; slice k sums angular points j = k, k + 4, ...;
-// slices are combined in order through LDS.
-__global__ void __launch_bounds__(256) k_project(AtomTables at, const double *__restrict__ dens,
-                                                 const double *__restrict__ bw,
-                                                 const double *__restrict__ yhat, int L,
-                                                 double *__restrict__ rho_lm) {
+// Threads: 64 lm x PSL angular slices; slice k sums angular points j = k, k + PSL, ... (two
+// independent loads in flight per step); slices are combined in order through LDS.
+__global__ void __launch_bounds__(64 * PSL) k_project(AtomTables at, const double *__restrict__ dens,
+                                                      const double *__restrict__ bw,
+                                                      const double *__restrict__ yhat, int L,
+                                                      double *__restrict__ rho_lm) {
   __shared__ double sf[PSB][PJ];
-  __shared__ double red[4][PSB][64];
+  __shared__ double red[PSL][PSB][64];
   const int A = blockIdx.z;
   const int i0 = blockIdx.x * PSB;
   const int nr = at.nrad[A];
@@ -138,27 +138,36 @@ __global__ void __launch_bounds__(256) k_project(AtomTables at, const double *__
       sf[k][j] = (k < ns && j < nj) ? dens[p] * bw[p] : 0.0;
     }
     __syncthreads();
-    if (lm < L)
-      for (int j = slice; j < nj; j += 4) {
+    if (lm < L) {
+      int j = slice;
+      for (; j + PSL < nj; j += 2 * PSL) {
+        const double y0 = Y[(long)(j0 + j) * L + lm], y1 = Y[(long)(j0 + j + PSL) * L + lm];
+#pragma unroll
+        for (int k = 0; k < PSB; ++k) acc[k] += sf[k][j] * y0 + sf[k][j + PSL] * y1;
+      }
+      for (; j < nj; j += PSL) {
         const double y = Y[(long)(j0 + j) * L + lm];
 #pragma unroll
         for (int k = 0; k < PSB; ++k) acc[k] += sf[k][j] * y;
       }
+    }
   }
 #pragma unroll
   for (int k = 0; k < PSB; ++k) red[slice][k][lml] = acc[k];
   __syncthreads();
   if (slice == 0 && lm < L)
-    for (int k = 0; k < ns; ++k)
-      rho_lm[(at.rad_off[A] + i0 + k) * (long)L + lm] =
-          ((red[0][k][lml] + red[1][k][lml]) + red[2][k][lml]) + red[3][k][lml];
+    for (int k = 0; k < ns; ++k) {
+      double t = 0.0;
+      for (int sl = 0; sl < PSL; ++sl) t += red[sl][k][lml];
+      rho_lm[(at.rad_off[A] + i0 + k) * (long)L + lm] = t;
+    }
 }
 
 void launch_project(const AtomTables &at, const double *dens, const double *bw, const double *yhat,
                     int lmax, double *rho_lm, hipStream_t s) {
   const int L = (lmax + 1) * (lmax + 1);
   dim3 grid(div_up(at.max_nrad, PSB), div_up(L, 64), at.M);
-  k_project<<<grid, 256, 0, s>>>(at, dens, bw, yhat, L, rho_lm);
+  k_project<<<grid, 64 * PSL, 0, s>>>(at, dens, bw, yhat, L, rho_lm);
   NAPMO_HIP(hipGetLastError());
 }
 
