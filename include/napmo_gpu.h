@@ -191,8 +191,10 @@ int napmo_poisson_solve(void *grid, const double *dens, int lmax, double *U_y, d
  * and synthesised at every point. Chebyshev radial grids of >= 8 points. */
 int napmo_compute_kinetic(void *grid, const double *psi, int lmax, double *out);
 /* Its radial half: the spline values and slopes (y, dt) of deriv2 - l(l+1)/r^2 of every
- * (atom, l, m) component, U_y, U_dt [sum_A nrad_A x L] (as napmo_poisson_solve). */
-int napmo_kinetic_expand(void *grid, const double *psi, int lmax, double *U_y, double *U_dt);
+ * (atom, l, m) component, U_y, U_dt [sum_A nrad_A x L] (as napmo_poisson_solve), and the
+ * projected components themselves into rho_lm (same shape; may be NULL). */
+int napmo_kinetic_expand(void *grid, const double *psi, int lmax, double *U_y, double *U_dt,
+                         double *rho_lm);
 /* Point-charge potential on the grid (napmo/wavefunction/nnuclear.py:15-37): V_out [N] =
  * sum_A charges[A] / |r - origins[A]|, origins [ncharge x 3]. */
 int napmo_compute_nuclear(void *grid, int ncharge, const double *charges, const double *origins,

@@ -527,7 +527,8 @@ int napmo_compute_kinetic(void *grid, const double *psi, int lmax, double *out) 
   });
 }
 
-int napmo_kinetic_expand(void *grid, const double *psi, int lmax, double *U_y, double *U_dt) {
+int napmo_kinetic_expand(void *grid, const double *psi, int lmax, double *U_y, double *U_dt,
+                         double *rho_lm) {
   return guarded([&] {
     DeviceGrid &g = ((BeckeHandle *)grid)->grid;
     DeviceBuffer<double> d;
@@ -536,6 +537,7 @@ int napmo_kinetic_expand(void *grid, const double *psi, int lmax, double *U_y, d
     PoissonPlan &pl = g.kinetic_plan(lmax);
     pl.work.Uy.download(U_y, pl.work.Uy.size(), g.stream);
     pl.work.Udt.download(U_dt, pl.work.Udt.size(), g.stream);
+    if (rho_lm) pl.work.rho_lm.download(rho_lm, pl.work.rho_lm.size(), g.stream);
     NAPMO_HIP(hipStreamSynchronize(g.stream));
   });
 }

@@ -25,6 +25,17 @@ constexpr int MODE_P = 0, MODE_ORB = 1, MODE_STORE_X = 2;
 
 constexpr int bstride(int nb) { return (16 * nb) % 32 == 16 ? 16 * nb : 16 * nb + 16; }
 
+// Workgroup g runs on XCD g % 8 (round-robin dispatch); the column tiles of one point
+// tile are given ids that are congruent mod 8, so they share that XCD's L2 and the point
+// tile's Phi rows come from HBM once: g = 8 i + x, column tile i % nct, point tile
+// (i / nct) * 8 + x.
+__device__ inline bool xcd_tile(int g, int nct, int npt, int &pt, int &ct) {
+  const int x = g & 7, i = g >> 3;
+  ct = i % nct;
+  pt = (i / nct) * 8 + x;
+  return pt < npt;
+}
+
 template <int NB, int MODE, int DKS>
 __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__ phi, int ld,
                                                       long Npad, const double *__restrict__ B,
@@ -33,8 +44,10 @@ __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__
   __shared__ double As[2][DKS][ASTRIDE];  // As[.][k][m] = Phi[p0 + m][k0 + k]
   __shared__ double Bs[2][DKS][BS];       // Bs[.][k][c] = B[k0 + k][c0 + c]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const long p0 = (long)blockIdx.x * DPT;
-  const int c0 = blockIdx.y * 16 * NB;
+  int pt, ct;
+  if (!xcd_tile(blockIdx.x, ldb / (16 * NB), (int)(Npad / DPT), pt, ct)) return;
+  const long p0 = (long)pt * DPT;
+  const int c0 = ct * 16 * NB;
   // this thread's share of a stage: Phi row (tid / 2), 8 columns from (tid % 2) * 8;
   // B elements tid + q * 256 (q < NB) of the 16 x 16 NB stage
   constexpr int HALF = DKS / 2, NA = HALF / 2, NBQ = NB * DKS / 16;
@@ -119,13 +132,15 @@ __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__
       sum += __shfl_xor(sum, 4);
       sum += __shfl_xor(sum, 2);
       sum += __shfl_xor(sum, 1);
-      if (mi == 0) partial[(long)blockIdx.y * Npad + p] = sum;
+      if (mi == 0) partial[(long)ct * Npad + p] = sum;
     }
 }
 
 template <int MODE, int DKS>
-void launch_density_ks(int nb, dim3 grid, const double *phi, int ld, long Npad, const double *B,
+void launch_density_ks(int nb, dim3 tiles, const double *phi, int ld, long Npad, const double *B,
                        int ldb, double *partial, hipStream_t s) {
+  // tiles = (point tiles, column tiles) -> one 1-D grid in the XCD-aware order
+  const dim3 grid((unsigned)(round_up(tiles.x, 8) * tiles.y));
   switch (nb) {
     case 1: k_density_mfma<1, MODE, DKS><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
     case 2: k_density_mfma<2, MODE, DKS><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;

@@ -561,16 +561,22 @@ constexpr int FSTRIDE = TILE + 16;
 __global__ void __launch_bounds__(256) k_fock_f64(const double *__restrict__ X,
                                                   const double *__restrict__ Yv,
                                                   const double *__restrict__ s, int ld, int nt,
-                                                  int symmetric, long Npad, long chunk,
-                                                  double *__restrict__ partial) {
+                                                  int symmetric, int ntiles, int nchunk, long Npad,
+                                                  long chunk, double *__restrict__ partial) {
   __shared__ double As[2][KS][FSTRIDE];  // As[.][k][m] = X[p0 + k][mu0 + m]
   __shared__ double Bs[2][KS][FSTRIDE];  // Bs[.][k][n] = s[p0 + k] Y[p0 + k][nu0 + n]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // XCD-aware order: workgroup g runs on XCD g % 8; all tiles of one chunk of points get
+  // ids congruent mod 8 (g = 8 i + x: tile i % ntiles, chunk (i / ntiles) * 8 + x), so the
+  // chunk's Phi rows are shared through one L2
+  const int g = blockIdx.x, xcd = g & 7, gi = g >> 3;
+  const int tile = gi % ntiles, chunk_id = (gi / ntiles) * 8 + xcd;
+  if (chunk_id >= nchunk) return;
   int ti, tj;
-  tile_coords(blockIdx.x, nt, symmetric, ti, tj);
+  tile_coords(tile, nt, symmetric, ti, tj);
   const int mu0 = ti * TILE, nu0 = tj * TILE;
   const bool same = (X == Yv) && (ti == tj);
-  const long pbeg = (long)blockIdx.y * chunk;
+  const long pbeg = (long)chunk_id * chunk;
   const long pend = min(Npad, pbeg + chunk);
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int lr = tid >> 4, lc = (tid & 15) * 4;  // this thread's stage row and 4 columns
@@ -617,8 +623,8 @@ __global__ void __launch_bounds__(256) k_fock_f64(const double *__restrict__ X,
     __syncthreads();
     buf ^= 1;
   }
-  double *out = partial + (long)blockIdx.y * ((long)gridDim.x * TILE * TILE + 32) +
-                (long)blockIdx.x * TILE * TILE;
+  double *out = partial + (long)chunk_id * ((long)ntiles * TILE * TILE + 32) +
+                (long)tile * TILE * TILE;
   const int col16 = lane & 15, rg = lane >> 4;
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
@@ -628,9 +634,9 @@ __global__ void __launch_bounds__(256) k_fock_f64(const double *__restrict__ X,
 
 void launch_fock_f64(const FockShape &sh, const double *X, const double *Y, const double *s,
                      double *partial, hipStream_t st) {
-  dim3 grid(sh.ntiles, sh.nchunk);
-  k_fock_f64<<<grid, 256, 0, st>>>(X, Y, s, sh.ld, sh.nt, sh.symmetric ? 1 : 0, sh.Npad, sh.chunk,
-                                   partial);
+  const dim3 grid((unsigned)(sh.ntiles * round_up(sh.nchunk, 8)));
+  k_fock_f64<<<grid, 256, 0, st>>>(X, Y, s, sh.ld, sh.nt, sh.symmetric ? 1 : 0, sh.ntiles,
+                                   sh.nchunk, sh.Npad, sh.chunk, partial);
   NAPMO_HIP(hipGetLastError());
 }
 

@@ -137,7 +137,7 @@ _SIGNATURES = {
     'napmo_compute_coulomb': (c_int, [vp, a1df, c_int, a1df]),
     'napmo_poisson_solve': (c_int, [vp, a1df, c_int, adf, adf]),
     'napmo_compute_kinetic': (c_int, [vp, a1df, c_int, a1df]),
-    'napmo_kinetic_expand': (c_int, [vp, a1df, c_int, adf, adf]),
+    'napmo_kinetic_expand': (c_int, [vp, a1df, c_int, adf, adf, vp]),
     'napmo_compute_nuclear': (c_int, [vp, c_int, a1df, a2df, a1df]),
     'napmo_fock_plan_new': (c_int, [vp, vp, c_int, c_int, ctypes.POINTER(vp)]),
     'napmo_fock_plan_del': (None, [vp]),

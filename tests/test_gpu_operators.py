@@ -21,21 +21,30 @@ def test_kinetic_matches_oracle_and_closed_form():
     # the radial half: spline values and slopes per (atom, l, m) against the oracle's
     from napmo_amd.system import cext
     nrad = [ag.r.size for ag in o.atgrids]
-    Uy, Ud = np.zeros(sum(nrad) * L), np.zeros(sum(nrad) * L)
-    cext.check(cext.napmo_kinetic_expand(g._this, np.ascontiguousarray(psi), lmax, Uy, Ud))
+    Uy, Ud, Rl = np.zeros(sum(nrad) * L), np.zeros(sum(nrad) * L), np.zeros(sum(nrad) * L)
+    cext.check(cext.napmo_kinetic_expand(g._this, np.ascontiguousarray(psi), lmax, Uy, Ud, Rl.ctypes.data))
     off = 0
-    worst_l = np.zeros(lmax + 1)
+    worst_l = np.zeros(lmax + 1)     # GPU y against the oracle chain
+    worst_rho = np.zeros(lmax + 1)   # GPU projection against the oracle's
+    worst_rad = np.zeros(lmax + 1)   # GPU y against the oracle deriv2 of the GPU projection
     for k, ag in enumerate(o.atgrids):
         f = psi[o.offsets[k]:o.offsets[k + 1]] * o.becke_weights[o.offsets[k]:o.offsets[k + 1]]
         sph = core.spherical_expansion(ag, f, lmax)
         r = ag.r
         yg = Uy[off * L:(off + r.size) * L].reshape(r.size, L)
+        rg = Rl[off * L:(off + r.size) * L].reshape(r.size, L)
         for lm in range(L):
             l = int(np.sqrt(lm))
             y = core.radial_deriv2(ag.rtf.radii, r, sph[:, lm]) - l * (l + 1) / (r * r) * sph[:, lm]
-            worst_l[l] = max(worst_l[l], np.abs(yg[:, lm] - y).max() / max(np.abs(y).max(), 1e-300))
+            yr = core.radial_deriv2(ag.rtf.radii, r, rg[:, lm]) - l * (l + 1) / (r * r) * rg[:, lm]
+            sc = max(np.abs(y).max(), 1e-300)
+            worst_l[l] = max(worst_l[l], np.abs(yg[:, lm] - y).max() / sc)
+            worst_rad[l] = max(worst_rad[l], np.abs(yg[:, lm] - yr).max() / sc)
+            worst_rho[l] = max(worst_rho[l], np.abs(rg[:, lm] - sph[:, lm]).max() / max(np.abs(sph[:, lm]).max(), 1e-300))
         off += r.size
     print('kinetic expansion worst rel per l', worst_l)
+    print('kinetic projection worst rel per l', worst_rho)
+    print('kinetic radial step worst rel per l (same projection)', worst_rad)
     T = napmo_amd.compute_kinetic(g, psi, lmax)
     ref = core.compute_kinetic(o, psi, lmax)
     i = int(np.argmax(np.abs(T - ref)))
