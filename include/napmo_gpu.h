@@ -259,7 +259,8 @@ enum {
   NAPMO_TUNE_FOCK_WG = 0,  /* workgroups of the per-tile Fock kernels (chunks x tiles); 0 = default */
   NAPMO_TUNE_PASS_WG = 1,  /* workgroups of the one-pass BF16x3 Fock kernel; 0 = default */
   NAPMO_TUNE_USE_PASS = 2, /* 1: one-pass BF16x3 kernel, 0: per-tile BF16x3 kernel */
-  NAPMO_TUNE_DENSITY_KS = 3 /* K per LDS stage of the density kernels: 16 or 32; 0 = 16 */
+  NAPMO_TUNE_DENSITY_KS = 3, /* K per LDS stage of the density kernels: 16 or 32; 0 = 16 */
+  NAPMO_TUNE_DENSITY_PF = 4  /* stages in flight in the density kernels: 1 or 2; 0 = 1 */
 };
 int napmo_fock_plan_set_tuning(void *plan, int key, long value);
 int napmo_fock_plan_get_tuning(void *plan, int key, long *value);

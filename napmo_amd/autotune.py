@@ -3,7 +3,8 @@
 For a plan (one molecule's grid, basis and precision) it times every candidate launch
 shape of the Fock contraction on the device (the per-tile FP64 / BF16x3 kernels' workgroup
 count, the one-pass BF16x3 kernel's workgroup count, and the choice between the two
-BF16x3 kernels) together with the K depth of the density kernels' LDS stages, keeps the
+BF16x3 kernels) together with the K depth and prefetch depth of the density kernels' LDS
+stages, keeps the
 fastest, and stores it in a JSON cache keyed by the shape class
 (precision, padded basis size, point count rounded to a power of two), so later plans of
 the same class start from the cached choice without timing. The cache lives at
@@ -17,10 +18,10 @@ import time
 
 from napmo_amd.system import cext
 
-TUNE_FOCK_WG, TUNE_PASS_WG, TUNE_USE_PASS, TUNE_DENSITY_KS = 0, 1, 2, 3
+TUNE_FOCK_WG, TUNE_PASS_WG, TUNE_USE_PASS, TUNE_DENSITY_KS, TUNE_DENSITY_PF = 0, 1, 2, 3, 4
 _KEYS = {'fock_wg': TUNE_FOCK_WG, 'pass_wg': TUNE_PASS_WG, 'use_pass': TUNE_USE_PASS,
-         'density_ks': TUNE_DENSITY_KS}
-DEFAULTS = {'fock_wg': 0, 'pass_wg': 0, 'density_ks': 16}   # 0: the library's default shape
+         'density_ks': TUNE_DENSITY_KS, 'density_pf': TUNE_DENSITY_PF}
+DEFAULTS = {'fock_wg': 0, 'pass_wg': 0, 'density_ks': 16, 'density_pf': 1}   # 0: the library's default shape
 
 
 def cache_path():
@@ -42,7 +43,7 @@ def candidates(fock):
     else:
         out = [{'fock_wg': w} for w in (512, 1024, 2048, 4096, 8192)]
     ks = [16, 32] if (-(-fock.nbasis // 64) * 64) % 32 == 0 else [16]
-    return [dict(c, density_ks=k) for c in out for k in ks]
+    return [dict(c, density_ks=k, density_pf=f) for c in out for k in ks for f in (1, 2)]
 
 
 def apply(fock, setting):

@@ -36,7 +36,7 @@ __device__ inline bool xcd_tile(int g, int nct, int npt, int &pt, int &ct) {
   return pt < npt;
 }
 
-template <int NB, int MODE, int DKS>
+template <int NB, int MODE, int DKS, int PF>
 __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__ phi, int ld,
                                                       long Npad, const double *__restrict__ B,
                                                       int ldb, double *__restrict__ partial) {
@@ -52,9 +52,7 @@ __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__
   // B elements tid + q * 256 (q < NB) of the 16 x 16 NB stage
   constexpr int HALF = DKS / 2, NA = HALF / 2, NBQ = NB * DKS / 16;
   const double *arow = phi + (p0 + (tid >> 1)) * ld + (tid & 1) * HALF;
-  d2 ra[NA];
-  double rb[NBQ];
-  auto load = [&](int k0) {
+  auto load = [&](d2 (&ra)[NA], double (&rb)[NBQ], int k0) {
 #pragma unroll
     for (int i = 0; i < NA; ++i) ra[i] = *(const d2 *)(arow + k0 + 2 * i);
 #pragma unroll
@@ -63,7 +61,7 @@ __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__
       rb[q] = B[(long)(k0 + k) * ldb + c0 + c];
     }
   };
-  auto store = [&](int buf) {
+  auto store = [&](const d2 (&ra)[NA], const double (&rb)[NBQ], int buf) {
     const int m = tid >> 1, kb = (tid & 1) * HALF;
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
@@ -81,14 +79,8 @@ __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[a][b] = (d4){0, 0, 0, 0};
-  const int nstage = ld / DKS;
-  load(0);
-  store(0);
-  __syncthreads();
   const int kr = lane >> 4, mi = lane & 15;
-  for (int st = 0; st < nstage; ++st) {
-    const int buf = st & 1;
-    if (st + 1 < nstage) load((st + 1) * DKS);
+  auto compute = [&](int buf) {
 #pragma unroll
     for (int ks = 0; ks < DKS; ks += 4) {
       double a[2], b[NB];
@@ -102,8 +94,40 @@ __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__
         for (int j = 0; j < NB; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
-    if (st + 1 < nstage) store(buf ^ 1);
+  };
+  const int nstage = ld / DKS;
+  d2 ra0[NA], ra1[NA];
+  double rb0[NBQ], rb1[NBQ];
+  if (PF == 1) {
+    // one stage in flight: load st + 1 while computing st
+    load(ra0, rb0, 0);
+    store(ra0, rb0, 0);
     __syncthreads();
+    for (int st = 0; st < nstage; ++st) {
+      const int buf = st & 1;
+      if (st + 1 < nstage) load(ra0, rb0, (st + 1) * DKS);
+      compute(buf);
+      if (st + 1 < nstage) store(ra0, rb0, buf ^ 1);
+      __syncthreads();
+    }
+  } else {
+    // two stages in flight: registers hold st + 1 (to store) and st + 2 (arriving) while
+    // stage st is computed from LDS
+    load(ra0, rb0, 0);
+    store(ra0, rb0, 0);
+    if (nstage > 1) load(ra1, rb1, DKS);
+    __syncthreads();
+    for (int st = 0; st < nstage; st += 2) {
+      if (st + 2 < nstage) load(ra0, rb0, (st + 2) * DKS);
+      compute(0);
+      if (st + 1 < nstage) store(ra1, rb1, 1);
+      __syncthreads();
+      if (st + 1 >= nstage) break;
+      if (st + 3 < nstage) load(ra1, rb1, (st + 3) * DKS);
+      compute(1);
+      if (st + 2 < nstage) store(ra0, rb0, 0);
+      __syncthreads();
+    }
   }
   // f64 16x16x4 output: lane i, element r -> row (i / 16) + 4 r, column i % 16
   if (MODE == MODE_STORE_X) {
@@ -136,36 +160,44 @@ __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__
     }
 }
 
-template <int MODE, int DKS>
+template <int MODE, int DKS, int PF>
 void launch_density_ks(int nb, dim3 tiles, const double *phi, int ld, long Npad, const double *B,
                        int ldb, double *partial, hipStream_t s) {
   // tiles = (point tiles, column tiles) -> one 1-D grid in the XCD-aware order
   const dim3 grid((unsigned)(round_up(tiles.x, 8) * tiles.y));
   switch (nb) {
-    case 1: k_density_mfma<1, MODE, DKS><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
-    case 2: k_density_mfma<2, MODE, DKS><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
-    case 3: k_density_mfma<3, MODE, DKS><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
-    default: k_density_mfma<4, MODE, DKS><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
+    case 1: k_density_mfma<1, MODE, DKS, PF><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
+    case 2: k_density_mfma<2, MODE, DKS, PF><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
+    case 3: k_density_mfma<3, MODE, DKS, PF><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
+    default: k_density_mfma<4, MODE, DKS, PF><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
   }
   NAPMO_HIP(hipGetLastError());
 }
 
-// ks: K per stage, 16 or 32 (32 needs ld % 32 == 0)
+// ks: K per stage, 16 or 32 (32 needs ld % 32 == 0); pf: stages in flight, 1 or 2
 template <int MODE>
 void launch_density_nb(int nb, dim3 grid, const double *phi, int ld, long Npad, const double *B,
-                       int ldb, double *partial, hipStream_t s, int ks = 16) {
-  if (ks == 32 && ld % 32 == 0)
-    launch_density_ks<MODE, 32>(nb, grid, phi, ld, Npad, B, ldb, partial, s);
-  else
-    launch_density_ks<MODE, 16>(nb, grid, phi, ld, Npad, B, ldb, partial, s);
+                       int ldb, double *partial, hipStream_t s, int ks = 16, int pf = 1) {
+  if (ks == 32 && ld % 32 == 0) {
+    if (pf == 2)
+      launch_density_ks<MODE, 32, 2>(nb, grid, phi, ld, Npad, B, ldb, partial, s);
+    else
+      launch_density_ks<MODE, 32, 1>(nb, grid, phi, ld, Npad, B, ldb, partial, s);
+  } else {
+    if (pf == 2)
+      launch_density_ks<MODE, 16, 2>(nb, grid, phi, ld, Npad, B, ldb, partial, s);
+    else
+      launch_density_ks<MODE, 16, 1>(nb, grid, phi, ld, Npad, B, ldb, partial, s);
+  }
 }
 
 }  // namespace
 
 void launch_density_f64(const double *phi, int ld, long Npad, const double *P, double *partial,
-                        hipStream_t s, int ks) {
+                        hipStream_t s, int ks, int pf) {
   NAPMO_REQUIRE(ld % TILE == 0 && Npad % DPT == 0, "density: padding mismatch");
-  launch_density_nb<MODE_P>(4, dim3(Npad / DPT, ld / TILE), phi, ld, Npad, P, ld, partial, s, ks);
+  launch_density_nb<MODE_P>(4, dim3(Npad / DPT, ld / TILE), phi, ld, Npad, P, ld, partial, s, ks,
+                            pf);
 }
 
 int density_orb_cols(int nocc) {
@@ -176,12 +208,12 @@ int density_orb_cols(int nocc) {
 int density_orb_tiles(int ldc) { return ldc <= 64 ? 1 : ldc / 64; }
 
 void launch_density_orb_f64(const double *phi, int ld, long Npad, const double *C, int ldc,
-                            double *partial, hipStream_t s, int ks) {
+                            double *partial, hipStream_t s, int ks, int pf) {
   NAPMO_REQUIRE(Npad % DPT == 0 && ld % 16 == 0 && ldc % 16 == 0, "density_orb: padding");
   const int nb = ldc <= 64 ? ldc / 16 : 4;
   NAPMO_REQUIRE(ldc <= 64 || ldc % 64 == 0, "density_orb: orbital padding");
   launch_density_nb<MODE_ORB>(nb, dim3(Npad / DPT, density_orb_tiles(ldc)), phi, ld, Npad, C, ldc,
-                              partial, s, ks);
+                              partial, s, ks, pf);
 }
 
 void launch_phi_times(const double *phi, int ld, long Npad, const double *B, int ldb, double *X,

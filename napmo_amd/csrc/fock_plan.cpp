@@ -87,6 +87,10 @@ void FockPlan::set_tuning(int key, long value) {
                     "one-pass kernel not available for this plan");
       use_pass = value != 0;
       break;
+    case NAPMO_TUNE_DENSITY_PF:
+      NAPMO_REQUIRE(value == 0 || value == 1 || value == 2, "density prefetch depth must be 1 or 2");
+      density_pf = value ? (int)value : 1;
+      break;
     case NAPMO_TUNE_DENSITY_KS:
       NAPMO_REQUIRE(value == 0 || value == 16 || (value == 32 && ld % 32 == 0),
                     "density stage depth must be 16 or 32");
@@ -104,6 +108,7 @@ long FockPlan::get_tuning(int key) const {
     case NAPMO_TUNE_PASS_WG: return (long)psh.nchunk;
     case NAPMO_TUNE_USE_PASS: return use_pass ? 1 : 0;
     case NAPMO_TUNE_DENSITY_KS: return density_ks;
+    case NAPMO_TUNE_DENSITY_PF: return density_pf;
     default: throw Error("unknown tuning key " + std::to_string(key));
   }
 }
@@ -154,7 +159,7 @@ void FockPlan::density_P(int slot, const double *dP) {
   // P (n x n) into the zero-padded ld x ld operand (a kernel, so the build can be captured)
   launch_pad2d(dP, n, n, P.get(), ld, ld, st);
   mark(slot, 0);
-  launch_density_f64(phi64.get(), ld, Npad, P.get(), rho_part.get(), st, density_ks);
+  launch_density_f64(phi64.get(), ld, Npad, P.get(), rho_part.get(), st, density_ks, density_pf);
   mark(slot, 1);
   launch_rho_reduce(rho_part.get(), ld / TILE, Npad, grid.N, rho.get(), st);
 }
@@ -169,7 +174,7 @@ void FockPlan::density_C(int slot, const double *dC, int nocc, double eta) {
   launch_pad2d(dC, n, nocc, Cpad.get(), ld, ldc, st);
   launch_gram(dC, n, nocc, eta, Pfull.get(), st);
   mark(slot, 0);
-  launch_density_orb_f64(phi64.get(), ld, Npad, Cpad.get(), ldc, rho_part.get(), st, density_ks);
+  launch_density_orb_f64(phi64.get(), ld, Npad, Cpad.get(), ldc, rho_part.get(), st, density_ks, density_pf);
   mark(slot, 1);
   launch_rho_reduce(rho_part.get(), nct, Npad, grid.N, rho.get(), st, eta);
 }

@@ -113,7 +113,7 @@ void launch_axpby(double a, const double *x, double b, const double *y, long N, 
 
 // rho[p] = Phi_p^T P Phi_p with FP64 MFMA (P [ld x ld] zero padded). partial [ntile x Npad].
 void launch_density_f64(const double *phi, int ld, long Npad, const double *P, double *partial,
-                        hipStream_t s, int ks = 16);
+                        hipStream_t s, int ks = 16, int pf = 1);
 // rho[p] = scale * sum_t partial[t][p] for p < N.
 void launch_rho_reduce(const double *partial, int ntile, long Npad, long N, double *rho,
                        hipStream_t s, double scale = 1.0);
@@ -180,7 +180,7 @@ void launch_grad_centres(const double *x, const double *y, const double *z, long
                          double *grad, hipStream_t st);
 int density_orb_tiles(int ldc);
 void launch_density_orb_f64(const double *phi, int ld, long Npad, const double *C, int ldc,
-                            double *partial, hipStream_t s, int ks = 16);
+                            double *partial, hipStream_t s, int ks = 16, int pf = 1);
 void launch_gram(const double *C, int n, int nocc, double eta, double *P, hipStream_t s);
 void launch_pad2d(const double *src, int rows, int cols, double *dst, int drows, int dcols,
                   hipStream_t s);

@@ -115,11 +115,10 @@ __global__ void __launch_bounds__(256) k_synth(AtomTables at, const double *__re
 template <int LMAX, int REG>
 __device__ __forceinline__ double synth_harmonics(const double *c, double u, double ux, double uy,
                                                   double uz, double d, double inv_d) {
-  // For each m the cos and sin parts are summed over l with the PI_l^m(z) factor only and
-  // multiplied by Re / Im (x + i y)^m once per m. Inside the spline (REG 1) the sum over l
-  // runs on the four Horner coefficients separately, sum_l PI c_k, and the cubic in u is
-  // formed once per (m, cos|sin): every coefficient enters one FMA as a (wave-uniform)
-  // scalar operand.
+  // Harmonic sum of one (point, centre) pair in the synthesis order q (m-major): the
+  // spline value of every (l, m, cos|sin) by Horner in u (REG 1) or the extrapolation
+  // amplitude times d^l / d^-(l+1) (REG 0 / 2), times the scaled PI_l^m(z) recurrence
+  // (pi_sigma folded into the tables) and Re / Im (x + i y)^m.
   const double base = (REG == 0) ? d : inv_d;
   double acc = 0.0;
   int q = 0;
@@ -134,11 +133,8 @@ __device__ __forceinline__ double synth_harmonics(const double *c, double u, dou
       if (REG != 1) pwm *= base;
     }
     double pi2 = 0.0, pi1 = 1.0, pw = pwm;
-    double kc[4] = {0.0, 0.0, 0.0, 0.0}, ks[4] = {0.0, 0.0, 0.0, 0.0};
-    double sc = 0.0, ss = 0.0;
 #pragma unroll
     for (int l = m; l <= LMAX; ++l) {
-      // scaled recurrence Pt_l^m (pi_sigma(l, m) is folded into the tables)
       double pil;
       if (l == m) {
         pil = 1.0;
@@ -154,45 +150,31 @@ __device__ __forceinline__ double synth_harmonics(const double *c, double u, dou
       if (REG != 1 && l > m) pw *= base;
 #pragma unroll
       for (int cs = 0; cs < (m ? 2 : 1); ++cs) {
+        const double harm = (cs ? Bm : Am) * pil;
+        double sv;
         if (REG == 1) {
           const double *cq = c + q * 4;
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            if (cs)
-              ks[k] += pil * cq[k];
-            else
-              kc[k] += pil * cq[k];
-          }
+          sv = cq[0] + u * (cq[1] + u * (cq[2] + u * cq[3]));
         } else {
-          const double sv = c[q] * pw;
-          if (cs)
-            ss += sv * pil;
-          else
-            sc += sv * pil;
+          sv = c[q] * pw;
         }
+        acc += sv * harm;
         ++q;
       }
     }
-    if (REG == 1) {
-      sc = kc[0] + u * (kc[1] + u * (kc[2] + u * kc[3]));
-      ss = ks[0] + u * (ks[1] + u * (ks[2] + u * ks[3]));
-    }
-    acc += Am * sc + Bm * ss;
   }
   return acc;
 }
 
 // Bucketed variant: the (point, centre) pairs are sorted by (centre, spline interval)
-// once per geometry; each wavefront handles 64 pairs of one bucket, so the bucket's spline
-// row is wave-uniform: its coefficients are read with scalar loads (one fetch per wave
-// through the scalar cache, operands of the FMAs straight from SGPRs) rather than
-// broadcast from LDS, which would bound the kernel by LDS bandwidth. Only the Horner
-// variable u differs per lane.
+// once per geometry; each wavefront handles 64 pairs of one bucket, copies that bucket's
+// spline row into its own LDS slice once, and every lane then reads the coefficients as
+// LDS broadcasts; only the Horner variable u differs per lane.
 // bucket = A * (max_nrad + 1) + loc, loc 0 = left extrapolation, 1 + j = interval j,
 // nrad_A = right extrapolation. Writes contrib[A * N + p].
 template <int LMAX> struct Synth2Cfg {
   static constexpr int L = (LMAX + 1) * (LMAX + 1);
-  static constexpr int WPB = 4;  // waves per block
+  static constexpr int WPB = (L * 4 * 8 * 4 <= 65536) ? 4 : 2;  // waves per block (LDS <= 64 KB)
 };
 
 template <int LMAX>
@@ -207,14 +189,21 @@ __global__ void __launch_bounds__(256) k_synth2(AtomTables at, const double *__r
                                                 double *__restrict__ contrib) {
   constexpr int L = Synth2Cfg<LMAX>::L;
   constexpr int WPB = Synth2Cfg<LMAX>::WPB;
+  __shared__ double srow[WPB][L * 4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const long wid = (long)blockIdx.x * WPB + wave;
-  if (wid >= nwaves) return;
-  const int gb = __builtin_amdgcn_readfirstlane(wave_bucket[wid]);
+  const bool active = wid < nwaves;
+  const int gb = active ? __builtin_amdgcn_readfirstlane(wave_bucket[wid]) : 0;
   const int A = gb / (at.max_nrad + 1), loc = gb - A * (at.max_nrad + 1);
   const int nr = at.nrad[A];
   const int reg = (loc == 0) ? 0 : (loc == nr ? 2 : 1);
   const int j = (reg == 1) ? loc - 1 : 0;
+  if (active && reg == 1) {
+    const double *row = T + ((long)A * (at.max_nrad - 1) + j) * (long)L * 4;
+    for (int e = lane; e < L * 4; e += 64) srow[wave][e] = row[e];
+  }
+  __syncthreads();
+  if (!active) return;
   const int p = list[wid * 64 + lane];
   const long pp = p >= 0 ? p : 0;
   const double dx = px[pp] - at.center[3 * A], dy = py[pp] - at.center[3 * A + 1],
@@ -224,9 +213,8 @@ __global__ void __launch_bounds__(256) k_synth2(AtomTables at, const double *__r
   const double ux = dx * inv_d, uy = dy * inv_d, uz = dz * inv_d;
   double acc;
   if (reg == 1) {
-    const double *row = T + ((long)A * (at.max_nrad - 1) + j) * (long)L * 4;
     const double u = rtf_inv(at.rkind[A], at.rp0[A], at.rp1[A], nr, d) - j;
-    acc = synth_harmonics<LMAX, 1>(row, u, ux, uy, uz, d, inv_d);
+    acc = synth_harmonics<LMAX, 1>(srow[wave], u, ux, uy, uz, d, inv_d);
   } else if (reg == 0) {
     acc = synth_harmonics<LMAX, 0>(ampL + (long)A * L, 0.0, ux, uy, uz, d, inv_d);
   } else {

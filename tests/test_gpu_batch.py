@@ -111,3 +111,22 @@ def test_autotuner_choice_and_cache(tmp_path):
         assert np.abs(F1 - F0).max() < tol * np.abs(F0).max()
         setting2, src2 = autotune.tune(f, dP[0].ptr, dF[0].ptr, dE[0].ptr, path=path)
         assert src2 == 'cache' and setting2 == setting
+
+
+def test_every_launch_shape_gives_the_same_build():
+    # all autotuner candidates (Fock workgroup counts, one-pass vs per-tile BF16x3, density
+    # stage depth 16 / 32 and 1 / 2 stages in flight) on one molecule: rho and the energy are
+    # identical to rounding, F within each precision's tolerance
+    from napmo_amd import autotune, workloads
+    mols = workloads.batch_molecules(count=1, seed=9)
+    for prec in ('fp64', 'bf16x3'):
+        grids, bases, plans, Ps = _setup(mols, prec)
+        f = plans[0]
+        F0, E0, Eg0 = f.build(Ps[0])
+        tol = 1e-12 if prec == 'fp64' else 2e-5
+        for cand in autotune.candidates(f):
+            autotune.apply(f, dict(autotune.DEFAULTS, **cand))
+            F1, E1, Eg1 = f.build(Ps[0])
+            assert abs(Eg1 - Eg0) < 1e-12 * abs(Eg0), cand
+            assert np.abs(F1 - F0).max() < tol * np.abs(F0).max(), cand
+        print(prec, len(autotune.candidates(f)), 'launch shapes agree')
