@@ -40,7 +40,8 @@ def test_kinetic_matches_oracle_and_closed_form():
             sc = max(np.abs(y).max(), 1e-300)
             worst_l[l] = max(worst_l[l], np.abs(yg[:, lm] - y).max() / sc)
             worst_rad[l] = max(worst_rad[l], np.abs(yg[:, lm] - yr).max() / sc)
-            worst_rho[l] = max(worst_rho[l], np.abs(rg[:, lm] - sph[:, lm]).max() / max(np.abs(sph[:, lm]).max(), 1e-300))
+            # relative to the size of rho_00: the l >= 1 components are small sums that cancel
+            worst_rho[l] = max(worst_rho[l], np.abs(rg[:, lm] - sph[:, lm]).max() / np.abs(sph[:, 0]).max())
         off += r.size
     print('kinetic expansion worst rel per l', worst_l)
     print('kinetic projection worst rel per l', worst_rho)
@@ -51,12 +52,12 @@ def test_kinetic_matches_oracle_and_closed_form():
     print('kinetic max|dT| %.3e max|T| %.3e at point %d (distances %s) T %.12e ref %.12e' %
           (np.abs(T - ref).max(), np.abs(ref).max(), i,
            [round(float(np.linalg.norm(o.points[i] - np.array(c))), 6) for _, c in atoms], T[i], ref[i]))
-    # The projection agrees to rounding (1e-12 relative for the small l >= 2 components,
-    # whose angular sums cancel) and the radial step applied to the same projection agrees
-    # to 1e-9. The finite-difference Laplacian of the reference (deriv2 - l(l+1)/r^2 near
+    # The projection agrees to rounding (relative to rho_00; the small l >= 2 components,
+    # whose angular sums cancel, carry 1e-12..1e-10 of their own size) and the radial step
+    # applied to the same projection agrees to 1e-9. The finite-difference Laplacian of the reference (deriv2 - l(l+1)/r^2 near
     # r -> 0) amplifies the projection's rounding about 1e5-fold, so the full operator
     # agrees to ~1e-6 of max|T|, worst at points within the first radial node of a nucleus.
-    assert np.max(worst_rho) < 1e-11
+    assert np.max(worst_rho) < 1e-13
     assert np.max(worst_rad) < 1e-9
     assert np.max(np.abs(T - ref)) < 1e-5 * np.abs(ref).max()
     # one centre, fine radial grid: -1/2 laplacian of a Gaussian in closed form
