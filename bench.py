@@ -172,7 +172,7 @@ def apmo_leg(args, log):
 def batch_leg(args, log):
     """64 independent molecules (2-12 atoms, uneven 35-110 x 110-590 grids, 20-200
     functions): one step = one grid Fock build of every molecule, launched as one batch
-    over 4 streams and replayed from a captured HIP graph. FP64 and BF16x3, launch shapes
+    over --lanes streams and replayed from a captured HIP graph. FP64 and BF16x3, launch shapes
     from the autotuner; per-molecule |dE| of the mixed build against FP64."""
     import napmo_amd
     from napmo_amd import autotune, workloads
@@ -207,7 +207,7 @@ def batch_leg(args, log):
             _, src = autotune.tune(f, a.ptr, b_.ptr, c.ptr)
             sources[src] = sources.get(src, 0) + 1
         log('batch %s: plans + tuning %.1fs (%s)' % (prec, time.perf_counter() - t0, sources))
-        batch = napmo_amd.BatchFock(plans, nlanes=4)
+        batch = napmo_amd.BatchFock(plans, nlanes=args.lanes)
         ptr = lambda xs: [x.ptr for x in xs]  # noqa: E731
         for _ in range(args.warmup):
             batch.build(ptr(dP), ptr(dF), ptr(dE))
@@ -246,8 +246,8 @@ def batch_leg(args, log):
         'scaling': 'weak', 'vs_baseline': None, 'dtype': 'bf16x3+f64' if best == 'bf16x3' else 'f64',
         'data': 'synthetic (seeded molecules, bases and densities)',
         'config': {'workload': 'batch of 64 molecules, 2-12 atoms, uneven 35-110 x 110-590 grids '
-                               '(%d points), 20-200 functions, HIP-graph replay over 4 streams'
-                               % sum(g.size for g in grids), 'precision': best},
+                               '(%d points), 20-200 functions, HIP-graph replay over %d streams'
+                               % (sum(g.size for g in grids), args.lanes), 'precision': best},
         'modes': {p: {'ms_per_batch_eager': r['ms_eager'], 'ms_per_batch_graph': r['ms_graph']}
                   for p, r in out.items()},
         'per_molecule_dE_grid_vs_fp64': dEg.tolist(), 'per_molecule_dE_trace_vs_fp64': dEt.tolist(),
@@ -395,6 +395,7 @@ def main():
                     help='comma list of modes (%s); fp64 is always timed' % ', '.join(MODES))
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-tune', action='store_true', help='keep the default launch shapes')
+    ap.add_argument('--lanes', type=int, default=8, help='HIP streams of the batch64 workload')
     ap.add_argument('--copy-calibration', type=float, default=0.0, metavar='GIB',
                     help='only run a plain device copy of GIB gibibytes 5 times (counter calibration)')
     ap.add_argument('--cpu-seconds', type=float, default=30.0)
