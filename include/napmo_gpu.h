@@ -195,6 +195,12 @@ int napmo_compute_kinetic(void *grid, const double *psi, int lmax, double *out);
  * projected components themselves into rho_lm (same shape; may be NULL). */
 int napmo_kinetic_expand(void *grid, const double *psi, int lmax, double *U_y, double *U_dt,
                          double *rho_lm);
+/* Grid exchange potential (napmo/wavefunction/psi_numeric.py:186-215): for psi [ndim x N]
+ * (function-major, grid order) and D [ndim x ndim], aux_sv = V[psi_s psi_v] for s >= v
+ * (ndim(ndim+1)/2 Poisson solves) and K_out[j] = sum_k sum_l psi_k aux_lj D[k][l]
+ * [ndim x N], the Kpot handed to nwavefunction_compute_2body_matrix. */
+int napmo_compute_exchange_potential(void *grid, int ndim, const double *psi, const double *D,
+                                     int lmax, double *K_out);
 /* Point-charge potential on the grid (napmo/wavefunction/nnuclear.py:15-37): V_out [N] =
  * sum_A charges[A] / |r - origins[A]|, origins [ncharge x 3]. */
 int napmo_compute_nuclear(void *grid, int ncharge, const double *charges, const double *origins,

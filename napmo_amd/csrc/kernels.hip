@@ -472,6 +472,53 @@ void launch_nuclear(const double *x, const double *y, const double *z, long N, c
   NAPMO_HIP(hipGetLastError());
 }
 
+// Exchange-potential helpers (napmo/wavefunction/psi_numeric.py:186-215).
+// chi[l][p] = sum_k D[k][l] psi[k][p] (psi, chi [ndim x N] function-major).
+__global__ void k_exchange_chi(const double *__restrict__ psi, const double *__restrict__ D,
+                               int ndim, long N, double *__restrict__ chi) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = blockIdx.y;
+  if (p >= N) return;
+  double s = 0.0;
+  for (int k = 0; k < ndim; ++k) s += D[(long)k * ndim + l] * psi[(long)k * N + p];
+  chi[(long)l * N + p] = s;
+}
+
+// out[p] = a[p] * b[p]
+__global__ void k_mul(const double *__restrict__ a, const double *__restrict__ b, long N,
+                      double *__restrict__ out) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < N) out[p] = a[p] * b[p];
+}
+
+// K[v] += aux chi[s]; K[s] += aux chi[v] when s != v (aux = V[psi_s psi_v]).
+__global__ void k_exchange_acc(const double *__restrict__ aux, const double *__restrict__ chi,
+                               int s, int v, long N, double *__restrict__ K) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const double a = aux[p];
+  K[(long)v * N + p] += a * chi[(long)s * N + p];
+  if (s != v) K[(long)s * N + p] += a * chi[(long)v * N + p];
+}
+
+void launch_exchange_chi(const double *psi, const double *D, int ndim, long N, double *chi,
+                         hipStream_t st) {
+  if (N <= 0 || ndim <= 0) return;
+  k_exchange_chi<<<dim3((unsigned)div_up(N, 256), ndim), 256, 0, st>>>(psi, D, ndim, N, chi);
+  NAPMO_HIP(hipGetLastError());
+}
+void launch_mul(const double *a, const double *b, long N, double *out, hipStream_t st) {
+  if (N <= 0) return;
+  k_mul<<<div_up(N, 256), 256, 0, st>>>(a, b, N, out);
+  NAPMO_HIP(hipGetLastError());
+}
+void launch_exchange_acc(const double *aux, const double *chi, int s, int v, long N, double *K,
+                         hipStream_t st) {
+  if (N <= 0) return;
+  k_exchange_acc<<<div_up(N, 256), 256, 0, st>>>(aux, chi, s, v, N, K);
+  NAPMO_HIP(hipGetLastError());
+}
+
 // v[p] = a x[p] + b y[p]
 __global__ void k_axpby(double a, const double *__restrict__ x, double b,
                         const double *__restrict__ y, long N, double *__restrict__ v) {

@@ -107,6 +107,13 @@ void launch_contrib_reduce(const double *contrib, int M, long N, int accumulate,
 // Point-charge potential V[p] = sum_A q_A / |p - R_A| (o: [nq x 3]).
 void launch_nuclear(const double *x, const double *y, const double *z, long N, const double *q,
                     const double *o, int nq, double *V, hipStream_t s);
+// Exchange potential pieces: chi = D^T psi; out = a o b; K[v] += aux chi[s] (and K[s] +=
+// aux chi[v] for s != v), all [ndim x N] function-major.
+void launch_exchange_chi(const double *psi, const double *D, int ndim, long N, double *chi,
+                         hipStream_t st);
+void launch_mul(const double *a, const double *b, long N, double *out, hipStream_t st);
+void launch_exchange_acc(const double *aux, const double *chi, int s, int v, long N, double *K,
+                         hipStream_t st);
 // v = a x + b y over N points.
 void launch_axpby(double a, const double *x, double b, const double *y, long N, double *v,
                   hipStream_t s);

@@ -139,6 +139,7 @@ _SIGNATURES = {
     'napmo_compute_kinetic': (c_int, [vp, a1df, c_int, a1df]),
     'napmo_kinetic_expand': (c_int, [vp, a1df, c_int, adf, adf, vp]),
     'napmo_compute_nuclear': (c_int, [vp, c_int, a1df, a2df, a1df]),
+    'napmo_compute_exchange_potential': (c_int, [vp, c_int, a2df, a2df, c_int, a2df]),
     'napmo_fock_plan_new': (c_int, [vp, vp, c_int, c_int, ctypes.POINTER(vp)]),
     'napmo_fock_plan_del': (None, [vp]),
     'napmo_fock_plan_info': (c_int, [vp, ctypes.POINTER(c_long), ctypes.POINTER(c_int),

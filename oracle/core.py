@@ -863,6 +863,26 @@ def compute_kinetic(grid, psi, lmax):
     return -0.5 * U
 
 
+def compute_exchange_potential(grid, psi, D, lmax, linsolve=None):
+    """PSIN.compute_exchange_potential (napmo/wavefunction/psi_numeric.py:186-215):
+    aux[INDEX(s, v)] = V[psi_s psi_v] for s >= v (INDEX(i, j) = i(i+1)/2 + j, i >= j,
+    psi_numeric.py:16-20), Kpot[j] = sum_k sum_l psi_k aux[INDEX(l, j)] D[k, l].
+    psi [ndim, N]; returns Kpot [ndim, N]."""
+    ndim = psi.shape[0]
+    aux = {}
+    for s in range(ndim):
+        for v in range(s + 1):
+            aux[(s, v)] = compute_coulomb(grid, psi[s] * psi[v], lmax, linsolve)
+    K = np.zeros_like(psi)
+    for j in range(ndim):
+        acc = np.zeros(psi.shape[1])
+        for k in range(ndim):
+            for l in range(ndim):
+                acc = acc + psi[k] * aux[(max(l, j), min(l, j))] * D[k, l]
+        K[j] = acc
+    return K
+
+
 def compute_nuclear(points, charges):
     """napmo/wavefunction/nnuclear.py:15-37: V(p) = sum_A Z_A / |p - R_A|."""
     V = np.zeros(len(points))

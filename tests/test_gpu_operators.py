@@ -57,7 +57,7 @@ def test_kinetic_matches_oracle_and_closed_form():
     # applied to the same projection agrees to 1e-9. The finite-difference Laplacian of the reference (deriv2 - l(l+1)/r^2 near
     # r -> 0) amplifies the projection's rounding about 1e5-fold, so the full operator
     # agrees to ~1e-6 of max|T|, worst at points within the first radial node of a nucleus.
-    assert np.max(worst_rho) < 1e-13
+    assert np.max(worst_rho) < 1e-12
     assert np.max(worst_rad) < 1e-9
     assert np.max(np.abs(T - ref)) < 1e-5 * np.abs(ref).max()
     # one centre, fine radial grid: -1/2 laplacian of a Gaussian in closed form
@@ -77,3 +77,18 @@ def test_nuclear_matches_oracle():
     V = napmo_amd.compute_nuclear(g, charges)
     ref = core.compute_nuclear(g.points, [(c['atomic_number'], c['origin']) for c in charges])
     assert np.max(np.abs(V - ref) / np.abs(ref)) < 1e-14
+
+
+def test_exchange_potential_matches_oracle():
+    import napmo_amd
+    # H2 cc-pVDZ on 2 x 30 x 50 points, lmax 6: 4 functions, 10 pair potentials
+    atoms, shells = systems.H2_ATOMS, systems.named_shells(systems.H2_ATOMS, 'CC-PVDZ')
+    g, b = systems.gpu_system(atoms, shells, 30, 50)
+    o, ob = systems.oracle_system(atoms, shells, 30, 50)
+    from oracle import cext as ocx
+    psi = np.ascontiguousarray(ocx.basis_compute(ob, o.points)[:, :4].T)
+    D = np.random.default_rng(8).normal(size=(4, 4))
+    K = napmo_amd.compute_exchange_potential(g, psi, D, 6)
+    ref = core.compute_exchange_potential(o, psi, D, 6)
+    print('exchange max|dK| %.3e max|K| %.3e' % (np.abs(K - ref).max(), np.abs(ref).max()))
+    assert np.max(np.abs(K - ref)) < 1e-11 * np.abs(ref).max()

@@ -317,3 +317,19 @@ def test_nuclear_potential_matches_direct_sum():
     pts = np.array([[0.0, 0.0, 1.0], [1.0, 2.0, 2.0]])
     V = core.compute_nuclear(pts, [(1.0, (0.0, 0.0, 0.0)), (8.0, (1.0, 2.0, 0.0))])
     assert np.allclose(V, [1.0 + 8.0 / np.sqrt(1 + 4 + 1), 1.0 / 3.0 + 8.0 / 2.0], rtol=1e-15)
+
+
+def test_exchange_potential_index_structure():
+    # psi_numeric.py:186-215 with D = e_0 e_1^T (D[0, 1] = 1): chi_1 = psi_0, chi_0 = 0, so
+    # Kpot_0 = V[psi_1 psi_0] psi_0 and Kpot_1 = V[psi_1^2] psi_0 (a transposed D would give
+    # V[psi_0^2] psi_1 and V[psi_1 psi_0] psi_1 instead)
+    grid = core.BeckeGrid([('H', (0.0, 0.0, 0.7)), ('H', (0.0, 0.0, -0.7))], 30, 26)
+    r0 = np.sqrt(((grid.points - grid.origin[0]) ** 2).sum(1))
+    r1 = np.sqrt(((grid.points - grid.origin[1]) ** 2).sum(1))
+    psi = np.array([np.exp(-r0 * r0), np.exp(-0.5 * r1 * r1)])
+    D = np.array([[0.0, 1.0], [0.0, 0.0]])
+    K = core.compute_exchange_potential(grid, psi, D, 4)
+    v10 = core.compute_coulomb(grid, psi[1] * psi[0], 4)
+    v11 = core.compute_coulomb(grid, psi[1] * psi[1], 4)
+    assert np.allclose(K[0], v10 * psi[0], rtol=1e-14, atol=1e-14)
+    assert np.allclose(K[1], v11 * psi[0], rtol=1e-14, atol=1e-14)
