@@ -205,6 +205,28 @@ int napmo_compute_exchange_potential(void *grid, int ndim, const double *psi, co
  * sum_A charges[A] / |r - origins[A]|, origins [ncharge x 3]. */
 int napmo_compute_nuclear(void *grid, int ncharge, const double *charges, const double *origins,
                           double *V_out);
+/* Becke-Dickson orbital correction (napmo/wavefunction/ndpsi.py:15-89, compute_dpsi), for
+ * norb orbitals at once: phi, ri, V_tot [norb x N] (grid order), doi, oi [norb]; dpsi_out
+ * [norb x N]. Per atom and (l, m) it solves u'' + (2/r) u' + a u = -2/m_inv mu_lm with
+ * a = -2/m_inv (l(l+1)/r^2 + V_sph - oi - doi), u = 0 at both radial ends, mu the spherical
+ * expansion of (phi doi - ri) bw and V_sph the spherical average of V_tot bw; the solutions
+ * are synthesised at every point. Radial grids of 3 to 240 points. */
+int napmo_compute_dpsi(void *grid, int lmax, int norb, const double *phi, const double *doi,
+                       const double *oi, const double *ri, const double *V_tot, double mass_inv,
+                       double *dpsi_out);
+/* The per-orbital steps of PSIO_BECKE.optimize (napmo/wavefunction/psi_optimization.py
+ * :175-203): R = T + V_tot - e_prev psi (Rgrid), de = integrate(R psi) (delta_e), then
+ * dpsi = compute_dpsi(psi, de, e_prev, R, V_tot). psi, T, V_tot [norb x N]; outputs dpsi
+ * [norb x N], de [norb], R [norb x N] (R_out may be NULL). */
+int napmo_becke_dickson_step(void *grid, int lmax, int norb, const double *psi, const double *T,
+                             const double *V_tot, const double *e_prev, double mass_inv,
+                             double *dpsi_out, double *de_out, double *R_out);
+/* Same on device pointers (all arrays resident on the grid's device, R_out required);
+ * returns after the step has finished. */
+int napmo_becke_dickson_step_device(void *grid, int lmax, int norb, const double *psi,
+                                    const double *T, const double *V_tot, const double *e_prev,
+                                    double mass_inv, double *dpsi_out, double *de_out,
+                                    double *R_out);
 
 /* Grid Fock plan: device-resident Phi (collocated once), Poisson plan, workspace. */
 int napmo_fock_plan_new(void *grid, void *basis, int lmax, int prec_mode, void **plan_out);

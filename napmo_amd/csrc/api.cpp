@@ -566,6 +566,55 @@ int napmo_compute_exchange_potential(void *grid, int ndim, const double *psi, co
   });
 }
 
+int napmo_compute_dpsi(void *grid, int lmax, int norb, const double *phi, const double *doi,
+                       const double *oi, const double *ri, const double *V_tot, double mass_inv,
+                       double *dpsi_out) {
+  return guarded([&] {
+    DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    NAPMO_REQUIRE(norb >= 0, "napmo_compute_dpsi: negative orbital count");
+    if (norb == 0) return;
+    const size_t n = (size_t)norb * g.N;
+    DeviceBuffer<double> dphi, dri, dV, ddoi, doi_, out(n);
+    dphi.upload(phi, n, g.stream), dri.upload(ri, n, g.stream), dV.upload(V_tot, n, g.stream);
+    ddoi.upload(doi, norb, g.stream), doi_.upload(oi, norb, g.stream);
+    g.orbital_correction(lmax, norb, dphi.get(), ddoi.get(), doi_.get(), dri.get(), dV.get(),
+                         mass_inv, out.get());
+    out.download(dpsi_out, n, g.stream);
+    NAPMO_HIP(hipStreamSynchronize(g.stream));
+  });
+}
+
+int napmo_becke_dickson_step(void *grid, int lmax, int norb, const double *psi, const double *T,
+                             const double *V_tot, const double *e_prev, double mass_inv,
+                             double *dpsi_out, double *de_out, double *R_out) {
+  return guarded([&] {
+    DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    NAPMO_REQUIRE(norb >= 0, "napmo_becke_dickson_step: negative orbital count");
+    if (norb == 0) return;
+    const size_t n = (size_t)norb * g.N;
+    DeviceBuffer<double> dpsi, dT, dV, de, dout(n), dde(norb), dR(n);
+    dpsi.upload(psi, n, g.stream), dT.upload(T, n, g.stream), dV.upload(V_tot, n, g.stream);
+    de.upload(e_prev, norb, g.stream);
+    g.becke_dickson_step(lmax, norb, dpsi.get(), dT.get(), dV.get(), de.get(), mass_inv,
+                         dout.get(), dde.get(), dR.get());
+    dout.download(dpsi_out, n, g.stream);
+    dde.download(de_out, norb, g.stream);
+    if (R_out) dR.download(R_out, n, g.stream);
+    NAPMO_HIP(hipStreamSynchronize(g.stream));
+  });
+}
+
+int napmo_becke_dickson_step_device(void *grid, int lmax, int norb, const double *psi,
+                                    const double *T, const double *V_tot, const double *e_prev,
+                                    double mass_inv, double *dpsi_out, double *de_out,
+                                    double *R_out) {
+  return guarded([&] {
+    DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    NAPMO_REQUIRE(R_out != nullptr, "napmo_becke_dickson_step_device: R_out is required");
+    g.becke_dickson_step(lmax, norb, psi, T, V_tot, e_prev, mass_inv, dpsi_out, de_out, R_out);
+  });
+}
+
 int napmo_compute_nuclear(void *grid, int ncharge, const double *charges, const double *origins,
                           double *V_out) {
   return guarded([&] {

@@ -1,0 +1,293 @@
+// Becke-Dickson orbital correction on the GPU (napmo/wavefunction/ndpsi.py:15-89 and the
+// per-orbital steps of napmo/wavefunction/psi_optimization.py:175-203).
+//
+// Per orbital o, atom A and angular momentum l the correction solves, on A's radial grid,
+//   u'' + (2/r) u' + a u = c mu_lm,  a = c (l(l+1)/r^2 + Vsph - oi - doi),  c = -2/m_inv,
+// with u(r_0) = u(r_max) = 0, for the 2l+1 projected sources mu_lm of (phi doi - ri) bw.
+// a depends on the orbital, so unlike the Poisson solve the matrix cannot be factored once
+// per geometry: one wavefront per (A, l, o) assembles the Hermite finite-element matrix
+// from per-grid tables, factors it as a band matrix with partial pivoting and solves the
+// 2l+1 right-hand sides, one lane per right-hand side.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace napmo {
+
+namespace {
+
+// out[o][p] = a[o][p] s[o] - b[o][p]
+__global__ void k_scaled_diff(const double *__restrict__ a, const double *__restrict__ s,
+                              const double *__restrict__ b, long N, double *__restrict__ out) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int o = blockIdx.y;
+  if (p < N) out[(long)o * N + p] = a[(long)o * N + p] * s[o] - b[(long)o * N + p];
+}
+
+// One block per (shell i, atom A, orbital o): sum_j (f bw) w / sum_j w.
+__global__ void __launch_bounds__(64) k_shell_average(AtomTables at, const double *__restrict__ V,
+                                                      long N, const double *__restrict__ w,
+                                                      const double *__restrict__ bw,
+                                                      long nrad_total, double *__restrict__ Vsph) {
+  const int i = blockIdx.x, A = blockIdx.y, o = blockIdx.z;
+  if (i >= at.nrad[A]) return;
+  const int na = at.nang[A];
+  const long p0 = at.point_off[A] + (long)i * na;
+  const double *f = V + (long)o * N;
+  double num = 0.0, den = 0.0;
+  for (int j = threadIdx.x; j < na; j += 64) {
+    const long p = p0 + j;
+    num += (f[p] * bw[p]) * w[p];
+    den += w[p];
+  }
+  for (int k = 32; k > 0; k >>= 1) {
+    num += __shfl_xor(num, k);
+    den += __shfl_xor(den, k);
+  }
+  if (threadIdx.x == 0) Vsph[(long)o * nrad_total + at.rad_off[A] + i] = num / den;
+}
+
+// Slopes of the C2 cubic spline through y (the (2, 4, ..., 4, 2 | 1) system of
+// src/horton/cubic_spline.cpp:54-103, same elimination order); up, rhs: n scratch each.
+__device__ void spline_slopes(const double *y, double *dt, int n, double *up, double *rhs) {
+  for (int i = 0; i < n - 1; ++i) up[i] = 1.0;
+  rhs[0] = 3.0 * (y[1] - y[0]);
+  for (int i = 1; i < n - 1; ++i) rhs[i] = 3.0 * (y[i + 1] - y[i - 1]);
+  rhs[n - 1] = 3.0 * (y[n - 1] - y[n - 2]);
+  double tmp1 = up[0], tmp2 = 0.0;
+  up[0] /= 2.0;
+  rhs[0] /= 2.0;
+  for (int i = 1; i < n; ++i) {
+    const double den = (i == n - 1 ? 2.0 : 4.0) - up[i - 1] * tmp1;
+    if (i < n - 1) {
+      tmp2 = up[i];
+      up[i] /= den;
+    }
+    rhs[i] = (rhs[i] - rhs[i - 1] * tmp1) / den;
+    tmp1 = tmp2;
+  }
+  dt[n - 1] = rhs[n - 1];
+  for (int i = n - 2; i >= 0; --i) dt[i] = rhs[i] - up[i] * dt[i + 1];
+}
+
+// Band storage: row k holds columns k - 3 .. k + 6 in slots 0 .. 9 (slot = col - row + 3);
+// after pivoting the upper band grows to k + 6. L multipliers of column k in Lf[k][0..2].
+constexpr int WSLOT = 10;
+
+__global__ void __launch_bounds__(64) k_helmholtz_radial(
+    AtomTables at, HelmholtzTables ht, int L, long nrad_total, const double *__restrict__ Vsph,
+    const double *__restrict__ oi, const double *__restrict__ doi, double mass_inv,
+    const double *__restrict__ mu, int mc, double *__restrict__ Uy, double *__restrict__ Udt,
+    int *__restrict__ status) {
+  extern __shared__ double sm[];
+  const int A = blockIdx.x, l = blockIdx.y, o = blockIdx.z;
+  const int lane = threadIdx.x;
+  const int nr = at.nrad[A], nfn = 2 * nr, nm = 2 * l + 1;
+  double *W = sm;                   // [nfn][WSLOT]
+  double *Lf = W + nfn * WSLOT;     // [nfn][3]
+  double *ay = Lf + nfn * 3;        // [nr]
+  double *ad = ay + nr;             // [nr]
+  double *an = ad + nr;             // [nfn]
+  double *X = an + nfn;             // [nfn][mc], also the slope solver's scratch
+  int *ipiv = (int *)(X + (long)nfn * mc);
+  const long r0 = at.rad_off[A];
+  const double c = -2.0 / mass_inv;
+  const double e_o = oi[o], d_o = doi[o];
+  const double *rr = at.r + r0, *j1 = ht.j1 + r0, *j2 = ht.j2 + r0;
+
+  // a = c (l(l+1)/r^2 + Vsph - oi - doi), its spline slopes, then the t-axis transform
+  // a_y j1^2, (a_x j1^2 + 2 a_y j2) j1 (napmo/utilities/ode2.py:35-118)
+  for (int i = lane; i < nr; i += 64) {
+    const double r2 = rr[i] * rr[i];
+    const double r2inv = 1.0 / r2;
+    ay[i] = c * ((l * (l + 1.0) * r2inv) + (Vsph[(long)o * nrad_total + r0 + i] - e_o - d_o));
+  }
+  __syncthreads();
+  if (lane == 0) spline_slopes(ay, ad, nr, X, X + nr);
+  __syncthreads();
+  for (int i = lane; i < nr; i += 64) {
+    const double j1sq = j1[i] * j1[i];
+    const double ax = ad[i] / j1[i];
+    an[2 * i] = ay[i] * j1sq;
+    an[2 * i + 1] = (ax * j1sq + 2 * ay[i] * j2[i]) * j1[i];
+  }
+  __syncthreads();
+
+  // Galerkin matrix: c0 + sum_k a_k Oa[k] on the band; rows 0 and nfn - 2 are u = 0
+  const double *C0 = ht.c0 + ht.c0_off[A], *OA = ht.oa + ht.oa_off[A];
+  for (int row = lane; row < nfn; row += 64) {
+    double *wr = W + row * WSLOT;
+    for (int s = 0; s < WSLOT; ++s) wr[s] = 0.0;
+    if (row == 0 || row == nfn - 2) {
+      wr[3] = 1.0;
+      continue;
+    }
+    const int begin = max(2 * (row / 2 - 1), 0), end = min(2 * (row / 2 + 2), nfn);
+    for (int off = -3; off <= 3; ++off) {
+      const int col = row + off;
+      if (col < begin || col >= end) continue;
+      double v = C0[row * 7 + off + 3];
+      const double *oa = OA + (row * 7 + off + 3) * 6;
+      for (int kk = 0; kk < 6 && begin + kk < end; ++kk) v += an[begin + kk] * oa[kk];
+      wr[off + 3] = v;
+    }
+  }
+  __syncthreads();
+
+  // band LU with partial pivoting (pivot among rows k .. k + 3)
+  for (int k = 0; k < nfn; ++k) {
+    int p = k;
+    double best = fabs(W[k * WSLOT + 3]);
+    for (int d = 1; d <= 3 && k + d < nfn; ++d) {
+      const double v = fabs(W[(k + d) * WSLOT + 3 - d]);
+      if (v > best) best = v, p = k + d;
+    }
+    if (lane == 0) {
+      ipiv[k] = p;
+      if (best == 0.0) status[0] = 1;
+    }
+    if (p != k && lane < 7) {
+      const int j = k + lane;
+      if (j < nfn) {
+        double *a = W + k * WSLOT + lane + 3, *b = W + p * WSLOT + (j - p + 3);
+        const double t = *a;
+        *a = *b;
+        *b = t;
+      }
+    }
+    __syncthreads();
+    const double piv = W[k * WSLOT + 3];
+    if (lane < 18) {
+      const int d = 1 + lane / 6, jj = 1 + lane % 6, i = k + d, j = k + jj;
+      if (i < nfn && j < nfn) {
+        const double f = W[i * WSLOT + 3 - d] / piv;
+        W[i * WSLOT + j - i + 3] -= f * W[k * WSLOT + jj + 3];
+      }
+    } else if (lane < 21) {
+      const int d = lane - 17;
+      Lf[k * 3 + d - 1] = (k + d < nfn) ? W[(k + d) * WSLOT + 3 - d] / piv : 0.0;
+    }
+    __syncthreads();
+  }
+
+  // right-hand sides in chunks of mc columns; lane q solves column q
+  const double *R = ht.rmap + ht.rmap_off[A];
+  const long ubase = ((long)o * nrad_total + r0) * L + l * l;
+  for (int m0 = 0; m0 < nm; m0 += mc) {
+    const int cnt = min(mc, nm - m0);
+    for (int e = lane; e < nfn * cnt; e += 64) {
+      const int row = e / cnt, q = e - row * cnt;
+      const double *rrow = R + (long)row * nr;
+      double acc = 0.0;
+      for (int j = 0; j < nr; ++j) acc += rrow[j] * mu[ubase + (long)j * L + m0 + q];
+      X[row * mc + q] = c * acc;
+    }
+    __syncthreads();
+    if (lane < cnt) {
+      const int q = lane;
+      for (int k = 0; k < nfn; ++k) {
+        const int p = ipiv[k];
+        if (p != k) {
+          const double t = X[k * mc + q];
+          X[k * mc + q] = X[p * mc + q];
+          X[p * mc + q] = t;
+        }
+        const double xk = X[k * mc + q];
+        for (int d = 1; d <= 3 && k + d < nfn; ++d) X[(k + d) * mc + q] -= Lf[k * 3 + d - 1] * xk;
+      }
+      for (int k = nfn - 1; k >= 0; --k) {
+        double s = X[k * mc + q];
+        for (int jj = 1; jj <= 6 && k + jj < nfn; ++jj) s -= W[k * WSLOT + jj + 3] * X[(k + jj) * mc + q];
+        X[k * mc + q] = s / W[k * WSLOT + 3];
+      }
+      // solution spline: y = u, dt = (u_t / j1) j1 (the reference's dx round trip)
+      for (int i = 0; i < nr; ++i) {
+        Uy[ubase + (long)i * L + m0 + q] = X[(2 * i) * mc + q];
+        Udt[ubase + (long)i * L + m0 + q] = (X[(2 * i + 1) * mc + q] / j1[i]) * j1[i];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// R = T + V - e psi; per-block partial sums of R psi w bw, then a fixed-order final sum.
+__global__ void __launch_bounds__(256) k_residual(const double *__restrict__ psi,
+                                                  const double *__restrict__ T,
+                                                  const double *__restrict__ V,
+                                                  const double *__restrict__ e,
+                                                  const double *__restrict__ wbw, long N,
+                                                  double *__restrict__ R,
+                                                  double *__restrict__ work) {
+  __shared__ double red[4];
+  const int o = blockIdx.y;
+  const long base = (long)o * N;
+  const double eo = e[o];
+  double acc = 0.0;
+  for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < N; p += (long)gridDim.x * 256) {
+    const double r = T[base + p] + V[base + p] - eo * psi[base + p];
+    R[base + p] = r;
+    acc += (r * psi[base + p]) * wbw[p];
+  }
+  for (int k = 32; k > 0; k >>= 1) acc += __shfl_xor(acc, k);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) work[(long)o * gridDim.x + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+__global__ void k_residual_final(const double *__restrict__ work, int nblk, int norb,
+                                 double *__restrict__ de) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= norb) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += work[(long)o * nblk + b];
+  de[o] = s;
+}
+
+}  // namespace
+
+void launch_scaled_diff(const double *a, const double *s, const double *b, int norb, long N,
+                        double *out, hipStream_t st) {
+  if (norb <= 0 || N <= 0) return;
+  k_scaled_diff<<<dim3((unsigned)div_up(N, 256), norb), 256, 0, st>>>(a, s, b, N, out);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_shell_average(const AtomTables &at, const double *V, int norb, long N, const double *w,
+                          const double *bw, long nrad_total, double *Vsph, hipStream_t st) {
+  if (norb <= 0) return;
+  k_shell_average<<<dim3(at.max_nrad, at.M, norb), 64, 0, st>>>(at, V, N, w, bw, nrad_total, Vsph);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_helmholtz_radial(const AtomTables &at, const HelmholtzTables &ht, int lmax, int norb,
+                             long nrad_total, const double *Vsph, const double *oi,
+                             const double *doi, double mass_inv, const double *mu, double *Uy,
+                             double *Udt, int *status, hipStream_t st) {
+  if (norb <= 0) return;
+  NAPMO_REQUIRE(at.max_nrad <= HELMHOLTZ_MAX_NRAD,
+                "orbital correction: radial grids of more than " +
+                    std::to_string(HELMHOLTZ_MAX_NRAD) + " points are not supported");
+  NAPMO_REQUIRE(at.max_nrad >= 3, "orbital correction: radial grids need at least 3 points");
+  const int L = (lmax + 1) * (lmax + 1);
+  const long nfn = 2L * at.max_nrad;
+  const long fixed = nfn * (WSLOT + 3) + 2L * at.max_nrad + nfn;  // W, Lf, ay, ad, an
+  const long budget = 8192;                                        // doubles (64 KB)
+  long mc = (budget - fixed - nfn / 2 - 1) / nfn;                  // ipiv ints take nfn/2
+  mc = std::min<long>(mc, 2L * lmax + 1);
+  NAPMO_REQUIRE(mc >= 1, "orbital correction: radial system does not fit in LDS");
+  const size_t bytes = (size_t)(fixed + nfn * mc) * sizeof(double) + (size_t)nfn * sizeof(int);
+  k_helmholtz_radial<<<dim3(at.M, lmax + 1, norb), 64, bytes, st>>>(
+      at, ht, L, nrad_total, Vsph, oi, doi, mass_inv, mu, (int)mc, Uy, Udt, status);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_residual(const double *psi, const double *T, const double *V, const double *e,
+                     const double *wbw, int norb, long N, double *R, double *de, double *work,
+                     hipStream_t st) {
+  if (norb <= 0) return;
+  const int nblk = (int)std::max<long>(1, std::min<long>(RESID_BLOCKS, div_up(N, 256)));
+  k_residual<<<dim3(nblk, norb), 256, 0, st>>>(psi, T, V, e, wbw, N, R, work);
+  k_residual_final<<<div_up(norb, 64), 64, 0, st>>>(work, nblk, norb, de);
+  NAPMO_HIP(hipGetLastError());
+}
+
+}  // namespace napmo

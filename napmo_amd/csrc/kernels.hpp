@@ -118,6 +118,37 @@ void launch_exchange_acc(const double *aux, const double *chi, int s, int v, lon
 void launch_axpby(double a, const double *x, double b, const double *y, long N, double *v,
                   hipStream_t s);
 
+// ---- helmholtz.hip: the Becke-Dickson orbital correction (napmo/wavefunction/ndpsi.py)
+// Finite-element tables of one radial grid (helmholtz_tables in poisson_plan.cpp), per atom
+// offsets into concatenated arrays: c0 [2n x 7] band of the Galerkin matrix without the
+// a term, oa [2n x 7 x 6] a-term integrals, rmap [2n x n] right-hand side of f = mu,
+// j1 / j2 [sum nrad] (indexed like AtomTables::r).
+struct HelmholtzTables {
+  const double *c0, *oa, *rmap, *j1, *j2;
+  const long *c0_off, *oa_off, *rmap_off;
+};
+constexpr int HELMHOLTZ_MAX_NRAD = 240;  // LDS of one radial system (64 KB: 2n x 15 doubles + RHS)
+// out[o][p] = a[o][p] * s[o] - b[o][p] for norb rows of N points (s: device [norb]).
+void launch_scaled_diff(const double *a, const double *s, const double *b, int norb, long N,
+                        double *out, hipStream_t st);
+// Vsph[o][rad_off[A] + i] = sum_j f w bw / sum_j w over shell i of atom A (f = V[o]).
+void launch_shell_average(const AtomTables &at, const double *V, int norb, long N,
+                          const double *w, const double *bw, long nrad_total, double *Vsph,
+                          hipStream_t st);
+// Per (atom, l, orbital): assemble u'' + (2/r) u' + a u = c mu (c = -2/m_inv,
+// a = c (l(l+1)/r^2 + Vsph - oi - doi)), factor it (band LU, partial pivoting) and solve
+// the 2l+1 right-hand sides: Uy / Udt [norb][nrad_total x L]; status[0] = 1 on a zero pivot.
+void launch_helmholtz_radial(const AtomTables &at, const HelmholtzTables &ht, int lmax, int norb,
+                             long nrad_total, const double *Vsph, const double *oi,
+                             const double *doi, double mass_inv, const double *mu, double *Uy,
+                             double *Udt, int *status, hipStream_t st);
+// Residual R[o] = T[o] + V[o] - e[o] psi[o] and de[o] = sum_p R psi w bw (fixed order).
+// work: device scratch of norb * RESID_BLOCKS doubles.
+constexpr int RESID_BLOCKS = 256;
+void launch_residual(const double *psi, const double *T, const double *V, const double *e,
+                     const double *wbw, int norb, long N, double *R, double *de, double *work,
+                     hipStream_t st);
+
 // rho[p] = Phi_p^T P Phi_p with FP64 MFMA (P [ld x ld] zero padded). partial [ntile x Npad].
 void launch_density_f64(const double *phi, int ld, long Npad, const double *P, double *partial,
                         hipStream_t s, int ks = 16, int pf = 1);

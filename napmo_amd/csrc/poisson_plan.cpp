@@ -147,51 +147,26 @@ void fe_assemble(const double *b, const double *a, const double *f, const double
   }
 }
 
-void radial_green(const RTransform &rtf, const std::vector<double> &rw, int l,
-                  std::vector<double> &Gy, std::vector<double> &Gd) {
-  const int n = rtf.npoint, nfn = 2 * n;
-  const long xmax = n - 1;
-  std::vector<double> r(n), j1(n), j2(n), j3(n);
-  for (int i = 0; i < n; ++i) {
-    r[i] = rtf.radius(i);
-    j1[i] = rtf.deriv(i);
-    j2[i] = rtf.deriv2(i);
-    j3[i] = rtf.deriv3(i);
-  }
-  // b = 2/r (dx -2/r^2), a = -l(l+1)/r^2 (dx 2l(l+1)/r^3) transformed to the t-axis
-  // (napmo/utilities/ode2.py:75-90); merged [value, slope] node coefficients.
-  const double l2l = (double)l * (l + 1);
-  std::vector<double> b(nfn), a(nfn);
-  for (int i = 0; i < n; ++i) {
-    const double r2 = r[i] * r[i], r3 = r2 * r[i], j1sq = j1[i] * j1[i];
-    const double by = 2 / r[i], bd = -2 / r2, ay = -l2l / r2, ad = 2 * l2l / r3;
-    b[2 * i] = j1[i] * by - j2[i] / j1[i];
-    b[2 * i + 1] = j2[i] * by + j1sq * bd + (j2[i] * j2[i] - j1[i] * j3[i]) / j1sq;
-    a[2 * i] = ay * j1sq;
-    a[2 * i + 1] = (ad * j1sq + 2 * ay * j2[i]) * j1[i];
-  }
-  // Galerkin matrix; rows 0 and nfn-2 are the value boundary conditions at rmin, rmax.
-  std::vector<double> A((size_t)nfn * nfn, 0.0), zf(nfn, 0.0), zr(nfn, 0.0);
-  const double bcs[4] = {0.0, NAN, 0.0, NAN};
-  fe_assemble(b.data(), a.data(), zf.data(), bcs, A.data(), zr.data(), n);
-  // B: rho_j -> right-hand side. Spline slopes of rho: dt = Tinv rho (tridiagonal solve),
-  // f = -4 pi (rho, dt/j1), transformed: fy = f j1^2, fd = (fd j1^2 + 2 fy j2) j1.
-  std::vector<double> B((size_t)nfn * n, 0.0);
-  std::vector<double> e(n), dt(n);
-  const double pi4 = 4.0 * M_PI;
+// Right-hand-side map of build_ode2 for f = fscale (rho, rho_x) with rho_x the slopes of
+// rho's cubic spline (dt / j1), transformed to the t-axis (fy j1^2, (fx j1^2 + 2 fy j2) j1):
+// B [2n x n], column jc = unit rho_jc; rows 0 and 2n-2 (the value conditions) left zero.
+static void fe_rhs_map(int n, const std::vector<double> &j1, const std::vector<double> &j2,
+                       double fscale, std::vector<double> &B) {
+  const long nfn = 2L * n, xmax = n - 1;
+  B.assign((size_t)nfn * n, 0.0);
+  std::vector<double> e(n), dt(n), f(nfn);
   for (int jc = 0; jc < n; ++jc) {
     std::fill(e.begin(), e.end(), 0.0);
     e[jc] = 1.0;
     spline_derivatives(e.data(), dt.data(), n);
-    std::vector<double> f(nfn);
     for (int i = 0; i < n; ++i) {
-      const double fy = -pi4 * e[i], fdx = -pi4 * (dt[i] / j1[i]), j1sq = j1[i] * j1[i];
+      const double fy = fscale * e[i], fdx = fscale * (dt[i] / j1[i]), j1sq = j1[i] * j1[i];
       f[2 * i] = fy * j1sq;
       f[2 * i + 1] = (fdx * j1sq + 2 * fy * j2[i]) * j1[i];
     }
     for (long irow = 0; irow < nfn; ++irow) {
       if (irow == 0 || irow == nfn - 2) continue;
-      long begin = std::max(2 * (irow / 2 - 1), 0L), end = std::min(2 * (irow / 2 + 2), (long)nfn);
+      long begin = std::max(2 * (irow / 2 - 1), 0L), end = std::min(2 * (irow / 2 + 2), nfn);
       double acc = 0.0;
       for (long k = begin; k < end; ++k) {
         if (f[k] == 0) continue;
@@ -201,10 +176,56 @@ void radial_green(const RTransform &rtf, const std::vector<double> &rw, int l,
       }
       B[(size_t)irow * n + jc] = acc;
     }
-    // boundary values (napmo/grids/poisson_solver.py:92-105)
-    B[jc] = (e[jc] * std::pow(r[jc], -l - 1.0) * rw[jc]) * (std::pow(r[0], (double)l) / (2 * l + 1));
+  }
+}
+
+// r(t) and its t-derivatives at the nodes; b = 2/r (b_x = -2/r^2) merged [value, slope]
+// on the t-axis (napmo/utilities/ode2.py:75-90).
+static void radial_coeffs(const RTransform &rtf, std::vector<double> &r, std::vector<double> &j1,
+                          std::vector<double> &j2, std::vector<double> &j3, std::vector<double> &b) {
+  const int n = rtf.npoint;
+  r.resize(n), j1.resize(n), j2.resize(n), j3.resize(n), b.resize(2 * n);
+  for (int i = 0; i < n; ++i) {
+    r[i] = rtf.radius(i);
+    j1[i] = rtf.deriv(i);
+    j2[i] = rtf.deriv2(i);
+    j3[i] = rtf.deriv3(i);
+  }
+  for (int i = 0; i < n; ++i) {
+    const double r2 = r[i] * r[i], j1sq = j1[i] * j1[i];
+    const double by = 2 / r[i], bd = -2 / r2;
+    b[2 * i] = j1[i] * by - j2[i] / j1[i];
+    b[2 * i + 1] = j2[i] * by + j1sq * bd + (j2[i] * j2[i] - j1[i] * j3[i]) / j1sq;
+  }
+}
+
+void radial_green(const RTransform &rtf, const std::vector<double> &rw, int l,
+                  std::vector<double> &Gy, std::vector<double> &Gd) {
+  const int n = rtf.npoint, nfn = 2 * n;
+  std::vector<double> r, j1, j2, j3, b;
+  radial_coeffs(rtf, r, j1, j2, j3, b);
+  // a = -l(l+1)/r^2 (dx 2l(l+1)/r^3) transformed to the t-axis, merged [value, slope]
+  const double l2l = (double)l * (l + 1);
+  std::vector<double> a(nfn);
+  for (int i = 0; i < n; ++i) {
+    const double r2 = r[i] * r[i], r3 = r2 * r[i], j1sq = j1[i] * j1[i];
+    const double ay = -l2l / r2, ad = 2 * l2l / r3;
+    a[2 * i] = ay * j1sq;
+    a[2 * i + 1] = (ad * j1sq + 2 * ay * j2[i]) * j1[i];
+  }
+  // Galerkin matrix; rows 0 and nfn-2 are the value boundary conditions at rmin, rmax.
+  std::vector<double> A((size_t)nfn * nfn, 0.0), zf(nfn, 0.0), zr(nfn, 0.0);
+  const double bcs[4] = {0.0, NAN, 0.0, NAN};
+  fe_assemble(b.data(), a.data(), zf.data(), bcs, A.data(), zr.data(), n);
+  // B: rho_j -> right-hand side, f = -4 pi (rho, rho_x); then the boundary values
+  // (napmo/grids/poisson_solver.py:92-105)
+  std::vector<double> B;
+  fe_rhs_map(n, j1, j2, -4.0 * M_PI, B);
+  for (int jc = 0; jc < n; ++jc) {
+    const double e = 1.0;
+    B[jc] = (e * std::pow(r[jc], -l - 1.0) * rw[jc]) * (std::pow(r[0], (double)l) / (2 * l + 1));
     B[(size_t)(nfn - 2) * n + jc] =
-        (e[jc] * std::pow(r[jc], (double)l) * rw[jc]) * (1.0 / std::pow(r[n - 1], l + 1.0) / (2 * l + 1));
+        (e * std::pow(r[jc], (double)l) * rw[jc]) * (1.0 / std::pow(r[n - 1], l + 1.0) / (2 * l + 1));
   }
   lu_solve(A, B, nfn, n);
   Gy.assign((size_t)n * n, 0.0);
@@ -214,6 +235,35 @@ void radial_green(const RTransform &rtf, const std::vector<double> &rw, int l,
       Gy[(size_t)i * n + jc] = B[(size_t)(2 * i) * n + jc];
       Gd[(size_t)i * n + jc] = B[(size_t)(2 * i + 1) * n + jc];
     }
+}
+
+void helmholtz_tables(const RTransform &rtf, std::vector<double> &c0, std::vector<double> &oa,
+                      std::vector<double> &rmap, std::vector<double> &j1, std::vector<double> &j2) {
+  const int n = rtf.npoint, nfn = 2 * n;
+  const long xmax = n - 1;
+  std::vector<double> r, j3, b;
+  radial_coeffs(rtf, r, j1, j2, j3, b);
+  std::vector<double> A((size_t)nfn * nfn, 0.0), za(nfn, 0.0), zr(nfn, 0.0);
+  const double bcs[4] = {0.0, NAN, 0.0, NAN};
+  fe_assemble(b.data(), za.data(), za.data(), bcs, A.data(), zr.data(), n);
+  c0.assign((size_t)nfn * 7, 0.0);
+  oa.assign((size_t)nfn * 7 * 6, 0.0);
+  for (long row = 0; row < nfn; ++row) {
+    const bool cond = row == 0 || row == nfn - 2;
+    const long begin = std::max(2 * (row / 2 - 1), 0L), end = std::min(2 * (row / 2 + 2), (long)nfn);
+    for (int off = -3; off <= 3; ++off) {
+      const long col = row + off;
+      if (col < 0 || col >= nfn) continue;
+      c0[(size_t)row * 7 + off + 3] = A[(size_t)row * nfn + col];
+      if (cond || col < begin || col >= end) continue;
+      for (long k = begin; k < end; ++k) {
+        long idx[3] = {row, col, k};
+        int der[3] = {0, 0, 0};
+        oa[((size_t)row * 7 + off + 3) * 6 + (k - begin)] = fe_overlap(xmax, idx, der, 3);
+      }
+    }
+  }
+  fe_rhs_map(n, j1, j2, 1.0, rmap);
 }
 
 void kinetic_green(const RTransform &rtf, double radii, const std::vector<double> &r, int l,
@@ -457,6 +507,100 @@ void DeviceGrid::kinetic(const double *d_psi, int lmax, double *d_out) {
   launch_synth2(at, x.get(), y.get(), z.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves,
                 w.T.get(), w.ampL.get(), w.ampR.get(), lmax, sp.contrib.get(), stream);
   launch_contrib_reduce(sp.contrib.get(), M, N, 0, d_out, stream, -0.5);
+}
+
+HelmholtzTables HelmholtzPlan::tables() const {
+  HelmholtzTables t;
+  t.c0 = c0.get(), t.oa = oa.get(), t.rmap = rmap.get(), t.j1 = j1.get(), t.j2 = j2.get();
+  t.c0_off = c0_off.get(), t.oa_off = oa_off.get(), t.rmap_off = rmap_off.get();
+  return t;
+}
+
+HelmholtzPlan &DeviceGrid::helmholtz() {
+  if (hplan) return *hplan;
+  auto h = std::make_unique<HelmholtzPlan>();
+  std::vector<double> c0, oa, rm, j1all, j2all;
+  std::vector<long> c0o(M), oao(M), rmo(M);
+  std::vector<std::pair<std::vector<double>, int>> seen;  // (radial grid key, first atom)
+  for (int a = 0; a < M; ++a) {
+    const AtomInfo &at = atoms[a];
+    NAPMO_REQUIRE(at.nrad >= 3 && at.nrad <= HELMHOLTZ_MAX_NRAD,
+                  "orbital correction: radial grids need 3 to " +
+                      std::to_string(HELMHOLTZ_MAX_NRAD) + " points");
+    std::vector<double> tc0, toa, trm, tj1, tj2;
+    helmholtz_tables(at.rtf, tc0, toa, trm, tj1, tj2);
+    j1all.insert(j1all.end(), tj1.begin(), tj1.end());
+    j2all.insert(j2all.end(), tj2.begin(), tj2.end());
+    std::vector<double> key = {(double)at.rtf.kind, at.rtf.p0, at.rtf.p1, (double)at.nrad};
+    int first = -1;
+    for (auto &s : seen)
+      if (s.first == key) first = s.second;
+    if (first >= 0) {
+      c0o[a] = c0o[first], oao[a] = oao[first], rmo[a] = rmo[first];
+      continue;
+    }
+    seen.push_back({key, a});
+    c0o[a] = (long)c0.size(), oao[a] = (long)oa.size(), rmo[a] = (long)rm.size();
+    c0.insert(c0.end(), tc0.begin(), tc0.end());
+    oa.insert(oa.end(), toa.begin(), toa.end());
+    rm.insert(rm.end(), trm.begin(), trm.end());
+  }
+  h->c0.upload(c0, stream), h->oa.upload(oa, stream), h->rmap.upload(rm, stream);
+  h->j1.upload(j1all, stream), h->j2.upload(j2all, stream);
+  h->c0_off.upload(c0o, stream), h->oa_off.upload(oao, stream), h->rmap_off.upload(rmo, stream);
+  NAPMO_HIP(hipStreamSynchronize(stream));
+  hplan = std::move(h);
+  return *hplan;
+}
+
+void DeviceGrid::orbital_correction(int lmax, int norb, const double *d_phi, const double *d_doi,
+                                    const double *d_oi, const double *d_ri, const double *d_V,
+                                    double mass_inv, double *d_out) {
+  NAPMO_REQUIRE(norb >= 0, "orbital correction: negative orbital count");
+  NAPMO_REQUIRE(mass_inv != 0.0, "orbital correction: mass_inv must be non-zero");
+  if (norb == 0) return;
+  PoissonPlan &pl = plan(lmax);
+  HelmholtzPlan &hp = helmholtz();
+  AtomTables at = tables(pl);
+  SynthPairs &sp = synth_pairs();
+  const int L = (lmax + 1) * (lmax + 1);
+  long S = 0;
+  for (auto &a : atoms) S += a.nrad;
+  DeviceBuffer<double> src((size_t)norb * N), mu((size_t)norb * S * L), Uy((size_t)norb * S * L),
+      Udt((size_t)norb * S * L), Vsph((size_t)norb * S);
+  DeviceBuffer<int> status(1);
+  PoissonWork wk;
+  alloc_work(lmax, wk);
+  NAPMO_HIP(hipMemsetAsync(status.get(), 0, sizeof(int), stream));
+  // sources (phi doi - ri) bw projected per atom, V_sph = spherical average of V bw
+  launch_scaled_diff(d_phi, d_doi, d_ri, norb, N, src.get(), stream);
+  for (int o = 0; o < norb; ++o)
+    launch_project(at, src.get() + (long)o * N, bw.get(), pl.yhat.get(), lmax,
+                   mu.get() + (long)o * S * L, stream);
+  launch_shell_average(at, d_V, norb, N, w.get(), bw.get(), S, Vsph.get(), stream);
+  launch_helmholtz_radial(at, hp.tables(), lmax, norb, S, Vsph.get(), d_oi, d_doi, mass_inv,
+                          mu.get(), Uy.get(), Udt.get(), status.get(), stream);
+  for (int o = 0; o < norb; ++o) {
+    launch_synth_tables(at, Uy.get() + (long)o * S * L, Udt.get() + (long)o * S * L, lmax,
+                        wk.T.get(), wk.ampL.get(), wk.ampR.get(), stream);
+    launch_synth2(at, x.get(), y.get(), z.get(), N, sp.list.get(), sp.wave_bucket.get(),
+                  sp.nwaves, wk.T.get(), wk.ampL.get(), wk.ampR.get(), lmax, sp.contrib.get(), stream);
+    launch_contrib_reduce(sp.contrib.get(), M, N, 0, d_out + (long)o * N, stream);
+  }
+  int h_status = 0;
+  NAPMO_HIP(hipMemcpyAsync(&h_status, status.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
+  NAPMO_HIP(hipStreamSynchronize(stream));
+  NAPMO_REQUIRE(h_status == 0, "orbital correction: singular radial system (zero pivot)");
+}
+
+void DeviceGrid::becke_dickson_step(int lmax, int norb, const double *d_psi, const double *d_T,
+                                    const double *d_V, const double *d_e, double mass_inv,
+                                    double *d_dpsi, double *d_de, double *d_R) {
+  NAPMO_REQUIRE(norb >= 0, "Becke-Dickson step: negative orbital count");
+  if (norb == 0) return;
+  DeviceBuffer<double> work((size_t)norb * RESID_BLOCKS);
+  launch_residual(d_psi, d_T, d_V, d_e, wbw.get(), norb, N, d_R, d_de, work.get(), stream);
+  orbital_correction(lmax, norb, d_psi, d_de, d_e, d_R, d_V, mass_inv, d_dpsi);
 }
 
 void DeviceGrid::poisson_expand(const double *d_dens, int lmax) {

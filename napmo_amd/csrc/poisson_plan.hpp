@@ -34,6 +34,15 @@ void decomposition_on_points(CubicSpline **sp, int nspline, int lmax, const doub
 void kinetic_green(const RTransform &rtf, double radii, const std::vector<double> &r, int l,
                    std::vector<double> &Gy, std::vector<double> &Gd);
 
+// Finite-element tables of the orbital-correction equation u'' + (2/r) u' + a u = f on one
+// radial grid (napmo/wavefunction/ndpsi.py:52-81 through napmo/utilities/ode2.py:35-118 and
+// src/horton/ode2.cpp:232-321), with u = 0 at both ends: c0 [2n x 7] the Galerkin band
+// without the a term (slot = col - row + 3), oa [2n x 7 x 6] the integrals multiplying the
+// merged a coefficients begin(row) + k, rmap [2n x n] the right-hand side of f = mu, and
+// the t-derivatives j1, j2 of r(t).
+void helmholtz_tables(const RTransform &rtf, std::vector<double> &c0, std::vector<double> &oa,
+                      std::vector<double> &rmap, std::vector<double> &j1, std::vector<double> &j2);
+
 // Per-Lebedev-rule projection table Yhat[j][lm] = aw_j S_lm(u_j) sqrt(4 pi (2l+1)) / sum aw.
 std::vector<double> projection_table(const AngularGrid &ang, int lmax);
 
@@ -67,6 +76,13 @@ struct SynthPairs {
   DeviceBuffer<double> contrib;  // [M x N]
 };
 
+// Orbital-correction tables of every atom (one copy per distinct radial grid).
+struct HelmholtzPlan {
+  DeviceBuffer<double> c0, oa, rmap, j1, j2;
+  DeviceBuffer<long> c0_off, oa_off, rmap_off;
+  HelmholtzTables tables() const;
+};
+
 struct DeviceGrid {
   hipStream_t stream = nullptr;
   long N = 0;
@@ -84,6 +100,7 @@ struct DeviceGrid {
   int max_nrad = 0;
   std::map<int, std::unique_ptr<PoissonPlan>> plans, kplans;  // Poisson / kinetic, per lmax
   std::unique_ptr<SynthPairs> pairs;
+  std::unique_ptr<HelmholtzPlan> hplan;
 
   ~DeviceGrid();
   void build(const std::vector<AtomInfo> &atoms);
@@ -103,6 +120,18 @@ struct DeviceGrid {
   void poisson_expand(const double *d_dens, int lmax);
   void poisson_synthesize(int lmax, double *d_v);
   SynthPairs &synth_pairs();
+  HelmholtzPlan &helmholtz();
+  // Becke-Dickson orbital corrections (napmo/wavefunction/ndpsi.py:15-89) of norb orbitals,
+  // all device arrays [norb x N] (doi, oi: device [norb]): out[o] = compute_dpsi(phi[o],
+  // doi[o], oi[o], ri[o], V[o]). Synchronises; throws on a singular radial system.
+  void orbital_correction(int lmax, int norb, const double *d_phi, const double *d_doi,
+                          const double *d_oi, const double *d_ri, const double *d_V,
+                          double mass_inv, double *d_out);
+  // Residual R = T + V - e psi and de = integrate(R psi) of norb orbitals, then their
+  // corrections (napmo/wavefunction/psi_optimization.py:175-203).
+  void becke_dickson_step(int lmax, int norb, const double *d_psi, const double *d_T,
+                          const double *d_V, const double *d_e, double mass_inv, double *d_dpsi,
+                          double *d_de, double *d_R);
   // Atom arrays without a Poisson plan (enough for bucketing pairs).
   AtomTables geometry_tables() const;
   // Synthesis of this grid's expansion (plan(lmax), after poisson_expand / expand) at the

@@ -140,6 +140,10 @@ _SIGNATURES = {
     'napmo_kinetic_expand': (c_int, [vp, a1df, c_int, adf, adf, vp]),
     'napmo_compute_nuclear': (c_int, [vp, c_int, a1df, a2df, a1df]),
     'napmo_compute_exchange_potential': (c_int, [vp, c_int, a2df, a2df, c_int, a2df]),
+    'napmo_compute_dpsi': (c_int, [vp, c_int, c_int, vp, vp, vp, vp, vp, c_double, vp]),
+    'napmo_becke_dickson_step': (c_int, [vp, c_int, c_int, vp, vp, vp, vp, c_double, vp, vp, vp]),
+    'napmo_becke_dickson_step_device': (c_int, [vp, c_int, c_int, vp, vp, vp, vp, c_double, vp,
+                                                vp, vp]),
     'napmo_fock_plan_new': (c_int, [vp, vp, c_int, c_int, ctypes.POINTER(vp)]),
     'napmo_fock_plan_del': (None, [vp]),
     'napmo_fock_plan_info': (c_int, [vp, ctypes.POINTER(c_long), ctypes.POINTER(c_int),

@@ -883,6 +883,76 @@ def compute_exchange_potential(grid, psi, D, lmax, linsolve=None):
     return K
 
 
+def spherical_average(atgrid, f):
+    """AtomicGrid.spherical_average (napmo/grids/atomic.py:77-90): per radial shell,
+    sum_j f w / sum_j w, both sums in point order (src/grid/atomic_grid.cpp:53-80)."""
+    nr, na = atgrid.nrad, atgrid.nang
+    fw = (np.asarray(f, dtype=np.float64) * atgrid.weights).reshape(nr, na)
+    w = atgrid.weights.reshape(nr, na)
+    num = np.zeros(nr)
+    den = np.zeros(nr)
+    for j in range(na):
+        num += fw[:, j]
+        den += w[:, j]
+    return num / den
+
+
+def compute_dpsi(grid, lmax, phi, doi, oi, ri, V_tot, mass_inv, linsolve=None, evaluate=None):
+    """compute_dpsi (napmo/wavefunction/ndpsi.py:15-89), the Becke-Dickson orbital
+    correction: per atom, the spherical expansion mu_lm of (phi doi - ri) bw and, per
+    (l, m), the radial solution of
+
+        u'' + (2/r) u' + a u = f,   a = -2/m_inv (l(l+1)/r^2 + V_sph - oi - doi),
+        f = -2/m_inv mu_lm,   u(r_0) = u(r_max) = 0,
+
+    with V_sph the spherical average of V_tot bw; the (atom, l, m) solutions are splined
+    with PotentialExtrapolation(l) and synthesised at every grid point. ``a`` is a spline
+    built from its values (slopes from the spline system), as the reference builds it;
+    ``evaluate(splines, centre, points)`` may replace eval_decomposition."""
+    evaluate = evaluate or eval_decomposition
+    bw = grid.becke_weights
+    src = (phi * doi - ri) * bw
+    c = -2.0 / mass_inv
+    out = np.zeros(grid.size)
+    for k, ag in enumerate(grid.atgrids):
+        sl = slice(grid.offsets[k], grid.offsets[k + 1])
+        sph = spherical_expansion(ag, src[sl].copy(), lmax)
+        V_sph = spherical_average(ag, V_tot[sl] * bw[sl])
+        rtf = ag.rtf
+        t = np.arange(rtf.npoint, dtype=np.float64)
+        r = ag.r
+        j1, j2, j3 = rtf.deriv(t), rtf.deriv2(t), rtf.deriv3(t)
+        r2 = r * r
+        rinv = 1.0 / r
+        r2inv = 1.0 / r2
+        b = (2.0 * rinv, -2.0 * r2inv)
+        splines = []
+        idx = 0
+        for l in range(lmax + 1):
+            ay = c * ((l * (l + 1.0) * r2inv) + (V_sph - oi - doi))
+            a = (ay, spline_dt(ay) / j1)
+            for _m in range(-l, l + 1):
+                aux = np.array(sph[:, idx])
+                f = (c * aux, c * (spline_dt(aux) / j1))
+                splines.append(solve_ode2(b, a, f, (0.0, None, 0.0, None), rtf, j1, j2, j3,
+                                          PotentialExtrapolation(l), linsolve))
+                idx += 1
+        out += evaluate(splines, grid.origin[k], grid.points)
+    return out
+
+
+def becke_dickson_step(grid, lmax, psi, T, V_tot, e_prev, mass_inv, linsolve=None, evaluate=None):
+    """The per-orbital part of PSIO_BECKE.optimize (napmo/wavefunction/psi_optimization.py
+    :175-203): residual R_i = T_i + V_i - e_i psi_i, energy correction
+    de_i = integrate(R_i psi_i), correction dpsi_i = compute_dpsi(psi_i, de_i, e_i, R_i, V_i).
+    psi, T, V_tot [norb, N] (T and V_tot already applied to psi); returns (dpsi, de, R)."""
+    R = np.array([Ti + Vi - (e * p) for Ti, Vi, e, p in zip(T, V_tot, e_prev, psi)])
+    de = np.array([grid.integrate(r * p) for r, p in zip(R, psi)])
+    dpsi = np.array([compute_dpsi(grid, lmax, p, d, e, r, V, mass_inv, linsolve, evaluate)
+                     for p, d, e, r, V in zip(psi, de, e_prev, R, V_tot)])
+    return dpsi, de, R
+
+
 def compute_nuclear(points, charges):
     """napmo/wavefunction/nnuclear.py:15-37: V(p) = sum_A Z_A / |p - R_A|."""
     V = np.zeros(len(points))

@@ -333,3 +333,56 @@ def test_exchange_potential_index_structure():
     v11 = core.compute_coulomb(grid, psi[1] * psi[1], 4)
     assert np.allclose(K[0], v10 * psi[0], rtol=1e-14, atol=1e-14)
     assert np.allclose(K[1], v11 * psi[0], rtol=1e-14, atol=1e-14)
+
+
+def _dpsi_closed_form_inputs(grid, alpha, beta, oi, doi, mass_inv):
+    # target u(r) = r^2 exp(-alpha r^2) in the l = 0 and the l = 1 (z / r) channel, with
+    # V_tot = beta r^2 (its spherical average on one centre is itself). For each channel
+    #   u'' + 2 u'/r + c (l(l+1)/r^2 + V - E) u = c g_l,  c = -2 / m_inv,  E = oi + doi,
+    # and u'' + 2u'/r = (6 - 14 a r^2 + 4 a^2 r^4) exp(-a r^2) gives the source g_l.
+    c = -2.0 / mass_inv
+    p = grid.points
+    r = np.sqrt((p ** 2).sum(1))
+    e = np.exp(-alpha * r * r)
+    lap = (6 - 14 * alpha * r * r + 4 * alpha * alpha * r ** 4) * e
+    V = beta * r * r
+    E = oi + doi
+    g0 = (lap + c * (V - E) * r * r * e) / c
+    g1 = (lap + c * (2.0 + (V - E) * r * r) * e) / c
+    src = g0 + g1 * p[:, 2] / r
+    phi = np.exp(-0.3 * r)             # any orbital: ri = phi doi - src
+    ri = phi * doi - src
+    exact = r * r * e * (1.0 + p[:, 2] / r)
+    return phi, ri, V, exact
+
+
+@pytest.mark.parametrize('mass_inv', [1.0, 0.5])
+def test_dpsi_closed_form_convergence(mass_inv):
+    # ndpsi.py:15-89 on one centre against the exact solution of its radial equations
+    # (doubled centrifugal term, -2/m_inv scaling, V_sph - oi - doi, zero boundary values):
+    # a dropped factor or sign in any of them moves the l = 0 or l = 1 channel by O(1)
+    errs = []
+    for nr in (40, 80):
+        grid = core.BeckeGrid([('O', (0.0, 0.0, 0.0))], nr, 26)
+        phi, ri, V, exact = _dpsi_closed_form_inputs(grid, 0.9, 0.3, -0.6, 0.05, mass_inv)
+        d = core.compute_dpsi(grid, 2, phi, 0.05, -0.6, ri, V, mass_inv)
+        errs.append(np.max(np.abs(d - exact)))
+    # fourth order in the radial spacing (1.9e-4 -> 1.2e-5 for max|u| = 0.82 at m_inv = 1)
+    print('dpsi closed form errors', errs)
+    assert errs[1] < 3e-5 and errs[1] < errs[0] / 8
+
+
+def test_becke_dickson_step_residual_and_energy():
+    # psi_optimization.py:175-203: R = T + V - e psi, de = integrate(R psi); dpsi uses them
+    grid = core.BeckeGrid([('H', (0.0, 0.0, 0.7)), ('H', (0.0, 0.0, -0.7))], 20, 26)
+    r0 = np.sqrt(((grid.points - grid.origin[0]) ** 2).sum(1))
+    r1 = np.sqrt(((grid.points - grid.origin[1]) ** 2).sum(1))
+    psi = np.array([np.exp(-r0) + np.exp(-r1)])
+    T = np.array([0.3 * psi[0]])
+    V = np.array([-(1 / r0 + 1 / r1) * psi[0]])
+    e = np.array([-0.55])
+    dpsi, de, R = core.becke_dickson_step(grid, 2, psi, T, V, e, 1.0)
+    assert np.array_equal(R[0], T[0] + V[0] - e[0] * psi[0])
+    assert de[0] == grid.integrate(R[0] * psi[0])
+    ref = core.compute_dpsi(grid, 2, psi[0], de[0], e[0], R[0], V[0], 1.0)
+    assert np.array_equal(dpsi[0], ref)
