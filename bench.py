@@ -13,6 +13,7 @@ Lebedev points per atom (N = 472,000), 300 Cartesian s/p/d Gaussians, P = 2 C C^
 Prints one JSON line. ``--gpus N`` with N > 1 exits: this build runs on one GPU.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -167,6 +168,127 @@ def apmo_leg(args, log):
         'dE_grid_vs_fp64': dEg, 'dE_trace_vs_fp64': dEt, 'dF_max_vs_fp64': dF,
         'speedup_vs_fp64': ref['ms_per_build'] / mix['ms_per_build'],
     }
+
+
+DPSI_PHASES = ('residual', 'project', 'shell_average', 'radial', 'synth_tables', 'synth',
+               'reduce')
+
+
+def dpsi_orbitals(points, origin, norb, seed=4):
+    """Smooth synthetic orbitals on the grid: per orbital a sum of centre Gaussians of
+    one exponent with random weights, times a linear factor (no orbital data files)."""
+    rng = np.random.default_rng(seed)
+    d2 = np.stack([((points - o) ** 2).sum(1) for o in origin])
+    psi = np.empty((norb, points.shape[0]))
+    for k in range(norb):
+        w = rng.uniform(-1.0, 1.0, len(origin))
+        a = 0.3 + 0.05 * (k % 7)
+        psi[k] = (w[:, None] * np.exp(-a * d2)).sum(0) * (1.0 + 0.1 * (k % 3) * points[:, k % 3])
+    return psi
+
+
+def dpsi_leg(args, log):
+    """One Becke-Dickson orbital update step of 40 orbitals on the 16-atom 472,000-point grid
+    (lmax 12): residuals R = T + V - e psi, energy corrections, and the corrections dpsi
+    (projection, 40 x 16 x 13 radial band solves, synthesis) with psi, T = -1/2 lap psi
+    (the GPU kinetic operator) and V = V_nuc psi resident in HBM."""
+    import napmo_amd
+    from napmo_amd import workloads
+    from napmo_amd.device import DeviceArray
+    from napmo_amd.system import cext
+    from napmo_amd.wavefunction.ndpsi import becke_dickson_step_device
+    cext.require_gpu()
+    atoms, _, nrad, nang, lmax = workloads.chon16_system()
+    norb = 40
+    grid = napmo_amd.BeckeGrid(napmo_amd.make_species(atoms), nrad, nang)
+    pts, org = grid.points, grid.origin
+    N = grid.size
+    t0 = time.perf_counter()
+    psi = dpsi_orbitals(pts, org, norb)
+    T = np.array([napmo_amd.compute_kinetic(grid, p, lmax) for p in psi])
+    Z = [workloads.ZNUC[s] for s, _ in atoms]
+    Vn = napmo_amd.compute_nuclear(grid, [{'atomic_number': z, 'origin': o} for z, o in zip(Z, org)])
+    V = -Vn[None, :] * psi
+    e = -0.5 - 0.02 * np.arange(norb)
+    log('dpsi inputs: %d orbitals on %d points (%.1fs)' % (norb, N, time.perf_counter() - t0))
+    d = {k: DeviceArray.from_host(np.ascontiguousarray(v)) for k, v in
+         (('psi', psi), ('T', T), ('V', V), ('e', e))}
+    out = {k: DeviceArray(s) for k, s in (('dpsi', (norb, N)), ('de', (norb,)), ('R', (norb, N)))}
+
+    def step():
+        becke_dickson_step_device(grid, lmax, norb, d['psi'].ptr, d['T'].ptr, d['V'].ptr, d['e'].ptr,
+                                  1.0, out['dpsi'].ptr, out['de'].ptr, out['R'].ptr)
+
+    for _ in range(args.warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    dt = (time.perf_counter() - t0) / args.steps
+    # per-phase HIP-event times in a separate pass (the events add a few us per launch)
+    cext.check(cext.napmo_grid_profile(grid._this, 1))
+    for _ in range(min(args.steps, 5)):
+        step()
+    ph = np.zeros(len(DPSI_PHASES))
+    nc = ctypes.c_int(0)
+    cext.check(cext.napmo_grid_profile_read(grid._this, ph.ctypes.data, len(DPSI_PHASES), ctypes.byref(nc)))
+    cext.check(cext.napmo_grid_profile(grid._this, 0))
+    phases = dict(zip(DPSI_PHASES, ph.tolist()))
+    de = out['de'].to_host()
+    dpsi = out['dpsi'].to_host()
+    log('dpsi step: %.3f ms (%.1f steps/s), phases %s; de[0:3] %s, max|dpsi| %.3e' % (
+        dt * 1e3, 1.0 / dt, {k: round(v, 4) for k, v in phases.items()}, de[:3], np.abs(dpsi).max()))
+    M, L = len(atoms), (lmax + 1) ** 2
+    # dominant kernel: the synthesis, 2 M N L flops per orbital by the counting convention
+    name = max(phases, key=phases.get)
+    ms = phases[name]
+    if name == 'synth':
+        flops = 2.0 * M * N * L * norb
+        roof = {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': PEAK['fp64_mfma'] / 1e12,
+                'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / PEAK['fp64_mfma'], 'traffic': None}
+    else:
+        nbytes = 8.0 * N * norb * 4
+        roof = {'bound': 'hbm', 'achieved': nbytes / (ms * 1e-3) / 1e9, 'peak': PEAK['hbm'] / 1e9,
+                'unit': 'GB/s', 'frac': nbytes / (ms * 1e-3) / PEAK['hbm'], 'traffic': None}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = dpsi_cpu_baseline(atoms, nrad, nang, lmax, norb, log)
+    return {
+        'metric': 'Becke-Dickson orbital update steps/s on 1 MI355X', 'value': 1.0 / dt,
+        'unit': 'steps/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': dt * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f64', 'data': 'synthetic (seeded orbitals; T from the GPU kinetic operator, V = V_nuc psi)',
+        'config': {'workload': 'Becke-Dickson update of 40 orbitals, 16 atoms, 50x590 Becke grid '
+                               '(472,000 points), lmax 12', 'norb': norb, 'npoint': N, 'lmax': lmax},
+        'roofline': dict(roof, kernel=name), 'cpu_baseline': cpu, 'phases_ms': phases,
+    }
+
+
+def dpsi_cpu_baseline(atoms, nrad, nang, lmax, norb, log):
+    """The oracle's compute_dpsi (numpy projection, SciPy sparse radial solves as the
+    reference, C/OpenMP synthesis) for one orbital on the full grid, scaled to norb."""
+    from oracle import cext as ocx
+    from oracle import core
+    grid = core.BeckeGrid(atoms, nrad, nang)
+    psi = dpsi_orbitals(grid.points, grid.origin, 1)[0]
+    r = np.sqrt(((grid.points - grid.origin[0]) ** 2).sum(1))
+    V = -psi / np.maximum(r, 1e-3)
+
+    def ev(splines, centre, points):
+        out = np.zeros(len(points))
+        ocx.eval_decomposition(splines, centre, points, out)
+        return out
+
+    t = time.perf_counter()
+    core.compute_dpsi(grid, lmax, psi, 0.01, -0.5, 0.1 * psi, V, 1.0, evaluate=ev)
+    one = time.perf_counter() - t
+    total = one * norb
+    log('dpsi cpu baseline (%d threads): %.2fs per orbital, %.1fs per %d-orbital step' % (
+        ocx.max_threads(), one, total, norb))
+    return {'value': 1.0 / total, 'unit': 'steps/s', 'cores': ocx.max_threads(), 'kind': 'port',
+            'sample': 'oracle compute_dpsi of 1 orbital on the full 472,000-point grid (numpy '
+                      'projection, SciPy sparse solves, C/OpenMP synthesis), scaled to %d orbitals' % norb,
+            'seconds_per_step': total}
 
 
 def batch_leg(args, log):
@@ -399,9 +521,10 @@ def main():
     ap.add_argument('--copy-calibration', type=float, default=0.0, metavar='GIB',
                     help='only run a plain device copy of GIB gibibytes 5 times (counter calibration)')
     ap.add_argument('--cpu-seconds', type=float, default=30.0)
-    ap.add_argument('--workload', default='chon16', choices=('chon16', 'apmo', 'batch64'),
+    ap.add_argument('--workload', default='chon16', choices=('chon16', 'apmo', 'batch64', 'dpsi'),
                     help='chon16: 16-atom 472,000-point build (default); apmo: electrons + '
-                         'quantum protons on separate grids; batch64: 64 small molecules')
+                         'quantum protons on separate grids; batch64: 64 small molecules; dpsi: '
+                         'Becke-Dickson update of 40 orbitals on the 16-atom grid')
     args = ap.parse_args()
 
     def log(msg):
@@ -425,6 +548,9 @@ def main():
         return
     if args.workload == 'batch64':
         print(json.dumps(batch_leg(args, log)), flush=True)
+        return
+    if args.workload == 'dpsi':
+        print(json.dumps(dpsi_leg(args, log)), flush=True)
         return
     results, N, n, lmax, M = gpu_leg(args, log)
     L = (lmax + 1) ** 2

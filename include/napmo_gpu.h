@@ -227,6 +227,11 @@ int napmo_becke_dickson_step_device(void *grid, int lmax, int norb, const double
                                     const double *T, const double *V_tot, const double *e_prev,
                                     double mass_inv, double *dpsi_out, double *de_out,
                                     double *R_out);
+/* HIP-event timing of the orbital corrections on this grid: start (enable = 1) or stop;
+ * read gives the mean milliseconds per call of each phase (residual, sources and projection,
+ * shell averages, radial solves, synthesis tables, synthesis, reduction) since the start. */
+int napmo_grid_profile(void *grid, int enable);
+int napmo_grid_profile_read(void *grid, double *avg_ms, int nphase, int *ncalls);
 
 /* Grid Fock plan: device-resident Phi (collocated once), Poisson plan, workspace. */
 int napmo_fock_plan_new(void *grid, void *basis, int lmax, int prec_mode, void **plan_out);
@@ -288,7 +293,9 @@ enum {
   NAPMO_TUNE_PASS_WG = 1,  /* workgroups of the one-pass BF16x3 Fock kernel; 0 = default */
   NAPMO_TUNE_USE_PASS = 2, /* 1: one-pass BF16x3 kernel, 0: per-tile BF16x3 kernel */
   NAPMO_TUNE_DENSITY_KS = 3, /* K per LDS stage of the density kernels: 16 or 32; 0 = 16 */
-  NAPMO_TUNE_DENSITY_PF = 4  /* stages in flight in the density kernels: 1 or 2; 0 = 1 */
+  NAPMO_TUNE_DENSITY_PF = 4, /* stages in flight in the density kernels: 1 or 2; 0 = 1 */
+  NAPMO_TUNE_SYNTH_NP = 5    /* (point, centre) pairs per lane of the synthesis, 1 or 2; 0 = 2
+                                (a property of the grid: shared by every plan on it) */
 };
 int napmo_fock_plan_set_tuning(void *plan, int key, long value);
 int napmo_fock_plan_get_tuning(void *plan, int key, long *value);

@@ -4,8 +4,7 @@ For a plan (one molecule's grid, basis and precision) it times every candidate l
 shape of the Fock contraction on the device (the per-tile FP64 / BF16x3 kernels' workgroup
 count, the one-pass BF16x3 kernel's workgroup count, and the choice between the two
 BF16x3 kernels) together with the K depth and prefetch depth of the density kernels' LDS
-stages, keeps the
-fastest, and stores it in a JSON cache keyed by the shape class
+stages, then the synthesis' pairs per lane (1 or 2), keeps the fastest, and stores it in a JSON cache keyed by the shape class
 (precision, padded basis size, point count rounded to a power of two), so later plans of
 the same class start from the cached choice without timing. The cache lives at
 $NAPMO_TUNING_CACHE or ~/.cache/napmo_amd/tuning.json.
@@ -18,10 +17,11 @@ import time
 
 from napmo_amd.system import cext
 
-TUNE_FOCK_WG, TUNE_PASS_WG, TUNE_USE_PASS, TUNE_DENSITY_KS, TUNE_DENSITY_PF = 0, 1, 2, 3, 4
+TUNE_FOCK_WG, TUNE_PASS_WG, TUNE_USE_PASS, TUNE_DENSITY_KS, TUNE_DENSITY_PF, TUNE_SYNTH_NP = range(6)
 _KEYS = {'fock_wg': TUNE_FOCK_WG, 'pass_wg': TUNE_PASS_WG, 'use_pass': TUNE_USE_PASS,
-         'density_ks': TUNE_DENSITY_KS, 'density_pf': TUNE_DENSITY_PF}
-DEFAULTS = {'fock_wg': 0, 'pass_wg': 0, 'density_ks': 16, 'density_pf': 1}   # 0: the library's default shape
+         'density_ks': TUNE_DENSITY_KS, 'density_pf': TUNE_DENSITY_PF, 'synth_np': TUNE_SYNTH_NP}
+DEFAULTS = {'fock_wg': 0, 'pass_wg': 0, 'density_ks': 16, 'density_pf': 1,
+            'synth_np': 2}   # 0: the library's default shape
 
 
 def cache_path():
@@ -93,21 +93,30 @@ def tune(fock, dP, dF, dE, reps=3, path=None, force=False, build=None, tag=''):
             return setting, 'cache'
         except cext.NapmoError:
             pass   # not applicable to this plan: time again
-    best = None
-    for cand in candidates(fock):
-        try:
-            apply(fock, dict(DEFAULTS, **cand))
-        except cext.NapmoError:
-            continue
+    def timed(setting):
         run()
         fock.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
             run()
         fock.synchronize()
-        t = (time.perf_counter() - t0) / reps
+        return (time.perf_counter() - t0) / reps
+
+    best = None
+    for cand in candidates(fock):
+        try:
+            apply(fock, dict(DEFAULTS, **cand))
+        except cext.NapmoError:
+            continue
+        t = timed(cand)
         if best is None or t < best[1]:
             best = (cand, t)
+    per_np = []
+    for np_ in (1, 2):
+        cand = dict(best[0], synth_np=np_)
+        apply(fock, dict(DEFAULTS, **cand))
+        per_np.append((cand, timed(cand)))
+    best = min(per_np, key=lambda r: r[1])
     apply(fock, dict(DEFAULTS, **best[0]))
     cache[key] = dict(best[0], ms=best[1] * 1e3)
     _save(path, cache)

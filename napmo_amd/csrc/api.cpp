@@ -615,6 +615,24 @@ int napmo_becke_dickson_step_device(void *grid, int lmax, int norb, const double
   });
 }
 
+int napmo_grid_profile(void *grid, int enable) {
+  return guarded([&] {
+    DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    if (enable)
+      g.profile_start();
+    else
+      g.prof_on = false;
+  });
+}
+
+int napmo_grid_profile_read(void *grid, double *avg_ms, int nphase, int *ncalls) {
+  return guarded([&] {
+    DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    const int n = g.profile_read(avg_ms, nphase);
+    if (ncalls) *ncalls = n;
+  });
+}
+
 int napmo_compute_nuclear(void *grid, int ncharge, const double *charges, const double *origins,
                           double *V_out) {
   return guarded([&] {

@@ -96,6 +96,10 @@ void FockPlan::set_tuning(int key, long value) {
                     "density stage depth must be 16 or 32");
       density_ks = value ? (int)value : 16;
       break;
+    case NAPMO_TUNE_SYNTH_NP:
+      NAPMO_REQUIRE(value == 0 || value == 1 || value == 2, "synthesis pairs per lane must be 1 or 2");
+      grid.synth_np = value ? (int)value : 2;
+      break;
     default: throw Error("unknown tuning key " + std::to_string(key));
   }
   NAPMO_HIP(hipStreamSynchronize(stream));
@@ -109,6 +113,7 @@ long FockPlan::get_tuning(int key) const {
     case NAPMO_TUNE_USE_PASS: return use_pass ? 1 : 0;
     case NAPMO_TUNE_DENSITY_KS: return density_ks;
     case NAPMO_TUNE_DENSITY_PF: return density_pf;
+    case NAPMO_TUNE_SYNTH_NP: return grid.synth_np;
     default: throw Error("unknown tuning key " + std::to_string(key));
   }
 }
@@ -200,7 +205,7 @@ void FockPlan::synth_self(int slot) {
   mark(slot, 5);
   launch_synth2(at, grid.x.get(), grid.y.get(), grid.z.get(), grid.N, sp.list.get(),
                 sp.wave_bucket.get(), sp.nwaves, work.T.get(), work.ampL.get(), work.ampR.get(), lmax,
-                sp.contrib.get(), st);
+                sp.contrib.get(), st, grid.synth_np);
   mark(slot, 6);  // the "scale" phase: centre sum, then w bw v
   launch_contrib_reduce(sp.contrib.get(), grid.M, grid.N, 0, v.get(), st);
 }

@@ -410,7 +410,7 @@ void launch_pair_fill(const AtomTables &at, const double *x, const double *y, co
 }
 
 template <int LM>
-static void synth2_dispatch(int lmax, dim3 g, dim3 b, hipStream_t s, const AtomTables &at,
+static void synth2_dispatch(int lmax, int np, hipStream_t s, const AtomTables &at,
                             const double *x, const double *y, const double *z, long N,
                             const int *list, const int *wb, long nw, const double *T,
                             const double *aL, const double *aR, double *contrib) {
@@ -418,20 +418,22 @@ static void synth2_dispatch(int lmax, dim3 g, dim3 b, hipStream_t s, const AtomT
     throw Error("lmax above the supported maximum " + std::to_string(MAX_LMAX));
   } else {
     if (lmax == LM)
-      launch_synth2_t<LM>(g, b, s, at, x, y, z, N, list, wb, nw, T, aL, aR, contrib);
+      launch_synth2_t<LM>(np, s, at, x, y, z, N, list, wb, nw, T, aL, aR, contrib);
     else
-      synth2_dispatch<LM + 1>(lmax, g, b, s, at, x, y, z, N, list, wb, nw, T, aL, aR, contrib);
+      synth2_dispatch<LM + 1>(lmax, np, s, at, x, y, z, N, list, wb, nw, T, aL, aR, contrib);
   }
 }
 
 void launch_synth2(const AtomTables &at, const double *x, const double *y, const double *z, long N,
                    const int *list, const int *wave_bucket, long nwaves, const double *T,
                    const double *ampL, const double *ampR, int lmax, double *contrib,
-                   hipStream_t s) {
+                   hipStream_t s, int np) {
   NAPMO_REQUIRE(lmax >= 0 && lmax <= MAX_LMAX, "lmax out of range for synthesis");
+  NAPMO_REQUIRE(np == 1 || np == 2, "synthesis: 1 or 2 pairs per lane");
+  NAPMO_REQUIRE(nwaves % (SYNTH_PAD / 64) == 0, "synthesis: pair lists must be padded to 128");
   if (nwaves <= 0) return;
-  synth2_dispatch<0>(lmax, dim3(div_up(nwaves, 4)), dim3(256), s, at, x, y, z, N, list,
-                     wave_bucket, nwaves, T, ampL, ampR, contrib);
+  synth2_dispatch<0>(lmax, np, s, at, x, y, z, N, list, wave_bucket, nwaves, T, ampL, ampR,
+                     contrib);
   NAPMO_HIP(hipGetLastError());
 }
 

@@ -86,7 +86,7 @@ void launch_synth_t(dim3 g, dim3 b, hipStream_t s, const AtomTables &at, const d
                     const double *aR, int acc, int ext_mode, double *v);
 
 template <int LMAX>
-void launch_synth2_t(dim3 g, dim3 b, hipStream_t s, const AtomTables &at, const double *x,
+void launch_synth2_t(int np, hipStream_t s, const AtomTables &at, const double *x,
                      const double *y, const double *z, long N, const int *list,
                      const int *wave_bucket, long nwaves, const double *T, const double *aL,
                      const double *aR, double *contrib);
@@ -98,10 +98,12 @@ void launch_pair_count(const AtomTables &at, const double *x, const double *y, c
 void launch_pair_fill(const AtomTables &at, const double *x, const double *y, const double *z,
                       long N, const long *off, int *cursor, int *list, hipStream_t s);
 // Bucketed synthesis into contrib [M x N], then v[p] (+)= scale * sum_A contrib[A][p].
+// np = (point, centre) pairs per lane, 1 or 2 (the lists are padded to 128 per bucket).
+constexpr int SYNTH_PAD = 128;
 void launch_synth2(const AtomTables &at, const double *x, const double *y, const double *z, long N,
                    const int *list, const int *wave_bucket, long nwaves, const double *T,
                    const double *ampL, const double *ampR, int lmax, double *contrib,
-                   hipStream_t s);
+                   hipStream_t s, int np = 2);
 void launch_contrib_reduce(const double *contrib, int M, long N, int accumulate, double *v,
                            hipStream_t s, double scale = 1.0);
 // Point-charge potential V[p] = sum_A q_A / |p - R_A| (o: [nq x 3]).

@@ -307,6 +307,7 @@ std::vector<double> projection_table(const AngularGrid &ang, int lmax) {
 
 DeviceGrid::~DeviceGrid() {
   plans.clear();
+  for (auto e : prof_pool) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -505,8 +506,50 @@ void DeviceGrid::kinetic(const double *d_psi, int lmax, double *d_out) {
   launch_synth_tables(at, w.Uy.get(), w.Udt.get(), lmax, w.T.get(), w.ampL.get(), w.ampR.get(),
                       stream);
   launch_synth2(at, x.get(), y.get(), z.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves,
-                w.T.get(), w.ampL.get(), w.ampR.get(), lmax, sp.contrib.get(), stream);
+                w.T.get(), w.ampL.get(), w.ampR.get(), lmax, sp.contrib.get(), stream, synth_np);
   launch_contrib_reduce(sp.contrib.get(), M, N, 0, d_out, stream, -0.5);
+}
+
+void DeviceGrid::tic(int phase) {
+  if (!prof_on) return;
+  if (prof_next + 2 > prof_pool.size()) {
+    hipEvent_t a, b;
+    NAPMO_HIP(hipEventCreate(&a));
+    NAPMO_HIP(hipEventCreate(&b));
+    prof_pool.push_back(a), prof_pool.push_back(b);
+  }
+  prof_start_ev = prof_pool[prof_next++];
+  prof_phase = phase;
+  NAPMO_HIP(hipEventRecord(prof_start_ev, stream));
+}
+
+void DeviceGrid::toc() {
+  if (!prof_on) return;
+  hipEvent_t e = prof_pool[prof_next++];
+  NAPMO_HIP(hipEventRecord(e, stream));
+  prof_marks.push_back({prof_phase, {prof_start_ev, e}});
+}
+
+void DeviceGrid::profile_start() {
+  NAPMO_HIP(hipStreamSynchronize(stream));
+  prof_on = true;
+  prof_calls = 0;
+  prof_next = 0;
+  prof_marks.clear();
+}
+
+int DeviceGrid::profile_read(double *avg_ms, int nphase) {
+  NAPMO_HIP(hipStreamSynchronize(stream));
+  const int k = std::min(nphase, DPSI_NPHASE);
+  for (int p = 0; p < k; ++p) avg_ms[p] = 0.0;
+  for (auto &m : prof_marks) {
+    float ms = 0.f;
+    NAPMO_HIP(hipEventElapsedTime(&ms, m.second.first, m.second.second));
+    if (m.first < k) avg_ms[m.first] += ms;
+  }
+  if (prof_calls)
+    for (int p = 0; p < k; ++p) avg_ms[p] /= prof_calls;
+  return prof_calls;
 }
 
 HelmholtzTables HelmholtzPlan::tables() const {
@@ -566,29 +609,50 @@ void DeviceGrid::orbital_correction(int lmax, int norb, const double *d_phi, con
   const int L = (lmax + 1) * (lmax + 1);
   long S = 0;
   for (auto &a : atoms) S += a.nrad;
-  DeviceBuffer<double> src((size_t)norb * N), mu((size_t)norb * S * L), Uy((size_t)norb * S * L),
-      Udt((size_t)norb * S * L), Vsph((size_t)norb * S);
-  DeviceBuffer<int> status(1);
-  PoissonWork wk;
-  alloc_work(lmax, wk);
-  NAPMO_HIP(hipMemsetAsync(status.get(), 0, sizeof(int), stream));
+  if (!owork) owork = std::make_unique<OrbitalWork>();
+  OrbitalWork &ow = *owork;
+  ow.src.resize((size_t)norb * N);
+  ow.mu.resize((size_t)norb * S * L);
+  ow.Uy.resize((size_t)norb * S * L);
+  ow.Udt.resize((size_t)norb * S * L);
+  ow.Vsph.resize((size_t)norb * S);
+  ow.status.resize(1);
+  if (ow.lmax != lmax) {
+    alloc_work(lmax, ow.pw);
+    ow.lmax = lmax;
+  }
+  PoissonWork &wk = ow.pw;
+  NAPMO_HIP(hipMemsetAsync(ow.status.get(), 0, sizeof(int), stream));
+  if (prof_on) ++prof_calls;
   // sources (phi doi - ri) bw projected per atom, V_sph = spherical average of V bw
-  launch_scaled_diff(d_phi, d_doi, d_ri, norb, N, src.get(), stream);
+  tic(1);
+  launch_scaled_diff(d_phi, d_doi, d_ri, norb, N, ow.src.get(), stream);
   for (int o = 0; o < norb; ++o)
-    launch_project(at, src.get() + (long)o * N, bw.get(), pl.yhat.get(), lmax,
-                   mu.get() + (long)o * S * L, stream);
-  launch_shell_average(at, d_V, norb, N, w.get(), bw.get(), S, Vsph.get(), stream);
-  launch_helmholtz_radial(at, hp.tables(), lmax, norb, S, Vsph.get(), d_oi, d_doi, mass_inv,
-                          mu.get(), Uy.get(), Udt.get(), status.get(), stream);
+    launch_project(at, ow.src.get() + (long)o * N, bw.get(), pl.yhat.get(), lmax,
+                   ow.mu.get() + (long)o * S * L, stream);
+  toc();
+  tic(2);
+  launch_shell_average(at, d_V, norb, N, w.get(), bw.get(), S, ow.Vsph.get(), stream);
+  toc();
+  tic(3);
+  launch_helmholtz_radial(at, hp.tables(), lmax, norb, S, ow.Vsph.get(), d_oi, d_doi, mass_inv,
+                          ow.mu.get(), ow.Uy.get(), ow.Udt.get(), ow.status.get(), stream);
+  toc();
   for (int o = 0; o < norb; ++o) {
-    launch_synth_tables(at, Uy.get() + (long)o * S * L, Udt.get() + (long)o * S * L, lmax,
+    tic(4);
+    launch_synth_tables(at, ow.Uy.get() + (long)o * S * L, ow.Udt.get() + (long)o * S * L, lmax,
                         wk.T.get(), wk.ampL.get(), wk.ampR.get(), stream);
+    toc();
+    tic(5);
     launch_synth2(at, x.get(), y.get(), z.get(), N, sp.list.get(), sp.wave_bucket.get(),
-                  sp.nwaves, wk.T.get(), wk.ampL.get(), wk.ampR.get(), lmax, sp.contrib.get(), stream);
+                  sp.nwaves, wk.T.get(), wk.ampL.get(), wk.ampR.get(), lmax, sp.contrib.get(), stream, synth_np);
+    toc();
+    tic(6);
     launch_contrib_reduce(sp.contrib.get(), M, N, 0, d_out + (long)o * N, stream);
+    toc();
   }
   int h_status = 0;
-  NAPMO_HIP(hipMemcpyAsync(&h_status, status.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
+  NAPMO_HIP(hipMemcpyAsync(&h_status, ow.status.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
   NAPMO_HIP(hipStreamSynchronize(stream));
   NAPMO_REQUIRE(h_status == 0, "orbital correction: singular radial system (zero pivot)");
 }
@@ -598,8 +662,11 @@ void DeviceGrid::becke_dickson_step(int lmax, int norb, const double *d_psi, con
                                     double *d_dpsi, double *d_de, double *d_R) {
   NAPMO_REQUIRE(norb >= 0, "Becke-Dickson step: negative orbital count");
   if (norb == 0) return;
-  DeviceBuffer<double> work((size_t)norb * RESID_BLOCKS);
-  launch_residual(d_psi, d_T, d_V, d_e, wbw.get(), norb, N, d_R, d_de, work.get(), stream);
+  if (!owork) owork = std::make_unique<OrbitalWork>();
+  owork->resid.resize((size_t)norb * RESID_BLOCKS);
+  tic(0);
+  launch_residual(d_psi, d_T, d_V, d_e, wbw.get(), norb, N, d_R, d_de, owork->resid.get(), stream);
+  toc();
   orbital_correction(lmax, norb, d_psi, d_de, d_e, d_R, d_V, mass_inv, d_dpsi);
 }
 
@@ -617,7 +684,7 @@ void DeviceGrid::poisson_synthesize(int lmax, double *d_v) {
   launch_synth_tables(at, pl.work.Uy.get(), pl.work.Udt.get(), lmax, pl.work.T.get(), pl.work.ampL.get(),
                       pl.work.ampR.get(), stream);
   launch_synth2(at, x.get(), y.get(), z.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves,
-                pl.work.T.get(), pl.work.ampL.get(), pl.work.ampR.get(), lmax, sp.contrib.get(), stream);
+                pl.work.T.get(), pl.work.ampL.get(), pl.work.ampR.get(), lmax, sp.contrib.get(), stream, synth_np);
   launch_contrib_reduce(sp.contrib.get(), M, N, 0, d_v, stream);
 }
 
@@ -659,7 +726,8 @@ std::unique_ptr<SynthPairs> build_synth_pairs(const DeviceGrid &src, const doubl
   long total = 0;
   for (int b = 0; b < nb; ++b) {
     off[b] = total;
-    const long waves = div_up(hc[b], 64);
+    // padded to SYNTH_PAD pairs, so a wavefront may take two 64-pair slots of one bucket
+    const long waves = div_up(hc[b], SYNTH_PAD) * (SYNTH_PAD / 64);
     for (long w = 0; w < waves; ++w) wb.push_back(b);
     total += waves * 64;
   }
@@ -689,7 +757,7 @@ void DeviceGrid::synthesize_at(int lmax, const PoissonWork &w, const SynthPairs 
   PoissonPlan &pl = plan(lmax);
   AtomTables at = tables(pl);
   launch_synth2(at, tx, ty, tz, tN, sp.list.get(), sp.wave_bucket.get(), sp.nwaves, w.T.get(),
-                w.ampL.get(), w.ampR.get(), lmax, sp.contrib.get(), st);
+                w.ampL.get(), w.ampR.get(), lmax, sp.contrib.get(), st, synth_np);
   launch_contrib_reduce(sp.contrib.get(), M, tN, accumulate, d_v, st, scale);
 }
 

@@ -83,6 +83,15 @@ struct HelmholtzPlan {
   HelmholtzTables tables() const;
 };
 
+// Workspace of the orbital corrections, kept between calls (resized when norb or lmax
+// change): sources, projections mu, solutions (y, dt), shell averages, synthesis tables.
+struct OrbitalWork {
+  DeviceBuffer<double> src, mu, Uy, Udt, Vsph, resid;
+  DeviceBuffer<int> status;
+  PoissonWork pw;
+  int lmax = -1;
+};
+
 struct DeviceGrid {
   hipStream_t stream = nullptr;
   long N = 0;
@@ -98,9 +107,25 @@ struct DeviceGrid {
   DeviceBuffer<int> d_nrad, d_nang, d_rkind;
   DeviceBuffer<double> d_rp0, d_rp1, d_first, d_last, d_center, d_r;
   int max_nrad = 0;
+  int synth_np = 2;  // (point, centre) pairs per lane of the synthesis (1 or 2)
   std::map<int, std::unique_ptr<PoissonPlan>> plans, kplans;  // Poisson / kinetic, per lmax
   std::unique_ptr<SynthPairs> pairs;
   std::unique_ptr<HelmholtzPlan> hplan;
+  std::unique_ptr<OrbitalWork> owork;
+  // Per-phase HIP-event timing of the orbital corrections (phases: residual, sources and
+  // projection, shell averages, radial solves, synthesis tables, synthesis, reduction).
+  static constexpr int DPSI_NPHASE = 7;
+  bool prof_on = false;
+  int prof_calls = 0;
+  size_t prof_next = 0;
+  std::vector<hipEvent_t> prof_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_marks;
+  hipEvent_t prof_start_ev = nullptr;
+  int prof_phase = -1;
+  void tic(int phase);
+  void toc();
+  void profile_start();
+  int profile_read(double *avg_ms, int nphase);
 
   ~DeviceGrid();
   void build(const std::vector<AtomInfo> &atoms);

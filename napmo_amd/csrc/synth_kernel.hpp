@@ -166,6 +166,94 @@ __device__ __forceinline__ double synth_harmonics(const double *c, double u, dou
   return acc;
 }
 
+// Same sum for NP pairs of one bucket at once: every coefficient read (from LDS or the
+// amplitude table) serves NP pairs. Per pair the operations and their order are those of
+// synth_harmonics, so the result is bit-identical to it.
+template <int LMAX, int REG, int NP>
+__device__ __forceinline__ void synth_harmonics_np(const double *c, const double (&u)[NP],
+                                                   const double (&ux)[NP], const double (&uy)[NP],
+                                                   const double (&uz)[NP], const double (&d)[NP],
+                                                   const double (&inv_d)[NP], double (&acc)[NP]) {
+  // every multiply-add is an explicit fma and nothing else is contracted, so the result
+  // does not depend on how the compiler schedules NP pairs
+#pragma clang fp contract(off)
+  double Am[NP], Bm[NP], pwm[NP], pi1[NP], pi2[NP], pw[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    acc[k] = 0.0;
+    Am[k] = 1.0, Bm[k] = 0.0;
+    pwm[k] = (REG == 2) ? inv_d[k] : 1.0;
+  }
+  int q = 0;
+#pragma unroll
+  for (int m = 0; m <= LMAX; ++m) {
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      if (m > 0) {
+        const double a2 = fma(ux[k], Am[k], -(uy[k] * Bm[k]));
+        Bm[k] = fma(ux[k], Bm[k], uy[k] * Am[k]);
+        Am[k] = a2;
+        if (REG != 1) pwm[k] *= (REG == 0) ? d[k] : inv_d[k];
+      }
+      pi2[k] = 0.0, pi1[k] = 1.0, pw[k] = pwm[k];
+    }
+    // per m, the cos and sin parts are summed over l first and multiplied by Re / Im
+    // (x + i y)^m once
+    double accc[NP], accs[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) accc[k] = 0.0, accs[k] = 0.0;
+#pragma unroll
+    for (int l = m; l <= LMAX; ++l) {
+      double pil[NP];
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        if (l == m) {
+          pil[k] = 1.0;
+        } else if (l == m + 1) {
+          pil[k] = uz[k];
+          pi2[k] = pi1[k];
+          pi1[k] = pil[k];
+        } else {
+          pil[k] = fma(uz[k], pi1[k], -(pi_beta(l, m) * pi2[k]));
+          pi2[k] = pi1[k];
+          pi1[k] = pil[k];
+        }
+        if (REG != 1 && l > m) pw[k] *= (REG == 0) ? d[k] : inv_d[k];
+      }
+#pragma unroll
+      for (int cs = 0; cs < (m ? 2 : 1); ++cs) {
+        if (REG == 1) {
+          const double *cq = c + q * 4;
+          const double c0 = cq[0], c1 = cq[1], c2 = cq[2], c3 = cq[3];
+#pragma unroll
+          for (int k = 0; k < NP; ++k) {
+            const double sv = fma(u[k], fma(u[k], fma(u[k], c3, c2), c1), c0);
+            if (cs)
+              accs[k] = fma(sv, pil[k], accs[k]);
+            else
+              accc[k] = fma(sv, pil[k], accc[k]);
+          }
+        } else {
+          const double cq = c[q];
+#pragma unroll
+          for (int k = 0; k < NP; ++k) {
+            if (cs)
+              accs[k] = fma(cq * pw[k], pil[k], accs[k]);
+            else
+              accc[k] = fma(cq * pw[k], pil[k], accc[k]);
+          }
+        }
+        ++q;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      acc[k] = fma(Am[k], accc[k], acc[k]);
+      if (m > 0) acc[k] = fma(Bm[k], accs[k], acc[k]);
+    }
+  }
+}
+
 // Bucketed variant: the (point, centre) pairs are sorted by (centre, spline interval)
 // once per geometry; each wavefront handles 64 pairs of one bucket, copies that bucket's
 // spline row into its own LDS slice once, and every lane then reads the coefficients as
@@ -177,23 +265,25 @@ template <int LMAX> struct Synth2Cfg {
   static constexpr int WPB = (L * 4 * 8 * 4 <= 65536) ? 4 : 2;  // waves per block (LDS <= 64 KB)
 };
 
-template <int LMAX>
-__global__ void __launch_bounds__(256) k_synth2(AtomTables at, const double *__restrict__ px,
-                                                const double *__restrict__ py,
-                                                const double *__restrict__ pz, long N,
-                                                const int *__restrict__ list,
-                                                const int *__restrict__ wave_bucket, long nwaves,
-                                                const double *__restrict__ T,
-                                                const double *__restrict__ ampL,
-                                                const double *__restrict__ ampR,
-                                                double *__restrict__ contrib) {
+// NP pairs per lane: a wavefront takes NP consecutive 64-pair slots of one bucket (the
+// lists are padded to 64 NP per bucket) and reads each staged coefficient once for all.
+template <int LMAX, int NP>
+__global__ void __launch_bounds__(256) k_synth2n(AtomTables at, const double *__restrict__ px,
+                                                 const double *__restrict__ py,
+                                                 const double *__restrict__ pz, long N,
+                                                 const int *__restrict__ list,
+                                                 const int *__restrict__ wave_bucket, long ngroups,
+                                                 const double *__restrict__ T,
+                                                 const double *__restrict__ ampL,
+                                                 const double *__restrict__ ampR,
+                                                 double *__restrict__ contrib) {
   constexpr int L = Synth2Cfg<LMAX>::L;
   constexpr int WPB = Synth2Cfg<LMAX>::WPB;
   __shared__ double srow[WPB][L * 4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long wid = (long)blockIdx.x * WPB + wave;
-  const bool active = wid < nwaves;
-  const int gb = active ? __builtin_amdgcn_readfirstlane(wave_bucket[wid]) : 0;
+  const long gid = (long)blockIdx.x * WPB + wave;
+  const bool active = gid < ngroups;
+  const int gb = active ? __builtin_amdgcn_readfirstlane(wave_bucket[gid * NP]) : 0;
   const int A = gb / (at.max_nrad + 1), loc = gb - A * (at.max_nrad + 1);
   const int nr = at.nrad[A];
   const int reg = (loc == 0) ? 0 : (loc == nr ? 2 : 1);
@@ -204,35 +294,52 @@ __global__ void __launch_bounds__(256) k_synth2(AtomTables at, const double *__r
   }
   __syncthreads();
   if (!active) return;
-  const int p = list[wid * 64 + lane];
-  const long pp = p >= 0 ? p : 0;
-  const double dx = px[pp] - at.center[3 * A], dy = py[pp] - at.center[3 * A + 1],
-               dz = pz[pp] - at.center[3 * A + 2];
-  const double d = sqrt(dx * dx + dy * dy + dz * dz);
-  const double inv_d = 1.0 / d;
-  const double ux = dx * inv_d, uy = dy * inv_d, uz = dz * inv_d;
-  double acc;
-  if (reg == 1) {
-    const double u = rtf_inv(at.rkind[A], at.rp0[A], at.rp1[A], nr, d) - j;
-    acc = synth_harmonics<LMAX, 1>(srow[wave], u, ux, uy, uz, d, inv_d);
-  } else if (reg == 0) {
-    acc = synth_harmonics<LMAX, 0>(ampL + (long)A * L, 0.0, ux, uy, uz, d, inv_d);
-  } else {
-    acc = synth_harmonics<LMAX, 2>(ampR + (long)A * L, 0.0, ux, uy, uz, d, inv_d);
+  const double cx = at.center[3 * A], cy = at.center[3 * A + 1], cz = at.center[3 * A + 2];
+  int p[NP];
+  double u[NP], ux[NP], uy[NP], uz[NP], d[NP], inv_d[NP], acc[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    p[k] = list[(gid * NP + k) * 64 + lane];
+    const long pp = p[k] >= 0 ? p[k] : 0;
+    const double dx = px[pp] - cx, dy = py[pp] - cy, dz = pz[pp] - cz;
+    d[k] = sqrt(dx * dx + dy * dy + dz * dz);
+    inv_d[k] = 1.0 / d[k];
+    ux[k] = dx * inv_d[k], uy[k] = dy * inv_d[k], uz[k] = dz * inv_d[k];
+    u[k] = (reg == 1) ? rtf_inv(at.rkind[A], at.rp0[A], at.rp1[A], nr, d[k]) - j : 0.0;
   }
-  if (p >= 0) contrib[(long)A * N + p] = acc;
+  if (reg == 1)
+    synth_harmonics_np<LMAX, 1, NP>(srow[wave], u, ux, uy, uz, d, inv_d, acc);
+  else if (reg == 0)
+    synth_harmonics_np<LMAX, 0, NP>(ampL + (long)A * L, u, ux, uy, uz, d, inv_d, acc);
+  else
+    synth_harmonics_np<LMAX, 2, NP>(ampR + (long)A * L, u, ux, uy, uz, d, inv_d, acc);
+#pragma unroll
+  for (int k = 0; k < NP; ++k)
+    if (p[k] >= 0) contrib[(long)A * N + p[k]] = acc[k];
 }
 
 template <int LMAX>
-void launch_synth2_t(dim3 g, dim3 b, hipStream_t s, const AtomTables &at, const double *x,
+void launch_synth2n_t(int np, hipStream_t s, const AtomTables &at, const double *x,
+                      const double *y, const double *z, long N, const int *list,
+                      const int *wave_bucket, long nwaves, const double *T, const double *aL,
+                      const double *aR, double *contrib) {
+  constexpr int WPB = Synth2Cfg<LMAX>::WPB;
+  if (np == 2) {
+    const long ng = nwaves / 2;
+    k_synth2n<LMAX, 2><<<dim3(div_up(ng, WPB)), dim3(64 * WPB), 0, s>>>(
+        at, x, y, z, N, list, wave_bucket, ng, T, aL, aR, contrib);
+  } else {
+    k_synth2n<LMAX, 1><<<dim3(div_up(nwaves, WPB)), dim3(64 * WPB), 0, s>>>(
+        at, x, y, z, N, list, wave_bucket, nwaves, T, aL, aR, contrib);
+  }
+}
+
+template <int LMAX>
+void launch_synth2_t(int np, hipStream_t s, const AtomTables &at, const double *x,
                      const double *y, const double *z, long N, const int *list,
                      const int *wave_bucket, long nwaves, const double *T, const double *aL,
                      const double *aR, double *contrib) {
-  (void)g, (void)b;
-  constexpr int WPB = Synth2Cfg<LMAX>::WPB;
-  k_synth2<LMAX><<<dim3(div_up(nwaves, WPB)), dim3(64 * WPB), 0, s>>>(at, x, y, z, N, list,
-                                                                      wave_bucket, nwaves, T, aL,
-                                                                      aR, contrib);
+  launch_synth2n_t<LMAX>(np, s, at, x, y, z, N, list, wave_bucket, nwaves, T, aL, aR, contrib);
 }
 
 template <int LMAX>

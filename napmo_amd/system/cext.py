@@ -144,6 +144,8 @@ _SIGNATURES = {
     'napmo_becke_dickson_step': (c_int, [vp, c_int, c_int, vp, vp, vp, vp, c_double, vp, vp, vp]),
     'napmo_becke_dickson_step_device': (c_int, [vp, c_int, c_int, vp, vp, vp, vp, c_double, vp,
                                                 vp, vp]),
+    'napmo_grid_profile': (c_int, [vp, c_int]),
+    'napmo_grid_profile_read': (c_int, [vp, vp, c_int, ctypes.POINTER(c_int)]),
     'napmo_fock_plan_new': (c_int, [vp, vp, c_int, c_int, ctypes.POINTER(vp)]),
     'napmo_fock_plan_del': (None, [vp]),
     'napmo_fock_plan_info': (c_int, [vp, ctypes.POINTER(c_long), ctypes.POINTER(c_int),

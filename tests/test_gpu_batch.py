@@ -129,4 +129,11 @@ def test_every_launch_shape_gives_the_same_build():
             F1, E1, Eg1 = f.build(Ps[0])
             assert abs(Eg1 - Eg0) < 1e-12 * abs(Eg0), cand
             assert np.abs(F1 - F0).max() < tol * np.abs(F0).max(), cand
+        # one or two (point, centre) pairs per lane of the synthesis: the same arithmetic
+        # per pair, so the potential and F are bit-identical
+        builds = []
+        for np_ in (1, 2):
+            autotune.apply(f, dict(autotune.DEFAULTS, synth_np=np_))
+            builds.append(f.build(Ps[0]))
+        assert np.array_equal(builds[0][0], builds[1][0]) and builds[0][2] == builds[1][2]
         print(prec, len(autotune.candidates(f)), 'launch shapes agree')
