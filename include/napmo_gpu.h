@@ -101,6 +101,28 @@ double *BeckeGrid_get_origin(void *grid);
 double *BeckeGrid_get_becke_weights(void *grid);
 double BeckeGrid_integrate(void *grid, double *f);
 void BeckeGrid_del(void *grid);
+/* Lowdin2 compatibility grid (src/grid/becke.cpp:57-77, bound at napmo/__init__.py:368-369,
+ * read by napmo/grids/becke.py:42-50): pw [size x 4] rows (x, y, z, Becke weight), weights
+ * 1, ncenter centres without atomic grids (get_origin / get_radii return NULL). Poisson,
+ * kinetic and orbital-correction entry points report an error on such a grid. */
+void *BeckeGrid_from_points(double *pw, int size, int ncenter);
+
+/* ------------------------------------------------------------ MultiGrid (f4 glue) */
+/* src/grid/multi_grid.cpp:14-132 (bound at napmo/__init__.py:400-419): common points of two
+ * species' grids by exact coordinate equality. The reference compares every pair of
+ * points (O(Na Nb)); here the smaller grid's points are hashed (O(Na + Nb)) and the result
+ * is the same list: rows (i, j), i in the larger grid (grid a on a tie), j in the
+ * other, ordered by i then j. Pairs are cached per unordered (a, b); with three or more
+ * grids every pair is kept separately (the reference's pair slot formula collides there). */
+void *MultiGrid_new(int nspecies);
+void MultiGrid_del(void *multi_grid);
+int MultiGrid_add_grid(void *multi_grid, void *grid);
+/* Borrowed [size x 2] int array, valid until MultiGrid_del. a == b: rows (i, i). */
+int *MultiGrid_get_common_index(void *multi_grid, int id_grid_a, int id_grid_b);
+/* Row count of a computed pair, -1 before MultiGrid_get_common_index was called for it. */
+int MultiGrid_get_common_index_size(void *multi_grid, int id_grid_a, int id_grid_b);
+int MultiGrid_get_ngrids(void *multi_grid);
+int MultiGrid_get_nspecies(void *multi_grid);
 
 /* ------------------------------------------- splines, extrapolation, ODE (horton) */
 /* src/horton/cubic_spline.cpp:81-103 */
@@ -227,6 +249,20 @@ int napmo_becke_dickson_step_device(void *grid, int lmax, int norb, const double
                                     const double *T, const double *V_tot, const double *e_prev,
                                     double mass_inv, double *dpsi_out, double *de_out,
                                     double *R_out);
+/* epc17-2 electron-proton correlation (napmo/functional/interspecies_correlation.py:36-66),
+ * elementwise on n densities: c_ene = -rhoN / (a - b sqrt(rhoE rhoN) + c rhoE rhoN) and the
+ * potentials c_pot_a = d(rhoE c_ene)/d rhoE, c_pot_b = d(rhoE c_ene)/d rhoN, with the
+ * reference's choice of formula when any density is below 1e-200. */
+int napmo_epc17_2(long n, const double *rhoE, const double *rhoN, double *c_ene, double *c_pot_a,
+                  double *c_pot_b);
+/* The functional part of compute_c_2species_grid (napmo/wavefunction/psi_analytic.py
+ * :214-262): index [npair x 2] common points (MultiGrid_get_common_index) of grid_a
+ * (electrons, column 0) and grid_b (column 1), rho_a [Na], rho_b [Nb] the densities on
+ * the grids; vrho_a[i] += c_pot_a, vrho_b[j] += c_pot_b and energy = sum over the pairs of
+ * bw_b w_b c_ene rho_a. Every index is checked against the grid sizes. */
+int napmo_epc17_2_grid(void *grid_a, void *grid_b, const int *index, int npair,
+                       const double *rho_a, const double *rho_b, double *vrho_a, double *vrho_b,
+                       double *energy);
 /* HIP-event timing of the orbital corrections on this grid: start (enable = 1) or stop;
  * read gives the mean milliseconds per call of each phase (residual, sources and projection,
  * shell averages, radial solves, synthesis tables, synthesis, reduction) since the start. */

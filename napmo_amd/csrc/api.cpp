@@ -54,10 +54,6 @@ template <class F> auto legacy(const char *name, F &&f) -> decltype(f()) {
   }
 }
 
-// BeckeGrid handle: the device grid plus the atomic grids it was built from.
-struct BeckeHandle {
-  DeviceGrid grid;
-};
 
 static AtomInfo atom_info(const AtomicGrid &g) {
   AtomInfo a;
@@ -281,6 +277,20 @@ void *BeckeGrid_new(void **atgrids, int ncenter, int ablmax, double abldep) {
     return (void *)h;
   });
 }
+void *BeckeGrid_from_points(double *pw, int size, int ncenter) {
+  // src/grid/becke.cpp:57-77 (Lowdin2 grids: rows x, y, z, Becke weight; weights 1)
+  return legacy(__func__, [&] {
+    auto *h = new BeckeHandle;
+    try {
+      h->grid.build_points(pw, size, ncenter);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    return (void *)h;
+  });
+}
+
 int BeckeGrid_get_ncenter(void *g) { return ((BeckeHandle *)g)->grid.M; }
 int BeckeGrid_get_size(void *g) { return (int)((BeckeHandle *)g)->grid.N; }
 double *BeckeGrid_get_radii(void *g) { return ((BeckeHandle *)g)->grid.radii.data(); }
@@ -612,6 +622,48 @@ int napmo_becke_dickson_step_device(void *grid, int lmax, int norb, const double
     DeviceGrid &g = ((BeckeHandle *)grid)->grid;
     NAPMO_REQUIRE(R_out != nullptr, "napmo_becke_dickson_step_device: R_out is required");
     g.becke_dickson_step(lmax, norb, psi, T, V_tot, e_prev, mass_inv, dpsi_out, de_out, R_out);
+  });
+}
+
+int napmo_epc17_2(long n, const double *rhoE, const double *rhoN, double *c_ene, double *c_pot_a,
+                  double *c_pot_b) {
+  return guarded([&] {
+    NAPMO_REQUIRE(n >= 0, "napmo_epc17_2: negative length");
+    if (n == 0) return;
+    hipStream_t st = scratch_stream();
+    DeviceBuffer<double> e, p, ene(n), pa(n), pb(n);
+    DeviceBuffer<int> flag(1);
+    e.upload(rhoE, n, st), p.upload(rhoN, n, st);
+    NAPMO_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), st));
+    launch_epc17_2(e.get(), p.get(), n, flag.get(), ene.get(), pa.get(), pb.get(), st);
+    ene.download(c_ene, n, st), pa.download(c_pot_a, n, st), pb.download(c_pot_b, n, st);
+    NAPMO_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int napmo_epc17_2_grid(void *grid_a, void *grid_b, const int *index, int npair,
+                       const double *rho_a, const double *rho_b, double *vrho_a, double *vrho_b,
+                       double *energy) {
+  return guarded([&] {
+    DeviceGrid &ga = ((BeckeHandle *)grid_a)->grid, &gb = ((BeckeHandle *)grid_b)->grid;
+    NAPMO_REQUIRE(npair >= 0, "napmo_epc17_2_grid: negative pair count");
+    for (long k = 0; k < npair; ++k)
+      NAPMO_REQUIRE(index[2 * k] >= 0 && index[2 * k] < ga.N && index[2 * k + 1] >= 0 &&
+                        index[2 * k + 1] < gb.N,
+                    "napmo_epc17_2_grid: common-point index out of range of the grids");
+    hipStream_t st = gb.stream;
+    if (ga.stream != gb.stream) NAPMO_HIP(hipStreamSynchronize(ga.stream));
+    DeviceBuffer<double> ra, rb, va, vb, part(EPC_PART), e(1);
+    DeviceBuffer<int> idx, flag(1);
+    ra.upload(rho_a, ga.N, st), rb.upload(rho_b, gb.N, st);
+    va.upload(vrho_a, ga.N, st), vb.upload(vrho_b, gb.N, st);
+    idx.upload(index, 2 * (size_t)npair, st);
+    NAPMO_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), st));
+    launch_epc17_2_pairs(ra.get(), rb.get(), idx.get(), npair, gb.wbw.get(), flag.get(), va.get(),
+                         vb.get(), part.get(), e.get(), st);
+    va.download(vrho_a, ga.N, st), vb.download(vrho_b, gb.N, st);
+    e.download(energy, 1, st);
+    NAPMO_HIP(hipStreamSynchronize(st));
   });
 }
 

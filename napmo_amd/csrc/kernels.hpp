@@ -151,6 +151,17 @@ void launch_residual(const double *psi, const double *T, const double *V, const 
                      const double *wbw, int norb, long N, double *R, double *de, double *work,
                      hipStream_t st);
 
+// ---- epc.hip: epc17-2 electron-proton correlation (napmo/functional/interspecies_correlation.py)
+// Elementwise: ene, potE, potN [n] of (rE, rN) [n]; flag: device int set to 0 beforehand.
+void launch_epc17_2(const double *rE, const double *rN, long n, int *flag, double *ene,
+                    double *potE, double *potN, hipStream_t st);
+// On the common points index [n x 2] of two grids: vE[i] += potE, vN[j] += potN and
+// energy[0] = sum (wbwN[j] ene) rE[i]; part: EPC_PART doubles of scratch.
+constexpr int EPC_PART = 256;
+void launch_epc17_2_pairs(const double *rE, const double *rN, const int *index, long n,
+                          const double *wbwN, int *flag, double *vE, double *vN, double *part,
+                          double *energy, hipStream_t st);
+
 // rho[p] = Phi_p^T P Phi_p with FP64 MFMA (P [ld x ld] zero padded). partial [ntile x Npad].
 void launch_density_f64(const double *phi, int ld, long Npad, const double *P, double *partial,
                         hipStream_t s, int ks = 16, int pf = 1);

@@ -392,6 +392,38 @@ void DeviceGrid::build(const std::vector<AtomInfo> &in) {
   NAPMO_HIP(hipStreamSynchronize(stream));
 }
 
+void DeviceGrid::build_points(const double *pw, long n, int ncenter) {
+  require_device();
+  NAPMO_REQUIRE(n > 0, "BeckeGrid_from_points: empty point set");
+  NAPMO_REQUIRE(ncenter >= 0, "BeckeGrid_from_points: negative centre count");
+  if (!stream) NAPMO_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  atoms.clear();
+  M = ncenter;
+  N = n;
+  offsets.assign(1, 0);
+  points.resize(3 * N);
+  weights.assign(N, 1.0);
+  becke.resize(N);
+  origin.clear();
+  radii.clear();
+  std::vector<double> hx(N), hy(N), hz(N);
+  for (long p = 0; p < N; ++p) {
+    for (int c = 0; c < 3; ++c) points[3 * p + c] = pw[4 * p + c];
+    becke[p] = pw[4 * p + 3];
+    hx[p] = pw[4 * p], hy[p] = pw[4 * p + 1], hz[p] = pw[4 * p + 2];
+  }
+  x.upload(hx, stream), y.upload(hy, stream), z.upload(hz, stream), w.upload(weights, stream);
+  bw.upload(becke, stream);
+  wbw.resize(N);
+  launch_scale3(w.get(), bw.get(), nullptr, N, N, wbw.get(), stream);
+  NAPMO_HIP(hipStreamSynchronize(stream));
+}
+
+void DeviceGrid::require_atoms(const char *what) const {
+  NAPMO_REQUIRE(!atoms.empty(), std::string(what) +
+                                    ": this grid was given as points only (no atomic grids)");
+}
+
 AtomTables DeviceGrid::tables(const PoissonPlan &plan) const {
   AtomTables t;
   t.M = M;
@@ -428,6 +460,7 @@ PoissonPlan &DeviceGrid::plan(int lmax) { return build_plan(lmax, false); }
 PoissonPlan &DeviceGrid::kinetic_plan(int lmax) { return build_plan(lmax, true); }
 
 PoissonPlan &DeviceGrid::build_plan(int lmax, bool kinetic) {
+  require_atoms(kinetic ? "kinetic operator" : "Poisson solver");
   NAPMO_REQUIRE(lmax >= 0 && lmax <= MAX_LMAX,
                 "lmax must be in [0, " + std::to_string(MAX_LMAX) + "]");
   auto &cache = kinetic ? kplans : plans;
@@ -561,6 +594,7 @@ HelmholtzTables HelmholtzPlan::tables() const {
 
 HelmholtzPlan &DeviceGrid::helmholtz() {
   if (hplan) return *hplan;
+  require_atoms("orbital correction");
   auto h = std::make_unique<HelmholtzPlan>();
   std::vector<double> c0, oa, rm, j1all, j2all;
   std::vector<long> c0o(M), oao(M), rmo(M);
@@ -689,6 +723,7 @@ void DeviceGrid::poisson_synthesize(int lmax, double *d_v) {
 }
 
 AtomTables DeviceGrid::geometry_tables() const {
+  require_atoms("synthesis");
   AtomTables at;
   at.M = M;
   at.point_off = d_point_off.get();

@@ -129,6 +129,12 @@ struct DeviceGrid {
 
   ~DeviceGrid();
   void build(const std::vector<AtomInfo> &atoms);
+  // A grid given only as points and Becke weights (the Lowdin2 compatibility grid of
+  // src/grid/becke.cpp:57-77): pw [N x 4] rows (x, y, z, bw), weights 1, ncenter centres
+  // without atomic grids. Integration, collocation and contractions work on it; the
+  // Poisson, kinetic and orbital-correction paths need atomic grids and refuse it.
+  void build_points(const double *pw, long n, int ncenter);
+  void require_atoms(const char *what) const;
   AtomTables tables(const PoissonPlan &plan) const;
   PoissonPlan &plan(int lmax);
   // Same projection tables; radial matrices of the kinetic operator: U_lm = the spline
@@ -164,6 +170,11 @@ struct DeviceGrid {
   void synthesize_at(int lmax, const PoissonWork &w, const SynthPairs &sp, const double *tx,
                      const double *ty, const double *tz, long tN, double scale, int accumulate,
                      double *d_v, hipStream_t st);
+};
+
+// What a BeckeGrid_* handle points to.
+struct BeckeHandle {
+  DeviceGrid grid;
 };
 
 // (point, centre) pair buckets of src's centres and radial grids at the target points

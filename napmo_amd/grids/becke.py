@@ -3,6 +3,12 @@
 The Becke weights are computed on the GPU when the grid is built. Beyond the
 reference, atomic grids may differ in size: pass ``sizes=[(nrad, nang), ...]`` (one
 per centre); with equal sizes the result is the reference's.
+
+``file=`` reads a Lowdin2 grid (napmo/grids/becke.py:42-50): a Fortran unformatted file
+with one int64 record (4 x the point count) and one float64 record of the points and
+Becke weights in column-major [npoint x 4] order. Such a grid has points and Becke
+weights only (weights 1, no atomic grids): integration and the contractions work on it,
+the Poisson and kinetic paths do not.
 """
 import ctypes
 
@@ -12,16 +18,29 @@ from napmo_amd.grids.atomic import AtomicGrid
 from napmo_amd.system import cext
 
 
+def read_lowdin2_grid(path):
+    """[npoint x 4] C-ordered (x, y, z, Becke weight) from a Lowdin2 grid file."""
+    from scipy.io import FortranFile
+    with FortranFile(path, 'r') as f:
+        size = int(f.read_ints(dtype=np.int64)[0])
+        pw = f.read_reals(dtype=np.float64)
+    pw = pw.reshape(int(size / 4), 4, order='F')
+    return np.ascontiguousarray(pw)
+
+
 class BeckeGrid(object):
 
     def __init__(self, species, n_radial=40, n_angular=110, rtransform=None, file=None, ablmax=None,
                  abldep=None, sizes=None):
         assert isinstance(species, dict)
-        if file is not None:
-            raise NotImplementedError('Lowdin2 grid files are outside the grid hot path')
         centers = species.get('particles')
         ncenter = len(centers)
         self._symbol = species.get('symbol')
+        if file is not None:
+            pw = read_lowdin2_grid(file)
+            self.atgrids = []
+            self._this = cext.BeckeGrid_from_points(pw, pw.shape[0], ncenter)
+            return
         ablmax = 0 if ablmax is None else ablmax
         abldep = 1.0e-6 if abldep is None else abldep
         self._nrad = n_radial

@@ -101,6 +101,14 @@ _SIGNATURES = {
     'BeckeGrid_get_becke_weights': (dptr, [vp]),
     'BeckeGrid_integrate': (c_double, [vp, a1df]),
     'BeckeGrid_del': (None, [vp]),
+    'BeckeGrid_from_points': (vp, [a2df, c_int, c_int]),
+    'MultiGrid_new': (vp, [c_int]),
+    'MultiGrid_del': (None, [vp]),
+    'MultiGrid_add_grid': (c_int, [vp, vp]),
+    'MultiGrid_get_common_index': (ctypes.POINTER(c_int), [vp, c_int, c_int]),
+    'MultiGrid_get_common_index_size': (c_int, [vp, c_int, c_int]),
+    'MultiGrid_get_ngrids': (c_int, [vp]),
+    'MultiGrid_get_nspecies': (c_int, [vp]),
     # splines, extrapolation, ODE
     'solve_cubic_spline_system': (None, [a1df, a1df, c_int]),
     'CubicSpline_new': (vp, [a1df, a1df, vp, vp, c_int]),
@@ -145,6 +153,8 @@ _SIGNATURES = {
     'napmo_becke_dickson_step_device': (c_int, [vp, c_int, c_int, vp, vp, vp, vp, c_double, vp,
                                                 vp, vp]),
     'napmo_grid_profile': (c_int, [vp, c_int]),
+    'napmo_epc17_2': (c_int, [c_long, vp, vp, vp, vp, vp]),
+    'napmo_epc17_2_grid': (c_int, [vp, vp, vp, c_int, vp, vp, vp, vp, vp]),
     'napmo_grid_profile_read': (c_int, [vp, vp, c_int, ctypes.POINTER(c_int)]),
     'napmo_fock_plan_new': (c_int, [vp, vp, c_int, c_int, ctypes.POINTER(vp)]),
     'napmo_fock_plan_del': (None, [vp]),

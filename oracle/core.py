@@ -24,6 +24,13 @@ Pinning (what fixes each function besides the reference text):
 * eval_decomposition    -- oracle/_ref eval_decomposition_grid; re-synthesis of rho
 * basis_compute         -- oracle/_ref BasisSet_compute, reference test constants
 * contractions, energy  -- textbook identities (einsum), trace identities
+* compute_dpsi          -- exact solution of its radial equations on one centre
+                           (fourth-order convergence), planted-mistake checks
+* common_index          -- constructed grids with known shared points, swap and order
+                           rules, signed zero
+* epc17_2               -- the published closed form at hand-computed points; both
+                           potential forms equal the derivatives of rhoE c_ene
+* read_lowdin2_grid     -- a file written with known content
 """
 from fractions import Fraction
 import json
@@ -1271,3 +1278,63 @@ def grid_fock_build(grid, phi, P, lmax, linsolve=None):
     v = compute_coulomb(grid, rho, lmax, linsolve)
     J = coupling_matrix(phi, v, grid.becke_weights, grid.weights)
     return rho, v, J, coulomb_energy(P, J)
+
+
+# ----------------------------------------------------------------------------------
+# Cross-species glue: Lowdin2 grids, common points, epc17-2
+# ----------------------------------------------------------------------------------
+
+def read_lowdin2_grid(path):
+    """napmo/grids/becke.py:42-50: int64 record (4 x npoint), float64 record reshaped
+    [npoint x 4] column-major; rows (x, y, z, Becke weight), src/grid/becke.cpp:57-77."""
+    from scipy.io import FortranFile
+    with FortranFile(path, 'r') as f:
+        size = f.read_ints(dtype=np.int64)[0]
+        pw = f.read_reals(dtype=np.float64)
+    pw = pw.reshape(int(size / 4), 4, order='F')
+    return np.ascontiguousarray(pw)
+
+
+def common_index(points_a, points_b):
+    """MultiGrid::get_common_index (src/grid/multi_grid.cpp:37-86): the grids are swapped
+    when b is larger, then every (i, j) with equal coordinates in i-major, j-minor order,
+    i in the (possibly swapped) first grid. Returns int array [n x 2]."""
+    A, B = np.asarray(points_a), np.asarray(points_b)
+    if B.shape[0] > A.shape[0]:
+        A, B = B, A
+    rows = []
+    for i in range(A.shape[0]):
+        eq = (A[i, 0] == B[:, 0]) & (A[i, 1] == B[:, 1]) & (A[i, 2] == B[:, 2])
+        for j in np.flatnonzero(eq):
+            rows.append((i, int(j)))
+    return np.array(rows, dtype=np.int64).reshape(-1, 2)
+
+
+def epc17_2(rhoE, rhoN):
+    """napmo/functional/interspecies_correlation.py:36-66 (same expressions, same branch
+    rule: the simplified potentials for the whole array when any density < 1e-200)."""
+    a, b, c = 2.35, 2.4, 6.6
+    denominator = a - b * np.sqrt(rhoE * rhoN) + c * rhoE * rhoN
+    c_ene = -rhoN / denominator
+    if np.any(np.abs(rhoE) < 1.0e-200) or np.any(np.abs(rhoN) < 1.0e-200):
+        c_pot_A = (b * np.sqrt(rhoE) * rhoN ** (1.5) - 2.0 * a * rhoN) / denominator ** 2 / 2.0
+        c_pot_B = (b * np.sqrt(rhoN) * rhoE ** (1.5) - 2.0 * a * rhoE) / denominator ** 2 / 2.0
+    else:
+        c_pot_A = (rhoE * rhoN * (c * rhoN - b * rhoN / (2.0 * np.sqrt(rhoE * rhoN)))
+                   - rhoN * denominator) / denominator ** 2
+        c_pot_B = (rhoN * rhoE * (c * rhoE - b * rhoE / (2.0 * np.sqrt(rhoE * rhoN)))
+                   - rhoE * denominator) / denominator ** 2
+    return c_ene, c_pot_A, c_pot_B
+
+
+def c_2species_grid(index, rho_a, rho_b, bw_b, w_b, vrho_a, vrho_b):
+    """The functional step of compute_c_2species_grid (napmo/wavefunction/psi_analytic.py
+    :241-258): densities taken at the common points, epc17-2, energy
+    (bw_b w_b c_zk rho).sum(), potentials added at the common points (in place)."""
+    rho = np.take(rho_a, index[:, 0], axis=0)
+    other_rho = np.take(rho_b, index[:, 1], axis=0)
+    c_zk, c_vrho, c_other_vrho = epc17_2(rho, other_rho)
+    aux = (bw_b[index[:, 1]] * w_b[index[:, 1]] * c_zk * rho).sum()
+    vrho_a[index[:, 0]] += c_vrho
+    vrho_b[index[:, 1]] += c_other_vrho
+    return aux

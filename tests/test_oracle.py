@@ -386,3 +386,66 @@ def test_becke_dickson_step_residual_and_energy():
     assert de[0] == grid.integrate(R[0] * psi[0])
     ref = core.compute_dpsi(grid, 2, psi[0], de[0], e[0], R[0], V[0], 1.0)
     assert np.array_equal(dpsi[0], ref)
+
+
+def test_common_index_known_answer():
+    # multi_grid.cpp:37-86: rows (i in the larger grid, j), i-major then j; -0.0 == 0.0;
+    # a duplicated point of the smaller grid is listed for each of its copies
+    A = np.array([[0.0, 0.0, 0.0], [1.0, 2.0, -0.0], [5.0, 5.0, 5.0], [0.5, 0.25, 3.0],
+                  [9.0, 9.0, 9.0]])
+    B = np.array([[0.5, 0.25, 3.0], [7.0, 7.0, 7.0], [1.0, 2.0, 0.0], [0.5, 0.25, 3.0]])
+    idx = core.common_index(A, B)
+    assert idx.tolist() == [[1, 2], [3, 0], [3, 3]]
+    # a smaller first grid is swapped: column 0 then indexes B
+    idx = core.common_index(A[:3], B)
+    assert idx.tolist() == [[2, 1]]
+    assert core.common_index(A[2:3], B).shape == (0, 2)
+
+
+def test_epc17_2_closed_form_and_derivatives():
+    # the published form eps = -rN / (2.35 - 2.4 sqrt(rE rN) + 6.6 rE rN) at rE = rN = 1:
+    # 2.35 - 2.4 + 6.6 = 6.55
+    e, _, _ = core.epc17_2(np.array([1.0]), np.array([1.0]))
+    assert e[0] == -1.0 / 6.55
+    # both potential forms are the derivatives of rE eps (central differences)
+    rng = np.random.default_rng(2)
+    rE = rng.uniform(0.01, 3.0, 50)
+    rN = rng.uniform(0.01, 3.0, 50)
+    for small in (False, True):
+        xE, xN = rE.copy(), rN.copy()
+        if small:       # one zero density anywhere switches the whole array's form
+            xE = np.append(xE, 0.0)
+            xN = np.append(xN, 1.0)
+        _, pA, pB = core.epc17_2(xE, xN)
+        h = 1e-6
+        f = lambda a, b: a * core.epc17_2(a, b)[0]  # noqa: E731
+        with np.errstate(invalid='ignore'):   # the appended zero density goes negative
+            dA = (f(xE + h, xN) - f(xE - h, xN)) / (2 * h)
+            dB = (f(xE, xN + h) - f(xE, xN - h)) / (2 * h)
+        n = len(rE)
+        assert np.allclose(pA[:n], dA[:n], rtol=1e-7, atol=1e-9)
+        assert np.allclose(pB[:n], dB[:n], rtol=1e-7, atol=1e-9)
+
+
+def test_lowdin2_grid_file(tmp_path):
+    from scipy.io import FortranFile
+    rng = np.random.default_rng(4)
+    pw = rng.normal(size=(7, 4))
+    path = str(tmp_path / 'grid.dat')
+    with FortranFile(path, 'w') as f:
+        f.write_record(np.array([pw.size], dtype=np.int64))
+        f.write_record(np.asfortranarray(pw).ravel(order='F'))
+    out = core.read_lowdin2_grid(path)
+    assert np.array_equal(out, pw)
+
+
+def test_c_2species_grid_scatter_and_energy():
+    index = np.array([[2, 0], [0, 1]])
+    rho_a = np.array([0.5, 0.1, 2.0])
+    rho_b = np.array([1.5, 0.7])
+    bw, w = np.array([0.9, 0.4]), np.array([2.0, 3.0])
+    va, vb = np.zeros(3), np.zeros(2)
+    E = core.c_2species_grid(index, rho_a, rho_b, bw, w, va, vb)
+    ene, pA, pB = core.epc17_2(np.array([2.0, 0.5]), np.array([1.5, 0.7]))
+    assert np.isclose(E, 0.9 * 2.0 * ene[0] * 2.0 + 0.4 * 3.0 * ene[1] * 0.5, rtol=1e-15)
+    assert va.tolist() == [pA[1], 0.0, pA[0]] and vb.tolist() == [pB[0], pB[1]]
