@@ -77,7 +77,7 @@ def _save(path, cache):
     os.replace(tmp, path)
 
 
-def tune(fock, dP, dF, dE, reps=3, path=None, force=False, build=None, tag=''):
+def tune(fock, dP, dF, dE, reps=5, path=None, force=False, build=None, tag=''):
     """Choose and apply the fastest launch shape for ``fock`` (device buffers dP, dF, dE
     of the right sizes are used for the timing builds, or ``build()`` when given, e.g. an
     orbital-form build; ``tag`` then names that build in the cache key). Returns

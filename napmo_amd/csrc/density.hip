@@ -9,6 +9,8 @@
 // Phi rows and B rows are loaded into registers while the MFMAs of the current stage run,
 // then stored to the other LDS buffer (one barrier per stage). LDS rows are padded so each
 // 16 x 4 fragment read of a wave touches every bank once per half.
+#include <algorithm>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -39,7 +41,9 @@ __device__ inline bool xcd_tile(int g, int nct, int npt, int &pt, int &ct) {
 template <int NB, int MODE, int DKS, int PF>
 __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__ phi, int ld,
                                                       long Npad, const double *__restrict__ B,
-                                                      int ldb, double *__restrict__ partial) {
+                                                      int ldb, int kend, double scale,
+                                                      long nvalid,
+                                                      double *__restrict__ partial) {
   constexpr int BS = bstride(NB);
   __shared__ double As[2][DKS][ASTRIDE];  // As[.][k][m] = Phi[p0 + m][k0 + k]
   __shared__ double Bs[2][DKS][BS];       // Bs[.][k][c] = B[k0 + k][c0 + c]
@@ -95,7 +99,7 @@ __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
   };
-  const int nstage = ld / DKS;
+  const int nstage = kend / DKS;  // K stages holding real functions (the rest are zero padding)
   d2 ra0[NA], ra1[NA];
   double rb0[NBQ], rb1[NBQ];
   if (PF == 1) {
@@ -156,20 +160,24 @@ __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__
       sum += __shfl_xor(sum, 4);
       sum += __shfl_xor(sum, 2);
       sum += __shfl_xor(sum, 1);
-      if (mi == 0) partial[(long)ct * Npad + p] = sum;
+      if (mi == 0 && p < nvalid) partial[(long)ct * Npad + p] = scale * sum;
     }
 }
 
 template <int MODE, int DKS, int PF>
 void launch_density_ks(int nb, dim3 tiles, const double *phi, int ld, long Npad, const double *B,
-                       int ldb, double *partial, hipStream_t s) {
+                       int ldb, int nreal, double scale, long nvalid, double *partial,
+                       hipStream_t s) {
   // tiles = (point tiles, column tiles) -> one 1-D grid in the XCD-aware order
   const dim3 grid((unsigned)(round_up(tiles.x, 8) * tiles.y));
+  // Phi columns and B rows from nreal on are zero: the K loop stops at the first stage
+  // boundary past them (the skipped products are exact zeros)
+  const int kend = (nreal > 0 && nreal < ld) ? (int)std::min<long>(ld, round_up(nreal, DKS)) : ld;
   switch (nb) {
-    case 1: k_density_mfma<1, MODE, DKS, PF><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
-    case 2: k_density_mfma<2, MODE, DKS, PF><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
-    case 3: k_density_mfma<3, MODE, DKS, PF><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
-    default: k_density_mfma<4, MODE, DKS, PF><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, partial); break;
+    case 1: k_density_mfma<1, MODE, DKS, PF><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, kend, scale, nvalid, partial); break;
+    case 2: k_density_mfma<2, MODE, DKS, PF><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, kend, scale, nvalid, partial); break;
+    case 3: k_density_mfma<3, MODE, DKS, PF><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, kend, scale, nvalid, partial); break;
+    default: k_density_mfma<4, MODE, DKS, PF><<<grid, 256, 0, s>>>(phi, ld, Npad, B, ldb, kend, scale, nvalid, partial); break;
   }
   NAPMO_HIP(hipGetLastError());
 }
@@ -177,27 +185,29 @@ void launch_density_ks(int nb, dim3 tiles, const double *phi, int ld, long Npad,
 // ks: K per stage, 16 or 32 (32 needs ld % 32 == 0); pf: stages in flight, 1 or 2
 template <int MODE>
 void launch_density_nb(int nb, dim3 grid, const double *phi, int ld, long Npad, const double *B,
-                       int ldb, double *partial, hipStream_t s, int ks = 16, int pf = 1) {
+                       int ldb, double *partial, hipStream_t s, int ks = 16, int pf = 1,
+                       int nreal = 0, double scale = 1.0, long nvalid = -1) {
+  if (nvalid < 0) nvalid = Npad;
   if (ks == 32 && ld % 32 == 0) {
     if (pf == 2)
-      launch_density_ks<MODE, 32, 2>(nb, grid, phi, ld, Npad, B, ldb, partial, s);
+      launch_density_ks<MODE, 32, 2>(nb, grid, phi, ld, Npad, B, ldb, nreal, scale, nvalid, partial, s);
     else
-      launch_density_ks<MODE, 32, 1>(nb, grid, phi, ld, Npad, B, ldb, partial, s);
+      launch_density_ks<MODE, 32, 1>(nb, grid, phi, ld, Npad, B, ldb, nreal, scale, nvalid, partial, s);
   } else {
     if (pf == 2)
-      launch_density_ks<MODE, 16, 2>(nb, grid, phi, ld, Npad, B, ldb, partial, s);
+      launch_density_ks<MODE, 16, 2>(nb, grid, phi, ld, Npad, B, ldb, nreal, scale, nvalid, partial, s);
     else
-      launch_density_ks<MODE, 16, 1>(nb, grid, phi, ld, Npad, B, ldb, partial, s);
+      launch_density_ks<MODE, 16, 1>(nb, grid, phi, ld, Npad, B, ldb, nreal, scale, nvalid, partial, s);
   }
 }
 
 }  // namespace
 
 void launch_density_f64(const double *phi, int ld, long Npad, const double *P, double *partial,
-                        hipStream_t s, int ks, int pf) {
+                        hipStream_t s, int ks, int pf, int nreal) {
   NAPMO_REQUIRE(ld % TILE == 0 && Npad % DPT == 0, "density: padding mismatch");
   launch_density_nb<MODE_P>(4, dim3(Npad / DPT, ld / TILE), phi, ld, Npad, P, ld, partial, s, ks,
-                            pf);
+                            pf, nreal);
 }
 
 int density_orb_cols(int nocc) {
@@ -208,12 +218,15 @@ int density_orb_cols(int nocc) {
 int density_orb_tiles(int ldc) { return ldc <= 64 ? 1 : ldc / 64; }
 
 void launch_density_orb_f64(const double *phi, int ld, long Npad, const double *C, int ldc,
-                            double *partial, hipStream_t s, int ks, int pf) {
+                            double *partial, hipStream_t s, int ks, int pf, int nreal, double scale,
+                            long nvalid) {
   NAPMO_REQUIRE(Npad % DPT == 0 && ld % 16 == 0 && ldc % 16 == 0, "density_orb: padding");
   const int nb = ldc <= 64 ? ldc / 16 : 4;
   NAPMO_REQUIRE(ldc <= 64 || ldc % 64 == 0, "density_orb: orbital padding");
+  NAPMO_REQUIRE(nvalid < 0 || density_orb_tiles(ldc) == 1,
+                "density_orb: a direct (scaled) write needs one column tile");
   launch_density_nb<MODE_ORB>(nb, dim3(Npad / DPT, density_orb_tiles(ldc)), phi, ld, Npad, C, ldc,
-                              partial, s, ks, pf);
+                              partial, s, ks, pf, nreal, scale, nvalid);
 }
 
 void launch_phi_times(const double *phi, int ld, long Npad, const double *B, int ldb, double *X,

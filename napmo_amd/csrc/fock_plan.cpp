@@ -164,7 +164,7 @@ void FockPlan::density_P(int slot, const double *dP) {
   // P (n x n) into the zero-padded ld x ld operand (a kernel, so the build can be captured)
   launch_pad2d(dP, n, n, P.get(), ld, ld, st);
   mark(slot, 0);
-  launch_density_f64(phi64.get(), ld, Npad, P.get(), rho_part.get(), st, density_ks, density_pf);
+  launch_density_f64(phi64.get(), ld, Npad, P.get(), rho_part.get(), st, density_ks, density_pf, n);
   mark(slot, 1);
   launch_rho_reduce(rho_part.get(), ld / TILE, Npad, grid.N, rho.get(), st);
 }
@@ -179,9 +179,17 @@ void FockPlan::density_C(int slot, const double *dC, int nocc, double eta) {
   launch_pad2d(dC, n, nocc, Cpad.get(), ld, ldc, st);
   launch_gram(dC, n, nocc, eta, Pfull.get(), st);
   mark(slot, 0);
-  launch_density_orb_f64(phi64.get(), ld, Npad, Cpad.get(), ldc, rho_part.get(), st, density_ks, density_pf);
-  mark(slot, 1);
-  launch_rho_reduce(rho_part.get(), nct, Npad, grid.N, rho.get(), st, eta);
+  if (nct == 1) {
+    // one column tile: the kernel's epilogue writes rho = eta sum_i X_pi^2 itself
+    launch_density_orb_f64(phi64.get(), ld, Npad, Cpad.get(), ldc, rho.get(), st, density_ks,
+                           density_pf, n, eta, grid.N);
+    mark(slot, 1);
+  } else {
+    launch_density_orb_f64(phi64.get(), ld, Npad, Cpad.get(), ldc, rho_part.get(), st, density_ks,
+                           density_pf, n);
+    mark(slot, 1);
+    launch_rho_reduce(rho_part.get(), nct, Npad, grid.N, rho.get(), st, eta);
+  }
 }
 
 void FockPlan::expand(int slot) {
@@ -197,7 +205,7 @@ void FockPlan::expand(int slot) {
                       work.ampR.get(), st);
 }
 
-void FockPlan::synth_self(int slot) {
+void FockPlan::synth_self(int slot, bool with_scale) {
   hipStream_t st = stream;
   PoissonPlan &pl = grid.plan(lmax);
   AtomTables at = grid.tables(pl);
@@ -207,12 +215,17 @@ void FockPlan::synth_self(int slot) {
                 sp.wave_bucket.get(), sp.nwaves, work.T.get(), work.ampL.get(), work.ampR.get(), lmax,
                 sp.contrib.get(), st, grid.synth_np);
   mark(slot, 6);  // the "scale" phase: centre sum, then w bw v
-  launch_contrib_reduce(sp.contrib.get(), grid.M, grid.N, 0, v.get(), st);
+  if (with_scale)
+    launch_contrib_reduce_scale(sp.contrib.get(), grid.M, grid.N, Npad, grid.wbw.get(), v.get(),
+                                s.get(), st);
+  else
+    launch_contrib_reduce(sp.contrib.get(), grid.M, grid.N, 0, v.get(), st);
 }
 
-void FockPlan::contract(int slot, const double *dP, const double *veff, double *dF, double *dE) {
+void FockPlan::contract(int slot, const double *dP, const double *veff, double *dF, double *dE,
+                        bool s_ready) {
   hipStream_t st = stream;
-  launch_scale3(veff, grid.wbw.get(), nullptr, grid.N, Npad, s.get(), st);
+  if (!s_ready) launch_scale3(veff, grid.wbw.get(), nullptr, grid.N, Npad, s.get(), st);
   mark(slot, 7);
   // the one-pass kernel writes FP32 partials into the same buffer
   float *part32 = reinterpret_cast<float *>(F_part.get());
@@ -254,8 +267,8 @@ void FockPlan::build_device(const double *dP, double *dF, double *dE) {
   const int slot = next_slot();
   density_P(slot, dP);
   expand(slot);
-  synth_self(slot);
-  contract(slot, dP, v.get(), dF, dE);
+  synth_self(slot, true);
+  contract(slot, dP, v.get(), dF, dE, true);
 }
 
 void FockPlan::build_orbitals_device(const double *dC, int nocc, double eta, double *dF,
@@ -263,8 +276,8 @@ void FockPlan::build_orbitals_device(const double *dC, int nocc, double eta, dou
   const int slot = next_slot();
   density_C(slot, dC, nocc, eta);
   expand(slot);
-  synth_self(slot);
-  contract(slot, Pfull.get(), v.get(), dF, dE);
+  synth_self(slot, true);
+  contract(slot, Pfull.get(), v.get(), dF, dE, true);
 }
 
 void FockPlan::build_orbitals_host(const double *C, int nocc, double eta, double *F_out,

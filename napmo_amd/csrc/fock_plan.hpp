@@ -63,9 +63,12 @@ struct FockPlan {
   // Poisson expansion of rho (projection, radial Green solves, synthesis tables),
   void expand(int slot);
   // v = synthesis of this grid's own expansion at its own points,
-  void synth_self(int slot);
+  // with_scale: also write s = w bw v (the contraction's weights) in the centre-sum pass
+  void synth_self(int slot, bool with_scale = false);
   // F = Phi^T diag(w bw veff) Phi and the FP64 traces with veff [N] (P: unpadded n x n).
-  void contract(int slot, const double *dP, const double *veff, double *dF, double *dE);
+  // s_ready: s = w bw veff was already written (synth_self with_scale, veff = v)
+  void contract(int slot, const double *dP, const double *veff, double *dF, double *dE,
+                bool s_ready = false);
   int next_slot();
   void build_host(const double *P, double *F_out, double *E_out, double *rho_out, double *v_out);
   // S = Phi^T diag(w bw) Phi [n x n] from the resident Phi (FP64).

@@ -446,10 +446,34 @@ __global__ void k_contrib_reduce(const double *__restrict__ contrib, int M, long
   v[p] = accumulate ? v[p] + scale * s : scale * s;
 }
 
+// v = sum_A contrib[A] and, in the same pass, sv = v wbw (zero for N <= p < Npad).
+__global__ void k_contrib_reduce_scale(const double *__restrict__ contrib, int M, long N, long Npad,
+                                       const double *__restrict__ wbw, double *__restrict__ v,
+                                       double *__restrict__ sv) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= Npad) return;
+  if (p >= N) {
+    sv[p] = 0.0;
+    return;
+  }
+  double s = 0.0;
+  for (int A = 0; A < M; ++A) s += contrib[(long)A * N + p];
+  const double vp = 1.0 * s;
+  v[p] = vp;
+  sv[p] = vp * wbw[p];
+}
+
 void launch_contrib_reduce(const double *contrib, int M, long N, int accumulate, double *v,
                            hipStream_t s, double scale) {
   if (N <= 0) return;
   k_contrib_reduce<<<div_up(N, 256), 256, 0, s>>>(contrib, M, N, accumulate, scale, v);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_contrib_reduce_scale(const double *contrib, int M, long N, long Npad, const double *wbw,
+                                 double *v, double *sv, hipStream_t s) {
+  if (Npad <= 0) return;
+  k_contrib_reduce_scale<<<div_up(Npad, 256), 256, 0, s>>>(contrib, M, N, Npad, wbw, v, sv);
   NAPMO_HIP(hipGetLastError());
 }
 

@@ -106,6 +106,10 @@ void launch_synth2(const AtomTables &at, const double *x, const double *y, const
                    hipStream_t s, int np = 2);
 void launch_contrib_reduce(const double *contrib, int M, long N, int accumulate, double *v,
                            hipStream_t s, double scale = 1.0);
+// v = sum_A contrib[A] and sv = v wbw in one pass (sv [Npad], zero beyond N): the same
+// values as launch_contrib_reduce followed by launch_scale3(v, wbw).
+void launch_contrib_reduce_scale(const double *contrib, int M, long N, long Npad, const double *wbw,
+                                 double *v, double *sv, hipStream_t s);
 // Point-charge potential V[p] = sum_A q_A / |p - R_A| (o: [nq x 3]).
 void launch_nuclear(const double *x, const double *y, const double *z, long N, const double *q,
                     const double *o, int nq, double *V, hipStream_t s);
@@ -163,8 +167,10 @@ void launch_epc17_2_pairs(const double *rE, const double *rN, const int *index, 
                           double *energy, hipStream_t st);
 
 // rho[p] = Phi_p^T P Phi_p with FP64 MFMA (P [ld x ld] zero padded). partial [ntile x Npad].
+// nreal: functions actually present (columns nreal .. ld - 1 of Phi and rows of P are zero),
+// 0 = ld.
 void launch_density_f64(const double *phi, int ld, long Npad, const double *P, double *partial,
-                        hipStream_t s, int ks = 16, int pf = 1);
+                        hipStream_t s, int ks = 16, int pf = 1, int nreal = 0);
 // rho[p] = scale * sum_t partial[t][p] for p < N.
 void launch_rho_reduce(const double *partial, int ntile, long Npad, long N, double *rho,
                        hipStream_t s, double scale = 1.0);
@@ -230,8 +236,11 @@ void launch_grad_centres(const double *x, const double *y, const double *z, long
                          const double *pc, const int *centre_of, int ncentre, double *partial,
                          double *grad, hipStream_t st);
 int density_orb_tiles(int ldc);
+// With one column tile (ldc <= 64), nvalid >= 0 makes the kernel write
+// partial[p] = scale * sum for p < nvalid directly (rho itself, no reduction pass).
 void launch_density_orb_f64(const double *phi, int ld, long Npad, const double *C, int ldc,
-                            double *partial, hipStream_t s, int ks = 16, int pf = 1);
+                            double *partial, hipStream_t s, int ks = 16, int pf = 1, int nreal = 0,
+                            double scale = 1.0, long nvalid = -1);
 void launch_gram(const double *C, int n, int nocc, double eta, double *P, hipStream_t s);
 void launch_pad2d(const double *src, int rows, int cols, double *dst, int drows, int dcols,
                   hipStream_t s);
