@@ -268,7 +268,7 @@ template <int LMAX> struct Synth2Cfg {
 // NP pairs per lane: a wavefront takes NP consecutive 64-pair slots of one bucket (the
 // lists are padded to 64 NP per bucket) and reads each staged coefficient once for all.
 template <int LMAX, int NP>
-__global__ void __launch_bounds__(256) k_synth2n(AtomTables at, const double *__restrict__ px,
+__global__ void __launch_bounds__(256, NP == 1 ? 5 : 4) k_synth2n(AtomTables at, const double *__restrict__ px,
                                                  const double *__restrict__ py,
                                                  const double *__restrict__ pz, long N,
                                                  const int *__restrict__ list,
