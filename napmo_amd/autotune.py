@@ -39,6 +39,8 @@ def candidates(fock):
     """Launch shapes of the Fock contraction, each with both density stage depths."""
     if fock.precision == 'bf16x3':
         out = [{'use_pass': 1, 'pass_wg': w} for w in (128, 256, 512, 1024)]
+        # the symmetric form only with short chunks (its FP32 chunk sums drift otherwise)
+        out += [{'use_pass': 2, 'pass_wg': w} for w in (1024, 2048)]
         out += [{'use_pass': 0, 'fock_wg': w} for w in (1024, 2048, 4096)]
     else:
         out = [{'fock_wg': w} for w in (512, 1024, 2048, 4096, 8192)]

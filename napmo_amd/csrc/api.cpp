@@ -762,6 +762,10 @@ int napmo_selftest_mfma(int which, double *out) {
 int napmo_grid_overlap(void *plan, double *S_out) {
   return guarded([&] { ((FockPlan *)plan)->overlap(S_out); });
 }
+
+int napmo_fock_plan_contract(void *plan, const double *v, double *F_out) {
+  return guarded([&] { ((FockPlan *)plan)->contract_potential(v, F_out); });
+}
 int napmo_fock_plan_profile(void *plan, int max_builds) {
   return guarded([&] { ((FockPlan *)plan)->profile_start(max_builds); });
 }

@@ -63,7 +63,7 @@ FockPlan::FockPlan(DeviceGrid &g, const BasisSet &b, int lmax_, int prec_mode)
   rho.resize(grid.N);
   v.resize(grid.N);
   s.resize(Npad);
-  use_pass = prec == NAPMO_PREC_BF16X3 && fock_pass_supported(ld);
+  use_pass = (prec == NAPMO_PREC_BF16X3 && fock_pass_supported(ld)) ? 1 : 0;
   reshape();
   F.resize((size_t)n * n);
   E.resize(2);
@@ -83,9 +83,11 @@ void FockPlan::set_tuning(int key, long value) {
     case NAPMO_TUNE_FOCK_WG: fock_wg = value; break;
     case NAPMO_TUNE_PASS_WG: pass_wg = value; break;
     case NAPMO_TUNE_USE_PASS:
-      NAPMO_REQUIRE(!value || (prec == NAPMO_PREC_BF16X3 && fock_pass_supported(ld)),
+      NAPMO_REQUIRE(value >= 0 && value <= 2, "one-pass kernel choice must be 0, 1 or 2");
+      NAPMO_REQUIRE(value == 0 || (prec == NAPMO_PREC_BF16X3 &&
+                                   (value == 1 ? fock_pass_supported(ld) : fock_syrk_supported(ld))),
                     "one-pass kernel not available for this plan");
-      use_pass = value != 0;
+      use_pass = (int)value;
       break;
     case NAPMO_TUNE_DENSITY_PF:
       NAPMO_REQUIRE(value == 0 || value == 1 || value == 2, "density prefetch depth must be 1 or 2");
@@ -110,7 +112,7 @@ long FockPlan::get_tuning(int key) const {
   switch (key) {
     case NAPMO_TUNE_FOCK_WG: return (long)fsh.nchunk * fsh.ntiles;
     case NAPMO_TUNE_PASS_WG: return (long)psh.nchunk;
-    case NAPMO_TUNE_USE_PASS: return use_pass ? 1 : 0;
+    case NAPMO_TUNE_USE_PASS: return use_pass;
     case NAPMO_TUNE_DENSITY_KS: return density_ks;
     case NAPMO_TUNE_DENSITY_PF: return density_pf;
     case NAPMO_TUNE_SYNTH_NP: return grid.synth_np;
@@ -229,7 +231,9 @@ void FockPlan::contract(int slot, const double *dP, const double *veff, double *
   mark(slot, 7);
   // the one-pass kernel writes FP32 partials into the same buffer
   float *part32 = reinterpret_cast<float *>(F_part.get());
-  if (use_pass)
+  if (use_pass == 2)
+    launch_fock_syrk_pass(psh, phiT32.get(), s.get(), part32, st);
+  else if (use_pass == 1)
     launch_fock_bf16x3_pass(psh, phiT_hi.get(), phiT_lo.get(), s.get(), part32, st);
   else if (prec == NAPMO_PREC_BF16X3)
     launch_fock_bf16x3(fsh, phiT_hi.get(), phiT_lo.get(), phiT32.get(), s.get(), F_part.get(), st);
@@ -290,6 +294,28 @@ void FockPlan::build_orbitals_host(const double *C, int nocc, double eta, double
   if (E_out) E.download(E_out, 2, st);
   if (rho_out) rho.download(rho_out, grid.N, st);
   if (v_out) v.download(v_out, grid.N, st);
+  NAPMO_HIP(hipStreamSynchronize(st));
+}
+
+void FockPlan::contract_potential(const double *v_host, double *F_out) {
+  hipStream_t st = stream;
+  DeviceBuffer<double> dv;
+  dv.upload(v_host, grid.N, st);
+  launch_scale3(dv.get(), grid.wbw.get(), nullptr, grid.N, Npad, s.get(), st);
+  float *part32 = reinterpret_cast<float *>(F_part.get());
+  if (use_pass == 2)
+    launch_fock_syrk_pass(psh, phiT32.get(), s.get(), part32, st);
+  else if (use_pass == 1)
+    launch_fock_bf16x3_pass(psh, phiT_hi.get(), phiT_lo.get(), s.get(), part32, st);
+  else if (prec == NAPMO_PREC_BF16X3)
+    launch_fock_bf16x3(fsh, phiT_hi.get(), phiT_lo.get(), phiT32.get(), s.get(), F_part.get(), st);
+  else
+    launch_fock_f64(fsh, phi64.get(), phi64.get(), s.get(), F_part.get(), st);
+  if (use_pass)
+    launch_fock_reduce_f32(psh, part32, n, F.get(), st);
+  else
+    launch_fock_reduce(fsh, F_part.get(), n, 0, 1.0, 0, F.get(), st);
+  F.download(F_out, (size_t)n * n, st);
   NAPMO_HIP(hipStreamSynchronize(st));
 }
 

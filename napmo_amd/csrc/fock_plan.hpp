@@ -32,7 +32,7 @@ struct FockPlan {
   DeviceBuffer<double> Cpad, Pfull;    // orbital-form inputs
   DeviceBuffer<double> P, rho_part, rho, v, s, F_part, F, E, trace_work;
   FockShape fsh, psh;  // per-tile shape; one-pass shape (mixed mode)
-  bool use_pass = false;
+  int use_pass = 0;  // 0: per-tile kernels, 1: one-pass BF16x3, 2: symmetric one-pass BF16x3
   long fock_wg = 0, pass_wg = 0;  // workgroup targets of the two shapes (0 = default)
   int density_ks = 16;            // K per LDS stage of the density kernels (16 or 32)
   int density_pf = 1;             // stages in flight in the density kernels (1 or 2)
@@ -73,6 +73,9 @@ struct FockPlan {
   void build_host(const double *P, double *F_out, double *E_out, double *rho_out, double *v_out);
   // S = Phi^T diag(w bw) Phi [n x n] from the resident Phi (FP64).
   void overlap(double *S_out);
+  // F = Phi^T diag(w bw v) Phi for a given host potential v [N], with the plan's precision
+  // and contraction kernel (the coupling matrix of src/wavefunction/nwavefunction.cpp:77-107).
+  void contract_potential(const double *v_host, double *F_out);
   // Basis centres (distinct function origins, in order of first appearance) and the
   // centre of every function; the gradient is reported per centre.
   std::vector<double> centres;   // [ncentre x 3]

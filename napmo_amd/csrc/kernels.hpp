@@ -218,6 +218,11 @@ void launch_fock_bf16x3(const FockShape &sh, const unsigned short *Ahi, const un
 // fock_shape(ld, Npad, true) with the pass's own chunking.
 FockShape fock_pass_shape(int ld, long Npad, long wg_target = 0);
 bool fock_pass_supported(int ld);
+// F = G^T diag(sign s) G, G = sqrt|s| Phi, on the same shape and FP32 partial layout
+// (phi32: the function-major FP32 copy of Phi).
+bool fock_syrk_supported(int ld);
+void launch_fock_syrk_pass(const FockShape &sh, const float *phi32, const double *s,
+                           float *partial, hipStream_t st);
 void launch_fock_bf16x3_pass(const FockShape &sh, const unsigned short *Ahi,
                              const unsigned short *Alo, const double *s, float *partial,
                              hipStream_t st);

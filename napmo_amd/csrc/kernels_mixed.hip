@@ -318,6 +318,180 @@ void launch_fock_bf16x3_pass(const FockShape &sh, const unsigned short *Ahi,
   NAPMO_HIP(hipGetLastError());
 }
 
+// ------------------------------------- BF16x3 Fock build, symmetric one-pass form
+// F = Phi^T diag(s) Phi = G^T diag(sigma) G with G = diag(sqrt|s|) Phi and sigma = sign(s),
+// so one operand matrix serves both MFMA sides: per stage of 32 points only G's bf16 hi / lo
+// go to LDS (half the staging of the two-operand pass), which leaves room for two LDS
+// buffers: the next stage is converted and stored while the MFMAs of the current one run,
+// with one barrier per stage. sigma enters as a sign flip of the B operand's K entries
+// (exact in bf16). Products per 16 x 16 block: Gl_a^T Gh_b + Gh_a^T Gl_b + Gh_a^T Gh_b (and
+// Gl_a^T Gl_a on the diagonal blocks, where that dropped term would be a square), FP32
+// accumulation per chunk, the same partial layout and reduction as the pass above.
+// Measured on 472,000 points: the MFMA's FP32 accumulation lets sums of one-signed terms
+// drift low as a chunk grows (the P-weighted trace of F moves by -6e-4 Eh at 1,844 points
+// per chunk, -1e-4 at 461), so this form is meant for short chunks (1024+ workgroups),
+// where it is also the more accurate one (max|dF| 1.3e-5 against 8.5e-5).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(512) k_fock_syrk_pass(const float *__restrict__ phi32,
+                                                        const double *__restrict__ s, int ld,
+                                                        int nt, int ntiles, long Npad, long chunk,
+                                                        float *__restrict__ partial) {
+  extern __shared__ unsigned short lds[];  // [buffer][hi, lo][ld][KPAD]
+  const size_t arr = (size_t)ld * KPAD;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tile0 = blockIdx.y * PASS_WAVES * TPW;
+  int tr[TPW], tc[TPW];
+  bool has[TPW];
+#pragma unroll
+  for (int k = 0; k < TPW; ++k) {
+    const int t = tile0 + wave * TPW + k;
+    has[k] = t < ntiles;
+    int i = 0, rem = has[k] ? t : 0;
+    while (rem >= nt - i) rem -= nt - i, ++i;
+    tr[k] = i * TILE, tc[k] = (i + rem) * TILE;
+  }
+  f4 acc[TPW][4][4];
+#pragma unroll
+  for (int k = 0; k < TPW; ++k)
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[k][a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+  const long pbeg = (long)blockIdx.x * chunk;
+  const long pend = min(Npad, pbeg + chunk);
+  const int units = ld * (KB / 8);
+  const int seg = (tid & 3) * 8;  // every unit of this thread covers points seg .. seg + 7
+                                  // (a repeated last unit has seg = 24 = (units - 1) & 3)
+  const int r16 = lane & 15, kq = (lane >> 4) * 8;
+  // staging units of this thread; past the last unit a thread repeats the last one (same
+  // values to the same LDS slots), so the staging code has no branches
+  constexpr int NTHREADS = 64 * PASS_WAVES;
+  int unit[PASS_UNITS];
+#pragma unroll
+  for (int q = 0; q < PASS_UNITS; ++q) unit[q] = min(tid + q * NTHREADS, units - 1);
+  f4 ph[PASS_UNITS][2];
+  double sv = 0.0;  // s of stage point (lane & 31)
+  auto load = [&](long k0) {
+    sv = s[k0 + (lane & 31)];
+#pragma unroll
+    for (int q = 0; q < PASS_UNITS; ++q) {
+      const float *src = phi32 + (size_t)(unit[q] >> 2) * Npad + k0 + ((unit[q] & 3) << 3);
+      ph[q][0] = *(const f4 *)src;
+      ph[q][1] = *(const f4 *)(src + 4);
+    }
+  };
+  // G = sqrt|s| Phi split into bf16 hi / lo for the loaded stage; returns the stage's sign
+  // mask (bit k: s of point k is negative)
+  auto convert = [&](int buf) -> unsigned {
+    const float as = sqrtf((float)fabs(sv));
+    const unsigned mask = (unsigned)__ballot(sv < 0.0);
+    float r[8], rl[8];  // scales of this thread's segment and of the last unit's (24 .. 31)
+#pragma unroll
+    for (int e = 0; e < 8; ++e) r[e] = __shfl(as, seg + e), rl[e] = __shfl(as, 24 + e);
+    unsigned short *gh = lds + (size_t)(2 * buf) * arr, *gl = gh + arr;
+#pragma unroll
+    for (int q = 0; q < PASS_UNITS; ++q) {
+      const int u = unit[q];
+      const int useg = (u & 3) << 3;
+      u16x8 hb, lb;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const float x = ph[q][e >> 2][e & 3] * (useg == seg ? r[e] : rl[e]);
+        const unsigned short h = bf16_bits(x);
+        hb[e] = h;
+        lb[e] = bf16_bits(x - bf16_float(h));
+      }
+      const int off = (u >> 2) * KPAD + useg;
+      *(u16x8 *)&gh[off] = hb;
+      *(u16x8 *)&gl[off] = lb;
+    }
+    return mask;
+  };
+  auto mfma_stage = [&](int buf, unsigned mask) {
+    // sign flips of this lane's B entries (points kq .. kq + 7), two bf16 per dword
+    const unsigned bits = (mask >> kq) & 0xFFu;
+    u32x4 flip;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      flip[j] = (((bits >> (2 * j)) & 1u) << 15) | (((bits >> (2 * j + 1)) & 1u) << 31);
+    const unsigned short *gh = lds + (size_t)(2 * buf) * arr, *gl = gh + arr;
+#pragma unroll
+    for (int k = 0; k < TPW; ++k) {
+      if (has[k]) {
+        bf16x8 ah[4], al[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int ra = (tr[k] + a * 16 + r16) * KPAD + kq;
+          ah[a] = __builtin_bit_cast(bf16x8, *(const u16x8 *)&gh[ra]);
+          al[a] = __builtin_bit_cast(bf16x8, *(const u16x8 *)&gl[ra]);
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int rb = (tc[k] + b * 16 + r16) * KPAD + kq;
+          const bf16x8 bh = __builtin_bit_cast(bf16x8, *(const u32x4 *)&gh[rb] ^ flip);
+          const bf16x8 bl = __builtin_bit_cast(bf16x8, *(const u32x4 *)&gl[rb] ^ flip);
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            acc[k][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh, acc[k][a][b], 0, 0, 0);
+            acc[k][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl, acc[k][a][b], 0, 0, 0);
+            if (a == b && tr[k] == tc[k])  // the square term of the diagonal blocks
+              acc[k][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bl, acc[k][a][b], 0, 0, 0);
+            acc[k][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh, acc[k][a][b], 0, 0, 0);
+          }
+        }
+      }
+    }
+  };
+  unsigned mask = 0;
+  if (pbeg < pend) {
+    load(pbeg);
+    mask = convert(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (long k0 = pbeg; k0 < pend; k0 += KB) {
+    const bool next = k0 + KB < pend;
+    if (next) load(k0 + KB);
+    mfma_stage(buf, mask);
+    unsigned mask_next = 0;
+    if (next) mask_next = convert(buf ^ 1);
+    __syncthreads();
+    mask = mask_next;
+    buf ^= 1;
+  }
+  const int col16 = lane & 15, rg = lane >> 4;
+#pragma unroll
+  for (int k = 0; k < TPW; ++k) {
+    if (!has[k]) continue;
+    float *out = partial + (long)blockIdx.x * ((long)ntiles * TILE * TILE + 32) +
+                 (long)(tile0 + wave * TPW + k) * TILE * TILE;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          out[(a * 16 + rg * 4 + r) * TILE + b * 16 + col16] = acc[k][a][b][r];
+  }
+}
+
+bool fock_syrk_supported(int ld) {
+  const size_t lds = (size_t)8 * ld * KPAD * sizeof(unsigned short) / 2;
+  return lds <= max_lds_per_block() && ld * (KB / 8) <= PASS_UNITS * 64 * PASS_WAVES;
+}
+
+void launch_fock_syrk_pass(const FockShape &sh, const float *phi32, const double *s,
+                           float *partial, hipStream_t st) {
+  NAPMO_REQUIRE(fock_syrk_supported(sh.ld), "fock syrk pass: basis too large for one workgroup");
+  NAPMO_REQUIRE(sh.chunk % KB == 0 && sh.Npad % KB == 0, "fock syrk pass: chunking mismatch");
+  const size_t lds = (size_t)4 * sh.ld * KPAD * sizeof(unsigned short);
+  const int groups = (int)div_up(sh.ntiles, PASS_WAVES * TPW);
+  k_fock_syrk_pass<<<dim3(sh.nchunk, groups), dim3(64 * PASS_WAVES), lds, st>>>(
+      phi32, s, sh.ld, sh.nt, sh.ntiles, sh.Npad, sh.chunk, partial);
+  NAPMO_HIP(hipGetLastError());
+}
+
 // P[i][j] = eta sum_k C[i][k] C[j][k] (C [n x nocc] row-major), P [n x n] row-major.
 __global__ void k_gram(const double *__restrict__ C, int n, int nocc, double eta,
                        double *__restrict__ P) {

@@ -77,6 +77,12 @@ class GridFock(object):
         """Asynchronous build on device pointers (ints), on the plan's stream."""
         cext.check(cext.napmo_grid_fock_build_device(self._this, dP, dF, dE))
 
+    def contract(self, v):
+        """Phi^T diag(w bw v) Phi for a potential v [N] on the grid (any sign)."""
+        F = np.zeros((self.nbasis, self.nbasis))
+        cext.check(cext.napmo_fock_plan_contract(self._this, np.ascontiguousarray(v, dtype=np.float64), F))
+        return F
+
     def overlap(self):
         """S = Phi^T diag(w bw) Phi from the resident Phi."""
         S = np.zeros((self.nbasis, self.nbasis))

@@ -183,6 +183,7 @@ _SIGNATURES = {
     'napmo_selftest_mfma': (c_int, [c_int, a1df]),
     'napmo_device_copy': (c_int, [vp, vp, c_long]),
     'napmo_grid_overlap': (c_int, [vp, a2df]),
+    'napmo_fock_plan_contract': (c_int, [vp, a1df, a2df]),
     'napmo_fock_plan_profile': (c_int, [vp, c_int]),
     'napmo_fock_plan_profile_read': (c_int, [vp, a1df, c_int, ctypes.POINTER(c_int)]),
     'napmo_fock_phase_name': (ctypes.c_char_p, [c_int]),

@@ -107,7 +107,9 @@ def test_autotuner_choice_and_cache(tmp_path):
         assert src == 'timed'
         F1, E1, Eg1 = f.build(Ps[0])
         assert abs(Eg1 - Eg0) < 1e-12 * abs(Eg0)           # rho and v do not depend on it
-        tol = 1e-12 if prec == 'fp64' else 1e-6
+        # FP64 kernels differ by summation order only; two BF16x3 kernels (different splits)
+        # each stay within 2e-5 max|F| of FP64, so within twice that of each other
+        tol = 1e-12 if prec == 'fp64' else 4e-5
         assert np.abs(F1 - F0).max() < tol * np.abs(F0).max()
         setting2, src2 = autotune.tune(f, dP[0].ptr, dF[0].ptr, dE[0].ptr, path=path)
         assert src2 == 'cache' and setting2 == setting

@@ -377,3 +377,34 @@ def test_density_from_dm_matches_oracle():
         assert 0 < written < g.size
         assert np.array_equal(out == -1.0, ref == -1.0)
         assert np.max(np.abs(out - ref)) < 1e-13 * np.abs(ref).max()
+
+
+def test_contract_signed_potential_every_kernel():
+    # F = Phi^T diag(w bw v) Phi for a potential with both signs (as the multi-species
+    # v_eff = V_e - V_p has): the FP64 kernel against the oracle, and every BF16x3 kernel
+    # (per-tile, one-pass, and the one-pass form on G = sqrt|s| Phi with the sign applied
+    # per point) within 2e-5 of sum_p |w bw v| |phi_mu phi_nu|
+    import napmo_amd
+    from napmo_amd import autotune
+    atoms = _mol(37, natom=5, box=5.0)
+    shells = systems.random_shells(np.random.default_rng(37), 5, 6)
+    g, b = systems.gpu_system(atoms, shells, 30, 110)
+    o, ob = systems.oracle_system(atoms, shells, 30, 110)
+    phi = ocx.basis_compute(ob, o.points)
+    p = o.points
+    v = np.cos(1.3 * p[:, 0]) * np.exp(-0.05 * (p ** 2).sum(1)) - 0.2 * np.sin(p[:, 1])
+    assert (v < 0).any() and (v > 0).any()
+    ref = core.coupling_matrix(phi, v, o.becke_weights, o.weights)
+    scale = core.coupling_matrix(np.abs(phi), np.abs(v), o.becke_weights, o.weights)
+    f64 = napmo_amd.GridFock(g, b, 6, 'fp64')
+    F = f64.contract(v)
+    print('contract fp64 max|dF| %.3e max|F| %.3e' % (np.abs(F - ref).max(), np.abs(ref).max()))
+    assert np.abs(F - ref).max() < 1e-13 * np.abs(scale).max()
+    mix = napmo_amd.GridFock(g, b, 6, 'bf16x3')
+    for use_pass in (0, 1, 2):
+        autotune.apply(mix, dict(autotune.DEFAULTS, use_pass=use_pass))
+        Fm = mix.contract(v)
+        err = np.abs(Fm - ref).max()
+        print('contract bf16x3 use_pass=%d max|dF| %.3e scale %.3e' % (use_pass, err, np.abs(scale).max()))
+        assert err < 2e-5 * np.abs(scale).max()
+        assert np.array_equal(Fm, Fm.T)
