@@ -206,20 +206,22 @@ __global__ void __launch_bounds__(512) k_fock_bf16x3_pass(const unsigned short *
   const int units = ld * (KB / 8);
   const int r16 = lane & 15, kq = (lane >> 4) * 8;
   __shared__ float sS[KB];
+  // staging units of this thread; past the last unit a thread repeats the last one (same
+  // values to the same LDS slots), so the staging code has no branches
+  constexpr int NTHREADS = 64 * PASS_WAVES;
+  int unit[PASS_UNITS];
+#pragma unroll
+  for (int q = 0; q < PASS_UNITS; ++q) unit[q] = min(tid + q * NTHREADS, units - 1);
   u16x8 ph[PASS_UNITS], pl[PASS_UNITS];
   double pref_s = 0.0;
   auto prefetch = [&](long k0) {
+    if (tid < KB) pref_s = s[k0 + tid];
 #pragma unroll
     for (int q = 0; q < PASS_UNITS; ++q) {
-      const int u = tid + q * (int)blockDim.x;
-      if (u < units) {
-        const int f = u / (KB / 8), seg = (u % (KB / 8)) * 8;
-        const size_t g = (size_t)f * Npad + k0 + seg;
-        ph[q] = *(const u16x8 *)&Ahi[g];
-        pl[q] = *(const u16x8 *)&Alo[g];
-      }
+      const size_t g = (size_t)(unit[q] >> 2) * Npad + k0 + ((unit[q] & 3) << 3);
+      ph[q] = *(const u16x8 *)&Ahi[g];
+      pl[q] = *(const u16x8 *)&Alo[g];
     }
-    if (tid < KB) pref_s = s[k0 + tid];
   };
   if (pbeg < pend) prefetch(pbeg);
   for (long k0 = pbeg; k0 < pend; k0 += KB) {
@@ -227,22 +229,19 @@ __global__ void __launch_bounds__(512) k_fock_bf16x3_pass(const unsigned short *
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < PASS_UNITS; ++q) {
-      const int u = tid + q * (int)blockDim.x;
-      if (u < units) {
-        const int f = u / (KB / 8), seg = (u % (KB / 8)) * 8;
-        *(u16x8 *)&sAh[f * KPAD + seg] = ph[q];
-        *(u16x8 *)&sAl[f * KPAD + seg] = pl[q];
-        u16x8 bh, bl;
+      const int f = unit[q] >> 2, seg = (unit[q] & 3) << 3;
+      *(u16x8 *)&sAh[f * KPAD + seg] = ph[q];
+      *(u16x8 *)&sAl[f * KPAD + seg] = pl[q];
+      u16x8 bh, bl;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const float x = (bf16_float(ph[q][e]) + bf16_float(pl[q][e])) * sS[seg + e];
-          const unsigned short hb = bf16_bits(x);
-          bh[e] = hb;
-          bl[e] = bf16_bits(x - bf16_float(hb));
-        }
-        *(u16x8 *)&sBh[f * KPAD + seg] = bh;
-        *(u16x8 *)&sBl[f * KPAD + seg] = bl;
+      for (int e = 0; e < 8; ++e) {
+        const float x = (bf16_float(ph[q][e]) + bf16_float(pl[q][e])) * sS[seg + e];
+        const unsigned short hb = bf16_bits(x);
+        bh[e] = hb;
+        bl[e] = bf16_bits(x - bf16_float(hb));
       }
+      *(u16x8 *)&sBh[f * KPAD + seg] = bh;
+      *(u16x8 *)&sBl[f * KPAD + seg] = bl;
     }
     __syncthreads();
     if (k0 + KB < pend) prefetch(k0 + KB);
