@@ -245,7 +245,9 @@ def dpsi_leg(args, log):
     if name == 'synth':
         flops = 2.0 * M * N * L * norb
         roof = {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': PEAK['fp64_mfma'] / 1e12,
-                'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / PEAK['fp64_mfma'], 'traffic': None}
+                'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / PEAK['fp64_mfma'],
+                # HBM bytes of one synthesis launch (one orbital)
+                'traffic': table_traffic(traffic_table('traffic_dpsi_chon16.json'), 'k_synth2n<')}
     else:
         nbytes = 8.0 * N * norb * 4
         roof = {'bound': 'hbm', 'achieved': nbytes / (ms * 1e-3) / 1e9, 'peak': PEAK['hbm'] / 1e9,
@@ -379,18 +381,35 @@ def batch_leg(args, log):
 
 
 PHASE_KERNELS = {'density': 'k_density_mfma', 'fock': ('k_fock_bf16x3_pass', 'k_fock_f64'),
-                 'synth': 'k_synth2<', 'project': 'k_project', 'radial': 'k_radial',
+                 'synth': 'k_synth2n<', 'project': 'k_project', 'radial': 'k_radial',
                  'fock_reduce': 'k_fock_reduce'}
+
+
+def traffic_table(name):
+    """Per-kernel HBM bytes of a committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary."""
+    path = os.path.join(ROOT, 'profiles', name)
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f).get('kernels', {})
+
+
+def table_traffic(kern, want):
+    """Read + write bytes per launch of the first kernel whose name contains want."""
+    for k in kern:
+        if want in k:
+            r, w = kern[k].get('read_bytes'), kern[k].get('write_bytes')
+            if r is not None and w is not None:
+                return r + w
+    return None
 
 
 def measured_traffic(res, phase):
     """HBM bytes per launch (read + write) of the phase's kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_chon16.json), or None."""
-    path = os.path.join(ROOT, 'profiles', 'traffic_chon16.json')
-    if not os.path.exists(path):
+    kern = traffic_table('traffic_chon16.json')
+    if not kern:
         return None
-    with open(path) as f:
-        kern = json.load(f).get('kernels', {})
     want = PHASE_KERNELS.get(phase)
     if want is None:
         return None
