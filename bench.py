@@ -293,6 +293,101 @@ def dpsi_cpu_baseline(atoms, nrad, nang, lmax, norb, log):
             'seconds_per_step': total}
 
 
+def operators_leg(args, log):
+    """Grid operators of the numeric-orbital SCF on the 16-atom 472,000-point grid (lmax 12),
+    inputs resident in HBM: one exchange-potential build of 10 orbitals (55 Poisson solves
+    of orbital products, napmo/wavefunction/psi_numeric.py:186-215) is the step; the
+    kinetic operator (nkinetic.py) on the same orbitals is timed beside it."""
+    import napmo_amd
+    from napmo_amd import workloads
+    from napmo_amd.device import DeviceArray
+    from napmo_amd.system import cext
+    cext.require_gpu()
+    atoms, _, nrad, nang, lmax = workloads.chon16_system()
+    ndim = 10
+    grid = napmo_amd.BeckeGrid(napmo_amd.make_species(atoms), nrad, nang)
+    N = grid.size
+    psi = dpsi_orbitals(grid.points, grid.origin, ndim)
+    D = np.eye(ndim)
+    dpsi, dD, dK, dT = (DeviceArray.from_host(np.ascontiguousarray(psi)), DeviceArray.from_host(D),
+                        DeviceArray((ndim, N)), DeviceArray((N,)))
+
+    def exchange():
+        cext.check(cext.napmo_compute_exchange_potential_device(grid._this, ndim, dpsi.ptr, dD.ptr,
+                                                                lmax, dK.ptr))
+
+    def kinetic():
+        for k in range(ndim):
+            cext.check(cext.napmo_compute_kinetic_device(grid._this, dpsi.ptr + 8 * k * N, lmax, dT.ptr))
+
+    for _ in range(max(1, args.warmup)):
+        exchange()
+        kinetic()
+    steps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        exchange()
+    dt = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        kinetic()
+    dt_kin = (time.perf_counter() - t0) / steps / ndim
+    cext.check(cext.napmo_grid_profile(grid._this, 1))
+    for _ in range(min(steps, 3)):
+        exchange()
+    ph = np.zeros(len(DPSI_PHASES))
+    nc = ctypes.c_int(0)
+    cext.check(cext.napmo_grid_profile_read(grid._this, ph.ctypes.data, len(DPSI_PHASES), ctypes.byref(nc)))
+    cext.check(cext.napmo_grid_profile(grid._this, 0))
+    phases = {'project_radial': float(ph[1]), 'synth_tables': float(ph[4]), 'synth': float(ph[5]),
+              'reduce_acc': float(ph[6])}
+    npair = ndim * (ndim + 1) // 2
+    log('exchange build (%d orbitals, %d solves): %.3f ms (%.2f ms per solve); phases %s; '
+        'kinetic %.3f ms per orbital' % (ndim, npair, dt * 1e3, dt * 1e3 / npair,
+                                         {k: round(v, 3) for k, v in phases.items()}, dt_kin * 1e3))
+    M, L = len(atoms), (lmax + 1) ** 2
+    flops = 2.0 * M * N * L * npair
+    ms = phases['synth']
+    roof = {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': PEAK['fp64_mfma'] / 1e12,
+            'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / PEAK['fp64_mfma'],
+            'traffic': table_traffic(traffic_table('traffic_dpsi_chon16.json'), 'k_synth2n<'),
+            'kernel': 'synth'}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = exchange_cpu_baseline(atoms, nrad, nang, lmax, ndim, log)
+    return {
+        'metric': 'grid exchange-potential builds/s on 1 MI355X', 'value': 1.0 / dt, 'unit': 'builds/s',
+        'n_gpus': 1, 'steps': steps, 'warmup': args.warmup, 'ms_per_step': dt * 1e3,
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+        'data': 'synthetic (seeded orbitals)',
+        'config': {'workload': 'exchange potential of 10 orbitals (55 Poisson solves), 16 atoms, 50x590 '
+                               'Becke grid (472,000 points), lmax 12', 'ndim': ndim, 'npoint': N, 'lmax': lmax},
+        'roofline': roof, 'cpu_baseline': cpu, 'phases_ms': phases, 'kinetic_ms_per_orbital': dt_kin * 1e3,
+    }
+
+
+def exchange_cpu_baseline(atoms, nrad, nang, lmax, ndim, log):
+    """The oracle's compute_coulomb of one orbital product on the full grid (numpy
+    projection, SciPy sparse solves, C/OpenMP synthesis), scaled to the ndim(ndim+1)/2
+    solves of an exchange build."""
+    from oracle import cext as ocx
+    from oracle import core
+    grid = core.BeckeGrid(atoms, nrad, nang)
+    psi = dpsi_orbitals(grid.points, grid.origin, 2)
+    t = time.perf_counter()
+    U = core.poisson_solver(grid, psi[0] * psi[1], lmax)
+    out = np.zeros(grid.size)
+    for k in range(grid.ncenter):
+        ocx.eval_decomposition(U[k], grid.origin[k], grid.points, out)
+    one = time.perf_counter() - t
+    total = one * ndim * (ndim + 1) // 2
+    log('exchange cpu baseline (%d threads): %.2fs per solve, %.1fs per build' % (ocx.max_threads(), one, total))
+    return {'value': 1.0 / total, 'unit': 'builds/s', 'cores': ocx.max_threads(), 'kind': 'port',
+            'sample': 'oracle Poisson solve of 1 orbital product on the full 472,000-point grid (numpy '
+                      'projection, SciPy sparse solves, C/OpenMP synthesis), scaled to %d solves'
+                      % (ndim * (ndim + 1) // 2), 'seconds_per_step': total}
+
+
 def batch_leg(args, log):
     """64 independent molecules (2-12 atoms, uneven 35-110 x 110-590 grids, 20-200
     functions): one step = one grid Fock build of every molecule, launched as one batch
@@ -540,10 +635,12 @@ def main():
     ap.add_argument('--copy-calibration', type=float, default=0.0, metavar='GIB',
                     help='only run a plain device copy of GIB gibibytes 5 times (counter calibration)')
     ap.add_argument('--cpu-seconds', type=float, default=30.0)
-    ap.add_argument('--workload', default='chon16', choices=('chon16', 'apmo', 'batch64', 'dpsi'),
+    ap.add_argument('--workload', default='chon16',
+                    choices=('chon16', 'apmo', 'batch64', 'dpsi', 'operators'),
                     help='chon16: 16-atom 472,000-point build (default); apmo: electrons + '
                          'quantum protons on separate grids; batch64: 64 small molecules; dpsi: '
-                         'Becke-Dickson update of 40 orbitals on the 16-atom grid')
+                         'Becke-Dickson update of 40 orbitals on the 16-atom grid; operators: '
+                         'exchange potential of 10 orbitals and the kinetic operator')
     args = ap.parse_args()
 
     def log(msg):
@@ -570,6 +667,9 @@ def main():
         return
     if args.workload == 'dpsi':
         print(json.dumps(dpsi_leg(args, log)), flush=True)
+        return
+    if args.workload == 'operators':
+        print(json.dumps(operators_leg(args, log)), flush=True)
         return
     results, N, n, lmax, M = gpu_leg(args, log)
     L = (lmax + 1) ** 2

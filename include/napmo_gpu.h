@@ -223,6 +223,11 @@ int napmo_kinetic_expand(void *grid, const double *psi, int lmax, double *U_y, d
  * [ndim x N], the Kpot handed to nwavefunction_compute_2body_matrix. */
 int napmo_compute_exchange_potential(void *grid, int ndim, const double *psi, const double *D,
                                      int lmax, double *K_out);
+/* napmo_compute_exchange_potential and napmo_compute_kinetic on device pointers (resident
+ * on the grid's device); they return when the result is complete. */
+int napmo_compute_exchange_potential_device(void *grid, int ndim, const double *psi,
+                                            const double *D, int lmax, double *K_out);
+int napmo_compute_kinetic_device(void *grid, const double *psi, int lmax, double *out);
 /* Point-charge potential on the grid (napmo/wavefunction/nnuclear.py:15-37): V_out [N] =
  * sum_A charges[A] / |r - origins[A]|, origins [ncharge x 3]. */
 int napmo_compute_nuclear(void *grid, int ncharge, const double *charges, const double *origins,
@@ -298,6 +303,11 @@ int napmo_device_copy(double *dst, const double *src, long n);
 /* Grid overlap S = Phi^T diag(w bw) Phi [n x n] from the plan's resident Phi (FP64).
  * Same contraction as nwavefunction_compute_coupling with other_J = 1. */
 int napmo_grid_overlap(void *plan, double *S_out);
+/* F_out [n x n] = Phi^T diag(w bw v) Phi for a host potential v [N] of any sign, with the
+ * plan's precision and contraction kernel: the coupling matrix of
+ * nwavefunction_compute_coupling (src/wavefunction/nwavefunction.cpp:77-107) on the
+ * plan's resident Phi. */
+int napmo_fock_plan_contract(void *plan, const double *v, double *F_out);
 /* Per-phase HIP-event timing of the next max_builds builds on the plan's stream; read
  * back the average milliseconds of each phase (napmo_fock_phase_name(k) names them). */
 int napmo_fock_plan_profile(void *plan, int max_builds);
@@ -327,7 +337,8 @@ int napmo_grid_fock_gradient_device(void *plan, const double *dP, double *dG);
 enum {
   NAPMO_TUNE_FOCK_WG = 0,  /* workgroups of the per-tile Fock kernels (chunks x tiles); 0 = default */
   NAPMO_TUNE_PASS_WG = 1,  /* workgroups of the one-pass BF16x3 Fock kernel; 0 = default */
-  NAPMO_TUNE_USE_PASS = 2, /* 1: one-pass BF16x3 kernel, 0: per-tile BF16x3 kernel */
+  NAPMO_TUNE_USE_PASS = 2, /* 0: per-tile BF16x3 kernel, 1: one-pass BF16x3 kernel,
+                              2: one-pass kernel on G = sqrt|s| Phi (F = G^T diag(sign s) G) */
   NAPMO_TUNE_DENSITY_KS = 3, /* K per LDS stage of the density kernels: 16 or 32; 0 = 16 */
   NAPMO_TUNE_DENSITY_PF = 4, /* stages in flight in the density kernels: 1 or 2; 0 = 1 */
   NAPMO_TUNE_SYNTH_NP = 5    /* (point, centre) pairs per lane of the synthesis, 1 or 2; 0 = 2

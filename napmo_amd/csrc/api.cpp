@@ -559,20 +559,30 @@ int napmo_compute_exchange_potential(void *grid, int ndim, const double *psi, co
     NAPMO_REQUIRE(ndim >= 0, "napmo_compute_exchange_potential: negative ndim");
     const long N = g.N;
     hipStream_t st = g.stream;
-    DeviceBuffer<double> dpsi, dD, chi((size_t)ndim * N), K((size_t)ndim * N), dens(N), aux(N);
+    DeviceBuffer<double> dpsi, dD, K((size_t)ndim * N);
     dpsi.upload(psi, (size_t)ndim * N, st);
     dD.upload(D, (size_t)ndim * ndim, st);
-    launch_exchange_chi(dpsi.get(), dD.get(), ndim, N, chi.get(), st);
-    launch_axpby(0.0, chi.get(), 0.0, chi.get(), (long)ndim * N, K.get(), st);  // zero K
-    for (int s = 0; s < ndim; ++s)
-      for (int v = 0; v <= s; ++v) {
-        launch_mul(dpsi.get() + (long)s * N, dpsi.get() + (long)v * N, N, dens.get(), st);
-        g.poisson_expand(dens.get(), lmax);
-        g.poisson_synthesize(lmax, aux.get());
-        launch_exchange_acc(aux.get(), chi.get(), s, v, N, K.get(), st);
-      }
+    g.exchange_potential(ndim, dpsi.get(), dD.get(), lmax, K.get());
     K.download(K_out, (size_t)ndim * N, st);
     NAPMO_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int napmo_compute_exchange_potential_device(void *grid, int ndim, const double *psi,
+                                            const double *D, int lmax, double *K_out) {
+  return guarded([&] {
+    DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    NAPMO_REQUIRE(ndim >= 0, "napmo_compute_exchange_potential_device: negative ndim");
+    g.exchange_potential(ndim, psi, D, lmax, K_out);
+    NAPMO_HIP(hipStreamSynchronize(g.stream));
+  });
+}
+
+int napmo_compute_kinetic_device(void *grid, const double *psi, int lmax, double *out) {
+  return guarded([&] {
+    DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    g.kinetic(psi, lmax, out);
+    NAPMO_HIP(hipStreamSynchronize(g.stream));
   });
 }
 

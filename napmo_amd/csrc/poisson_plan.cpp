@@ -691,6 +691,43 @@ void DeviceGrid::orbital_correction(int lmax, int norb, const double *d_phi, con
   NAPMO_REQUIRE(h_status == 0, "orbital correction: singular radial system (zero pivot)");
 }
 
+void DeviceGrid::exchange_potential(int ndim, const double *d_psi, const double *d_D, int lmax,
+                                    double *d_K) {
+  if (ndim <= 0) return;
+  PoissonPlan &pl = plan(lmax);
+  AtomTables at = tables(pl);
+  SynthPairs &sp = synth_pairs();
+  PoissonWork &w = pl.work;
+  xchi.resize((size_t)ndim * N);
+  xdens.resize(N);
+  xaux.resize(N);
+  if (prof_on) ++prof_calls;
+  launch_exchange_chi(d_psi, d_D, ndim, N, xchi.get(), stream);
+  launch_axpby(0.0, xchi.get(), 0.0, xchi.get(), (long)ndim * N, d_K, stream);  // K = 0
+  for (int s = 0; s < ndim; ++s)
+    for (int v = 0; v <= s; ++v) {
+      // V[psi_s psi_v]: projection and radial Green solves, tables, synthesis, centre sum
+      tic(1);
+      launch_mul(d_psi + (long)s * N, d_psi + (long)v * N, N, xdens.get(), stream);
+      launch_project(at, xdens.get(), bw.get(), pl.yhat.get(), lmax, w.rho_lm.get(), stream);
+      launch_radial(at, w.rho_lm.get(), pl.green.get(), lmax, w.Uy.get(), w.Udt.get(), stream);
+      toc();
+      tic(4);
+      launch_synth_tables(at, w.Uy.get(), w.Udt.get(), lmax, w.T.get(), w.ampL.get(), w.ampR.get(),
+                          stream);
+      toc();
+      tic(5);
+      launch_synth2(at, x.get(), y.get(), z.get(), N, sp.list.get(), sp.wave_bucket.get(),
+                    sp.nwaves, w.T.get(), w.ampL.get(), w.ampR.get(), lmax, sp.contrib.get(), stream,
+                    synth_np);
+      toc();
+      tic(6);
+      launch_contrib_reduce(sp.contrib.get(), M, N, 0, xaux.get(), stream);
+      launch_exchange_acc(xaux.get(), xchi.get(), s, v, N, d_K, stream);
+      toc();
+    }
+}
+
 void DeviceGrid::becke_dickson_step(int lmax, int norb, const double *d_psi, const double *d_T,
                                     const double *d_V, const double *d_e, double mass_inv,
                                     double *d_dpsi, double *d_de, double *d_R) {

@@ -158,6 +158,11 @@ struct DeviceGrid {
   void orbital_correction(int lmax, int norb, const double *d_phi, const double *d_doi,
                           const double *d_oi, const double *d_ri, const double *d_V,
                           double mass_inv, double *d_out);
+  // Grid exchange potential (napmo/wavefunction/psi_numeric.py:186-215), all device
+  // arrays: K [ndim x N] = sum_k sum_l psi_k V[psi_l psi_j] D[k][l] by ndim(ndim+1)/2
+  // Poisson solves of orbital products.
+  void exchange_potential(int ndim, const double *d_psi, const double *d_D, int lmax, double *d_K);
+  DeviceBuffer<double> xchi, xdens, xaux;  // its workspace, kept between calls
   // Residual R = T + V - e psi and de = integrate(R psi) of norb orbitals, then their
   // corrections (napmo/wavefunction/psi_optimization.py:175-203).
   void becke_dickson_step(int lmax, int norb, const double *d_psi, const double *d_T,

@@ -148,6 +148,8 @@ _SIGNATURES = {
     'napmo_kinetic_expand': (c_int, [vp, a1df, c_int, adf, adf, vp]),
     'napmo_compute_nuclear': (c_int, [vp, c_int, a1df, a2df, a1df]),
     'napmo_compute_exchange_potential': (c_int, [vp, c_int, a2df, a2df, c_int, a2df]),
+    'napmo_compute_exchange_potential_device': (c_int, [vp, c_int, vp, vp, c_int, vp]),
+    'napmo_compute_kinetic_device': (c_int, [vp, vp, c_int, vp]),
     'napmo_compute_dpsi': (c_int, [vp, c_int, c_int, vp, vp, vp, vp, vp, c_double, vp]),
     'napmo_becke_dickson_step': (c_int, [vp, c_int, c_int, vp, vp, vp, vp, c_double, vp, vp, vp]),
     'napmo_becke_dickson_step_device': (c_int, [vp, c_int, c_int, vp, vp, vp, vp, c_double, vp,

@@ -92,3 +92,24 @@ def test_exchange_potential_matches_oracle():
     ref = core.compute_exchange_potential(o, psi, D, 6)
     print('exchange max|dK| %.3e max|K| %.3e' % (np.abs(K - ref).max(), np.abs(ref).max()))
     assert np.max(np.abs(K - ref)) < 1e-11 * np.abs(ref).max()
+
+
+def test_device_entry_points_match_host_ones():
+    # napmo_compute_{exchange_potential,kinetic}_device on resident arrays give the host
+    # entry points' results bit for bit
+    import napmo_amd
+    from napmo_amd.device import DeviceArray
+    from napmo_amd.system import cext
+    atoms = [('O', (0.0, 0.0, 0.0)), ('H', (1.1, 0.9, 0.0))]
+    g, _ = systems.gpu_system(atoms, [[]] * 2, 24, 50)
+    p = g.points
+    psi = np.array([np.exp(-0.7 * ((p - np.array(c)) ** 2).sum(1)) * (1 + 0.2 * k * p[:, 0])
+                    for k, (_, c) in enumerate(atoms * 2)][:3])
+    D = np.array([[1.0, 0.2, 0.0], [0.2, 0.5, 0.1], [0.0, 0.1, 0.8]])
+    K = napmo_amd.compute_exchange_potential(g, psi, D, 4)
+    T = napmo_amd.compute_kinetic(g, psi[1], 4)
+    dpsi, dD, dK, dT = DeviceArray.from_host(psi), DeviceArray.from_host(D), DeviceArray(psi.shape), DeviceArray((g.size,))
+    cext.check(cext.napmo_compute_exchange_potential_device(g._this, 3, dpsi.ptr, dD.ptr, 4, dK.ptr))
+    cext.check(cext.napmo_compute_kinetic_device(g._this, dpsi.ptr + 8 * g.size, 4, dT.ptr))
+    assert np.array_equal(dK.to_host(), K)
+    assert np.array_equal(dT.to_host(), T)
