@@ -82,13 +82,14 @@ __global__ void __launch_bounds__(64) k_helmholtz_radial(
   const int A = blockIdx.x, l = blockIdx.y, o = blockIdx.z;
   const int lane = threadIdx.x;
   const int nr = at.nrad[A], nfn = 2 * nr, nm = 2 * l + 1;
-  double *W = sm;                   // [nfn][WSLOT]
-  double *Lf = W + nfn * WSLOT;     // [nfn][3]
-  double *ay = Lf + nfn * 3;        // [nr]
-  double *ad = ay + nr;             // [nr]
-  double *an = ad + nr;             // [nfn]
-  double *X = an + nfn;             // [nfn][mc], also the slope solver's scratch
-  int *ipiv = (int *)(X + (long)nfn * mc);
+  const int nfn_max = 2 * at.max_nrad;
+  double *W = sm;                          // [nfn][WSLOT]
+  double *Lf = W + nfn_max * WSLOT;        // [nfn][3]
+  double *sinv = Lf + nfn_max * 3;         // [nr] 1 / pivots of the spline system
+  double *supm = sinv + at.max_nrad;       // [nr] its eliminated super-diagonal
+  int *ipiv = (int *)(supm + at.max_nrad); // [nfn]
+  double *X = (double *)(ipiv + nfn_max);  // [nfn][mc] columns; ay, ad, an, scratch first
+  double *ay = X, *ad = X + nr, *an = X + nfn, *scr = X + 2 * nfn;
   const long r0 = at.rad_off[A];
   const double c = -2.0 / mass_inv;
   const double e_o = oi[o], d_o = doi[o];
@@ -102,7 +103,7 @@ __global__ void __launch_bounds__(64) k_helmholtz_radial(
     ay[i] = c * ((l * (l + 1.0) * r2inv) + (Vsph[(long)o * nrad_total + r0 + i] - e_o - d_o));
   }
   __syncthreads();
-  if (lane == 0) spline_slopes(ay, ad, nr, X, X + nr);
+  if (lane == 0) spline_slopes(ay, ad, nr, scr, scr + nr);
   __syncthreads();
   for (int i = lane; i < nr; i += 64) {
     const double j1sq = j1[i] * j1[i];
@@ -129,6 +130,20 @@ __global__ void __launch_bounds__(64) k_helmholtz_radial(
       const double *oa = OA + (row * 7 + off + 3) * 6;
       for (int kk = 0; kk < 6 && begin + kk < end; ++kk) v += an[begin + kk] * oa[kk];
       wr[off + 3] = v;
+    }
+  }
+  // Elimination factors of the spline-slope system (2, 4, ..., 4, 2 | 1), shared by every
+  // column: r_i = (rhs_i - r_{i-1}) / den_i forward, dt_i = r_i - upm_i dt_{i+1} back
+  // (the order of src/horton/cubic_spline.cpp:54-79 with unit off-diagonals)
+  if (lane == 0) {
+    double upm = 0.5;
+    sinv[0] = 0.5;
+    supm[0] = upm;
+    for (int i = 1; i < nr; ++i) {
+      const double den = (i == nr - 1 ? 2.0 : 4.0) - upm;
+      sinv[i] = 1.0 / den;
+      upm = (i < nr - 1) ? 1.0 / den : 0.0;
+      supm[i] = upm;
     }
   }
   __syncthreads();
@@ -169,40 +184,87 @@ __global__ void __launch_bounds__(64) k_helmholtz_radial(
     __syncthreads();
   }
 
-  // right-hand sides in chunks of mc columns; lane q solves column q
-  const double *R = ht.rmap + ht.rmap_off[A];
+  // right-hand sides in chunks of mc columns; lane q owns column q of X, which holds in
+  // turn the merged source f (value at even rows, slope at odd rows), the Galerkin
+  // right-hand side, and the forward-substituted vector
+  const double *O2 = ht.o2 + ht.o2_off[A];
   const long ubase = ((long)o * nrad_total + r0) * L + l * l;
   for (int m0 = 0; m0 < nm; m0 += mc) {
     const int cnt = min(mc, nm - m0);
-    for (int e = lane; e < nfn * cnt; e += 64) {
-      const int row = e / cnt, q = e - row * cnt;
-      const double *rrow = R + (long)row * nr;
-      double acc = 0.0;
-      for (int j = 0; j < nr; ++j) acc += rrow[j] * mu[ubase + (long)j * L + m0 + q];
-      X[row * mc + q] = c * acc;
-    }
-    __syncthreads();
     if (lane < cnt) {
       const int q = lane;
-      for (int k = 0; k < nfn; ++k) {
-        const int p = ipiv[k];
-        if (p != k) {
-          const double t = X[k * mc + q];
-          X[k * mc + q] = X[p * mc + q];
-          X[p * mc + q] = t;
-        }
-        const double xk = X[k * mc + q];
-        for (int d = 1; d <= 3 && k + d < nfn; ++d) X[(k + d) * mc + q] -= Lf[k * 3 + d - 1] * xk;
-      }
-      for (int k = nfn - 1; k >= 0; --k) {
-        double s = X[k * mc + q];
-        for (int jj = 1; jj <= 6 && k + jj < nfn; ++jj) s -= W[k * WSLOT + jj + 3] * X[(k + jj) * mc + q];
-        X[k * mc + q] = s / W[k * WSLOT + 3];
-      }
-      // solution spline: y = u, dt = (u_t / j1) j1 (the reference's dx round trip)
+      double *x = X + q;  // x[k * mc]: row k of this column
+      // mu and the slopes of its spline on the t-axis
+      for (int i = 0; i < nr; ++i) x[2 * i * mc] = mu[ubase + (long)i * L + m0 + q];
+      double rprev = 0.0;
       for (int i = 0; i < nr; ++i) {
-        Uy[ubase + (long)i * L + m0 + q] = X[(2 * i) * mc + q];
-        Udt[ubase + (long)i * L + m0 + q] = (X[(2 * i + 1) * mc + q] / j1[i]) * j1[i];
+        const double ya = x[2 * (i == 0 ? 0 : i - 1) * mc];
+        const double yb = x[2 * (i == nr - 1 ? nr - 1 : i + 1) * mc];
+        const double rhs = 3.0 * (yb - ya);
+        rprev = (i == 0) ? rhs * sinv[0] : (rhs - rprev) * sinv[i];
+        x[(2 * i + 1) * mc] = rprev;
+      }
+      double dtn = rprev;
+      for (int i = nr - 2; i >= 0; --i) {
+        dtn = x[(2 * i + 1) * mc] - supm[i] * dtn;
+        x[(2 * i + 1) * mc] = dtn;
+      }
+      // f = c (mu, mu_x) on the t-axis, (fy j1^2, (fx j1^2 + 2 fy j2) j1); the right-hand
+      // side of rows 2i, 2i+1 is the sum of f over nodes i-1, i, i+1 times the node
+      // integrals O2 (zero rows for the value conditions), written in place through a
+      // register window of the three nodes
+      auto fnode = [&](int i, double &f0, double &f1) {
+        const double j1i = j1[i], j1sq = j1i * j1i;
+        const double fy = c * x[2 * i * mc], fx = c * (x[(2 * i + 1) * mc] / j1i);
+        f0 = fy * j1sq;
+        f1 = (fx * j1sq + 2 * fy * j2[i]) * j1i;
+      };
+      double fa0 = 0.0, fa1 = 0.0, fb0, fb1, fc0 = 0.0, fc1 = 0.0;
+      fnode(0, fb0, fb1);
+      for (int i = 0; i < nr; ++i) {
+        if (i + 1 < nr)
+          fnode(i + 1, fc0, fc1);
+        else
+          fc0 = 0.0, fc1 = 0.0;
+        for (int r = 0; r < 2; ++r) {
+          const double *w6 = O2 + (2 * i + r) * 6;
+          x[(2 * i + r) * mc] = fa0 * w6[0] + fa1 * w6[1] + fb0 * w6[2] + fb1 * w6[3] +
+                                fc0 * w6[4] + fc1 * w6[5];
+        }
+        fa0 = fb0, fa1 = fb1, fb0 = fc0, fb1 = fc1;
+      }
+      // forward: row interchanges and L, window w = x[k .. k + 3]
+      double w0 = x[0], w1 = x[mc], w2 = x[2 * mc], w3 = (3 < nfn) ? x[3 * mc] : 0.0;
+      for (int k = 0; k < nfn; ++k) {
+        const int d = ipiv[k] - k;
+        if (d == 1) {
+          const double t = w0; w0 = w1; w1 = t;
+        } else if (d == 2) {
+          const double t = w0; w0 = w2; w2 = t;
+        } else if (d == 3) {
+          const double t = w0; w0 = w3; w3 = t;
+        }
+        const double xk = w0;
+        x[k * mc] = xk;
+        w0 = w1 - Lf[k * 3 + 0] * xk;
+        w1 = w2 - Lf[k * 3 + 1] * xk;
+        w2 = w3 - Lf[k * 3 + 2] * xk;
+        w3 = (k + 4 < nfn) ? x[(k + 4) * mc] : 0.0;
+      }
+      // backward: U, window u = x[k + 1 .. k + 6]; the solution's value y = u at even
+      // rows, slope dt = u at odd rows (written as (u / j1) j1, the reference's dx round trip)
+      double u1 = 0.0, u2 = 0.0, u3 = 0.0, u4 = 0.0, u5 = 0.0, u6 = 0.0;
+      for (int k = nfn - 1; k >= 0; --k) {
+        const double *wr = W + k * WSLOT;
+        const double s = x[k * mc] - wr[4] * u1 - wr[5] * u2 - wr[6] * u3 - wr[7] * u4 -
+                         wr[8] * u5 - wr[9] * u6;
+        const double xs = s / wr[3];
+        const int i = k >> 1;
+        if (k & 1)
+          Udt[ubase + (long)i * L + m0 + q] = (xs / j1[i]) * j1[i];
+        else
+          Uy[ubase + (long)i * L + m0 + q] = xs;
+        u6 = u5, u5 = u4, u4 = u3, u3 = u2, u2 = u1, u1 = xs;
       }
     }
     __syncthreads();
@@ -268,13 +330,14 @@ void launch_helmholtz_radial(const AtomTables &at, const HelmholtzTables &ht, in
                     std::to_string(HELMHOLTZ_MAX_NRAD) + " points are not supported");
   NAPMO_REQUIRE(at.max_nrad >= 3, "orbital correction: radial grids need at least 3 points");
   const int L = (lmax + 1) * (lmax + 1);
-  const long nfn = 2L * at.max_nrad;
-  const long fixed = nfn * (WSLOT + 3) + 2L * at.max_nrad + nfn;  // W, Lf, ay, ad, an
-  const long budget = 8192;                                        // doubles (64 KB)
-  long mc = (budget - fixed - nfn / 2 - 1) / nfn;                  // ipiv ints take nfn/2
-  mc = std::min<long>(mc, 2L * lmax + 1);
-  NAPMO_REQUIRE(mc >= 1, "orbital correction: radial system does not fit in LDS");
-  const size_t bytes = (size_t)(fixed + nfn * mc) * sizeof(double) + (size_t)nfn * sizeof(int);
+  const long nr = at.max_nrad, nfn = 2L * nr;
+  // W, Lf, spline factors, ipiv, then nfn x mc columns (at least 3 nfn for the assembly);
+  // mc sized so the block stays near 40 KB and several blocks share a CU
+  const long fixed = nfn * (WSLOT + 3) + 2 * nr + nfn / 2;  // doubles (ipiv: nfn ints)
+  const long target = 40 * 1024 / 8;
+  const long mc = std::max<long>(1, std::min<long>((target - fixed) / nfn, 2L * lmax + 1));
+  const size_t bytes = (size_t)(fixed + nfn * std::max<long>(mc, 3)) * sizeof(double);
+  NAPMO_REQUIRE(bytes <= max_lds_per_block(), "orbital correction: radial system does not fit in LDS");
   k_helmholtz_radial<<<dim3(at.M, lmax + 1, norb), 64, bytes, st>>>(
       at, ht, L, nrad_total, Vsph, oi, doi, mass_inv, mu, (int)mc, Uy, Udt, status);
   NAPMO_HIP(hipGetLastError());

@@ -127,13 +127,13 @@ void launch_axpby(double a, const double *x, double b, const double *y, long N, 
 // ---- helmholtz.hip: the Becke-Dickson orbital correction (napmo/wavefunction/ndpsi.py)
 // Finite-element tables of one radial grid (helmholtz_tables in poisson_plan.cpp), per atom
 // offsets into concatenated arrays: c0 [2n x 7] band of the Galerkin matrix without the
-// a term, oa [2n x 7 x 6] a-term integrals, rmap [2n x n] right-hand side of f = mu,
-// j1 / j2 [sum nrad] (indexed like AtomTables::r).
+// a term, oa [2n x 7 x 6] a-term integrals, o2 [2n x 6] right-hand-side integrals of the
+// merged source f, j1 / j2 [sum nrad] (indexed like AtomTables::r).
 struct HelmholtzTables {
-  const double *c0, *oa, *rmap, *j1, *j2;
-  const long *c0_off, *oa_off, *rmap_off;
+  const double *c0, *oa, *o2, *j1, *j2;
+  const long *c0_off, *oa_off, *o2_off;
 };
-constexpr int HELMHOLTZ_MAX_NRAD = 240;  // LDS of one radial system (64 KB: 2n x 15 doubles + RHS)
+constexpr int HELMHOLTZ_MAX_NRAD = 240;  // LDS of one radial system: 2n x 17 doubles + columns
 // out[o][p] = a[o][p] * s[o] - b[o][p] for norb rows of N points (s: device [norb]).
 void launch_scaled_diff(const double *a, const double *s, const double *b, int norb, long N,
                         double *out, hipStream_t st);

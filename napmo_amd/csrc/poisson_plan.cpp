@@ -238,7 +238,7 @@ void radial_green(const RTransform &rtf, const std::vector<double> &rw, int l,
 }
 
 void helmholtz_tables(const RTransform &rtf, std::vector<double> &c0, std::vector<double> &oa,
-                      std::vector<double> &rmap, std::vector<double> &j1, std::vector<double> &j2) {
+                      std::vector<double> &o2, std::vector<double> &j1, std::vector<double> &j2) {
   const int n = rtf.npoint, nfn = 2 * n;
   const long xmax = n - 1;
   std::vector<double> r, j3, b;
@@ -248,9 +248,19 @@ void helmholtz_tables(const RTransform &rtf, std::vector<double> &c0, std::vecto
   fe_assemble(b.data(), za.data(), za.data(), bcs, A.data(), zr.data(), n);
   c0.assign((size_t)nfn * 7, 0.0);
   oa.assign((size_t)nfn * 7 * 6, 0.0);
+  o2.assign((size_t)nfn * 6, 0.0);
   for (long row = 0; row < nfn; ++row) {
     const bool cond = row == 0 || row == nfn - 2;
     const long begin = std::max(2 * (row / 2 - 1), 0L), end = std::min(2 * (row / 2 + 2), (long)nfn);
+    // right-hand side integrals of f against the row's test function, f_k for
+    // k = 2 (row / 2 - 1) + kk (zero where k falls off the grid)
+    for (int kk = 0; kk < 6 && !cond; ++kk) {
+      const long k = 2 * (row / 2 - 1) + kk;
+      if (k < 0 || k >= nfn) continue;
+      long idx[2] = {row, k};
+      int der[2] = {0, 0};
+      o2[(size_t)row * 6 + kk] = fe_overlap(xmax, idx, der, 2);
+    }
     for (int off = -3; off <= 3; ++off) {
       const long col = row + off;
       if (col < 0 || col >= nfn) continue;
@@ -263,7 +273,6 @@ void helmholtz_tables(const RTransform &rtf, std::vector<double> &c0, std::vecto
       }
     }
   }
-  fe_rhs_map(n, j1, j2, 1.0, rmap);
 }
 
 void kinetic_green(const RTransform &rtf, double radii, const std::vector<double> &r, int l,
@@ -587,8 +596,8 @@ int DeviceGrid::profile_read(double *avg_ms, int nphase) {
 
 HelmholtzTables HelmholtzPlan::tables() const {
   HelmholtzTables t;
-  t.c0 = c0.get(), t.oa = oa.get(), t.rmap = rmap.get(), t.j1 = j1.get(), t.j2 = j2.get();
-  t.c0_off = c0_off.get(), t.oa_off = oa_off.get(), t.rmap_off = rmap_off.get();
+  t.c0 = c0.get(), t.oa = oa.get(), t.o2 = o2.get(), t.j1 = j1.get(), t.j2 = j2.get();
+  t.c0_off = c0_off.get(), t.oa_off = oa_off.get(), t.o2_off = o2_off.get();
   return t;
 }
 
@@ -596,16 +605,16 @@ HelmholtzPlan &DeviceGrid::helmholtz() {
   if (hplan) return *hplan;
   require_atoms("orbital correction");
   auto h = std::make_unique<HelmholtzPlan>();
-  std::vector<double> c0, oa, rm, j1all, j2all;
-  std::vector<long> c0o(M), oao(M), rmo(M);
+  std::vector<double> c0, oa, o2, j1all, j2all;
+  std::vector<long> c0o(M), oao(M), o2o(M);
   std::vector<std::pair<std::vector<double>, int>> seen;  // (radial grid key, first atom)
   for (int a = 0; a < M; ++a) {
     const AtomInfo &at = atoms[a];
     NAPMO_REQUIRE(at.nrad >= 3 && at.nrad <= HELMHOLTZ_MAX_NRAD,
                   "orbital correction: radial grids need 3 to " +
                       std::to_string(HELMHOLTZ_MAX_NRAD) + " points");
-    std::vector<double> tc0, toa, trm, tj1, tj2;
-    helmholtz_tables(at.rtf, tc0, toa, trm, tj1, tj2);
+    std::vector<double> tc0, toa, to2, tj1, tj2;
+    helmholtz_tables(at.rtf, tc0, toa, to2, tj1, tj2);
     j1all.insert(j1all.end(), tj1.begin(), tj1.end());
     j2all.insert(j2all.end(), tj2.begin(), tj2.end());
     std::vector<double> key = {(double)at.rtf.kind, at.rtf.p0, at.rtf.p1, (double)at.nrad};
@@ -613,18 +622,18 @@ HelmholtzPlan &DeviceGrid::helmholtz() {
     for (auto &s : seen)
       if (s.first == key) first = s.second;
     if (first >= 0) {
-      c0o[a] = c0o[first], oao[a] = oao[first], rmo[a] = rmo[first];
+      c0o[a] = c0o[first], oao[a] = oao[first], o2o[a] = o2o[first];
       continue;
     }
     seen.push_back({key, a});
-    c0o[a] = (long)c0.size(), oao[a] = (long)oa.size(), rmo[a] = (long)rm.size();
+    c0o[a] = (long)c0.size(), oao[a] = (long)oa.size(), o2o[a] = (long)o2.size();
     c0.insert(c0.end(), tc0.begin(), tc0.end());
     oa.insert(oa.end(), toa.begin(), toa.end());
-    rm.insert(rm.end(), trm.begin(), trm.end());
+    o2.insert(o2.end(), to2.begin(), to2.end());
   }
-  h->c0.upload(c0, stream), h->oa.upload(oa, stream), h->rmap.upload(rm, stream);
+  h->c0.upload(c0, stream), h->oa.upload(oa, stream), h->o2.upload(o2, stream);
   h->j1.upload(j1all, stream), h->j2.upload(j2all, stream);
-  h->c0_off.upload(c0o, stream), h->oa_off.upload(oao, stream), h->rmap_off.upload(rmo, stream);
+  h->c0_off.upload(c0o, stream), h->oa_off.upload(oao, stream), h->o2_off.upload(o2o, stream);
   NAPMO_HIP(hipStreamSynchronize(stream));
   hplan = std::move(h);
   return *hplan;

@@ -38,10 +38,11 @@ void kinetic_green(const RTransform &rtf, double radii, const std::vector<double
 // radial grid (napmo/wavefunction/ndpsi.py:52-81 through napmo/utilities/ode2.py:35-118 and
 // src/horton/ode2.cpp:232-321), with u = 0 at both ends: c0 [2n x 7] the Galerkin band
 // without the a term (slot = col - row + 3), oa [2n x 7 x 6] the integrals multiplying the
-// merged a coefficients begin(row) + k, rmap [2n x n] the right-hand side of f = mu, and
-// the t-derivatives j1, j2 of r(t).
+// merged a coefficients begin(row) + k, o2 [2n x 6] the right-hand-side integrals of the
+// merged source f_k, k = 2 (row / 2 - 1) + kk (zero rows for the two value conditions),
+// and the t-derivatives j1, j2 of r(t).
 void helmholtz_tables(const RTransform &rtf, std::vector<double> &c0, std::vector<double> &oa,
-                      std::vector<double> &rmap, std::vector<double> &j1, std::vector<double> &j2);
+                      std::vector<double> &o2, std::vector<double> &j1, std::vector<double> &j2);
 
 // Per-Lebedev-rule projection table Yhat[j][lm] = aw_j S_lm(u_j) sqrt(4 pi (2l+1)) / sum aw.
 std::vector<double> projection_table(const AngularGrid &ang, int lmax);
@@ -78,8 +79,8 @@ struct SynthPairs {
 
 // Orbital-correction tables of every atom (one copy per distinct radial grid).
 struct HelmholtzPlan {
-  DeviceBuffer<double> c0, oa, rmap, j1, j2;
-  DeviceBuffer<long> c0_off, oa_off, rmap_off;
+  DeviceBuffer<double> c0, oa, o2, j1, j2;
+  DeviceBuffer<long> c0_off, oa_off, o2_off;
   HelmholtzTables tables() const;
 };
 
