@@ -341,8 +341,10 @@ enum {
                               2: one-pass kernel on G = sqrt|s| Phi (F = G^T diag(sign s) G) */
   NAPMO_TUNE_DENSITY_KS = 3, /* K per LDS stage of the density kernels: 16 or 32; 0 = 16 */
   NAPMO_TUNE_DENSITY_PF = 4, /* stages in flight in the density kernels: 1 or 2; 0 = 1 */
-  NAPMO_TUNE_SYNTH_NP = 5    /* (point, centre) pairs per lane of the synthesis, 1 or 2; 0 = 2
+  NAPMO_TUNE_SYNTH_NP = 5,   /* (point, centre) pairs per lane of the synthesis, 1 or 2; 0 = 2
                                 (a property of the grid: shared by every plan on it) */
+  NAPMO_TUNE_DENSITY_STREAM = 6 /* orbital-form density: 1 (default) keeps C in LDS and streams
+                                   Phi when C fits (nocc <= 64), 0 uses the LDS-staged kernels */
 };
 int napmo_fock_plan_set_tuning(void *plan, int key, long value);
 int napmo_fock_plan_get_tuning(void *plan, int key, long *value);

@@ -164,6 +164,111 @@ __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__
     }
 }
 
+// Orbital form with C resident in LDS: the whole zero-padded C [kend x 16 NB] is copied
+// once per workgroup into fragment order, Cf[g][j][lane][s] = C[16 g + 4 (lane / 16) + s]
+// [16 j + lane % 16], and every wave then streams its own Phi rows from HBM straight into
+// MFMA operands with no LDS staging and no barrier: for K group g (16 functions) lane
+// (kr, mi) loads Phi[row mi][16 g + 4 kr .. + 3] as one 32-byte vector, and step s of the
+// group multiplies element s of it by Cf[g][j][lane][s] (the k order inside a group is
+// permuted identically on both operands). A wave owns 16 RB rows at a time and walks the
+// row tiles with a stride of all waves in the grid; two groups are in flight per wave.
+template <int NB, int RB, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_density_orb_stream(
+    const double *__restrict__ phi, int ld, long Npad, const double *__restrict__ C, int ldc,
+    int kend, double scale, long nvalid, long ntiles, double *__restrict__ out) {
+  extern __shared__ d4 Cf[];  // [G][NB][64] vectors of 4 steps
+  const int G = kend / 16;
+  for (int e = threadIdx.x; e < G * NB * 256; e += 64 * WAVES) {
+    const int st = e & 3, ln = (e >> 2) & 63, gj = e >> 8;
+    const int j = gj % NB, g = gj / NB;
+    ((double *)Cf)[e] = C[(long)(16 * g + 4 * (ln >> 4) + st) * ldc + 16 * j + (ln & 15)];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, kr = lane >> 4, mi = lane & 15;
+  const long nw = (long)gridDim.x * WAVES;
+  for (long t = (long)blockIdx.x * WAVES + wave; t < ntiles; t += nw) {
+    const long r0 = t * 16 * RB;
+    const double *ap[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+      const long row = min(r0 + rb * 16 + mi, Npad - 1);  // rows past Npad: read, never written
+      ap[rb] = phi + row * ld + 4 * kr;
+    }
+    d4 acc[RB][NB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) acc[rb][j] = (d4){0, 0, 0, 0};
+    auto load = [&](d4 (&a)[RB], int g) {
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) a[rb] = *(const d4 *)(ap[rb] + 16 * g);
+    };
+    auto compute = [&](const d4 (&a)[RB], int g) {
+      d4 b[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) b[j] = Cf[(g * NB + j) * 64 + lane];
+#pragma unroll
+      for (int st = 0; st < 4; ++st)
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            acc[rb][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb][st], b[j][st], acc[rb][j], 0, 0, 0);
+    };
+    d4 a0[RB], a1[RB];
+    load(a0, 0);
+    if (G > 1) load(a1, 1);
+    for (int g = 0; g < G; g += 2) {
+      compute(a0, g);
+      if (g + 2 < G) load(a0, g + 2);
+      if (g + 1 >= G) break;
+      compute(a1, g + 1);
+      if (g + 3 < G) load(a1, g + 3);
+    }
+    // f64 16x16x4 output: lane i, element r -> row (i / 16) + 4 r, column i % 16
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long p = r0 + rb * 16 + kr + 4 * r;
+        double sum = 0.0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) sum += acc[rb][j][r] * acc[rb][j][r];
+        sum += __shfl_xor(sum, 8);
+        sum += __shfl_xor(sum, 4);
+        sum += __shfl_xor(sum, 2);
+        sum += __shfl_xor(sum, 1);
+        if (mi == 0 && p < nvalid) out[p] = scale * sum;
+      }
+  }
+}
+
+constexpr int STREAM_RB = 3, STREAM_WAVES = 8;
+
+template <int NB>
+void launch_orb_stream_nb(const double *phi, int ld, long Npad, const double *C, int ldc, int kend,
+                          double scale, long nvalid, double *out, hipStream_t s) {
+  const size_t bytes = (size_t)kend * 16 * NB * sizeof(double);
+  const long ntiles = div_up(Npad, 16L * STREAM_RB);
+  auto kern = k_density_orb_stream<NB, STREAM_RB, STREAM_WAVES>;
+  static thread_local int configured = -1;  // per instantiation: dynamic LDS above 64 KB
+  int dev = 0;
+  NAPMO_HIP(hipGetDevice(&dev));
+  if (configured != dev) {
+    NAPMO_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)max_lds_per_block()));
+    configured = dev;
+  }
+  int occ = 0, cus = 0;
+  NAPMO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64 * STREAM_WAVES, bytes));
+  NAPMO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  NAPMO_REQUIRE(occ >= 1, "density_orb_stream: no occupancy");
+  const long blocks = std::min<long>((long)occ * cus, div_up(ntiles, STREAM_WAVES));
+  kern<<<dim3((unsigned)blocks), dim3(64 * STREAM_WAVES), bytes, s>>>(phi, ld, Npad, C, ldc, kend,
+                                                                       scale, nvalid, ntiles, out);
+  NAPMO_HIP(hipGetLastError());
+}
+
 template <int MODE, int DKS, int PF>
 void launch_density_ks(int nb, dim3 tiles, const double *phi, int ld, long Npad, const double *B,
                        int ldb, int nreal, double scale, long nvalid, double *partial,
@@ -227,6 +332,26 @@ void launch_density_orb_f64(const double *phi, int ld, long Npad, const double *
                 "density_orb: a direct (scaled) write needs one column tile");
   launch_density_nb<MODE_ORB>(nb, dim3(Npad / DPT, density_orb_tiles(ldc)), phi, ld, Npad, C, ldc,
                               partial, s, ks, pf, nreal, scale, nvalid);
+}
+
+bool density_orb_stream_supported(int ld, int ldc, int nreal) {
+  if (ldc > 64 || ldc % 16 || ld % 16) return false;
+  const int kend = (nreal > 0 && nreal < ld) ? (int)round_up(nreal, 16) : ld;
+  return (size_t)kend * ldc * sizeof(double) <= max_lds_per_block();
+}
+
+void launch_density_orb_stream(const double *phi, int ld, long Npad, const double *C, int ldc,
+                               double *out, hipStream_t s, int nreal, double scale, long nvalid) {
+  NAPMO_REQUIRE(density_orb_stream_supported(ld, ldc, nreal) && Npad % 16 == 0,
+                "density_orb_stream: C does not fit in LDS or bad padding");
+  const int kend = (nreal > 0 && nreal < ld) ? (int)round_up(nreal, 16) : ld;
+  if (nvalid < 0) nvalid = Npad;
+  switch (ldc / 16) {
+    case 1: launch_orb_stream_nb<1>(phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
+    case 2: launch_orb_stream_nb<2>(phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
+    case 3: launch_orb_stream_nb<3>(phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
+    default: launch_orb_stream_nb<4>(phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
+  }
 }
 
 void launch_phi_times(const double *phi, int ld, long Npad, const double *B, int ldb, double *X,

@@ -102,6 +102,10 @@ void FockPlan::set_tuning(int key, long value) {
       NAPMO_REQUIRE(value == 0 || value == 1 || value == 2, "synthesis pairs per lane must be 1 or 2");
       grid.synth_np = value ? (int)value : 2;
       break;
+    case NAPMO_TUNE_DENSITY_STREAM:
+      NAPMO_REQUIRE(value == 0 || value == 1, "density streaming choice must be 0 or 1");
+      density_stream = (int)value;
+      break;
     default: throw Error("unknown tuning key " + std::to_string(key));
   }
   NAPMO_HIP(hipStreamSynchronize(stream));
@@ -116,6 +120,7 @@ long FockPlan::get_tuning(int key) const {
     case NAPMO_TUNE_DENSITY_KS: return density_ks;
     case NAPMO_TUNE_DENSITY_PF: return density_pf;
     case NAPMO_TUNE_SYNTH_NP: return grid.synth_np;
+    case NAPMO_TUNE_DENSITY_STREAM: return density_stream;
     default: throw Error("unknown tuning key " + std::to_string(key));
   }
 }
@@ -181,7 +186,11 @@ void FockPlan::density_C(int slot, const double *dC, int nocc, double eta) {
   launch_pad2d(dC, n, nocc, Cpad.get(), ld, ldc, st);
   launch_gram(dC, n, nocc, eta, Pfull.get(), st);
   mark(slot, 0);
-  if (nct == 1) {
+  if (nct == 1 && density_stream && density_orb_stream_supported(ld, ldc, n)) {
+    // C resident in LDS, Phi streamed; the epilogue writes rho = eta sum_i X_pi^2 itself
+    launch_density_orb_stream(phi64.get(), ld, Npad, Cpad.get(), ldc, rho.get(), st, n, eta, grid.N);
+    mark(slot, 1);
+  } else if (nct == 1) {
     // one column tile: the kernel's epilogue writes rho = eta sum_i X_pi^2 itself
     launch_density_orb_f64(phi64.get(), ld, Npad, Cpad.get(), ldc, rho.get(), st, density_ks,
                            density_pf, n, eta, grid.N);

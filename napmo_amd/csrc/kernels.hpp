@@ -246,6 +246,12 @@ int density_orb_tiles(int ldc);
 void launch_density_orb_f64(const double *phi, int ld, long Npad, const double *C, int ldc,
                             double *partial, hipStream_t s, int ks = 16, int pf = 1, int nreal = 0,
                             double scale = 1.0, long nvalid = -1);
+// Orbital form with all of C [ld x ldc] (ldc <= 64, rows from round_up(nreal, 16) on zero)
+// resident in LDS, Phi streamed into MFMA operands: out[p] = scale sum_c X_pc^2 for
+// p < nvalid (nvalid < 0: Npad).
+bool density_orb_stream_supported(int ld, int ldc, int nreal);
+void launch_density_orb_stream(const double *phi, int ld, long Npad, const double *C, int ldc,
+                               double *out, hipStream_t s, int nreal, double scale, long nvalid);
 void launch_gram(const double *C, int n, int nocc, double eta, double *P, hipStream_t s);
 void launch_pad2d(const double *src, int rows, int cols, double *dst, int drows, int dcols,
                   hipStream_t s);

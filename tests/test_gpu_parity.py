@@ -298,6 +298,36 @@ def test_grid_fock_orbital_form_matches_density_form():
     assert np.max(np.abs(Fo - Fp)) < 1e-10
 
 
+def test_orbital_density_streamed_kernel_matches_staged():
+    # the orbital-form density with C resident in LDS (Phi streamed, k order permuted inside
+    # each 16-function group) against the LDS-staged kernel and the P form, for 1 to 4
+    # column blocks of 16 orbitals and a point count that leaves a partial last row tile
+    import napmo_amd
+    from napmo_amd import autotune
+    atoms = _mol(37, natom=5, box=5.0)
+    shells = systems.random_shells(np.random.default_rng(37), 5, 14)
+    g, b = systems.gpu_system(atoms, shells, 30, 110)
+    fock = napmo_amd.GridFock(g, b, 8, 'fp64')
+    n = b['nbasis']
+    assert n >= 64 and fock.npoint % 48, (n, fock.npoint)
+    rng = np.random.default_rng(5)
+    for nocc in (1, 17, 40, 64):
+        C = rng.standard_normal((n, nocc)) / np.sqrt(n)
+        out = {}
+        for ds in (0, 1):
+            autotune.apply(fock, dict(autotune.DEFAULTS, density_stream=ds))
+            out[ds] = fock.build_orbitals(C, 2.0, want_grid=True)
+        Fp, Ep, Egp, rhop, vp = fock.build(2.0 * C @ C.T, want_grid=True)
+        rs, r0 = out[1][3], out[0][3]
+        print('nocc %d n %d N %d: max|drho| stream-staged %.2e, stream-P %.2e (max rho %.2e)' % (
+            nocc, n, fock.npoint, np.abs(rs - r0).max(), np.abs(rs - rhop).max(), np.abs(rhop).max()))
+        assert np.abs(rs - r0).max() <= 1e-13 * np.abs(r0).max(), nocc
+        assert np.abs(rs - rhop).max() <= 1e-12 * np.abs(rhop).max(), nocc
+        assert abs(out[1][2] - out[0][2]) <= 1e-12 * abs(out[0][2]), nocc
+        assert np.abs(out[1][0] - out[0][0]).max() <= 1e-12 * np.abs(out[0][0]).max(), nocc
+    autotune.apply(fock, autotune.DEFAULTS)
+
+
 def test_grid_fock_bf16x3_mixed_mode():
     # mixed mode: rho, Poisson and the energy trace stay FP64, only F uses the 3-term
     # bf16 split. Stated tolerances: |dE_grid| <= 1e-10 Eh, max|dF| <= 2e-5 max|F|.
