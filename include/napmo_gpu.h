@@ -343,8 +343,9 @@ enum {
   NAPMO_TUNE_DENSITY_PF = 4, /* stages in flight in the density kernels: 1 or 2; 0 = 1 */
   NAPMO_TUNE_SYNTH_NP = 5,   /* (point, centre) pairs per lane of the synthesis, 1 or 2; 0 = 2
                                 (a property of the grid: shared by every plan on it) */
-  NAPMO_TUNE_DENSITY_STREAM = 6 /* orbital-form density: 1 (default) keeps C in LDS and streams
-                                   Phi when C fits (nocc <= 64), 0 uses the LDS-staged kernels */
+  NAPMO_TUNE_DENSITY_STREAM = 6 /* orbital-form density: 1..4 keep C in LDS and stream Phi when
+                                   C fits (nocc <= 64), in one of three launch shapes (1: default);
+                                   0 uses the LDS-staged kernels */
 };
 int napmo_fock_plan_set_tuning(void *plan, int key, long value);
 int napmo_fock_plan_get_tuning(void *plan, int key, long *value);

@@ -5,7 +5,8 @@ shape of the Fock contraction on the device (the per-tile FP64 / BF16x3 kernels'
 count, the one-pass BF16x3 kernel's workgroup count, and the choice between the two
 BF16x3 kernels) together with the K depth and prefetch depth of the density kernels' LDS
 stages, then the synthesis' pairs per lane (1 or 2) together with the orbital-form density's
-choice of kernel (C resident in LDS with Phi streamed, or the LDS-staged kernel), keeps the fastest, and stores it in a JSON cache keyed by the shape class
+choice of kernel (C resident in LDS with Phi streamed in one of two launch shapes, or the
+LDS-staged kernel), keeps the fastest, and stores it in a JSON cache keyed by the shape class
 (precision, padded basis size, point count rounded to a power of two), so later plans of
 the same class start from the cached choice without timing. The cache lives at
 $NAPMO_TUNING_CACHE or ~/.cache/napmo_amd/tuning.json.
@@ -118,7 +119,7 @@ def tune(fock, dP, dF, dE, reps=5, path=None, force=False, build=None, tag=''):
     # the three fastest again with twice the builds (single timings differ by a few %)
     scored.sort(key=lambda r: r[1])
     best = min(((c, timed(c, 2 * reps)) for c, _ in scored[:3]), key=lambda r: r[1])
-    per_np = [dict(best[0], synth_np=np_, density_stream=ds) for np_ in (1, 2) for ds in (0, 1)]
+    per_np = [dict(best[0], synth_np=np_, density_stream=ds) for np_ in (1, 2) for ds in (0, 1, 2)]
     best = min(((c, timed(c, 2 * reps)) for c in per_np), key=lambda r: r[1])
     apply(fock, dict(DEFAULTS, **best[0]))
     cache[key] = dict(best[0], ms=best[1] * 1e3)

@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__
 // group multiplies element s of it by Cf[g][j][lane][s] (the k order inside a group is
 // permuted identically on both operands). A wave owns 16 RB rows at a time and walks the
 // row tiles with a stride of all waves in the grid; two groups are in flight per wave.
-template <int NB, int RB, int WAVES>
+template <int NB, int RB, int WAVES, int PF>
 __global__ void __launch_bounds__(64 * WAVES) k_density_orb_stream(
     const double *__restrict__ phi, int ld, long Npad, const double *__restrict__ C, int ldc,
     int kend, double scale, long nvalid, long ntiles, double *__restrict__ out) {
@@ -215,15 +215,20 @@ __global__ void __launch_bounds__(64 * WAVES) k_density_orb_stream(
           for (int j = 0; j < NB; ++j)
             acc[rb][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb][st], b[j][st], acc[rb][j], 0, 0, 0);
     };
-    d4 a0[RB], a1[RB];
-    load(a0, 0);
-    if (G > 1) load(a1, 1);
-    for (int g = 0; g < G; g += 2) {
-      compute(a0, g);
-      if (g + 2 < G) load(a0, g + 2);
-      if (g + 1 >= G) break;
-      compute(a1, g + 1);
-      if (g + 3 < G) load(a1, g + 3);
+    // a ring of PF groups in flight: slot i holds group g0 + i
+    d4 a[PF][RB];
+#pragma unroll
+    for (int i = 0; i < PF; ++i)
+      if (i < G) load(a[i], i);
+    for (int g0 = 0; g0 < G; g0 += PF) {
+#pragma unroll
+      for (int i = 0; i < PF; ++i) {
+        const int g = g0 + i;
+        if (g < G) {
+          compute(a[i], g);
+          if (g + PF < G) load(a[i], g + PF);
+        }
+      }
     }
     // f64 16x16x4 output: lane i, element r -> row (i / 16) + 4 r, column i % 16
 #pragma unroll
@@ -243,14 +248,13 @@ __global__ void __launch_bounds__(64 * WAVES) k_density_orb_stream(
   }
 }
 
-constexpr int STREAM_RB = 3, STREAM_WAVES = 8;
-
-template <int NB>
+template <int NB, int RB, int WAVES, int PF>
 void launch_orb_stream_nb(const double *phi, int ld, long Npad, const double *C, int ldc, int kend,
                           double scale, long nvalid, double *out, hipStream_t s) {
+  constexpr int STREAM_RB = RB, STREAM_WAVES = WAVES;
   const size_t bytes = (size_t)kend * 16 * NB * sizeof(double);
   const long ntiles = div_up(Npad, 16L * STREAM_RB);
-  auto kern = k_density_orb_stream<NB, STREAM_RB, STREAM_WAVES>;
+  auto kern = k_density_orb_stream<NB, RB, WAVES, PF>;
   static thread_local int configured = -1;  // per instantiation: dynamic LDS above 64 KB
   int dev = 0;
   NAPMO_HIP(hipGetDevice(&dev));
@@ -340,17 +344,30 @@ bool density_orb_stream_supported(int ld, int ldc, int nreal) {
   return (size_t)kend * ldc * sizeof(double) <= max_lds_per_block();
 }
 
+template <int RB, int WAVES, int PF>
+static void orb_stream_shape(int nb, const double *phi, int ld, long Npad, const double *C, int ldc,
+                             int kend, double scale, long nvalid, double *out, hipStream_t s) {
+  switch (nb) {
+    case 1: launch_orb_stream_nb<1, RB, WAVES, PF>(phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
+    case 2: launch_orb_stream_nb<2, RB, WAVES, PF>(phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
+    case 3: launch_orb_stream_nb<3, RB, WAVES, PF>(phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
+    default: launch_orb_stream_nb<4, RB, WAVES, PF>(phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
+  }
+}
+
 void launch_density_orb_stream(const double *phi, int ld, long Npad, const double *C, int ldc,
-                               double *out, hipStream_t s, int nreal, double scale, long nvalid) {
+                               double *out, hipStream_t s, int nreal, double scale, long nvalid,
+                               int shape) {
   NAPMO_REQUIRE(density_orb_stream_supported(ld, ldc, nreal) && Npad % 16 == 0,
                 "density_orb_stream: C does not fit in LDS or bad padding");
   const int kend = (nreal > 0 && nreal < ld) ? (int)round_up(nreal, 16) : ld;
   if (nvalid < 0) nvalid = Npad;
-  switch (ldc / 16) {
-    case 1: launch_orb_stream_nb<1>(phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
-    case 2: launch_orb_stream_nb<2>(phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
-    case 3: launch_orb_stream_nb<3>(phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
-    default: launch_orb_stream_nb<4>(phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
+  const int nb = ldc / 16;
+  // shapes: rows per wave (16 RB), waves per workgroup, K groups in flight
+  switch (shape) {
+    case 2: orb_stream_shape<3, 8, 2>(nb, phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
+    case 3: orb_stream_shape<3, 8, 3>(nb, phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
+    default: orb_stream_shape<2, 12, 3>(nb, phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
   }
 }
 

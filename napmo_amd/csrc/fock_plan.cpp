@@ -103,7 +103,7 @@ void FockPlan::set_tuning(int key, long value) {
       grid.synth_np = value ? (int)value : 2;
       break;
     case NAPMO_TUNE_DENSITY_STREAM:
-      NAPMO_REQUIRE(value == 0 || value == 1, "density streaming choice must be 0 or 1");
+      NAPMO_REQUIRE(value >= 0 && value <= 3, "density streaming choice must be 0 to 3");
       density_stream = (int)value;
       break;
     default: throw Error("unknown tuning key " + std::to_string(key));
@@ -188,7 +188,8 @@ void FockPlan::density_C(int slot, const double *dC, int nocc, double eta) {
   mark(slot, 0);
   if (nct == 1 && density_stream && density_orb_stream_supported(ld, ldc, n)) {
     // C resident in LDS, Phi streamed; the epilogue writes rho = eta sum_i X_pi^2 itself
-    launch_density_orb_stream(phi64.get(), ld, Npad, Cpad.get(), ldc, rho.get(), st, n, eta, grid.N);
+    launch_density_orb_stream(phi64.get(), ld, Npad, Cpad.get(), ldc, rho.get(), st, n, eta, grid.N,
+                              density_stream);
     mark(slot, 1);
   } else if (nct == 1) {
     // one column tile: the kernel's epilogue writes rho = eta sum_i X_pi^2 itself

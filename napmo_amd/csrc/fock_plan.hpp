@@ -36,7 +36,8 @@ struct FockPlan {
   long fock_wg = 0, pass_wg = 0;  // workgroup targets of the two shapes (0 = default)
   int density_ks = 16;            // K per LDS stage of the density kernels (16 or 32)
   int density_pf = 1;             // stages in flight in the density kernels (1 or 2)
-  int density_stream = 1;         // orbital form: C resident in LDS, Phi streamed (when it fits)
+  int density_stream = 1;  // orbital form: C resident in LDS, Phi streamed (when it fits), in
+                           // launch shape 1..3; 0: the LDS-staged kernels
   hipStream_t stream = nullptr;   // every build launch goes here (default: the grid's)
   void reshape();
   // Launch-shape knobs (NAPMO_TUNE_*), for the autotuner; not while a build is running.

@@ -314,17 +314,20 @@ def test_orbital_density_streamed_kernel_matches_staged():
     for nocc in (1, 17, 40, 64):
         C = rng.standard_normal((n, nocc)) / np.sqrt(n)
         out = {}
-        for ds in (0, 1):
+        for ds in (0, 1, 2, 3):      # staged kernel, then the three streamed launch shapes
             autotune.apply(fock, dict(autotune.DEFAULTS, density_stream=ds))
             out[ds] = fock.build_orbitals(C, 2.0, want_grid=True)
         Fp, Ep, Egp, rhop, vp = fock.build(2.0 * C @ C.T, want_grid=True)
-        rs, r0 = out[1][3], out[0][3]
-        print('nocc %d n %d N %d: max|drho| stream-staged %.2e, stream-P %.2e (max rho %.2e)' % (
-            nocc, n, fock.npoint, np.abs(rs - r0).max(), np.abs(rs - rhop).max(), np.abs(rhop).max()))
-        assert np.abs(rs - r0).max() <= 1e-13 * np.abs(r0).max(), nocc
-        assert np.abs(rs - rhop).max() <= 1e-12 * np.abs(rhop).max(), nocc
-        assert abs(out[1][2] - out[0][2]) <= 1e-12 * abs(out[0][2]), nocc
-        assert np.abs(out[1][0] - out[0][0]).max() <= 1e-12 * np.abs(out[0][0]).max(), nocc
+        r0 = out[0][3]
+        for ds in (1, 2, 3):
+            rs = out[ds][3]
+            print('nocc %d n %d N %d shape %d: max|drho| stream-staged %.2e, stream-P %.2e '
+                  '(max rho %.2e)' % (nocc, n, fock.npoint, ds, np.abs(rs - r0).max(),
+                                      np.abs(rs - rhop).max(), np.abs(rhop).max()))
+            assert np.abs(rs - r0).max() <= 1e-13 * np.abs(r0).max(), (nocc, ds)
+            assert np.abs(rs - rhop).max() <= 1e-12 * np.abs(rhop).max(), (nocc, ds)
+            assert abs(out[ds][2] - out[0][2]) <= 1e-12 * abs(out[0][2]), (nocc, ds)
+            assert np.abs(out[ds][0] - out[0][0]).max() <= 1e-12 * np.abs(out[0][0]).max(), (nocc, ds)
     autotune.apply(fock, autotune.DEFAULTS)
 
 
