@@ -25,6 +25,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {'fp64_mfma': 78.6e12, 'fp32_mfma': 157.3e12, 'bf16_mfma': 2.5e15, 'hbm': 8.0e12}
+# Synthesis work per (point, centre, lm) term: the cubic spline of U_lm evaluated at the
+# point's radius (Horner, 3 multiply-adds) and its product with the solid harmonic added
+# to the sum (1 multiply-add); the harmonic's own recurrence is not counted.
+SYNTH_FLOPS = 8.0
 METRIC = 'grid Fock builds/s + % roofline, mixed vs FP64 on 1 MI355X at |dE|<=1uEh'
 
 
@@ -239,11 +243,11 @@ def dpsi_leg(args, log):
     log('dpsi step: %.3f ms (%.1f steps/s), phases %s; de[0:3] %s, max|dpsi| %.3e' % (
         dt * 1e3, 1.0 / dt, {k: round(v, 4) for k, v in phases.items()}, de[:3], np.abs(dpsi).max()))
     M, L = len(atoms), (lmax + 1) ** 2
-    # dominant kernel: the synthesis, 2 M N L flops per orbital by the counting convention
+    # dominant kernel: the synthesis, SYNTH_FLOPS M N L flops per orbital
     name = max(phases, key=phases.get)
     ms = phases[name]
     if name == 'synth':
-        flops = 2.0 * M * N * L * norb
+        flops = SYNTH_FLOPS * M * N * L * norb
         roof = {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': PEAK['fp64_mfma'] / 1e12,
                 'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / PEAK['fp64_mfma'],
                 # HBM bytes of one synthesis launch (one orbital)
@@ -346,7 +350,7 @@ def operators_leg(args, log):
         'kinetic %.3f ms per orbital' % (ndim, npair, dt * 1e3, dt * 1e3 / npair,
                                          {k: round(v, 3) for k, v in phases.items()}, dt_kin * 1e3))
     M, L = len(atoms), (lmax + 1) ** 2
-    flops = 2.0 * M * N * L * npair
+    flops = SYNTH_FLOPS * M * N * L * npair
     ms = phases['synth']
     roof = {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': PEAK['fp64_mfma'] / 1e12,
             'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / PEAK['fp64_mfma'],
@@ -548,8 +552,8 @@ def _roofline(res, name, ms, N, n, L, M):
         return {'bound': 'mfma', 'achieved': work / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
                       'unit': 'TFLOP/s', 'frac': work / (ms * 1e-3) / peak, 'traffic': None}
     if name == 'synth':
-        # 2 flops per (point, centre, lm); FP64 vector peak
-        flops = 2.0 * N * M * L
+        # SYNTH_FLOPS per (point, centre, lm); FP64 vector peak (equal to the FP64 MFMA peak)
+        flops = SYNTH_FLOPS * N * M * L
         peak = PEAK['fp64_mfma']
         return {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
                       'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / peak, 'traffic': None}

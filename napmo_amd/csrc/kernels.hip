@@ -139,9 +139,17 @@ __global__ void __launch_bounds__(64 * PSL) k_project(AtomTables at, const doubl
     }
     __syncthreads();
     if (lm < L) {
+      // pairs (j, j + PSL) in order; the Yhat loads of the next two pairs are in flight
+      // while one pair is summed
       int j = slice;
+      const double *Yc = Y + (long)j0 * L + lm;
+      double ya0 = 0, ya1 = 0, yb0 = 0, yb1 = 0;
+      if (j + PSL < nj) ya0 = Yc[(long)j * L], ya1 = Yc[(long)(j + PSL) * L];
+      if (j + 3 * PSL < nj) yb0 = Yc[(long)(j + 2 * PSL) * L], yb1 = Yc[(long)(j + 3 * PSL) * L];
       for (; j + PSL < nj; j += 2 * PSL) {
-        const double y0 = Y[(long)(j0 + j) * L + lm], y1 = Y[(long)(j0 + j + PSL) * L + lm];
+        const double y0 = ya0, y1 = ya1;
+        ya0 = yb0, ya1 = yb1;
+        if (j + 5 * PSL < nj) yb0 = Yc[(long)(j + 4 * PSL) * L], yb1 = Yc[(long)(j + 5 * PSL) * L];
 #pragma unroll
         for (int k = 0; k < PSB; ++k) acc[k] += sf[k][j] * y0 + sf[k][j + PSL] * y1;
       }
@@ -173,17 +181,21 @@ void launch_project(const AtomTables &at, const double *dens, const double *bw, 
 
 // ====================================================== Poisson: radial solves
 // U[A][i][lm] = sum_j G_{A,l}[i][j] rho_lm[A][j][lm] for y and dt; one workgroup per
-// (atom, l): the (nr x 2l+1) block of rho_lm is staged in LDS, each thread forms outputs
-// from a contiguous Green row. green block of an atom: for l = 0..lmax: Gy [nr x nr],
-// then Gd [nr x nr].
+// (atom, l, chunk of 256 outputs): the (nr x 2l+1) block of rho_lm is staged in LDS and
+// each thread forms one output from a contiguous Green row. green block of an atom: for
+// l = 0..lmax: Gy [nr x nr], then Gd [nr x nr].
 constexpr int RMAX_STAGE = 4096;  // doubles of rho_lm staged per (atom, l)
+constexpr int RADIAL_CHUNK = 256;
 
-__global__ void __launch_bounds__(256) k_radial(AtomTables at, const double *__restrict__ rho_lm,
+__global__ void __launch_bounds__(RADIAL_CHUNK) k_radial(AtomTables at, const double *__restrict__ rho_lm,
                                                 const double *__restrict__ green, int L,
                                                 double *__restrict__ Uy, double *__restrict__ Udt) {
   __shared__ double sr[RMAX_STAGE];
   const int A = blockIdx.x, l = blockIdx.y;
   const int nr = at.nrad[A], nm = 2 * l + 1;
+  const int total = 2 * nr * nm;
+  const int o0 = blockIdx.z * RADIAL_CHUNK;
+  if (o0 >= total) return;
   const double *G = green + at.green_off[A] + (long)l * 2 * nr * nr;
   const long roff = at.rad_off[A] * (long)L;
   const bool staged = nr * nm <= RMAX_STAGE;
@@ -191,27 +203,26 @@ __global__ void __launch_bounds__(256) k_radial(AtomTables at, const double *__r
     for (int e = threadIdx.x; e < nr * nm; e += blockDim.x)
       sr[e] = rho_lm[roff + (long)(e / nm) * L + l * l + e % nm];
   __syncthreads();
-  const int total = 2 * nr * nm;
-  for (int o = threadIdx.x; o < total; o += blockDim.x) {
-    const int m = o % nm, row = o / nm;  // row in [0, 2 nr): which * nr + i
-    const int which = row / nr, i = row - which * nr;
-    const double *g = G + (long)which * nr * nr + (long)i * nr;
-    double acc = 0.0;
-    if (staged) {
-      for (int j = 0; j < nr; ++j) acc += g[j] * sr[j * nm + m];
-    } else {
-      const double *r = rho_lm + roff + l * l + m;
-      for (int j = 0; j < nr; ++j) acc += g[j] * r[(long)j * L];
-    }
-    (which ? Udt : Uy)[roff + (long)i * L + l * l + m] = acc;
+  const int o = o0 + threadIdx.x;
+  if (o >= total) return;
+  const int m = o % nm, row = o / nm;  // row in [0, 2 nr): which * nr + i
+  const int which = row / nr, i = row - which * nr;
+  const double *g = G + (long)which * nr * nr + (long)i * nr;
+  double acc = 0.0;
+  if (staged) {
+    for (int j = 0; j < nr; ++j) acc += g[j] * sr[j * nm + m];
+  } else {
+    const double *r = rho_lm + roff + l * l + m;
+    for (int j = 0; j < nr; ++j) acc += g[j] * r[(long)j * L];
   }
+  (which ? Udt : Uy)[roff + (long)i * L + l * l + m] = acc;
 }
 
 void launch_radial(const AtomTables &at, const double *rho_lm, const double *green, int lmax,
                    double *Uy, double *Udt, hipStream_t s) {
   const int L = (lmax + 1) * (lmax + 1);
-  dim3 grid(at.M, lmax + 1);
-  k_radial<<<grid, 256, 0, s>>>(at, rho_lm, green, L, Uy, Udt);
+  dim3 grid(at.M, lmax + 1, (unsigned)div_up(2L * at.max_nrad * (2 * lmax + 1), RADIAL_CHUNK));
+  k_radial<<<grid, RADIAL_CHUNK, 0, s>>>(at, rho_lm, green, L, Uy, Udt);
   NAPMO_HIP(hipGetLastError());
 }
 
@@ -266,33 +277,37 @@ void synth_order(int lmax, std::vector<int> &l, std::vector<int> &m, std::vector
       }
 }
 
+constexpr int SYNTH_TAB_J = 4;  // intervals per workgroup (the factors are formed once for them)
+
 __global__ void k_synth_tables(AtomTables at, const double *__restrict__ Uy,
                                const double *__restrict__ Udt, int lmax, int L,
                                int write_amps, double *__restrict__ T,
                                double *__restrict__ ampL, double *__restrict__ ampR) {
-  const int A = blockIdx.y, j = blockIdx.x;
+  const int A = blockIdx.y, jb = blockIdx.x * SYNTH_TAB_J;
   const int nr = at.nrad[A];
-  if (j >= nr - 1) return;
+  if (jb >= nr - 1) return;
   const long roff = at.rad_off[A];
   for (int q = threadIdx.x; q < L; q += blockDim.x) {
     int l, m, cs;
     q_decode(q, lmax, l, m, cs);
     const int ref = l * l + (m ? 2 * m - 1 + cs : 0);
     const double f = q_factor(l, m) * pi_sigma(l, m);
-    const double y0 = Uy[(roff + j) * L + ref], y1 = Uy[(roff + j + 1) * L + ref];
-    const double d0 = Udt[(roff + j) * L + ref], d1 = Udt[(roff + j + 1) * L + ref];
-    const double zz = y1 - y0;
-    double *t = T + (((long)A * (at.max_nrad - 1) + j) * L + q) * 4;
-    t[0] = y0 * f;
-    t[1] = d0 * f;
-    t[2] = (3 * zz - 2 * d0 - d1) * f;
-    t[3] = (-2 * zz + d0 + d1) * f;
-    if (j == 0 && write_amps) {
-      // PotentialExtrapolation::prepare (src/horton/extrapolation.cpp:156-159)
-      const double r0 = at.r[roff], rn = at.r[roff + nr - 1];
-      const double yn = Uy[(roff + nr - 1) * L + ref];
-      ampL[(long)A * L + q] = y0 / pow(r0, (double)l) * f;
-      ampR[(long)A * L + q] = yn * pow(rn, (double)(l + 1)) * f;
+    for (int j = jb; j < min(jb + SYNTH_TAB_J, nr - 1); ++j) {
+      const double y0 = Uy[(roff + j) * L + ref], y1 = Uy[(roff + j + 1) * L + ref];
+      const double d0 = Udt[(roff + j) * L + ref], d1 = Udt[(roff + j + 1) * L + ref];
+      const double zz = y1 - y0;
+      double *t = T + (((long)A * (at.max_nrad - 1) + j) * L + q) * 4;
+      t[0] = y0 * f;
+      t[1] = d0 * f;
+      t[2] = (3 * zz - 2 * d0 - d1) * f;
+      t[3] = (-2 * zz + d0 + d1) * f;
+      if (j == 0 && write_amps) {
+        // PotentialExtrapolation::prepare (src/horton/extrapolation.cpp:156-159)
+        const double r0 = at.r[roff], rn = at.r[roff + nr - 1];
+        const double yn = Uy[(roff + nr - 1) * L + ref];
+        ampL[(long)A * L + q] = y0 / pow(r0, (double)l) * f;
+        ampR[(long)A * L + q] = yn * pow(rn, (double)(l + 1)) * f;
+      }
     }
   }
 }
@@ -300,7 +315,7 @@ __global__ void k_synth_tables(AtomTables at, const double *__restrict__ Uy,
 void launch_synth_tables(const AtomTables &at, const double *Uy, const double *Udt, int lmax,
                          double *T, double *ampL, double *ampR, hipStream_t s, int write_amps) {
   const int L = (lmax + 1) * (lmax + 1);
-  dim3 grid(at.max_nrad - 1, at.M);
+  dim3 grid((unsigned)div_up(at.max_nrad - 1, SYNTH_TAB_J), at.M);
   k_synth_tables<<<grid, 256, 0, s>>>(at, Uy, Udt, lmax, L, write_amps, T, ampL, ampR);
   NAPMO_HIP(hipGetLastError());
 }
