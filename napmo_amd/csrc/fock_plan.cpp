@@ -223,7 +223,7 @@ void FockPlan::synth_self(int slot, bool with_scale) {
   AtomTables at = grid.tables(pl);
   SynthPairs &sp = grid.synth_pairs();
   mark(slot, 5);
-  launch_synth2(at, grid.x.get(), grid.y.get(), grid.z.get(), grid.N, sp.list.get(),
+  launch_synth2(at, sp.geo.get(), grid.N, sp.list.get(),
                 sp.wave_bucket.get(), sp.nwaves, work.T.get(), work.ampL.get(), work.ampR.get(), lmax,
                 sp.contrib.get(), st, grid.synth_np);
   mark(slot, 6);  // the "scale" phase: centre sum, then w bw v

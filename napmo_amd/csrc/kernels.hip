@@ -7,6 +7,7 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "synth_kernel.hpp"
 
 namespace napmo {
 
@@ -424,22 +425,60 @@ void launch_pair_fill(const AtomTables &at, const double *x, const double *y, co
   NAPMO_HIP(hipGetLastError());
 }
 
+// One wavefront per 64-pair slot: the geometry each synthesis launch would otherwise form
+// per pair (gathers of the point, distance, unit vector and the inverse radial transform),
+// formed once per grid. Slots past a bucket's pairs (list entry -1) get the centre's own
+// first point, so every lane of the synthesis computes on finite values.
+__global__ void __launch_bounds__(64) k_pair_geometry(AtomTables at, const double *__restrict__ px,
+                                                      const double *__restrict__ py,
+                                                      const double *__restrict__ pz,
+                                                      const int *__restrict__ list,
+                                                      const int *__restrict__ wave_bucket,
+                                                      double *__restrict__ geo) {
+  const long w = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int gb = wave_bucket[w];
+  const int A = gb / (at.max_nrad + 1), loc = gb - A * (at.max_nrad + 1);
+  const int nr = at.nrad[A];
+  const int reg = (loc == 0) ? 0 : (loc == nr ? 2 : 1);
+  const int j = (reg == 1) ? loc - 1 : 0;
+  const int p = list[w * 64 + lane];
+  const long pp = p >= 0 ? p : 0;
+  const double cx = at.center[3 * A], cy = at.center[3 * A + 1], cz = at.center[3 * A + 2];
+  const double dx = px[pp] - cx, dy = py[pp] - cy, dz = pz[pp] - cz;
+  const double d = sqrt(dx * dx + dy * dy + dz * dz);
+  const double inv_d = 1.0 / d;
+  double *g = geo + w * 256 + lane;
+  g[0] = (reg == 1) ? rtf_inv(at.rkind[A], at.rp0[A], at.rp1[A], nr, d) - j : d;
+  g[64] = dx * inv_d;
+  g[128] = dy * inv_d;
+  g[192] = dz * inv_d;
+}
+
+void launch_pair_geometry(const AtomTables &at, const double *x, const double *y,
+                          const double *z, const int *list, const int *wave_bucket, long nwaves,
+                          double *geo, hipStream_t s) {
+  if (nwaves <= 0) return;
+  k_pair_geometry<<<dim3((unsigned)nwaves), 64, 0, s>>>(at, x, y, z, list, wave_bucket, geo);
+  NAPMO_HIP(hipGetLastError());
+}
+
 template <int LM>
 static void synth2_dispatch(int lmax, int np, hipStream_t s, const AtomTables &at,
-                            const double *x, const double *y, const double *z, long N,
+                            const double *geo, long N,
                             const int *list, const int *wb, long nw, const double *T,
                             const double *aL, const double *aR, double *contrib) {
   if constexpr (LM > MAX_LMAX) {
     throw Error("lmax above the supported maximum " + std::to_string(MAX_LMAX));
   } else {
     if (lmax == LM)
-      launch_synth2_t<LM>(np, s, at, x, y, z, N, list, wb, nw, T, aL, aR, contrib);
+      launch_synth2_t<LM>(np, s, at, geo, N, list, wb, nw, T, aL, aR, contrib);
     else
-      synth2_dispatch<LM + 1>(lmax, np, s, at, x, y, z, N, list, wb, nw, T, aL, aR, contrib);
+      synth2_dispatch<LM + 1>(lmax, np, s, at, geo, N, list, wb, nw, T, aL, aR, contrib);
   }
 }
 
-void launch_synth2(const AtomTables &at, const double *x, const double *y, const double *z, long N,
+void launch_synth2(const AtomTables &at, const double *geo, long N,
                    const int *list, const int *wave_bucket, long nwaves, const double *T,
                    const double *ampL, const double *ampR, int lmax, double *contrib,
                    hipStream_t s, int np) {
@@ -447,8 +486,7 @@ void launch_synth2(const AtomTables &at, const double *x, const double *y, const
   NAPMO_REQUIRE(np == 1 || np == 2, "synthesis: 1 or 2 pairs per lane");
   NAPMO_REQUIRE(nwaves % (SYNTH_PAD / 64) == 0, "synthesis: pair lists must be padded to 128");
   if (nwaves <= 0) return;
-  synth2_dispatch<0>(lmax, np, s, at, x, y, z, N, list, wave_bucket, nwaves, T, ampL, ampR,
-                     contrib);
+  synth2_dispatch<0>(lmax, np, s, at, geo, N, list, wave_bucket, nwaves, T, ampL, ampR, contrib);
   NAPMO_HIP(hipGetLastError());
 }
 

@@ -86,10 +86,9 @@ void launch_synth_t(dim3 g, dim3 b, hipStream_t s, const AtomTables &at, const d
                     const double *aR, int acc, int ext_mode, double *v);
 
 template <int LMAX>
-void launch_synth2_t(int np, hipStream_t s, const AtomTables &at, const double *x,
-                     const double *y, const double *z, long N, const int *list,
-                     const int *wave_bucket, long nwaves, const double *T, const double *aL,
-                     const double *aR, double *contrib);
+void launch_synth2_t(int np, hipStream_t s, const AtomTables &at, const double *geo, long N,
+                     const int *list, const int *wave_bucket, long nwaves, const double *T,
+                     const double *aL, const double *aR, double *contrib);
 
 // Pair buckets of the bucketed synthesis (per geometry): count[gb] += 1 for every
 // (point, centre); then list[off[gb] + k] = p (slots pre-filled with -1).
@@ -97,10 +96,15 @@ void launch_pair_count(const AtomTables &at, const double *x, const double *y, c
                        long N, int *count, hipStream_t s);
 void launch_pair_fill(const AtomTables &at, const double *x, const double *y, const double *z,
                       long N, const long *off, int *cursor, int *list, hipStream_t s);
+// Per-pair geometry of the bucketed synthesis, geo [nwaves][4][64] in list order: u (interval
+// buckets) or d (extrapolation buckets), then the unit vector (x, y, z) from the centre.
+void launch_pair_geometry(const AtomTables &at, const double *x, const double *y,
+                          const double *z, const int *list, const int *wave_bucket, long nwaves,
+                          double *geo, hipStream_t s);
 // Bucketed synthesis into contrib [M x N], then v[p] (+)= scale * sum_A contrib[A][p].
 // np = (point, centre) pairs per lane, 1 or 2 (the lists are padded to 128 per bucket).
 constexpr int SYNTH_PAD = 128;
-void launch_synth2(const AtomTables &at, const double *x, const double *y, const double *z, long N,
+void launch_synth2(const AtomTables &at, const double *geo, long N,
                    const int *list, const int *wave_bucket, long nwaves, const double *T,
                    const double *ampL, const double *ampR, int lmax, double *contrib,
                    hipStream_t s, int np = 2);

@@ -547,7 +547,7 @@ void DeviceGrid::kinetic(const double *d_psi, int lmax, double *d_out) {
   SynthPairs &sp = synth_pairs();
   launch_synth_tables(at, w.Uy.get(), w.Udt.get(), lmax, w.T.get(), w.ampL.get(), w.ampR.get(),
                       stream);
-  launch_synth2(at, x.get(), y.get(), z.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves,
+  launch_synth2(at, sp.geo.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves,
                 w.T.get(), w.ampL.get(), w.ampR.get(), lmax, sp.contrib.get(), stream, synth_np);
   launch_contrib_reduce(sp.contrib.get(), M, N, 0, d_out, stream, -0.5);
 }
@@ -687,7 +687,7 @@ void DeviceGrid::orbital_correction(int lmax, int norb, const double *d_phi, con
                         wk.T.get(), wk.ampL.get(), wk.ampR.get(), stream);
     toc();
     tic(5);
-    launch_synth2(at, x.get(), y.get(), z.get(), N, sp.list.get(), sp.wave_bucket.get(),
+    launch_synth2(at, sp.geo.get(), N, sp.list.get(), sp.wave_bucket.get(),
                   sp.nwaves, wk.T.get(), wk.ampL.get(), wk.ampR.get(), lmax, sp.contrib.get(), stream, synth_np);
     toc();
     tic(6);
@@ -726,7 +726,7 @@ void DeviceGrid::exchange_potential(int ndim, const double *d_psi, const double 
                           stream);
       toc();
       tic(5);
-      launch_synth2(at, x.get(), y.get(), z.get(), N, sp.list.get(), sp.wave_bucket.get(),
+      launch_synth2(at, sp.geo.get(), N, sp.list.get(), sp.wave_bucket.get(),
                     sp.nwaves, w.T.get(), w.ampL.get(), w.ampR.get(), lmax, sp.contrib.get(), stream,
                     synth_np);
       toc();
@@ -763,7 +763,7 @@ void DeviceGrid::poisson_synthesize(int lmax, double *d_v) {
   SynthPairs &sp = synth_pairs();
   launch_synth_tables(at, pl.work.Uy.get(), pl.work.Udt.get(), lmax, pl.work.T.get(), pl.work.ampL.get(),
                       pl.work.ampR.get(), stream);
-  launch_synth2(at, x.get(), y.get(), z.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves,
+  launch_synth2(at, sp.geo.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves,
                 pl.work.T.get(), pl.work.ampL.get(), pl.work.ampR.get(), lmax, sp.contrib.get(), stream, synth_np);
   launch_contrib_reduce(sp.contrib.get(), M, N, 0, d_v, stream);
 }
@@ -822,6 +822,9 @@ std::unique_ptr<SynthPairs> build_synth_pairs(const DeviceGrid &src, const doubl
   NAPMO_HIP(hipMemsetAsync(cursor.get(), 0, sizeof(int) * nb, stream));
   launch_pair_fill(at, x, y, z, N, doff.get(), cursor.get(), p->list.get(), stream);
   p->wave_bucket.upload(wb, stream);
+  p->geo.resize((size_t)std::max<long>(p->nwaves, 1) * 256);
+  launch_pair_geometry(at, x, y, z, p->list.get(), p->wave_bucket.get(), p->nwaves, p->geo.get(),
+                       stream);
   p->contrib.resize((size_t)M * N);
   NAPMO_HIP(hipStreamSynchronize(stream));
   return p;
@@ -837,7 +840,7 @@ void DeviceGrid::synthesize_at(int lmax, const PoissonWork &w, const SynthPairs 
                                double scale, int accumulate, double *d_v, hipStream_t st) {
   PoissonPlan &pl = plan(lmax);
   AtomTables at = tables(pl);
-  launch_synth2(at, tx, ty, tz, tN, sp.list.get(), sp.wave_bucket.get(), sp.nwaves, w.T.get(),
+  launch_synth2(at, sp.geo.get(), tN, sp.list.get(), sp.wave_bucket.get(), sp.nwaves, w.T.get(),
                 w.ampL.get(), w.ampR.get(), lmax, sp.contrib.get(), st, synth_np);
   launch_contrib_reduce(sp.contrib.get(), M, tN, accumulate, d_v, st, scale);
 }

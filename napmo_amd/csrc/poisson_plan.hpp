@@ -74,6 +74,7 @@ struct PoissonPlan {
 struct SynthPairs {
   long nwaves = 0;
   DeviceBuffer<int> list, wave_bucket;
+  DeviceBuffer<double> geo;  // [nwaves][4][64]: u or d, unit vector (launch_pair_geometry)
   DeviceBuffer<double> contrib;  // [M x N]
 };
 

@@ -23,32 +23,25 @@ template void launch_synth_t<5>(dim3, dim3, hipStream_t, const AtomTables &, con
 template void launch_synth_t<6>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
                                   const double *, const double *, long, const double *,
                                   const double *, const double *, int, int, double *);
-template void launch_synth2_t<0>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<0>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
-template void launch_synth2_t<1>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<1>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
-template void launch_synth2_t<2>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<2>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
-template void launch_synth2_t<3>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<3>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
-template void launch_synth2_t<4>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<4>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
-template void launch_synth2_t<5>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<5>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
-template void launch_synth2_t<6>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<6>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
 }  // namespace napmo

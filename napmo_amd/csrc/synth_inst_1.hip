@@ -17,24 +17,19 @@ template void launch_synth_t<10>(dim3, dim3, hipStream_t, const AtomTables &, co
 template void launch_synth_t<11>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
                                   const double *, const double *, long, const double *,
                                   const double *, const double *, int, int, double *);
-template void launch_synth2_t<7>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<7>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
-template void launch_synth2_t<8>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<8>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
-template void launch_synth2_t<9>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<9>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
-template void launch_synth2_t<10>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<10>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
-template void launch_synth2_t<11>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<11>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
 }  // namespace napmo

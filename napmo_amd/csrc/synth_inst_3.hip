@@ -11,16 +11,13 @@ template void launch_synth_t<16>(dim3, dim3, hipStream_t, const AtomTables &, co
 template void launch_synth_t<17>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
                                   const double *, const double *, long, const double *,
                                   const double *, const double *, int, int, double *);
-template void launch_synth2_t<15>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<15>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
-template void launch_synth2_t<16>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<16>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
-template void launch_synth2_t<17>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<17>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
 }  // namespace napmo

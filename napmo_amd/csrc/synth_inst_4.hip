@@ -8,12 +8,10 @@ template void launch_synth_t<18>(dim3, dim3, hipStream_t, const AtomTables &, co
 template void launch_synth_t<19>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
                                   const double *, const double *, long, const double *,
                                   const double *, const double *, int, int, double *);
-template void launch_synth2_t<18>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<18>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
-template void launch_synth2_t<19>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<19>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
 }  // namespace napmo

@@ -5,8 +5,7 @@ namespace napmo {
 template void launch_synth_t<24>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
                                   const double *, const double *, long, const double *,
                                   const double *, const double *, int, int, double *);
-template void launch_synth2_t<24>(int, hipStream_t, const AtomTables &, const double *,
-                                   const double *, const double *, long, const int *,
+template void launch_synth2_t<24>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
 }  // namespace napmo

@@ -268,9 +268,8 @@ template <int LMAX> struct Synth2Cfg {
 // NP pairs per lane: a wavefront takes NP consecutive 64-pair slots of one bucket (the
 // lists are padded to 64 NP per bucket) and reads each staged coefficient once for all.
 template <int LMAX, int NP>
-__global__ void __launch_bounds__(256, NP == 1 ? 5 : 4) k_synth2n(AtomTables at, const double *__restrict__ px,
-                                                 const double *__restrict__ py,
-                                                 const double *__restrict__ pz, long N,
+__global__ void __launch_bounds__(256, NP == 1 ? 5 : 4) k_synth2n(AtomTables at,
+                                                 const double *__restrict__ geo, long N,
                                                  const int *__restrict__ list,
                                                  const int *__restrict__ wave_bucket, long ngroups,
                                                  const double *__restrict__ T,
@@ -294,18 +293,19 @@ __global__ void __launch_bounds__(256, NP == 1 ? 5 : 4) k_synth2n(AtomTables at,
   }
   __syncthreads();
   if (!active) return;
-  const double cx = at.center[3 * A], cy = at.center[3 * A + 1], cz = at.center[3 * A + 2];
+  // per-pair geometry formed once per grid (launch_pair_geometry): u or d, unit vector
   int p[NP];
   double u[NP], ux[NP], uy[NP], uz[NP], d[NP], inv_d[NP], acc[NP];
 #pragma unroll
   for (int k = 0; k < NP; ++k) {
-    p[k] = list[(gid * NP + k) * 64 + lane];
-    const long pp = p[k] >= 0 ? p[k] : 0;
-    const double dx = px[pp] - cx, dy = py[pp] - cy, dz = pz[pp] - cz;
-    d[k] = sqrt(dx * dx + dy * dy + dz * dz);
-    inv_d[k] = 1.0 / d[k];
-    ux[k] = dx * inv_d[k], uy[k] = dy * inv_d[k], uz[k] = dz * inv_d[k];
-    u[k] = (reg == 1) ? rtf_inv(at.rkind[A], at.rp0[A], at.rp1[A], nr, d[k]) - j : 0.0;
+    const long slot = gid * NP + k;
+    p[k] = list[slot * 64 + lane];
+    const double *g = geo + slot * 256 + lane;
+    const double g0 = g[0];
+    ux[k] = g[64], uy[k] = g[128], uz[k] = g[192];
+    u[k] = (reg == 1) ? g0 : 0.0;
+    d[k] = (reg == 1) ? 0.0 : g0;
+    inv_d[k] = (reg == 2) ? 1.0 / g0 : 0.0;
   }
   if (reg == 1)
     synth_harmonics_np<LMAX, 1, NP>(srow[wave], u, ux, uy, uz, d, inv_d, acc);
@@ -319,27 +319,18 @@ __global__ void __launch_bounds__(256, NP == 1 ? 5 : 4) k_synth2n(AtomTables at,
 }
 
 template <int LMAX>
-void launch_synth2n_t(int np, hipStream_t s, const AtomTables &at, const double *x,
-                      const double *y, const double *z, long N, const int *list,
-                      const int *wave_bucket, long nwaves, const double *T, const double *aL,
-                      const double *aR, double *contrib) {
+void launch_synth2_t(int np, hipStream_t s, const AtomTables &at, const double *geo, long N,
+                     const int *list, const int *wave_bucket, long nwaves, const double *T,
+                     const double *aL, const double *aR, double *contrib) {
   constexpr int WPB = Synth2Cfg<LMAX>::WPB;
   if (np == 2) {
     const long ng = nwaves / 2;
     k_synth2n<LMAX, 2><<<dim3(div_up(ng, WPB)), dim3(64 * WPB), 0, s>>>(
-        at, x, y, z, N, list, wave_bucket, ng, T, aL, aR, contrib);
+        at, geo, N, list, wave_bucket, ng, T, aL, aR, contrib);
   } else {
     k_synth2n<LMAX, 1><<<dim3(div_up(nwaves, WPB)), dim3(64 * WPB), 0, s>>>(
-        at, x, y, z, N, list, wave_bucket, nwaves, T, aL, aR, contrib);
+        at, geo, N, list, wave_bucket, nwaves, T, aL, aR, contrib);
   }
-}
-
-template <int LMAX>
-void launch_synth2_t(int np, hipStream_t s, const AtomTables &at, const double *x,
-                     const double *y, const double *z, long N, const int *list,
-                     const int *wave_bucket, long nwaves, const double *T, const double *aL,
-                     const double *aR, double *contrib) {
-  launch_synth2n_t<LMAX>(np, s, at, x, y, z, N, list, wave_bucket, nwaves, T, aL, aR, contrib);
 }
 
 template <int LMAX>
