@@ -114,10 +114,14 @@ constexpr int PSB = 5, PJ = 256, PSL = 8;
 __global__ void __launch_bounds__(64 * PSL) k_project(AtomTables at, const double *__restrict__ dens,
                                                       const double *__restrict__ bw,
                                                       const double *__restrict__ yhat, int L,
-                                                      double *__restrict__ rho_lm) {
+                                                      double *__restrict__ rho_lm, long dstride,
+                                                      long ostride) {
   __shared__ double sf[PSB][PJ];
   __shared__ double red[PSL][PSB][64];
-  const int A = blockIdx.z;
+  // blockIdx.z = o M + A: density o (dens + o dstride -> rho_lm + o ostride), atom A
+  const int A = blockIdx.z % at.M, o = blockIdx.z / at.M;
+  dens += o * dstride;
+  rho_lm += o * ostride;
   const int i0 = blockIdx.x * PSB;
   const int nr = at.nrad[A];
   if (i0 >= nr) return;
@@ -173,10 +177,11 @@ __global__ void __launch_bounds__(64 * PSL) k_project(AtomTables at, const doubl
 }
 
 void launch_project(const AtomTables &at, const double *dens, const double *bw, const double *yhat,
-                    int lmax, double *rho_lm, hipStream_t s) {
+                    int lmax, double *rho_lm, hipStream_t s, int nset, long dstride, long ostride) {
   const int L = (lmax + 1) * (lmax + 1);
-  dim3 grid(div_up(at.max_nrad, PSB), div_up(L, 64), at.M);
-  k_project<<<grid, 64 * PSL, 0, s>>>(at, dens, bw, yhat, L, rho_lm);
+  NAPMO_REQUIRE(nset >= 1 && (long)nset * at.M <= 65535, "projection: too many density sets");
+  dim3 grid(div_up(at.max_nrad, PSB), div_up(L, 64), nset * at.M);
+  k_project<<<grid, 64 * PSL, 0, s>>>(at, dens, bw, yhat, L, rho_lm, dstride, ostride);
   NAPMO_HIP(hipGetLastError());
 }
 
@@ -190,12 +195,16 @@ constexpr int RADIAL_CHUNK = 256;
 
 __global__ void __launch_bounds__(RADIAL_CHUNK) k_radial(AtomTables at, const double *__restrict__ rho_lm,
                                                 const double *__restrict__ green, int L,
-                                                double *__restrict__ Uy, double *__restrict__ Udt) {
+                                                double *__restrict__ Uy, double *__restrict__ Udt,
+                                                int nchunk, long stride) {
   __shared__ double sr[RMAX_STAGE];
   const int A = blockIdx.x, l = blockIdx.y;
+  // blockIdx.z = set * nchunk + chunk: set k reads rho_lm + k stride, writes U + k stride
+  const long set = blockIdx.z / nchunk;
+  rho_lm += set * stride, Uy += set * stride, Udt += set * stride;
   const int nr = at.nrad[A], nm = 2 * l + 1;
   const int total = 2 * nr * nm;
-  const int o0 = blockIdx.z * RADIAL_CHUNK;
+  const int o0 = (blockIdx.z % nchunk) * RADIAL_CHUNK;
   if (o0 >= total) return;
   const double *G = green + at.green_off[A] + (long)l * 2 * nr * nr;
   const long roff = at.rad_off[A] * (long)L;
@@ -220,10 +229,12 @@ __global__ void __launch_bounds__(RADIAL_CHUNK) k_radial(AtomTables at, const do
 }
 
 void launch_radial(const AtomTables &at, const double *rho_lm, const double *green, int lmax,
-                   double *Uy, double *Udt, hipStream_t s) {
+                   double *Uy, double *Udt, hipStream_t s, int nset, long stride) {
   const int L = (lmax + 1) * (lmax + 1);
-  dim3 grid(at.M, lmax + 1, (unsigned)div_up(2L * at.max_nrad * (2 * lmax + 1), RADIAL_CHUNK));
-  k_radial<<<grid, RADIAL_CHUNK, 0, s>>>(at, rho_lm, green, L, Uy, Udt);
+  const int nchunk = (int)div_up(2L * at.max_nrad * (2 * lmax + 1), RADIAL_CHUNK);
+  NAPMO_REQUIRE(nset >= 1 && (long)nset * nchunk <= 65535, "radial solves: too many sets");
+  dim3 grid(at.M, lmax + 1, (unsigned)(nset * nchunk));
+  k_radial<<<grid, RADIAL_CHUNK, 0, s>>>(at, rho_lm, green, L, Uy, Udt, nchunk, stride);
   NAPMO_HIP(hipGetLastError());
 }
 
@@ -283,8 +294,13 @@ constexpr int SYNTH_TAB_J = 4;  // intervals per workgroup (the factors are form
 __global__ void k_synth_tables(AtomTables at, const double *__restrict__ Uy,
                                const double *__restrict__ Udt, int lmax, int L,
                                int write_amps, double *__restrict__ T,
-                               double *__restrict__ ampL, double *__restrict__ ampR) {
+                               double *__restrict__ ampL, double *__restrict__ ampR,
+                               long ustride, long tstride, long astride) {
   const int A = blockIdx.y, jb = blockIdx.x * SYNTH_TAB_J;
+  // blockIdx.z = set: U + set ustride -> T + set tstride, amplitudes + set astride
+  const long set = blockIdx.z;
+  Uy += set * ustride, Udt += set * ustride, T += set * tstride;
+  ampL += set * astride, ampR += set * astride;
   const int nr = at.nrad[A];
   if (jb >= nr - 1) return;
   const long roff = at.rad_off[A];
@@ -313,11 +329,19 @@ __global__ void k_synth_tables(AtomTables at, const double *__restrict__ Uy,
   }
 }
 
+long synth_table_size(const AtomTables &at, int lmax) {
+  return (long)at.M * std::max(at.max_nrad - 1, 1) * (lmax + 1) * (lmax + 1) * 4;
+}
+
 void launch_synth_tables(const AtomTables &at, const double *Uy, const double *Udt, int lmax,
-                         double *T, double *ampL, double *ampR, hipStream_t s, int write_amps) {
+                         double *T, double *ampL, double *ampR, hipStream_t s, int write_amps,
+                         int nset, long ustride) {
   const int L = (lmax + 1) * (lmax + 1);
-  dim3 grid((unsigned)div_up(at.max_nrad - 1, SYNTH_TAB_J), at.M);
-  k_synth_tables<<<grid, 256, 0, s>>>(at, Uy, Udt, lmax, L, write_amps, T, ampL, ampR);
+  NAPMO_REQUIRE(nset >= 1 && nset <= 65535, "synthesis tables: too many sets");
+  dim3 grid((unsigned)div_up(at.max_nrad - 1, SYNTH_TAB_J), at.M, nset);
+  const long tstride = synth_table_size(at, lmax), astride = (long)at.M * L;
+  k_synth_tables<<<grid, 256, 0, s>>>(at, Uy, Udt, lmax, L, write_amps, T, ampL, ampR, ustride,
+                                      tstride, astride);
   NAPMO_HIP(hipGetLastError());
 }
 

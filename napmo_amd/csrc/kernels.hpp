@@ -60,16 +60,23 @@ struct AtomTables {
 };
 
 // rho_lm[A][i][lm] = sum_j f_Aij Yhat[j][lm], f = dens * bw  (spherical expansion).
+// nset densities at once: density k at dens + k dstride, its projections at rho_lm + k ostride.
 void launch_project(const AtomTables &at, const double *dens, const double *bw, const double *yhat,
-                    int lmax, double *rho_lm, hipStream_t s);
+                    int lmax, double *rho_lm, hipStream_t s, int nset = 1, long dstride = 0,
+                    long ostride = 0);
 // U[A] (y and dt at the radial nodes, lm reference order) = G_{A,l} rho_lm[A].
+// nset sets at once: set k reads rho_lm + k stride and writes Uy, Udt + k stride.
 void launch_radial(const AtomTables &at, const double *rho_lm, const double *green, int lmax,
-                   double *Uy, double *Udt, hipStream_t s);
+                   double *Uy, double *Udt, hipStream_t s, int nset = 1, long stride = 0);
 // Synthesis tables: T[A][j][q][4] Horner coefficients in synthesis order q, amp_left /
 // amp_right [A][q], all scaled by sqrt(2l+1)/sqrt(4 pi).
 // write_amps = 0 keeps ampL / ampR as provided (non-potential extrapolations).
+// nset sets at once: set k reads Uy, Udt + k ustride and writes T + k synth_table_size,
+// ampL, ampR + k M L.
+long synth_table_size(const AtomTables &at, int lmax);
 void launch_synth_tables(const AtomTables &at, const double *Uy, const double *Udt, int lmax,
-                         double *T, double *ampL, double *ampR, hipStream_t s, int write_amps = 1);
+                         double *T, double *ampL, double *ampR, hipStream_t s, int write_amps = 1,
+                         int nset = 1, long ustride = 0);
 // Synthesis-order index q -> (l, m, cos/sin) and its coefficient scale (host copy).
 void synth_order(int lmax, std::vector<int> &l, std::vector<int> &m, std::vector<int> &cs,
                  std::vector<double> &scale);

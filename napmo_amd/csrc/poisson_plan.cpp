@@ -670,9 +670,9 @@ void DeviceGrid::orbital_correction(int lmax, int norb, const double *d_phi, con
   // sources (phi doi - ri) bw projected per atom, V_sph = spherical average of V bw
   tic(1);
   launch_scaled_diff(d_phi, d_doi, d_ri, norb, N, ow.src.get(), stream);
-  for (int o = 0; o < norb; ++o)
-    launch_project(at, ow.src.get() + (long)o * N, bw.get(), pl.yhat.get(), lmax,
-                   ow.mu.get() + (long)o * S * L, stream);
+  for (int o0 = 0; o0 < norb; o0 += SET_BATCH)
+    launch_project(at, ow.src.get() + (long)o0 * N, bw.get(), pl.yhat.get(), lmax,
+                   ow.mu.get() + (long)o0 * S * L, stream, std::min(SET_BATCH, norb - o0), N, S * L);
   toc();
   tic(2);
   launch_shell_average(at, d_V, norb, N, w.get(), bw.get(), S, ow.Vsph.get(), stream);
@@ -681,18 +681,28 @@ void DeviceGrid::orbital_correction(int lmax, int norb, const double *d_phi, con
   launch_helmholtz_radial(at, hp.tables(), lmax, norb, S, ow.Vsph.get(), d_oi, d_doi, mass_inv,
                           ow.mu.get(), ow.Uy.get(), ow.Udt.get(), ow.status.get(), stream);
   toc();
-  for (int o = 0; o < norb; ++o) {
+  // synthesis tables of up to SET_BATCH orbitals per launch, then one synthesis per orbital
+  const long tsz = synth_table_size(at, lmax);
+  const int nb = std::min(SET_BATCH, norb);
+  ow.T.resize((size_t)nb * tsz);
+  ow.ampL.resize((size_t)nb * M * L);
+  ow.ampR.resize((size_t)nb * M * L);
+  for (int o0 = 0; o0 < norb; o0 += SET_BATCH) {
+    const int cnt = std::min(SET_BATCH, norb - o0);
     tic(4);
-    launch_synth_tables(at, ow.Uy.get() + (long)o * S * L, ow.Udt.get() + (long)o * S * L, lmax,
-                        wk.T.get(), wk.ampL.get(), wk.ampR.get(), stream);
+    launch_synth_tables(at, ow.Uy.get() + (long)o0 * S * L, ow.Udt.get() + (long)o0 * S * L, lmax,
+                        ow.T.get(), ow.ampL.get(), ow.ampR.get(), stream, 1, cnt, S * L);
     toc();
-    tic(5);
-    launch_synth2(at, sp.geo.get(), N, sp.list.get(), sp.wave_bucket.get(),
-                  sp.nwaves, wk.T.get(), wk.ampL.get(), wk.ampR.get(), lmax, sp.contrib.get(), stream, synth_np);
-    toc();
-    tic(6);
-    launch_contrib_reduce(sp.contrib.get(), M, N, 0, d_out + (long)o * N, stream);
-    toc();
+    for (int k = 0; k < cnt; ++k) {
+      tic(5);
+      launch_synth2(at, sp.geo.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves,
+                    ow.T.get() + k * tsz, ow.ampL.get() + (long)k * M * L,
+                    ow.ampR.get() + (long)k * M * L, lmax, sp.contrib.get(), stream, synth_np);
+      toc();
+      tic(6);
+      launch_contrib_reduce(sp.contrib.get(), M, N, 0, d_out + (long)(o0 + k) * N, stream);
+      toc();
+    }
   }
   int h_status = 0;
   NAPMO_HIP(hipMemcpyAsync(&h_status, ow.status.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -706,35 +716,55 @@ void DeviceGrid::exchange_potential(int ndim, const double *d_psi, const double 
   PoissonPlan &pl = plan(lmax);
   AtomTables at = tables(pl);
   SynthPairs &sp = synth_pairs();
-  PoissonWork &w = pl.work;
+  // orbital pairs (s, v), v <= s, in order, SET_BATCH at a time: pair densities, their
+  // projections, radial solves and synthesis tables in one launch each, then per pair the
+  // synthesis, centre sum and accumulation into K
+  std::vector<std::pair<int, int>> prs;
+  for (int s = 0; s < ndim; ++s)
+    for (int v = 0; v <= s; ++v) prs.push_back({s, v});
+  const int npair = (int)prs.size(), nb = std::min(SET_BATCH, npair);
+  const int L = (lmax + 1) * (lmax + 1);
+  long S = 0;
+  for (auto &a : atoms) S += a.nrad;
+  const long tsz = synth_table_size(at, lmax);
   xchi.resize((size_t)ndim * N);
-  xdens.resize(N);
+  xdens.resize((size_t)nb * N);
   xaux.resize(N);
+  xrho.resize((size_t)nb * S * L);
+  xUy.resize((size_t)nb * S * L);
+  xUdt.resize((size_t)nb * S * L);
+  xT.resize((size_t)nb * tsz);
+  xampL.resize((size_t)nb * M * L);
+  xampR.resize((size_t)nb * M * L);
   if (prof_on) ++prof_calls;
   launch_exchange_chi(d_psi, d_D, ndim, N, xchi.get(), stream);
   launch_axpby(0.0, xchi.get(), 0.0, xchi.get(), (long)ndim * N, d_K, stream);  // K = 0
-  for (int s = 0; s < ndim; ++s)
-    for (int v = 0; v <= s; ++v) {
-      // V[psi_s psi_v]: projection and radial Green solves, tables, synthesis, centre sum
-      tic(1);
-      launch_mul(d_psi + (long)s * N, d_psi + (long)v * N, N, xdens.get(), stream);
-      launch_project(at, xdens.get(), bw.get(), pl.yhat.get(), lmax, w.rho_lm.get(), stream);
-      launch_radial(at, w.rho_lm.get(), pl.green.get(), lmax, w.Uy.get(), w.Udt.get(), stream);
-      toc();
-      tic(4);
-      launch_synth_tables(at, w.Uy.get(), w.Udt.get(), lmax, w.T.get(), w.ampL.get(), w.ampR.get(),
-                          stream);
-      toc();
+  for (int c0 = 0; c0 < npair; c0 += SET_BATCH) {
+    const int cnt = std::min(SET_BATCH, npair - c0);
+    tic(1);
+    for (int k = 0; k < cnt; ++k)
+      launch_mul(d_psi + (long)prs[c0 + k].first * N, d_psi + (long)prs[c0 + k].second * N, N,
+                 xdens.get() + (long)k * N, stream);
+    launch_project(at, xdens.get(), bw.get(), pl.yhat.get(), lmax, xrho.get(), stream, cnt, N, S * L);
+    launch_radial(at, xrho.get(), pl.green.get(), lmax, xUy.get(), xUdt.get(), stream, cnt, S * L);
+    toc();
+    tic(4);
+    launch_synth_tables(at, xUy.get(), xUdt.get(), lmax, xT.get(), xampL.get(), xampR.get(), stream,
+                        1, cnt, S * L);
+    toc();
+    for (int k = 0; k < cnt; ++k) {
       tic(5);
-      launch_synth2(at, sp.geo.get(), N, sp.list.get(), sp.wave_bucket.get(),
-                    sp.nwaves, w.T.get(), w.ampL.get(), w.ampR.get(), lmax, sp.contrib.get(), stream,
-                    synth_np);
+      launch_synth2(at, sp.geo.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves,
+                    xT.get() + k * tsz, xampL.get() + (long)k * M * L, xampR.get() + (long)k * M * L,
+                    lmax, sp.contrib.get(), stream, synth_np);
       toc();
       tic(6);
       launch_contrib_reduce(sp.contrib.get(), M, N, 0, xaux.get(), stream);
-      launch_exchange_acc(xaux.get(), xchi.get(), s, v, N, d_K, stream);
+      launch_exchange_acc(xaux.get(), xchi.get(), prs[c0 + k].first, prs[c0 + k].second, N, d_K,
+                          stream);
       toc();
     }
+  }
 }
 
 void DeviceGrid::becke_dickson_step(int lmax, int norb, const double *d_psi, const double *d_T,

@@ -70,6 +70,10 @@ struct PoissonPlan {
   PoissonWork work;
 };
 
+// Densities (orbitals, orbital pairs) whose projections, radial solves and synthesis tables
+// go into one launch each in the orbital update and the exchange build.
+constexpr int SET_BATCH = 64;
+
 // Per-geometry pair buckets of the bucketed synthesis (see launch_synth2).
 struct SynthPairs {
   long nwaves = 0;
@@ -89,6 +93,7 @@ struct HelmholtzPlan {
 // change): sources, projections mu, solutions (y, dt), shell averages, synthesis tables.
 struct OrbitalWork {
   DeviceBuffer<double> src, mu, Uy, Udt, Vsph, resid;
+  DeviceBuffer<double> T, ampL, ampR;  // synthesis tables of up to SET_BATCH orbitals
   DeviceBuffer<int> status;
   PoissonWork pw;
   int lmax = -1;
@@ -165,6 +170,7 @@ struct DeviceGrid {
   // Poisson solves of orbital products.
   void exchange_potential(int ndim, const double *d_psi, const double *d_D, int lmax, double *d_K);
   DeviceBuffer<double> xchi, xdens, xaux;  // its workspace, kept between calls
+  DeviceBuffer<double> xrho, xUy, xUdt, xT, xampL, xampR;  // per batch of orbital pairs
   // Residual R = T + V - e psi and de = integrate(R psi) of norb orbitals, then their
   // corrections (napmo/wavefunction/psi_optimization.py:175-203).
   void becke_dickson_step(int lmax, int norb, const double *d_psi, const double *d_T,
