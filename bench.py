@@ -521,8 +521,11 @@ def measured_traffic(res, phase):
         want = want[0] if res['prec'] == 'bf16x3' else want[1]
     names = [k for k in kern if want in k]
     if phase == 'density':
+        # orbital form: the streamed kernel (C resident in LDS) when the profile has it
+        stream = [k for k in kern if 'k_density_orb_stream' in k]
         tag = ', 1, ' if res['form'] == 'C' else ', 0, '
-        names = [k for k in names if tag in k] or names
+        names = (stream if res['form'] == 'C' and stream else
+                 [k for k in names if tag in k] or names)
     for k in names:
         r, w = kern[k].get('read_bytes'), kern[k].get('write_bytes')
         if r is not None and w is not None:
