@@ -174,14 +174,16 @@ __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__
 // row tiles with a stride of all waves in the grid; two groups are in flight per wave.
 template <int NB, int RB, int WAVES, int PF>
 __global__ void __launch_bounds__(64 * WAVES) k_density_orb_stream(
-    const double *__restrict__ phi, int ld, long Npad, const double *__restrict__ C, int ldc,
-    int kend, double scale, long nvalid, long ntiles, double *__restrict__ out) {
+    const double *__restrict__ phi, int ld, long Npad, const double *__restrict__ C, int nrow,
+    int ncol, int kend, double scale, long nvalid, long ntiles, double *__restrict__ out) {
   extern __shared__ d4 Cf[];  // [G][NB][64] vectors of 4 steps
   const int G = kend / 16;
+  // C [nrow x ncol] row-major as given; entries past it are the zero padding
   for (int e = threadIdx.x; e < G * NB * 256; e += 64 * WAVES) {
     const int st = e & 3, ln = (e >> 2) & 63, gj = e >> 8;
     const int j = gj % NB, g = gj / NB;
-    ((double *)Cf)[e] = C[(long)(16 * g + 4 * (ln >> 4) + st) * ldc + 16 * j + (ln & 15)];
+    const int k = 16 * g + 4 * (ln >> 4) + st, c = 16 * j + (ln & 15);
+    ((double *)Cf)[e] = (k < nrow && c < ncol) ? C[(long)k * ncol + c] : 0.0;
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, kr = lane >> 4, mi = lane & 15;
@@ -249,8 +251,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_density_orb_stream(
 }
 
 template <int NB, int RB, int WAVES, int PF>
-void launch_orb_stream_nb(const double *phi, int ld, long Npad, const double *C, int ldc, int kend,
-                          double scale, long nvalid, double *out, hipStream_t s) {
+void launch_orb_stream_nb(const double *phi, int ld, long Npad, const double *C, int nrow, int ncol,
+                          int kend, double scale, long nvalid, double *out, hipStream_t s) {
   constexpr int STREAM_RB = RB, STREAM_WAVES = WAVES;
   const size_t bytes = (size_t)kend * 16 * NB * sizeof(double);
   const long ntiles = div_up(Npad, 16L * STREAM_RB);
@@ -268,8 +270,8 @@ void launch_orb_stream_nb(const double *phi, int ld, long Npad, const double *C,
   NAPMO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   NAPMO_REQUIRE(occ >= 1, "density_orb_stream: no occupancy");
   const long blocks = std::min<long>((long)occ * cus, div_up(ntiles, STREAM_WAVES));
-  kern<<<dim3((unsigned)blocks), dim3(64 * STREAM_WAVES), bytes, s>>>(phi, ld, Npad, C, ldc, kend,
-                                                                       scale, nvalid, ntiles, out);
+  kern<<<dim3((unsigned)blocks), dim3(64 * STREAM_WAVES), bytes, s>>>(phi, ld, Npad, C, nrow, ncol,
+                                                                       kend, scale, nvalid, ntiles, out);
   NAPMO_HIP(hipGetLastError());
 }
 
@@ -345,29 +347,32 @@ bool density_orb_stream_supported(int ld, int ldc, int nreal) {
 }
 
 template <int RB, int WAVES, int PF>
-static void orb_stream_shape(int nb, const double *phi, int ld, long Npad, const double *C, int ldc,
-                             int kend, double scale, long nvalid, double *out, hipStream_t s) {
+static void orb_stream_shape(int nb, const double *phi, int ld, long Npad, const double *C, int nrow,
+                             int ncol, int kend, double scale, long nvalid, double *out,
+                             hipStream_t s) {
   switch (nb) {
-    case 1: launch_orb_stream_nb<1, RB, WAVES, PF>(phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
-    case 2: launch_orb_stream_nb<2, RB, WAVES, PF>(phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
-    case 3: launch_orb_stream_nb<3, RB, WAVES, PF>(phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
-    default: launch_orb_stream_nb<4, RB, WAVES, PF>(phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
+    case 1: launch_orb_stream_nb<1, RB, WAVES, PF>(phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, s); break;
+    case 2: launch_orb_stream_nb<2, RB, WAVES, PF>(phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, s); break;
+    case 3: launch_orb_stream_nb<3, RB, WAVES, PF>(phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, s); break;
+    default: launch_orb_stream_nb<4, RB, WAVES, PF>(phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, s); break;
   }
 }
 
-void launch_density_orb_stream(const double *phi, int ld, long Npad, const double *C, int ldc,
-                               double *out, hipStream_t s, int nreal, double scale, long nvalid,
+void launch_density_orb_stream(const double *phi, int ld, long Npad, const double *C, int nrow,
+                               int ncol, double *out, hipStream_t s, double scale, long nvalid,
                                int shape) {
-  NAPMO_REQUIRE(density_orb_stream_supported(ld, ldc, nreal) && Npad % 16 == 0,
+  const int ldc = density_orb_cols(ncol);
+  NAPMO_REQUIRE(nrow >= 1 && nrow <= ld && ncol >= 1, "density_orb_stream: bad C shape");
+  NAPMO_REQUIRE(density_orb_stream_supported(ld, ldc, nrow) && Npad % 16 == 0,
                 "density_orb_stream: C does not fit in LDS or bad padding");
-  const int kend = (nreal > 0 && nreal < ld) ? (int)round_up(nreal, 16) : ld;
+  const int kend = (int)std::min<long>(ld, round_up(nrow, 16));
   if (nvalid < 0) nvalid = Npad;
   const int nb = ldc / 16;
   // shapes: rows per wave (16 RB), waves per workgroup, K groups in flight
   switch (shape) {
-    case 2: orb_stream_shape<3, 8, 2>(nb, phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
-    case 3: orb_stream_shape<3, 8, 3>(nb, phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
-    default: orb_stream_shape<2, 12, 3>(nb, phi, ld, Npad, C, ldc, kend, scale, nvalid, out, s); break;
+    case 2: orb_stream_shape<3, 8, 2>(nb, phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, s); break;
+    case 3: orb_stream_shape<3, 8, 3>(nb, phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, s); break;
+    default: orb_stream_shape<2, 12, 3>(nb, phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, s); break;
   }
 }
 

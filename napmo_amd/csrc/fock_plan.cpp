@@ -180,18 +180,23 @@ void FockPlan::density_C(int slot, const double *dC, int nocc, double eta) {
   NAPMO_REQUIRE(nocc > 0 && nocc <= n, "orbital count out of range");
   hipStream_t st = stream;
   const int ldc = density_orb_cols(nocc), nct = density_orb_tiles(ldc);
-  Cpad.resize((size_t)ld * ldc);
   Pfull.resize((size_t)n * n);
+  if (nct == 1 && density_stream && density_orb_stream_supported(ld, ldc, n)) {
+    // C resident in LDS (read as given, no padded copy), Phi streamed; the epilogue writes
+    // rho = eta sum_i X_pi^2 itself. P = eta C C^T feeds the energy trace only.
+    launch_gram(dC, n, nocc, eta, Pfull.get(), st);
+    mark(slot, 0);
+    launch_density_orb_stream(phi64.get(), ld, Npad, dC, n, nocc, rho.get(), st, eta, grid.N,
+                              density_stream);
+    mark(slot, 1);
+    return;
+  }
+  Cpad.resize((size_t)ld * ldc);
   if (rho_part.size() < (size_t)nct * Npad) rho_part.resize((size_t)nct * Npad);
   launch_pad2d(dC, n, nocc, Cpad.get(), ld, ldc, st);
   launch_gram(dC, n, nocc, eta, Pfull.get(), st);
   mark(slot, 0);
-  if (nct == 1 && density_stream && density_orb_stream_supported(ld, ldc, n)) {
-    // C resident in LDS, Phi streamed; the epilogue writes rho = eta sum_i X_pi^2 itself
-    launch_density_orb_stream(phi64.get(), ld, Npad, Cpad.get(), ldc, rho.get(), st, n, eta, grid.N,
-                              density_stream);
-    mark(slot, 1);
-  } else if (nct == 1) {
+  if (nct == 1) {
     // one column tile: the kernel's epilogue writes rho = eta sum_i X_pi^2 itself
     launch_density_orb_f64(phi64.get(), ld, Npad, Cpad.get(), ldc, rho.get(), st, density_ks,
                            density_pf, n, eta, grid.N);

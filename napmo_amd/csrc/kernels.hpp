@@ -257,13 +257,13 @@ int density_orb_tiles(int ldc);
 void launch_density_orb_f64(const double *phi, int ld, long Npad, const double *C, int ldc,
                             double *partial, hipStream_t s, int ks = 16, int pf = 1, int nreal = 0,
                             double scale = 1.0, long nvalid = -1);
-// Orbital form with all of C [ld x ldc] (ldc <= 64, rows from round_up(nreal, 16) on zero)
-// resident in LDS, Phi streamed into MFMA operands: out[p] = scale sum_c X_pc^2 for
-// p < nvalid (nvalid < 0: Npad). shape 1..3: (rows per wave, waves per workgroup, K groups
-// in flight) = (32, 12, 3), (48, 8, 2), (48, 8, 3).
+// Orbital form with all of C [nrow x ncol] (row-major as given, ncol <= 64, zero-padded to
+// 16-column blocks in LDS) resident in LDS, Phi streamed into MFMA operands:
+// out[p] = scale sum_c X_pc^2 for p < nvalid (nvalid < 0: Npad). shape 1..3: (rows per
+// wave, waves per workgroup, K groups in flight) = (32, 12, 3), (48, 8, 2), (48, 8, 3).
 bool density_orb_stream_supported(int ld, int ldc, int nreal);
-void launch_density_orb_stream(const double *phi, int ld, long Npad, const double *C, int ldc,
-                               double *out, hipStream_t s, int nreal, double scale, long nvalid,
+void launch_density_orb_stream(const double *phi, int ld, long Npad, const double *C, int nrow,
+                               int ncol, double *out, hipStream_t s, double scale, long nvalid,
                                int shape = 1);
 void launch_gram(const double *C, int n, int nocc, double eta, double *P, hipStream_t s);
 void launch_pad2d(const double *src, int rows, int cols, double *dst, int drows, int dcols,

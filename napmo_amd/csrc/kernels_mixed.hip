@@ -492,18 +492,34 @@ void launch_fock_syrk_pass(const FockShape &sh, const float *phi32, const double
 }
 
 // P[i][j] = eta sum_k C[i][k] C[j][k] (C [n x nocc] row-major), P [n x n] row-major.
-__global__ void k_gram(const double *__restrict__ C, int n, int nocc, double eta,
-                       double *__restrict__ P) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long)n * n) return;
-  const int i = (int)(e / n), j = (int)(e % n);
+// One workgroup per 16 x 16 tile of P: the tile's 16 + 16 rows of C go through LDS in chunks
+// of GRAM_K columns; each thread sums its element over k in order.
+constexpr int GRAM_K = 64;
+
+__global__ void __launch_bounds__(256) k_gram(const double *__restrict__ C, int n, int nocc,
+                                              double eta, double *__restrict__ P) {
+  __shared__ double ci[16][GRAM_K + 1], cj[16][GRAM_K + 1];
+  const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+  const int i0 = blockIdx.y * 16, j0 = blockIdx.x * 16;
+  const int i = i0 + ti, j = j0 + tj;
   double s = 0.0;
-  for (int k = 0; k < nocc; ++k) s += C[(long)i * nocc + k] * C[(long)j * nocc + k];
-  P[e] = eta * s;
+  for (int k0 = 0; k0 < nocc; k0 += GRAM_K) {
+    const int kc = min(GRAM_K, nocc - k0);
+    for (int e = threadIdx.x; e < 16 * GRAM_K; e += 256) {
+      const int r = e / GRAM_K, k = e % GRAM_K;
+      ci[r][k] = (i0 + r < n && k < kc) ? C[(long)(i0 + r) * nocc + k0 + k] : 0.0;
+      cj[r][k] = (j0 + r < n && k < kc) ? C[(long)(j0 + r) * nocc + k0 + k] : 0.0;
+    }
+    __syncthreads();
+    for (int k = 0; k < kc; ++k) s += ci[ti][k] * cj[tj][k];
+    __syncthreads();
+  }
+  if (i < n && j < n) P[(long)i * n + j] = eta * s;
 }
 
 void launch_gram(const double *C, int n, int nocc, double eta, double *P, hipStream_t s) {
-  k_gram<<<div_up((long)n * n, 256), 256, 0, s>>>(C, n, nocc, eta, P);
+  if (n <= 0) return;
+  k_gram<<<dim3((unsigned)div_up(n, 16), (unsigned)div_up(n, 16)), 256, 0, s>>>(C, n, nocc, eta, P);
   NAPMO_HIP(hipGetLastError());
 }
 
