@@ -278,7 +278,7 @@ __global__ void __launch_bounds__(256, NP == 1 ? 5 : 4) k_synth2n(AtomTables at,
                                                  double *__restrict__ contrib) {
   constexpr int L = Synth2Cfg<LMAX>::L;
   constexpr int WPB = Synth2Cfg<LMAX>::WPB;
-  __shared__ double srow[WPB][L * 4];
+  __shared__ __align__(16) double srow[WPB][L * 4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const long gid = (long)blockIdx.x * WPB + wave;
   const bool active = gid < ngroups;
@@ -287,25 +287,40 @@ __global__ void __launch_bounds__(256, NP == 1 ? 5 : 4) k_synth2n(AtomTables at,
   const int nr = at.nrad[A];
   const int reg = (loc == 0) ? 0 : (loc == nr ? 2 : 1);
   const int j = (reg == 1) ? loc - 1 : 0;
-  if (active && reg == 1) {
-    const double *row = T + ((long)A * (at.max_nrad - 1) + j) * (long)L * 4;
-    for (int e = lane; e < L * 4; e += 64) srow[wave][e] = row[e];
-  }
-  __syncthreads();
   if (!active) return;
-  // per-pair geometry formed once per grid (launch_pair_geometry): u or d, unit vector
+  // per-pair geometry formed once per grid (launch_pair_geometry): u or d, unit vector; its
+  // loads and the spline row's are all issued before any of them is waited for
   int p[NP];
-  double u[NP], ux[NP], uy[NP], uz[NP], d[NP], inv_d[NP], acc[NP];
+  double g0[NP], ux[NP], uy[NP], uz[NP];
 #pragma unroll
   for (int k = 0; k < NP; ++k) {
     const long slot = gid * NP + k;
     p[k] = list[slot * 64 + lane];
     const double *g = geo + slot * 256 + lane;
-    const double g0 = g[0];
-    ux[k] = g[64], uy[k] = g[128], uz[k] = g[192];
-    u[k] = (reg == 1) ? g0 : 0.0;
-    d[k] = (reg == 1) ? 0.0 : g0;
-    inv_d[k] = (reg == 2) ? 1.0 / g0 : 0.0;
+    g0[k] = g[0], ux[k] = g[64], uy[k] = g[128], uz[k] = g[192];
+  }
+  // each wave stages its own row into its own LDS slice (16-byte loads, all in flight), so
+  // a wavefront-scope sync suffices and the waves of a workgroup start independently
+  if (reg == 1) {
+    constexpr int NV = L * 2;  // double2 per row
+    constexpr int NLD = (NV + 63) / 64;
+    const double2 *row = (const double2 *)(T + ((long)A * (at.max_nrad - 1) + j) * (long)L * 4);
+    double2 rv[NLD];
+#pragma unroll
+    for (int t = 0; t < NLD; ++t) rv[t] = row[min(lane + 64 * t, NV - 1)];
+#pragma unroll
+    for (int t = 0; t < NLD; ++t)
+      if (lane + 64 * t < NV) ((double2 *)srow[wave])[lane + 64 * t] = rv[t];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  double u[NP], d[NP], inv_d[NP], acc[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    u[k] = (reg == 1) ? g0[k] : 0.0;
+    d[k] = (reg == 1) ? 0.0 : g0[k];
+    inv_d[k] = (reg == 2) ? 1.0 / g0[k] : 0.0;
   }
   if (reg == 1)
     synth_harmonics_np<LMAX, 1, NP>(srow[wave], u, ux, uy, uz, d, inv_d, acc);
