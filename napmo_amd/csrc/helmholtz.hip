@@ -197,6 +197,7 @@ __global__ void __launch_bounds__(64) k_helmholtz_radial(
       // mu and the slopes of its spline on the t-axis
       for (int i = 0; i < nr; ++i) x[2 * i * mc] = mu[ubase + (long)i * L + m0 + q];
       double rprev = 0.0;
+#pragma unroll 4
       for (int i = 0; i < nr; ++i) {
         const double ya = x[2 * (i == 0 ? 0 : i - 1) * mc];
         const double yb = x[2 * (i == nr - 1 ? nr - 1 : i + 1) * mc];
@@ -205,6 +206,7 @@ __global__ void __launch_bounds__(64) k_helmholtz_radial(
         x[(2 * i + 1) * mc] = rprev;
       }
       double dtn = rprev;
+#pragma unroll 4
       for (int i = nr - 2; i >= 0; --i) {
         dtn = x[(2 * i + 1) * mc] - supm[i] * dtn;
         x[(2 * i + 1) * mc] = dtn;
@@ -221,6 +223,7 @@ __global__ void __launch_bounds__(64) k_helmholtz_radial(
       };
       double fa0 = 0.0, fa1 = 0.0, fb0, fb1, fc0 = 0.0, fc1 = 0.0;
       fnode(0, fb0, fb1);
+#pragma unroll 2
       for (int i = 0; i < nr; ++i) {
         if (i + 1 < nr)
           fnode(i + 1, fc0, fc1);
@@ -235,6 +238,7 @@ __global__ void __launch_bounds__(64) k_helmholtz_radial(
       }
       // forward: row interchanges and L, window w = x[k .. k + 3]
       double w0 = x[0], w1 = x[mc], w2 = x[2 * mc], w3 = (3 < nfn) ? x[3 * mc] : 0.0;
+#pragma unroll 4
       for (int k = 0; k < nfn; ++k) {
         const int d = ipiv[k] - k;
         if (d == 1) {
@@ -254,6 +258,7 @@ __global__ void __launch_bounds__(64) k_helmholtz_radial(
       // backward: U, window u = x[k + 1 .. k + 6]; the solution's value y = u at even
       // rows, slope dt = u at odd rows (written as (u / j1) j1, the reference's dx round trip)
       double u1 = 0.0, u2 = 0.0, u3 = 0.0, u4 = 0.0, u5 = 0.0, u6 = 0.0;
+#pragma unroll 4
       for (int k = nfn - 1; k >= 0; --k) {
         const double *wr = W + k * WSLOT;
         const double s = x[k * mc] - wr[4] * u1 - wr[5] * u2 - wr[6] * u3 - wr[7] * u4 -

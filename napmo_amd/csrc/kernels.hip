@@ -107,10 +107,11 @@ void launch_collocate(const double *x, const double *y, const double *z, long N,
 // atom: one workgroup per (atom, block of PSB radial shells) so the Lebedev table
 // Yhat[j][lm] = aw_j S_lm(u_j) sqrt(4 pi (2l+1)) / sum aw is read once per block; the
 // shells' f values are staged in LDS in chunks of PJ angular points.
-constexpr int PSB = 5, PJ = 256, PSL = 8;
+constexpr int PSB = 5, PJ = 256;
 
 // Threads: 64 lm x PSL angular slices; slice k sums angular points j = k, k + PSL, ... (two
 // independent loads in flight per step); slices are combined in order through LDS.
+template <int PSL>
 __global__ void __launch_bounds__(64 * PSL) k_project(AtomTables at, const double *__restrict__ dens,
                                                       const double *__restrict__ bw,
                                                       const double *__restrict__ yhat, int L,
@@ -181,7 +182,9 @@ void launch_project(const AtomTables &at, const double *dens, const double *bw, 
   const int L = (lmax + 1) * (lmax + 1);
   NAPMO_REQUIRE(nset >= 1 && (long)nset * at.M <= 65535, "projection: too many density sets");
   dim3 grid(div_up(at.max_nrad, PSB), div_up(L, 64), nset * at.M);
-  k_project<<<grid, 64 * PSL, 0, s>>>(at, dens, bw, yhat, L, rho_lm, dstride, ostride);
+  // one slice count for every batch size, so a density's projections do not depend on
+  // how many densities share the launch
+  k_project<16><<<grid, 64 * 16, 0, s>>>(at, dens, bw, yhat, L, rho_lm, dstride, ostride);
   NAPMO_HIP(hipGetLastError());
 }
 
