@@ -75,9 +75,14 @@ def gpu_leg(args, log):
         from napmo_amd import autotune
         for prec, plan in plans.items():
             t0 = time.perf_counter()
-            setting, src = autotune.tune(
-                plan, dP.ptr, dF.ptr, dE.ptr, tag='orb',
-                build=lambda plan=plan: plan.build_orbitals_device(dC.ptr, C.shape[1], 2.0, dF.ptr, dE.ptr))
+            if prec == 'fp64':
+                # the FP64 plan is tuned on the P-form build of the fp64 mode (the density's
+                # stage knobs only act there; fp64-orb shares its Fock contraction shape)
+                setting, src = autotune.tune(plan, dP.ptr, dF.ptr, dE.ptr)
+            else:
+                setting, src = autotune.tune(
+                    plan, dP.ptr, dF.ptr, dE.ptr, tag='orb',
+                    build=lambda plan=plan: plan.build_orbitals_device(dC.ptr, C.shape[1], 2.0, dF.ptr, dE.ptr))
             log('autotune %s: %s (%s, %.1fs)' % (prec, setting, src, time.perf_counter() - t0))
     results = {}
     for m in modes:
