@@ -19,6 +19,7 @@
 #ifndef NAPMO_GPU_H
 #define NAPMO_GPU_H
 
+#include <stdbool.h>
 #include <stddef.h>
 
 #ifdef __cplusplus
@@ -66,6 +67,16 @@ double *AngularGrid_get_points(void *grid);
 double *AngularGrid_get_weights(void *grid);
 double AngularGrid_integrate(void *grid, int nfunc, double *f);
 void AngularGrid_del(void *grid);
+/* src/grid/angular.cpp:39-48 (napmo/__init__.py:284-285): points -> (1, theta, phi) in place */
+void AngularGrid_spherical(void *grid);
+/* src/grid/angular.cpp:50-93 (napmo/__init__.py:287-289): output[k][lm] = 4 pi sum_j aw_j
+ * f[k][j] Y_lm(u_j) for size_f rows of lorder values; real Y_lm of GSL's convention
+ * (Condon-Shortley phase), lm ordered m = -l..l per l */
+void AngularGrid_spherical_expansion(void *grid, int lmax, int size_f, double *f, double *output);
+/* src/grid/angular.cpp:95-139 (napmo/__init__.py:291-293): output[k][j] = sum_lm
+ * decomposition[k][lm] Y_lm(u_j) */
+void AngularGrid_eval_expansion(void *grid, int lmax, int size_f, double *decomposition,
+                                double *output);
 /* src/grid/radial.cpp:14-51 */
 void *RadialGrid_new(void *rtf, double radii);
 int RadialGrid_get_size(void *grid);
@@ -107,6 +118,18 @@ void BeckeGrid_del(void *grid);
  * kinetic and orbital-correction entry points report an error on such a grid. */
 void *BeckeGrid_from_points(double *pw, int size, int ncenter);
 
+/* ------------------------------------------- auxiliary basis (auxiliary_basis.cpp) */
+/* src/grid/auxiliary_basis.cpp:10-280 (napmo/__init__.py:422-432): orthonormal auxiliary
+ * functions of the grid (ablmax, abldep from BeckeGrid_new); computed on creation. */
+void *AuxiliaryBasis_new(void *grid);
+void AuxiliaryBasis_del(void *auxiliary_basis);
+/* no-op: the basis is computed by AuxiliaryBasis_new (auxiliary_basis.h:59) */
+void AuxiliaryBasis_compute_aobasis(void *auxiliary_basis);
+/* borrowed [npoint x nao] point-major */
+double *AuxiliaryBasis_get_aobasis(void *auxiliary_basis);
+int AuxiliaryBasis_get_nao(void *auxiliary_basis);
+int AuxiliaryBasis_get_ncenter(void *auxiliary_basis);
+
 /* ------------------------------------------------------------ MultiGrid (f4 glue) */
 /* src/grid/multi_grid.cpp:14-132 (bound at napmo/__init__.py:400-419): common points of two
  * species' grids by exact coordinate equality. The reference compares every pair of
@@ -130,6 +153,14 @@ void solve_cubic_spline_system(double *y, double *dt, int npoint);
 /* src/horton/cubic_spline.cpp:138-181; aliases y and dt (caller keeps them alive) */
 void *CubicSpline_new(double *y, double *dt, void *extrapolation, void *rtf, int n);
 void CubicSpline_eval(void *spline, double *new_x, double *new_y, int new_n);
+/* src/horton/cubic_spline.cpp:183-214 (napmo/__init__.py:446-448): dy/dr */
+void CubicSpline_eval_deriv(void *spline, double *new_x, double *new_dx, int new_n);
+/* napmo/__init__.py:456-460: the transform / extrapolation the spline was built with */
+void *CubicSpline_get_rtransform(void *spline);
+void *CubicSpline_get_extrapolation(void *spline);
+/* src/horton/cubic_spline.cpp:105-131 (napmo/__init__.py:574-575): integration weights of
+ * the cubic spline on t = 0..npoint-1 */
+void compute_cubic_spline_int_weights(double *weights, int npoint);
 double CubicSpline_get_first_x(void *spline);
 double CubicSpline_get_last_x(void *spline);
 void CubicSpline_del(void *spline);
@@ -139,6 +170,16 @@ void *ZeroExtrapolation_new(void);
 void *CuspExtrapolation_new(void);
 void *PowerExtrapolation_new(double power);
 void Extrapolation_del(void *extrapolation);
+/* src/horton/extrapolation.cpp:11-37 (napmo/__init__.py:468-488) */
+void Extrapolation_prepare(void *extrapolation, void *spline);
+double Extrapolation_eval_left(void *extrapolation, double x);
+double Extrapolation_eval_right(void *extrapolation, double x);
+double Extrapolation_deriv_left(void *extrapolation, double x);
+double Extrapolation_deriv_right(void *extrapolation, double x);
+/* true for the power-law kinds (Power, Potential), src/horton/extrapolation.h:64-149 */
+bool Extrapolation_has_tail(void *extrapolation);
+/* napmo/__init__.py:496-497 */
+double PowerExtrapolation_get_power(void *extrapolation);
 /* src/horton/ode2.cpp:232-321: coeffs [2n x 2n] (+=, pre-zeroed) and rhs [2n] */
 void build_ode2(double *b, double *a, double *f, double *bcs, double *coeffs, double *rhs,
                 long npoint);
@@ -159,12 +200,36 @@ void dot_multi_moments(long npoint, long nvector, double *data, double *points, 
 /* src/system/gto.cpp:17-51, 161-227; src/system/basis_set.cpp:12-82 */
 void *PrimitiveGaussian_new(int *l, double *origin, double zeta, double coeff);
 double PrimitiveGaussian_get_norma(void *p);
+/* src/system/gto.cpp:85-129 (napmo/__init__.py:203-223). _compute writes out[size] at the
+ * size points r [size x 3]; its double result (declared by the reference's ctypes table,
+ * void in C) is 0. */
+double PrimitiveGaussian_compute(void *p, double *r, double *output, int size);
+double PrimitiveGaussian_overlap(void *p, void *other);
+void PrimitiveGaussian_get_l(void *p, int *l);
+void PrimitiveGaussian_get_origin(void *p, double *origin);
+double PrimitiveGaussian_get_zeta(void *p);
+double PrimitiveGaussian_get_coeff(void *p);
 void *ContractedGaussian_new(void **primitives, int nprim);
 double ContractedGaussian_get_norma(void *c);
+/* src/system/gto.cpp:208-260 (napmo/__init__.py:229-243) */
+int ContractedGaussian_get_nprim(void *c);
+void ContractedGaussian_get_l(void *c, int *l);
+void ContractedGaussian_get_origin(void *c, double *origin);
+void ContractedGaussian_compute(void *c, double *r, double *output, int size);
+double ContractedGaussian_overlap(void *c, void *other);
 void *BasisSet_new(void **contractions, int n);
 void *BasisSet_new_empty(void);
 void BasisSet_update(void *basis, void *other);
 int BasisSet_get_nbasis(void *basis);
+/* src/system/basis_set.cpp:10-19, 115-119 (napmo/__init__.py:268-274): largest x power
+ * (the shell l for complete Cartesian shells) and largest primitive count */
+int BasisSet_get_max_l(void *basis);
+int BasisSet_get_max_nprim(void *basis);
+/* napmo/__init__.py:259-261, napmo/gto/basis_set.py:160-173: out [npoint x nbasis x 3],
+ * the Cartesian gradient of every function at every point, on the GPU. (The reference's
+ * src/system/basis_set.cpp:84-100 copies function values with indexes past its buffer;
+ * the derivative is what the Python API documents.) */
+void BasisSet_deriv(void *basis, double *r, double *out, int npoint);
 /* out [npoint x nbasis] point-major; GPU collocation */
 void BasisSet_compute(void *basis, double *r, double *out, int npoint);
 
@@ -207,6 +272,12 @@ int napmo_compute_coulomb(void *grid, const double *dens, int lmax, double *v_ou
  * lm, the potential spline on A's radial grid: U_y, U_dt [sum_A nrad_A x L] (atom-major,
  * then radial, then lm). */
 int napmo_poisson_solve(void *grid, const double *dens, int lmax, double *U_y, double *U_dt);
+/* napmo/grids/poisson_solver.py:47-50 with sph_exp given: the radial solves of a
+ * precomputed spherical expansion rho_lm [sum_A nrad_A x (lmax+1)^2] (atoms in grid order,
+ * each atom's [nrad x L] block as AtomicGrid_spherical_expansion returns it, i.e. already
+ * including the Becke weight), into U_y / U_dt as napmo_poisson_solve. */
+int napmo_poisson_solve_expansion(void *grid, const double *rho_lm, int lmax, double *U_y,
+                                  double *U_dt);
 /* Kinetic operator on the grid (napmo/wavefunction/nkinetic.py:15-73): out [N] =
  * -1/2 laplacian of psi [N] from the Becke partition of psi, per (atom, l, m)
  * deriv2 - l(l+1)/r^2 of its spherical expansion, splined with PotentialExtrapolation(l)

@@ -19,6 +19,7 @@ from napmo_amd.wavefunction.nnuclear import compute_nuclear  # noqa: F401
 from napmo_amd.wavefunction.nexchange import compute_exchange_potential  # noqa: F401
 from napmo_amd.wavefunction.ndpsi import compute_dpsi, becke_dickson_step  # noqa: F401
 from napmo_amd.grids.multi_grid import MultiGrid  # noqa: F401
+from napmo_amd.grids.auxiliary_basis import AuxiliaryBasis  # noqa: F401
 from napmo_amd.functional import isc_functional_selector, epc17_2  # noqa: F401
 from napmo_amd.wavefunction.psi_base import PSIB  # noqa: F401
 from napmo_amd.system.elements import AtomicElementsDatabase  # noqa: F401

@@ -187,6 +187,60 @@ double AngularGrid_integrate(void *g, int nfunc, double *f) {
   return s * 4.0 * M_PI;
 }
 void AngularGrid_del(void *g) { delete (AngularGrid *)g; }
+void AngularGrid_spherical(void *g) {
+  // points become (1, theta, phi) in place (src/grid/angular.cpp:39-48)
+  AngularGrid *a = (AngularGrid *)g;
+  for (int j = 0; j < a->lorder; ++j) {
+    double *u = &a->points[3 * (size_t)j];
+    double t, p;
+    unit_to_angles(u[0], u[1], u[2], &t, &p);
+    u[0] = 1.0, u[1] = t, u[2] = p;
+  }
+}
+void AngularGrid_spherical_expansion(void *g, int lmax, int size_f, double *f, double *out) {
+  // out[k][lm] = 4 pi sum_j aw_j f[k][j] Y_lm(u_j), lm in the order m = -l..l per l
+  // (src/grid/angular.cpp:50-93)
+  legacy(__func__, [&] {
+    NAPMO_REQUIRE(lmax >= 0, "lmax can not be negative.");
+    AngularGrid *a = (AngularGrid *)g;
+    const int L = (lmax + 1) * (lmax + 1), na = a->lorder;
+    std::vector<double> Y((size_t)na * L);
+    for (int j = 0; j < na; ++j) {
+      const double *u = &a->points[3 * (size_t)j];
+      double t, p;
+      unit_to_angles(u[0], u[1], u[2], &t, &p);
+      real_ylm(t, p, lmax, &Y[(size_t)j * L]);
+    }
+    for (int k = 0; k < size_f; ++k)
+      for (int lm = 0; lm < L; ++lm) {
+        double acc = 0.0;
+        for (int j = 0; j < na; ++j) acc += f[(size_t)k * na + j] * a->weights[j] * Y[(size_t)j * L + lm];
+        out[(size_t)k * L + lm] = acc * 4.0 * M_PI;
+      }
+    return 0;
+  });
+}
+void AngularGrid_eval_expansion(void *g, int lmax, int size_f, double *decomposition, double *out) {
+  // out[k][j] = sum_lm decomposition[k][lm] Y_lm(u_j) (src/grid/angular.cpp:95-139)
+  legacy(__func__, [&] {
+    NAPMO_REQUIRE(lmax >= 0, "lmax can not be negative.");
+    AngularGrid *a = (AngularGrid *)g;
+    const int L = (lmax + 1) * (lmax + 1), na = a->lorder;
+    std::vector<double> Y(L);
+    for (int j = 0; j < na; ++j) {
+      const double *u = &a->points[3 * (size_t)j];
+      double t, p;
+      unit_to_angles(u[0], u[1], u[2], &t, &p);
+      real_ylm(t, p, lmax, Y.data());
+      for (int k = 0; k < size_f; ++k) {
+        double acc = 0.0;
+        for (int lm = 0; lm < L; ++lm) acc += decomposition[(size_t)k * L + lm] * Y[lm];
+        out[(size_t)k * na + j] = acc;
+      }
+    }
+    return 0;
+  });
+}
 
 void *RadialGrid_new(void *rtf, double radii) {
   return legacy(__func__, [&] { return (void *)make_radial((RTransform *)rtf, radii); });
@@ -268,11 +322,12 @@ double *AtomicGrid_spherical_expansion(void *g, double *f, int lmax) {
 }
 
 void *BeckeGrid_new(void **atgrids, int ncenter, int ablmax, double abldep) {
-  (void)ablmax, (void)abldep;
   return legacy(__func__, [&] {
     std::vector<AtomInfo> atoms;
     for (int i = 0; i < ncenter; ++i) atoms.push_back(atom_info(*(AtomicGrid *)atgrids[i]));
     auto *h = new BeckeHandle;
+    h->ablmax = ablmax;
+    h->abldep = abldep;
     h->grid.build(atoms);
     return (void *)h;
   });
@@ -307,6 +362,18 @@ double BeckeGrid_integrate(void *g, double *f) {
 }
 void BeckeGrid_del(void *g) { delete (BeckeHandle *)g; }
 
+void *AuxiliaryBasis_new(void *grid) {
+  return legacy(__func__, [&] {
+    BeckeHandle *h = (BeckeHandle *)grid;
+    return (void *)make_aux_basis(h->grid, h->ablmax, h->abldep);
+  });
+}
+void AuxiliaryBasis_del(void *ab) { delete (AuxBasis *)ab; }
+void AuxiliaryBasis_compute_aobasis(void *ab) { (void)ab; }  // computed by AuxiliaryBasis_new
+double *AuxiliaryBasis_get_aobasis(void *ab) { return ((AuxBasis *)ab)->aobasis.data(); }
+int AuxiliaryBasis_get_nao(void *ab) { return ((AuxBasis *)ab)->nao; }
+int AuxiliaryBasis_get_ncenter(void *ab) { return ((AuxBasis *)ab)->ncenter; }
+
 // ----------------------------------------------------- splines, extrapolation, ODE
 void solve_cubic_spline_system(double *y, double *dt, int npoint) {
   legacy(__func__, [&] {
@@ -324,6 +391,17 @@ void *CubicSpline_new(double *y, double *dt, void *ext, void *rtf, int n) {
   });
 }
 void CubicSpline_eval(void *s, double *x, double *out, int n) { ((CubicSpline *)s)->eval(x, out, n); }
+void CubicSpline_eval_deriv(void *s, double *x, double *out, int n) {
+  ((CubicSpline *)s)->eval_deriv(x, out, n);
+}
+void *CubicSpline_get_rtransform(void *s) { return ((CubicSpline *)s)->rtf; }
+void *CubicSpline_get_extrapolation(void *s) { return ((CubicSpline *)s)->ext; }
+void compute_cubic_spline_int_weights(double *weights, int npoint) {
+  legacy(__func__, [&] {
+    cubic_spline_int_weights(weights, npoint);
+    return 0;
+  });
+}
 double CubicSpline_get_first_x(void *s) { return ((CubicSpline *)s)->first_x; }
 double CubicSpline_get_last_x(void *s) { return ((CubicSpline *)s)->last_x; }
 void CubicSpline_del(void *s) { delete (CubicSpline *)s; }
@@ -344,6 +422,13 @@ void *PowerExtrapolation_new(double power) {
   return e;
 }
 void Extrapolation_del(void *e) { delete (Extrapolation *)e; }
+void Extrapolation_prepare(void *e, void *cs) { ((Extrapolation *)e)->prepare(*(CubicSpline *)cs); }
+double Extrapolation_eval_left(void *e, double x) { return ((Extrapolation *)e)->eval_left(x); }
+double Extrapolation_eval_right(void *e, double x) { return ((Extrapolation *)e)->eval_right(x); }
+double Extrapolation_deriv_left(void *e, double x) { return ((Extrapolation *)e)->deriv_left(x); }
+double Extrapolation_deriv_right(void *e, double x) { return ((Extrapolation *)e)->deriv_right(x); }
+bool Extrapolation_has_tail(void *e) { return ((Extrapolation *)e)->has_tail(); }
+double PowerExtrapolation_get_power(void *e) { return ((Extrapolation *)e)->power; }
 
 void *Cell_new(double *rvecs, int nvec) {
   (void)rvecs;
@@ -416,10 +501,47 @@ void *PrimitiveGaussian_new(int *l, double *origin, double zeta, double coeff) {
   return legacy(__func__, [&] { return (void *)make_primitive(l, origin, zeta, coeff); });
 }
 double PrimitiveGaussian_get_norma(void *p) { return ((Primitive *)p)->norma; }
+double PrimitiveGaussian_get_zeta(void *p) { return ((Primitive *)p)->zeta; }
+double PrimitiveGaussian_get_coeff(void *p) { return ((Primitive *)p)->coeff; }
+void PrimitiveGaussian_get_l(void *p, int *l) {
+  for (int i = 0; i < 3; ++i) l[i] = ((Primitive *)p)->l[i];
+}
+void PrimitiveGaussian_get_origin(void *p, double *o) {
+  for (int i = 0; i < 3; ++i) o[i] = ((Primitive *)p)->origin[i];
+}
+double PrimitiveGaussian_compute(void *p, double *r, double *out, int size) {
+  // values at size points (src/system/gto.cpp:97-104); the reference declares a double
+  // result (napmo/__init__.py:206) but returns nothing: 0 here
+  const Primitive *g = (const Primitive *)p;
+  for (int i = 0; i < size; ++i) out[i] = g->value(&r[3 * (size_t)i]);
+  return 0.0;
+}
+double PrimitiveGaussian_overlap(void *p, void *q) {
+  return primitive_overlap(*(Primitive *)p, *(Primitive *)q);
+}
 void *ContractedGaussian_new(void **prims, int n) {
   return legacy(__func__, [&] { return (void *)make_contraction((Primitive **)prims, n); });
 }
 double ContractedGaussian_get_norma(void *c) { return ((Contraction *)c)->norma; }
+int ContractedGaussian_get_nprim(void *c) { return (int)((Contraction *)c)->prim.size(); }
+void ContractedGaussian_get_l(void *c, int *l) {
+  for (int i = 0; i < 3; ++i) l[i] = ((Contraction *)c)->l[i];
+}
+void ContractedGaussian_get_origin(void *c, double *o) {
+  for (int i = 0; i < 3; ++i) o[i] = ((Contraction *)c)->origin[i];
+}
+void ContractedGaussian_compute(void *c, double *r, double *out, int size) {
+  const Contraction *g = (const Contraction *)c;
+  for (int i = 0; i < size; ++i) out[i] = g->value(&r[3 * (size_t)i]);
+}
+double ContractedGaussian_overlap(void *c, void *d) {
+  // norma_c norma_d sum over primitive pairs (src/system/gto.cpp:196-206)
+  const Contraction *a = (const Contraction *)c, *b = (const Contraction *)d;
+  double s = 0.0;
+  for (const auto &pa : a->prim)
+    for (const auto &pb : b->prim) s += primitive_overlap(pa, pb);
+  return s * a->norma * b->norma;
+}
 void *BasisSet_new(void **conts, int n) {
   auto *b = new BasisSet;
   for (int i = 0; i < n; ++i) b->cont.push_back(*(Contraction *)conts[i]);
@@ -431,6 +553,28 @@ void BasisSet_update(void *basis, void *other) {
   for (auto &c : ((BasisSet *)other)->cont) b->cont.push_back(c);
 }
 int BasisSet_get_nbasis(void *basis) { return (int)((BasisSet *)basis)->cont.size(); }
+int BasisSet_get_max_l(void *basis) { return ((BasisSet *)basis)->max_l(); }
+int BasisSet_get_max_nprim(void *basis) { return ((BasisSet *)basis)->max_nprim(); }
+
+void BasisSet_deriv(void *basis, double *r, double *out, int npoint) {
+  // out [npoint x nbasis x 3]: the Cartesian gradient of every function, on the GPU
+  legacy(__func__, [&] {
+    hipStream_t st = scratch_stream();
+    FlatBasis fb = flatten(*(BasisSet *)basis);
+    if (npoint <= 0 || fb.nbasis == 0) return 0;
+    DevicePoints pts;
+    pts.upload(r, npoint, st);
+    DeviceBasis db;
+    db.upload(fb, st);
+    DeviceBuffer<double> dout((size_t)npoint * fb.nbasis * 3);
+    launch_basis_gradient(pts.x.get(), pts.y.get(), pts.z.get(), npoint, fb.nbasis,
+                          db.origin.get(), db.lxyz.get(), db.first.get(), db.nprim.get(),
+                          db.norm.get(), db.zeta.get(), db.coef.get(), dout.get(), st);
+    dout.download(out, (size_t)npoint * fb.nbasis * 3, st);
+    NAPMO_HIP(hipStreamSynchronize(st));
+    return 0;
+  });
+}
 
 void BasisSet_compute(void *basis, double *r, double *out, int npoint) {
   legacy(__func__, [&] {
@@ -716,6 +860,22 @@ int napmo_poisson_solve(void *grid, const double *dens, int lmax, double *U_y, d
     d.upload(dens, g.N, g.stream);
     g.poisson_expand(d.get(), lmax);
     PoissonPlan &pl = g.plan(lmax);
+    pl.work.Uy.download(U_y, pl.work.Uy.size(), g.stream);
+    pl.work.Udt.download(U_dt, pl.work.Udt.size(), g.stream);
+    NAPMO_HIP(hipStreamSynchronize(g.stream));
+  });
+}
+
+int napmo_poisson_solve_expansion(void *grid, const double *rho_lm, int lmax, double *U_y,
+                                  double *U_dt) {
+  return guarded([&] {
+    DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    PoissonPlan &pl = g.plan(lmax);
+    NAPMO_REQUIRE(rho_lm != nullptr, "napmo_poisson_solve_expansion: null expansion");
+    pl.work.rho_lm.upload(rho_lm, pl.work.rho_lm.size(), g.stream);
+    AtomTables at = g.tables(pl);
+    launch_radial(at, pl.work.rho_lm.get(), pl.green.get(), lmax, pl.work.Uy.get(), pl.work.Udt.get(),
+                  g.stream);
     pl.work.Uy.download(U_y, pl.work.Uy.size(), g.stream);
     pl.work.Udt.download(U_dt, pl.work.Udt.size(), g.stream);
     NAPMO_HIP(hipStreamSynchronize(g.stream));

@@ -75,7 +75,10 @@ struct Extrapolation {
   void prepare(const CubicSpline &cs);
   double eval_left(double x) const;
   double eval_right(double x) const;
-  bool has_tail() const { return kind == EX_POTENTIAL; }
+  double deriv_left(double x) const;
+  double deriv_right(double x) const;
+  // power-law tails (src/horton/extrapolation.h: Power and Potential kinds)
+  bool has_tail() const { return kind == EX_POWER || kind == EX_POTENTIAL; }
 };
 
 struct CubicSpline {
@@ -85,24 +88,34 @@ struct CubicSpline {
   double *y, *dt;  // aliased caller buffers (src/horton/cubic_spline.cpp:138-145)
   int n;
   void eval(const double *x, double *out, int m) const;
+  // dy/dr: the interpolant's t-derivative divided by dr/dt (src/horton/cubic_spline.cpp:183-214)
+  void eval_deriv(const double *x, double *out, int m) const;
 };
 
 void spline_derivatives(const double *y, double *dt, int n);
+// Quadrature weights of the natural cubic spline through (t, y_t), t = 0..n-1: the integral
+// of the spline of the unit vector e_i (src/horton/cubic_spline.cpp:105-131 semantics).
+void cubic_spline_int_weights(double *weights, int n);
 
 // ---------------------------------------------------------------- Gaussian basis
 struct Primitive {
   int l[3];
   double origin[3];
   double zeta, coeff, norma;
+  // coeff * norma * dx^lx dy^ly dz^lz exp(-zeta r^2) at r (src/system/gto.cpp:41-51)
+  double value(const double *r) const;
 };
 struct Contraction {
   double origin[3];
   int l[3];
   std::vector<Primitive> prim;
   double norma;
+  double value(const double *r) const;  // norma * sum of the primitives
 };
 struct BasisSet {
   std::vector<Contraction> cont;
+  int max_l() const;      // largest lx + ly + lz over the contractions (0 when empty)
+  int max_nprim() const;  // largest primitive count over the contractions (0 when empty)
 };
 Primitive *make_primitive(const int *l, const double *origin, double zeta, double coeff);
 Contraction *make_contraction(Primitive **prims, int n);
@@ -122,5 +135,14 @@ FlatBasis flatten(const BasisSet &b);
 // Real solid harmonics of a unit vector in the reference order
 // [C_l0, C_l1, S_l1, ..., C_ll, S_ll] for l = 0..lmax (src/horton/moments.cpp:59-125).
 void solid_harmonics(double x, double y, double z, int lmax, double *out);
+
+// Polar and azimuthal angles of a unit vector: theta = acos(z), phi from x / sqrt(x^2 + y^2)
+// with the sign of y (src/grid/sphere_lebedev_rule.cpp:6718-6760).
+void unit_to_angles(double x, double y, double z, double *theta, double *phi);
+// Real spherical harmonics Y_lm(theta, phi) for l = 0..lmax in the order m = -l..l per l:
+// m = 0: N_l0 P_l(cos theta); m > 0: sqrt2 N_lm P_l^m cos(m phi); m < 0: sqrt2 N_l|m| P_l^|m|
+// sin(|m| phi), with N_lm P_l^m the orthonormalised associated Legendre function including
+// the Condon-Shortley phase (-1)^m (src/utils/spherical_harmonics.cpp:11-27, GSL sphPlm).
+void real_ylm(double theta, double phi, int lmax, double *out);
 
 }  // namespace napmo

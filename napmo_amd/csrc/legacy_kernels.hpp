@@ -26,4 +26,12 @@ void launch_dm_density(const double *phi, int ld, long N, int n, const double *d
                        const double *dmmaxrow, double eps, double *absmax, double *upper,
                        double *thr, long *stop, double *out, hipStream_t s);
 
+// Cartesian gradient of every contracted function at every point (BasisSet_deriv):
+// out[(p n + mu) 3 + k] = d phi_mu / d r_k at point p; points x, y, z [N] (device), the
+// flattened basis arrays as for the collocation.
+void launch_basis_gradient(const double *x, const double *y, const double *z, long N, int n,
+                           const double *fo, const int *fl, const int *ff, const int *fnp,
+                           const double *fnorm, const double *pz, const double *pc, double *out,
+                           hipStream_t s);
+
 }  // namespace napmo

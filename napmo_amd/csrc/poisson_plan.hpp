@@ -188,7 +188,17 @@ struct DeviceGrid {
 // What a BeckeGrid_* handle points to.
 struct BeckeHandle {
   DeviceGrid grid;
+  int ablmax = 0;        // auxiliary-basis settings kept from BeckeGrid_new
+  double abldep = 1e-6;
 };
+
+// Orthonormal auxiliary basis on a Becke grid (AuxiliaryBasis_*): aobasis [N x nao]
+// point-major.
+struct AuxBasis {
+  int ncenter = 0, nao = 0;
+  std::vector<double> aobasis;
+};
+AuxBasis *make_aux_basis(const DeviceGrid &g, int ablmax, double abldep);
 
 // (point, centre) pair buckets of src's centres and radial grids at the target points
 // x, y, z [N] (device): the synthesis lists of src's expansion evaluated at those points.

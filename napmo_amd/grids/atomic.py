@@ -28,6 +28,14 @@ class AtomicGrid(object):
         ptr = cext.AtomicGrid_spherical_expansion(self._this, np.ascontiguousarray(f, dtype=np.float64), lmax)
         return cext.take_array(ptr, (self.radial_grid.size, lsize))
 
+    def evaluate_expansion(self, lmax, expansion):
+        """f(r_i, u_j) = sum_lm expansion[i][lm] Y_lm(u_j) for one atom, with the real
+        spherical harmonics of AngularGrid_spherical_expansion (napmo/grids/atomic.py:57-75)."""
+        output = np.empty(self.size, dtype=np.float64)
+        cext.AngularGrid_eval_expansion(self.angular_grid._this, lmax, self.radial_grid.size,
+                                        np.ascontiguousarray(expansion, dtype=np.float64), output)
+        return output
+
     def spherical_average(self, *args):
         f = np.concatenate(args)
         s = self.integrate(segmented=True)

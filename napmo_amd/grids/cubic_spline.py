@@ -56,6 +56,17 @@ class CubicSpline(object):
         cext.CubicSpline_eval(self._this, new_x, new_y, new_n)
         return new_y
 
+    def deriv(self, new_x, new_dx=None):
+        """dy/dx at new_x (napmo/grids/cubic_spline.py:96-117)."""
+        new_x = np.ascontiguousarray(new_x, dtype=np.float64)
+        new_n = new_x.shape[0]
+        if new_dx is None:
+            new_dx = np.zeros(new_n, np.float64)
+        else:
+            assert new_dx.shape[0] == new_n
+        cext.CubicSpline_eval_deriv(self._this, new_x, new_dx, new_n)
+        return new_dx
+
     @property
     def y(self):
         return self._y

@@ -14,14 +14,23 @@ from napmo_amd.system import cext
 
 
 def poisson_solver(grid, _dens, lmax, sph_exp=None):
-    if sph_exp is not None:
-        raise NotImplementedError('precomputed spherical expansions are not supported')
-    dens = np.ascontiguousarray(_dens, dtype=np.float64)
+    """U[atom][lm] splines of the potential of _dens. With ``sph_exp`` (one [nrad x L]
+    expansion per atom, as AtomicGrid.spherical_expansion of dens * becke_weights returns
+    it), the projection is skipped and those expansions are solved, as the reference does
+    (napmo/grids/poisson_solver.py:47-50)."""
     L = (lmax + 1) ** 2
     nrad_total = sum(g.radial_grid.size for g in grid.atgrids)
     Uy = np.zeros((nrad_total, L))
     Udt = np.zeros((nrad_total, L))
-    cext.check(cext.napmo_poisson_solve(grid._this, dens, lmax, Uy, Udt))
+    if sph_exp:
+        assert len(sph_exp) == grid.ncenter
+        rho_lm = np.ascontiguousarray(np.concatenate(
+            [np.asarray(e, dtype=np.float64).reshape(g.radial_grid.size, L)
+             for e, g in zip(sph_exp, grid.atgrids)]))
+        cext.check(cext.napmo_poisson_solve_expansion(grid._this, rho_lm, lmax, Uy, Udt))
+    else:
+        dens = np.ascontiguousarray(_dens, dtype=np.float64)
+        cext.check(cext.napmo_poisson_solve(grid._this, dens, lmax, Uy, Udt))
     U = []
     off = 0
     for atgrid in grid.atgrids:

@@ -75,6 +75,21 @@ class BasisSet(dict):
         cext.BasisSet_compute(self._this, coord, out, coord.shape[0])
         return out
 
+    def deriv(self, coord=np.zeros([1, 3], dtype=np.float64)):
+        """Cartesian gradient [n_coord x nbasis x 3] on the GPU (napmo/gto/basis_set.py:160-173)."""
+        coord = np.ascontiguousarray(coord, dtype=np.float64)
+        out = np.empty([coord.shape[0], self.get('nbasis'), 3])
+        cext.BasisSet_deriv(self._this, coord, out, coord.shape[0])
+        return out
+
+    @property
+    def max_l(self):
+        return cext.BasisSet_get_max_l(self._this)
+
+    @property
+    def max_nprim(self):
+        return cext.BasisSet_get_max_nprim(self._this)
+
     @property
     def lmax(self):
         return int(max(c.l.sum() for c in self['cont']))

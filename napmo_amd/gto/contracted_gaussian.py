@@ -21,11 +21,19 @@ class ContractedGaussian(object):
 
     @property
     def nprim(self):
-        return len(self._primitives)
+        return cext.ContractedGaussian_get_nprim(self._this)
 
     @property
     def normalization(self):
         return cext.ContractedGaussian_get_norma(self._this)
 
     def compute(self, coord):
-        return self.normalization * sum(p.compute(coord) for p in self._primitives)
+        """Values at coord [n x 3] (napmo/gto/contracted_gaussian.py:79-90)."""
+        coord = np.ascontiguousarray(np.atleast_2d(coord), dtype=np.float64)
+        out = np.empty(coord.shape[0])
+        cext.ContractedGaussian_compute(self._this, coord, out, coord.shape[0])
+        return out
+
+    def overlap(self, other):
+        """Normalised <self|other> (napmo/gto/contracted_gaussian.py:92-102)."""
+        return cext.ContractedGaussian_overlap(self._this, other._this)

@@ -74,6 +74,9 @@ _SIGNATURES = {
     'AngularGrid_get_weights': (dptr, [vp]),
     'AngularGrid_integrate': (c_double, [vp, c_int, a1df]),
     'AngularGrid_del': (None, [vp]),
+    'AngularGrid_spherical': (None, [vp]),
+    'AngularGrid_spherical_expansion': (None, [vp, c_int, c_int, a1df, a2df]),
+    'AngularGrid_eval_expansion': (None, [vp, c_int, c_int, a2df, a1df]),
     'RadialGrid_new': (vp, [vp, c_double]),
     'RadialGrid_get_size': (c_int, [vp]),
     'RadialGrid_get_radii': (c_double, [vp]),
@@ -102,6 +105,12 @@ _SIGNATURES = {
     'BeckeGrid_integrate': (c_double, [vp, a1df]),
     'BeckeGrid_del': (None, [vp]),
     'BeckeGrid_from_points': (vp, [a2df, c_int, c_int]),
+    'AuxiliaryBasis_new': (vp, [vp]),
+    'AuxiliaryBasis_del': (None, [vp]),
+    'AuxiliaryBasis_compute_aobasis': (None, [vp]),
+    'AuxiliaryBasis_get_aobasis': (dptr, [vp]),
+    'AuxiliaryBasis_get_nao': (c_int, [vp]),
+    'AuxiliaryBasis_get_ncenter': (c_int, [vp]),
     'MultiGrid_new': (vp, [c_int]),
     'MultiGrid_del': (None, [vp]),
     'MultiGrid_add_grid': (c_int, [vp, vp]),
@@ -113,6 +122,10 @@ _SIGNATURES = {
     'solve_cubic_spline_system': (None, [a1df, a1df, c_int]),
     'CubicSpline_new': (vp, [a1df, a1df, vp, vp, c_int]),
     'CubicSpline_eval': (None, [vp, a1df, a1df, c_int]),
+    'CubicSpline_eval_deriv': (None, [vp, a1df, a1df, c_int]),
+    'CubicSpline_get_rtransform': (vp, [vp]),
+    'CubicSpline_get_extrapolation': (vp, [vp]),
+    'compute_cubic_spline_int_weights': (None, [a1df, c_int]),
     'CubicSpline_get_first_x': (c_double, [vp]),
     'CubicSpline_get_last_x': (c_double, [vp]),
     'CubicSpline_del': (None, [vp]),
@@ -121,6 +134,13 @@ _SIGNATURES = {
     'CuspExtrapolation_new': (vp, []),
     'PowerExtrapolation_new': (vp, [c_double]),
     'Extrapolation_del': (None, [vp]),
+    'Extrapolation_prepare': (None, [vp, vp]),
+    'Extrapolation_eval_left': (c_double, [vp, c_double]),
+    'Extrapolation_eval_right': (c_double, [vp, c_double]),
+    'Extrapolation_deriv_left': (c_double, [vp, c_double]),
+    'Extrapolation_deriv_right': (c_double, [vp, c_double]),
+    'Extrapolation_has_tail': (ctypes.c_bool, [vp]),
+    'PowerExtrapolation_get_power': (c_double, [vp]),
     'build_ode2': (None, [a1df, a1df, a1df, a1df, a2df, a1df, c_long]),
     'Cell_new': (vp, [vp, c_int]),
     'Cell_del': (None, [vp]),
@@ -129,13 +149,27 @@ _SIGNATURES = {
     # Gaussian basis
     'PrimitiveGaussian_new': (vp, [a1di, a1df, c_double, c_double]),
     'PrimitiveGaussian_get_norma': (c_double, [vp]),
+    'PrimitiveGaussian_compute': (c_double, [vp, a2df, a1df, c_int]),
+    'PrimitiveGaussian_overlap': (c_double, [vp, vp]),
+    'PrimitiveGaussian_get_l': (None, [vp, a1di]),
+    'PrimitiveGaussian_get_origin': (None, [vp, a1df]),
+    'PrimitiveGaussian_get_zeta': (c_double, [vp]),
+    'PrimitiveGaussian_get_coeff': (c_double, [vp]),
     'ContractedGaussian_new': (vp, [aptr, c_int]),
     'ContractedGaussian_get_norma': (c_double, [vp]),
+    'ContractedGaussian_get_nprim': (c_int, [vp]),
+    'ContractedGaussian_get_l': (None, [vp, a1di]),
+    'ContractedGaussian_get_origin': (None, [vp, a1df]),
+    'ContractedGaussian_compute': (None, [vp, a2df, a1df, c_int]),
+    'ContractedGaussian_overlap': (c_double, [vp, vp]),
     'BasisSet_new': (vp, [aptr, c_int]),
     'BasisSet_new_empty': (vp, []),
     'BasisSet_update': (None, [vp, vp]),
     'BasisSet_get_nbasis': (c_int, [vp]),
     'BasisSet_compute': (None, [vp, a2df, a2df, c_int]),
+    'BasisSet_deriv': (None, [vp, a2df, a3df, c_int]),
+    'BasisSet_get_max_l': (c_int, [vp]),
+    'BasisSet_get_max_nprim': (c_int, [vp]),
     # contractions
     'nwavefunction_compute_2body_matrix': (None, [ctypes.POINTER(WaveFunction), vp, a2df, a1df, a2df]),
     'nwavefunction_compute_coupling': (None, [ctypes.POINTER(WaveFunction), vp, a2df, a1df, a2df]),
@@ -144,6 +178,7 @@ _SIGNATURES = {
     # device-resident hot path
     'napmo_compute_coulomb': (c_int, [vp, a1df, c_int, a1df]),
     'napmo_poisson_solve': (c_int, [vp, a1df, c_int, adf, adf]),
+    'napmo_poisson_solve_expansion': (c_int, [vp, adf, c_int, adf, adf]),
     'napmo_compute_kinetic': (c_int, [vp, a1df, c_int, a1df]),
     'napmo_kinetic_expand': (c_int, [vp, a1df, c_int, adf, adf, vp]),
     'napmo_compute_nuclear': (c_int, [vp, c_int, a1df, a2df, a1df]),
