@@ -63,10 +63,11 @@ class MultiGrid(object):
             return np.arange(self._by_id[ga].size)
         if ga is None or gb is None:
             return None
+        ptr = cext.MultiGrid_get_common_index(self._this, ga, gb)  # computes the pair's index
         nrow = int(cext.MultiGrid_get_common_index_size(self._this, ga, gb))
-        view = np.ctypeslib.as_array(cext.MultiGrid_get_common_index(self._this, ga, gb),
-                                     shape=(nrow, 2))
-        return view
+        if nrow <= 0 or not ptr:
+            return np.zeros((0, 2), dtype=np.int32)
+        return np.ctypeslib.as_array(ptr, shape=(nrow, 2))
 
     def show(self):
         print('\nMultiGrid: %d grid(s), %d species' % (self.ngrids, self.nspecies))
