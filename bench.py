@@ -24,17 +24,24 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {'fp64_mfma': 78.6e12, 'fp32_mfma': 157.3e12, 'bf16_mfma': 2.5e15, 'hbm': 8.0e12}
-# Synthesis work per (point, centre, lm) term: the cubic spline of U_lm evaluated at the
-# point's radius (Horner, 3 multiply-adds) and its product with the solid harmonic added
-# to the sum (1 multiply-add); the harmonic's own recurrence is not counted.
-SYNTH_FLOPS = 8.0
+PEAK = {'fp64_mfma': 78.6e12, 'fp64_valu': 78.6e12, 'fp32_mfma': 157.3e12, 'bf16_mfma': 2.5e15,
+        'int8_mfma': 5.0e15, 'hbm': 8.0e12}
+# Algorithmic work of the synthesis (the measurement plan's 2 M N L): one multiply-add per
+# (point, centre, lm) term. The kernel's instruction-level count (Horner spline, 3 FMAs, and
+# the harmonic product, 1 FMA: 8 flops per term) is reported beside it.
+SYNTH_FLOPS = 2.0
+SYNTH_FLOPS_ISSUED = 8.0
+# digit products per element of the int8 contraction (pairs i + j <= 5 of 4 digits each)
+INT8_PRODUCTS = 10
+GATE_EH = 1e-6   # |dE_trace| of a mixed mode against the FP64 mode, in Hartree
 METRIC = 'grid Fock builds/s + % roofline, mixed vs FP64 on 1 MI355X at |dE|<=1uEh'
 
 
 MODES = {  # name: (plan precision, density input)
     'fp64': ('fp64', 'P'),
     'fp64-orb': ('fp64', 'C'),
+    'int8-orb': ('int8', 'C'),
+    'int8': ('int8', 'P'),
     'bf16x3-orb': ('bf16x3', 'C'),
     'bf16x3': ('bf16x3', 'P'),
 }
@@ -146,7 +153,7 @@ def apmo_leg(args, log):
     dF = [DeviceArray(P.shape) for P in Ps]
     dE = DeviceArray((8,))
     out = {}
-    for prec in ('fp64', 'bf16x3'):
+    for prec in ('fp64', 'int8', 'bf16x3'):
         plan = napmo_amd.APMOFock(species, prec)
         step = lambda: plan.build_device([a.ptr for a in dP], [a.ptr for a in dF], dE.ptr)  # noqa: E731
         for _ in range(args.warmup):
@@ -161,26 +168,33 @@ def apmo_leg(args, log):
         out[prec] = {'ms_per_build': dt * 1e3, 'builds_per_s': 1.0 / dt, 'E': E.tolist(),
                      'F': [a.to_host() for a in dF]}
         log('apmo %s: %.3f ms/build  E_e=%s E_n=%s' % (prec, dt * 1e3, E[0], E[1]))
-    ref, mix = out['fp64'], out['bf16x3']
-    dEg = max(abs(mix['E'][s][1] - ref['E'][s][1]) for s in range(2))
-    dEt = max(abs(mix['E'][s][0] - ref['E'][s][0]) for s in range(2))
-    dF = max(float(np.max(np.abs(a - b))) for a, b in zip(mix['F'], ref['F']))
-    log('apmo bf16x3 vs fp64: max dE_grid %.3e, max dE_trace %.3e, max|dF| %.3e, speed-up %.2fx' %
-        (dEg, dEt, dF, ref['ms_per_build'] / mix['ms_per_build']))
-    best = 'bf16x3' if dEg <= 1e-6 and mix['builds_per_s'] > ref['builds_per_s'] else 'fp64'
+    ref = out['fp64']
+    best = 'fp64'
+    for prec in ('int8', 'bf16x3'):
+        mix = out[prec]
+        mix['dE_grid'] = max(abs(mix['E'][s][1] - ref['E'][s][1]) for s in range(2))
+        mix['dE_trace'] = max(abs(mix['E'][s][0] - ref['E'][s][0]) for s in range(2))
+        mix['dF'] = max(float(np.max(np.abs(a - b))) for a, b in zip(mix['F'], ref['F']))
+        log('apmo %s vs fp64: max dE_trace %.3e, max dE_grid %.3e, max|dF| %.3e, speed-up %.2fx' %
+            (prec, mix['dE_trace'], mix['dE_grid'], mix['dF'], ref['ms_per_build'] / mix['ms_per_build']))
+        if mix['dE_trace'] <= GATE_EH and mix['builds_per_s'] > out[best]['builds_per_s']:
+            best = prec
+    mix = out['int8']
+    dEg, dEt, dF = mix['dE_grid'], mix['dE_trace'], mix['dF']
     return {
         'metric': METRIC, 'value': out[best]['builds_per_s'], 'unit': 'builds/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': out[best]['ms_per_build'],
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'bf16x3+f64' if best == 'bf16x3' else 'f64',
+        'dtype': {'fp64': 'f64', 'int8': 'int8+f64', 'bf16x3': 'bf16x3+f64'}[best],
         'data': 'synthetic (seeded geometry, basis and densities)',
         'config': {'workload': 'multi-species Coulomb build: electrons on 8 x 100 x 302 points (120 '
                                'functions) + 4 quantum protons on 4 x 60 x 194 points (320 functions), '
                                'cross-species Poisson coupling', 'precision': best},
-        'modes': {p: {'builds_per_s': r['builds_per_s'], 'ms_per_build': r['ms_per_build'], 'E': r['E']}
+        'modes': {p: {'builds_per_s': r['builds_per_s'], 'ms_per_build': r['ms_per_build'], 'E': r['E'],
+                      'dE_trace_vs_fp64': r.get('dE_trace', 0.0), 'dF_max_vs_fp64': r.get('dF', 0.0)}
                   for p, r in out.items()},
-        'dE_grid_vs_fp64': dEg, 'dE_trace_vs_fp64': dEt, 'dF_max_vs_fp64': dF,
-        'speedup_vs_fp64': ref['ms_per_build'] / mix['ms_per_build'],
+        'int8_dE_grid_vs_fp64': dEg, 'int8_dE_trace_vs_fp64': dEt, 'int8_dF_max_vs_fp64': dF,
+        'speedup_vs_fp64': ref['ms_per_build'] / out[best]['ms_per_build'],
     }
 
 
@@ -258,8 +272,8 @@ def dpsi_leg(args, log):
     ms = phases[name]
     if name == 'synth':
         flops = SYNTH_FLOPS * M * N * L * norb
-        roof = {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': PEAK['fp64_mfma'] / 1e12,
-                'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / PEAK['fp64_mfma'],
+        roof = {'bound': 'valu', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': PEAK['fp64_valu'] / 1e12,
+                'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / PEAK['fp64_valu'],
                 # HBM bytes of one synthesis launch (one orbital)
                 'traffic': table_traffic(traffic_table('traffic_dpsi_chon16.json'), 'k_synth2n<')}
     else:
@@ -362,8 +376,8 @@ def operators_leg(args, log):
     M, L = len(atoms), (lmax + 1) ** 2
     flops = SYNTH_FLOPS * M * N * L * npair
     ms = phases['synth']
-    roof = {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': PEAK['fp64_mfma'] / 1e12,
-            'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / PEAK['fp64_mfma'],
+    roof = {'bound': 'valu', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': PEAK['fp64_valu'] / 1e12,
+            'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / PEAK['fp64_valu'],
             'traffic': table_traffic(traffic_table('traffic_dpsi_chon16.json'), 'k_synth2n<'),
             'kernel': 'synth'}
     cpu = None
@@ -424,7 +438,7 @@ def batch_leg(args, log):
                                                            time.perf_counter() - t0))
     dP, dF, dE = [], [], []
     out = {}
-    for prec in ('fp64', 'bf16x3'):
+    for prec in ('fp64', 'int8'):
         t0 = time.perf_counter()
         plans = [napmo_amd.GridFock(g, b, m['lmax'], prec) for g, b, m in zip(grids, bases, mols)]
         if not Ps:
@@ -465,18 +479,18 @@ def batch_leg(args, log):
         log('batch %s: %.3f ms per batch eager, %.3f ms graph replay (%.1f molecule builds/s)' %
             (prec, t_eager * 1e3, t_graph * 1e3, len(mols) / t_graph))
         del batch, plans
-    ref, mix = out['fp64'], out['bf16x3']
+    ref, mix = out['fp64'], out['int8']
     dEg = np.abs(mix['E'][:, 1] - ref['E'][:, 1])
     dEt = np.abs(mix['E'][:, 0] - ref['E'][:, 0])
     dF = [float(np.max(np.abs(a - b))) for a, b in zip(mix['F'], ref['F'])]
-    log('batch bf16x3 vs fp64 per molecule: max |dE_grid| %.3e, max |dE_trace| %.3e, max |dF| %.3e' %
-        (dEg.max(), dEt.max(), max(dF)))
-    best = 'bf16x3' if dEg.max() <= 1e-6 and mix['ms_graph'] < ref['ms_graph'] else 'fp64'
+    log('batch int8 vs fp64 per molecule: max |dE_trace| %.3e, max |dE_grid| %.3e, max |dF| %.3e' %
+        (dEt.max(), dEg.max(), max(dF)))
+    best = 'int8' if dEt.max() <= GATE_EH and mix['ms_graph'] < ref['ms_graph'] else 'fp64'
     t = out[best]['ms_graph'] * 1e-3
     return {
         'metric': METRIC, 'value': len(mols) / t, 'unit': 'molecule builds/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': t * 1e3, 'higher_is_better': True,
-        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'bf16x3+f64' if best == 'bf16x3' else 'f64',
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'int8+f64' if best == 'int8' else 'f64',
         'data': 'synthetic (seeded molecules, bases and densities)',
         'config': {'workload': 'batch of 64 molecules, 2-12 atoms, uneven 35-110 x 110-590 grids '
                                '(%d points), 20-200 functions, HIP-graph replay over %d streams'
@@ -489,7 +503,8 @@ def batch_leg(args, log):
     }
 
 
-PHASE_KERNELS = {'density': 'k_density_mfma', 'fock': ('k_fock_bf16x3_pass', 'k_fock_f64'),
+PHASE_KERNELS = {'density': 'k_density_mfma',
+                 'fock': {'bf16x3': 'k_fock_bf16x3_pass', 'fp64': 'k_fock_f64', 'int8': 'k_ozaki_fock'},
                  'synth': 'k_synth2n<', 'project': 'k_project', 'radial': 'k_radial',
                  'fock_reduce': 'k_fock_reduce'}
 
@@ -522,8 +537,8 @@ def measured_traffic(res, phase):
     want = PHASE_KERNELS.get(phase)
     if want is None:
         return None
-    if isinstance(want, tuple):
-        want = want[0] if res['prec'] == 'bf16x3' else want[1]
+    if isinstance(want, dict):
+        want = want[res['prec']]
     names = [k for k in kern if want in k]
     if phase == 'density':
         # orbital form: the streamed kernel (C resident in LDS) when the profile has it
@@ -538,41 +553,55 @@ def measured_traffic(res, phase):
     return None
 
 
+def phase_work(res, name, N, n, L, M):
+    """(work, peak, unit, bound) of a phase per the measurement plan: density 2 N n nocc (orbital
+    form) or 2 N n^2 on FP64 MFMA; F 2 N n^2 (FP64 MFMA; int8: INT8_PRODUCTS digit products of
+    it on int8 MFMA; bf16x3: 3 bf16 products); projection 2 N L and synthesis 2 M N L on the FP64
+    vector units; other phases by their HBM bytes."""
+    if name == 'density':
+        k = n if res['form'] == 'P' else res['nocc']
+        return 2.0 * N * n * k, PEAK['fp64_mfma'], 'TFLOP/s', 'mfma'
+    if name == 'fock':
+        if res['prec'] == 'int8':
+            return INT8_PRODUCTS * 2.0 * N * n * n, PEAK['int8_mfma'], 'TOP/s', 'mfma'
+        if res['prec'] == 'bf16x3':
+            return 3 * 2.0 * N * n * n, PEAK['bf16_mfma'], 'TFLOP/s', 'mfma'
+        return 2.0 * N * n * n, PEAK['fp64_mfma'], 'TFLOP/s', 'mfma'
+    if name == 'synth':
+        return SYNTH_FLOPS * M * N * L, PEAK['fp64_valu'], 'TFLOP/s', 'valu'
+    if name == 'project':
+        return 2.0 * N * L, PEAK['fp64_valu'], 'TFLOP/s', 'valu'
+    return 8.0 * N * 4, PEAK['hbm'], 'GB/s', 'hbm'
+
+
 def roofline(res, N, n, L, M):
-    """Roofline of the dominant kernel, algorithmic work per launch / measured time."""
+    """Roofline of the dominant phase: algorithmic work per launch / its mean HIP-event time."""
     ph = res['phases_ms']
     name = max(ph, key=ph.get)
     ms = ph[name]
-    roof = _roofline(res, name, ms, N, n, L, M)
-    roof['traffic'] = measured_traffic(res, name)
+    work, peak, unit, bound = phase_work(res, name, N, n, L, M)
+    scale = 1e9 if unit == 'GB/s' else 1e12
+    roof = {'bound': bound, 'achieved': work / (ms * 1e-3) / scale, 'peak': peak / scale,
+            'unit': unit, 'frac': work / (ms * 1e-3) / peak, 'traffic': measured_traffic(res, name)}
+    if name == 'synth':
+        roof['issued_flops_per_term'] = SYNTH_FLOPS_ISSUED
+        roof['achieved_issued'] = SYNTH_FLOPS_ISSUED * M * N * L / (ms * 1e-3) / 1e12
     return name, roof
 
 
-def _roofline(res, name, ms, N, n, L, M):
-    if name == 'density':
-        # rho from P: 2 N n^2; orbital form: 2 N n nocc (both FP64 MFMA)
-        flops = 2.0 * N * n * (n if res['form'] == 'P' else res['nocc'])
-        peak = PEAK['fp64_mfma']
-        return {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
-                      'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / peak, 'traffic': None}
-    if name == 'fock':
-        # algorithmic 2 N n^2; the bf16x3 split runs 3 bf16 products of it
-        flops = 2.0 * N * n * n
-        if res['prec'] == 'bf16x3':
-            peak, work = PEAK['bf16_mfma'], 3.0 * flops
-        else:
-            peak, work = PEAK['fp64_mfma'], flops
-        return {'bound': 'mfma', 'achieved': work / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
-                      'unit': 'TFLOP/s', 'frac': work / (ms * 1e-3) / peak, 'traffic': None}
-    if name == 'synth':
-        # SYNTH_FLOPS per (point, centre, lm); FP64 vector peak (equal to the FP64 MFMA peak)
-        flops = SYNTH_FLOPS * N * M * L
-        peak = PEAK['fp64_mfma']
-        return {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': peak / 1e12,
-                      'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / peak, 'traffic': None}
-    nbytes = 8.0 * N * 4
-    return {'bound': 'hbm', 'achieved': nbytes / (ms * 1e-3) / 1e9, 'peak': PEAK['hbm'] / 1e9,
-                  'unit': 'GB/s', 'frac': nbytes / (ms * 1e-3) / PEAK['hbm'], 'traffic': None}
+def build_roofline(res, N, n, L, M):
+    """Whole-build roofline time per the measurement plan: sum of the compute phases at their
+    peaks (density and F on MFMA, projection and synthesis on the FP64 vector units), or the
+    algorithmic HBM bytes (Phi read twice at its stored width, the grid arrays, P and F), if
+    larger; fraction = t_roof / t_measured."""
+    t = 0.0
+    for name in ('density', 'fock', 'project', 'synth'):
+        work, peak, _, _ = phase_work(res, name, N, n, L, M)
+        t += work / peak
+    phi_bytes = {'fp64': 8.0, 'int8': 4.0, 'bf16x3': 4.0}[res['prec']]
+    nbytes = N * n * 8.0 + N * n * phi_bytes + 40.0 * N + 16.0 * n * n
+    t = max(t, nbytes / PEAK['hbm'])
+    return {'t_roof_ms': t * 1e3, 't_ms': res['ms_per_build'], 'frac': t * 1e3 / res['ms_per_build']}
 
 
 def cpu_baseline(budget_s, log):
@@ -622,18 +651,35 @@ def cpu_baseline(budget_s, log):
         t_fock = (time.perf_counter() - t) * N / ns
         return t_density, t_synth, t_fock
 
-    d_all, s_all, f_all = stages(16384, threads)
+    # warm-up, then the median of 5 samples of every stage (all host threads), and one
+    # single-thread sample
+    stages(4096, threads)
+    reps = [stages(16384, threads) for _ in range(5)]
+    d_all, s_all, f_all = (float(np.median([r[k] for r in reps])) for k in range(3))
+    pois = []
+    for _ in range(3):
+        t = time.perf_counter()
+        core.poisson_radial(ag, core.spherical_expansion(ag, f, lmax), lmax)
+        pois.append((time.perf_counter() - t) * M)
+    t_poisson = float(np.median(pois + [t_poisson]))
     d_one, s_one, f_one = stages(2048, 1)
     ocx.set_threads(threads)
     total = d_all + t_poisson + s_all + f_all
     total_one = d_one + t_poisson + s_one + f_one
-    log('cpu baseline (%d threads): density %.2fs poisson %.2fs synth %.2fs fock %.2fs per build; '
-        '1 thread: %.1fs per build (sampled in %.1fs)' % (threads, d_all, t_poisson, s_all, f_all,
-                                                          total_one, time.perf_counter() - t0))
-    return {'value': 1.0 / total, 'unit': 'builds/s', 'cores': threads, 'kind': 'port',
+    model = ''
+    try:
+        with open('/proc/cpuinfo') as fh:
+            model = next((l.split(':', 1)[1].strip() for l in fh if l.startswith('model name')), '')
+    except OSError:
+        pass
+    log('cpu baseline (%d threads, %s): density %.2fs poisson %.2fs synth %.2fs fock %.2fs per build '
+        '(median of 5); 1 thread: %.1fs per build (sampled in %.1fs)' % (
+            threads, model, d_all, t_poisson, s_all, f_all, total_one, time.perf_counter() - t0))
+    return {'value': 1.0 / total, 'unit': 'builds/s', 'cores': threads, 'kind': 'port', 'cpu_model': model,
             'sample': 'oracle C/OpenMP stages on 16384 of 472000 points (density, synthesis over 1 of '
-                      '16 centres, contraction) and the full Poisson of 1 of 16 atoms, each scaled to '
-                      'one 16-atom 472,000-point build; Becke/collocation set-up excluded as on the GPU',
+                      '16 centres, contraction; median of 5 after a warm-up) and the full Poisson of 1 of '
+                      '16 atoms (median of 4), each scaled to one 16-atom 472,000-point build; '
+                      'Becke/collocation set-up excluded as on the GPU',
             'seconds_per_build': total,
             'one_thread': {'value': 1.0 / total_one, 'seconds_per_build': total_one,
                            'sample': 'same stages on 2048 points, 1 thread'}}
@@ -644,7 +690,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--modes', default='fp64,fp64-orb,bf16x3-orb',
+    ap.add_argument('--modes', default='fp64,fp64-orb,int8-orb,bf16x3-orb',
                     help='comma list of modes (%s); fp64 is always timed' % ', '.join(MODES))
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-tune', action='store_true', help='keep the default launch shapes')
@@ -691,24 +737,26 @@ def main():
     results, N, n, lmax, M = gpu_leg(args, log)
     L = (lmax + 1) ** 2
     ref = results['fp64']
-    best, dE = 'fp64', 0.0
+    best = 'fp64'
     for m, r in results.items():
         r['nocc'] = 40
         if m == 'fp64':
             continue
-        # energy of the build: the FP64 grid trace 1/2 sum w bw rho v (= 1/2 tr(P F) exactly)
-        r['dE'] = r['E_grid'] - ref['E_grid']
+        # the energy of the F each mode returns (1/2 tr(P F)) against the FP64 build's; the FP64
+        # grid trace 1/2 sum w bw rho v is reported beside it
         r['dE_trace'] = r['E'] - ref['E']
+        r['dE_grid'] = r['E_grid'] - ref['E_grid']
         r['dF_max'] = float(np.max(np.abs(r['F'] - ref['F'])))
-        log('%s vs fp64: dE_grid = %.3e Eh, dE_trace = %.3e Eh, max|dF| = %.3e Eh, speed-up %.2fx' %
-            (m, r['dE'], r['dE_trace'], r['dF_max'], ref['ms_per_build'] / r['ms_per_build']))
-        if abs(r['dE']) <= 1e-6 and r['builds_per_s'] > results[best]['builds_per_s']:
-            best, dE = m, r['dE']
+        log('%s vs fp64: dE_trace = %.3e Eh, dE_grid = %.3e Eh, max|dF| = %.3e Eh, speed-up %.2fx' %
+            (m, r['dE_trace'], r['dE_grid'], r['dF_max'], ref['ms_per_build'] / r['ms_per_build']))
+        if abs(r['dE_trace']) <= GATE_EH and r['builds_per_s'] > results[best]['builds_per_s']:
+            best = m
     head = results[best]
     kname, roof = roofline(head, N, n, L, M)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_seconds, log)
+    fp64_same_form = 'fp64-orb' if head['form'] == 'C' and 'fp64-orb' in results else 'fp64'
     line = {
         'metric': METRIC,
         'value': head['builds_per_s'],
@@ -720,20 +768,26 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'bf16x3+f64' if 'bf16' in best else 'f64',
+        'dtype': {'fp64': 'f64', 'int8': 'int8+f64', 'bf16x3': 'bf16x3+f64'}[head['prec']],
         'data': 'synthetic (seeded geometry, basis and density; no reference data files)',
         'config': {'workload': 'grid Fock build, 16 atoms, 50x590 Becke grid (472,000 points), '
                                '300 s/p/d Gaussians, lmax 12, P resident in HBM',
                    'npoint': N, 'nbasis': n, 'ncenter': M, 'lmax': lmax, 'precision': best},
         'roofline': dict(roof, kernel=kname),
+        'build_roofline': build_roofline(head, N, n, L, M),
         'cpu_baseline': cpu,
+        'gate_dE_trace_eh': GATE_EH,
+        'dE_trace_vs_fp64': head.get('dE_trace', 0.0),
+        'dF_max_vs_fp64': head.get('dF_max', 0.0),
         'modes': {m: {'builds_per_s': r['builds_per_s'], 'ms_per_build': r['ms_per_build'],
-                      'E_grid': r['E_grid'], 'E_trace': r['E'], 'dE_grid_vs_fp64': r.get('dE', 0.0),
+                      'E_grid': r['E_grid'], 'E_trace': r['E'], 'dE_grid_vs_fp64': r.get('dE_grid', 0.0),
                       'dE_trace_vs_fp64': r.get('dE_trace', 0.0), 'dF_max_vs_fp64': r.get('dF_max', 0.0),
-                      'phases_ms': r['phases_ms']}
+                      'meets_gate': abs(r.get('dE_trace', 0.0)) <= GATE_EH,
+                      'phases_ms': r['phases_ms'],
+                      'build_roofline': build_roofline(r, N, n, L, M)}
                   for m, r in results.items()},
-        'dE_vs_fp64': dE,
         'speedup_vs_fp64': ref['ms_per_build'] / head['ms_per_build'],
+        'speedup_vs_fp64_same_density_form': results[fp64_same_form]['ms_per_build'] / head['ms_per_build'],
     }
     print(json.dumps(line), flush=True)
 

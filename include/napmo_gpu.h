@@ -263,7 +263,10 @@ void nwavefunction_compute_density_from_dm(void *basis, void *grid, double *dm, 
 enum {
   NAPMO_PREC_FP64 = 0,   /* everything float64 (FP64 MFMA) */
   NAPMO_PREC_FP32 = 1,   /* Phi and contractions float32 (FP32 MFMA), FP64 Poisson + traces */
-  NAPMO_PREC_BF16X3 = 2  /* 3-term bf16 split on BF16 MFMA, FP32 accumulate, FP64 Poisson */
+  NAPMO_PREC_BF16X3 = 2, /* 3-term bf16 split on BF16 MFMA, FP32 accumulate, FP64 Poisson */
+  NAPMO_PREC_INT8 = 3    /* F on int8 MFMA: 28-bit digit slices of sqrt|w bw| Phi (per geometry)
+                            and of v times them (per build), exact int32 accumulation per chunk,
+                            FP64 across chunks; rho, Poisson and traces FP64 */
 };
 
 /* Poisson: dens [N] (grid order) -> potential v [N] (napmo/wavefunction/ntwobody.py:15-46). */
@@ -414,9 +417,13 @@ enum {
   NAPMO_TUNE_DENSITY_PF = 4, /* stages in flight in the density kernels: 1 or 2; 0 = 1 */
   NAPMO_TUNE_SYNTH_NP = 5,   /* (point, centre) pairs per lane of the synthesis, 1 or 2; 0 = 2
                                 (a property of the grid: shared by every plan on it) */
-  NAPMO_TUNE_DENSITY_STREAM = 6 /* orbital-form density: 1..4 keep C in LDS and stream Phi when
+  NAPMO_TUNE_DENSITY_STREAM = 6, /* orbital-form density: 1..4 keep C in LDS and stream Phi when
                                    C fits (nocc <= 64), in one of three launch shapes (1: default);
                                    0 uses the LDS-staged kernels */
+  NAPMO_TUNE_INT8_CHUNK = 7,  /* int8 mode: points per chunk (a multiple of 64, at most 4032;
+                                 0 = about N / 256). Setting it re-slices the per-geometry digits */
+  NAPMO_TUNE_INT8_ALPHA = 8   /* int8 mode: weight (x 100) of a block row's per-build digit
+                                 formation against its blocks in the row-group split; 0 = 1000 */
 };
 int napmo_fock_plan_set_tuning(void *plan, int key, long value);
 int napmo_fock_plan_get_tuning(void *plan, int key, long *value);

@@ -20,10 +20,12 @@ import time
 from napmo_amd.system import cext
 
 (TUNE_FOCK_WG, TUNE_PASS_WG, TUNE_USE_PASS, TUNE_DENSITY_KS, TUNE_DENSITY_PF, TUNE_SYNTH_NP,
- TUNE_DENSITY_STREAM) = range(7)
+ TUNE_DENSITY_STREAM, TUNE_INT8_CHUNK, TUNE_INT8_ALPHA) = range(9)
 _KEYS = {'fock_wg': TUNE_FOCK_WG, 'pass_wg': TUNE_PASS_WG, 'use_pass': TUNE_USE_PASS,
          'density_ks': TUNE_DENSITY_KS, 'density_pf': TUNE_DENSITY_PF, 'synth_np': TUNE_SYNTH_NP,
-         'density_stream': TUNE_DENSITY_STREAM}
+         'density_stream': TUNE_DENSITY_STREAM, 'int8_chunk': TUNE_INT8_CHUNK,
+         'int8_alpha': TUNE_INT8_ALPHA}
+_INT8_KEYS = ('int8_chunk', 'int8_alpha')   # only int8 plans have them
 DEFAULTS = {'fock_wg': 0, 'pass_wg': 0, 'density_ks': 16, 'density_pf': 1,
             'synth_np': 2, 'density_stream': 1}   # 0: the library's default shape
 
@@ -41,7 +43,12 @@ def shape_key(fock):
 
 def candidates(fock):
     """Launch shapes of the Fock contraction, each with both density stage depths."""
-    if fock.precision == 'bf16x3':
+    if fock.precision == 'int8':
+        # points per chunk (about N / 128 .. N / 512 chunks) and the row-group balance
+        N = fock.npoint
+        chunks = sorted({min(4032, max(64, -(-N // k) + 63) // 64 * 64) for k in (128, 192, 256, 384, 512)})
+        out = [{'int8_chunk': c, 'int8_alpha': a} for c in chunks for a in (500, 1000, 2000)]
+    elif fock.precision == 'bf16x3':
         out = [{'use_pass': 1, 'pass_wg': w} for w in (128, 256, 512, 1024)]
         # the symmetric form only with short chunks (its FP32 chunk sums drift otherwise)
         out += [{'use_pass': 2, 'pass_wg': w} for w in (1024, 2048)]
@@ -49,18 +56,22 @@ def candidates(fock):
     else:
         out = [{'fock_wg': w} for w in (512, 1024, 2048, 4096, 8192)]
     ks = [16, 32] if (-(-fock.nbasis // 64) * 64) % 32 == 0 else [16]
+    if fock.precision == 'int8':   # the orbital-form density's knobs are tuned in the second stage
+        return out
     return [dict(c, density_ks=k, density_pf=f) for c in out for k in ks for f in (1, 2)]
 
 
 def apply(fock, setting):
     for k, v in setting.items():
-        if k in _KEYS:
+        if k in _KEYS and (k not in _INT8_KEYS or fock.precision == 'int8'):
             cext.check(cext.napmo_fock_plan_set_tuning(fock._this, _KEYS[k], int(v)))
 
 
 def current(fock):
     out = {}
     for k, code in _KEYS.items():
+        if k in _INT8_KEYS and fock.precision != 'int8':
+            continue
         v = ctypes.c_long()
         cext.check(cext.napmo_fock_plan_get_tuning(fock._this, code, ctypes.byref(v)))
         out[k] = v.value

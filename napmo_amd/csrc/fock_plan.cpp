@@ -2,6 +2,7 @@
 #include "fock_plan.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 namespace napmo {
@@ -25,8 +26,9 @@ void DeviceBasis::collocate(const double *x, const double *y, const double *z, l
 
 FockPlan::FockPlan(DeviceGrid &g, const BasisSet &b, int lmax_, int prec_mode)
     : grid(g), lmax(lmax_), prec(prec_mode) {
-  NAPMO_REQUIRE(prec == NAPMO_PREC_FP64 || prec == NAPMO_PREC_BF16X3,
-                "precision mode not available: use NAPMO_PREC_FP64 (0) or NAPMO_PREC_BF16X3 (2)");
+  NAPMO_REQUIRE(prec == NAPMO_PREC_FP64 || prec == NAPMO_PREC_BF16X3 || prec == NAPMO_PREC_INT8,
+                "precision mode not available: use NAPMO_PREC_FP64 (0), NAPMO_PREC_BF16X3 (2) or "
+                "NAPMO_PREC_INT8 (3)");
   FlatBasis fb = flatten(b);
   n = fb.nbasis;
   NAPMO_REQUIRE(n > 0, "empty basis");
@@ -54,6 +56,7 @@ FockPlan::FockPlan(DeviceGrid &g, const BasisSet &b, int lmax_, int prec_mode)
     phiT32.resize((size_t)ld * Npad);
     launch_split_phi(phi64.get(), ld, Npad, phiT_hi.get(), phiT_lo.get(), phiT32.get(), st);
   }
+  if (prec == NAPMO_PREC_INT8) ozaki_setup();
   grid.plan(lmax);     // Poisson tables (projection, Green matrices) built once per grid
   grid.alloc_work(lmax, work);
   grid.synth_pairs();  // synthesis pair buckets built once per geometry
@@ -106,6 +109,19 @@ void FockPlan::set_tuning(int key, long value) {
       NAPMO_REQUIRE(value >= 0 && value <= 3, "density streaming choice must be 0 to 3");
       density_stream = (int)value;
       break;
+    case NAPMO_TUNE_INT8_CHUNK:
+    case NAPMO_TUNE_INT8_ALPHA:
+      NAPMO_REQUIRE(prec == NAPMO_PREC_INT8, "int8 tuning keys need an NAPMO_PREC_INT8 plan");
+      NAPMO_REQUIRE(key != NAPMO_TUNE_INT8_CHUNK || value == 0 ||
+                        (value % OZ_STAGE == 0 && value <= (long)OZ_MAX_STAGES * OZ_STAGE),
+                    "int8 chunk must be a multiple of 64 points, at most 4032");
+      NAPMO_HIP(hipStreamSynchronize(stream));
+      if (key == NAPMO_TUNE_INT8_CHUNK)
+        oz_chunk = value;
+      else
+        oz_alpha = value ? value / 100.0 : 10.0;
+      ozaki_setup();
+      break;
     default: throw Error("unknown tuning key " + std::to_string(key));
   }
   NAPMO_HIP(hipStreamSynchronize(stream));
@@ -121,6 +137,8 @@ long FockPlan::get_tuning(int key) const {
     case NAPMO_TUNE_DENSITY_PF: return density_pf;
     case NAPMO_TUNE_SYNTH_NP: return grid.synth_np;
     case NAPMO_TUNE_DENSITY_STREAM: return density_stream;
+    case NAPMO_TUNE_INT8_CHUNK: return prec == NAPMO_PREC_INT8 ? oz.chunk : 0;
+    case NAPMO_TUNE_INT8_ALPHA: return prec == NAPMO_PREC_INT8 ? (long)std::lround(oz_alpha * 100) : 0;
     default: throw Error("unknown tuning key " + std::to_string(key));
   }
 }
@@ -239,9 +257,42 @@ void FockPlan::synth_self(int slot, bool with_scale) {
     launch_contrib_reduce(sp.contrib.get(), grid.M, grid.N, 0, v.get(), st);
 }
 
+void FockPlan::ozaki_setup() {
+  hipStream_t st = stream;
+  oz = ozaki_layout(grid.N, n, ld, oz_alpha, max_lds_per_block(), oz_chunk);
+  oz_planes.resize((size_t)4 * ld * oz.Nq);
+  oz_eA.resize((size_t)ld * oz.nchunk);
+  oz_maxd1.resize((size_t)ld * (oz.Nq / OZ_STAGE));
+  oz_vs.resize(oz.Nq);
+  oz_part.resize((size_t)oz.nchunk * oz.partial_stride());
+  std::vector<int> tab = oz.tasks;
+  tab.insert(tab.end(), oz.row0.begin(), oz.row0.end());
+  oz_tasks.upload(tab, st);
+  launch_ozaki_setup(oz, phi64.get(), grid.wbw.get(), oz_planes.get(), oz_eA.get(), oz_maxd1.get(), st);
+  NAPMO_HIP(hipStreamSynchronize(st));
+}
+
+void FockPlan::ozaki_contract(const double *veff, double *dF, int slot) {
+  hipStream_t st = stream;
+  launch_ozaki_vs(veff, grid.wbw.get(), grid.N, oz.Nq, oz_vs.get(), st);
+  mark(slot, 7);
+  launch_ozaki_fock(oz, oz_tasks.get(), oz_planes.get(), oz_eA.get(), oz_maxd1.get(), oz_vs.get(),
+                    oz_part.get(), st);
+  mark(slot, 8);
+  launch_ozaki_reduce(oz, oz_part.get(), dF, st);
+}
+
 void FockPlan::contract(int slot, const double *dP, const double *veff, double *dF, double *dE,
                         bool s_ready) {
   hipStream_t st = stream;
+  if (prec == NAPMO_PREC_INT8) {
+    ozaki_contract(veff, dF, slot);
+    mark(slot, 9);
+    launch_traces(dP, dF, n, grid.wbw.get(), rho.get(), veff, grid.N, dE, trace_work.get(), st);
+    mark(slot, 10);
+    built = true;
+    return;
+  }
   if (!s_ready) launch_scale3(veff, grid.wbw.get(), nullptr, grid.N, Npad, s.get(), st);
   mark(slot, 7);
   // the one-pass kernel writes FP32 partials into the same buffer
@@ -316,6 +367,12 @@ void FockPlan::contract_potential(const double *v_host, double *F_out) {
   hipStream_t st = stream;
   DeviceBuffer<double> dv;
   dv.upload(v_host, grid.N, st);
+  if (prec == NAPMO_PREC_INT8) {
+    ozaki_contract(dv.get(), F.get(), -1);
+    F.download(F_out, (size_t)n * n, st);
+    NAPMO_HIP(hipStreamSynchronize(st));
+    return;
+  }
   launch_scale3(dv.get(), grid.wbw.get(), nullptr, grid.N, Npad, s.get(), st);
   float *part32 = reinterpret_cast<float *>(F_part.get());
   if (use_pass == 2)

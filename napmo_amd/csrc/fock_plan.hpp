@@ -6,6 +6,7 @@
 #include "common.hpp"
 #include "host.hpp"
 #include "kernels.hpp"
+#include "ozaki.hpp"
 #include "poisson_plan.hpp"
 
 namespace napmo {
@@ -32,6 +33,17 @@ struct FockPlan {
   DeviceBuffer<double> Cpad, Pfull;    // orbital-form inputs
   DeviceBuffer<double> P, rho_part, rho, v, s, F_part, F, E, trace_work;
   FockShape fsh, psh;  // per-tile shape; one-pass shape (mixed mode)
+  // int8 mode: digit planes of sqrt|w bw| Phi, chunk exponents, per-stage max |d1|, the
+  // per-build signed potential vs = sign(w bw) v, partials, wave tasks (+ group rows)
+  OzakiLayout oz;
+  DeviceBuffer<signed char> oz_planes;
+  DeviceBuffer<int> oz_eA, oz_tasks;
+  DeviceBuffer<unsigned char> oz_maxd1;
+  DeviceBuffer<double> oz_vs, oz_part;
+  long oz_chunk = 0;       // NAPMO_TUNE_INT8_CHUNK (0: default)
+  double oz_alpha = 10.0;  // NAPMO_TUNE_INT8_ALPHA / 100
+  void ozaki_setup();
+  void ozaki_contract(const double *veff, double *dF, int slot);
   int use_pass = 0;  // 0: per-tile kernels, 1: one-pass BF16x3, 2: symmetric one-pass BF16x3
   long fock_wg = 0, pass_wg = 0;  // workgroup targets of the two shapes (0 = default)
   int density_ks = 16;            // K per LDS stage of the density kernels (16 or 32)

@@ -1,0 +1,441 @@
+// Exact-accumulation grid Fock contraction on int8 MFMA (see ozaki.hpp for the scheme).
+//
+// v_mfma_i32_16x16x64_i8 operand layout (lane i of 64), the same for both operands, so the
+// order of k inside a lane does not matter: A[m][k] row m = i % 16, k = 16 (i / 16) .. + 15;
+// B[k][n] column n = i % 16, same k; D[16 x 16] int32: lane i holds D[4 (i / 16) + r][i % 16].
+// Operands come from LDS planes [digit][function][64 points] (64 B per function, its four
+// 16-byte segments stored in the order q ^ ((f >> 2) & 3), so the 16 functions of a block
+// read one segment each without bank conflicts).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "ozaki.hpp"
+
+namespace napmo {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+// Smallest e with 2^e >= m (m > 0); 0 for m == 0 (all digits are then zero).
+__host__ __device__ inline int pow2_ceil_exp(double m) {
+  if (!(m > 0.0)) return 0;
+  int ex;
+  const double mant = frexp(m, &ex);  // m = mant 2^ex, mant in [0.5, 1)
+  return mant == 0.5 ? ex - 1 : ex;
+}
+
+// J in [-2^27, 2^27] -> balanced base-128 digits, each in [-64, 64].
+__device__ inline void split4(int J, int &d1, int &d2, int &d3, int &d4) {
+  d1 = (J + (1 << 20)) >> 21;
+  int r = J - d1 * (1 << 21);
+  d2 = (r + (1 << 13)) >> 14;
+  r -= d2 * (1 << 14);
+  d3 = (r + (1 << 6)) >> 7;
+  d4 = r - d3 * (1 << 7);
+}
+
+__device__ inline int sbyte(unsigned w, int b) { return (int)(signed char)(w >> (8 * b)); }
+__device__ inline unsigned pack4(int a, int b, int c, int d) {
+  return (unsigned)(a & 0xFF) | ((unsigned)(b & 0xFF) << 8) | ((unsigned)(c & 0xFF) << 16) |
+         ((unsigned)(d & 0xFF) << 24);
+}
+
+__device__ inline size_t seg_off(int nf, int plane, int fl, int q) {
+  return ((size_t)(plane * nf + fl) * 4 + (q ^ ((fl >> 2) & 3))) * 16;
+}
+
+// ------------------------------------------------------------------ per geometry
+// eA[f][c] = exponent of the chunk maximum of |sqrt|wbw| phi| (one workgroup per chunk).
+__global__ void __launch_bounds__(256) k_oz_scales(const double *__restrict__ phi64, int ld,
+                                                   const double *__restrict__ wbw, long N, long chunk,
+                                                   int nchunk, int *__restrict__ eA) {
+  const int c = blockIdx.x;
+  const long p0 = (long)c * chunk, p1 = min(N, p0 + chunk);
+  for (int f = threadIdx.x; f < ld; f += blockDim.x) {
+    double m = 0.0;
+    for (long p = p0; p < p1; ++p) m = fmax(m, fabs(sqrt(fabs(wbw[p])) * phi64[p * ld + f]));
+    eA[(size_t)f * nchunk + c] = pow2_ceil_exp(m);
+  }
+}
+
+// Digit planes and per-stage max |d1|: one workgroup per (stage of 64 points, 64 functions).
+__global__ void __launch_bounds__(256) k_oz_digits(const double *__restrict__ phi64, int ld,
+                                                   const double *__restrict__ wbw, long N, long Nq,
+                                                   long chunk, int nchunk, const int *__restrict__ eA,
+                                                   signed char *__restrict__ planes,
+                                                   unsigned char *__restrict__ maxd1) {
+  __shared__ double tile[OZ_STAGE][65];
+  const long p0 = (long)blockIdx.x * OZ_STAGE;
+  const int f0 = blockIdx.y * 64;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < OZ_STAGE * 64; e += 256) {
+    const int pr = e / 64, fc = e % 64;
+    const long p = p0 + pr;
+    tile[pr][fc] = p < N ? sqrt(fabs(wbw[p])) * phi64[p * ld + f0 + fc] : 0.0;
+  }
+  __syncthreads();
+  const int fl = tid >> 2, q = tid & 3, f = f0 + fl;
+  const int c = (int)(p0 / chunk);
+  const int e = eA[(size_t)f * nchunk + c];
+  unsigned w[4][4];
+  int m1 = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const double x = ldexp(tile[16 * q + k][fl], 27 - e);
+    const int J = (int)rint(x);
+    int d1, d2, d3, d4;
+    split4(J, d1, d2, d3, d4);
+    m1 = max(m1, abs(d1));
+    const int b = k & 3;
+    if (b == 0) w[0][k >> 2] = w[1][k >> 2] = w[2][k >> 2] = w[3][k >> 2] = 0u;
+    w[0][k >> 2] |= (unsigned)(d1 & 0xFF) << (8 * b);
+    w[1][k >> 2] |= (unsigned)(d2 & 0xFF) << (8 * b);
+    w[2][k >> 2] |= (unsigned)(d3 & 0xFF) << (8 * b);
+    w[3][k >> 2] |= (unsigned)(d4 & 0xFF) << (8 * b);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    u4 v = {w[k][0], w[k][1], w[k][2], w[k][3]};
+    *(u4 *)&planes[((size_t)k * ld + f) * Nq + p0 + 16 * q] = v;
+  }
+  m1 = max(m1, __shfl_xor(m1, 1));
+  m1 = max(m1, __shfl_xor(m1, 2));
+  if (q == 0) maxd1[(size_t)f * (Nq / OZ_STAGE) + blockIdx.x] = (unsigned char)m1;
+}
+
+__global__ void k_oz_vs(const double *__restrict__ v, const double *__restrict__ wbw, long N, long Nq,
+                        double *__restrict__ vs) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= Nq) return;
+  vs[p] = p < N ? (wbw[p] < 0.0 ? -v[p] : v[p]) : 0.0;
+}
+
+// --------------------------------------------------------------------- per build
+struct OzArgs {
+  const signed char *planes;
+  const int *eA;
+  const unsigned char *maxd1;
+  const double *vs;
+  const int *tasks;   // [ngroups][OZ_WAVES][OZ_MAXC][3]
+  const int *row0;    // [ngroups + 1]
+  double *partial;
+  long Nq, chunk, pstride;
+  int ld, nb, nchunk, ngroups, nstage_total;
+};
+
+__device__ inline v4i mfma8(v4i a, v4i b, v4i c) {
+  return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+}
+
+// One 16 x 16 block: rows r[0..3] (digits of B, the v-weighted side), columns c[0..3].
+__device__ inline void block_products(const v4i r[4], const v4i c[4], v4i &W, v4i &X) {
+  const v4i z = {0, 0, 0, 0};
+  v4i t = mfma8(r[0], c[0], z);
+  t = t * 128;
+  t = mfma8(r[0], c[1], t);
+  t = mfma8(r[1], c[0], t);
+  W += t;
+  v4i u = mfma8(r[0], c[3], z);
+  u = mfma8(r[1], c[2], u);
+  u = mfma8(r[2], c[1], u);
+  u = mfma8(r[3], c[0], u);
+  u = (u + 64) >> 7;
+  u = mfma8(r[0], c[2], u);
+  u = mfma8(r[1], c[1], u);
+  u = mfma8(r[2], c[0], u);
+  X += u;
+}
+
+__global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
+  extern __shared__ __align__(16) char lds[];
+  // chunk c on XCD c % 8 for every group (workgroups are dealt to the 8 XCDs in turn)
+  const int id = blockIdx.x;
+  const int c8 = id % 8, rest = id / 8;
+  const int g = rest % a.ngroups, c = (rest / a.ngroups) * 8 + c8;
+  if (c >= a.nchunk) return;
+  const int r0 = a.row0[g], r1 = a.row0[g + 1];
+  const int nA = 16 * (a.nb - r0), nB = 16 * (r1 - r0);
+  char *ldsA = lds;
+  char *ldsB = lds + (size_t)4 * nA * 64;
+  double *vsS = (double *)(ldsB + (size_t)4 * nB * 64);   // [OZ_STAGE]
+  double *scB = vsS + OZ_STAGE;                            // [nB]
+  int *eRow = (int *)(scB + nB);                           // [nB]
+  double *maxv = (double *)(eRow + nB + (nB & 1));          // [OZ_MAX_STAGES]
+  int *s_eV_ptr = (int *)(maxv + OZ_MAX_STAGES);             // [1] (all LDS is dynamic)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ns = (int)(a.chunk / OZ_STAGE);
+  const long pc = (long)c * a.chunk;
+  const int stage0 = (int)(pc / OZ_STAGE);
+
+  // ---- prologue: stage maxima of |vs|, the chunk exponent eV, and per B function the
+  // scale of its digits: bound = 2^-6 max_t (maxv_t 2^-eV) (maxd1 + 1) >= max |vs At| 2^-(eA+eV)
+  for (int t = wave; t < ns; t += OZ_WAVES) {
+    double m = fabs(a.vs[pc + (long)t * OZ_STAGE + lane]);
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if (lane == 0) maxv[t] = m;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double m = 0.0;
+    for (int t = 0; t < ns; ++t) m = fmax(m, maxv[t]);
+    *s_eV_ptr = pow2_ceil_exp(m);
+  }
+  __syncthreads();
+  const int eV = *s_eV_ptr;
+  for (int fl = tid; fl < nB; fl += 64 * OZ_WAVES) {
+    const int f = 16 * r0 + fl;
+    const unsigned char *md = a.maxd1 + (size_t)f * a.nstage_total + stage0;
+    double bound = 0.0;
+    for (int t = 0; t < ns; ++t) bound = fmax(bound, ldexp(maxv[t], -eV) * (double)(md[t] + 1));
+    bound = ldexp(bound, -6);
+    const int eB = pow2_ceil_exp(bound);
+    scB[fl] = ldexp(1.0, -eV - eB);
+    eRow[fl] = a.eA[(size_t)f * a.nchunk + c] + eV + eB;
+  }
+
+  v4i W[2 * OZ_MAXC], X[2 * OZ_MAXC];
+#pragma unroll
+  for (int k = 0; k < 2 * OZ_MAXC; ++k) W[k] = X[k] = (v4i){0, 0, 0, 0};
+  const int *task = a.tasks + ((size_t)(g * OZ_WAVES + wave) * OZ_MAXC) * 3;
+  const int r16 = lane & 15, q = lane >> 4;
+
+  for (int t = 0; t < ns; ++t) {
+    const long p0 = pc + (long)t * OZ_STAGE;
+    __syncthreads();  // the previous stage's MFMAs are done with the LDS planes
+    if (tid < OZ_STAGE) vsS[tid] = a.vs[p0 + tid];
+    __syncthreads();
+    // ---- staging: A digits of the group's column functions; B digits of its row functions
+    for (int it = tid; it < nA * 4; it += 64 * OZ_WAVES) {
+      const int fl = it >> 2, sq = it & 3;
+      const int f = 16 * r0 + fl;
+      u4 d[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        d[k] = *(const u4 *)&a.planes[((size_t)k * a.ld + f) * a.Nq + p0 + 16 * sq];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) *(u4 *)&ldsA[seg_off(nA, k, fl, sq)] = d[k];
+      if (fl < nB) {
+        const double sc = scB[fl];
+        u4 o[4];
+#pragma unroll
+        for (int wd = 0; wd < 4; ++wd) {
+          int b[4][4];
+#pragma unroll
+          for (int by = 0; by < 4; ++by) {
+            const int JA = sbyte(d[0][wd], by) * (1 << 21) + sbyte(d[1][wd], by) * (1 << 14) +
+                           sbyte(d[2][wd], by) * (1 << 7) + sbyte(d[3][wd], by);
+            const double y = vsS[16 * sq + 4 * wd + by] * sc;
+            const int JB = (int)rint((double)JA * y);
+            split4(JB, b[0][by], b[1][by], b[2][by], b[3][by]);
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k) o[k][wd] = pack4(b[k][0], b[k][1], b[k][2], b[k][3]);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) *(u4 *)&ldsB[seg_off(nB, k, fl, sq)] = o[k];
+      }
+    }
+    __syncthreads();
+    // ---- MFMA phase: this wave's column items (up to two blocks each)
+    int curI0 = -2, curI1 = -2;
+    v4i ra[4], rb[4];
+#pragma unroll
+    for (int k = 0; k < OZ_MAXC; ++k) {
+      const int I0 = __builtin_amdgcn_readfirstlane(task[3 * k]);
+      const int I1 = __builtin_amdgcn_readfirstlane(task[3 * k + 1]);
+      const int J = __builtin_amdgcn_readfirstlane(task[3 * k + 2]);
+      if (J < 0) continue;
+      if (I0 != curI0) {
+        const int fl = 16 * (I0 - r0) + r16;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) ra[d] = *(const v4i *)&ldsB[seg_off(nB, d, fl, q)];
+        curI0 = I0;
+      }
+      if (I1 >= 0 && I1 != curI1) {
+        const int fl = 16 * (I1 - r0) + r16;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) rb[d] = *(const v4i *)&ldsB[seg_off(nB, d, fl, q)];
+        curI1 = I1;
+      }
+      v4i cf[4];
+      {
+        const int fl = 16 * (J - r0) + r16;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) cf[d] = *(const v4i *)&ldsA[seg_off(nA, d, fl, q)];
+      }
+      block_products(ra, cf, W[2 * k], X[2 * k]);
+      if (I1 >= 0 && J >= I1) block_products(rb, cf, W[2 * k + 1], X[2 * k + 1]);
+    }
+  }
+  // ---- epilogue: scaled FP64 partial of every block of this wave
+  double *part = a.partial + (size_t)c * a.pstride;
+#pragma unroll
+  for (int k = 0; k < OZ_MAXC; ++k) {
+    const int J = __builtin_amdgcn_readfirstlane(task[3 * k + 2]);
+    if (J < 0) continue;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int I = __builtin_amdgcn_readfirstlane(task[3 * k + h]);
+      if (I < 0 || J < I) continue;
+      const int blk = I * a.nb - I * (I - 1) / 2 + (J - I);
+      const int col = 16 * J + r16;
+      const int ec = a.eA[(size_t)col * a.nchunk + c];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = 4 * q + r;
+        const double val = (double)W[2 * k + h][r] * 0x1p-19 + (double)X[2 * k + h][r] * 0x1p-26;
+        part[(size_t)blk * 256 + m * 16 + r16] = ldexp(val, eRow[16 * (I - r0) + m] + ec);
+      }
+    }
+  }
+}
+
+// F[mu][nu] = F[nu][mu] = sum over chunks (in order) of the block element, mu <= nu < n.
+__global__ void k_ozaki_reduce(const double *__restrict__ partial, long pstride, int nchunk, int nb,
+                               int n, double *__restrict__ F) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  const int mu = (int)(e / n), nu = (int)(e % n);
+  if (mu > nu) return;
+  const int I = mu >> 4, J = nu >> 4;
+  const long off = ((long)(I * nb - I * (I - 1) / 2 + (J - I))) * 256 + (mu & 15) * 16 + (nu & 15);
+  double s = 0.0;
+  for (int c = 0; c < nchunk; ++c) s += partial[(long)c * pstride + off];
+  F[(long)mu * n + nu] = s;
+  F[(long)nu * n + mu] = s;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------ host
+static bool pack_waves(int r0, int r1, int nb, std::vector<int> &tab) {
+  // column items (row pair, column J) in order; contiguous runs to the waves, balancing blocks
+  struct Item { int I0, I1, J, nblk; };
+  std::vector<Item> items;
+  int total = 0;
+  for (int I = r0; I < r1; I += 2) {
+    const int I1 = I + 1 < r1 ? I + 1 : -1;
+    for (int J = I; J < nb; ++J) {
+      const int nbk = 1 + (I1 >= 0 && J >= I1 ? 1 : 0);
+      items.push_back({I, I1, J, nbk});
+      total += nbk;
+    }
+  }
+  tab.assign((size_t)OZ_WAVES * OZ_MAXC * 3, -1);
+  size_t it = 0;
+  int done = 0;
+  for (int w = 0; w < OZ_WAVES && it < items.size(); ++w) {
+    const int left = OZ_WAVES - w;
+    const int target = (total - done + left - 1) / left;
+    int got = 0;
+    for (int k = 0; k < OZ_MAXC && it < items.size() && got < target; ++k, ++it) {
+      int *t = &tab[((size_t)w * OZ_MAXC + k) * 3];
+      t[0] = items[it].I0, t[1] = items[it].I1, t[2] = items[it].J;
+      got += items[it].nblk;
+    }
+    done += got;
+  }
+  return it == items.size();
+}
+
+static size_t group_lds(int r0, int r1, int nb) {
+  const size_t nA = 16 * (size_t)(nb - r0), nB = 16 * (size_t)(r1 - r0);
+  return 4 * (nA + nB) * 64 + OZ_STAGE * 8 + nB * 12 + 8 + OZ_MAX_STAGES * 8 + 16;
+}
+
+OzakiLayout ozaki_layout(long N, int n, int ld, double alpha, size_t max_lds, long chunk_target) {
+  NAPMO_REQUIRE(N > 0 && n > 0, "ozaki layout: empty grid or basis");
+  OzakiLayout L;
+  L.N = N, L.n = n, L.ld = ld;
+  L.nb = (int)div_up(n, 16);
+  NAPMO_REQUIRE(16 * L.nb <= ld && ld % 64 == 0, "ozaki layout: leading dimension too small");
+  L.nblk = L.nb * (L.nb + 1) / 2;
+  long chunk = chunk_target > 0 ? chunk_target : div_up(N, 256);
+  chunk = std::min<long>(std::max<long>(round_up(chunk, OZ_STAGE), OZ_STAGE), (long)OZ_MAX_STAGES * OZ_STAGE);
+  L.chunk = chunk;
+  L.nchunk = (int)div_up(N, chunk);
+  L.Nq = (long)L.nchunk * chunk;
+  // row groups: consecutive block rows, cost of row I = (nb - I) blocks + alpha (its 16
+  // functions' digits of B formed per build); the fewest groups (at least two when
+  // possible) whose LDS and wave slots fit
+  const int nb = L.nb;
+  for (int ng = std::min(2, nb); ng <= nb; ++ng) {
+    double total = 0.0;
+    for (int I = 0; I < nb; ++I) total += (nb - I) + alpha;
+    std::vector<int> row0{0};
+    double acc = 0.0;
+    for (int I = 0; I < nb && (int)row0.size() < ng; ++I) {
+      acc += (nb - I) + alpha;
+      if (acc >= total * row0.size() / ng && I + 1 < nb) row0.push_back(I + 1);
+    }
+    row0.push_back(nb);
+    const int gcount = (int)row0.size() - 1;
+    bool ok = true;
+    size_t lds = 0;
+    std::vector<int> tasks;
+    for (int g = 0; g < gcount && ok; ++g) {
+      std::vector<int> tab;
+      ok = pack_waves(row0[g], row0[g + 1], nb, tab) && group_lds(row0[g], row0[g + 1], nb) <= max_lds;
+      lds = std::max(lds, group_lds(row0[g], row0[g + 1], nb));
+      tasks.insert(tasks.end(), tab.begin(), tab.end());
+    }
+    if (ok) {
+      L.ngroups = gcount;
+      L.row0 = row0;
+      L.tasks = tasks;
+      L.lds_bytes = lds;
+      return L;
+    }
+  }
+  throw Error("int8 Fock contraction: basis of " + std::to_string(n) +
+              " functions does not fit the workgroup's LDS (use the FP64 mode)");
+}
+
+void launch_ozaki_setup(const OzakiLayout &L, const double *phi64, const double *wbw,
+                        signed char *planes, int *eA, unsigned char *maxd1, hipStream_t s) {
+  k_oz_scales<<<L.nchunk, 256, 0, s>>>(phi64, L.ld, wbw, L.N, L.chunk, L.nchunk, eA);
+  NAPMO_HIP(hipGetLastError());
+  k_oz_digits<<<dim3((unsigned)(L.Nq / OZ_STAGE), L.ld / 64), 256, 0, s>>>(
+      phi64, L.ld, wbw, L.N, L.Nq, L.chunk, L.nchunk, eA, planes, maxd1);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_ozaki_vs(const double *v, const double *wbw, long N, long Nq, double *vs, hipStream_t s) {
+  k_oz_vs<<<div_up(Nq, 256), 256, 0, s>>>(v, wbw, N, Nq, vs);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_ozaki_fock(const OzakiLayout &L, const int *d_tasks, const signed char *planes,
+                       const int *eA, const unsigned char *maxd1, const double *vs,
+                       double *partial, hipStream_t s) {
+  static bool attr = false;
+  if (!attr) {
+    NAPMO_HIP(hipFuncSetAttribute((const void *)k_ozaki_fock, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)max_lds_per_block()));
+    attr = true;
+  }
+  NAPMO_REQUIRE(L.lds_bytes <= max_lds_per_block(), "int8 Fock contraction: LDS layout too large");
+  OzArgs a;
+  a.planes = planes, a.eA = eA, a.maxd1 = maxd1, a.vs = vs;
+  a.tasks = d_tasks;
+  a.row0 = d_tasks + (size_t)L.ngroups * OZ_WAVES * OZ_MAXC * 3;
+  a.partial = partial;
+  a.Nq = L.Nq, a.chunk = L.chunk, a.pstride = L.partial_stride();
+  a.ld = L.ld, a.nb = L.nb, a.nchunk = L.nchunk, a.ngroups = L.ngroups;
+  a.nstage_total = (int)(L.Nq / OZ_STAGE);
+  const int nc8 = (int)round_up(L.nchunk, 8);
+  k_ozaki_fock<<<nc8 * L.ngroups, 64 * OZ_WAVES, L.lds_bytes, s>>>(a);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_ozaki_reduce(const OzakiLayout &L, const double *partial, double *F, hipStream_t s) {
+  const long nn = (long)L.n * L.n;
+  k_ozaki_reduce<<<div_up(nn, 256), 256, 0, s>>>(partial, L.partial_stride(), L.nchunk, L.nb, L.n, F);
+  NAPMO_HIP(hipGetLastError());
+}
+
+}  // namespace napmo

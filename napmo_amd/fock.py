@@ -14,7 +14,7 @@ import numpy as np
 
 from napmo_amd.system import cext
 
-PRECISIONS = {'fp64': 0, 'fp32': 1, 'bf16x3': 2}
+PRECISIONS = {'fp64': 0, 'fp32': 1, 'bf16x3': 2, 'int8': 3}
 
 
 class GridFock(object):

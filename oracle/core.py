@@ -1338,3 +1338,81 @@ def c_2species_grid(index, rho_a, rho_b, bw_b, w_b, vrho_a, vrho_b):
     vrho_a[index[:, 0]] += c_vrho
     vrho_b[index[:, 1]] += c_other_vrho
     return aux
+
+
+# ----------------------------------------------------------------------------------
+# Digit-sliced (int8) grid contraction: an exact restatement of the product's integer
+# arithmetic (TEST INFRASTRUCTURE: it checks the GPU kernel bit for bit; its own accuracy is
+# checked against the FP64 contraction coupling_matrix).
+# ----------------------------------------------------------------------------------
+
+def _pow2_ceil_exp(m):
+    """Smallest e with 2^e >= m elementwise (0 where m == 0)."""
+    m = np.asarray(m, dtype=np.float64)
+    mant, ex = np.frexp(m)
+    e = np.where(mant == 0.5, ex - 1, ex)
+    return np.where(m > 0, e, 0).astype(np.int64)
+
+
+def _split4(J):
+    """J in [-2^27, 2^27] (int64) -> balanced base-128 digits d1..d4 in [-64, 64]."""
+    d1 = (J + (1 << 20)) >> 21
+    r = J - d1 * (1 << 21)
+    d2 = (r + (1 << 13)) >> 14
+    r = r - d2 * (1 << 14)
+    d3 = (r + (1 << 6)) >> 7
+    d4 = r - d3 * (1 << 7)
+    return d1, d2, d3, d4
+
+
+def int8_fock(phi, wbw, v, chunk, stage=64):
+    """F = Phi^T diag(wbw v) Phi as the int8 MFMA kernel computes it: At = sqrt|wbw| Phi as
+    28-bit digits per (function, chunk) at the chunk's power-of-two maximum; B = sign(wbw) v At
+    as 28-bit digits at the bound 2^-6 max_stages(max|vs| 2^-eV (max|d1| + 1)); per stage of 64
+    points W += T2 128 + T3, X += T4 + floor((T5 + 64) / 128) with T_L the exact sum of the
+    digit products of level L; chunk value 2^(e_row + e_col) (W 2^-19 + X 2^-26); chunks
+    summed in order. phi [N, n]."""
+    N, n = phi.shape
+    nchunk = -(-N // chunk)
+    Nq = nchunk * chunk
+    G = np.zeros((Nq, n))
+    G[:N] = np.sqrt(np.abs(wbw))[:, None] * phi
+    vs = np.zeros(Nq)
+    vs[:N] = np.where(wbw < 0.0, -v, v)
+    F = np.zeros((n, n))
+    for c in range(nchunk):
+        sl = slice(c * chunk, (c + 1) * chunk)
+        Gc = G[sl]
+        eA = _pow2_ceil_exp(np.abs(Gc).max(axis=0))
+        JA = np.rint(np.ldexp(Gc, (27 - eA)[None, :])).astype(np.int64)
+        dA = _split4(JA)
+        vc = vs[sl]
+        ns = chunk // stage
+        maxv = np.abs(vc).reshape(ns, stage).max(axis=1)
+        eV = int(_pow2_ceil_exp(maxv.max()))
+        md1 = np.abs(dA[0]).reshape(ns, stage, n).max(axis=1)       # [ns, n]
+        bound = np.zeros(n)
+        for t in range(ns):
+            bound = np.fmax(bound, np.ldexp(maxv[t], -eV) * (md1[t] + 1).astype(np.float64))
+        bound = np.ldexp(bound, -6)
+        eB = _pow2_ceil_exp(bound)
+        scB = np.ldexp(1.0, -eV - eB)
+        y = vc[:, None] * scB[None, :]
+        JB = np.rint(JA.astype(np.float64) * y).astype(np.int64)
+        dB = _split4(JB)
+        W = np.zeros((n, n))
+        X = np.zeros((n, n))
+        for t in range(ns):
+            st = slice(t * stage, (t + 1) * stage)
+            r = [d[st].astype(np.float64) for d in dB]   # rows: B digits
+            a = [d[st].astype(np.float64) for d in dA]   # columns: A digits
+            T2 = r[0].T @ a[0]
+            T3 = r[0].T @ a[1] + r[1].T @ a[0]
+            T4 = r[0].T @ a[2] + r[1].T @ a[1] + r[2].T @ a[0]
+            T5 = r[0].T @ a[3] + r[1].T @ a[2] + r[2].T @ a[1] + r[3].T @ a[0]
+            W += T2 * 128 + T3
+            X += T4 + np.floor((T5 + 64) / 128)
+        val = W * 2.0 ** -19 + X * 2.0 ** -26
+        F = F + np.ldexp(val, (eA + eV + eB)[:, None] + eA[None, :])
+    U = np.triu(F)
+    return U + np.triu(U, 1).T

@@ -1,0 +1,100 @@
+"""int8 digit-sliced grid Fock contraction (NAPMO_PREC_INT8) on the GPU:
+  * bit for bit against the oracle's exact restatement of its integer arithmetic
+    (oracle.core.int8_fock) on the same Phi, weights and potential, over several chunks,
+    stages, row groups and a grid with negative Lebedev weights;
+  * its accuracy against the FP64 build: |dE_trace| = |1/2 tr(P (F_int8 - F_fp64))| <= 1e-6 Eh
+    on the 16-atom 472,000-point system (the stated mixed-precision tolerance), and far below
+    it on the small systems."""
+import numpy as np
+import pytest
+
+from oracle import core
+from tests import systems
+
+pytestmark = pytest.mark.gpu
+
+TUNE_INT8_CHUNK = 7
+
+
+def _plan_chunk(fock, chunk):
+    import ctypes
+    from napmo_amd.system import cext
+    cext.check(cext.napmo_fock_plan_set_tuning(fock._this, TUNE_INT8_CHUNK, chunk))
+    v = ctypes.c_long()
+    cext.check(cext.napmo_fock_plan_get_tuning(fock._this, TUNE_INT8_CHUNK, ctypes.byref(v)))
+    return v.value
+
+
+@pytest.mark.parametrize('case', ['five_atoms', 'negative_weights'])
+def test_int8_contraction_is_bit_exact_with_the_oracle_restatement(case):
+    import napmo_amd
+    rng = np.random.default_rng(71)
+    if case == 'five_atoms':
+        atoms = systems.random_molecule(rng, 5, 5.0, min_sep=1.2)
+        shells = systems.random_shells(rng, 5, 7)            # 70 functions: 5 blocks, two groups
+        g, b = systems.gpu_system(atoms, shells, 24, 110)
+    else:
+        atoms = systems.random_molecule(rng, 3, 4.0, min_sep=1.2)
+        shells = systems.random_shells(rng, 3, 4)
+        g, b = systems.gpu_system(atoms, shells, None, None, sizes=[(20, 230), (18, 266), (16, 74)])
+        assert (g.weights < 0).any()
+    fock = napmo_amd.GridFock(g, b, 8, 'int8')
+    chunk = _plan_chunk(fock, 320)  # 5 stages per chunk, many chunks
+    assert chunk == 320
+    phi = b.compute(g.points)
+    wbw = g.weights * g.becke_weights
+    v = 1.0 / (0.5 + ((g.points - g.origin[0]) ** 2).sum(1)) - 0.3 * np.exp(-((g.points - g.origin[1]) ** 2).sum(1))
+    F = fock.contract(v)
+    Fo = core.int8_fock(phi, wbw, v, chunk)
+    Fr = core.coupling_matrix(phi, v, g.becke_weights, g.weights)
+    print('%s: n %d N %d max|F - oracle| %.3e max|F - fp64| %.3e (max|F| %.3e)' % (
+        case, b['nbasis'], g.size, np.abs(F - Fo).max(), np.abs(F - Fr).max(), np.abs(Fr).max()))
+    assert np.array_equal(F, Fo)
+    # 28-bit operands: ~1e-8 of max|F| here; the sign-changing potential on the grid with
+    # negative Lebedev weights cancels more (measured 2.8e-8)
+    tol = 1e-8 if case == 'five_atoms' else 1e-7
+    assert np.abs(F - Fr).max() <= tol * np.abs(Fr).max()
+    # the default chunking gives the same accuracy
+    _plan_chunk(fock, 0)
+    assert np.abs(fock.contract(v) - Fr).max() <= tol * np.abs(Fr).max()
+
+
+def test_int8_build_matches_fp64_build_small():
+    import napmo_amd
+    for atoms, shells, nr, na, lmax in (
+            (systems.H2_ATOMS, systems.named_shells(systems.H2_ATOMS, 'CC-PVDZ'), 100, 110, 10),
+            (systems.random_molecule(np.random.default_rng(31), 5, 5.0, min_sep=1.2),
+             systems.random_shells(np.random.default_rng(31), 5, 6), 30, 110, 8)):
+        g, b = systems.gpu_system(atoms, shells, nr, na)
+        f64 = napmo_amd.GridFock(g, b, lmax, 'fp64')
+        i8 = napmo_amd.GridFock(g, b, lmax, 'int8')
+        P = systems.density_matrix(np.random.default_rng(3), f64.overlap(), 2)
+        F1, E1, Eg1 = f64.build(P)
+        F2, E2, Eg2 = i8.build(P)
+        print('int8 n=%d dE_trace %.3e max|dF| %.3e (max|F| %.3e)' % (
+            b['nbasis'], E2 - E1, np.abs(F2 - F1).max(), np.abs(F1).max()))
+        assert abs(E2 - E1) <= 1e-7    # the stated mixed-mode tolerance is 1e-6 Eh
+        assert abs(Eg2 - Eg1) <= 1e-12 * abs(Eg1)
+        assert np.abs(F2 - F1).max() <= 1e-8 * np.abs(F1).max()
+        assert np.array_equal(F2, F2.T)
+
+
+def test_int8_build_meets_the_stated_tolerance_on_the_16_atom_system():
+    # 16 atoms, 472,000 points, 300 functions, lmax 12: |1/2 tr(P F_int8) - 1/2 tr(P F_fp64)|
+    # <= 1e-6 Eh, with the orbital-form density (the benchmarked build) and the P form
+    import napmo_amd
+    atoms, shells, nr, na, lmax = systems.chon16_system()
+    g, b = systems.gpu_system(atoms, shells, nr, na)
+    f64 = napmo_amd.GridFock(g, b, lmax, 'fp64')
+    C = systems.occupied_orbitals(np.random.default_rng(2), f64.overlap(), 40)
+    F1, E1, Eg1 = f64.build_orbitals(C, 2.0)
+    del f64
+    i8 = napmo_amd.GridFock(g, b, lmax, 'int8')
+    F2, E2, Eg2 = i8.build_orbitals(C, 2.0)
+    F3, E3, Eg3 = i8.build(2.0 * C @ C.T)
+    print('chon16 int8: E %.10f dE_trace %.3e (P form %.3e) max|dF| %.3e max|F| %.3e' % (
+        E1, E2 - E1, E3 - E1, np.abs(F2 - F1).max(), np.abs(F1).max()))
+    assert abs(E2 - E1) <= 1e-6
+    assert abs(E3 - E1) <= 1e-6
+    assert np.abs(F2 - F1).max() <= 1e-7 * np.abs(F1).max()
+    assert np.array_equal(F2, F2.T)
