@@ -44,9 +44,11 @@ def shape_key(fock):
 def candidates(fock):
     """Launch shapes of the Fock contraction, each with both density stage depths."""
     if fock.precision == 'int8':
-        # points per chunk (about N / 128 .. N / 512 chunks) and the row-group balance
+        # points per chunk (about N / 256 .. N / 512 chunks, at most 1920 points: longer chunks
+        # widen the range each function's fixed-point scale must cover, and the 16-atom
+        # system's |dE_trace| grew to 5e-7 Eh at 3712) and the row-group balance
         N = fock.npoint
-        chunks = sorted({min(4032, max(64, -(-N // k) + 63) // 64 * 64) for k in (128, 192, 256, 384, 512)})
+        chunks = sorted({min(1920, max(64, -(-N // k) + 63) // 64 * 64) for k in (256, 320, 384, 512)})
         out = [{'int8_chunk': c, 'int8_alpha': a} for c in chunks for a in (500, 1000, 2000)]
     elif fock.precision == 'bf16x3':
         out = [{'use_pass': 1, 'pass_wg': w} for w in (128, 256, 512, 1024)]

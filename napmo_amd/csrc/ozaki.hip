@@ -303,8 +303,17 @@ __global__ void k_ozaki_reduce(const double *__restrict__ partial, long pstride,
   if (mu > nu) return;
   const int I = mu >> 4, J = nu >> 4;
   const long off = ((long)(I * nb - I * (I - 1) / 2 + (J - I))) * 256 + (mu & 15) * 16 + (nu & 15);
+  // loads issued 8 at a time, added in chunk order
   double s = 0.0;
-  for (int c = 0; c < nchunk; ++c) s += partial[(long)c * pstride + off];
+  int c = 0;
+  for (; c + 8 <= nchunk; c += 8) {
+    double x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = partial[(long)(c + j) * pstride + off];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s += x[j];
+  }
+  for (; c < nchunk; ++c) s += partial[(long)c * pstride + off];
   F[(long)mu * n + nu] = s;
   F[(long)nu * n + mu] = s;
 }
