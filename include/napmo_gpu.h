@@ -342,6 +342,18 @@ int napmo_epc17_2(long n, const double *rhoE, const double *rhoN, double *c_ene,
 int napmo_epc17_2_grid(void *grid_a, void *grid_b, const int *index, int npair,
                        const double *rho_a, const double *rho_b, double *vrho_a, double *vrho_b,
                        double *energy);
+/* No counterpart in the reference: napmo_epc17_2_grid for callers that cannot allocate
+ * (HIP graph capture, tight serving loops). All arrays are device pointers: d_index
+ * [npair x 2] int, d_rho_a [grid_a size], d_rho_b [grid_b size], d_vrho_a / d_vrho_b (+=),
+ * d_energy [1]. Runs on grid_b's stream without synchronising; calls no allocator. Its
+ * scratch (the small-density flag and 256 partial energies, 2.3 KB) is a static device array
+ * per GPU shared by every call: calls that overlap on one GPU race. Index rows outside the
+ * grids are skipped (napmo_epc17_2_grid reports them instead). */
+int napmo_epc17_2_grid_noalloc(void *grid_a, void *grid_b, const int *d_index, int npair,
+                               const double *d_rho_a, const double *d_rho_b, double *d_vrho_a,
+                               double *d_vrho_b, double *d_energy);
+/* Waits for the work queued on a grid's stream (device-pointer grid entry points do not). */
+int napmo_grid_sync(void *grid);
 /* HIP-event timing of the orbital corrections on this grid: start (enable = 1) or stop;
  * read gives the mean milliseconds per call of each phase (residual, sources and projection,
  * shell averages, radial solves, synthesis tables, synthesis, reduction) since the start. */

@@ -813,12 +813,27 @@ int napmo_epc17_2_grid(void *grid_a, void *grid_b, const int *index, int npair,
     va.upload(vrho_a, ga.N, st), vb.upload(vrho_b, gb.N, st);
     idx.upload(index, 2 * (size_t)npair, st);
     NAPMO_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), st));
-    launch_epc17_2_pairs(ra.get(), rb.get(), idx.get(), npair, gb.wbw.get(), flag.get(), va.get(),
-                         vb.get(), part.get(), e.get(), st);
+    launch_epc17_2_pairs(ra.get(), rb.get(), idx.get(), npair, ga.N, gb.N, gb.wbw.get(), flag.get(),
+                         va.get(), vb.get(), part.get(), e.get(), st);
     va.download(vrho_a, ga.N, st), vb.download(vrho_b, gb.N, st);
     e.download(energy, 1, st);
     NAPMO_HIP(hipStreamSynchronize(st));
   });
+}
+
+int napmo_epc17_2_grid_noalloc(void *grid_a, void *grid_b, const int *d_index, int npair,
+                               const double *d_rho_a, const double *d_rho_b, double *d_vrho_a,
+                               double *d_vrho_b, double *d_energy) {
+  return guarded([&] {
+    NAPMO_REQUIRE(grid_a && grid_b && npair >= 0, "napmo_epc17_2_grid_noalloc: bad arguments");
+    DeviceGrid &ga = ((BeckeHandle *)grid_a)->grid, &gb = ((BeckeHandle *)grid_b)->grid;
+    launch_epc17_2_pairs_noalloc(d_rho_a, d_rho_b, d_index, npair, ga.N, gb.N, gb.wbw.get(),
+                                 d_vrho_a, d_vrho_b, d_energy, gb.stream);
+  });
+}
+
+int napmo_grid_sync(void *grid) {
+  return guarded([&] { NAPMO_HIP(hipStreamSynchronize(((BeckeHandle *)grid)->grid.stream)); });
 }
 
 int napmo_grid_profile(void *grid, int enable) {

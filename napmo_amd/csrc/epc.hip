@@ -7,6 +7,9 @@
 // is taken for the whole array when any density on it is below 1e-200, as the reference
 // decides. Contraction is off so every product and sum is rounded as the reference's numpy
 // expression is.
+#include <map>
+#include <mutex>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -16,13 +19,25 @@ namespace {
 
 constexpr double EPC_A = 2.35, EPC_B = 2.4, EPC_C = 6.6, EPC_TINY = 1.0e-200;
 
+// index rows outside [0, Na) x [0, Nb) are skipped (the host entry point rejects them first)
+__device__ inline bool epc_pair_ok(const int *index, long k, long Na, long Nb) {
+  const int i = index[2 * k], j = index[2 * k + 1];
+  return i >= 0 && i < Na && j >= 0 && j < Nb;
+}
+
 __global__ void k_epc_small(const double *__restrict__ rE, const double *__restrict__ rN,
-                            const int *__restrict__ index, long n, int *__restrict__ flag) {
+                            const int *__restrict__ index, long n, long Na, long Nb,
+                            int *__restrict__ flag) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
+  if (index && !epc_pair_ok(index, k, Na, Nb)) return;
   const double e = index ? rE[index[2 * k]] : rE[k];
   const double p = index ? rN[index[2 * k + 1]] : rN[k];
   if (fabs(e) < EPC_TINY || fabs(p) < EPC_TINY) atomicOr(flag, 1);
+}
+
+__global__ void k_epc_clear(int *flag) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *flag = 0;
 }
 
 struct EpcOut {
@@ -62,7 +77,7 @@ constexpr int EPC_BLOCKS = 256;
 __global__ void __launch_bounds__(256) k_epc_pairs(const double *__restrict__ rE,
                                                    const double *__restrict__ rN,
                                                    const int *__restrict__ index, long n,
-                                                   const int *__restrict__ flag,
+                                                   long Na, long Nb, const int *__restrict__ flag,
                                                    const double *__restrict__ wbwN,
                                                    double *__restrict__ vE, double *__restrict__ vN,
                                                    double *__restrict__ part) {
@@ -71,6 +86,7 @@ __global__ void __launch_bounds__(256) k_epc_pairs(const double *__restrict__ rE
   const bool small = *flag != 0;
   double acc = 0.0;
   for (long k = (long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long)gridDim.x * 256) {
+    if (!epc_pair_ok(index, k, Na, Nb)) continue;
     const int i = index[2 * k], j = index[2 * k + 1];
     const double e = rE[i];
     const EpcOut o = epc17_2(e, rN[j], small);
@@ -96,19 +112,56 @@ __global__ void k_epc_final(const double *__restrict__ part, int nb, double *__r
 void launch_epc17_2(const double *rE, const double *rN, long n, int *flag, double *ene,
                     double *potE, double *potN, hipStream_t st) {
   if (n <= 0) return;
-  k_epc_small<<<div_up(n, 256), 256, 0, st>>>(rE, rN, nullptr, n, flag);
+  k_epc_small<<<div_up(n, 256), 256, 0, st>>>(rE, rN, nullptr, n, 0, 0, flag);
   k_epc_arrays<<<div_up(n, 256), 256, 0, st>>>(rE, rN, n, flag, ene, potE, potN);
   NAPMO_HIP(hipGetLastError());
 }
 
-void launch_epc17_2_pairs(const double *rE, const double *rN, const int *index, long n,
-                          const double *wbwN, int *flag, double *vE, double *vN, double *part,
-                          double *energy, hipStream_t st) {
-  const int nb = (int)std::max<long>(1, std::min<long>(EPC_BLOCKS, div_up(n, 256)));
-  if (n > 0) k_epc_small<<<div_up(n, 256), 256, 0, st>>>(rE, rN, index, n, flag);
-  k_epc_pairs<<<nb, 256, 0, st>>>(rE, rN, index, n, flag, wbwN, vE, vN, part);
+int epc_pair_blocks(long n) { return (int)std::max<long>(1, std::min<long>(EPC_BLOCKS, div_up(n, 256))); }
+
+void launch_epc17_2_pairs(const double *rE, const double *rN, const int *index, long n, long Na,
+                          long Nb, const double *wbwN, int *flag, double *vE, double *vN,
+                          double *part, double *energy, hipStream_t st) {
+  const int nb = epc_pair_blocks(n);
+  if (n > 0) k_epc_small<<<div_up(n, 256), 256, 0, st>>>(rE, rN, index, n, Na, Nb, flag);
+  k_epc_pairs<<<nb, 256, 0, st>>>(rE, rN, index, n, Na, Nb, flag, wbwN, vE, vN, part);
   k_epc_final<<<1, 64, 0, st>>>(part, nb, energy);
   NAPMO_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------- allocation-free variant
+// Scratch of napmo_epc17_2_grid_noalloc: the partial energies (EPC_PART doubles) and the
+// small-density flag, in one file-scope device array; each GPU holds its own copy, found once
+// per device by hipGetSymbolAddress. Every call on a GPU shares it (calls that overlap on one
+// GPU race); the flag is cleared by a kernel at the start of each call.
+constexpr int EPC_SCRATCH_DOUBLES = EPC_PART + 32;
+static __device__ __align__(256) double scratch[EPC_SCRATCH_DOUBLES];
+
+static double *epc_scratch_ptr() {
+  static std::mutex mu;
+  static std::map<int, double *> ptrs;
+  int dev = 0;
+  NAPMO_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = ptrs.find(dev);
+  if (it != ptrs.end()) return it->second;
+  void *p = nullptr;
+  NAPMO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  NAPMO_REQUIRE(p != nullptr, "epc17-2 scratch: no device address");
+  ptrs[dev] = (double *)p;
+  return (double *)p;
+}
+
+void launch_epc17_2_pairs_noalloc(const double *rE, const double *rN, const int *index, long n,
+                                  long Na, long Nb, const double *wbwN, double *vE, double *vN,
+                                  double *energy, hipStream_t st) {
+  const int nb = epc_pair_blocks(n);
+  NAPMO_REQUIRE(nb + 1 <= EPC_SCRATCH_DOUBLES,
+                "epc17-2 (no allocation): the partial energies exceed the static scratch");
+  double *s = epc_scratch_ptr();
+  int *flag = (int *)(s + EPC_PART);
+  k_epc_clear<<<1, 64, 0, st>>>(flag);
+  launch_epc17_2_pairs(rE, rN, index, n, Na, Nb, wbwN, flag, vE, vN, s, energy, st);
 }
 
 }  // namespace napmo

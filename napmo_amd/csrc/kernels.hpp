@@ -172,10 +172,16 @@ void launch_epc17_2(const double *rE, const double *rN, long n, int *flag, doubl
                     double *potE, double *potN, hipStream_t st);
 // On the common points index [n x 2] of two grids: vE[i] += potE, vN[j] += potN and
 // energy[0] = sum (wbwN[j] ene) rE[i]; part: EPC_PART doubles of scratch.
+// Index rows outside [0, Na) x [0, Nb) are skipped.
 constexpr int EPC_PART = 256;
-void launch_epc17_2_pairs(const double *rE, const double *rN, const int *index, long n,
-                          const double *wbwN, int *flag, double *vE, double *vN, double *part,
-                          double *energy, hipStream_t st);
+void launch_epc17_2_pairs(const double *rE, const double *rN, const int *index, long n, long Na,
+                          long Nb, const double *wbwN, int *flag, double *vE, double *vN,
+                          double *part, double *energy, hipStream_t st);
+// The same with the flag and partial energies in the file's static device scratch (no
+// allocation; the scratch address is looked up once per device).
+void launch_epc17_2_pairs_noalloc(const double *rE, const double *rN, const int *index, long n,
+                                  long Na, long Nb, const double *wbwN, double *vE, double *vN,
+                                  double *energy, hipStream_t st);
 
 // rho[p] = Phi_p^T P Phi_p with FP64 MFMA (P [ld x ld] zero padded). partial [ntile x Npad].
 // nreal: functions actually present (columns nreal .. ld - 1 of Phi and rows of P are zero),

@@ -192,6 +192,8 @@ _SIGNATURES = {
     'napmo_grid_profile': (c_int, [vp, c_int]),
     'napmo_epc17_2': (c_int, [c_long, vp, vp, vp, vp, vp]),
     'napmo_epc17_2_grid': (c_int, [vp, vp, vp, c_int, vp, vp, vp, vp, vp]),
+    'napmo_epc17_2_grid_noalloc': (c_int, [vp, vp, vp, c_int, vp, vp, vp, vp, vp]),
+    'napmo_grid_sync': (c_int, [vp]),
     'napmo_grid_profile_read': (c_int, [vp, vp, c_int, ctypes.POINTER(c_int)]),
     'napmo_fock_plan_new': (c_int, [vp, vp, c_int, c_int, ctypes.POINTER(vp)]),
     'napmo_fock_plan_del': (None, [vp]),
