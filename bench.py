@@ -31,8 +31,8 @@ PEAK = {'fp64_mfma': 78.6e12, 'fp64_valu': 78.6e12, 'fp32_mfma': 157.3e12, 'bf16
 # the harmonic product, 1 FMA: 8 flops per term) is reported beside it.
 SYNTH_FLOPS = 2.0
 SYNTH_FLOPS_ISSUED = 8.0
-# digit products per element of the int8 contraction (pairs i + j <= 5 of 4 digits each)
-INT8_PRODUCTS = 10
+# digit products per element of the int8 contraction (pairs i + j <= 6 of 5 digits each)
+INT8_PRODUCTS = 15
 GATE_EH = 1e-6   # |dE_trace| of a mixed mode against the FP64 mode, in Hartree
 METRIC = 'grid Fock builds/s + % roofline, mixed vs FP64 on 1 MI355X at |dE|<=1uEh'
 

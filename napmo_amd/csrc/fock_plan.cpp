@@ -260,7 +260,7 @@ void FockPlan::synth_self(int slot, bool with_scale) {
 void FockPlan::ozaki_setup() {
   hipStream_t st = stream;
   oz = ozaki_layout(grid.N, n, ld, oz_alpha, max_lds_per_block(), oz_chunk);
-  oz_planes.resize((size_t)4 * ld * oz.Nq);
+  oz_planes.resize((size_t)OZ_DIGITS * ld * oz.Nq);
   oz_eA.resize((size_t)ld * oz.nchunk);
   oz_maxd1.resize((size_t)ld * (oz.Nq / OZ_STAGE));
   oz_vs.resize(oz.Nq);

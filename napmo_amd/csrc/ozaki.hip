@@ -6,6 +6,13 @@
 // Operands come from LDS planes [digit][function][64 points] (64 B per function, its four
 // 16-byte segments stored in the order q ^ ((f >> 2) & 3), so the 16 functions of a block
 // read one segment each without bank conflicts).
+//
+// Digit split of J (|J| <= 2^34, integer-valued): the low part lo = J - 2^14 h with
+// h = floor((J + 2^13) / 2^14), so lo in [-2^13, 2^13) and |h| <= 2^20; then
+//   d1 = floor((h + 2^13) / 2^14), r = h - 2^14 d1, d2 = floor((r + 64) / 128), d3 = r - 128 d2,
+//   d4 = floor((lo + 64) / 128), d5 = lo - 128 d4,
+// every digit in [-64, 64]. h and lo come from FP64 arithmetic that is exact on these values,
+// the rest is int32.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,14 +35,21 @@ __host__ __device__ inline int pow2_ceil_exp(double m) {
   return mant == 0.5 ? ex - 1 : ex;
 }
 
-// J in [-2^27, 2^27] -> balanced base-128 digits, each in [-64, 64].
-__device__ inline void split4(int J, int &d1, int &d2, int &d3, int &d4) {
-  d1 = (J + (1 << 20)) >> 21;
-  int r = J - d1 * (1 << 21);
-  d2 = (r + (1 << 13)) >> 14;
-  r -= d2 * (1 << 14);
-  d3 = (r + (1 << 6)) >> 7;
-  d4 = r - d3 * (1 << 7);
+// J integer-valued, |J| <= 2^34 -> (h, lo) with J = 2^14 h + lo, lo in [-2^13, 2^13).
+__device__ inline void split_hl(double J, int &h, int &lo) {
+  const double hd = floor(fma(J, 0x1p-14, 0.5));
+  h = (int)hd;
+  lo = (int)fma(-hd, 16384.0, J);
+}
+
+// (h, lo) -> the five digits d[0..4] (d1..d5).
+__device__ inline void split5(int h, int lo, int d[OZ_DIGITS]) {
+  d[0] = (h + (1 << 13)) >> 14;
+  const int r = h - d[0] * (1 << 14);
+  d[1] = (r + 64) >> 7;
+  d[2] = r - d[1] * 128;
+  d[3] = (lo + 64) >> 7;
+  d[4] = lo - d[3] * 128;
 }
 
 __device__ inline int sbyte(unsigned w, int b) { return (int)(signed char)(w >> (8 * b)); }
@@ -81,24 +95,24 @@ __global__ void __launch_bounds__(256) k_oz_digits(const double *__restrict__ ph
   const int fl = tid >> 2, q = tid & 3, f = f0 + fl;
   const int c = (int)(p0 / chunk);
   const int e = eA[(size_t)f * nchunk + c];
-  unsigned w[4][4];
+  unsigned w[OZ_DIGITS][4];
   int m1 = 0;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    const double x = ldexp(tile[16 * q + k][fl], 27 - e);
-    const int J = (int)rint(x);
-    int d1, d2, d3, d4;
-    split4(J, d1, d2, d3, d4);
-    m1 = max(m1, abs(d1));
+    const double J = rint(ldexp(tile[16 * q + k][fl], 34 - e));
+    int h, lo, d[OZ_DIGITS];
+    split_hl(J, h, lo);
+    split5(h, lo, d);
+    m1 = max(m1, abs(d[0]));
     const int b = k & 3;
-    if (b == 0) w[0][k >> 2] = w[1][k >> 2] = w[2][k >> 2] = w[3][k >> 2] = 0u;
-    w[0][k >> 2] |= (unsigned)(d1 & 0xFF) << (8 * b);
-    w[1][k >> 2] |= (unsigned)(d2 & 0xFF) << (8 * b);
-    w[2][k >> 2] |= (unsigned)(d3 & 0xFF) << (8 * b);
-    w[3][k >> 2] |= (unsigned)(d4 & 0xFF) << (8 * b);
+#pragma unroll
+    for (int x = 0; x < OZ_DIGITS; ++x) {
+      if (b == 0) w[x][k >> 2] = 0u;
+      w[x][k >> 2] |= (unsigned)(d[x] & 0xFF) << (8 * b);
+    }
   }
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < OZ_DIGITS; ++k) {
     u4 v = {w[k][0], w[k][1], w[k][2], w[k][3]};
     *(u4 *)&planes[((size_t)k * ld + f) * Nq + p0 + 16 * q] = v;
   }
@@ -131,20 +145,29 @@ __device__ inline v4i mfma8(v4i a, v4i b, v4i c) {
   return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
 }
 
-// One 16 x 16 block: rows r[0..3] (digits of B, the v-weighted side), columns c[0..3].
-__device__ inline void block_products(const v4i r[4], const v4i c[4], v4i &W, v4i &X) {
+// One 16 x 16 block: rows r[0..4] (digits of B, the v-weighted side), columns c[0..4]; the
+// 15 digit products of levels 2..6 (1-based) added exactly into W, X and Y.
+__device__ inline void block_products(const v4i r[OZ_DIGITS], const v4i c[OZ_DIGITS], v4i &W, v4i &X,
+                                      v4i &Y) {
   const v4i z = {0, 0, 0, 0};
-  v4i t = mfma8(r[0], c[0], z);
+  v4i t = mfma8(r[0], c[0], z);            // T2
   t = t * 128;
-  t = mfma8(r[0], c[1], t);
+  t = mfma8(r[0], c[1], t);                // + T3
   t = mfma8(r[1], c[0], t);
   W += t;
-  v4i u = mfma8(r[0], c[3], z);
+  v4i u = mfma8(r[0], c[3], z);            // T5
   u = mfma8(r[1], c[2], u);
   u = mfma8(r[2], c[1], u);
   u = mfma8(r[3], c[0], u);
-  u = (u + 64) >> 7;
-  u = mfma8(r[0], c[2], u);
+  const v4i q5 = (u + 64) >> 7;
+  v4i y = (u - q5 * 128) * 128;            // + T6
+  y = mfma8(r[0], c[4], y);
+  y = mfma8(r[1], c[3], y);
+  y = mfma8(r[2], c[2], y);
+  y = mfma8(r[3], c[1], y);
+  y = mfma8(r[4], c[0], y);
+  Y += y;
+  u = mfma8(r[0], c[2], q5);               // T4 + q5
   u = mfma8(r[1], c[1], u);
   u = mfma8(r[2], c[0], u);
   X += u;
@@ -160,8 +183,8 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
   const int r0 = a.row0[g], r1 = a.row0[g + 1];
   const int nA = 16 * (a.nb - r0), nB = 16 * (r1 - r0);
   char *ldsA = lds;
-  char *ldsB = lds + (size_t)4 * nA * 64;
-  double *vsS = (double *)(ldsB + (size_t)4 * nB * 64);   // [OZ_STAGE]
+  char *ldsB = lds + (size_t)OZ_DIGITS * nA * 64;
+  double *vsS = (double *)(ldsB + (size_t)OZ_DIGITS * nB * 64);   // [OZ_STAGE]
   double *scB = vsS + OZ_STAGE;                            // [nB]
   int *eRow = (int *)(scB + nB);                           // [nB]
   double *maxv = (double *)(eRow + nB + (nB & 1));          // [OZ_MAX_STAGES]
@@ -197,9 +220,9 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
     eRow[fl] = a.eA[(size_t)f * a.nchunk + c] + eV + eB;
   }
 
-  v4i W[2 * OZ_MAXC], X[2 * OZ_MAXC];
+  v4i W[2 * OZ_MAXC], X[2 * OZ_MAXC], Y[2 * OZ_MAXC];
 #pragma unroll
-  for (int k = 0; k < 2 * OZ_MAXC; ++k) W[k] = X[k] = (v4i){0, 0, 0, 0};
+  for (int k = 0; k < 2 * OZ_MAXC; ++k) W[k] = X[k] = Y[k] = (v4i){0, 0, 0, 0};
   const int *task = a.tasks + ((size_t)(g * OZ_WAVES + wave) * OZ_MAXC) * 3;
   const int r16 = lane & 15, q = lane >> 4;
 
@@ -212,37 +235,41 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
     for (int it = tid; it < nA * 4; it += 64 * OZ_WAVES) {
       const int fl = it >> 2, sq = it & 3;
       const int f = 16 * r0 + fl;
-      u4 d[4];
+      u4 d[OZ_DIGITS];
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
+      for (int k = 0; k < OZ_DIGITS; ++k)
         d[k] = *(const u4 *)&a.planes[((size_t)k * a.ld + f) * a.Nq + p0 + 16 * sq];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) *(u4 *)&ldsA[seg_off(nA, k, fl, sq)] = d[k];
+      for (int k = 0; k < OZ_DIGITS; ++k) *(u4 *)&ldsA[seg_off(nA, k, fl, sq)] = d[k];
       if (fl < nB) {
+        // JB = rint(JA vs 2^-(eV + eB)), JA = 2^14 (2^14 d1 + 2^7 d2 + d3) + 2^7 d4 + d5
         const double sc = scB[fl];
-        u4 o[4];
 #pragma unroll
         for (int wd = 0; wd < 4; ++wd) {
-          int b[4][4];
+          int b[OZ_DIGITS][4];
 #pragma unroll
           for (int by = 0; by < 4; ++by) {
-            const int JA = sbyte(d[0][wd], by) * (1 << 21) + sbyte(d[1][wd], by) * (1 << 14) +
-                           sbyte(d[2][wd], by) * (1 << 7) + sbyte(d[3][wd], by);
+            const int hi = sbyte(d[0][wd], by) * (1 << 14) + sbyte(d[1][wd], by) * 128 + sbyte(d[2][wd], by);
+            const int lo = sbyte(d[3][wd], by) * 128 + sbyte(d[4][wd], by);
+            const double JA = fma((double)hi, 16384.0, (double)lo);
             const double y = vsS[16 * sq + 4 * wd + by] * sc;
-            const int JB = (int)rint((double)JA * y);
-            split4(JB, b[0][by], b[1][by], b[2][by], b[3][by]);
+            const double JB = rint(JA * y);
+            int h, l, e[OZ_DIGITS];
+            split_hl(JB, h, l);
+            split5(h, l, e);
+#pragma unroll
+            for (int k = 0; k < OZ_DIGITS; ++k) b[k][by] = e[k];
           }
 #pragma unroll
-          for (int k = 0; k < 4; ++k) o[k][wd] = pack4(b[k][0], b[k][1], b[k][2], b[k][3]);
+          for (int k = 0; k < OZ_DIGITS; ++k)
+            *(unsigned *)&ldsB[seg_off(nB, k, fl, sq) + 4 * wd] = pack4(b[k][0], b[k][1], b[k][2], b[k][3]);
         }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) *(u4 *)&ldsB[seg_off(nB, k, fl, sq)] = o[k];
       }
     }
     __syncthreads();
     // ---- MFMA phase: this wave's column items (up to two blocks each)
     int curI0 = -2, curI1 = -2;
-    v4i ra[4], rb[4];
+    v4i ra[OZ_DIGITS], rb[OZ_DIGITS];
 #pragma unroll
     for (int k = 0; k < OZ_MAXC; ++k) {
       const int I0 = __builtin_amdgcn_readfirstlane(task[3 * k]);
@@ -252,23 +279,23 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
       if (I0 != curI0) {
         const int fl = 16 * (I0 - r0) + r16;
 #pragma unroll
-        for (int d = 0; d < 4; ++d) ra[d] = *(const v4i *)&ldsB[seg_off(nB, d, fl, q)];
+        for (int d = 0; d < OZ_DIGITS; ++d) ra[d] = *(const v4i *)&ldsB[seg_off(nB, d, fl, q)];
         curI0 = I0;
       }
       if (I1 >= 0 && I1 != curI1) {
         const int fl = 16 * (I1 - r0) + r16;
 #pragma unroll
-        for (int d = 0; d < 4; ++d) rb[d] = *(const v4i *)&ldsB[seg_off(nB, d, fl, q)];
+        for (int d = 0; d < OZ_DIGITS; ++d) rb[d] = *(const v4i *)&ldsB[seg_off(nB, d, fl, q)];
         curI1 = I1;
       }
-      v4i cf[4];
+      v4i cf[OZ_DIGITS];
       {
         const int fl = 16 * (J - r0) + r16;
 #pragma unroll
-        for (int d = 0; d < 4; ++d) cf[d] = *(const v4i *)&ldsA[seg_off(nA, d, fl, q)];
+        for (int d = 0; d < OZ_DIGITS; ++d) cf[d] = *(const v4i *)&ldsA[seg_off(nA, d, fl, q)];
       }
-      block_products(ra, cf, W[2 * k], X[2 * k]);
-      if (I1 >= 0 && J >= I1) block_products(rb, cf, W[2 * k + 1], X[2 * k + 1]);
+      block_products(ra, cf, W[2 * k], X[2 * k], Y[2 * k]);
+      if (I1 >= 0 && J >= I1) block_products(rb, cf, W[2 * k + 1], X[2 * k + 1], Y[2 * k + 1]);
     }
   }
   // ---- epilogue: scaled FP64 partial of every block of this wave
@@ -287,7 +314,8 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = 4 * q + r;
-        const double val = (double)W[2 * k + h][r] * 0x1p-19 + (double)X[2 * k + h][r] * 0x1p-26;
+        const double val = ((double)W[2 * k + h][r] * 0x1p-19 + (double)X[2 * k + h][r] * 0x1p-26) +
+                           (double)Y[2 * k + h][r] * 0x1p-40;
         part[(size_t)blk * 256 + m * 16 + r16] = ldexp(val, eRow[16 * (I - r0) + m] + ec);
       }
     }
@@ -353,7 +381,7 @@ static bool pack_waves(int r0, int r1, int nb, std::vector<int> &tab) {
 
 static size_t group_lds(int r0, int r1, int nb) {
   const size_t nA = 16 * (size_t)(nb - r0), nB = 16 * (size_t)(r1 - r0);
-  return 4 * (nA + nB) * 64 + OZ_STAGE * 8 + nB * 12 + 8 + OZ_MAX_STAGES * 8 + 16;
+  return OZ_DIGITS * (nA + nB) * 64 + OZ_STAGE * 8 + nB * 12 + 8 + OZ_MAX_STAGES * 8 + 16;
 }
 
 OzakiLayout ozaki_layout(long N, int n, int ld, double alpha, size_t max_lds, long chunk_target) {

@@ -2,15 +2,18 @@
 //
 // F = Phi^T diag(w bw v) Phi = At^T B with At = sqrt|w bw| Phi (per geometry) and
 // B = sign(w bw) v At (per build). Both operands are written, per (function, chunk of
-// points), as a 28-bit fixed-point integer J in [-2^27, 2^27] relative to a power-of-two
-// scale, split into four balanced base-128 digits d1..d4 in [-64, 64]:
-//   J = d1 2^21 + d2 2^14 + d3 2^7 + d4.
-// The products of digit pairs (i, j) with i + j <= 5 (1-based) run on
-// v_mfma_i32_16x16x64_i8, whose int32 accumulation is exact; per 64-point stage
-//   W += (T2 << 7) + T3,  X += T4 + round(T5 / 128)      (T_L = sum over i + j = L)
-// so a chunk's block is 2^(e_row + e_col) (W 2^-19 + X 2^-26). Chunks are summed in FP64
-// in a fixed order. The only rounding is the 28-bit representation of both operands and
-// the dropped digit products below 2^-28 of the leading one.
+// points), as a 35-bit fixed-point integer J in [-2^34, 2^34] relative to a power-of-two
+// scale, split into five balanced base-128 digits d1..d5 in [-64, 64]:
+//   J = d1 2^28 + d2 2^21 + d3 2^14 + d4 2^7 + d5.
+// The products of digit pairs (i, j) with i + j <= 6 (1-based, 15 of the 25) run on
+// v_mfma_i32_16x16x64_i8, whose int32 accumulation is exact; per 64-point stage, with T_L the
+// sum of the level-L products and q5 = floor((T5 + 64) / 128),
+//   W += 128 T2 + T3,   X += T4 + q5,   Y += 128 (T5 - 128 q5) + T6
+// (all exact), so a chunk's block is 2^(e_row + e_col) ((W 2^-19 + X 2^-26) + Y 2^-40). Chunks
+// are summed in FP64 in a fixed order. The only roundings are the 35-bit representation of
+// both operands and the dropped products of level >= 7 (below 2^-35 of the leading one); the
+// energy of the returned F stays within 1e-6 Eh of FP64 on molecules whose near-nucleus shells
+// make 28-bit digits miss it by up to 7e-5 Eh.
 #pragma once
 
 #include "common.hpp"
@@ -18,9 +21,10 @@
 namespace napmo {
 
 constexpr int OZ_STAGE = 64;      // points per stage (the MFMA k depth)
-constexpr int OZ_MAXC = 8;        // column items per wave (each up to two 16 x 16 blocks)
+constexpr int OZ_DIGITS = 5;      // base-128 digits per operand
+constexpr int OZ_MAXC = 4;        // column items per wave (each up to two 16 x 16 blocks)
 constexpr int OZ_WAVES = 8;       // waves per workgroup
-constexpr int OZ_MAX_STAGES = 63; // stages per chunk: keeps W inside int32
+constexpr int OZ_MAX_STAGES = 63; // stages per chunk: keeps W, X and Y inside int32
 
 // Per-plan layout: chunks of points, 16-function blocks, groups of block rows.
 struct OzakiLayout {
@@ -42,7 +46,7 @@ struct OzakiLayout {
 OzakiLayout ozaki_layout(long N, int n, int ld, double alpha, size_t max_lds, long chunk_target = 0);
 
 // Per-geometry operand: digits of At = sqrt|wbw| Phi. phi64 [Npad x ld] point-major (Npad >= N),
-// wbw [N]; out: planes [4][ld][Nq] int8 function-major, eA [ld][nchunk] int, maxd1
+// wbw [N]; out: planes [OZ_DIGITS][ld][Nq] int8 function-major, eA [ld][nchunk] int, maxd1
 // [ld][Nq / OZ_STAGE] uint8 (largest |d1| per stage).
 void launch_ozaki_setup(const OzakiLayout &L, const double *phi64, const double *wbw,
                         signed char *planes, int *eA, unsigned char *maxd1, hipStream_t s);

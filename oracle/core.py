@@ -1354,24 +1354,35 @@ def _pow2_ceil_exp(m):
     return np.where(m > 0, e, 0).astype(np.int64)
 
 
-def _split4(J):
-    """J in [-2^27, 2^27] (int64) -> balanced base-128 digits d1..d4 in [-64, 64]."""
-    d1 = (J + (1 << 20)) >> 21
-    r = J - d1 * (1 << 21)
-    d2 = (r + (1 << 13)) >> 14
-    r = r - d2 * (1 << 14)
-    d3 = (r + (1 << 6)) >> 7
-    d4 = r - d3 * (1 << 7)
-    return d1, d2, d3, d4
+def _split5(J):
+    """J in [-2^34, 2^34] (int64) -> five balanced base-128 digits, J = sum d_k 2^(7 (5 - k)),
+    split as the kernel does: h = floor((J + 2^13) / 2^14), lo = J - 2^14 h in [-2^13, 2^13);
+    d1 = floor((h + 2^13) / 2^14), r = h - 2^14 d1, d2 = floor((r + 64) / 128), d3 = r - 128 d2,
+    d4 = floor((lo + 64) / 128), d5 = lo - 128 d4 (each in [-64, 64])."""
+    h = (J + (1 << 13)) >> 14
+    lo = J - (h << 14)
+    d1 = (h + (1 << 13)) >> 14
+    r = h - (d1 << 14)
+    d2 = (r + 64) >> 7
+    d3 = r - (d2 << 7)
+    d4 = (lo + 64) >> 7
+    d5 = lo - (d4 << 7)
+    return [d1, d2, d3, d4, d5]
 
 
 def int8_fock(phi, wbw, v, chunk, stage=64):
-    """F = Phi^T diag(wbw v) Phi as the int8 MFMA kernel computes it: At = sqrt|wbw| Phi as
-    28-bit digits per (function, chunk) at the chunk's power-of-two maximum; B = sign(wbw) v At
-    as 28-bit digits at the bound 2^-6 max_stages(max|vs| 2^-eV (max|d1| + 1)); per stage of 64
-    points W += T2 128 + T3, X += T4 + floor((T5 + 64) / 128) with T_L the exact sum of the
-    digit products of level L; chunk value 2^(e_row + e_col) (W 2^-19 + X 2^-26); chunks
-    summed in order. phi [N, n]."""
+    """F = Phi^T diag(wbw v) Phi as the int8 MFMA kernel computes it (napmo_amd/csrc/ozaki.hip).
+
+    At = sqrt|wbw| Phi is written per (function, chunk) as a 35-bit integer at the chunk's
+    power-of-two maximum (J = rint(At 2^(34 - eA)), |J| <= 2^34); B = sign(wbw) v At as
+    JB = rint(JA vs 2^-(eV + eB)) with eV the exponent of the chunk's max|vs| and eB that of the
+    bound 2^-6 max_stages(max|vs| 2^-eV (max|d1| + 1)), so |JB| <= 2^34. Both split into five
+    balanced base-128 digits; per stage of 64 points the digit products of level L = i + j
+    (1-based) <= 6 are summed exactly (T_L) and accumulated in three integers:
+        W += 128 T2 + T3,   X += T4 + q5,   Y += 128 (T5 - 128 q5) + T6,   q5 = floor((T5 + 64) / 128)
+    The chunk's block is 2^(e_row + e_col) ((W 2^-19 + X 2^-26) + Y 2^-40); chunks are summed
+    in order. Products of level >= 7 (below 2^-35 of the leading one) are the only ones
+    dropped. phi [N, n]."""
     N, n = phi.shape
     nchunk = -(-N // chunk)
     Nq = nchunk * chunk
@@ -1384,8 +1395,8 @@ def int8_fock(phi, wbw, v, chunk, stage=64):
         sl = slice(c * chunk, (c + 1) * chunk)
         Gc = G[sl]
         eA = _pow2_ceil_exp(np.abs(Gc).max(axis=0))
-        JA = np.rint(np.ldexp(Gc, (27 - eA)[None, :])).astype(np.int64)
-        dA = _split4(JA)
+        JA = np.rint(np.ldexp(Gc, (34 - eA)[None, :])).astype(np.int64)
+        dA = _split5(JA)
         vc = vs[sl]
         ns = chunk // stage
         maxv = np.abs(vc).reshape(ns, stage).max(axis=1)
@@ -1399,20 +1410,22 @@ def int8_fock(phi, wbw, v, chunk, stage=64):
         scB = np.ldexp(1.0, -eV - eB)
         y = vc[:, None] * scB[None, :]
         JB = np.rint(JA.astype(np.float64) * y).astype(np.int64)
-        dB = _split4(JB)
-        W = np.zeros((n, n))
-        X = np.zeros((n, n))
+        dB = _split5(JB)
+        W = np.zeros((n, n), dtype=np.int64)
+        X = np.zeros((n, n), dtype=np.int64)
+        Y = np.zeros((n, n), dtype=np.int64)
         for t in range(ns):
             st = slice(t * stage, (t + 1) * stage)
             r = [d[st].astype(np.float64) for d in dB]   # rows: B digits
             a = [d[st].astype(np.float64) for d in dA]   # columns: A digits
-            T2 = r[0].T @ a[0]
-            T3 = r[0].T @ a[1] + r[1].T @ a[0]
-            T4 = r[0].T @ a[2] + r[1].T @ a[1] + r[2].T @ a[0]
-            T5 = r[0].T @ a[3] + r[1].T @ a[2] + r[2].T @ a[1] + r[3].T @ a[0]
-            W += T2 * 128 + T3
-            X += T4 + np.floor((T5 + 64) / 128)
-        val = W * 2.0 ** -19 + X * 2.0 ** -26
+            T = {L: sum(r[i].T @ a[L - 2 - i] for i in range(L - 1) if L - 2 - i < 5 and i < 5)
+                 for L in range(2, 7)}
+            T = {L: np.rint(x).astype(np.int64) for L, x in T.items()}   # exact integers
+            q5 = (T[5] + 64) >> 7
+            W += 128 * T[2] + T[3]
+            X += T[4] + q5
+            Y += 128 * (T[5] - 128 * q5) + T[6]
+        val = (W * 2.0 ** -19 + X * 2.0 ** -26) + Y * 2.0 ** -40
         F = F + np.ldexp(val, (eA + eV + eB)[:, None] + eA[None, :])
     U = np.triu(F)
     return U + np.triu(U, 1).T

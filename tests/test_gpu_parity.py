@@ -241,12 +241,14 @@ def test_grid_fock_chon16_properties():
     assert np.array_equal(F, F.T)
     assert abs(E - Egrid) < 1e-9 * abs(E)
     assert E > 0
-    # mixed mode on the same system (one-pass BF16x3 contraction): energy unchanged,
-    # F within the stated tolerance
+    # BF16x3 on the same system (one-pass contraction). E_grid comes from the FP64 rho and v
+    # and cannot see F; the energy of the F it returns, 1/2 tr(P F), moves by about 2.2e-4 Eh
+    # here, so this mode does not meet 1e-6 Eh (the int8 mode does, tests/test_gpu_int8.py)
     mix = napmo_amd.GridFock(g, b, lmax, 'bf16x3')
     F2, E2, Eg2 = mix.build(P)
     print('chon16 bf16x3 max|dF| %.3e dE_grid %.3e dE_trace %.3e' % (np.abs(F2 - F).max(), Eg2 - Egrid, E2 - E))
     assert abs(Eg2 - Egrid) < 1e-9
+    assert abs(E2 - E) < 1e-3
     assert np.abs(F2 - F).max() < 2e-5 * np.abs(F).max()
 
 
@@ -332,8 +334,9 @@ def test_orbital_density_streamed_kernel_matches_staged():
 
 
 def test_grid_fock_bf16x3_mixed_mode():
-    # mixed mode: rho, Poisson and the energy trace stay FP64, only F uses the 3-term
-    # bf16 split. Stated tolerances: |dE_grid| <= 1e-10 Eh, max|dF| <= 2e-5 max|F|.
+    # BF16x3: rho, Poisson and E_grid stay FP64, only F uses the 3-term bf16 split with FP32
+    # accumulation. Stated tolerances: max|dF| <= 2e-5 max|F| and |1/2 tr(P dF)| <= 1e-5 |E|
+    # (the energy of the returned F; E_grid is FP64 by construction and checked to 1e-10).
     import napmo_amd
     for atoms, shells, nr, na, lmax in (
             (systems.H2_ATOMS, systems.named_shells(systems.H2_ATOMS, 'CC-PVDZ'), 100, 110, 10),
@@ -349,6 +352,7 @@ def test_grid_fock_bf16x3_mixed_mode():
         print('bf16x3 n=%d max|dF|=%.3e max|F|=%.3e dE_trace=%.3e dE_grid=%.3e' %
               (b['nbasis'], dF, np.abs(F1).max(), E2 - E1, Eg2 - Eg1))
         assert abs(Eg2 - Eg1) < 1e-10
+        assert abs(E2 - E1) <= 1e-5 * abs(E1)
         assert dF < 2e-5 * np.abs(F1).max()
         assert np.array_equal(F2, F2.T)
 
