@@ -434,8 +434,8 @@ enum {
                                    0 uses the LDS-staged kernels */
   NAPMO_TUNE_INT8_CHUNK = 7,  /* int8 mode: points per chunk (a multiple of 64, at most 4032;
                                  0 = about N / 256). Setting it re-slices the per-geometry digits */
-  NAPMO_TUNE_INT8_ALPHA = 8   /* int8 mode: weight (x 100) of a block row's per-build digit
-                                 formation against its blocks in the row-group split; 0 = 1000 */
+  NAPMO_TUNE_INT8_WAVE_BLOCKS = 8 /* int8 mode: 16 x 16 blocks of F per wave in the row-group
+                                     packing, 1..5; 0 = 5 (fewer: more, smaller row groups) */
 };
 int napmo_fock_plan_set_tuning(void *plan, int key, long value);
 int napmo_fock_plan_get_tuning(void *plan, int key, long *value);

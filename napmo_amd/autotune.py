@@ -20,12 +20,12 @@ import time
 from napmo_amd.system import cext
 
 (TUNE_FOCK_WG, TUNE_PASS_WG, TUNE_USE_PASS, TUNE_DENSITY_KS, TUNE_DENSITY_PF, TUNE_SYNTH_NP,
- TUNE_DENSITY_STREAM, TUNE_INT8_CHUNK, TUNE_INT8_ALPHA) = range(9)
+ TUNE_DENSITY_STREAM, TUNE_INT8_CHUNK, TUNE_INT8_WAVE_BLOCKS) = range(9)
 _KEYS = {'fock_wg': TUNE_FOCK_WG, 'pass_wg': TUNE_PASS_WG, 'use_pass': TUNE_USE_PASS,
          'density_ks': TUNE_DENSITY_KS, 'density_pf': TUNE_DENSITY_PF, 'synth_np': TUNE_SYNTH_NP,
          'density_stream': TUNE_DENSITY_STREAM, 'int8_chunk': TUNE_INT8_CHUNK,
-         'int8_alpha': TUNE_INT8_ALPHA}
-_INT8_KEYS = ('int8_chunk', 'int8_alpha')   # only int8 plans have them
+         'int8_wave_blocks': TUNE_INT8_WAVE_BLOCKS}
+_INT8_KEYS = ('int8_chunk', 'int8_wave_blocks')   # only int8 plans have them
 DEFAULTS = {'fock_wg': 0, 'pass_wg': 0, 'density_ks': 16, 'density_pf': 1,
             'synth_np': 2, 'density_stream': 1}   # 0: the library's default shape
 
@@ -44,12 +44,11 @@ def shape_key(fock):
 def candidates(fock):
     """Launch shapes of the Fock contraction, each with both density stage depths."""
     if fock.precision == 'int8':
-        # points per chunk (about N / 256 .. N / 512 chunks, at most 1920 points: longer chunks
-        # widen the range each function's fixed-point scale must cover, and the 16-atom
-        # system's |dE_trace| grew to 5e-7 Eh at 3712) and the row-group balance
+        # points per chunk (about N / 128 .. N / 512 chunks; every chunk is one workgroup per
+        # row group and one FP64 partial of F) and the blocks per wave of the row-group packing
         N = fock.npoint
-        chunks = sorted({min(1920, max(64, -(-N // k) + 63) // 64 * 64) for k in (256, 320, 384, 512)})
-        out = [{'int8_chunk': c, 'int8_alpha': a} for c in chunks for a in (500, 1000, 2000)]
+        chunks = sorted({min(4032, max(64, -(-N // k) + 63) // 64 * 64) for k in (128, 192, 256, 384, 512)})
+        out = [{'int8_chunk': c, 'int8_wave_blocks': b} for c in chunks for b in (3, 4, 5)]
     elif fock.precision == 'bf16x3':
         out = [{'use_pass': 1, 'pass_wg': w} for w in (128, 256, 512, 1024)]
         # the symmetric form only with short chunks (its FP32 chunk sums drift otherwise)
@@ -129,6 +128,8 @@ def tune(fock, dP, dF, dE, reps=5, path=None, force=False, build=None, tag=''):
             scored.append((cand, timed(cand, reps)))
         except cext.NapmoError:
             continue
+    if not scored:
+        raise cext.NapmoError('autotune: no launch shape applies to this plan')
     # the three fastest again with twice the builds (single timings differ by a few %)
     scored.sort(key=lambda r: r[1])
     best = min(((c, timed(c, 2 * reps)) for c, _ in scored[:3]), key=lambda r: r[1])

@@ -1,6 +1,10 @@
 // Multi-species (APMO) grid Coulomb build (see apmo_plan.hpp).
 #include "apmo_plan.hpp"
 
+#include <algorithm>
+#include <cmath>
+#include <string>
+
 namespace napmo {
 
 // E[2] = q^2 * E2[1] (own Coulomb energy), E[3] = E[1] - E[2] (coupling energy).
@@ -105,6 +109,10 @@ void ApmoPlan::build_host(const double *const *P, double *const *F_out, double *
   std::vector<double *> pF(ns);
   for (int s = 0; s < ns; ++s) {
     FockPlan &p = *sp[s].plan;
+    for (size_t i = 0; i < (size_t)p.n * p.n; ++i)
+      NAPMO_REQUIRE(std::isfinite(P[s][i]), "APMO build: P of species " + std::to_string(s) +
+                                                " holds a non-finite value at row " + std::to_string(i / p.n) +
+                                                ", column " + std::to_string(i % p.n));
     dP[s].upload(P[s], (size_t)p.n * p.n, p.stream);
     pP[s] = dP[s].get();
     pF[s] = p.F.get();
@@ -115,8 +123,12 @@ void ApmoPlan::build_host(const double *const *P, double *const *F_out, double *
     FockPlan &p = *sp[s].plan;
     if (F_out && F_out[s]) p.F.download(F_out[s], (size_t)p.n * p.n, p.stream);
   }
-  if (E_out) E.download(E_out, 4 * ns, sp[ns - 1].plan->stream);
+  std::vector<double> Eh(4 * ns);
+  E.download(Eh.data(), 4 * ns, sp[ns - 1].plan->stream);
   synchronize();
+  for (double e : Eh)
+    NAPMO_REQUIRE(std::isfinite(e), "APMO build: non-finite energy (a density or potential is not finite)");
+  if (E_out) std::copy(Eh.begin(), Eh.end(), E_out);
 }
 
 void ApmoPlan::synchronize() {

@@ -110,16 +110,18 @@ void FockPlan::set_tuning(int key, long value) {
       density_stream = (int)value;
       break;
     case NAPMO_TUNE_INT8_CHUNK:
-    case NAPMO_TUNE_INT8_ALPHA:
+    case NAPMO_TUNE_INT8_WAVE_BLOCKS:
       NAPMO_REQUIRE(prec == NAPMO_PREC_INT8, "int8 tuning keys need an NAPMO_PREC_INT8 plan");
       NAPMO_REQUIRE(key != NAPMO_TUNE_INT8_CHUNK || value == 0 ||
                         (value % OZ_STAGE == 0 && value <= (long)OZ_MAX_STAGES * OZ_STAGE),
                     "int8 chunk must be a multiple of 64 points, at most 4032");
+      NAPMO_REQUIRE(key != NAPMO_TUNE_INT8_WAVE_BLOCKS || (value >= 0 && value <= OZ_MAXB),
+                    "int8 blocks per wave must be 1 to " + std::to_string(OZ_MAXB) + " (0: the most)");
       NAPMO_HIP(hipStreamSynchronize(stream));
       if (key == NAPMO_TUNE_INT8_CHUNK)
         oz_chunk = value;
       else
-        oz_alpha = value ? value / 100.0 : 10.0;
+        oz_wave_blocks = value ? (int)value : OZ_MAXB;
       ozaki_setup();
       break;
     default: throw Error("unknown tuning key " + std::to_string(key));
@@ -138,7 +140,7 @@ long FockPlan::get_tuning(int key) const {
     case NAPMO_TUNE_SYNTH_NP: return grid.synth_np;
     case NAPMO_TUNE_DENSITY_STREAM: return density_stream;
     case NAPMO_TUNE_INT8_CHUNK: return prec == NAPMO_PREC_INT8 ? oz.chunk : 0;
-    case NAPMO_TUNE_INT8_ALPHA: return prec == NAPMO_PREC_INT8 ? (long)std::lround(oz_alpha * 100) : 0;
+    case NAPMO_TUNE_INT8_WAVE_BLOCKS: return prec == NAPMO_PREC_INT8 ? oz_wave_blocks : 0;
     default: throw Error("unknown tuning key " + std::to_string(key));
   }
 }
@@ -259,7 +261,7 @@ void FockPlan::synth_self(int slot, bool with_scale) {
 
 void FockPlan::ozaki_setup() {
   hipStream_t st = stream;
-  oz = ozaki_layout(grid.N, n, ld, oz_alpha, max_lds_per_block(), oz_chunk);
+  oz = ozaki_layout(grid.N, n, ld, oz_wave_blocks, max_lds_per_block(), oz_chunk);
   oz_planes.resize((size_t)OZ_DIGITS * ld * oz.Nq);
   oz_eA.resize((size_t)ld * oz.nchunk);
   oz_maxd1.resize((size_t)ld * (oz.Nq / OZ_STAGE));
@@ -350,21 +352,45 @@ void FockPlan::build_orbitals_device(const double *dC, int nocc, double eta, dou
   contract(slot, Pfull.get(), v.get(), dF, dE, true);
 }
 
+// Host-pointer builds check their inputs and the energies they return: a non-finite value
+// in P (or C, eta, v) is reported with its position, and a non-finite energy (a density or
+// potential that is not finite, e.g. from a basis with a non-finite exponent or coefficient)
+// is reported instead of being returned.
+static void require_finite(const double *x, size_t count, long ncol, const char *what) {
+  for (size_t i = 0; i < count; ++i)
+    if (!std::isfinite(x[i]))
+      throw Error(std::string(what) + " holds a non-finite value at row " + std::to_string(i / ncol) +
+                  ", column " + std::to_string(i % ncol));
+}
+
+static void require_finite_energy(const double *E) {
+  if (!std::isfinite(E[0]) || !std::isfinite(E[1]))
+    throw Error("grid Fock build: non-finite energy (the density or the potential on the grid is "
+                "not finite)");
+}
+
 void FockPlan::build_orbitals_host(const double *C, int nocc, double eta, double *F_out,
                                    double *E_out, double *rho_out, double *v_out) {
   hipStream_t st = stream;
+  NAPMO_REQUIRE(nocc >= 0, "orbital build: negative orbital count");
+  require_finite(C, (size_t)n * nocc, std::max(nocc, 1), "C");
+  require_finite(&eta, 1, 1, "eta");
   DeviceBuffer<double> dC;
   dC.upload(C, (size_t)n * nocc, st);
   build_orbitals_device(dC.get(), nocc, eta, F.get(), E.get());
+  double Eh[2];
+  E.download(Eh, 2, st);
   if (F_out) F.download(F_out, (size_t)n * n, st);
-  if (E_out) E.download(E_out, 2, st);
   if (rho_out) rho.download(rho_out, grid.N, st);
   if (v_out) v.download(v_out, grid.N, st);
   NAPMO_HIP(hipStreamSynchronize(st));
+  require_finite_energy(Eh);
+  if (E_out) E_out[0] = Eh[0], E_out[1] = Eh[1];
 }
 
 void FockPlan::contract_potential(const double *v_host, double *F_out) {
   hipStream_t st = stream;
+  require_finite(v_host, grid.N, 1, "v");
   DeviceBuffer<double> dv;
   dv.upload(v_host, grid.N, st);
   if (prec == NAPMO_PREC_INT8) {
@@ -403,14 +429,18 @@ void FockPlan::overlap(double *S_out) {
 void FockPlan::build_host(const double *hP, double *F_out, double *E_out, double *rho_out,
                           double *v_out) {
   hipStream_t st = stream;
+  require_finite(hP, (size_t)n * n, n, "P");
   DeviceBuffer<double> dP;
   dP.upload(hP, (size_t)n * n, st);
   build_device(dP.get(), F.get(), E.get());
+  double Eh[2];
+  E.download(Eh, 2, st);
   if (F_out) F.download(F_out, (size_t)n * n, st);
-  if (E_out) E.download(E_out, 2, st);
   if (rho_out) rho.download(rho_out, grid.N, st);
   if (v_out) v.download(v_out, grid.N, st);
   NAPMO_HIP(hipStreamSynchronize(st));
+  require_finite_energy(Eh);
+  if (E_out) E_out[0] = Eh[0], E_out[1] = Eh[1];
 }
 
 }  // namespace napmo

@@ -41,7 +41,7 @@ struct FockPlan {
   DeviceBuffer<unsigned char> oz_maxd1;
   DeviceBuffer<double> oz_vs, oz_part;
   long oz_chunk = 0;       // NAPMO_TUNE_INT8_CHUNK (0: default)
-  double oz_alpha = 10.0;  // NAPMO_TUNE_INT8_ALPHA / 100
+  int oz_wave_blocks = OZ_MAXB;  // NAPMO_TUNE_INT8_WAVE_BLOCKS
   void ozaki_setup();
   void ozaki_contract(const double *veff, double *dF, int slot);
   int use_pass = 0;  // 0: per-tile kernels, 1: one-pass BF16x3, 2: symmetric one-pass BF16x3

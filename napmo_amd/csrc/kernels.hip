@@ -454,8 +454,8 @@ void launch_pair_fill(const AtomTables &at, const double *x, const double *y, co
 
 // One wavefront per 64-pair slot: the geometry each synthesis launch would otherwise form
 // per pair (gathers of the point, distance, unit vector and the inverse radial transform),
-// formed once per grid. Slots past a bucket's pairs (list entry -1) get the centre's own
-// first point, so every lane of the synthesis computes on finite values.
+// formed once per grid. Slots past a bucket's pairs (list entry -1) get a fixed finite
+// geometry (unit distance along x, offset 0 in the interval) whose results are never used.
 __global__ void __launch_bounds__(64) k_pair_geometry(AtomTables at, const double *__restrict__ px,
                                                       const double *__restrict__ py,
                                                       const double *__restrict__ pz,
@@ -470,12 +470,18 @@ __global__ void __launch_bounds__(64) k_pair_geometry(AtomTables at, const doubl
   const int reg = (loc == 0) ? 0 : (loc == nr ? 2 : 1);
   const int j = (reg == 1) ? loc - 1 : 0;
   const int p = list[w * 64 + lane];
-  const long pp = p >= 0 ? p : 0;
+  double *g = geo + w * 256 + lane;
+  if (p < 0) {
+    g[0] = (reg == 1) ? 0.0 : 1.0;
+    g[64] = 1.0;
+    g[128] = 0.0;
+    g[192] = 0.0;
+    return;
+  }
   const double cx = at.center[3 * A], cy = at.center[3 * A + 1], cz = at.center[3 * A + 2];
-  const double dx = px[pp] - cx, dy = py[pp] - cy, dz = pz[pp] - cz;
+  const double dx = px[p] - cx, dy = py[p] - cy, dz = pz[p] - cz;
   const double d = sqrt(dx * dx + dy * dy + dz * dz);
   const double inv_d = 1.0 / d;
-  double *g = geo + w * 256 + lane;
   g[0] = (reg == 1) ? rtf_inv(at.rkind[A], at.rp0[A], at.rp1[A], nr, d) - j : d;
   g[64] = dx * inv_d;
   g[128] = dy * inv_d;

@@ -134,7 +134,7 @@ struct OzArgs {
   const int *eA;
   const unsigned char *maxd1;
   const double *vs;
-  const int *tasks;   // [ngroups][OZ_WAVES][OZ_MAXC][3]
+  const int *tasks;   // [ngroups][OZ_WAVES][OZ_TASK]
   const int *row0;    // [ngroups + 1]
   double *partial;
   long Nq, chunk, pstride;
@@ -145,34 +145,70 @@ __device__ inline v4i mfma8(v4i a, v4i b, v4i c) {
   return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
 }
 
-// One 16 x 16 block: rows r[0..4] (digits of B, the v-weighted side), columns c[0..4]; the
-// 15 digit products of levels 2..6 (1-based) added exactly into W, X and Y.
-__device__ inline void block_products(const v4i r[OZ_DIGITS], const v4i c[OZ_DIGITS], v4i &W, v4i &X,
-                                      v4i &Y) {
-  const v4i z = {0, 0, 0, 0};
-  v4i t = mfma8(r[0], c[0], z);            // T2
-  t = t * 128;
-  t = mfma8(r[0], c[1], t);                // + T3
-  t = mfma8(r[1], c[0], t);
-  W += t;
-  v4i u = mfma8(r[0], c[3], z);            // T5
-  u = mfma8(r[1], c[2], u);
-  u = mfma8(r[2], c[1], u);
-  u = mfma8(r[3], c[0], u);
-  const v4i q5 = (u + 64) >> 7;
-  v4i y = (u - q5 * 128) * 128;            // + T6
-  y = mfma8(r[0], c[4], y);
-  y = mfma8(r[1], c[3], y);
-  y = mfma8(r[2], c[2], y);
-  y = mfma8(r[3], c[1], y);
-  y = mfma8(r[4], c[0], y);
-  Y += y;
-  u = mfma8(r[0], c[2], q5);               // T4 + q5
-  u = mfma8(r[1], c[1], u);
-  u = mfma8(r[2], c[0], u);
-  X += u;
+// One 16 x 16 block: rows r[0..4] (digits of B, the v-weighted side), columns c[0..4]; the 15
+// digit products of levels 2..6 (1-based) chained into one int32 accumulator per level
+// (acc[L - 2] += T_L), so the stage needs no vector instructions.
+__device__ inline void block_products(const v4i r[OZ_DIGITS], const v4i c[OZ_DIGITS], v4i acc[5]) {
+  acc[0] = mfma8(r[0], c[0], acc[0]);
+  acc[1] = mfma8(r[1], c[0], mfma8(r[0], c[1], acc[1]));
+  acc[2] = mfma8(r[2], c[0], mfma8(r[1], c[1], mfma8(r[0], c[2], acc[2])));
+  acc[3] = mfma8(r[3], c[0], mfma8(r[2], c[1], mfma8(r[1], c[2], mfma8(r[0], c[3], acc[3]))));
+  acc[4] = mfma8(r[4], c[0], mfma8(r[3], c[1], mfma8(r[2], c[2], mfma8(r[1], c[3], mfma8(r[0], c[4], acc[4])))));
 }
 
+// Four bytes (the low byte of each) packed little-endian into a word.
+__device__ inline unsigned pack_bytes(int b0, int b1, int b2, int b3) {
+  const unsigned lo = __builtin_amdgcn_perm((unsigned)b1, (unsigned)b0, 0x0c0c0400u);
+  const unsigned hi = __builtin_amdgcn_perm((unsigned)b3, (unsigned)b2, 0x0c0c0400u);
+  return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+}
+
+// B digits of the group's row functions for one stage into an LDS buffer: one (function,
+// 8 points) item per thread and pass. JB = rint(JA vs 2^-(eV + eB)) with
+// JA = 2^14 (2^14 d1 + 2^7 d2 + d3) + 2^7 d4 + d5 read from the digit planes.
+__device__ inline void stage_b_digits(const OzArgs &a, int r0, int nB, long pn, const double *scB,
+                                      char *buf, int tid) {
+  for (int it = tid; it < nB * 8; it += 64 * OZ_WAVES) {
+    const int fl = it >> 3, s8 = it & 7;
+    const int f = 16 * r0 + fl;
+    const long p = pn + 8 * s8;
+    uint2 dg[OZ_DIGITS];
+#pragma unroll
+    for (int k = 0; k < OZ_DIGITS; ++k) dg[k] = *(const uint2 *)&a.planes[((size_t)k * a.ld + f) * a.Nq + p];
+    const double sc = scB[fl];
+    unsigned out[OZ_DIGITS][2];
+#pragma unroll
+    for (int wd = 0; wd < 2; ++wd) {
+      const double2 va = *(const double2 *)&a.vs[p + 4 * wd], vb = *(const double2 *)&a.vs[p + 4 * wd + 2];
+      const double vsv[4] = {va.x, va.y, vb.x, vb.y};
+      const unsigned w0 = wd ? dg[0].y : dg[0].x, w1 = wd ? dg[1].y : dg[1].x, w2 = wd ? dg[2].y : dg[2].x,
+                     w3 = wd ? dg[3].y : dg[3].x, w4 = wd ? dg[4].y : dg[4].x;
+      int d[4][OZ_DIGITS];
+#pragma unroll
+      for (int by = 0; by < 4; ++by) {
+        const int hi = sbyte(w0, by) * (1 << 14) + sbyte(w1, by) * 128 + sbyte(w2, by);
+        const int lo = sbyte(w3, by) * 128 + sbyte(w4, by);
+        const double JA = fma((double)hi, 16384.0, (double)lo);
+        const double JB = rint(JA * (vsv[by] * sc));
+        int h, l;
+        split_hl(JB, h, l);
+        split5(h, l, d[by]);
+      }
+#pragma unroll
+      for (int k = 0; k < OZ_DIGITS; ++k) out[k][wd] = pack_bytes(d[0][k], d[1][k], d[2][k], d[3][k]);
+    }
+#pragma unroll
+    for (int k = 0; k < OZ_DIGITS; ++k)
+      *(uint2 *)&buf[seg_off(nB, k, fl, s8 >> 1) + 8 * (s8 & 1)] = make_uint2(out[k][0], out[k][1]);
+  }
+}
+
+// Workgroup = (row group g, chunk c): block rows [r0, r1) against every column block >= r0.
+// The B digits of the group's rows are formed per stage into a double-buffered LDS region;
+// each wave reads the A digits of its (at most OZ_MAXCOL) column blocks straight from the
+// planes into registers and owns at most OZ_MAXB blocks. The two halves of the workgroup (one
+// wave of each per SIMD) run the stage's MFMAs and the next stage's digit formation in
+// opposite order, so matrix and vector work of a SIMD overlap; one barrier per stage.
 __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
   extern __shared__ __align__(16) char lds[];
   // chunk c on XCD c % 8 for every group (workgroups are dealt to the 8 XCDs in turn)
@@ -181,11 +217,10 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
   const int g = rest % a.ngroups, c = (rest / a.ngroups) * 8 + c8;
   if (c >= a.nchunk) return;
   const int r0 = a.row0[g], r1 = a.row0[g + 1];
-  const int nA = 16 * (a.nb - r0), nB = 16 * (r1 - r0);
-  char *ldsA = lds;
-  char *ldsB = lds + (size_t)OZ_DIGITS * nA * 64;
-  double *vsS = (double *)(ldsB + (size_t)OZ_DIGITS * nB * 64);   // [OZ_STAGE]
-  double *scB = vsS + OZ_STAGE;                            // [nB]
+  const int nB = 16 * (r1 - r0);
+  const size_t bufsz = (size_t)OZ_DIGITS * nB * 64;
+  char *ldsB = lds;                                         // [2][OZ_DIGITS][nB][64]
+  double *scB = (double *)(lds + 2 * bufsz);                // [nB]
   int *eRow = (int *)(scB + nB);                           // [nB]
   double *maxv = (double *)(eRow + nB + (nB & 1));          // [OZ_MAX_STAGES]
   int *s_eV_ptr = (int *)(maxv + OZ_MAX_STAGES);             // [1] (all LDS is dynamic)
@@ -219,105 +254,93 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
     scB[fl] = ldexp(1.0, -eV - eB);
     eRow[fl] = a.eA[(size_t)f * a.nchunk + c] + eV + eB;
   }
+  __syncthreads();
+  stage_b_digits(a, r0, nB, pc, scB, ldsB, tid);
+  __syncthreads();
 
-  v4i W[2 * OZ_MAXC], X[2 * OZ_MAXC], Y[2 * OZ_MAXC];
+  v4i acc[OZ_MAXB][5];
 #pragma unroll
-  for (int k = 0; k < 2 * OZ_MAXC; ++k) W[k] = X[k] = Y[k] = (v4i){0, 0, 0, 0};
-  const int *task = a.tasks + ((size_t)(g * OZ_WAVES + wave) * OZ_MAXC) * 3;
+  for (int k = 0; k < OZ_MAXB; ++k)
+#pragma unroll
+    for (int L = 0; L < 5; ++L) acc[k][L] = (v4i){0, 0, 0, 0};
+  // this wave's task, read once into scalar registers
+  const int *task = a.tasks + (size_t)(g * OZ_WAVES + wave) * OZ_TASK;
+  int tI[OZ_MAXB], tJ[OZ_MAXB], tC[OZ_MAXB], cJ[OZ_MAXCOL];
+#pragma unroll
+  for (int k = 0; k < OZ_MAXB; ++k) {
+    tI[k] = __builtin_amdgcn_readfirstlane(task[3 * k]);
+    tJ[k] = __builtin_amdgcn_readfirstlane(task[3 * k + 1]);
+    tC[k] = __builtin_amdgcn_readfirstlane(task[3 * k + 2]);
+  }
+#pragma unroll
+  for (int cc = 0; cc < OZ_MAXCOL; ++cc) cJ[cc] = __builtin_amdgcn_readfirstlane(task[3 * OZ_MAXB + cc]);
   const int r16 = lane & 15, q = lane >> 4;
-
+  // waves 0-3 and 4-7 (one of each per SIMD) run the MFMAs of a stage and the digit formation
+  // of the next in opposite order; both load their column digits before the formation, whose
+  // vector work then covers the load latency
+  const bool mfma_first = ((wave >> 2) & 1) == 0;
+  v4i cf[OZ_MAXCOL][OZ_DIGITS];
+  auto load_columns = [&](long p0) {
+#pragma unroll
+    for (int cc = 0; cc < OZ_MAXCOL; ++cc) {
+      const int J = cJ[cc];
+      if (J < 0) continue;
+      const signed char *src = a.planes + (size_t)(16 * J + r16) * a.Nq + p0 + 16 * q;
+#pragma unroll
+      for (int d = 0; d < OZ_DIGITS; ++d) cf[cc][d] = *(const v4i *)(src + (size_t)d * a.ld * a.Nq);
+    }
+  };
+  auto mfma_phase = [&](const char *bufB) {
+#pragma unroll
+    for (int k = 0; k < OZ_MAXB; ++k) {
+      const int I = tI[k], ci = tC[k];
+      if (I < 0) continue;
+      v4i ra[OZ_DIGITS];
+      const int fl = 16 * (I - r0) + r16;
+#pragma unroll
+      for (int d = 0; d < OZ_DIGITS; ++d) ra[d] = *(const v4i *)&bufB[seg_off(nB, d, fl, q)];
+      if (ci == 0)
+        block_products(ra, cf[0], acc[k]);
+      else if (ci == 1)
+        block_products(ra, cf[1], acc[k]);
+      else
+        block_products(ra, cf[2], acc[k]);
+    }
+  };
+  if (mfma_first) load_columns(pc);
   for (int t = 0; t < ns; ++t) {
     const long p0 = pc + (long)t * OZ_STAGE;
-    __syncthreads();  // the previous stage's MFMAs are done with the LDS planes
-    if (tid < OZ_STAGE) vsS[tid] = a.vs[p0 + tid];
-    __syncthreads();
-    // ---- staging: A digits of the group's column functions; B digits of its row functions
-    for (int it = tid; it < nA * 4; it += 64 * OZ_WAVES) {
-      const int fl = it >> 2, sq = it & 3;
-      const int f = 16 * r0 + fl;
-      u4 d[OZ_DIGITS];
-#pragma unroll
-      for (int k = 0; k < OZ_DIGITS; ++k)
-        d[k] = *(const u4 *)&a.planes[((size_t)k * a.ld + f) * a.Nq + p0 + 16 * sq];
-#pragma unroll
-      for (int k = 0; k < OZ_DIGITS; ++k) *(u4 *)&ldsA[seg_off(nA, k, fl, sq)] = d[k];
-      if (fl < nB) {
-        // JB = rint(JA vs 2^-(eV + eB)), JA = 2^14 (2^14 d1 + 2^7 d2 + d3) + 2^7 d4 + d5
-        const double sc = scB[fl];
-#pragma unroll
-        for (int wd = 0; wd < 4; ++wd) {
-          int b[OZ_DIGITS][4];
-#pragma unroll
-          for (int by = 0; by < 4; ++by) {
-            const int hi = sbyte(d[0][wd], by) * (1 << 14) + sbyte(d[1][wd], by) * 128 + sbyte(d[2][wd], by);
-            const int lo = sbyte(d[3][wd], by) * 128 + sbyte(d[4][wd], by);
-            const double JA = fma((double)hi, 16384.0, (double)lo);
-            const double y = vsS[16 * sq + 4 * wd + by] * sc;
-            const double JB = rint(JA * y);
-            int h, l, e[OZ_DIGITS];
-            split_hl(JB, h, l);
-            split5(h, l, e);
-#pragma unroll
-            for (int k = 0; k < OZ_DIGITS; ++k) b[k][by] = e[k];
-          }
-#pragma unroll
-          for (int k = 0; k < OZ_DIGITS; ++k)
-            *(unsigned *)&ldsB[seg_off(nB, k, fl, sq) + 4 * wd] = pack4(b[k][0], b[k][1], b[k][2], b[k][3]);
-        }
+    const char *bufB = ldsB + (t & 1) * bufsz;
+    char *bufN = ldsB + ((t + 1) & 1) * bufsz;
+    const bool more = t + 1 < ns;
+    if (mfma_first) {
+      mfma_phase(bufB);
+      if (more) {
+        load_columns(p0 + OZ_STAGE);
+        stage_b_digits(a, r0, nB, p0 + OZ_STAGE, scB, bufN, tid);
       }
+    } else {
+      load_columns(p0);
+      if (more) stage_b_digits(a, r0, nB, p0 + OZ_STAGE, scB, bufN, tid);
+      mfma_phase(bufB);
     }
     __syncthreads();
-    // ---- MFMA phase: this wave's column items (up to two blocks each)
-    int curI0 = -2, curI1 = -2;
-    v4i ra[OZ_DIGITS], rb[OZ_DIGITS];
-#pragma unroll
-    for (int k = 0; k < OZ_MAXC; ++k) {
-      const int I0 = __builtin_amdgcn_readfirstlane(task[3 * k]);
-      const int I1 = __builtin_amdgcn_readfirstlane(task[3 * k + 1]);
-      const int J = __builtin_amdgcn_readfirstlane(task[3 * k + 2]);
-      if (J < 0) continue;
-      if (I0 != curI0) {
-        const int fl = 16 * (I0 - r0) + r16;
-#pragma unroll
-        for (int d = 0; d < OZ_DIGITS; ++d) ra[d] = *(const v4i *)&ldsB[seg_off(nB, d, fl, q)];
-        curI0 = I0;
-      }
-      if (I1 >= 0 && I1 != curI1) {
-        const int fl = 16 * (I1 - r0) + r16;
-#pragma unroll
-        for (int d = 0; d < OZ_DIGITS; ++d) rb[d] = *(const v4i *)&ldsB[seg_off(nB, d, fl, q)];
-        curI1 = I1;
-      }
-      v4i cf[OZ_DIGITS];
-      {
-        const int fl = 16 * (J - r0) + r16;
-#pragma unroll
-        for (int d = 0; d < OZ_DIGITS; ++d) cf[d] = *(const v4i *)&ldsA[seg_off(nA, d, fl, q)];
-      }
-      block_products(ra, cf, W[2 * k], X[2 * k], Y[2 * k]);
-      if (I1 >= 0 && J >= I1) block_products(rb, cf, W[2 * k + 1], X[2 * k + 1], Y[2 * k + 1]);
-    }
   }
   // ---- epilogue: scaled FP64 partial of every block of this wave
   double *part = a.partial + (size_t)c * a.pstride;
 #pragma unroll
-  for (int k = 0; k < OZ_MAXC; ++k) {
-    const int J = __builtin_amdgcn_readfirstlane(task[3 * k + 2]);
-    if (J < 0) continue;
+  for (int k = 0; k < OZ_MAXB; ++k) {
+    const int I = tI[k], J = tJ[k];
+    if (I < 0) continue;
+    const int blk = I * a.nb - I * (I - 1) / 2 + (J - I);
+    const int col = 16 * J + r16;
+    const int ec = a.eA[(size_t)col * a.nchunk + c];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int I = __builtin_amdgcn_readfirstlane(task[3 * k + h]);
-      if (I < 0 || J < I) continue;
-      const int blk = I * a.nb - I * (I - 1) / 2 + (J - I);
-      const int col = 16 * J + r16;
-      const int ec = a.eA[(size_t)col * a.nchunk + c];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = 4 * q + r;
-        const double val = ((double)W[2 * k + h][r] * 0x1p-19 + (double)X[2 * k + h][r] * 0x1p-26) +
-                           (double)Y[2 * k + h][r] * 0x1p-40;
-        part[(size_t)blk * 256 + m * 16 + r16] = ldexp(val, eRow[16 * (I - r0) + m] + ec);
-      }
+    for (int r = 0; r < 4; ++r) {
+      const int m = 4 * q + r;
+      const double val = (((acc[k][0][r] * 0x1p-12 + acc[k][1][r] * 0x1p-19) + acc[k][2][r] * 0x1p-26) +
+                          acc[k][3][r] * 0x1p-33) + acc[k][4][r] * 0x1p-40;
+      part[(size_t)blk * 256 + m * 16 + r16] = ldexp(val, eRow[16 * (I - r0) + m] + ec);
     }
   }
 }
@@ -349,43 +372,46 @@ __global__ void k_ozaki_reduce(const double *__restrict__ partial, long pstride,
 }  // namespace
 
 // ------------------------------------------------------------------------ host
-static bool pack_waves(int r0, int r1, int nb, std::vector<int> &tab) {
-  // column items (row pair, column J) in order; contiguous runs to the waves, balancing blocks
-  struct Item { int I0, I1, J, nblk; };
-  std::vector<Item> items;
-  int total = 0;
-  for (int I = r0; I < r1; I += 2) {
-    const int I1 = I + 1 < r1 ? I + 1 : -1;
-    for (int J = I; J < nb; ++J) {
-      const int nbk = 1 + (I1 >= 0 && J >= I1 ? 1 : 0);
-      items.push_back({I, I1, J, nbk});
-      total += nbk;
-    }
-  }
-  tab.assign((size_t)OZ_WAVES * OZ_MAXC * 3, -1);
+// Blocks (I, J) of rows [r0, r1) x columns [r0, nb), I <= J, column-major, cut into OZ_WAVES
+// contiguous runs of at most wave_blocks blocks and OZ_MAXCOL distinct columns, balanced.
+static bool pack_waves(int r0, int r1, int nb, int wave_blocks, std::vector<int> &tab) {
+  std::vector<std::pair<int, int>> blocks;
+  for (int J = r0; J < nb; ++J)
+    for (int I = r0; I < r1 && I <= J; ++I) blocks.push_back({I, J});
+  const int total = (int)blocks.size();
+  tab.assign((size_t)OZ_WAVES * OZ_TASK, -1);
   size_t it = 0;
-  int done = 0;
-  for (int w = 0; w < OZ_WAVES && it < items.size(); ++w) {
+  for (int w = 0; w < OZ_WAVES && it < blocks.size(); ++w) {
     const int left = OZ_WAVES - w;
-    const int target = (total - done + left - 1) / left;
-    int got = 0;
-    for (int k = 0; k < OZ_MAXC && it < items.size() && got < target; ++k, ++it) {
-      int *t = &tab[((size_t)w * OZ_MAXC + k) * 3];
-      t[0] = items[it].I0, t[1] = items[it].I1, t[2] = items[it].J;
-      got += items[it].nblk;
+    const int target = std::min(wave_blocks, (int)((total - (int)it + left - 1) / left));
+    int *t = &tab[(size_t)w * OZ_TASK];
+    int *cols = t + 3 * OZ_MAXB;
+    int ncol = 0, k = 0;
+    while (it < blocks.size() && k < target) {
+      const int I = blocks[it].first, J = blocks[it].second;
+      int ci = -1;
+      for (int x = 0; x < ncol; ++x)
+        if (cols[x] == J) ci = x;
+      if (ci < 0) {
+        if (ncol == OZ_MAXCOL) break;
+        cols[ncol] = J;
+        ci = ncol++;
+      }
+      t[3 * k] = I, t[3 * k + 1] = J, t[3 * k + 2] = ci;
+      ++k, ++it;
     }
-    done += got;
   }
-  return it == items.size();
+  return it == blocks.size();
 }
 
-static size_t group_lds(int r0, int r1, int nb) {
-  const size_t nA = 16 * (size_t)(nb - r0), nB = 16 * (size_t)(r1 - r0);
-  return OZ_DIGITS * (nA + nB) * 64 + OZ_STAGE * 8 + nB * 12 + 8 + OZ_MAX_STAGES * 8 + 16;
+static size_t group_lds(int r0, int r1) {
+  const size_t nB = 16 * (size_t)(r1 - r0);
+  return 2 * OZ_DIGITS * nB * 64 + nB * 12 + 8 + OZ_MAX_STAGES * 8 + 16;
 }
 
-OzakiLayout ozaki_layout(long N, int n, int ld, double alpha, size_t max_lds, long chunk_target) {
+OzakiLayout ozaki_layout(long N, int n, int ld, int wave_blocks, size_t max_lds, long chunk_target) {
   NAPMO_REQUIRE(N > 0 && n > 0, "ozaki layout: empty grid or basis");
+  NAPMO_REQUIRE(wave_blocks >= 1 && wave_blocks <= OZ_MAXB, "ozaki layout: blocks per wave out of range");
   OzakiLayout L;
   L.N = N, L.n = n, L.ld = ld;
   L.nb = (int)div_up(n, 16);
@@ -396,40 +422,32 @@ OzakiLayout ozaki_layout(long N, int n, int ld, double alpha, size_t max_lds, lo
   L.chunk = chunk;
   L.nchunk = (int)div_up(N, chunk);
   L.Nq = (long)L.nchunk * chunk;
-  // row groups: consecutive block rows, cost of row I = (nb - I) blocks + alpha (its 16
-  // functions' digits of B formed per build); the fewest groups (at least two when
-  // possible) whose LDS and wave slots fit
+  // row groups: from the first free block row, the most rows whose blocks the waves take
   const int nb = L.nb;
-  for (int ng = std::min(2, nb); ng <= nb; ++ng) {
-    double total = 0.0;
-    for (int I = 0; I < nb; ++I) total += (nb - I) + alpha;
-    std::vector<int> row0{0};
-    double acc = 0.0;
-    for (int I = 0; I < nb && (int)row0.size() < ng; ++I) {
-      acc += (nb - I) + alpha;
-      if (acc >= total * row0.size() / ng && I + 1 < nb) row0.push_back(I + 1);
+  std::vector<int> row0{0};
+  std::vector<int> tasks, tab;
+  size_t lds = 0;
+  int r0 = 0;
+  while (r0 < nb) {
+    int best = -1;
+    for (int r1 = r0 + 1; r1 <= nb; ++r1) {
+      if (group_lds(r0, r1) > max_lds || !pack_waves(r0, r1, nb, wave_blocks, tab)) break;
+      best = r1;
     }
-    row0.push_back(nb);
-    const int gcount = (int)row0.size() - 1;
-    bool ok = true;
-    size_t lds = 0;
-    std::vector<int> tasks;
-    for (int g = 0; g < gcount && ok; ++g) {
-      std::vector<int> tab;
-      ok = pack_waves(row0[g], row0[g + 1], nb, tab) && group_lds(row0[g], row0[g + 1], nb) <= max_lds;
-      lds = std::max(lds, group_lds(row0[g], row0[g + 1], nb));
-      tasks.insert(tasks.end(), tab.begin(), tab.end());
-    }
-    if (ok) {
-      L.ngroups = gcount;
-      L.row0 = row0;
-      L.tasks = tasks;
-      L.lds_bytes = lds;
-      return L;
-    }
+    if (best < 0)
+      throw Error("int8 Fock contraction: basis of " + std::to_string(n) +
+                  " functions does not fit the wave slots (use the FP64 mode)");
+    pack_waves(r0, best, nb, wave_blocks, tab);
+    tasks.insert(tasks.end(), tab.begin(), tab.end());
+    lds = std::max(lds, group_lds(r0, best));
+    row0.push_back(best);
+    r0 = best;
   }
-  throw Error("int8 Fock contraction: basis of " + std::to_string(n) +
-              " functions does not fit the workgroup's LDS (use the FP64 mode)");
+  L.ngroups = (int)row0.size() - 1;
+  L.row0 = row0;
+  L.tasks = tasks;
+  L.lds_bytes = lds;
+  return L;
 }
 
 void launch_ozaki_setup(const OzakiLayout &L, const double *phi64, const double *wbw,
@@ -459,7 +477,7 @@ void launch_ozaki_fock(const OzakiLayout &L, const int *d_tasks, const signed ch
   OzArgs a;
   a.planes = planes, a.eA = eA, a.maxd1 = maxd1, a.vs = vs;
   a.tasks = d_tasks;
-  a.row0 = d_tasks + (size_t)L.ngroups * OZ_WAVES * OZ_MAXC * 3;
+  a.row0 = d_tasks + (size_t)L.ngroups * OZ_WAVES * OZ_TASK;
   a.partial = partial;
   a.Nq = L.Nq, a.chunk = L.chunk, a.pstride = L.partial_stride();
   a.ld = L.ld, a.nb = L.nb, a.nchunk = L.nchunk, a.ngroups = L.ngroups;

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <string>
 #include <cstring>
 
 namespace napmo {
@@ -75,7 +76,8 @@ static void lu_solve(std::vector<double> &A, std::vector<double> &B, int n, int 
     int piv = k;
     for (int i = k + 1; i < n; ++i)
       if (std::fabs(A[(size_t)i * n + k]) > std::fabs(A[(size_t)piv * n + k])) piv = i;
-    NAPMO_REQUIRE(A[(size_t)piv * n + k] != 0.0, "singular radial Poisson system");
+    NAPMO_REQUIRE(A[(size_t)piv * n + k] != 0.0 && std::isfinite(A[(size_t)piv * n + k]),
+                  "singular radial Poisson system (zero or non-finite pivot)");
     if (piv != k) {
       for (int c = 0; c < n; ++c) std::swap(A[(size_t)k * n + c], A[(size_t)piv * n + c]);
       for (int c = 0; c < m; ++c) std::swap(B[(size_t)k * m + c], B[(size_t)piv * m + c]);
@@ -191,6 +193,13 @@ static void radial_coeffs(const RTransform &rtf, std::vector<double> &r, std::ve
     j2[i] = rtf.deriv2(i);
     j3[i] = rtf.deriv3(i);
   }
+  // the radial ODE divides by r and by dr/dt: a node at r = 0 (the identity transform's
+  // first point, a zero Chebyshev radius) or a non-increasing grid makes the system singular
+  for (int i = 0; i < n; ++i)
+    NAPMO_REQUIRE(std::isfinite(r[i]) && r[i] > 0.0 && std::isfinite(j1[i]) && j1[i] > 0.0 &&
+                      (i == 0 || r[i] > r[i - 1]),
+                  "singular radial Poisson system: radial point " + std::to_string(i) + " at r = " +
+                      std::to_string(r[i]) + " (the points must be positive and strictly increasing)");
   for (int i = 0; i < n; ++i) {
     const double r2 = r[i] * r[i], j1sq = j1[i] * j1[i];
     const double by = 2 / r[i], bd = -2 / r2;
@@ -321,10 +330,19 @@ DeviceGrid::~DeviceGrid() {
 }
 
 void DeviceGrid::build(const std::vector<AtomInfo> &in) {
+  NAPMO_REQUIRE(!in.empty(), "BeckeGrid needs at least one centre");
+  // the Becke cell functions divide by the distance of every centre pair
+  // (src/grid/becke.cpp:150): checked before any device work
+  for (size_t i = 0; i < in.size(); ++i)
+    for (size_t j = 0; j < i; ++j) {
+      const double dx = in[i].origin[0] - in[j].origin[0], dy = in[i].origin[1] - in[j].origin[1],
+                   dz = in[i].origin[2] - in[j].origin[2];
+      NAPMO_REQUIRE(dx * dx + dy * dy + dz * dz > 0.0,
+                    "coincident centres in BeckeGrid (" + std::to_string(j) + " and " + std::to_string(i) + ")");
+    }
   require_device();
   atoms = in;
   M = (int)atoms.size();
-  NAPMO_REQUIRE(M > 0, "BeckeGrid needs at least one centre");
   if (!stream) NAPMO_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   offsets.assign(M + 1, 0);
   for (int a = 0; a < M; ++a) offsets[a + 1] = offsets[a] + (long)atoms[a].nrad * atoms[a].nang;

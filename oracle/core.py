@@ -1377,12 +1377,11 @@ def int8_fock(phi, wbw, v, chunk, stage=64):
     power-of-two maximum (J = rint(At 2^(34 - eA)), |J| <= 2^34); B = sign(wbw) v At as
     JB = rint(JA vs 2^-(eV + eB)) with eV the exponent of the chunk's max|vs| and eB that of the
     bound 2^-6 max_stages(max|vs| 2^-eV (max|d1| + 1)), so |JB| <= 2^34. Both split into five
-    balanced base-128 digits; per stage of 64 points the digit products of level L = i + j
-    (1-based) <= 6 are summed exactly (T_L) and accumulated in three integers:
-        W += 128 T2 + T3,   X += T4 + q5,   Y += 128 (T5 - 128 q5) + T6,   q5 = floor((T5 + 64) / 128)
-    The chunk's block is 2^(e_row + e_col) ((W 2^-19 + X 2^-26) + Y 2^-40); chunks are summed
-    in order. Products of level >= 7 (below 2^-35 of the leading one) are the only ones
-    dropped. phi [N, n]."""
+    balanced base-128 digits; the digit products of level L = i + j (1-based) <= 6 are summed
+    exactly over the chunk, one integer S_L per level (|S_L| < 2^27 for up to 63 stages), and the
+    chunk's block is 2^(e_row + e_col) ((((S2 2^-12 + S3 2^-19) + S4 2^-26) + S5 2^-33) + S6 2^-40);
+    chunks are summed in order. Products of level >= 7 (below 2^-35 of the leading one) are the
+    only ones dropped. phi [N, n]."""
     N, n = phi.shape
     nchunk = -(-N // chunk)
     Nq = nchunk * chunk
@@ -1411,21 +1410,15 @@ def int8_fock(phi, wbw, v, chunk, stage=64):
         y = vc[:, None] * scB[None, :]
         JB = np.rint(JA.astype(np.float64) * y).astype(np.int64)
         dB = _split5(JB)
-        W = np.zeros((n, n), dtype=np.int64)
-        X = np.zeros((n, n), dtype=np.int64)
-        Y = np.zeros((n, n), dtype=np.int64)
+        S = {L: np.zeros((n, n), dtype=np.int64) for L in range(2, 7)}
         for t in range(ns):
             st = slice(t * stage, (t + 1) * stage)
             r = [d[st].astype(np.float64) for d in dB]   # rows: B digits
             a = [d[st].astype(np.float64) for d in dA]   # columns: A digits
-            T = {L: sum(r[i].T @ a[L - 2 - i] for i in range(L - 1) if L - 2 - i < 5 and i < 5)
-                 for L in range(2, 7)}
-            T = {L: np.rint(x).astype(np.int64) for L, x in T.items()}   # exact integers
-            q5 = (T[5] + 64) >> 7
-            W += 128 * T[2] + T[3]
-            X += T[4] + q5
-            Y += 128 * (T[5] - 128 * q5) + T[6]
-        val = (W * 2.0 ** -19 + X * 2.0 ** -26) + Y * 2.0 ** -40
+            for L in range(2, 7):
+                T = sum(r[i].T @ a[L - 2 - i] for i in range(L - 1))
+                S[L] += np.rint(T).astype(np.int64)       # exact integers
+        val = (((S[2] * 2.0 ** -12 + S[3] * 2.0 ** -19) + S[4] * 2.0 ** -26) + S[5] * 2.0 ** -33) + S[6] * 2.0 ** -40
         F = F + np.ldexp(val, (eA + eV + eB)[:, None] + eA[None, :])
     U = np.triu(F)
     return U + np.triu(U, 1).T
