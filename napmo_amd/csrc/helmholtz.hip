@@ -343,6 +343,15 @@ void launch_helmholtz_radial(const AtomTables &at, const HelmholtzTables &ht, in
   const long mc = std::max<long>(1, std::min<long>((target - fixed) / nfn, 2L * lmax + 1));
   const size_t bytes = (size_t)(fixed + nfn * std::max<long>(mc, 3)) * sizeof(double);
   NAPMO_REQUIRE(bytes <= max_lds_per_block(), "orbital correction: radial system does not fit in LDS");
+  // past 64 KB (about 230 radial points) the kernel needs the device's full LDS per block
+  static int configured = -1;
+  int dev = 0;
+  NAPMO_HIP(hipGetDevice(&dev));
+  if (configured != dev) {
+    NAPMO_HIP(hipFuncSetAttribute((const void *)k_helmholtz_radial, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)max_lds_per_block()));
+    configured = dev;
+  }
   k_helmholtz_radial<<<dim3(at.M, lmax + 1, norb), 64, bytes, st>>>(
       at, ht, L, nrad_total, Vsph, oi, doi, mass_inv, mu, (int)mc, Uy, Udt, status);
   NAPMO_HIP(hipGetLastError());

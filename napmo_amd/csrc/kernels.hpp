@@ -144,7 +144,9 @@ struct HelmholtzTables {
   const double *c0, *oa, *o2, *j1, *j2;
   const long *c0_off, *oa_off, *o2_off;
 };
-constexpr int HELMHOLTZ_MAX_NRAD = 240;  // LDS of one radial system: 2n x 17 doubles + columns
+// LDS of one radial system of n points: 2n x 13 band and multiplier doubles, 2n spline
+// factors and n/2 pivot words, and at least 3 columns of 2n: 280 n bytes (143 KB at 512)
+constexpr int HELMHOLTZ_MAX_NRAD = 512;
 // out[o][p] = a[o][p] * s[o] - b[o][p] for norb rows of N points (s: device [norb]).
 void launch_scaled_diff(const double *a, const double *s, const double *b, int norb, long N,
                         double *out, hipStream_t st);

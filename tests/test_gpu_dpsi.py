@@ -87,9 +87,27 @@ def test_becke_dickson_step_matches_oracle():
         assert err < 1e-10 * scale
 
 
+def test_dpsi_300_radial_points_matches_oracle():
+    # the reference's numerical He grid size (300 radial points, 110 angular; lmax 8): the
+    # radial system needs 84 KB of LDS per workgroup
+    import napmo_amd
+    atoms = [('He', (0.0, 0.0, 0.0))]
+    g, _ = systems.gpu_system(atoms, [[]], 300, 110)
+    o, _ = systems.oracle_system(atoms, [[]], 300, 110)
+    r = np.sqrt((o.points ** 2).sum(1))
+    phi = np.exp(-1.7 * r) * (1.0 + 0.2 * o.points[:, 2])
+    ri = 0.05 * phi * np.cos(o.points[:, 0]) + 0.01 * np.exp(-r ** 2)
+    V = -2.0 / np.maximum(r, 1e-3) * phi
+    d = napmo_amd.compute_dpsi(g, 8, phi, 0.01, -0.9, ri, V, 1.0)
+    ref = core.compute_dpsi(o, 8, phi, 0.01, -0.9, ri, V, 1.0, evaluate=_evaluate_c)
+    scale, err = np.abs(ref).max(), np.abs(d - ref).max()
+    print('dpsi 300 x 110: max|d| %.3e max|err| %.3e' % (scale, err))
+    assert err < 1e-10 * scale
+
+
 def test_dpsi_rejects_oversized_radial_grid():
     import napmo_amd
-    g, _ = systems.gpu_system([('H', (0.0, 0.0, 0.0))], [[]], 250, 6)
-    with pytest.raises(RuntimeError, match='radial grids need 3 to 240 points'):
+    g, _ = systems.gpu_system([('H', (0.0, 0.0, 0.0))], [[]], 520, 6)
+    with pytest.raises(RuntimeError, match='radial grids need 3 to 512 points'):
         napmo_amd.compute_dpsi(g, 0, np.zeros(g.size), 0.0, -0.5, np.zeros(g.size),
                                np.zeros(g.size), 1.0)
