@@ -411,12 +411,21 @@ int napmo_fock_plan_sync(void *plan);
  * reference has no grid gradients). Basis centres are the distinct function origins in
  * order of first appearance; xyz [ncentre x 3] may be NULL. */
 int napmo_fock_plan_centres(void *plan, int *ncentre, double *xyz);
-/* dE/dR_c [ncentre x 3] for the plan's last build, whose density matrix P must be passed
- * again: the grid stays fixed and every basis function moves with its centre; grad phi is
- * recomputed on the fly. dE/dP is the build's F itself (F = dE/dP, to the asymmetry of the
- * discrete Poisson operator). */
+/* Derivatives of the last build's E_grid = 1/2 sum_p w bw rho v = 1/2 rho^T W V rho, exact for
+ * the discrete Becke-Poisson operator V (no counterpart in the reference; the backward of the
+ * torch glue). With ssym = (W v + V^T (W rho)) / 2:
+ *  - napmo_grid_fock_gradient: dE/dR_c [ncentre x 3], the grid fixed and every basis function
+ *    moving with its centre; the last build's density matrix P must be passed again; grad phi
+ *    is recomputed on the fly;
+ *  - napmo_grid_fock_energy_dP: dE/dP [n x n] = Phi^T diag(ssym) Phi (FP64); the build's own F
+ *    uses W v alone, so F differs from dE/dP by the asymmetry of V;
+ *  - napmo_grid_poisson_adjoint: z [N] = V^T y [N] on the plan's grid and lmax.
+ * The _device forms take device pointers and return without synchronising. */
 int napmo_grid_fock_gradient(void *plan, const double *P, double *grad_out);
 int napmo_grid_fock_gradient_device(void *plan, const double *dP, double *dG);
+int napmo_grid_fock_energy_dP(void *plan, double *dEdP_out);
+int napmo_grid_fock_energy_dP_device(void *plan, double *d_dEdP);
+int napmo_grid_poisson_adjoint(void *plan, const double *y, double *z_out);
 
 /* Launch-shape knobs of a plan, for the autotuner (no reference counterpart). Set them
  * before a batch is captured; not while a build is running. */

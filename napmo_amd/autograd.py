@@ -4,10 +4,12 @@ E = 1/2 sum_p w_p bw_p rho_p v_p with rho = diag(Phi(R) P Phi(R)^T) and v the Be
 potential of rho, on a fixed Becke grid; R are the centres the basis functions move with.
 The backward is hand-written on the GPU:
 
-* dE/dP = F = Phi^T diag(w bw v) Phi, the build's own Fock matrix (exact when the discrete
-  Coulomb operator is self-adjoint in the w bw inner product; measured on H2 cc-pVDZ the
-  central difference differs by 5e-7 relative, the operator's asymmetry);
-* dE/dR_c = -2 sum_p w bw v sum_{mu on c} grad phi_mu (Phi P)_p mu, with grad Phi
+* with W = w bw and V the discrete Becke-Poisson operator, E = 1/2 rho^T W V rho, so
+  dE/drho = ssym = (W v + V^T (W rho)) / 2, V^T the exactly transposed projection, radial
+  solve, synthesis tables and synthesis (napmo_grid_poisson_adjoint);
+* dE/dP = Phi^T diag(ssym) Phi (napmo_grid_fock_energy_dP). The build's own F uses W v alone
+  and equals dE/dP only as far as V is self-adjoint in the W inner product;
+* dE/dR_c = -2 sum_p ssym_p sum_{mu on c} grad phi_mu (Phi P)_p mu, with grad Phi
   recomputed in the kernel rather than stored (napmo_grid_fock_gradient).
 
 Tensors may live on the GPU (device pointers go straight to the library) or on the CPU.
@@ -79,31 +81,38 @@ class GridEnergy(torch.autograd.Function):
             F = torch.from_numpy(Fn)
             e_grid = torch.tensor(e, dtype=torch.float64)
         ctx.model = model
-        ctx.save_for_backward(P, R, F)
+        ctx.save_for_backward(P, R)
         return e_grid
 
     @staticmethod
     def backward(ctx, gE):
-        P, R, F = ctx.saved_tensors
+        P, R = ctx.saved_tensors
         model = ctx.model
         gP = gR = None
-        if ctx.needs_input_grad[0]:
-            gP = gE * F
-        if ctx.needs_input_grad[1]:
-            plan = model.plan(R.detach().cpu().numpy())
-            # rebuild with this P so the plan's potential is the one E was taken from
-            if _dev(P):
-                Fd = torch.empty_like(P)
-                E = torch.empty(2, dtype=torch.float64, device=P.device)
+        plan = model.plan(R.detach().cpu().numpy())
+        # rebuild with this P so the plan's density and potential are the ones E was taken from
+        if _dev(P):
+            Fd = torch.empty_like(P)
+            E = torch.empty(2, dtype=torch.float64, device=P.device)
+            torch.cuda.synchronize(P.device)
+            plan.build_device(P.data_ptr(), Fd.data_ptr(), E.data_ptr())
+            if ctx.needs_input_grad[0]:
+                dP = torch.empty_like(P)
+                plan.energy_dP_device(dP.data_ptr())
+                plan.synchronize()
+                gP = gE * dP
+            if ctx.needs_input_grad[1]:
                 G = torch.empty((len(model._atom_of), 3), dtype=torch.float64, device=P.device)
-                torch.cuda.synchronize(P.device)
-                plan.build_device(P.data_ptr(), Fd.data_ptr(), E.data_ptr())
                 plan.gradient_device(P.data_ptr(), G.data_ptr())
                 plan.synchronize()
                 gc = G.cpu().numpy()
-            else:
-                plan.build(P.numpy())
+        else:
+            plan.build(P.numpy())
+            if ctx.needs_input_grad[0]:
+                gP = gE * torch.from_numpy(plan.energy_dP())
+            if ctx.needs_input_grad[1]:
                 gc = plan.gradient(P.numpy())
+        if ctx.needs_input_grad[1]:
             g = np.zeros((R.shape[0], 3))
             for c, a in enumerate(model._atom_of):
                 g[a] += gc[c]

@@ -985,6 +985,29 @@ int napmo_grid_fock_gradient(void *plan, const double *P, double *grad_out) {
 int napmo_grid_fock_gradient_device(void *plan, const double *dP, double *dG) {
   return guarded([&] { ((FockPlan *)plan)->gradient_device(dP, dG); });
 }
+int napmo_grid_fock_energy_dP(void *plan, double *dEdP_out) {
+  return guarded([&] {
+    FockPlan *p = (FockPlan *)plan;
+    DeviceBuffer<double> d((size_t)p->n * p->n);
+    p->energy_dP(d.get());
+    d.download(dEdP_out, (size_t)p->n * p->n, p->stream);
+    NAPMO_HIP(hipStreamSynchronize(p->stream));
+  });
+}
+int napmo_grid_fock_energy_dP_device(void *plan, double *d_dEdP) {
+  return guarded([&] { ((FockPlan *)plan)->energy_dP(d_dEdP); });
+}
+int napmo_grid_poisson_adjoint(void *plan, const double *y, double *z_out) {
+  return guarded([&] {
+    FockPlan *p = (FockPlan *)plan;
+    hipStream_t st = p->stream;
+    DeviceBuffer<double> dy, dz(std::max<long>(p->grid.N, 1));
+    dy.upload(y, p->grid.N, st);
+    p->adjoint_potential(dy.get(), dz.get());
+    dz.download(z_out, p->grid.N, st);
+    NAPMO_HIP(hipStreamSynchronize(st));
+  });
+}
 
 int napmo_fock_plan_set_tuning(void *plan, int key, long value) {
   return guarded([&] { ((FockPlan *)plan)->set_tuning(key, value); });

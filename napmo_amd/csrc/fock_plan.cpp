@@ -318,16 +318,53 @@ void FockPlan::contract(int slot, const double *dP, const double *veff, double *
   built = true;
 }
 
+void FockPlan::adjoint_potential(const double *y, double *z) {
+  hipStream_t st = stream;
+  PoissonPlan &pl = grid.plan(lmax);
+  AtomTables at = grid.tables(pl);
+  SynthPairs &sp = grid.synth_pairs();
+  const long L = (long)(lmax + 1) * (lmax + 1);
+  long nrad_total = 0;
+  for (const auto &a : grid.atoms) nrad_total += a.nrad;
+  adj_T.resize(std::max<long>(synth_table_size(at, lmax), 1));
+  adj_aL.resize((size_t)grid.M * L);
+  adj_aR.resize((size_t)grid.M * L);
+  adj_Uy.resize((size_t)nrad_total * L);
+  adj_Udt.resize((size_t)nrad_total * L);
+  adj_rt.resize((size_t)nrad_total * L);
+  launch_poisson_adjoint(at, lmax, sp.geo.get(), sp.list.get(), sp.slot0.get(), grid.owner.get(),
+                         grid.bw.get(), pl.yhat.get(), pl.green.get(), grid.N, y, adj_T.get(),
+                         adj_aL.get(), adj_aR.get(), adj_Uy.get(), adj_Udt.get(), adj_rt.get(), z,
+                         st);
+}
+
+void FockPlan::symmetric_weight() {
+  NAPMO_REQUIRE(built, "energy derivatives: run a build with this density first");
+  hipStream_t st = stream;
+  adj_y.resize(Npad);
+  adj_z.resize(Npad);
+  ssym.resize(Npad);
+  // y = W rho, z = V^T y, ssym = (W v + z) / 2 with W v = s of the build
+  launch_scale3(rho.get(), grid.wbw.get(), nullptr, grid.N, Npad, adj_y.get(), st);
+  adjoint_potential(adj_y.get(), adj_z.get());
+  launch_half_sum(s.get(), adj_z.get(), grid.N, Npad, ssym.get(), st);
+}
+
+void FockPlan::energy_dP(double *dEdP) {
+  symmetric_weight();
+  launch_fock_f64(fsh, phi64.get(), phi64.get(), ssym.get(), F_part.get(), stream);
+  launch_fock_reduce(fsh, F_part.get(), n, 0, 1.0, 0, dEdP, stream);
+}
+
 void FockPlan::gradient_device(const double *dP, double *dG) {
-  NAPMO_REQUIRE(built, "gradient: run a build with this P first");
+  symmetric_weight();
   hipStream_t st = stream;
   launch_pad2d(dP, n, n, P.get(), ld, ld, st);
   X.resize((size_t)Npad * ld);
   launch_phi_times(phi64.get(), ld, Npad, P.get(), ld, X.get(), st);
   const int ncentre = (int)centres.size() / 3;
   grad_part.resize((size_t)n * grad_chunks(grid.N) * 3);
-  // s = w bw v of the last build (contract)
-  launch_grad_centres(grid.x.get(), grid.y.get(), grid.z.get(), grid.N, s.get(), X.get(), ld, n,
+  launch_grad_centres(grid.x.get(), grid.y.get(), grid.z.get(), grid.N, ssym.get(), X.get(), ld, n,
                       basis.origin.get(), basis.lxyz.get(), basis.first.get(), basis.nprim.get(),
                       basis.norm.get(), basis.zeta.get(), basis.coef.get(), centre_of.get(),
                       ncentre, grad_part.get(), dG, st);

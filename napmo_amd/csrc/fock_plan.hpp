@@ -96,8 +96,18 @@ struct FockPlan {
   DeviceBuffer<int> centre_of;   // [n]
   DeviceBuffer<double> X, grad_part;
   bool built = false;
-  // dE/dR_c of E = 1/2 sum w bw rho v for the last build (same P), grid fixed, each
-  // function moving with its centre: dG [ncentre x 3] (device), on the plan's stream.
+  // Derivatives of E = 1/2 sum_p w bw rho v = 1/2 rho^T W V rho of the last build (grid
+  // fixed, every function moving with its centre), exact for the discrete operator V: with
+  // ssym = (W v + V^T (W rho)) / 2 = dE/drho,
+  //   dE/dP = Phi^T diag(ssym) Phi   (energy_dP, FP64 contraction, dEdP [n x n] device)
+  //   dE/dR_c = -2 sum_p ssym_p sum_{mu on c} grad phi_mu(p) (Phi P)_p mu
+  //                                 (gradient_device, dG [ncentre x 3] device).
+  // grad Phi is recomputed in the kernel, not stored. V^T is the transposed projection,
+  // radial solve, synthesis tables and synthesis (adjoint_potential: z = V^T y on device).
+  DeviceBuffer<double> adj_y, adj_z, adj_T, adj_aL, adj_aR, adj_Uy, adj_Udt, adj_rt, ssym;
+  void adjoint_potential(const double *y, double *z);
+  void symmetric_weight();
+  void energy_dP(double *dEdP);
   void gradient_device(const double *dP, double *dG);
   void profile_start(int max_builds);
   // Average milliseconds per phase over the builds recorded since profile_start.

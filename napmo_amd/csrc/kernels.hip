@@ -348,6 +348,187 @@ void launch_synth_tables(const AtomTables &at, const double *Uy, const double *U
   NAPMO_HIP(hipGetLastError());
 }
 
+// ============================================== Poisson: the transposed operator
+// z = V^T y for the discrete Becke-Poisson operator v = V rho (projection, radial Green
+// solves, synthesis tables, synthesis over the (point, centre) pairs, sum over centres),
+// each step transposed exactly, in the reverse order. Used for the derivative of
+// E = 1/2 sum_p w bw rho v = 1/2 rho^T W V rho: dE/drho = 1/2 (W v + V^T (W rho)).
+
+// synthesis^T: one workgroup per (centre, bucket), one thread per synthesis index q; the
+// bucket's pairs (64 per slot, slots of a bucket contiguous) are staged in LDS slot by slot
+// and every thread adds y_p harm_q(p) u^k (spline rows) or y_p harm_q(p) d^l / d^-(l+1)
+// (extrapolation amplitudes) in pair order, so the result is deterministic.
+__global__ void __launch_bounds__(1024) k_synth_adjoint(AtomTables at, int lmax, int L,
+                                                        const double *__restrict__ geo,
+                                                        const int *__restrict__ list,
+                                                        const long *__restrict__ slot0,
+                                                        const double *__restrict__ y,
+                                                        double *__restrict__ Tt,
+                                                        double *__restrict__ aLt,
+                                                        double *__restrict__ aRt) {
+  __shared__ double sg[5][64];
+  const int b = blockIdx.x, nb = at.max_nrad + 1;
+  const int A = b / nb, loc = b % nb;
+  const int nr = at.nrad[A];
+  if (loc > nr) return;
+  const int reg = loc == 0 ? 0 : (loc == nr ? 2 : 1);
+  const int j = reg == 1 ? loc - 1 : 0;
+  const int q = threadIdx.x;
+  int l = 0, m = 0, cs = 0;
+  if (q < L) q_decode(q, lmax, l, m, cs);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  const long s0 = slot0[b], s1 = slot0[b + 1];
+  for (long w = s0; w < s1; ++w) {
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const int lane = threadIdx.x;
+      const int p = list[w * 64 + lane];
+      const double *g = geo + w * 256 + lane;
+      sg[0][lane] = g[0], sg[1][lane] = g[64], sg[2][lane] = g[128], sg[3][lane] = g[192];
+      sg[4][lane] = p >= 0 ? y[p] : 0.0;
+    }
+    __syncthreads();
+    if (q >= L) continue;
+    for (int k = 0; k < 64; ++k) {
+      const double yy = sg[4][k];
+      if (yy == 0.0) continue;  // padding slots and zero cotangents add nothing
+      const double g0 = sg[0][k], ux = sg[1][k], uy = sg[2][k], uz = sg[3][k];
+      double Am = 1.0, Bm = 0.0;  // (ux + i uy)^m as the forward synthesis forms it
+      for (int t = 0; t < m; ++t) {
+        const double re = ux * Am - uy * Bm;
+        Bm = ux * Bm + uy * Am;
+        Am = re;
+      }
+      double pil = 1.0, pi1 = 1.0, pi2 = 0.0;  // scaled PI_l^m(uz), the forward recurrence
+      for (int ll = m + 1; ll <= l; ++ll) {
+        pil = (ll == m + 1) ? uz : uz * pi1 - pi_beta(ll, m) * pi2;
+        pi2 = pi1;
+        pi1 = pil;
+      }
+      const double h = (cs ? Bm : Am) * pil * yy;
+      if (reg == 1) {
+        const double u = g0;
+        a0 += h;
+        a1 += h * u;
+        a2 += h * u * u;
+        a3 += h * u * u * u;
+      } else {
+        // d^l (left) or d^-(l+1) (right), by the forward's running products
+        const double base = (reg == 0) ? g0 : 1.0 / g0;
+        double pw = (reg == 2) ? base : 1.0;
+        for (int t = 0; t < l; ++t) pw *= base;
+        a0 += h * pw;
+      }
+    }
+  }
+  if (q >= L) return;
+  if (reg == 1) {
+    double *t = Tt + (((long)A * (at.max_nrad - 1) + j) * L + q) * 4;
+    t[0] = a0, t[1] = a1, t[2] = a2, t[3] = a3;
+  } else {
+    (reg == 0 ? aLt : aRt)[(long)A * L + q] = a0;
+  }
+}
+
+// synthesis tables^T: one workgroup per (radial node, centre), one thread per q; node i
+// collects interval i (as its left end) and interval i - 1 (as its right end), and the
+// extrapolation amplitudes at the first and last node.
+__global__ void k_synth_tables_adjoint(AtomTables at, int lmax, int L, const double *__restrict__ Tt,
+                                       const double *__restrict__ aLt, const double *__restrict__ aRt,
+                                       double *__restrict__ Uyt, double *__restrict__ Udtt) {
+  const int i = blockIdx.x, A = blockIdx.y;
+  const int nr = at.nrad[A];
+  if (i >= nr) return;
+  const long roff = at.rad_off[A];
+  for (int q = threadIdx.x; q < L; q += blockDim.x) {
+    int l, m, cs;
+    q_decode(q, lmax, l, m, cs);
+    const int ref = l * l + (m ? 2 * m - 1 + cs : 0);
+    const double f = q_factor(l, m) * pi_sigma(l, m);
+    double gy = 0.0, gd = 0.0;
+    if (i <= nr - 2) {
+      const double *t = Tt + (((long)A * (at.max_nrad - 1) + i) * L + q) * 4;
+      gy += f * (t[0] - 3 * t[2] + 2 * t[3]);
+      gd += f * (t[1] - 2 * t[2] + t[3]);
+    }
+    if (i >= 1) {
+      const double *t = Tt + (((long)A * (at.max_nrad - 1) + i - 1) * L + q) * 4;
+      gy += f * (3 * t[2] - 2 * t[3]);
+      gd += f * (t[3] - t[2]);
+    }
+    if (i == 0) gy += aLt[(long)A * L + q] * f / pow(at.r[roff], (double)l);
+    if (i == nr - 1) gy += aRt[(long)A * L + q] * f * pow(at.r[roff + nr - 1], (double)(l + 1));
+    Uyt[(roff + i) * L + ref] = gy;
+    Udtt[(roff + i) * L + ref] = gd;
+  }
+}
+
+// radial solves^T: rho_lm^T[A][j][lm] = sum_i Gy[i][j] Uy^T[A][i][lm] + Gd[i][j] Udt^T[A][i][lm].
+__global__ void __launch_bounds__(256) k_radial_adjoint(AtomTables at, const double *__restrict__ green,
+                                                        int L, const double *__restrict__ Uyt,
+                                                        const double *__restrict__ Udtt,
+                                                        double *__restrict__ rt) {
+  const int A = blockIdx.z, lm = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nr = at.nrad[A];
+  if (j >= nr) return;
+  const int l = (int)sqrt((double)lm + 0.5);
+  const double *G = green + at.green_off[A] + (long)l * 2 * nr * nr;
+  const long roff = at.rad_off[A];
+  double acc = 0.0;
+  for (int i = 0; i < nr; ++i)
+    acc += G[(long)i * nr + j] * Uyt[(roff + i) * L + lm] + G[(long)nr * nr + (long)i * nr + j] * Udtt[(roff + i) * L + lm];
+  rt[(roff + j) * L + lm] = acc;
+}
+
+// projection^T: z_p = bw_p sum_lm Yhat[j][lm] rho_lm^T[A][i][lm] on atom A's own points
+// (p = point_off[A] + i na + j).
+__global__ void __launch_bounds__(256) k_project_adjoint(AtomTables at, const int *__restrict__ owner,
+                                                         const double *__restrict__ bw,
+                                                         const double *__restrict__ yhat, int L, long N,
+                                                         const double *__restrict__ rt,
+                                                         double *__restrict__ z) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int A = owner[p];
+  const int na = at.nang[A];
+  const long loc = p - at.point_off[A];
+  const long i = loc / na, j = loc - i * na;
+  const double *Y = yhat + at.yhat_off[A] + j * L;
+  const double *r = rt + (at.rad_off[A] + i) * L;
+  double acc = 0.0;
+  for (int lm = 0; lm < L; ++lm) acc += Y[lm] * r[lm];
+  z[p] = bw[p] * acc;
+}
+
+__global__ void k_half_sum(const double *__restrict__ a, const double *__restrict__ b, long N, long Npad,
+                           double *__restrict__ out) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < Npad) out[p] = p < N ? 0.5 * (a[p] + b[p]) : 0.0;
+}
+
+void launch_half_sum(const double *a, const double *b, long N, long Npad, double *out, hipStream_t st) {
+  if (Npad <= 0) return;
+  k_half_sum<<<div_up(Npad, 256), 256, 0, st>>>(a, b, N, Npad, out);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_poisson_adjoint(const AtomTables &at, int lmax, const double *geo, const int *list,
+                            const long *slot0, const int *owner, const double *bw,
+                            const double *yhat, const double *green, long N, const double *y,
+                            double *Tt, double *aLt, double *aRt, double *Uyt, double *Udtt,
+                            double *rt, double *z, hipStream_t s) {
+  NAPMO_REQUIRE(lmax >= 0 && lmax <= MAX_LMAX, "lmax out of range for the transposed synthesis");
+  const int L = (lmax + 1) * (lmax + 1);
+  const int nb = at.M * (at.max_nrad + 1);
+  const int threads = (int)std::min<long>(1024, round_up(L, 64));
+  k_synth_adjoint<<<nb, threads, 0, s>>>(at, lmax, L, geo, list, slot0, y, Tt, aLt, aRt);
+  k_synth_tables_adjoint<<<dim3(at.max_nrad, at.M), 256, 0, s>>>(at, lmax, L, Tt, aLt, aRt, Uyt, Udtt);
+  k_radial_adjoint<<<dim3(div_up(at.max_nrad, 256), L, at.M), 256, 0, s>>>(at, green, L, Uyt, Udtt, rt);
+  if (N > 0) k_project_adjoint<<<div_up(N, 256), 256, 0, s>>>(at, owner, bw, yhat, L, N, rt, z);
+  NAPMO_HIP(hipGetLastError());
+}
+
 // ================================================================ Poisson: synthesis
 // (kernel template in synth_kernel.hpp, instantiated per lmax in synth_inst_*.hip)
 template <int LM>

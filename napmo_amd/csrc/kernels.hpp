@@ -74,6 +74,15 @@ void launch_radial(const AtomTables &at, const double *rho_lm, const double *gre
 // nset sets at once: set k reads Uy, Udt + k ustride and writes T + k synth_table_size,
 // ampL, ampR + k M L.
 long synth_table_size(const AtomTables &at, int lmax);
+// z [N] = V^T y [N]: the transposed Becke-Poisson operator of the grid (its (point, centre)
+// pair buckets: geo, list and the first slot of every bucket slot0 [M (max_nrad + 1) + 1];
+// owner [N] the atom of every point). Work arrays: Tt (synth_table_size), aLt, aRt [M L],
+// Uyt, Udtt, rt [sum nrad x L].
+void launch_poisson_adjoint(const AtomTables &at, int lmax, const double *geo, const int *list,
+                            const long *slot0, const int *owner, const double *bw,
+                            const double *yhat, const double *green, long N, const double *y,
+                            double *Tt, double *aLt, double *aRt, double *Uyt, double *Udtt,
+                            double *rt, double *z, hipStream_t s);
 void launch_synth_tables(const AtomTables &at, const double *Uy, const double *Udt, int lmax,
                          double *T, double *ampL, double *ampR, hipStream_t s, int write_amps = 1,
                          int nset = 1, long ustride = 0);
@@ -194,6 +203,8 @@ void launch_density_f64(const double *phi, int ld, long Npad, const double *P, d
 void launch_rho_reduce(const double *partial, int ntile, long Npad, long N, double *rho,
                        hipStream_t s, double scale = 1.0);
 
+// out[p] = (a[p] + b[p]) / 2 for p < N, zero for p < Npad beyond.
+void launch_half_sum(const double *a, const double *b, long N, long Npad, double *out, hipStream_t st);
 // Scale vector s[p] = a[p] * b[p] * c[p] (c may be null), zero for p >= N.
 void launch_scale3(const double *a, const double *b, const double *c, long N, long Npad, double *s,
                    hipStream_t st);

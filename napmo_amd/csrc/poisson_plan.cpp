@@ -870,6 +870,10 @@ std::unique_ptr<SynthPairs> build_synth_pairs(const DeviceGrid &src, const doubl
   NAPMO_HIP(hipMemsetAsync(cursor.get(), 0, sizeof(int) * nb, stream));
   launch_pair_fill(at, x, y, z, N, doff.get(), cursor.get(), p->list.get(), stream);
   p->wave_bucket.upload(wb, stream);
+  std::vector<long> s0(nb + 1);
+  for (int b = 0; b < nb; ++b) s0[b] = off[b] / 64;
+  s0[nb] = total / 64;
+  p->slot0.upload(s0, stream);
   p->geo.resize((size_t)std::max<long>(p->nwaves, 1) * 256);
   launch_pair_geometry(at, x, y, z, p->list.get(), p->wave_bucket.get(), p->nwaves, p->geo.get(),
                        stream);

@@ -78,6 +78,7 @@ constexpr int SET_BATCH = 64;
 struct SynthPairs {
   long nwaves = 0;
   DeviceBuffer<int> list, wave_bucket;
+  DeviceBuffer<long> slot0;  // [M (max_nrad + 1) + 1] first slot of every bucket
   DeviceBuffer<double> geo;  // [nwaves][4][64]: u or d, unit vector (launch_pair_geometry)
   DeviceBuffer<double> contrib;  // [M x N]
 };

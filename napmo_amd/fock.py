@@ -97,9 +97,27 @@ class GridFock(object):
         cext.check(cext.napmo_fock_plan_centres(self._this, ctypes.byref(nc), xyz.ctypes.data))
         return xyz
 
+    def energy_dP(self):
+        """dE/dP [n x n] of the last build's E_grid = 1/2 rho^T W V rho, exact for the discrete
+        Poisson operator V: Phi^T diag((W v + V^T (W rho)) / 2) Phi."""
+        out = np.zeros((self.nbasis, self.nbasis))
+        cext.check(cext.napmo_grid_fock_energy_dP(self._this, out.ctypes.data))
+        return out
+
+    def energy_dP_device(self, d_out):
+        cext.check(cext.napmo_grid_fock_energy_dP_device(self._this, d_out))
+
+    def poisson_adjoint(self, y):
+        """V^T y [N] for the plan's grid and lmax (V the Becke-Poisson operator of the build)."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        z = np.zeros(self.npoint)
+        cext.check(cext.napmo_grid_poisson_adjoint(self._this, y.ctypes.data, z.ctypes.data))
+        return z
+
     def gradient(self, P):
         """dE/dR_c [ncentre x 3] of the last build's E = 1/2 sum w bw rho v (same P),
-        grid fixed, basis functions moving with their centres."""
+        grid fixed, basis functions moving with their centres; exact for the discrete
+        operator (the symmetrised weight (W v + V^T (W rho)) / 2)."""
         P = np.ascontiguousarray(P, dtype=np.float64)
         g = np.zeros((len(self.centres()), 3))
         cext.check(cext.napmo_grid_fock_gradient(self._this, P.ctypes.data, g.ctypes.data))

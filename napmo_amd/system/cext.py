@@ -203,6 +203,9 @@ _SIGNATURES = {
     'napmo_fock_plan_centres': (c_int, [vp, ctypes.POINTER(c_int), vp]),
     'napmo_grid_fock_gradient': (c_int, [vp, vp, vp]),
     'napmo_grid_fock_gradient_device': (c_int, [vp, vp, vp]),
+    'napmo_grid_fock_energy_dP': (c_int, [vp, vp]),
+    'napmo_grid_fock_energy_dP_device': (c_int, [vp, vp]),
+    'napmo_grid_poisson_adjoint': (c_int, [vp, vp, vp]),
     'napmo_fock_plan_set_tuning': (c_int, [vp, c_int, c_long]),
     'napmo_fock_plan_get_tuning': (c_int, [vp, c_int, ctypes.POINTER(c_long)]),
     'napmo_batch_plan_new': (c_int, [c_int, vp, c_int, ctypes.POINTER(vp)]),
