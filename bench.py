@@ -199,7 +199,37 @@ def apmo_leg(args, log):
 
 
 DPSI_PHASES = ('residual', 'project', 'shell_average', 'radial', 'synth_tables', 'synth',
-               'reduce')
+               'reduce', 'synth_mfma')
+SYNTH_MFMA_MIN_KEY = 9          # NAPMO_TUNE_SYNTH_MFMA_MIN
+
+
+def mfma_densities(grid, nset):
+    """How many of nset densities the grid's many-density synthesis puts on MFMA: whole
+    tiles of 16 and a last tile of at least the grid's threshold."""
+    thr = grid.get_tuning(SYNTH_MFMA_MIN_KEY)
+    if thr > 16:
+        return 0
+    full, rest = 16 * (nset // 16), nset % 16
+    return full + (rest if rest >= thr else 0)
+
+
+def synth_roofline(phases, M, N, L, nset, n_mfma, traffic):
+    """Roofline of the dominant synthesis kernel of a many-density workload. MFMA synthesis:
+    the product of the pair features (u^c Y_q, 4 L per pair) with the spline coefficients,
+    SYNTH_FLOPS_ISSUED M N L flops per density, against the FP64 MFMA peak. One-density
+    synthesis: SYNTH_FLOPS M N L per density against the FP64 VALU peak."""
+    if n_mfma and phases.get('synth_mfma', 0.0) >= phases.get('synth', 0.0):
+        ms = phases['synth_mfma']
+        flops = SYNTH_FLOPS_ISSUED * M * N * L * n_mfma
+        return {'bound': 'mfma', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': PEAK['fp64_mfma'] / 1e12,
+                'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / PEAK['fp64_mfma'],
+                'traffic': table_traffic(traffic, 'k_synth_mfma<'), 'kernel': 'synth_mfma',
+                'densities': n_mfma}
+    ms = phases['synth']
+    flops = SYNTH_FLOPS * M * N * L * (nset - n_mfma)
+    return {'bound': 'valu', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': PEAK['fp64_valu'] / 1e12,
+            'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / PEAK['fp64_valu'],
+            'traffic': table_traffic(traffic, 'k_synth2n<'), 'kernel': 'synth', 'densities': nset - n_mfma}
 
 
 def dpsi_orbitals(points, origin, norb, seed=4):
@@ -267,15 +297,13 @@ def dpsi_leg(args, log):
     log('dpsi step: %.3f ms (%.1f steps/s), phases %s; de[0:3] %s, max|dpsi| %.3e' % (
         dt * 1e3, 1.0 / dt, {k: round(v, 4) for k, v in phases.items()}, de[:3], np.abs(dpsi).max()))
     M, L = len(atoms), (lmax + 1) ** 2
-    # dominant kernel: the synthesis, SYNTH_FLOPS M N L flops per orbital
+    # dominant kernel: the synthesis (MFMA tiles and one-density launches)
     name = max(phases, key=phases.get)
     ms = phases[name]
-    if name == 'synth':
-        flops = SYNTH_FLOPS * M * N * L * norb
-        roof = {'bound': 'valu', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': PEAK['fp64_valu'] / 1e12,
-                'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / PEAK['fp64_valu'],
-                # HBM bytes of one synthesis launch (one orbital)
-                'traffic': table_traffic(traffic_table('traffic_dpsi_chon16.json'), 'k_synth2n<')}
+    if name in ('synth', 'synth_mfma'):
+        roof = synth_roofline(phases, M, N, L, norb, mfma_densities(grid, norb),
+                              traffic_table('traffic_dpsi_chon16.json'))
+        name = roof.pop('kernel')
     else:
         nbytes = 8.0 * N * norb * 4
         roof = {'bound': 'hbm', 'achieved': nbytes / (ms * 1e-3) / 1e9, 'peak': PEAK['hbm'] / 1e9,
@@ -368,18 +396,14 @@ def operators_leg(args, log):
     cext.check(cext.napmo_grid_profile_read(grid._this, ph.ctypes.data, len(DPSI_PHASES), ctypes.byref(nc)))
     cext.check(cext.napmo_grid_profile(grid._this, 0))
     phases = {'project_radial': float(ph[1]), 'synth_tables': float(ph[4]), 'synth': float(ph[5]),
-              'reduce_acc': float(ph[6])}
+              'synth_mfma': float(ph[7]), 'reduce_acc': float(ph[6])}
     npair = ndim * (ndim + 1) // 2
     log('exchange build (%d orbitals, %d solves): %.3f ms (%.2f ms per solve); phases %s; '
         'kinetic %.3f ms per orbital' % (ndim, npair, dt * 1e3, dt * 1e3 / npair,
                                          {k: round(v, 3) for k, v in phases.items()}, dt_kin * 1e3))
     M, L = len(atoms), (lmax + 1) ** 2
-    flops = SYNTH_FLOPS * M * N * L * npair
-    ms = phases['synth']
-    roof = {'bound': 'valu', 'achieved': flops / (ms * 1e-3) / 1e12, 'peak': PEAK['fp64_valu'] / 1e12,
-            'unit': 'TFLOP/s', 'frac': flops / (ms * 1e-3) / PEAK['fp64_valu'],
-            'traffic': table_traffic(traffic_table('traffic_dpsi_chon16.json'), 'k_synth2n<'),
-            'kernel': 'synth'}
+    roof = synth_roofline(phases, M, N, L, npair, mfma_densities(grid, npair),
+                          traffic_table('traffic_dpsi_chon16.json'))
     cpu = None
     if not args.no_cpu_baseline:
         cpu = exchange_cpu_baseline(atoms, nrad, nang, lmax, ndim, log)

@@ -443,11 +443,19 @@ enum {
                                    0 uses the LDS-staged kernels */
   NAPMO_TUNE_INT8_CHUNK = 7,  /* int8 mode: points per chunk (a multiple of 64, at most 4032;
                                  0 = about N / 256). Setting it re-slices the per-geometry digits */
-  NAPMO_TUNE_INT8_WAVE_BLOCKS = 8 /* int8 mode: 16 x 16 blocks of F per wave in the row-group
+  NAPMO_TUNE_INT8_WAVE_BLOCKS = 8, /* int8 mode: 16 x 16 blocks of F per wave in the row-group
                                      packing, 1..5; 0 = 5 (fewer: more, smaller row groups) */
+  NAPMO_TUNE_SYNTH_MFMA_MIN = 9 /* many-density synthesis (exchange potential, orbital
+                                   corrections): whole tiles of 16 densities, and a last tile
+                                   of at least this many, run on FP64 MFMA, the others one at a
+                                   time; 1..16, 17 = one at a time only; 0 = 10 (a property of
+                                   the grid) */
 };
 int napmo_fock_plan_set_tuning(void *plan, int key, long value);
 int napmo_fock_plan_get_tuning(void *plan, int key, long *value);
+/* The grid-level keys (NAPMO_TUNE_SYNTH_NP, NAPMO_TUNE_SYNTH_MFMA_MIN) on a BeckeGrid handle. */
+int napmo_grid_set_tuning(void *grid, int key, long value);
+int napmo_grid_get_tuning(void *grid, int key, long *value);
 
 /* Batched builds of independent molecules (no reference counterpart; the reference
  * runs molecules one at a time). Each plan keeps its own grid, basis and buffers; the

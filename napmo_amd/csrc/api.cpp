@@ -1016,6 +1016,32 @@ int napmo_fock_plan_get_tuning(void *plan, int key, long *value) {
   return guarded([&] { *value = ((FockPlan *)plan)->get_tuning(key); });
 }
 
+int napmo_grid_set_tuning(void *grid, int key, long value) {
+  return guarded([&] {
+    DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    if (key == NAPMO_TUNE_SYNTH_NP) {
+      NAPMO_REQUIRE(value == 0 || value == 1 || value == 2, "synthesis pairs per lane must be 1 or 2");
+      g.synth_np = value ? (int)value : 2;
+    } else if (key == NAPMO_TUNE_SYNTH_MFMA_MIN) {
+      NAPMO_REQUIRE(value >= 0 && value <= 17, "MFMA synthesis threshold must be 0 to 17");
+      g.synth_mfma_min = value ? (int)value : SYNTH_MFMA_MIN;
+    } else {
+      throw Error("unknown grid tuning key " + std::to_string(key));
+    }
+  });
+}
+int napmo_grid_get_tuning(void *grid, int key, long *value) {
+  return guarded([&] {
+    const DeviceGrid &g = ((BeckeHandle *)grid)->grid;
+    if (key == NAPMO_TUNE_SYNTH_NP)
+      *value = g.synth_np;
+    else if (key == NAPMO_TUNE_SYNTH_MFMA_MIN)
+      *value = g.synth_mfma_min;
+    else
+      throw Error("unknown grid tuning key " + std::to_string(key));
+  });
+}
+
 int napmo_batch_plan_new(int nplan, void **plans, int nlanes, void **batch_out) {
   return guarded([&] {
     NAPMO_REQUIRE(nplan >= 1 && plans && batch_out, "napmo_batch_plan_new: bad arguments");

@@ -105,6 +105,10 @@ void FockPlan::set_tuning(int key, long value) {
       NAPMO_REQUIRE(value == 0 || value == 1 || value == 2, "synthesis pairs per lane must be 1 or 2");
       grid.synth_np = value ? (int)value : 2;
       break;
+    case NAPMO_TUNE_SYNTH_MFMA_MIN:
+      NAPMO_REQUIRE(value >= 0 && value <= 17, "MFMA synthesis threshold must be 0 to 17");
+      grid.synth_mfma_min = value ? (int)value : SYNTH_MFMA_MIN;
+      break;
     case NAPMO_TUNE_DENSITY_STREAM:
       NAPMO_REQUIRE(value >= 0 && value <= 3, "density streaming choice must be 0 to 3");
       density_stream = (int)value;
@@ -138,6 +142,7 @@ long FockPlan::get_tuning(int key) const {
     case NAPMO_TUNE_DENSITY_KS: return density_ks;
     case NAPMO_TUNE_DENSITY_PF: return density_pf;
     case NAPMO_TUNE_SYNTH_NP: return grid.synth_np;
+    case NAPMO_TUNE_SYNTH_MFMA_MIN: return grid.synth_mfma_min;
     case NAPMO_TUNE_DENSITY_STREAM: return density_stream;
     case NAPMO_TUNE_INT8_CHUNK: return prec == NAPMO_PREC_INT8 ? oz.chunk : 0;
     case NAPMO_TUNE_INT8_WAVE_BLOCKS: return prec == NAPMO_PREC_INT8 ? oz_wave_blocks : 0;

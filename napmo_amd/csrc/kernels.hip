@@ -294,16 +294,26 @@ void synth_order(int lmax, std::vector<int> &l, std::vector<int> &m, std::vector
 
 constexpr int SYNTH_TAB_J = 4;  // intervals per workgroup (the factors are formed once for them)
 
+// Layout of the tables: coefficient c of (set, centre A, interval j, q) at
+// T[set tset + (A (max_nrad - 1) + j) trow + q tq + c], amplitudes at
+// amp[set aset + A arow + q aq]. Sets nvalid .. gridDim.z - 1 are written as zeros (the
+// padding columns of the density-interleaved layout).
+struct SynthTabLayout {
+  long tset, trow, tq, aset, arow, aq;
+  int nvalid;
+};
+
 __global__ void k_synth_tables(AtomTables at, const double *__restrict__ Uy,
                                const double *__restrict__ Udt, int lmax, int L,
                                int write_amps, double *__restrict__ T,
                                double *__restrict__ ampL, double *__restrict__ ampR,
-                               long ustride, long tstride, long astride) {
+                               long ustride, SynthTabLayout lay) {
   const int A = blockIdx.y, jb = blockIdx.x * SYNTH_TAB_J;
-  // blockIdx.z = set: U + set ustride -> T + set tstride, amplitudes + set astride
+  // blockIdx.z = set: U + set ustride -> its tables
   const long set = blockIdx.z;
-  Uy += set * ustride, Udt += set * ustride, T += set * tstride;
-  ampL += set * astride, ampR += set * astride;
+  const bool pad = set >= lay.nvalid;
+  Uy += set * ustride, Udt += set * ustride, T += set * lay.tset;
+  ampL += set * lay.aset, ampR += set * lay.aset;
   const int nr = at.nrad[A];
   if (jb >= nr - 1) return;
   const long roff = at.rad_off[A];
@@ -313,10 +323,15 @@ __global__ void k_synth_tables(AtomTables at, const double *__restrict__ Uy,
     const int ref = l * l + (m ? 2 * m - 1 + cs : 0);
     const double f = q_factor(l, m) * pi_sigma(l, m);
     for (int j = jb; j < min(jb + SYNTH_TAB_J, nr - 1); ++j) {
+      double *t = T + ((long)A * (at.max_nrad - 1) + j) * lay.trow + q * lay.tq;
+      if (pad) {
+        t[0] = t[1] = t[2] = t[3] = 0.0;
+        if (j == 0 && write_amps) ampL[A * lay.arow + q * lay.aq] = ampR[A * lay.arow + q * lay.aq] = 0.0;
+        continue;
+      }
       const double y0 = Uy[(roff + j) * L + ref], y1 = Uy[(roff + j + 1) * L + ref];
       const double d0 = Udt[(roff + j) * L + ref], d1 = Udt[(roff + j + 1) * L + ref];
       const double zz = y1 - y0;
-      double *t = T + (((long)A * (at.max_nrad - 1) + j) * L + q) * 4;
       t[0] = y0 * f;
       t[1] = d0 * f;
       t[2] = (3 * zz - 2 * d0 - d1) * f;
@@ -325,8 +340,8 @@ __global__ void k_synth_tables(AtomTables at, const double *__restrict__ Uy,
         // PotentialExtrapolation::prepare (src/horton/extrapolation.cpp:156-159)
         const double r0 = at.r[roff], rn = at.r[roff + nr - 1];
         const double yn = Uy[(roff + nr - 1) * L + ref];
-        ampL[(long)A * L + q] = y0 / pow(r0, (double)l) * f;
-        ampR[(long)A * L + q] = yn * pow(rn, (double)(l + 1)) * f;
+        ampL[A * lay.arow + q * lay.aq] = y0 / pow(r0, (double)l) * f;
+        ampR[A * lay.arow + q * lay.aq] = yn * pow(rn, (double)(l + 1)) * f;
       }
     }
   }
@@ -342,9 +357,20 @@ void launch_synth_tables(const AtomTables &at, const double *Uy, const double *U
   const int L = (lmax + 1) * (lmax + 1);
   NAPMO_REQUIRE(nset >= 1 && nset <= 65535, "synthesis tables: too many sets");
   dim3 grid((unsigned)div_up(at.max_nrad - 1, SYNTH_TAB_J), at.M, nset);
-  const long tstride = synth_table_size(at, lmax), astride = (long)at.M * L;
-  k_synth_tables<<<grid, 256, 0, s>>>(at, Uy, Udt, lmax, L, write_amps, T, ampL, ampR, ustride,
-                                      tstride, astride);
+  const SynthTabLayout lay{synth_table_size(at, lmax), (long)L * 4, 4, (long)at.M * L, L, 1, nset};
+  k_synth_tables<<<grid, 256, 0, s>>>(at, Uy, Udt, lmax, L, write_amps, T, ampL, ampR, ustride, lay);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_synth_tables_cols(const AtomTables &at, const double *Uy, const double *Udt, int lmax,
+                              int nset, long ustride, int ndp, double *T, double *ampL,
+                              double *ampR, hipStream_t s) {
+  const int L = (lmax + 1) * (lmax + 1);
+  NAPMO_REQUIRE(nset >= 1 && nset <= ndp && ndp % 16 == 0 && ndp <= 65535,
+                "interleaved synthesis tables: 1 <= sets <= columns, columns a multiple of 16");
+  dim3 grid((unsigned)div_up(at.max_nrad - 1, SYNTH_TAB_J), at.M, ndp);
+  const SynthTabLayout lay{4, (long)L * ndp * 4, (long)ndp * 4, 1, (long)L * ndp, ndp, nset};
+  k_synth_tables<<<grid, 256, 0, s>>>(at, Uy, Udt, lmax, L, 1, T, ampL, ampR, ustride, lay);
   NAPMO_HIP(hipGetLastError());
 }
 
@@ -734,6 +760,55 @@ void launch_contrib_reduce(const double *contrib, int M, long N, int accumulate,
                            hipStream_t s, double scale) {
   if (N <= 0) return;
   k_contrib_reduce<<<div_up(N, 256), 256, 0, s>>>(contrib, M, N, accumulate, scale, v);
+  NAPMO_HIP(hipGetLastError());
+}
+
+// One thread per (point, density) of a tile, 16 densities of a point adjacent: the sum over
+// centres in index order, as k_contrib_reduce.
+__global__ void k_contrib_reduce_tile(const double *__restrict__ tile, int M, long N, int nvalid,
+                                      double scale, double *__restrict__ out, long ostride) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long p = e >> 4;
+  const int k = (int)(e & 15);
+  if (p >= N || k >= nvalid) return;
+  double s = 0.0;
+  for (int A = 0; A < M; ++A) s += tile[((long)A * N + p) * 16 + k];
+  out[k * ostride + p] = scale * s;
+}
+
+void launch_contrib_reduce_tile(const double *tile, int M, long N, int nvalid, double scale,
+                                double *out, long ostride, hipStream_t s) {
+  if (N <= 0 || nvalid <= 0) return;
+  NAPMO_REQUIRE(nvalid <= 16, "tile reduction: at most 16 densities");
+  k_contrib_reduce_tile<<<div_up(N * 16, 256), 256, 0, s>>>(tile, M, N, nvalid, scale, out, ostride);
+  NAPMO_HIP(hipGetLastError());
+}
+
+template <int LM>
+static void synth_mfma_dispatch(int lmax, hipStream_t s, const AtomTables &at, const double *geo,
+                                long N, const int *list, const int *wb, long nw, int ntile,
+                                const double *T, const double *aL, const double *aR,
+                                double *contrib) {
+  if constexpr (LM > MAX_LMAX) {
+    throw Error("lmax above the supported maximum " + std::to_string(MAX_LMAX));
+  } else {
+    if (lmax == LM)
+      launch_synth_mfma_t<LM>(s, at, geo, N, list, wb, nw, ntile, 16 * ntile, T, aL, aR,
+                              (long)at.M * N * 16, contrib);
+    else
+      synth_mfma_dispatch<LM + 1>(lmax, s, at, geo, N, list, wb, nw, ntile, T, aL, aR, contrib);
+  }
+}
+
+void launch_synth_mfma(const AtomTables &at, const double *geo, long N, const int *list,
+                       const int *wave_bucket, long nwaves, int ntile, const double *T,
+                       const double *ampL, const double *ampR, int lmax, double *contrib,
+                       hipStream_t s) {
+  NAPMO_REQUIRE(lmax >= 0 && lmax <= MAX_LMAX, "lmax out of range for synthesis");
+  NAPMO_REQUIRE(ntile >= 1 && ntile <= 4096, "MFMA synthesis: 1 to 4096 tiles of 16 densities");
+  if (nwaves <= 0) return;
+  synth_mfma_dispatch<0>(lmax, s, at, geo, N, list, wave_bucket, nwaves, ntile, T, ampL, ampR,
+                         contrib);
   NAPMO_HIP(hipGetLastError());
 }
 

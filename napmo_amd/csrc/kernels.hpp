@@ -126,6 +126,26 @@ void launch_synth2(const AtomTables &at, const double *geo, long N,
                    hipStream_t s, int np = 2);
 void launch_contrib_reduce(const double *contrib, int M, long N, int accumulate, double *v,
                            hipStream_t s, double scale = 1.0);
+// Synthesis of many densities on FP64 MFMA (synth_mfma.hpp): 16 densities per tile.
+// Tables in the density-interleaved layout of ndp = 16 ntile columns (sets nset .. ndp - 1
+// written as zeros): T [A][interval][q][ndp][4], ampL / ampR [A][q][ndp], from the radial
+// solutions of set s at Uy, Udt + s ustride.
+void launch_synth_tables_cols(const AtomTables &at, const double *Uy, const double *Udt, int lmax,
+                              int nset, long ustride, int ndp, double *T, double *ampL,
+                              double *ampR, hipStream_t s);
+// contrib tile t [M][N][16] at contrib + t M N 16 for t < ntile (pairs as for launch_synth2).
+void launch_synth_mfma(const AtomTables &at, const double *geo, long N, const int *list,
+                       const int *wave_bucket, long nwaves, int ntile, const double *T,
+                       const double *ampL, const double *ampR, int lmax, double *contrib,
+                       hipStream_t s);
+template <int LMAX>
+void launch_synth_mfma_t(hipStream_t s, const AtomTables &at, const double *geo, long N,
+                         const int *list, const int *wave_bucket, long nslots, int ntile,
+                         int ndp, const double *T, const double *aL, const double *aR,
+                         long tstride, double *contrib);
+// out[k ostride + p] = scale sum_A tile[A][p][k] for the nvalid densities of one tile.
+void launch_contrib_reduce_tile(const double *tile, int M, long N, int nvalid, double scale,
+                                double *out, long ostride, hipStream_t s);
 // v = sum_A contrib[A] and sv = v wbw in one pass (sv [Npad], zero beyond N): the same
 // values as launch_contrib_reduce followed by launch_scale3(v, wbw).
 void launch_contrib_reduce_scale(const double *contrib, int M, long N, long Npad, const double *wbw,

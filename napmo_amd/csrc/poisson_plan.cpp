@@ -666,7 +666,6 @@ void DeviceGrid::orbital_correction(int lmax, int norb, const double *d_phi, con
   PoissonPlan &pl = plan(lmax);
   HelmholtzPlan &hp = helmholtz();
   AtomTables at = tables(pl);
-  SynthPairs &sp = synth_pairs();
   const int L = (lmax + 1) * (lmax + 1);
   long S = 0;
   for (auto &a : atoms) S += a.nrad;
@@ -699,29 +698,7 @@ void DeviceGrid::orbital_correction(int lmax, int norb, const double *d_phi, con
   launch_helmholtz_radial(at, hp.tables(), lmax, norb, S, ow.Vsph.get(), d_oi, d_doi, mass_inv,
                           ow.mu.get(), ow.Uy.get(), ow.Udt.get(), ow.status.get(), stream);
   toc();
-  // synthesis tables of up to SET_BATCH orbitals per launch, then one synthesis per orbital
-  const long tsz = synth_table_size(at, lmax);
-  const int nb = std::min(SET_BATCH, norb);
-  ow.T.resize((size_t)nb * tsz);
-  ow.ampL.resize((size_t)nb * M * L);
-  ow.ampR.resize((size_t)nb * M * L);
-  for (int o0 = 0; o0 < norb; o0 += SET_BATCH) {
-    const int cnt = std::min(SET_BATCH, norb - o0);
-    tic(4);
-    launch_synth_tables(at, ow.Uy.get() + (long)o0 * S * L, ow.Udt.get() + (long)o0 * S * L, lmax,
-                        ow.T.get(), ow.ampL.get(), ow.ampR.get(), stream, 1, cnt, S * L);
-    toc();
-    for (int k = 0; k < cnt; ++k) {
-      tic(5);
-      launch_synth2(at, sp.geo.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves,
-                    ow.T.get() + k * tsz, ow.ampL.get() + (long)k * M * L,
-                    ow.ampR.get() + (long)k * M * L, lmax, sp.contrib.get(), stream, synth_np);
-      toc();
-      tic(6);
-      launch_contrib_reduce(sp.contrib.get(), M, N, 0, d_out + (long)(o0 + k) * N, stream);
-      toc();
-    }
-  }
+  synth_sets(lmax, norb, ow.Uy.get(), ow.Udt.get(), S * L, d_out, N);
   int h_status = 0;
   NAPMO_HIP(hipMemcpyAsync(&h_status, ow.status.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
   NAPMO_HIP(hipStreamSynchronize(stream));
@@ -733,10 +710,9 @@ void DeviceGrid::exchange_potential(int ndim, const double *d_psi, const double 
   if (ndim <= 0) return;
   PoissonPlan &pl = plan(lmax);
   AtomTables at = tables(pl);
-  SynthPairs &sp = synth_pairs();
   // orbital pairs (s, v), v <= s, in order, SET_BATCH at a time: pair densities, their
-  // projections, radial solves and synthesis tables in one launch each, then per pair the
-  // synthesis, centre sum and accumulation into K
+  // projections and radial solves in one launch each, the synthesis of the batch
+  // (synth_sets), then per pair the accumulation into K
   std::vector<std::pair<int, int>> prs;
   for (int s = 0; s < ndim; ++s)
     for (int v = 0; v <= s; ++v) prs.push_back({s, v});
@@ -744,16 +720,12 @@ void DeviceGrid::exchange_potential(int ndim, const double *d_psi, const double 
   const int L = (lmax + 1) * (lmax + 1);
   long S = 0;
   for (auto &a : atoms) S += a.nrad;
-  const long tsz = synth_table_size(at, lmax);
   xchi.resize((size_t)ndim * N);
   xdens.resize((size_t)nb * N);
-  xaux.resize(N);
+  xaux.resize((size_t)nb * N);
   xrho.resize((size_t)nb * S * L);
   xUy.resize((size_t)nb * S * L);
   xUdt.resize((size_t)nb * S * L);
-  xT.resize((size_t)nb * tsz);
-  xampL.resize((size_t)nb * M * L);
-  xampR.resize((size_t)nb * M * L);
   if (prof_on) ++prof_calls;
   launch_exchange_chi(d_psi, d_D, ndim, N, xchi.get(), stream);
   launch_axpby(0.0, xchi.get(), 0.0, xchi.get(), (long)ndim * N, d_K, stream);  // K = 0
@@ -766,22 +738,70 @@ void DeviceGrid::exchange_potential(int ndim, const double *d_psi, const double 
     launch_project(at, xdens.get(), bw.get(), pl.yhat.get(), lmax, xrho.get(), stream, cnt, N, S * L);
     launch_radial(at, xrho.get(), pl.green.get(), lmax, xUy.get(), xUdt.get(), stream, cnt, S * L);
     toc();
-    tic(4);
-    launch_synth_tables(at, xUy.get(), xUdt.get(), lmax, xT.get(), xampL.get(), xampR.get(), stream,
-                        1, cnt, S * L);
+    synth_sets(lmax, cnt, xUy.get(), xUdt.get(), S * L, xaux.get(), N);
+    tic(6);
+    for (int k = 0; k < cnt; ++k)
+      launch_exchange_acc(xaux.get() + (long)k * N, xchi.get(), prs[c0 + k].first,
+                          prs[c0 + k].second, N, d_K, stream);
     toc();
-    for (int k = 0; k < cnt; ++k) {
-      tic(5);
-      launch_synth2(at, sp.geo.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves,
-                    xT.get() + k * tsz, xampL.get() + (long)k * M * L, xampR.get() + (long)k * M * L,
-                    lmax, sp.contrib.get(), stream, synth_np);
-      toc();
-      tic(6);
-      launch_contrib_reduce(sp.contrib.get(), M, N, 0, xaux.get(), stream);
-      launch_exchange_acc(xaux.get(), xchi.get(), prs[c0 + k].first, prs[c0 + k].second, N, d_K,
-                          stream);
-      toc();
-    }
+  }
+}
+
+template <class T> static void grow(DeviceBuffer<T> &b, size_t n) {
+  if (b.size() < n) b.resize(n);
+}
+
+void DeviceGrid::synth_sets(int lmax, int nset, const double *Uy, const double *Udt, long ustride,
+                            double *out, long ostride, double scale) {
+  if (nset <= 0) return;
+  PoissonPlan &pl = plan(lmax);
+  AtomTables at = tables(pl);
+  SynthPairs &sp = synth_pairs();
+  const long L = (long)(lmax + 1) * (lmax + 1);
+  const long tsz = synth_table_size(at, lmax);
+  int nmfma = synth_mfma_min > 16 ? 0 : (nset / 16) * 16;
+  if (synth_mfma_min <= 16 && nset - nmfma >= std::max(synth_mfma_min, 1)) nmfma = nset;
+  const long tile = (long)M * N * 16;
+  const int per = (int)std::max<long>(1, SYNTH_MFMA_BUDGET / (tile * (long)sizeof(double)));
+  for (int s0 = 0; s0 < nmfma; s0 += 16 * per) {
+    const int cnt = std::min(16 * per, nmfma - s0);
+    const int ntile = (cnt + 15) / 16, ndp = 16 * ntile;
+    grow(mT, (size_t)tsz * ndp);
+    grow(maL, (size_t)M * L * ndp);
+    grow(maR, (size_t)M * L * ndp);
+    grow(mcontrib, (size_t)tile * ntile);
+    tic(4);
+    launch_synth_tables_cols(at, Uy + s0 * ustride, Udt + s0 * ustride, lmax, cnt, ustride, ndp,
+                             mT.get(), maL.get(), maR.get(), stream);
+    toc();
+    tic(7);
+    launch_synth_mfma(at, sp.geo.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves, ntile,
+                      mT.get(), maL.get(), maR.get(), lmax, mcontrib.get(), stream);
+    toc();
+    tic(6);
+    for (int t = 0; t < ntile; ++t)
+      launch_contrib_reduce_tile(mcontrib.get() + t * tile, M, N, std::min(16, cnt - 16 * t), scale,
+                                 out + (long)(s0 + 16 * t) * ostride, ostride, stream);
+    toc();
+  }
+  const int rest = nset - nmfma;
+  if (rest <= 0) return;
+  grow(sT, (size_t)tsz * rest);
+  grow(saL, (size_t)M * L * rest);
+  grow(saR, (size_t)M * L * rest);
+  tic(4);
+  launch_synth_tables(at, Uy + nmfma * ustride, Udt + nmfma * ustride, lmax, sT.get(), saL.get(),
+                      saR.get(), stream, 1, rest, ustride);
+  toc();
+  for (int k = 0; k < rest; ++k) {
+    tic(5);
+    launch_synth2(at, sp.geo.get(), N, sp.list.get(), sp.wave_bucket.get(), sp.nwaves,
+                  sT.get() + k * tsz, saL.get() + k * M * L, saR.get() + k * M * L, lmax,
+                  sp.contrib.get(), stream, synth_np);
+    toc();
+    tic(6);
+    launch_contrib_reduce(sp.contrib.get(), M, N, 0, out + (long)(nmfma + k) * ostride, stream, scale);
+    toc();
   }
 }
 

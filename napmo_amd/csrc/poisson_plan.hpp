@@ -73,6 +73,13 @@ struct PoissonPlan {
 // Densities (orbitals, orbital pairs) whose projections, radial solves and synthesis tables
 // go into one launch each in the orbital update and the exchange build.
 constexpr int SET_BATCH = 64;
+// MFMA synthesis: at most this many bytes of per-centre tile sums [M][N][16] at once (at
+// least one tile)
+constexpr long SYNTH_MFMA_BUDGET = 4L << 30;
+// A tile of 16 densities on MFMA costs about as much as ten one-density syntheses (2.7 ms
+// against 0.28 ms each on 16 centres x 472,000 points at lmax 12), so a last tile takes the
+// MFMA path from ten densities on.
+constexpr int SYNTH_MFMA_MIN = 10;
 
 // Per-geometry pair buckets of the bucketed synthesis (see launch_synth2).
 struct SynthPairs {
@@ -94,7 +101,6 @@ struct HelmholtzPlan {
 // change): sources, projections mu, solutions (y, dt), shell averages, synthesis tables.
 struct OrbitalWork {
   DeviceBuffer<double> src, mu, Uy, Udt, Vsph, resid;
-  DeviceBuffer<double> T, ampL, ampR;  // synthesis tables of up to SET_BATCH orbitals
   DeviceBuffer<int> status;
   PoissonWork pw;
   int lmax = -1;
@@ -120,9 +126,10 @@ struct DeviceGrid {
   std::unique_ptr<SynthPairs> pairs;
   std::unique_ptr<HelmholtzPlan> hplan;
   std::unique_ptr<OrbitalWork> owork;
-  // Per-phase HIP-event timing of the orbital corrections (phases: residual, sources and
-  // projection, shell averages, radial solves, synthesis tables, synthesis, reduction).
-  static constexpr int DPSI_NPHASE = 7;
+  // Per-phase HIP-event timing of the orbital corrections and exchange builds (phases:
+  // residual, sources and projection, shell averages, radial solves, synthesis tables,
+  // one-density synthesis, reduction, MFMA synthesis).
+  static constexpr int DPSI_NPHASE = 8;
   bool prof_on = false;
   int prof_calls = 0;
   size_t prof_next = 0;
@@ -171,7 +178,18 @@ struct DeviceGrid {
   // Poisson solves of orbital products.
   void exchange_potential(int ndim, const double *d_psi, const double *d_D, int lmax, double *d_K);
   DeviceBuffer<double> xchi, xdens, xaux;  // its workspace, kept between calls
-  DeviceBuffer<double> xrho, xUy, xUdt, xT, xampL, xampR;  // per batch of orbital pairs
+  DeviceBuffer<double> xrho, xUy, xUdt;    // per batch of orbital pairs
+  // Synthesis of nset densities on this grid from their radial solutions (set s at
+  // Uy, Udt + s ustride, plan(lmax) order): out[s ostride + p] = scale v_s[p]. Whole tiles
+  // of 16 densities, and a last tile of at least synth_mfma_min, go through the FP64 MFMA
+  // synthesis (launch_synth_mfma); the others one at a time through the bucketed synthesis.
+  // Profiling phases 4 (tables), 5 (one-density synthesis), 7 (MFMA synthesis), 6 (centre
+  // sums).
+  void synth_sets(int lmax, int nset, const double *Uy, const double *Udt, long ustride,
+                  double *out, long ostride, double scale = 1.0);
+  int synth_mfma_min = SYNTH_MFMA_MIN;
+  DeviceBuffer<double> mT, maL, maR, mcontrib;  // its workspace (interleaved tables, tiles)
+  DeviceBuffer<double> sT, saL, saR;            // tables of the one-at-a-time densities
   // Residual R = T + V - e psi and de = integrate(R psi) of norb orbitals, then their
   // corrections (napmo/wavefunction/psi_optimization.py:175-203).
   void becke_dickson_step(int lmax, int norb, const double *d_psi, const double *d_T,

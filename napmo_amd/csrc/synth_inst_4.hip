@@ -1,5 +1,6 @@
 // Explicit instantiations of the synthesis kernel for lmax 18..19.
 #include "synth_kernel.hpp"
+#include "synth_mfma.hpp"
 
 namespace napmo {
 template void launch_synth_t<18>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
@@ -14,4 +15,10 @@ template void launch_synth2_t<18>(int, hipStream_t, const AtomTables &, const do
 template void launch_synth2_t<19>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
+template void launch_synth_mfma_t<18>(hipStream_t, const AtomTables &, const double *, long,
+                                     const int *, const int *, long, int, int, const double *,
+                                     const double *, const double *, long, double *);
+template void launch_synth_mfma_t<19>(hipStream_t, const AtomTables &, const double *, long,
+                                     const int *, const int *, long, int, int, const double *,
+                                     const double *, const double *, long, double *);
 }  // namespace napmo

@@ -1,5 +1,6 @@
 // Explicit instantiations of the synthesis kernel for lmax 24..24.
 #include "synth_kernel.hpp"
+#include "synth_mfma.hpp"
 
 namespace napmo {
 template void launch_synth_t<24>(dim3, dim3, hipStream_t, const AtomTables &, const double *,
@@ -8,4 +9,7 @@ template void launch_synth_t<24>(dim3, dim3, hipStream_t, const AtomTables &, co
 template void launch_synth2_t<24>(int, hipStream_t, const AtomTables &, const double *, long, const int *,
                                    const int *, long, const double *, const double *,
                                    const double *, double *);
+template void launch_synth_mfma_t<24>(hipStream_t, const AtomTables &, const double *, long,
+                                     const int *, const int *, long, int, int, const double *,
+                                     const double *, const double *, long, double *);
 }  // namespace napmo

@@ -68,6 +68,16 @@ class BeckeGrid(object):
                                      np.ascontiguousarray(self.points), cell._this, len(cubic_splines),
                                      self.size)
 
+    def set_tuning(self, key, value):
+        """Grid-level launch knobs (napmo_grid_set_tuning): key 5 = synthesis pairs per lane,
+        9 = the smallest last tile of the FP64 MFMA many-density synthesis (17: never)."""
+        cext.check(cext.napmo_grid_set_tuning(self._this, int(key), int(value)))
+
+    def get_tuning(self, key):
+        v = ctypes.c_long()
+        cext.check(cext.napmo_grid_get_tuning(self._this, int(key), ctypes.byref(v)))
+        return v.value
+
     def integrate(self, f):
         return cext.BeckeGrid_integrate(self._this, np.ascontiguousarray(f, dtype=np.float64))
 

@@ -208,6 +208,8 @@ _SIGNATURES = {
     'napmo_grid_poisson_adjoint': (c_int, [vp, vp, vp]),
     'napmo_fock_plan_set_tuning': (c_int, [vp, c_int, c_long]),
     'napmo_fock_plan_get_tuning': (c_int, [vp, c_int, ctypes.POINTER(c_long)]),
+    'napmo_grid_set_tuning': (c_int, [vp, c_int, c_long]),
+    'napmo_grid_get_tuning': (c_int, [vp, c_int, ctypes.POINTER(c_long)]),
     'napmo_batch_plan_new': (c_int, [c_int, vp, c_int, ctypes.POINTER(vp)]),
     'napmo_batch_plan_del': (None, [vp]),
     'napmo_batch_build': (c_int, [vp, vp, vp, vp]),
