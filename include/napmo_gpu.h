@@ -458,11 +458,17 @@ int napmo_grid_set_tuning(void *grid, int key, long value);
 int napmo_grid_get_tuning(void *grid, int key, long *value);
 
 /* Batched builds of independent molecules (no reference counterpart; the reference
- * runs molecules one at a time). Each plan keeps its own grid, basis and buffers; the
- * batch spreads the plans over nlanes HIP streams forked from and joined to one origin
- * stream, and can capture one build of every plan into a HIP graph: after
- * napmo_batch_capture, napmo_batch_replay re-runs all builds on the same device pointers
- * (write the next P into them between replays). The plans must outlive the batch. */
+ * runs molecules one at a time). Each plan keeps its own grid, basis and buffers.
+ * nlanes 1..16: the batch spreads the plans' own launches over nlanes HIP streams forked
+ * from and joined to one origin stream. nlanes 0: one launch per phase for all molecules
+ * (FP64 plans with the per-tile contraction): the molecules' atoms and points side by
+ * side in one grid for the Poisson steps (each molecule keeps its own lmax and its own
+ * (point, centre) pairs), grouped launches for the per-molecule padding, density,
+ * centre sums, Fock partials, reduction and traces; the results equal the plans' own
+ * builds bit for bit. Either batch can capture one build of every plan into a HIP graph:
+ * after napmo_batch_capture, napmo_batch_replay re-runs all builds on the same device
+ * pointers (write the next P into them between replays). The plans must outlive the
+ * batch. */
 int napmo_batch_plan_new(int nplan, void **plans, int nlanes, void **batch_out);
 void napmo_batch_plan_del(void *batch);
 int napmo_batch_build(void *batch, const double **dP, double **dF, double **dE);

@@ -2,7 +2,8 @@
 HIP graph (napmo_batch_* in libnapmo_gpu.so).
 
 ``BatchFock(plans, nlanes)`` takes GridFock plans (one per molecule, each with its own
-grid and basis) and spreads their builds over ``nlanes`` HIP streams. ``capture`` records
+grid and basis) and spreads their builds over ``nlanes`` HIP streams; ``nlanes=0`` runs
+every phase of all the builds as one launch (FP64 plans). ``capture`` records
 one build of every molecule into a graph on fixed device buffers; ``replay`` re-runs it,
 one graph launch per SCF iteration, after the caller has written the next density
 matrices into the same buffers.

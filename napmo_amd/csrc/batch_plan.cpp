@@ -5,7 +5,11 @@ namespace napmo {
 
 BatchPlan::BatchPlan(const std::vector<FockPlan *> &p, int nlanes) : plans(p) {
   NAPMO_REQUIRE(!plans.empty(), "batch plan: no plans");
-  NAPMO_REQUIRE(nlanes >= 1 && nlanes <= 16, "batch plan: 1 to 16 lanes");
+  NAPMO_REQUIRE(nlanes >= 0 && nlanes <= 16, "batch plan: 0 (one launch per phase) to 16 lanes");
+  if (nlanes == 0) {
+    group = std::make_unique<GroupPlan>(plans);
+    return;
+  }
   for (size_t m = 0; m < plans.size(); ++m) {
     FockPlan *q = plans[m];
     NAPMO_REQUIRE(q != nullptr, "batch plan: null plan");
@@ -31,6 +35,7 @@ BatchPlan::BatchPlan(const std::vector<FockPlan *> &p, int nlanes) : plans(p) {
 }
 
 BatchPlan::~BatchPlan() {
+  if (group) return;
   (void)hipStreamSynchronize(origin);
   for (auto s : lanes) (void)hipStreamSynchronize(s);
   for (size_t m = 0; m < plans.size(); ++m) plans[m]->stream = saved[m];
@@ -43,6 +48,7 @@ BatchPlan::~BatchPlan() {
 }
 
 void BatchPlan::build(const double *const *dP, double *const *dF, double *const *dE) {
+  if (group) return group->build(dP, dF, dE);
   NAPMO_HIP(hipEventRecord(fork, origin));
   for (auto s : lanes) NAPMO_HIP(hipStreamWaitEvent(s, fork, 0));
   for (size_t m = 0; m < plans.size(); ++m) plans[m]->build_device(dP[m], dF[m], dE[m]);
@@ -53,6 +59,7 @@ void BatchPlan::build(const double *const *dP, double *const *dF, double *const 
 }
 
 void BatchPlan::capture(const double *const *dP, double *const *dF, double *const *dE) {
+  if (group) return group->capture(dP, dF, dE);
   for (auto *q : plans)
     NAPMO_REQUIRE(q->prof_used >= q->prof_cap, "batch capture: stop per-phase profiling first");
   synchronize();
@@ -72,10 +79,14 @@ void BatchPlan::capture(const double *const *dP, double *const *dF, double *cons
 }
 
 void BatchPlan::replay() {
+  if (group) return group->replay();
   NAPMO_REQUIRE(exec != nullptr, "batch replay: capture first");
   NAPMO_HIP(hipGraphLaunch(exec, origin));
 }
 
-void BatchPlan::synchronize() { NAPMO_HIP(hipStreamSynchronize(origin)); }
+void BatchPlan::synchronize() {
+  if (group) return group->synchronize();
+  NAPMO_HIP(hipStreamSynchronize(origin));
+}
 
 }  // namespace napmo

@@ -1,11 +1,15 @@
-// Many independent molecules (one FockPlan each) built together: the plans' launches are
-// spread over a few HIP streams (fork / join with events on one origin stream) and the
-// whole batch can be captured once into a HIP graph and replayed per SCF iteration.
+// Many independent molecules (one FockPlan each) built together, in one of two ways:
+// nlanes >= 1: the plans' own launches spread over nlanes HIP streams (fork / join with
+// events on one origin stream); nlanes = 0: one launch per phase for all molecules
+// (GroupPlan, FP64 plans). Either way the whole batch can be captured once into a HIP
+// graph and replayed per SCF iteration.
 #pragma once
 
+#include <memory>
 #include <vector>
 
 #include "fock_plan.hpp"
+#include "group_plan.hpp"
 
 namespace napmo {
 
@@ -18,6 +22,7 @@ struct BatchPlan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<hipStream_t> saved;      // the plans' own streams, restored on destruction
+  std::unique_ptr<GroupPlan> group;    // nlanes = 0
 
   BatchPlan(const std::vector<FockPlan *> &plans, int nlanes);
   ~BatchPlan();

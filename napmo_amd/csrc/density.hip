@@ -38,18 +38,18 @@ __device__ inline bool xcd_tile(int g, int nct, int npt, int &pt, int &ct) {
   return pt < npt;
 }
 
+// The kernel body for workgroup g (blockIdx.x, or the local index of a grouped launch).
 template <int NB, int MODE, int DKS, int PF>
-__global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__ phi, int ld,
-                                                      long Npad, const double *__restrict__ B,
-                                                      int ldb, int kend, double scale,
-                                                      long nvalid,
-                                                      double *__restrict__ partial) {
+__device__ __forceinline__ void density_mfma_body(const double *__restrict__ phi, int ld,
+                                                  long Npad, const double *__restrict__ B,
+                                                  int ldb, int kend, double scale, long nvalid,
+                                                  double *__restrict__ partial, int g) {
   constexpr int BS = bstride(NB);
   __shared__ double As[2][DKS][ASTRIDE];  // As[.][k][m] = Phi[p0 + m][k0 + k]
   __shared__ double Bs[2][DKS][BS];       // Bs[.][k][c] = B[k0 + k][c0 + c]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int pt, ct;
-  if (!xcd_tile(blockIdx.x, ldb / (16 * NB), (int)(Npad / DPT), pt, ct)) return;
+  if (!xcd_tile(g, ldb / (16 * NB), (int)(Npad / DPT), pt, ct)) return;
   const long p0 = (long)pt * DPT;
   const int c0 = ct * 16 * NB;
   // this thread's share of a stage: Phi row (tid / 2), 8 columns from (tid % 2) * 8;
@@ -162,6 +162,27 @@ __global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__
       sum += __shfl_xor(sum, 1);
       if (mi == 0 && p < nvalid) partial[(long)ct * Npad + p] = scale * sum;
     }
+}
+
+template <int NB, int MODE, int DKS, int PF>
+__global__ void __launch_bounds__(256) k_density_mfma(const double *__restrict__ phi, int ld,
+                                                      long Npad, const double *__restrict__ B,
+                                                      int ldb, int kend, double scale,
+                                                      long nvalid,
+                                                      double *__restrict__ partial) {
+  density_mfma_body<NB, MODE, DKS, PF>(phi, ld, Npad, B, ldb, kend, scale, nvalid, partial,
+                                       blockIdx.x);
+}
+
+// Many plans in one launch: workgroup blockIdx.x belongs to the member whose block range
+// holds it (block0 multiples of 8, so the XCD-aware tile order of each member is kept).
+template <int NB, int MODE, int DKS, int PF>
+__global__ void __launch_bounds__(256) k_density_mfma_group(const DensityMember *__restrict__ mem,
+                                                            int nmem) {
+  const long g = blockIdx.x;
+  const DensityMember &d = mem[group_member(mem, nmem, g)];
+  density_mfma_body<NB, MODE, DKS, PF>(d.phi, d.ld, d.Npad, d.B, d.ldb, d.kend, d.scale, d.nvalid,
+                                       d.partial, (int)(g - d.block0));
 }
 
 // Orbital form with C resident in LDS: the whole zero-padded C [kend x 16 NB] is copied
@@ -313,6 +334,34 @@ void launch_density_nb(int nb, dim3 grid, const double *phi, int ld, long Npad, 
 }
 
 }  // namespace
+
+long density_group_fill(DensityMember &d, const double *phi, int ld, long Npad, const double *P,
+                        double *partial, int nreal, int ks, long block0) {
+  NAPMO_REQUIRE(ld % TILE == 0 && Npad % DPT == 0 && block0 % 8 == 0, "grouped density: padding");
+  const int dks = (ks == 32 && ld % 32 == 0) ? 32 : 16;
+  d.phi = phi, d.B = P, d.partial = partial, d.Npad = Npad, d.nvalid = Npad, d.block0 = block0;
+  d.ld = ld, d.ldb = ld, d.scale = 1.0;
+  d.kend = (nreal > 0 && nreal < ld) ? (int)std::min<long>(ld, round_up(nreal, dks)) : ld;
+  return round_up(Npad / DPT, 8) * (ld / TILE);
+}
+
+void launch_density_group(const DensityMember *d_mem, int nmem, long nblocks, int ks, int pf,
+                          hipStream_t s) {
+  if (nmem <= 0 || nblocks <= 0) return;
+  const dim3 grid((unsigned)nblocks);
+  if (ks == 32) {
+    if (pf == 2)
+      k_density_mfma_group<4, MODE_P, 32, 2><<<grid, 256, 0, s>>>(d_mem, nmem);
+    else
+      k_density_mfma_group<4, MODE_P, 32, 1><<<grid, 256, 0, s>>>(d_mem, nmem);
+  } else {
+    if (pf == 2)
+      k_density_mfma_group<4, MODE_P, 16, 2><<<grid, 256, 0, s>>>(d_mem, nmem);
+    else
+      k_density_mfma_group<4, MODE_P, 16, 1><<<grid, 256, 0, s>>>(d_mem, nmem);
+  }
+  NAPMO_HIP(hipGetLastError());
+}
 
 void launch_density_f64(const double *phi, int ld, long Npad, const double *P, double *partial,
                         hipStream_t s, int ks, int pf, int nreal) {

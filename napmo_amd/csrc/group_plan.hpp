@@ -1,0 +1,65 @@
+// Many independent molecules built with one launch per phase (FP64 plans): the molecules'
+// atoms and points side by side in one grid, so the Poisson steps (projection, radial
+// Green solves, synthesis tables) are single launches and the synthesis over each
+// molecule's own (point, centre) pairs is one launch per distinct lmax (the kernel is
+// compiled per lmax); the per-molecule contractions (padding, density, centre sums, Fock
+// partials and reduction, traces) are grouped launches whose workgroups find their
+// molecule in a member table.
+#pragma once
+
+#include <vector>
+
+#include "fock_plan.hpp"
+
+namespace napmo {
+
+struct GroupPlan {
+  std::vector<FockPlan *> plans;  // borrowed; must outlive the group
+  hipStream_t stream = nullptr;
+  DeviceGrid ug;                  // every molecule's atoms and points, in plan order
+  int lmax = 0;                   // the largest lmax; each molecule keeps its own (lcut)
+  std::vector<long> p0;           // first point of each molecule in ug
+  std::vector<int> A0;            // first atom of each molecule in ug
+  DeviceBuffer<long> cbase;       // per atom of ug: its row of the per-centre sums
+  DeviceBuffer<int> lcut;         // per atom of ug: its molecule's lmax
+  DeviceBuffer<long> toff;        // per atom of ug: its synthesis-table rows (own lmax)
+  SynthPairs pairs;               // the molecules' own pair buckets, side by side, the
+                                  // molecules ordered by lmax
+  struct SynthRange {             // the slots of the molecules of one lmax
+    int lmax;
+    long begin, count;
+  };
+  std::vector<SynthRange> ranges;
+  DeviceBuffer<double> rho, v;    // [ug.N]
+  int ks = 16, pf = 1;            // density stage shape (does not change results)
+  // member tables (device) and workgroup counts of the grouped launches
+  DeviceBuffer<DensityMember> dmem;
+  DeviceBuffer<RhoReduceMember> rmem;
+  DeviceBuffer<ScaleMember> smem;
+  DeviceBuffer<FockMember> fmem;
+  DeviceBuffer<PadMember> pmem;
+  DeviceBuffer<FockReduceMember> frmem;
+  DeviceBuffer<TraceMember> tmem;
+  long dens_blocks = 0, rho_blocks = 0, scale_blocks = 0, fock_blocks = 0, pad_blocks = 0,
+       fred_blocks = 0;
+  // the input / output pointers the pad, reduction and trace tables were made for
+  std::vector<const double *> cur_P;
+  std::vector<double *> cur_F, cur_E;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  explicit GroupPlan(const std::vector<FockPlan *> &plans);
+  ~GroupPlan();
+  // One build of every plan: dP[m], dF[m] [n_m x n_m], dE[m] [2] (device pointers);
+  // asynchronous on the group's stream. Results equal the plans' own builds bit for bit.
+  void build(const double *const *dP, double *const *dF, double *const *dE);
+  void capture(const double *const *dP, double *const *dF, double *const *dE);
+  void replay();
+  void synchronize();
+
+ private:
+  void io_tables(const double *const *dP, double *const *dF, double *const *dE);
+  void launches();
+};
+
+}  // namespace napmo

@@ -132,6 +132,8 @@ __global__ void __launch_bounds__(64 * PSL) k_project(AtomTables at, const doubl
   const double *Y = yhat + at.yhat_off[A];
   const int lml = threadIdx.x & 63, slice = threadIdx.x >> 6;
   const int lm = blockIdx.y * 64 + lml;
+  // harmonics above the centre's cut are zero (written, not summed)
+  const int Lc = at.lcut ? (at.lcut[A] + 1) * (at.lcut[A] + 1) : L;
   double acc[PSB];
 #pragma unroll
   for (int k = 0; k < PSB; ++k) acc[k] = 0.0;
@@ -144,7 +146,7 @@ __global__ void __launch_bounds__(64 * PSL) k_project(AtomTables at, const doubl
       sf[k][j] = (k < ns && j < nj) ? dens[p] * bw[p] : 0.0;
     }
     __syncthreads();
-    if (lm < L) {
+    if (lm < Lc) {
       // pairs (j, j + PSL) in order; the Yhat loads of the next two pairs are in flight
       // while one pair is summed
       int j = slice;
@@ -173,7 +175,7 @@ __global__ void __launch_bounds__(64 * PSL) k_project(AtomTables at, const doubl
     for (int k = 0; k < ns; ++k) {
       double t = 0.0;
       for (int sl = 0; sl < PSL; ++sl) t += red[sl][k][lml];
-      rho_lm[(at.rad_off[A] + i0 + k) * (long)L + lm] = t;
+      rho_lm[(at.rad_off[A] + i0 + k) * (long)L + lm] = lm < Lc ? t : 0.0;
     }
 }
 
@@ -202,6 +204,7 @@ __global__ void __launch_bounds__(RADIAL_CHUNK) k_radial(AtomTables at, const do
                                                 int nchunk, long stride) {
   __shared__ double sr[RMAX_STAGE];
   const int A = blockIdx.x, l = blockIdx.y;
+  if (at.lcut && l > at.lcut[A]) return;  // above the centre's cut: never read
   // blockIdx.z = set * nchunk + chunk: set k reads rho_lm + k stride, writes U + k stride
   const long set = blockIdx.z / nchunk;
   rho_lm += set * stride, Uy += set * stride, Udt += set * stride;
@@ -317,13 +320,17 @@ __global__ void k_synth_tables(AtomTables at, const double *__restrict__ Uy,
   const int nr = at.nrad[A];
   if (jb >= nr - 1) return;
   const long roff = at.rad_off[A];
-  for (int q = threadIdx.x; q < L; q += blockDim.x) {
+  // a centre with its own cut: its harmonics only, in its own synthesis order, in its own
+  // rows (at.toff)
+  const int la = at.lcut ? at.lcut[A] : lmax, La = (la + 1) * (la + 1);
+  for (int q = threadIdx.x; q < La; q += blockDim.x) {
     int l, m, cs;
-    q_decode(q, lmax, l, m, cs);
+    q_decode(q, la, l, m, cs);
     const int ref = l * l + (m ? 2 * m - 1 + cs : 0);
     const double f = q_factor(l, m) * pi_sigma(l, m);
     for (int j = jb; j < min(jb + SYNTH_TAB_J, nr - 1); ++j) {
-      double *t = T + ((long)A * (at.max_nrad - 1) + j) * lay.trow + q * lay.tq;
+      double *t = at.toff ? T + at.toff[A] + ((long)j * La + q) * 4
+                          : T + ((long)A * (at.max_nrad - 1) + j) * lay.trow + q * lay.tq;
       if (pad) {
         t[0] = t[1] = t[2] = t[3] = 0.0;
         if (j == 0 && write_amps) ampL[A * lay.arow + q * lay.aq] = ampR[A * lay.arow + q * lay.aq] = 0.0;
@@ -812,6 +819,36 @@ void launch_synth_mfma(const AtomTables &at, const double *geo, long N, const in
   NAPMO_HIP(hipGetLastError());
 }
 
+__global__ void k_contrib_reduce_scale_group(const ScaleMember *__restrict__ mem, int nmem,
+                                             const double *__restrict__ contrib,
+                                             const long *__restrict__ cbase,
+                                             const double *__restrict__ wbw,
+                                             double *__restrict__ v) {
+  const long g = blockIdx.x;
+  const ScaleMember &m = mem[group_member(mem, nmem, g)];
+  const long q = (g - m.block0) * blockDim.x + threadIdx.x;  // the member's own point index
+  if (q >= m.Npad) return;
+  if (q >= m.N) {
+    m.s[q] = 0.0;
+    return;
+  }
+  const long p = m.p0 + q;
+  double s = 0.0;
+  for (int A = m.A0; A < m.A1; ++A) s += contrib[cbase[A] + p];
+  const double vp = 1.0 * s;
+  v[p] = vp;
+  m.s[q] = vp * wbw[p];
+}
+
+void launch_contrib_reduce_scale_group(const ScaleMember *d_mem, int nmem, long nblocks,
+                                       const double *contrib, const long *cbase,
+                                       const double *wbw, double *v, hipStream_t st) {
+  if (nmem <= 0 || nblocks <= 0) return;
+  k_contrib_reduce_scale_group<<<dim3((unsigned)nblocks), 256, 0, st>>>(d_mem, nmem, contrib, cbase,
+                                                                        wbw, v);
+  NAPMO_HIP(hipGetLastError());
+}
+
 void launch_contrib_reduce_scale(const double *contrib, int M, long N, long Npad, const double *wbw,
                                  double *v, double *sv, hipStream_t s) {
   if (Npad <= 0) return;
@@ -917,6 +954,22 @@ __global__ void k_rho_reduce(const double *__restrict__ partial, int nt, long Np
   rho[p] = scale * s;
 }
 
+__global__ void k_rho_reduce_group(const RhoReduceMember *__restrict__ mem, int nmem) {
+  const long g = blockIdx.x;
+  const RhoReduceMember &r = mem[group_member(mem, nmem, g)];
+  const long p = (g - r.block0) * blockDim.x + threadIdx.x;
+  if (p >= r.N) return;
+  double s = 0.0;
+  for (int t = 0; t < r.ntile; ++t) s += r.partial[(long)t * r.Npad + p];
+  r.rho[p] = 1.0 * s;
+}
+
+void launch_rho_reduce_group(const RhoReduceMember *d_mem, int nmem, long nblocks, hipStream_t st) {
+  if (nmem <= 0 || nblocks <= 0) return;
+  k_rho_reduce_group<<<dim3((unsigned)nblocks), 256, 0, st>>>(d_mem, nmem);
+  NAPMO_HIP(hipGetLastError());
+}
+
 void launch_rho_reduce(const double *partial, int ntile, long Npad, long N, double *rho,
                        hipStream_t s, double scale) {
   if (N <= 0) return;
@@ -973,18 +1026,18 @@ __device__ inline void tile_coords(int t, int nt, bool symmetric, int &ti, int &
 // per half-wave.
 constexpr int FSTRIDE = TILE + 16;
 
-__global__ void __launch_bounds__(256) k_fock_f64(const double *__restrict__ X,
-                                                  const double *__restrict__ Yv,
-                                                  const double *__restrict__ s, int ld, int nt,
-                                                  int symmetric, int ntiles, int nchunk, long Npad,
-                                                  long chunk, double *__restrict__ partial) {
+__device__ __forceinline__ void fock_f64_body(const double *__restrict__ X,
+                                              const double *__restrict__ Yv,
+                                              const double *__restrict__ s, int ld, int nt,
+                                              int symmetric, int ntiles, int nchunk, long Npad,
+                                              long chunk, double *__restrict__ partial, int g) {
   __shared__ double As[2][KS][FSTRIDE];  // As[.][k][m] = X[p0 + k][mu0 + m]
   __shared__ double Bs[2][KS][FSTRIDE];  // Bs[.][k][n] = s[p0 + k] Y[p0 + k][nu0 + n]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // XCD-aware order: workgroup g runs on XCD g % 8; all tiles of one chunk of points get
   // ids congruent mod 8 (g = 8 i + x: tile i % ntiles, chunk (i / ntiles) * 8 + x), so the
   // chunk's Phi rows are shared through one L2
-  const int g = blockIdx.x, xcd = g & 7, gi = g >> 3;
+  const int xcd = g & 7, gi = g >> 3;
   const int tile = gi % ntiles, chunk_id = (gi / ntiles) * 8 + xcd;
   if (chunk_id >= nchunk) return;
   int ti, tj;
@@ -1047,6 +1100,36 @@ __global__ void __launch_bounds__(256) k_fock_f64(const double *__restrict__ X,
         out[(wr + a * 16 + rg + 4 * r) * TILE + wc + b * 16 + col16] = acc[a][b][r];
 }
 
+__global__ void __launch_bounds__(256) k_fock_f64(const double *__restrict__ X,
+                                                  const double *__restrict__ Yv,
+                                                  const double *__restrict__ s, int ld, int nt,
+                                                  int symmetric, int ntiles, int nchunk, long Npad,
+                                                  long chunk, double *__restrict__ partial) {
+  fock_f64_body(X, Yv, s, ld, nt, symmetric, ntiles, nchunk, Npad, chunk, partial, blockIdx.x);
+}
+
+// symmetric F = Phi^T diag(s) Phi of many plans in one launch
+__global__ void __launch_bounds__(256) k_fock_f64_group(const FockMember *__restrict__ mem, int nmem) {
+  const long g = blockIdx.x;
+  const FockMember &f = mem[group_member(mem, nmem, g)];
+  fock_f64_body(f.X, f.X, f.s, f.ld, f.nt, 1, f.ntiles, f.nchunk, f.Npad, f.chunk, f.partial,
+                (int)(g - f.block0));
+}
+
+long fock_group_fill(FockMember &f, const FockShape &sh, const double *X, const double *s,
+                     double *partial, long block0) {
+  NAPMO_REQUIRE(sh.symmetric && block0 % 8 == 0, "grouped Fock: symmetric shapes, aligned blocks");
+  f.X = X, f.s = s, f.partial = partial, f.Npad = sh.Npad, f.chunk = sh.chunk, f.block0 = block0;
+  f.ld = sh.ld, f.nt = sh.nt, f.ntiles = sh.ntiles, f.nchunk = sh.nchunk;
+  return (long)sh.ntiles * round_up(sh.nchunk, 8);
+}
+
+void launch_fock_group(const FockMember *d_mem, int nmem, long nblocks, hipStream_t st) {
+  if (nmem <= 0 || nblocks <= 0) return;
+  k_fock_f64_group<<<dim3((unsigned)nblocks), 256, 0, st>>>(d_mem, nmem);
+  NAPMO_HIP(hipGetLastError());
+}
+
 void launch_fock_f64(const FockShape &sh, const double *X, const double *Y, const double *s,
                      double *partial, hipStream_t st) {
   const dim3 grid((unsigned)(sh.ntiles * round_up(sh.nchunk, 8)));
@@ -1061,13 +1144,13 @@ void launch_fock_f64(const FockShape &sh, const double *X, const double *Y, cons
 // image of an off-diagonal upper tile is written by the same thread. Partials are FP64
 // (per-tile kernels) or FP32 (the one-pass mixed kernel, whose accumulators are FP32).
 template <class T>
-__global__ void __launch_bounds__(256) k_fock_reduce(const T *__restrict__ partial, int nt,
-                                                     int ntiles, int nchunk, int symmetric, int n,
-                                                     int upper_mirror, double scale, int accumulate,
-                                                     double *__restrict__ F) {
+__device__ __forceinline__ void fock_reduce_body(const T *__restrict__ partial, int nt, int ntiles,
+                                                 int nchunk, int symmetric, int n,
+                                                 int upper_mirror, double scale, int accumulate,
+                                                 double *__restrict__ F, long g) {
   __shared__ double red[4][64];
   const int el = threadIdx.x & 63, slice = threadIdx.x >> 6;
-  const long u = (long)blockIdx.x * 64 + el;
+  const long u = g * 64 + el;
   const long nel = (long)ntiles * TILE * TILE;
   double sum = 0.0;
   if (u < nel) {
@@ -1120,6 +1203,36 @@ __global__ void __launch_bounds__(256) k_fock_reduce(const T *__restrict__ parti
   }
 }
 
+template <class T>
+__global__ void __launch_bounds__(256) k_fock_reduce(const T *__restrict__ partial, int nt,
+                                                     int ntiles, int nchunk, int symmetric, int n,
+                                                     int upper_mirror, double scale, int accumulate,
+                                                     double *__restrict__ F) {
+  fock_reduce_body<T>(partial, nt, ntiles, nchunk, symmetric, n, upper_mirror, scale, accumulate,
+                      F, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_fock_reduce_group(const FockReduceMember *__restrict__ mem,
+                                                           int nmem) {
+  const long g = blockIdx.x;
+  const FockReduceMember &f = mem[group_member(mem, nmem, g)];
+  fock_reduce_body<double>(f.partial, f.nt, f.ntiles, f.nchunk, 1, f.n, 0, 1.0, 0, f.F, g - f.block0);
+}
+
+long fock_reduce_group_fill(FockReduceMember &f, const FockShape &sh, const double *partial, int n,
+                            double *F, long block0) {
+  NAPMO_REQUIRE(sh.symmetric, "grouped Fock reduction: symmetric shapes");
+  f.partial = partial, f.F = F, f.block0 = block0;
+  f.nt = sh.nt, f.ntiles = sh.ntiles, f.nchunk = sh.nchunk, f.n = n;
+  return div_up((long)sh.ntiles * TILE * TILE, 64);
+}
+
+void launch_fock_reduce_group(const FockReduceMember *d_mem, int nmem, long nblocks, hipStream_t st) {
+  if (nmem <= 0 || nblocks <= 0) return;
+  k_fock_reduce_group<<<dim3((unsigned)nblocks), 256, 0, st>>>(d_mem, nmem);
+  NAPMO_HIP(hipGetLastError());
+}
+
 void launch_fock_reduce(const FockShape &sh, const double *partial, int n, int upper_mirror,
                         double scale, int accumulate, double *F, hipStream_t st) {
   const long nel = (long)sh.ntiles * TILE * TILE;
@@ -1142,20 +1255,21 @@ void launch_fock_reduce_f32(const FockShape &sh, const float *partial, int n, do
 // P o F and w bw rho v, then one block adds the partials.
 constexpr int TRACE_BLOCKS = 512;
 
-__global__ void __launch_bounds__(256) k_traces_partial(const double *__restrict__ P,
-                                                        const double *__restrict__ F, long nn,
-                                                        const double *__restrict__ wbw,
-                                                        const double *__restrict__ rho,
-                                                        const double *__restrict__ v, long N,
-                                                        double *__restrict__ part) {
+// block g of nb blocks (256 threads)
+__device__ __forceinline__ void traces_partial_body(const double *__restrict__ P,
+                                                    const double *__restrict__ F, long nn,
+                                                    const double *__restrict__ wbw,
+                                                    const double *__restrict__ rho,
+                                                    const double *__restrict__ v, long N,
+                                                    double *__restrict__ part, int g, int nb) {
   __shared__ double sa[256], sb[256];
   const int tid = threadIdx.x;
-  const long stride = (long)gridDim.x * blockDim.x;
+  const long stride = (long)nb * 256;
   double a = 0.0, b = 0.0;
   if (P && F)
-    for (long e = (long)blockIdx.x * blockDim.x + tid; e < nn; e += stride) a += P[e] * F[e];
+    for (long e = (long)g * 256 + tid; e < nn; e += stride) a += P[e] * F[e];
   if (wbw && rho && v)
-    for (long p = (long)blockIdx.x * blockDim.x + tid; p < N; p += stride) b += wbw[p] * rho[p] * v[p];
+    for (long p = (long)g * 256 + tid; p < N; p += stride) b += wbw[p] * rho[p] * v[p];
   sa[tid] = a;
   sb[tid] = b;
   __syncthreads();
@@ -1163,11 +1277,20 @@ __global__ void __launch_bounds__(256) k_traces_partial(const double *__restrict
     if (tid < w) sa[tid] += sa[tid + w], sb[tid] += sb[tid + w];
     __syncthreads();
   }
-  if (tid == 0) part[blockIdx.x] = sa[0], part[gridDim.x + blockIdx.x] = sb[0];
+  if (tid == 0) part[g] = sa[0], part[nb + g] = sb[0];
 }
 
-__global__ void __launch_bounds__(512) k_traces_final(const double *__restrict__ part, int nb,
-                                                      double *__restrict__ E) {
+__global__ void __launch_bounds__(256) k_traces_partial(const double *__restrict__ P,
+                                                        const double *__restrict__ F, long nn,
+                                                        const double *__restrict__ wbw,
+                                                        const double *__restrict__ rho,
+                                                        const double *__restrict__ v, long N,
+                                                        double *__restrict__ part) {
+  traces_partial_body(P, F, nn, wbw, rho, v, N, part, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void traces_final_body(const double *__restrict__ part, int nb,
+                                                  double *__restrict__ E) {
   __shared__ double sa[512], sb[512];
   const int tid = threadIdx.x;
   sa[tid] = tid < nb ? part[tid] : 0.0;
@@ -1180,10 +1303,35 @@ __global__ void __launch_bounds__(512) k_traces_final(const double *__restrict__
   if (tid == 0) E[0] = 0.5 * sa[0], E[1] = 0.5 * sb[0];
 }
 
+__global__ void __launch_bounds__(512) k_traces_final(const double *__restrict__ part, int nb,
+                                                      double *__restrict__ E) {
+  traces_final_body(part, nb, E);
+}
+
+// grouped: member = blockIdx.y, TRACE_BLOCKS partial blocks each (the same sums as one plan)
+__global__ void __launch_bounds__(256) k_traces_partial_group(const TraceMember *__restrict__ mem) {
+  const TraceMember &t = mem[blockIdx.y];
+  traces_partial_body(t.P, t.F, t.nn, t.wbw, t.rho, t.v, t.N, t.work, blockIdx.x, gridDim.x);
+}
+
+__global__ void __launch_bounds__(512) k_traces_final_group(const TraceMember *__restrict__ mem,
+                                                            int nb) {
+  const TraceMember &t = mem[blockIdx.x];
+  traces_final_body(t.work, nb, t.E);
+}
+
 void launch_traces(const double *P, const double *F, int n, const double *wbw, const double *rho,
                    const double *v, long N, double *E, double *work, hipStream_t st) {
   k_traces_partial<<<TRACE_BLOCKS, 256, 0, st>>>(P, F, (long)n * n, wbw, rho, v, N, work);
   k_traces_final<<<1, 512, 0, st>>>(work, TRACE_BLOCKS, E);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_traces_group(const TraceMember *d_mem, int nmem, hipStream_t st) {
+  if (nmem <= 0) return;
+  NAPMO_REQUIRE(nmem <= 65535, "grouped traces: too many members");
+  k_traces_partial_group<<<dim3(TRACE_BLOCKS, nmem), 256, 0, st>>>(d_mem);
+  k_traces_final_group<<<dim3(nmem), 512, 0, st>>>(d_mem, TRACE_BLOCKS);
   NAPMO_HIP(hipGetLastError());
 }
 

@@ -57,6 +57,17 @@ struct AtomTables {
   const double *center;     // [M*3]
   const double *r;          // [sum nrad] radial nodes
   int max_nrad;
+  // first element of centre A's row in the synthesis' per-centre sums (contrib); null: A N
+  // (a grid of one molecule; several molecules side by side give each its own [M_m x N_m])
+  const long *cbase = nullptr;
+  // per centre, the largest l of its expansion (null: lmax): the projection writes zeros
+  // above it without summing them, the radial solves skip those l, and the synthesis
+  // tables hold the centre's own harmonics only, in its own synthesis order
+  const int *lcut = nullptr;
+  // per centre, its first synthesis-table element (null: A (max_nrad - 1) L 4), rows of
+  // (lcut + 1)^2 x 4 coefficients; amplitude rows amp_stride apart (0: L)
+  const long *toff = nullptr;
+  int amp_stride = 0;
 };
 
 // rho_lm[A][i][lm] = sum_j f_Aij Yhat[j][lm], f = dens * bw  (spherical expansion).
@@ -308,6 +319,83 @@ void launch_gram(const double *C, int n, int nocc, double eta, double *P, hipStr
 void launch_pad2d(const double *src, int rows, int cols, double *dst, int drows, int dcols,
                   hipStream_t s);
 void launch_selftest_mfma(int which, double *out, hipStream_t s);
+
+// ---- grouped launches: many plans (independent molecules) in one launch per phase. Each
+// member carries its own arguments; block0 is its first workgroup (a multiple of 8 where
+// the kernel orders tiles by XCD), and a workgroup finds its member by binary search.
+struct DensityMember {   // P-form density of one plan
+  const double *phi, *B;
+  double *partial;
+  long Npad, nvalid, block0;
+  int ld, ldb, kend;
+  double scale;
+};
+struct FockMember {      // FP64 Fock partials of one plan
+  const double *X, *s;
+  double *partial;
+  long Npad, chunk, block0;
+  int ld, nt, ntiles, nchunk;
+};
+struct FockReduceMember {
+  const double *partial;
+  double *F;
+  long block0;
+  int nt, ntiles, nchunk, n;
+};
+struct TraceMember {     // E[0] = 1/2 sum P o F, E[1] = 1/2 sum wbw rho v
+  const double *P, *F, *wbw, *rho, *v;
+  double *E, *work;
+  long nn, N;
+};
+struct PadMember {       // src [rows x cols] -> dst [drows x dcols], zero padded
+  const double *src;
+  double *dst;
+  long block0;
+  int rows, cols, drows, dcols;
+};
+struct RhoReduceMember {  // rho[p] = sum_t partial[t][p], p < N
+  const double *partial;
+  double *rho;
+  long Npad, N, block0;
+  int ntile;
+};
+struct ScaleMember {     // union points p0 .. p0 + N - 1, atoms A0 .. A1 - 1; s [Npad]
+  double *s;
+  long p0, N, Npad, block0;
+  int A0, A1;
+};
+template <class T>
+__host__ __device__ inline int group_member(const T *mem, int nmem, long g) {
+  int lo = 0, hi = nmem - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (mem[mid].block0 <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+// Fills d for one plan's P-form density (ks as for launch_density_f64); returns its
+// workgroup count.
+long density_group_fill(DensityMember &d, const double *phi, int ld, long Npad, const double *P,
+                        double *partial, int nreal, int ks, long block0);
+void launch_density_group(const DensityMember *d_mem, int nmem, long nblocks, int ks, int pf,
+                          hipStream_t s);
+long fock_group_fill(FockMember &f, const FockShape &sh, const double *X, const double *s,
+                     double *partial, long block0);
+void launch_fock_group(const FockMember *d_mem, int nmem, long nblocks, hipStream_t st);
+long fock_reduce_group_fill(FockReduceMember &f, const FockShape &sh, const double *partial, int n,
+                            double *F, long block0);
+void launch_fock_reduce_group(const FockReduceMember *d_mem, int nmem, long nblocks, hipStream_t st);
+void launch_traces_group(const TraceMember *d_mem, int nmem, hipStream_t st);
+void launch_pad_group(const PadMember *d_mem, int nmem, long nblocks, hipStream_t st);
+void launch_rho_reduce_group(const RhoReduceMember *d_mem, int nmem, long nblocks, hipStream_t st);
+// v[p] = sum_{A0 <= A < A1} contrib[cbase[A] + p] (union point p of member m, its atoms in
+// order) and s_m[p - p0] = v[p] wbw[p]; s_m zero from N to Npad.
+void launch_contrib_reduce_scale_group(const ScaleMember *d_mem, int nmem, long nblocks,
+                                       const double *contrib, const long *cbase,
+                                       const double *wbw, double *v, hipStream_t st);
 void launch_copy(const double *src, double *dst, long n, hipStream_t s);
 
 }  // namespace napmo

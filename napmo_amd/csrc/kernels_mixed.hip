@@ -532,6 +532,21 @@ __global__ void k_pad2d(const double *__restrict__ src, int rows, int cols, int 
   dst[e] = (i < rows && j < cols) ? src[(long)i * cols + j] : 0.0;
 }
 
+__global__ void k_pad_group(const PadMember *__restrict__ mem, int nmem) {
+  const long g = blockIdx.x;
+  const PadMember &m = mem[group_member(mem, nmem, g)];
+  const long e = (g - m.block0) * blockDim.x + threadIdx.x;
+  if (e >= (long)m.drows * m.dcols) return;
+  const int i = (int)(e / m.dcols), j = (int)(e % m.dcols);
+  m.dst[e] = (i < m.rows && j < m.cols) ? m.src[(long)i * m.cols + j] : 0.0;
+}
+
+void launch_pad_group(const PadMember *d_mem, int nmem, long nblocks, hipStream_t st) {
+  if (nmem <= 0 || nblocks <= 0) return;
+  k_pad_group<<<dim3((unsigned)nblocks), 256, 0, st>>>(d_mem, nmem);
+  NAPMO_HIP(hipGetLastError());
+}
+
 void launch_pad2d(const double *src, int rows, int cols, double *dst, int drows, int dcols,
                   hipStream_t s) {
   k_pad2d<<<div_up((long)drows * dcols, 256), 256, 0, s>>>(src, rows, cols, dcols, dst, drows);

@@ -340,6 +340,11 @@ void DeviceGrid::build(const std::vector<AtomInfo> &in) {
       NAPMO_REQUIRE(dx * dx + dy * dy + dz * dz > 0.0,
                     "coincident centres in BeckeGrid (" + std::to_string(j) + " and " + std::to_string(i) + ")");
     }
+  build_geometry(in);
+  becke_weights();
+}
+
+void DeviceGrid::build_geometry(const std::vector<AtomInfo> &in) {
   require_device();
   atoms = in;
   M = (int)atoms.size();
@@ -377,6 +382,18 @@ void DeviceGrid::build(const std::vector<AtomInfo> &in) {
         own[p] = a;
       }
   }
+  x.upload(hx, stream), y.upload(hy, stream), z.upload(hz, stream), w.upload(weights, stream);
+  owner.upload(own, stream);
+  d_point_off.upload(offsets, stream);
+  d_rad_off.upload(rad_off, stream);
+  d_nrad.upload(nrad, stream), d_nang.upload(nang, stream), d_rkind.upload(rkind, stream);
+  d_rp0.upload(rp0, stream), d_rp1.upload(rp1, stream);
+  d_first.upload(first, stream), d_last.upload(last, stream);
+  d_center.upload(origin, stream);
+  d_r.upload(allr, stream);
+}
+
+void DeviceGrid::becke_weights() {
   // Becke pair tables (src/grid/becke.cpp:83-115) with the evaluation sign folded in.
   std::vector<double> R((size_t)M * M, 0.0), As((size_t)M * M, 0.0);
   for (int i = 0; i < M; ++i)
@@ -397,15 +414,6 @@ void DeviceGrid::build(const std::vector<AtomInfo> &in) {
   for (int i = 0; i < M; ++i)
     for (int j = 0; j < M; ++j)
       if (i != j) NAPMO_REQUIRE(R[(size_t)i * M + j] > 0.0, "coincident centres in BeckeGrid");
-  x.upload(hx, stream), y.upload(hy, stream), z.upload(hz, stream), w.upload(weights, stream);
-  owner.upload(own, stream);
-  d_point_off.upload(offsets, stream);
-  d_rad_off.upload(rad_off, stream);
-  d_nrad.upload(nrad, stream), d_nang.upload(nang, stream), d_rkind.upload(rkind, stream);
-  d_rp0.upload(rp0, stream), d_rp1.upload(rp1, stream);
-  d_first.upload(first, stream), d_last.upload(last, stream);
-  d_center.upload(origin, stream);
-  d_r.upload(allr, stream);
   DeviceBuffer<double> dR, dA;
   dR.upload(R, stream);
   dA.upload(As, stream);
@@ -496,22 +504,29 @@ PoissonPlan &DeviceGrid::build_plan(int lmax, bool kinetic) {
   auto p = std::make_unique<PoissonPlan>();
   p->lmax = lmax;
   const int L = (lmax + 1) * (lmax + 1);
-  // projection tables, one per distinct Lebedev rule
-  std::map<int, long> yoff;
+  // projection tables, one per distinct (Lebedev rule, harmonic cut): with a cut c below
+  // lmax (lcut, grids of several molecules) the columns of l > c are zero, so those
+  // projections, radial solutions and synthesis terms are exact zeros
+  std::map<std::pair<int, int>, long> yoff;
   std::vector<double> yh;
   std::vector<long> yh_off(M);
   for (int a = 0; a < M; ++a) {
     const int na = atoms[a].nang;
-    if (!yoff.count(na)) {
+    const int cut = lcut.empty() ? lmax : std::min(lcut[a], lmax);
+    const auto key = std::make_pair(na, cut);
+    if (!yoff.count(key)) {
       AngularGrid ag;
       ag.lorder = na;
       ag.points = atoms[a].u;
       ag.weights = atoms[a].aw;
-      yoff[na] = (long)yh.size();
+      yoff[key] = (long)yh.size();
       auto t = projection_table(ag, lmax);
+      const int Lc = (cut + 1) * (cut + 1);
+      for (size_t e = 0; e < t.size(); ++e)
+        if ((int)(e % L) >= Lc) t[e] = 0.0;
       yh.insert(yh.end(), t.begin(), t.end());
     }
-    yh_off[a] = yoff[na];
+    yh_off[a] = yoff[key];
   }
   // Green matrices, one set per distinct radial grid
   std::vector<double> gr;

@@ -144,6 +144,12 @@ struct DeviceGrid {
 
   ~DeviceGrid();
   void build(const std::vector<AtomInfo> &atoms);
+  // build = build_geometry (points, weights, atom tables) + becke_weights (bw, wbw).
+  void build_geometry(const std::vector<AtomInfo> &atoms);
+  void becke_weights();
+  // Per-atom harmonic cut of the Poisson plans (empty: none): projections of l > lcut[a]
+  // are zero (the grid of several molecules, each keeping its own lmax).
+  std::vector<int> lcut;
   // A grid given only as points and Becke weights (the Lowdin2 compatibility grid of
   // src/grid/becke.cpp:57-77): pw [N x 4] rows (x, y, z, bw), weights 1, ncenter centres
   // without atomic grids. Integration, collocation and contractions work on it; the

@@ -304,7 +304,8 @@ __global__ void __launch_bounds__(256, NP == 1 ? 5 : 4) k_synth2n(AtomTables at,
   if (reg == 1) {
     constexpr int NV = L * 2;  // double2 per row
     constexpr int NLD = (NV + 63) / 64;
-    const double2 *row = (const double2 *)(T + ((long)A * (at.max_nrad - 1) + j) * (long)L * 4);
+    const double2 *row = (const double2 *)(at.toff ? T + at.toff[A] + (long)j * L * 4
+                                                   : T + ((long)A * (at.max_nrad - 1) + j) * (long)L * 4);
     double2 rv[NLD];
 #pragma unroll
     for (int t = 0; t < NLD; ++t) rv[t] = row[min(lane + 64 * t, NV - 1)];
@@ -322,15 +323,17 @@ __global__ void __launch_bounds__(256, NP == 1 ? 5 : 4) k_synth2n(AtomTables at,
     d[k] = (reg == 1) ? 0.0 : g0[k];
     inv_d[k] = (reg == 2) ? 1.0 / g0[k] : 0.0;
   }
+  const long arow = (long)A * (at.amp_stride ? at.amp_stride : L);
   if (reg == 1)
     synth_harmonics_np<LMAX, 1, NP>(srow[wave], u, ux, uy, uz, d, inv_d, acc);
   else if (reg == 0)
-    synth_harmonics_np<LMAX, 0, NP>(ampL + (long)A * L, u, ux, uy, uz, d, inv_d, acc);
+    synth_harmonics_np<LMAX, 0, NP>(ampL + arow, u, ux, uy, uz, d, inv_d, acc);
   else
-    synth_harmonics_np<LMAX, 2, NP>(ampR + (long)A * L, u, ux, uy, uz, d, inv_d, acc);
+    synth_harmonics_np<LMAX, 2, NP>(ampR + arow, u, ux, uy, uz, d, inv_d, acc);
+  const long cb = at.cbase ? at.cbase[A] : (long)A * N;
 #pragma unroll
   for (int k = 0; k < NP; ++k)
-    if (p[k] >= 0) contrib[(long)A * N + p[k]] = acc[k];
+    if (p[k] >= 0) contrib[cb + p[k]] = acc[k];
 }
 
 template <int LMAX>

@@ -92,6 +92,46 @@ def test_batch_graph_replays_match_eager():
         del batch
 
 
+def test_one_launch_per_phase_matches_single_builds():
+    # nlanes = 0: every phase of the batch as one launch (FP64 plans); eager builds and graph
+    # replays on changing densities equal the plans' own builds bit for bit; the molecules'
+    # lmax values differ, so the shared Poisson tables cut each molecule's harmonics
+    import napmo_amd
+    from napmo_amd import workloads
+    from napmo_amd.system import cext
+    mols = workloads.batch_molecules(count=8, seed=12)
+    print('lmax per molecule', [m['lmax'] for m in mols])
+    assert len({m['lmax'] for m in mols}) > 1
+    grids, bases, plans, Ps = _setup(mols)
+    dP, dF, dE = _device(Ps)
+    batch = napmo_amd.BatchFock(plans, nlanes=0)
+    ptr = lambda xs: [a.ptr for a in xs]  # noqa: E731
+    batch.build(ptr(dP), ptr(dF), ptr(dE))
+    batch.synchronize()
+    for k, (f, P) in enumerate(zip(plans, Ps)):
+        F, E, Eg = f.build(P)
+        Eb = dE[k].to_host()
+        assert np.array_equal(dF[k].to_host(), F) and Eb[0] == E and Eb[1] == Eg, k
+    batch.capture(ptr(dP), ptr(dF), ptr(dE))
+    for it in range(3):
+        scale = 1.0 + 0.25 * it
+        for a, P in zip(dP, Ps):
+            Pi = np.ascontiguousarray(P * scale)
+            cext.check(cext.napmo_memcpy(a.ptr, Pi.ctypes.data, Pi.nbytes, 1))
+        batch.replay()
+        batch.synchronize()
+        for k, (f, P) in enumerate(zip(plans, Ps)):
+            F, E, Eg = f.build(P * scale)
+            Eb = dE[k].to_host()
+            assert np.array_equal(dF[k].to_host(), F) and Eb[0] == E and Eb[1] == Eg, (it, k)
+        print('replay', it, 'E', [float(e.to_host()[1]) for e in dE[:3]])
+    del batch
+    # the grouped contraction is FP64 only
+    g2, b2, p2, _ = _setup(mols[:2], 'int8')
+    with pytest.raises(Exception, match='FP64 plans'):
+        napmo_amd.BatchFock(p2, nlanes=0)
+
+
 def test_autotuner_choice_and_cache(tmp_path):
     import napmo_amd
     from napmo_amd import autotune, workloads

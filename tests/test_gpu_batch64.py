@@ -119,3 +119,32 @@ def test_batch64_int8_per_molecule_energy(batch64, fp64_builds):
     assert dEg.max() <= 1e-12 * max(abs(r[2]) for r in fp64_builds)  # rho, v FP64 in every mode
     assert dEt.max() <= GATE_EH
     assert dF.max() < 1e-7
+
+
+def test_batch64_one_launch_per_phase(batch64, fp64_builds):
+    # nlanes = 0: the 64 FP64 builds with one launch per phase (the molecules' atoms and
+    # points side by side, each molecule keeping its own lmax 8..12 and its own pairs):
+    # equal to the single builds bit for bit, eagerly and from three graph replays
+    import napmo_amd
+    from napmo_amd.system import cext
+    mols, grids, bases, Ps = batch64
+    plans = [napmo_amd.GridFock(g, b, m['lmax'], 'fp64') for g, b, m in zip(grids, bases, mols)]
+    dP, dF, dE = _device(Ps)
+    batch = napmo_amd.BatchFock(plans, nlanes=0)
+    batch.build(_ptr(dP), _ptr(dF), _ptr(dE))
+    batch.synchronize()
+    for k, (F, E, Eg) in enumerate(fp64_builds):
+        Fb, Eb = dF[k].to_host(), dE[k].to_host()
+        assert np.array_equal(Fb, F) and Eb[0] == E and Eb[1] == Eg, k
+    batch.capture(_ptr(dP), _ptr(dF), _ptr(dE))
+    for it in range(3):
+        for a, P in zip(dF, Ps):  # outputs cleared, so every replay rewrites them
+            Z = np.zeros_like(P)
+            cext.check(cext.napmo_memcpy(a.ptr, Z.ctypes.data, Z.nbytes, 1))
+        batch.replay()
+        batch.synchronize()
+        for k, (F, E, Eg) in enumerate(fp64_builds):
+            Eb = dE[k].to_host()
+            assert np.array_equal(dF[k].to_host(), F) and Eb[0] == E and Eb[1] == Eg, (it, k)
+    print('fp64, one launch per phase: 64 builds = single builds, eager and 3 graph replays')
+    del batch, plans
