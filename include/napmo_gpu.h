@@ -496,6 +496,12 @@ int napmo_apmo_build(void *plan, const double **P, double **F_out, double *E_out
 /* Device pointers; asynchronous on the species' streams (napmo_apmo_sync waits). */
 int napmo_apmo_build_device(void *plan, const double **dP, double **dF, double *dE);
 int napmo_apmo_sync(void *plan);
+/* on = 1: later builds run every phase for all species as one launch (FP64 plans, no
+ * reference counterpart): the species' atoms and points side by side, the synthesis of
+ * own and cross-species potentials over one pair list per lmax, grouped density, Fock,
+ * reduction and trace kernels. Equal to the per-species build to rounding. on = 0: back
+ * to one stream per species. */
+int napmo_apmo_plan_set_one_launch(void *plan, int on);
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,7 @@ from napmo_amd.system import cext
 
 class APMOFock(object):
 
-    def __init__(self, species, precision='fp64'):
+    def __init__(self, species, precision='fp64', one_launch=False):
         cext.require_gpu()
         self.species = list(species)   # keeps the grid / basis handles alive
         ns = len(self.species)
@@ -35,6 +35,12 @@ class APMOFock(object):
                                             ctypes.byref(plan)))
         self._this = plan.value
         self.nbasis = [b.get('nbasis') for _, b, _, _ in self.species]
+        if one_launch:
+            self.set_one_launch(True)
+
+    def set_one_launch(self, on):
+        """Every phase of a build as one launch for all species (FP64 plans)."""
+        cext.check(cext.napmo_apmo_plan_set_one_launch(self._this, int(bool(on))))
 
     def __del__(self):
         if getattr(self, '_this', None):

@@ -1090,6 +1090,9 @@ int napmo_apmo_build_device(void *plan, const double **dP, double **dF, double *
 int napmo_apmo_sync(void *plan) {
   return guarded([&] { ((ApmoPlan *)plan)->synchronize(); });
 }
+int napmo_apmo_plan_set_one_launch(void *plan, int on) {
+  return guarded([&] { ((ApmoPlan *)plan)->set_one_launch(on != 0); });
+}
 void *napmo_device_malloc(size_t bytes) {
   void *p = nullptr;
   if (guarded([&] {

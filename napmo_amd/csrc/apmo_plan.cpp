@@ -53,8 +53,34 @@ ApmoPlan::~ApmoPlan() {
   }
 }
 
+void ApmoPlan::set_one_launch(bool on) {
+  synchronize();
+  if (!on) {
+    group.reset();
+    return;
+  }
+  std::vector<FockPlan *> plans;
+  std::vector<double> charges;
+  for (auto &S : sp) plans.push_back(S.plan.get()), charges.push_back(S.charge);
+  group = std::make_unique<GroupPlan>(plans, &charges);
+}
+
 void ApmoPlan::build_device(const double *const *dP, double *const *dF, double *dE) {
   const int ns = nspecies();
+  if (group) {
+    // the group's stream after the species' streams (inputs uploaded there), and the
+    // species' streams after the group (outputs read there)
+    for (int s = 0; s < ns; ++s) {
+      NAPMO_HIP(hipEventRecord(sp[s].done, sp[s].plan->stream));
+      NAPMO_HIP(hipStreamWaitEvent(group->stream, sp[s].done, 0));
+    }
+    std::vector<double *> dEs(ns);
+    for (int s = 0; s < ns; ++s) dEs[s] = dE + 4 * s;
+    group->build(dP, dF, dEs.data());
+    NAPMO_HIP(hipEventRecord(sp[0].expanded, group->stream));
+    for (int s = 0; s < ns; ++s) NAPMO_HIP(hipStreamWaitEvent(sp[s].plan->stream, sp[0].expanded, 0));
+    return;
+  }
   // densities, expansions and own potentials; each species on its own grid's stream
   for (int s = 0; s < ns; ++s) {
     FockPlan &p = *sp[s].plan;
@@ -133,6 +159,7 @@ void ApmoPlan::build_host(const double *const *P, double *const *F_out, double *
 
 void ApmoPlan::synchronize() {
   for (auto &S : sp) NAPMO_HIP(hipStreamSynchronize(S.plan->stream));
+  if (group) group->synchronize();
 }
 
 }  // namespace napmo

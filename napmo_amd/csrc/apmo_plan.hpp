@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "fock_plan.hpp"
+#include "group_plan.hpp"
 
 namespace napmo {
 
@@ -28,6 +29,10 @@ struct ApmoPlan {
   };
   std::vector<Species> sp;
   DeviceBuffer<double> E;  // [4 * nspecies] for host-pointer builds
+  // one launch per phase for all species (FP64): the species' atoms and points side by
+  // side, own and cross-species pairs in one synthesis per lmax (GroupPlan, coupled)
+  std::unique_ptr<GroupPlan> group;
+  void set_one_launch(bool on);
 
   ApmoPlan(const std::vector<DeviceGrid *> &grids, const std::vector<const BasisSet *> &bases,
            const std::vector<double> &charges, const std::vector<int> &lmax, int prec);

@@ -3,12 +3,19 @@
 
 #include <algorithm>
 #include <map>
+#include <memory>
 
 namespace napmo {
 
-GroupPlan::GroupPlan(const std::vector<FockPlan *> &ps) : plans(ps) {
+GroupPlan::GroupPlan(const std::vector<FockPlan *> &ps, const std::vector<double> *charges)
+    : plans(ps) {
   NAPMO_REQUIRE(!plans.empty(), "group plan: no plans");
   const int nm = (int)plans.size();
+  coupled = charges != nullptr;
+  if (coupled) {
+    NAPMO_REQUIRE((int)charges->size() == nm, "group plan: one charge per species");
+    charge = *charges;
+  }
   std::vector<AtomInfo> all;
   std::vector<int> cut;
   p0.resize(nm + 1);
@@ -20,7 +27,7 @@ GroupPlan::GroupPlan(const std::vector<FockPlan *> &ps) : plans(ps) {
     NAPMO_REQUIRE(q->prec == NAPMO_PREC_FP64 && q->use_pass == 0,
                   "group plan: FP64 plans with the per-tile contraction only");
     NAPMO_REQUIRE(!q->grid.atoms.empty(), "group plan: every plan needs a grid of atoms");
-    for (int k = 0; k < m; ++k)
+    for (int k = 0; k < m && !coupled; ++k)
       NAPMO_REQUIRE(&plans[k]->grid != &q->grid, "group plan: every plan needs its own grid");
     NAPMO_HIP(hipStreamSynchronize(q->stream));
     p0[m] = np, A0[m] = (int)all.size();
@@ -45,7 +52,7 @@ GroupPlan::GroupPlan(const std::vector<FockPlan *> &ps) : plans(ps) {
     std::copy(g.becke.begin(), g.becke.end(), ug.becke.begin() + p0[m]);
   }
   // pair buckets: each molecule's own lists, its atoms and points shifted into the union,
-  // the molecules in order of lmax so each lmax's slots are one range
+  // the sources in order of lmax so each lmax's slots are one range
   std::vector<int> list, wb;
   std::vector<long> cb(ug.M), to(ug.M);
   long nwaves = 0, csize = 0, tsize = 0;
@@ -65,10 +72,38 @@ GroupPlan::GroupPlan(const std::vector<FockPlan *> &ps) : plans(ps) {
     }
     csize += (long)g.M * g.N;
   }
+  // (target molecule or species s, source t) lists: independent molecules have only s = t;
+  // coupled species every pair, t's own lists when s and t share a grid
+  struct Src {
+    int s, t;
+    SynthPairs *sp;
+  };
+  std::vector<Src> srcs;
+  std::vector<std::unique_ptr<SynthPairs>> cross;
+  for (int t : order)
+    for (int s = 0; s < nm; ++s) {
+      if (!coupled && s != t) continue;
+      DeviceGrid &gt = plans[t]->grid, &gs = plans[s]->grid;
+      if (s == t || &gs == &gt) {
+        srcs.push_back({s, t, &gt.synth_pairs()});
+      } else {
+        cross.push_back(build_synth_pairs(gt, gs.x.get(), gs.y.get(), gs.z.get(), gs.N, gs.stream));
+        srcs.push_back({s, t, cross.back().get()});
+      }
+    }
+  if (coupled) {
+    nwaves = 0;
+    for (auto &r : srcs) nwaves += r.sp->nwaves;
+    pairs.geo.resize((size_t)std::max<long>(nwaves, 1) * 256);
+    // dense per-centre rows over every point
+    for (int A = 0; A < ug.M; ++A) cb[A] = (long)A * ug.N;
+    csize = (long)ug.M * ug.N;
+  }
   long w0 = 0;
-  for (int m : order) {
-    DeviceGrid &g = plans[m]->grid;
-    SynthPairs &sp = g.synth_pairs();
+  for (const auto &r : srcs) {
+    const int m = r.s;
+    DeviceGrid &g = plans[r.t]->grid;
+    SynthPairs &sp = *r.sp;
     std::vector<int> l((size_t)sp.nwaves * 64), b(sp.nwaves);
     sp.list.download(l.data(), l.size(), g.stream);
     sp.wave_bucket.download(b.data(), b.size(), g.stream);
@@ -80,17 +115,18 @@ GroupPlan::GroupPlan(const std::vector<FockPlan *> &ps) : plans(ps) {
       }
     for (int &e : b) {
       const int A = e / (g.max_nrad + 1), loc = e - A * (g.max_nrad + 1);
-      e = (A + A0[m]) * (ug.max_nrad + 1) + loc;
+      e = (A + A0[r.t]) * (ug.max_nrad + 1) + loc;
     }
     list.insert(list.end(), l.begin(), l.end());
     wb.insert(wb.end(), b.begin(), b.end());
     NAPMO_HIP(hipMemcpyAsync(pairs.geo.get() + w0 * 256, sp.geo.get(), sizeof(double) * sp.nwaves * 256,
                              hipMemcpyDeviceToDevice, stream));
-    if (ranges.empty() || ranges.back().lmax != plans[m]->lmax)
-      ranges.push_back({plans[m]->lmax, w0, 0});
+    const int lt = plans[r.t]->lmax;
+    if (ranges.empty() || ranges.back().lmax != lt) ranges.push_back({lt, w0, 0});
     ranges.back().count += sp.nwaves;
     w0 += sp.nwaves;
   }
+  NAPMO_HIP(hipStreamSynchronize(stream));  // the geometry copies read the cross lists
   pairs.nwaves = nwaves;
   pairs.list.upload(list, stream);
   pairs.wave_bucket.upload(wb, stream);
@@ -128,6 +164,13 @@ GroupPlan::GroupPlan(const std::vector<FockPlan *> &ps) : plans(ps) {
   rmem.upload(rm, stream);
   smem.upload(sm, stream);
   fmem.upload(fm, stream);
+  if (coupled) {
+    std::vector<SpeciesSum> ss(nm);
+    for (int m = 0; m < nm; ++m) ss[m] = SpeciesSum{A0[m], A0[m + 1], charge[m]};
+    dspecies.upload(ss, stream);
+    veff.resize(ug.N);
+    E2.resize(2 * nm);
+  }
   NAPMO_HIP(hipStreamSynchronize(stream));
 }
 
@@ -145,7 +188,8 @@ void GroupPlan::io_tables(const double *const *dP, double *const *dF, double *co
   if (same) return;
   std::vector<PadMember> pm(nm);
   std::vector<FockReduceMember> frm(nm);
-  std::vector<TraceMember> tm(nm);
+  std::vector<TraceMember> tm(nm), t2m(nm);
+  std::vector<ApmoEnergyMember> em(nm);
   pad_blocks = fred_blocks = 0;
   for (int m = 0; m < nm; ++m) {
     FockPlan &q = *plans[m];
@@ -153,14 +197,26 @@ void GroupPlan::io_tables(const double *const *dP, double *const *dF, double *co
     pm[m] = PadMember{dP[m], q.P.get(), pad_blocks, q.n, q.n, q.ld, q.ld};
     pad_blocks += div_up((long)q.ld * q.ld, 256);
     fred_blocks += fock_reduce_group_fill(frm[m], q.fsh, q.F_part.get(), q.n, dF[m], fred_blocks);
-    tm[m] = TraceMember{dP[m], dF[m], ug.wbw.get() + p0[m], rho.get() + p0[m], v.get() + p0[m],
+    const double *vs = coupled ? veff.get() : v.get();
+    tm[m] = TraceMember{dP[m], dF[m], ug.wbw.get() + p0[m], rho.get() + p0[m], vs + p0[m],
                         dE[m], q.trace_work.get(), (long)q.n * q.n, q.grid.N};
+    if (coupled) {
+      // the species' own Coulomb energy 1/2 sum w bw rho V[rho_s] (its work buffer is free
+      // again once the first traces are final: same stream)
+      t2m[m] = TraceMember{nullptr, nullptr, ug.wbw.get() + p0[m], rho.get() + p0[m], v.get() + p0[m],
+                           E2.get() + 2 * m, q.trace_work.get(), 0, q.grid.N};
+      em[m] = ApmoEnergyMember{dE[m], E2.get() + 2 * m, charge[m] * charge[m]};
+    }
   }
   // the previous build may still read the old tables: replace them in stream order, and
   // keep the host copies alive until the copies are done
   pmem.upload(pm, stream);
   frmem.upload(frm, stream);
   tmem.upload(tm, stream);
+  if (coupled) {
+    t2mem.upload(t2m, stream);
+    emem.upload(em, stream);
+  }
   NAPMO_HIP(hipStreamSynchronize(stream));
   cur_P.assign(dP, dP + nm);
   cur_F.assign(dF, dF + nm);
@@ -186,11 +242,19 @@ void GroupPlan::launches() {
     launch_synth2(at, pairs.geo.get() + r.begin * 256, ug.N, pairs.list.get() + r.begin * 64,
                   pairs.wave_bucket.get() + r.begin, r.count, w.T.get(), w.ampL.get(), w.ampR.get(),
                   r.lmax, pairs.contrib.get(), stream, 2);
-  launch_contrib_reduce_scale_group(smem.get(), nm, scale_blocks, pairs.contrib.get(), cbase.get(),
-                                    ug.wbw.get(), v.get(), stream);
+  if (coupled)
+    launch_apmo_reduce_group(smem.get(), nm, scale_blocks, dspecies.get(), pairs.contrib.get(), ug.N,
+                             ug.wbw.get(), v.get(), veff.get(), stream);
+  else
+    launch_contrib_reduce_scale_group(smem.get(), nm, scale_blocks, pairs.contrib.get(), cbase.get(),
+                                      ug.wbw.get(), v.get(), stream);
   launch_fock_group(fmem.get(), nm, fock_blocks, stream);
   launch_fock_reduce_group(frmem.get(), nm, fred_blocks, stream);
   launch_traces_group(tmem.get(), nm, stream);
+  if (coupled) {
+    launch_traces_group(t2mem.get(), nm, stream);
+    launch_apmo_energy_group(emem.get(), nm, stream);
+  }
 }
 
 void GroupPlan::build(const double *const *dP, double *const *dF, double *const *dE) {

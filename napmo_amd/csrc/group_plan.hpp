@@ -48,10 +48,24 @@ struct GroupPlan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 
-  explicit GroupPlan(const std::vector<FockPlan *> &plans);
+  // Coupled species (APMO, charges given): the members are the species of one system; the
+  // pair lists also hold every other species' centres at a species' points, the per-centre
+  // sums are dense rows over all points, and the centre-sum pass forms
+  // veff_s = q_s^2 V[rho_s] + q_s sum_{t != s} q_t V[rho_t]; the traces add each species'
+  // own Coulomb energy (E[2]) and the coupling (E[3]). Species may share a grid.
+  bool coupled = false;
+  std::vector<double> charge;
+  DeviceBuffer<double> veff, E2;  // [ug.N], [2 x species]
+  DeviceBuffer<SpeciesSum> dspecies;
+  DeviceBuffer<TraceMember> t2mem;
+  DeviceBuffer<ApmoEnergyMember> emem;
+
+  explicit GroupPlan(const std::vector<FockPlan *> &plans,
+                     const std::vector<double> *charges = nullptr);
   ~GroupPlan();
-  // One build of every plan: dP[m], dF[m] [n_m x n_m], dE[m] [2] (device pointers);
-  // asynchronous on the group's stream. Results equal the plans' own builds bit for bit.
+  // One build of every plan: dP[m], dF[m] [n_m x n_m], dE[m] [2] (coupled: [4]) (device
+  // pointers); asynchronous on the group's stream. Independent molecules: results equal the
+  // plans' own builds bit for bit.
   void build(const double *const *dP, double *const *dF, double *const *dE);
   void capture(const double *const *dP, double *const *dF, double *const *dE);
   void replay();

@@ -849,6 +849,63 @@ void launch_contrib_reduce_scale_group(const ScaleMember *d_mem, int nmem, long 
   NAPMO_HIP(hipGetLastError());
 }
 
+__global__ void k_apmo_reduce_group(const ScaleMember *__restrict__ mem, int nmem,
+                                    const SpeciesSum *__restrict__ sp,
+                                    const double *__restrict__ contrib, long Nu,
+                                    const double *__restrict__ wbw, double *__restrict__ vown,
+                                    double *__restrict__ veff) {
+  const long g = blockIdx.x;
+  const int ms = group_member(mem, nmem, g);
+  const ScaleMember &m = mem[ms];
+  const long q = (g - m.block0) * blockDim.x + threadIdx.x;
+  if (q >= m.Npad) return;
+  if (q >= m.N) {
+    m.s[q] = 0.0;
+    return;
+  }
+  const long p = m.p0 + q;
+  double own = 0.0, vc = 0.0;
+  bool first = true;
+  for (int t = 0; t < nmem; ++t) {
+    double s = 0.0;
+    for (int A = sp[t].A0; A < sp[t].A1; ++A) s += contrib[(long)A * Nu + p];
+    if (t == ms) {
+      own = 1.0 * s;
+    } else {
+      vc = first ? sp[t].q * s : vc + sp[t].q * s;
+      first = false;
+    }
+  }
+  const double qs = sp[ms].q;
+  const double ve = first ? qs * qs * own : qs * qs * own + qs * vc;
+  vown[p] = own;
+  veff[p] = ve;
+  m.s[q] = ve * wbw[p];
+}
+
+void launch_apmo_reduce_group(const ScaleMember *d_mem, int nmem, long nblocks,
+                              const SpeciesSum *sp, const double *contrib, long Nu,
+                              const double *wbw, double *vown, double *veff, hipStream_t st) {
+  if (nmem <= 0 || nblocks <= 0) return;
+  k_apmo_reduce_group<<<dim3((unsigned)nblocks), 256, 0, st>>>(d_mem, nmem, sp, contrib, Nu, wbw,
+                                                               vown, veff);
+  NAPMO_HIP(hipGetLastError());
+}
+
+__global__ void k_apmo_energy_group(const ApmoEnergyMember *__restrict__ mem, int ns) {
+  const int s = threadIdx.x;
+  if (s >= ns) return;
+  const ApmoEnergyMember &e = mem[s];
+  e.E[2] = e.q2 * e.E2[1];
+  e.E[3] = e.E[1] - e.E[2];
+}
+
+void launch_apmo_energy_group(const ApmoEnergyMember *d_mem, int ns, hipStream_t st) {
+  NAPMO_REQUIRE(ns >= 1 && ns <= 256, "APMO energies: 1 to 256 species");
+  k_apmo_energy_group<<<1, 256, 0, st>>>(d_mem, ns);
+  NAPMO_HIP(hipGetLastError());
+}
+
 void launch_contrib_reduce_scale(const double *contrib, int M, long N, long Npad, const double *wbw,
                                  double *v, double *sv, hipStream_t s) {
   if (Npad <= 0) return;

@@ -391,6 +391,24 @@ void launch_fock_reduce_group(const FockReduceMember *d_mem, int nmem, long nblo
 void launch_traces_group(const TraceMember *d_mem, int nmem, hipStream_t st);
 void launch_pad_group(const PadMember *d_mem, int nmem, long nblocks, hipStream_t st);
 void launch_rho_reduce_group(const RhoReduceMember *d_mem, int nmem, long nblocks, hipStream_t st);
+// Coupled species (APMO), member m = species m with charge q_m: per union point p of m,
+// S_t = sum over t's centres of contrib[A Nu + p] (every species t, centres in order),
+// vown = S_m, vcross = sum_{t != m} q_t S_t (in species order), veff = q_m^2 vown +
+// q_m vcross; s_m[p - p0] = veff wbw[p] (zero from N to Npad).
+struct SpeciesSum {
+  int A0, A1;
+  double q;
+};
+void launch_apmo_reduce_group(const ScaleMember *d_mem, int nmem, long nblocks,
+                              const SpeciesSum *sp, const double *contrib, long Nu,
+                              const double *wbw, double *vown, double *veff, hipStream_t st);
+// E[2] = q2 E2[1] (own Coulomb energy), E[3] = E[1] - E[2] (coupling), per species.
+struct ApmoEnergyMember {
+  double *E;
+  const double *E2;
+  double q2;
+};
+void launch_apmo_energy_group(const ApmoEnergyMember *d_mem, int ns, hipStream_t st);
 // v[p] = sum_{A0 <= A < A1} contrib[cbase[A] + p] (union point p of member m, its atoms in
 // order) and s_m[p - p0] = v[p] wbw[p]; s_m zero from N to Npad.
 void launch_contrib_reduce_scale_group(const ScaleMember *d_mem, int nmem, long nblocks,

@@ -142,3 +142,31 @@ def test_apmo_full_size_properties():
                 assert abs(dEt) < 1e-3
                 assert dF < 2e-5 * np.abs(Fs[s]).max()
         del mix
+
+
+@pytest.mark.parametrize('shared', [False, True])
+def test_apmo_one_launch_matches_per_species_build(shared):
+    # every phase for both species as one launch (own and cross-species pairs in one pair
+    # list per lmax, grouped contractions): F and all four energies per species equal the
+    # per-species build bit for bit (the same operations in the same order), on separate
+    # and on shared grids
+    import napmo_amd
+    e_atoms, e_shells, hs, n_shells = _small_system(43)
+    gpu, ora, Ps = _species_pair(e_atoms, e_shells, hs, n_shells, (30, 110), (20, 50), shared)
+    ref = napmo_amd.APMOFock(gpu, 'fp64')
+    Fr, Er = ref.build(Ps)
+    one = napmo_amd.APMOFock(gpu, 'fp64', one_launch=True)
+    Fo, Eo = one.build(Ps)
+    for s in range(2):
+        dF = np.abs(Fo[s] - Fr[s]).max() / np.abs(Fr[s]).max()
+        dE = np.abs(Eo[s] - Er[s]).max() / np.abs(Er[s]).max()
+        print('shared %s species %d: rel max|dF| %.2e, rel max|dE| %.2e, E %s' % (shared, s, dF, dE, Eo[s]))
+        assert np.array_equal(Fo[s], Fr[s]) and np.array_equal(Eo[s], Er[s])
+    Fo2, Eo2 = one.build(Ps)
+    assert all(np.array_equal(a, b) for a, b in zip(Fo, Fo2)) and np.array_equal(Eo, Eo2)
+    # and back to the per-species path
+    one.set_one_launch(False)
+    Fb, Eb = one.build(Ps)
+    assert all(np.array_equal(a, b) for a, b in zip(Fb, Fr)) and np.array_equal(Eb, Er)
+    with pytest.raises(Exception, match='FP64 plans'):
+        napmo_amd.APMOFock(gpu, 'int8', one_launch=True)
