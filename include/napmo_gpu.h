@@ -445,11 +445,16 @@ enum {
                                  0 = about N / 256). Setting it re-slices the per-geometry digits */
   NAPMO_TUNE_INT8_WAVE_BLOCKS = 8, /* int8 mode: 16 x 16 blocks of F per wave in the row-group
                                      packing, 1..5; 0 = 5 (fewer: more, smaller row groups) */
-  NAPMO_TUNE_SYNTH_MFMA_MIN = 9 /* many-density synthesis (exchange potential, orbital
+  NAPMO_TUNE_SYNTH_MFMA_MIN = 9, /* many-density synthesis (exchange potential, orbital
                                    corrections): whole tiles of 16 densities, and a last tile
                                    of at least this many, run on FP64 MFMA, the others one at a
                                    time; 1..16, 17 = one at a time only; 0 = 10 (a property of
                                    the grid) */
+  NAPMO_TUNE_BACKWARD_POINTS = 10 /* energy derivatives (dE/dP, dE/dR): 0 = from the stored
+                                     Phi with Phi P for all points at once; k > 0 = Phi
+                                     recollocated in chunks of about k points (rounded up to
+                                     a multiple of the contraction chunks), neither the stored
+                                     Phi nor a full Phi P read: the same results bit for bit */
 };
 int napmo_fock_plan_set_tuning(void *plan, int key, long value);
 int napmo_fock_plan_get_tuning(void *plan, int key, long *value);

@@ -1,6 +1,8 @@
 // Device-resident grid Fock build (see fock_plan.hpp).
 #include "fock_plan.hpp"
 
+#include <numeric>
+
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -109,6 +111,7 @@ void FockPlan::set_tuning(int key, long value) {
       NAPMO_REQUIRE(value >= 0 && value <= 17, "MFMA synthesis threshold must be 0 to 17");
       grid.synth_mfma_min = value ? (int)value : SYNTH_MFMA_MIN;
       break;
+    case NAPMO_TUNE_BACKWARD_POINTS: bwd_points = value; break;
     case NAPMO_TUNE_DENSITY_STREAM:
       NAPMO_REQUIRE(value >= 0 && value <= 3, "density streaming choice must be 0 to 3");
       density_stream = (int)value;
@@ -143,6 +146,7 @@ long FockPlan::get_tuning(int key) const {
     case NAPMO_TUNE_DENSITY_PF: return density_pf;
     case NAPMO_TUNE_SYNTH_NP: return grid.synth_np;
     case NAPMO_TUNE_SYNTH_MFMA_MIN: return grid.synth_mfma_min;
+    case NAPMO_TUNE_BACKWARD_POINTS: return bwd_points ? backward_chunk() : 0;
     case NAPMO_TUNE_DENSITY_STREAM: return density_stream;
     case NAPMO_TUNE_INT8_CHUNK: return prec == NAPMO_PREC_INT8 ? oz.chunk : 0;
     case NAPMO_TUNE_INT8_WAVE_BLOCKS: return prec == NAPMO_PREC_INT8 ? oz_wave_blocks : 0;
@@ -355,24 +359,69 @@ void FockPlan::symmetric_weight() {
   launch_half_sum(s.get(), adj_z.get(), grid.N, Npad, ssym.get(), st);
 }
 
+// Points per chunk of the recollocating backward: a multiple of the Fock chunk, of the
+// gradient's 4096-point chunk and of the density tile, so every partial sum covers the same
+// points as in the stored-Phi path.
+long FockPlan::backward_chunk() const {
+  const long unit = std::lcm(std::lcm(fsh.chunk, grad_chunk_points()), 256L);
+  return std::max<long>(1, div_up(std::max<long>(bwd_points, 1), unit)) * unit;
+}
+
 void FockPlan::energy_dP(double *dEdP) {
   symmetric_weight();
-  launch_fock_f64(fsh, phi64.get(), phi64.get(), ssym.get(), F_part.get(), stream);
-  launch_fock_reduce(fsh, F_part.get(), n, 0, 1.0, 0, dEdP, stream);
+  hipStream_t st = stream;
+  if (bwd_points <= 0) {
+    launch_fock_f64(fsh, phi64.get(), phi64.get(), ssym.get(), F_part.get(), st);
+  } else {
+    // Phi recollocated chunk by chunk; each chunk's Fock partials land where the stored-Phi
+    // launch writes them
+    const long C = backward_chunk();
+    bwd_phi.resize((size_t)std::min(C, Npad) * ld);
+    for (long p0 = 0; p0 < Npad; p0 += C) {
+      const long cn = std::max<long>(0, std::min(C, grid.N - p0)), cpad = std::min(C, Npad - p0);
+      basis.collocate(grid.x.get() + p0, grid.y.get() + p0, grid.z.get() + p0, cn, cpad, ld,
+                      bwd_phi.get(), nullptr, st);
+      FockShape cs = fsh;
+      cs.Npad = cpad;
+      cs.nchunk = (int)div_up(cpad, fsh.chunk);
+      launch_fock_f64(cs, bwd_phi.get(), bwd_phi.get(), ssym.get() + p0,
+                      F_part.get() + (p0 / fsh.chunk) * fsh.cstride(), st);
+    }
+  }
+  launch_fock_reduce(fsh, F_part.get(), n, 0, 1.0, 0, dEdP, st);
 }
 
 void FockPlan::gradient_device(const double *dP, double *dG) {
   symmetric_weight();
   hipStream_t st = stream;
   launch_pad2d(dP, n, n, P.get(), ld, ld, st);
-  X.resize((size_t)Npad * ld);
-  launch_phi_times(phi64.get(), ld, Npad, P.get(), ld, X.get(), st);
   const int ncentre = (int)centres.size() / 3;
   grad_part.resize((size_t)n * grad_chunks(grid.N) * 3);
-  launch_grad_centres(grid.x.get(), grid.y.get(), grid.z.get(), grid.N, ssym.get(), X.get(), ld, n,
-                      basis.origin.get(), basis.lxyz.get(), basis.first.get(), basis.nprim.get(),
-                      basis.norm.get(), basis.zeta.get(), basis.coef.get(), centre_of.get(),
-                      ncentre, grad_part.get(), dG, st);
+  if (bwd_points <= 0) {
+    X.resize((size_t)Npad * ld);
+    launch_phi_times(phi64.get(), ld, Npad, P.get(), ld, X.get(), st);
+    launch_grad_centres(grid.x.get(), grid.y.get(), grid.z.get(), grid.N, ssym.get(), X.get(), ld,
+                        n, basis.origin.get(), basis.lxyz.get(), basis.first.get(),
+                        basis.nprim.get(), basis.norm.get(), basis.zeta.get(), basis.coef.get(),
+                        centre_of.get(), ncentre, grad_part.get(), dG, st);
+    return;
+  }
+  // Phi recollocated and Phi P formed chunk by chunk; the partial sums of every 4096-point
+  // chunk land where the one-pass kernel writes them, then one reduction
+  const long C = backward_chunk();
+  bwd_phi.resize((size_t)std::min(C, Npad) * ld);
+  bwd_X.resize((size_t)std::min(C, Npad) * ld);
+  for (long p0 = 0; p0 < grid.N; p0 += C) {
+    const long cn = std::min(C, grid.N - p0), cpad = std::min(C, Npad - p0);
+    basis.collocate(grid.x.get() + p0, grid.y.get() + p0, grid.z.get() + p0, cn, cpad, ld,
+                    bwd_phi.get(), nullptr, st);
+    launch_phi_times(bwd_phi.get(), ld, cpad, P.get(), ld, bwd_X.get(), st);
+    launch_grad_basis_part(grid.x.get(), grid.y.get(), grid.z.get(), cn, p0, grid.N, ssym.get(),
+                           bwd_X.get(), ld, n, basis.origin.get(), basis.lxyz.get(),
+                           basis.first.get(), basis.nprim.get(), basis.norm.get(),
+                           basis.zeta.get(), basis.coef.get(), grad_part.get(), st);
+  }
+  launch_grad_reduce(grad_part.get(), n, grid.N, centre_of.get(), ncentre, dG, st);
 }
 
 int FockPlan::next_slot() { return (prof_used < prof_cap) ? prof_used++ : -1; }

@@ -105,6 +105,10 @@ struct FockPlan {
   // grad Phi is recomputed in the kernel, not stored. V^T is the transposed projection,
   // radial solve, synthesis tables and synthesis (adjoint_potential: z = V^T y on device).
   DeviceBuffer<double> adj_y, adj_z, adj_T, adj_aL, adj_aR, adj_Uy, adj_Udt, adj_rt, ssym;
+  // NAPMO_TUNE_BACKWARD_POINTS: 0, or the points per chunk of recollocated Phi
+  long bwd_points = 0;
+  DeviceBuffer<double> bwd_phi, bwd_X;
+  long backward_chunk() const;
   void adjoint_potential(const double *y, double *z);
   void symmetric_weight();
   void energy_dP(double *dEdP);

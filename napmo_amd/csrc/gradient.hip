@@ -23,13 +23,16 @@ __device__ inline double ipow_g(double v, int e) {
   return r;
 }
 
-// Partial sums over one chunk of points for one function: partial[(mu nchunk + c) 3 + k].
+// Partial sums over one chunk of points for one function: partial[(mu cstride + c0 + c) 3 + k]
+// (cstride = nchunk, c0 = 0 for one call over all points; a call over the points from
+// 4096 c0 on writes its chunks at c0 ...).
 __global__ void __launch_bounds__(256) k_grad_basis(
     const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
-    long N, long chunk, int nchunk, const double *__restrict__ s, const double *__restrict__ X,
-    int ldx, const double *__restrict__ fo, const int *__restrict__ fl, const int *__restrict__ ff,
-    const int *__restrict__ fnp, const double *__restrict__ fnorm, const double *__restrict__ pz,
-    const double *__restrict__ pc, double *__restrict__ partial) {
+    long N, long chunk, int cstride, int c0, const double *__restrict__ s,
+    const double *__restrict__ X, int ldx, const double *__restrict__ fo,
+    const int *__restrict__ fl, const int *__restrict__ ff, const int *__restrict__ fnp,
+    const double *__restrict__ fnorm, const double *__restrict__ pz, const double *__restrict__ pc,
+    double *__restrict__ partial) {
   __shared__ double red[4][3];
   const int mu = blockIdx.y, c = blockIdx.x;
   const double ox = fo[3 * mu], oy = fo[3 * mu + 1], oz = fo[3 * mu + 2];
@@ -65,7 +68,7 @@ __global__ void __launch_bounds__(256) k_grad_basis(
   __syncthreads();
   if (threadIdx.x < 3) {
     const int k = threadIdx.x;
-    partial[((long)mu * nchunk + c) * 3 + k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+    partial[((long)mu * cstride + c0 + c) * 3 + k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
   }
 }
 
@@ -87,6 +90,7 @@ __global__ void k_grad_reduce(const double *__restrict__ partial, int n, int nch
 }  // namespace
 
 constexpr long GRAD_CHUNK = 4096;
+long grad_chunk_points() { return GRAD_CHUNK; }
 
 int grad_chunks(long N) { return (int)std::max<long>(1, div_up(N, GRAD_CHUNK)); }
 
@@ -98,10 +102,31 @@ void launch_grad_centres(const double *x, const double *y, const double *z, long
   if (n <= 0 || ncentre <= 0) return;
   const int nchunk = grad_chunks(N);
   if (N > 0)
-    k_grad_basis<<<dim3(nchunk, n), 256, 0, st>>>(x, y, z, N, GRAD_CHUNK, nchunk, s, X, ldx, fo,
+    k_grad_basis<<<dim3(nchunk, n), 256, 0, st>>>(x, y, z, N, GRAD_CHUNK, nchunk, 0, s, X, ldx, fo,
                                                    fl, ff, fnp, fnorm, pz, pc, partial);
   k_grad_reduce<<<div_up(ncentre * 3, 64), 64, 0, st>>>(partial, n, N > 0 ? nchunk : 0, centre_of,
                                                         ncentre, grad);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_grad_basis_part(const double *x, const double *y, const double *z, long N, long p0,
+                            long Ntotal, const double *s, const double *X, int ldx, int n,
+                            const double *fo, const int *fl, const int *ff, const int *fnp,
+                            const double *fnorm, const double *pz, const double *pc,
+                            double *partial, hipStream_t st) {
+  NAPMO_REQUIRE(p0 % GRAD_CHUNK == 0, "gradient: point ranges must start on a 4096-point chunk");
+  if (n <= 0 || N <= 0) return;
+  k_grad_basis<<<dim3(grad_chunks(N), n), 256, 0, st>>>(
+      x + p0, y + p0, z + p0, N, GRAD_CHUNK, grad_chunks(Ntotal), (int)(p0 / GRAD_CHUNK), s + p0,
+      X, ldx, fo, fl, ff, fnp, fnorm, pz, pc, partial);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_grad_reduce(const double *partial, int n, long Ntotal, const int *centre_of,
+                        int ncentre, double *grad, hipStream_t st) {
+  if (ncentre <= 0) return;
+  k_grad_reduce<<<div_up(ncentre * 3, 64), 64, 0, st>>>(partial, n, Ntotal > 0 ? grad_chunks(Ntotal) : 0,
+                                                        centre_of, ncentre, grad);
   NAPMO_HIP(hipGetLastError());
 }
 

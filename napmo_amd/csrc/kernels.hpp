@@ -301,6 +301,17 @@ void launch_grad_centres(const double *x, const double *y, const double *z, long
                          const int *ff, const int *fnp, const double *fnorm, const double *pz,
                          const double *pc, const int *centre_of, int ncentre, double *partial,
                          double *grad, hipStream_t st);
+// The same in pieces: partial sums of the points p0 .. p0 + N - 1 of an Ntotal-point grid
+// (p0 a multiple of grad_chunk_points(); X holds those points' rows), then the reduction
+// over all chunks: equal, bit for bit, to one launch_grad_centres over all points.
+long grad_chunk_points();
+void launch_grad_basis_part(const double *x, const double *y, const double *z, long N, long p0,
+                            long Ntotal, const double *s, const double *X, int ldx, int n,
+                            const double *fo, const int *fl, const int *ff, const int *fnp,
+                            const double *fnorm, const double *pz, const double *pc,
+                            double *partial, hipStream_t st);
+void launch_grad_reduce(const double *partial, int n, long Ntotal, const int *centre_of,
+                        int ncentre, double *grad, hipStream_t st);
 int density_orb_tiles(int ldc);
 // With one column tile (ldc <= 64), nvalid >= 0 makes the kernel write
 // partial[p] = scale * sum for p < nvalid directly (rho itself, no reduction pass).

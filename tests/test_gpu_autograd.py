@@ -149,3 +149,33 @@ def test_autograd_sixteen_atoms_against_finite_differences():
     dev = torch.device('cuda' if torch.cuda.is_available() else 'cpu')
     gP, gR = _fd_check(model, P, R0, 1e-4, dev, nprint=3)
     assert gR.shape == (16, 3)
+
+
+@pytest.mark.parametrize('case', ['h2', 'chon16'])
+def test_recollocating_backward_matches_stored_phi(case):
+    # NAPMO_TUNE_BACKWARD_POINTS > 0: dE/dP and dE/dR from Phi recollocated chunk by chunk
+    # (neither the stored Phi nor a full Phi P), bit for bit the stored-Phi results
+    import ctypes
+    import napmo_amd
+    from napmo_amd.system import cext
+    if case == 'h2':
+        atoms, shells, nr, na, lmax = TILTED_H2, systems.named_shells(TILTED_H2, 'CC-PVDZ'), 60, 110, 10
+        P_occ = 1
+    else:
+        atoms, shells, _, _, lmax = systems.chon16_system()
+        nr, na, P_occ = 20, 50, 40
+    g, b = systems.gpu_system(atoms, shells, nr, na)
+    f = napmo_amd.GridFock(g, b, lmax, 'fp64')
+    P = systems.density_matrix(np.random.default_rng(6), f.overlap(), P_occ)
+    f.build(P)
+    dP0, gR0 = f.energy_dP(), f.gradient(P)
+    key = 10  # NAPMO_TUNE_BACKWARD_POINTS
+    for pts in (1, 50000):
+        cext.check(cext.napmo_fock_plan_set_tuning(f._this, key, pts))
+        v = ctypes.c_long()
+        cext.check(cext.napmo_fock_plan_get_tuning(f._this, key, ctypes.byref(v)))
+        dP1, gR1 = f.energy_dP(), f.gradient(P)
+        print('%s: %d points per chunk (%d asked) on %d points: dE/dP and dE/dR equal: %s %s'
+              % (case, v.value, pts, g.size, np.array_equal(dP1, dP0), np.array_equal(gR1, gR0)))
+        assert np.array_equal(dP1, dP0) and np.array_equal(gR1, gR0)
+    cext.check(cext.napmo_fock_plan_set_tuning(f._this, key, 0))
