@@ -507,6 +507,11 @@ int napmo_apmo_sync(void *plan);
  * reduction and trace kernels. Equal to the per-species build to rounding. on = 0: back
  * to one stream per species. */
 int napmo_apmo_plan_set_one_launch(void *plan, int on);
+/* One-launch mode only: record one build on fixed device pointers into a HIP graph;
+ * napmo_apmo_replay re-runs it (write the next P into the same buffers between replays);
+ * napmo_apmo_sync waits. */
+int napmo_apmo_capture(void *plan, const double **dP, double **dF, double *dE);
+int napmo_apmo_replay(void *plan);
 
 #ifdef __cplusplus
 }

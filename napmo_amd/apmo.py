@@ -68,5 +68,15 @@ class APMOFock(object):
         pF = (ctypes.c_void_p * ns)(*dFs)
         cext.check(cext.napmo_apmo_build_device(self._this, pP, pF, dE))
 
+    def capture(self, dPs, dFs, dE):
+        """One-launch mode: record one build on these device pointers into a HIP graph."""
+        ns = len(self.species)
+        pP = (ctypes.c_void_p * ns)(*dPs)
+        pF = (ctypes.c_void_p * ns)(*dFs)
+        cext.check(cext.napmo_apmo_capture(self._this, pP, pF, dE))
+
+    def replay(self):
+        cext.check(cext.napmo_apmo_replay(self._this))
+
     def synchronize(self):
         cext.check(cext.napmo_apmo_sync(self._this))

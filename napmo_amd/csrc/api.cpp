@@ -1093,6 +1093,12 @@ int napmo_apmo_sync(void *plan) {
 int napmo_apmo_plan_set_one_launch(void *plan, int on) {
   return guarded([&] { ((ApmoPlan *)plan)->set_one_launch(on != 0); });
 }
+int napmo_apmo_capture(void *plan, const double **dP, double **dF, double *dE) {
+  return guarded([&] { ((ApmoPlan *)plan)->capture(dP, dF, dE); });
+}
+int napmo_apmo_replay(void *plan) {
+  return guarded([&] { ((ApmoPlan *)plan)->replay(); });
+}
 void *napmo_device_malloc(size_t bytes) {
   void *p = nullptr;
   if (guarded([&] {

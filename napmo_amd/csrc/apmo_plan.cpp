@@ -65,6 +65,20 @@ void ApmoPlan::set_one_launch(bool on) {
   group = std::make_unique<GroupPlan>(plans, &charges);
 }
 
+void ApmoPlan::capture(const double *const *dP, double *const *dF, double *dE) {
+  NAPMO_REQUIRE(group != nullptr, "APMO capture: switch on the one-launch mode first");
+  synchronize();
+  const int ns = nspecies();
+  std::vector<double *> dEs(ns);
+  for (int s = 0; s < ns; ++s) dEs[s] = dE + 4 * s;
+  group->capture(dP, dF, dEs.data());
+}
+
+void ApmoPlan::replay() {
+  NAPMO_REQUIRE(group != nullptr, "APMO replay: switch on the one-launch mode first");
+  group->replay();
+}
+
 void ApmoPlan::build_device(const double *const *dP, double *const *dF, double *dE) {
   const int ns = nspecies();
   if (group) {

@@ -33,6 +33,10 @@ struct ApmoPlan {
   // side, own and cross-species pairs in one synthesis per lmax (GroupPlan, coupled)
   std::unique_ptr<GroupPlan> group;
   void set_one_launch(bool on);
+  // One-launch mode: record one build on fixed device pointers into a HIP graph (every
+  // species at once); replay() re-runs it.
+  void capture(const double *const *dP, double *const *dF, double *dE);
+  void replay();
 
   ApmoPlan(const std::vector<DeviceGrid *> &grids, const std::vector<const BasisSet *> &bases,
            const std::vector<double> &charges, const std::vector<int> &lmax, int prec);

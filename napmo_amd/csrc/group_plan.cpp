@@ -180,17 +180,18 @@ GroupPlan::~GroupPlan() {
   if (graph) (void)hipGraphDestroy(graph);
 }
 
-void GroupPlan::io_tables(const double *const *dP, double *const *dF, double *const *dE) {
+void GroupPlan::io_tables(IoTables &io, const double *const *dP, double *const *dF,
+                          double *const *dE) {
   const int nm = (int)plans.size();
-  bool same = (int)cur_P.size() == nm;
+  bool same = (int)io.P.size() == nm;
   for (int m = 0; same && m < nm; ++m)
-    same = cur_P[m] == dP[m] && cur_F[m] == dF[m] && cur_E[m] == dE[m];
+    same = io.P[m] == dP[m] && io.F[m] == dF[m] && io.E[m] == dE[m];
   if (same) return;
   std::vector<PadMember> pm(nm);
   std::vector<FockReduceMember> frm(nm);
   std::vector<TraceMember> tm(nm), t2m(nm);
   std::vector<ApmoEnergyMember> em(nm);
-  pad_blocks = fred_blocks = 0;
+  long pad_blocks = 0, fred_blocks = 0;
   for (int m = 0; m < nm; ++m) {
     FockPlan &q = *plans[m];
     NAPMO_REQUIRE(dP[m] && dF[m] && dE[m], "group build: null pointer");
@@ -210,22 +211,23 @@ void GroupPlan::io_tables(const double *const *dP, double *const *dF, double *co
   }
   // the previous build may still read the old tables: replace them in stream order, and
   // keep the host copies alive until the copies are done
-  pmem.upload(pm, stream);
-  frmem.upload(frm, stream);
-  tmem.upload(tm, stream);
+  io.pmem.upload(pm, stream);
+  io.frmem.upload(frm, stream);
+  io.tmem.upload(tm, stream);
   if (coupled) {
-    t2mem.upload(t2m, stream);
-    emem.upload(em, stream);
+    io.t2mem.upload(t2m, stream);
+    io.emem.upload(em, stream);
   }
   NAPMO_HIP(hipStreamSynchronize(stream));
-  cur_P.assign(dP, dP + nm);
-  cur_F.assign(dF, dF + nm);
-  cur_E.assign(dE, dE + nm);
+  io.pad_blocks = pad_blocks, io.fred_blocks = fred_blocks;
+  io.P.assign(dP, dP + nm);
+  io.F.assign(dF, dF + nm);
+  io.E.assign(dE, dE + nm);
 }
 
-void GroupPlan::launches() {
+void GroupPlan::launches(const IoTables &io) {
   const int nm = (int)plans.size();
-  launch_pad_group(pmem.get(), nm, pad_blocks, stream);
+  launch_pad_group(io.pmem.get(), nm, io.pad_blocks, stream);
   launch_density_group(dmem.get(), nm, dens_blocks, ks, pf, stream);
   launch_rho_reduce_group(rmem.get(), nm, rho_blocks, stream);
   PoissonPlan &pl = ug.plan(lmax);
@@ -249,27 +251,28 @@ void GroupPlan::launches() {
     launch_contrib_reduce_scale_group(smem.get(), nm, scale_blocks, pairs.contrib.get(), cbase.get(),
                                       ug.wbw.get(), v.get(), stream);
   launch_fock_group(fmem.get(), nm, fock_blocks, stream);
-  launch_fock_reduce_group(frmem.get(), nm, fred_blocks, stream);
-  launch_traces_group(tmem.get(), nm, stream);
+  launch_fock_reduce_group(io.frmem.get(), nm, io.fred_blocks, stream);
+  launch_traces_group(io.tmem.get(), nm, stream);
   if (coupled) {
-    launch_traces_group(t2mem.get(), nm, stream);
-    launch_apmo_energy_group(emem.get(), nm, stream);
+    launch_traces_group(io.t2mem.get(), nm, stream);
+    launch_apmo_energy_group(io.emem.get(), nm, stream);
   }
 }
 
 void GroupPlan::build(const double *const *dP, double *const *dF, double *const *dE) {
-  io_tables(dP, dF, dE);
-  launches();
+  io_tables(eager, dP, dF, dE);
+  launches(eager);
 }
 
 void GroupPlan::capture(const double *const *dP, double *const *dF, double *const *dE) {
   synchronize();
-  io_tables(dP, dF, dE);
+  graphed.P.clear();  // a new capture: fresh tables even for the same pointers
+  io_tables(graphed, dP, dF, dE);
   if (exec) (void)hipGraphExecDestroy(exec), exec = nullptr;
   if (graph) (void)hipGraphDestroy(graph), graph = nullptr;
   NAPMO_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
   try {
-    launches();
+    launches(graphed);
   } catch (...) {
     hipGraph_t g = nullptr;
     (void)hipStreamEndCapture(stream, &g);

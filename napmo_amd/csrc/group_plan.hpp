@@ -37,14 +37,20 @@ struct GroupPlan {
   DeviceBuffer<RhoReduceMember> rmem;
   DeviceBuffer<ScaleMember> smem;
   DeviceBuffer<FockMember> fmem;
-  DeviceBuffer<PadMember> pmem;
-  DeviceBuffer<FockReduceMember> frmem;
-  DeviceBuffer<TraceMember> tmem;
-  long dens_blocks = 0, rho_blocks = 0, scale_blocks = 0, fock_blocks = 0, pad_blocks = 0,
-       fred_blocks = 0;
-  // the input / output pointers the pad, reduction and trace tables were made for
-  std::vector<const double *> cur_P;
-  std::vector<double *> cur_F, cur_E;
+  long dens_blocks = 0, rho_blocks = 0, scale_blocks = 0, fock_blocks = 0;
+  // tables that name the build's inputs and outputs: one set for eager builds, one for
+  // the captured graph (which reads its tables at every replay, so eager builds on other
+  // pointers must not rewrite them)
+  struct IoTables {
+    DeviceBuffer<PadMember> pmem;
+    DeviceBuffer<FockReduceMember> frmem;
+    DeviceBuffer<TraceMember> tmem, t2mem;
+    DeviceBuffer<ApmoEnergyMember> emem;
+    long pad_blocks = 0, fred_blocks = 0;
+    std::vector<const double *> P;  // the pointers the tables were made for
+    std::vector<double *> F, E;
+  };
+  IoTables eager, graphed;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 
@@ -57,8 +63,6 @@ struct GroupPlan {
   std::vector<double> charge;
   DeviceBuffer<double> veff, E2;  // [ug.N], [2 x species]
   DeviceBuffer<SpeciesSum> dspecies;
-  DeviceBuffer<TraceMember> t2mem;
-  DeviceBuffer<ApmoEnergyMember> emem;
 
   explicit GroupPlan(const std::vector<FockPlan *> &plans,
                      const std::vector<double> *charges = nullptr);
@@ -72,8 +76,8 @@ struct GroupPlan {
   void synchronize();
 
  private:
-  void io_tables(const double *const *dP, double *const *dF, double *const *dE);
-  void launches();
+  void io_tables(IoTables &t, const double *const *dP, double *const *dF, double *const *dE);
+  void launches(const IoTables &t);
 };
 
 }  // namespace napmo

@@ -170,3 +170,32 @@ def test_apmo_one_launch_matches_per_species_build(shared):
     assert all(np.array_equal(a, b) for a, b in zip(Fb, Fr)) and np.array_equal(Eb, Er)
     with pytest.raises(Exception, match='FP64 plans'):
         napmo_amd.APMOFock(gpu, 'int8', one_launch=True)
+
+
+def test_apmo_one_launch_graph_replays_match_eager():
+    # the one-launch build captured once into a HIP graph; replays on changing densities,
+    # with eager builds and host synchronisation between them, equal eager builds bit for bit
+    import napmo_amd
+    from napmo_amd.device import DeviceArray
+    from napmo_amd.system import cext
+    e_atoms, e_shells, hs, n_shells = _small_system(44)
+    gpu, ora, Ps = _species_pair(e_atoms, e_shells, hs, n_shells, (30, 110), (20, 50), False)
+    plan = napmo_amd.APMOFock(gpu, 'fp64', one_launch=True)
+    dP = [DeviceArray.from_host(P) for P in Ps]
+    dF = [DeviceArray(P.shape) for P in Ps]
+    dE = DeviceArray((8,))
+    plan.capture([a.ptr for a in dP], [a.ptr for a in dF], dE.ptr)
+    for it in range(3):
+        scale = 1.0 + 0.3 * it
+        for a, P in zip(dP, Ps):
+            Pi = np.ascontiguousarray(P * scale)
+            cext.check(cext.napmo_memcpy(a.ptr, Pi.ctypes.data, Pi.nbytes, 1))
+        plan.replay()
+        plan.synchronize()
+        got = [a.to_host() for a in dF], dE.to_host()
+        Fe, Ee = plan.build([P * scale for P in Ps])
+        assert all(np.array_equal(a, b) for a, b in zip(got[0], Fe)), it
+        assert np.array_equal(got[1], Ee.reshape(-1)), it
+        print('replay', it, 'E', Ee[:, 1])
+    with pytest.raises(Exception, match='one-launch'):
+        napmo_amd.APMOFock(gpu, 'fp64').replay()
