@@ -199,3 +199,26 @@ def test_apmo_one_launch_graph_replays_match_eager():
         print('replay', it, 'E', Ee[:, 1])
     with pytest.raises(Exception, match='one-launch'):
         napmo_amd.APMOFock(gpu, 'fp64').replay()
+
+
+def test_apmo_one_launch_three_species():
+    # electrons and two proton species on their own grids: every species couples to the two
+    # others; the one-launch build equals the per-species build bit for bit
+    import napmo_amd
+    from napmo_amd.workloads import even_tempered_shells
+    e_atoms, e_shells, hs, n_shells = _small_system(45)
+    ge, be = systems.gpu_system(e_atoms, e_shells, 30, 110)
+    species, Ps = [(ge, be, -1.0, 8)], []
+    for h in hs:
+        gn, bn = systems.gpu_system([h], [even_tempered_shells(nexp=3)], 20, 50)
+        species.append((gn, bn, 1.0, 6))
+    rng = np.random.default_rng(9)
+    for g, b, q, lm in species:
+        S = napmo_amd.GridFock(g, b, lm, 'fp64').overlap()
+        C = systems.occupied_orbitals(rng, S, 2 if q < 0 else 1)
+        Ps.append((2.0 if q < 0 else 1.0) * C @ C.T)
+    Fr, Er = napmo_amd.APMOFock(species, 'fp64').build(Ps)
+    Fo, Eo = napmo_amd.APMOFock(species, 'fp64', one_launch=True).build(Ps)
+    print('three species: E per species', Eo[:, 1], 'coupling', Eo[:, 3])
+    for s in range(3):
+        assert np.array_equal(Fo[s], Fr[s]) and np.array_equal(Eo[s], Er[s]), s

@@ -179,3 +179,25 @@ def test_every_launch_shape_gives_the_same_build():
             builds.append(f.build(Ps[0]))
         assert np.array_equal(builds[0][0], builds[1][0]) and builds[0][2] == builds[1][2]
         print(prec, len(autotune.candidates(f)), 'launch shapes agree')
+
+
+def test_one_launch_per_phase_single_molecule_and_repeated_pointers():
+    # a group of one plan, and a group rebuilt on new output buffers: both equal the plan's
+    # own build bit for bit
+    import napmo_amd
+    from napmo_amd import workloads
+    mols = workloads.batch_molecules(count=3, seed=21)
+    grids, bases, plans, Ps = _setup(mols)
+    for k in range(2):
+        dP, dF, dE = _device(Ps[k:k + 1])
+        batch = napmo_amd.BatchFock(plans[k:k + 1], nlanes=0)
+        batch.build([a.ptr for a in dP], [a.ptr for a in dF], [a.ptr for a in dE])
+        batch.synchronize()
+        F, E, Eg = plans[k].build(Ps[k])
+        Eb = dE[0].to_host()
+        assert np.array_equal(dF[0].to_host(), F) and Eb[0] == E and Eb[1] == Eg, k
+        dP2, dF2, dE2 = _device(Ps[k:k + 1])
+        batch.build([a.ptr for a in dP2], [a.ptr for a in dF2], [a.ptr for a in dE2])
+        batch.synchronize()
+        assert np.array_equal(dF2[0].to_host(), F), k
+        del batch

@@ -99,3 +99,20 @@ def test_extrapolation_buckets_mfma_matches_one_at_a_time():
     err = np.abs(a - b).max()
     print('extrapolation grid: max|MFMA - one at a time| %.3e (max|K| %.3e)' % (err, scale))
     assert err < 1e-13 * scale
+
+
+@pytest.mark.parametrize('norb', [1, 15, 17])
+def test_tile_boundaries(norb):
+    # a lone density, a tile short of one, and a full tile plus one: every threshold gives
+    # the one-at-a-time result to rounding, and the default threshold's split is reproducible
+    import napmo_amd
+    g, _ = systems.gpu_system(ATOMS, [[]] * 3, 20, 50)
+    o, _ = systems.oracle_system(ATOMS, [[]] * 3, 20, 50)
+    phi, ri, V = _orbital_inputs(o, norb, 8)
+    doi, oi = np.linspace(0.02, -0.01, norb), np.linspace(-0.6, -0.3, norb)
+    run = lambda: napmo_amd.compute_dpsi(g, 4, phi, doi, oi, ri, V, 1.0)  # noqa: E731
+    a, b = _both(g, run)
+    assert np.abs(a - b).max() < 1e-13 * np.abs(b).max()
+    d0 = run()
+    assert np.array_equal(d0, run())
+    print('norb %d: max|MFMA - one at a time| %.2e' % (norb, np.abs(a - b).max()))
