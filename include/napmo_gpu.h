@@ -478,6 +478,12 @@ int napmo_batch_plan_new(int nplan, void **plans, int nlanes, void **batch_out);
 void napmo_batch_plan_del(void *batch);
 int napmo_batch_build(void *batch, const double **dP, double **dF, double **dE);
 int napmo_batch_capture(void *batch, const double **dP, double **dF, double **dE);
+/* Orbital form (nlanes >= 1): dC[m] [n_m x nocc[m]] row-major, P_m = eta C C^T; the same
+ * results as the plans' own orbital-form builds. */
+int napmo_batch_build_orbitals(void *batch, const double **dC, const int *nocc, double eta,
+                               double **dF, double **dE);
+int napmo_batch_capture_orbitals(void *batch, const double **dC, const int *nocc, double eta,
+                                 double **dF, double **dE);
 int napmo_batch_replay(void *batch);
 int napmo_batch_sync(void *batch);
 

@@ -36,6 +36,17 @@ class BatchFock(object):
         """Asynchronous batch build on device pointers (lists of ints, one per molecule)."""
         cext.check(cext.napmo_batch_build(self._this, self._ptrs(dPs), self._ptrs(dFs), self._ptrs(dEs)))
 
+    def build_orbitals(self, dCs, noccs, eta, dFs, dEs):
+        """Orbital form (streams): dCs [n_m x nocc_m] device pointers, P_m = eta C C^T."""
+        nocc = (ctypes.c_int * len(noccs))(*[int(k) for k in noccs])
+        cext.check(cext.napmo_batch_build_orbitals(self._this, self._ptrs(dCs), nocc, float(eta),
+                                                   self._ptrs(dFs), self._ptrs(dEs)))
+
+    def capture_orbitals(self, dCs, noccs, eta, dFs, dEs):
+        nocc = (ctypes.c_int * len(noccs))(*[int(k) for k in noccs])
+        cext.check(cext.napmo_batch_capture_orbitals(self._this, self._ptrs(dCs), nocc, float(eta),
+                                                     self._ptrs(dFs), self._ptrs(dEs)))
+
     def capture(self, dPs, dFs, dEs):
         cext.check(cext.napmo_batch_capture(self._this, self._ptrs(dPs), self._ptrs(dFs), self._ptrs(dEs)))
 

@@ -1057,6 +1057,14 @@ int napmo_batch_build(void *batch, const double **dP, double **dF, double **dE) 
 int napmo_batch_capture(void *batch, const double **dP, double **dF, double **dE) {
   return guarded([&] { ((BatchPlan *)batch)->capture(dP, dF, dE); });
 }
+int napmo_batch_build_orbitals(void *batch, const double **dC, const int *nocc, double eta,
+                               double **dF, double **dE) {
+  return guarded([&] { ((BatchPlan *)batch)->build_orbitals(dC, nocc, eta, dF, dE); });
+}
+int napmo_batch_capture_orbitals(void *batch, const double **dC, const int *nocc, double eta,
+                                 double **dF, double **dE) {
+  return guarded([&] { ((BatchPlan *)batch)->capture_orbitals(dC, nocc, eta, dF, dE); });
+}
 int napmo_batch_replay(void *batch) {
   return guarded([&] { ((BatchPlan *)batch)->replay(); });
 }

@@ -1,6 +1,8 @@
 // Batched builds of independent molecules (see batch_plan.hpp).
 #include "batch_plan.hpp"
 
+#include <functional>
+
 namespace napmo {
 
 BatchPlan::BatchPlan(const std::vector<FockPlan *> &p, int nlanes) : plans(p) {
@@ -58,8 +60,31 @@ void BatchPlan::build(const double *const *dP, double *const *dF, double *const 
   }
 }
 
+void BatchPlan::build_orbitals(const double *const *dC, const int *nocc, double eta,
+                               double *const *dF, double *const *dE) {
+  NAPMO_REQUIRE(!group, "orbital-form batch: streams only (nlanes >= 1)");
+  NAPMO_HIP(hipEventRecord(fork, origin));
+  for (auto s : lanes) NAPMO_HIP(hipStreamWaitEvent(s, fork, 0));
+  for (size_t m = 0; m < plans.size(); ++m)
+    plans[m]->build_orbitals_device(dC[m], nocc[m], eta, dF[m], dE[m]);
+  for (size_t l = 0; l < lanes.size(); ++l) {
+    NAPMO_HIP(hipEventRecord(joins[l], lanes[l]));
+    NAPMO_HIP(hipStreamWaitEvent(origin, joins[l], 0));
+  }
+}
+
+void BatchPlan::capture_orbitals(const double *const *dC, const int *nocc, double eta,
+                                 double *const *dF, double *const *dE) {
+  NAPMO_REQUIRE(!group, "orbital-form batch: streams only (nlanes >= 1)");
+  record([&] { build_orbitals(dC, nocc, eta, dF, dE); });
+}
+
 void BatchPlan::capture(const double *const *dP, double *const *dF, double *const *dE) {
   if (group) return group->capture(dP, dF, dE);
+  record([&] { build(dP, dF, dE); });
+}
+
+void BatchPlan::record(const std::function<void()> &launch) {
   for (auto *q : plans)
     NAPMO_REQUIRE(q->prof_used >= q->prof_cap, "batch capture: stop per-phase profiling first");
   synchronize();
@@ -67,7 +92,7 @@ void BatchPlan::capture(const double *const *dP, double *const *dF, double *cons
   if (graph) (void)hipGraphDestroy(graph), graph = nullptr;
   NAPMO_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeRelaxed));
   try {
-    build(dP, dF, dE);
+    launch();
   } catch (...) {
     hipGraph_t g = nullptr;
     (void)hipStreamEndCapture(origin, &g);

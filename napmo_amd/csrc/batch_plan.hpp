@@ -5,6 +5,7 @@
 // graph and replayed per SCF iteration.
 #pragma once
 
+#include <functional>
 #include <memory>
 #include <vector>
 
@@ -29,10 +30,19 @@ struct BatchPlan {
   // One build of every plan: dP[m], dF[m] [n_m x n_m], dE[m] [2] (device pointers);
   // asynchronous on the origin stream.
   void build(const double *const *dP, double *const *dF, double *const *dE);
+  // Orbital form (streams only): dC[m] [n_m x nocc[m]] with P_m = eta C C^T (the plans'
+  // build_orbitals_device).
+  void build_orbitals(const double *const *dC, const int *nocc, double eta, double *const *dF,
+                      double *const *dE);
   // Records the same batch into a graph (pointers baked in); replay() launches it.
   void capture(const double *const *dP, double *const *dF, double *const *dE);
+  void capture_orbitals(const double *const *dC, const int *nocc, double eta, double *const *dF,
+                        double *const *dE);
   void replay();
   void synchronize();
+
+ private:
+  void record(const std::function<void()> &launch);  // capture of one batch build
 };
 
 }  // namespace napmo

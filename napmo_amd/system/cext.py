@@ -214,6 +214,8 @@ _SIGNATURES = {
     'napmo_batch_plan_del': (None, [vp]),
     'napmo_batch_build': (c_int, [vp, vp, vp, vp]),
     'napmo_batch_capture': (c_int, [vp, vp, vp, vp]),
+    'napmo_batch_build_orbitals': (c_int, [vp, vp, vp, c_double, vp, vp]),
+    'napmo_batch_capture_orbitals': (c_int, [vp, vp, vp, c_double, vp, vp]),
     'napmo_batch_replay': (c_int, [vp]),
     'napmo_batch_sync': (c_int, [vp]),
     'napmo_apmo_plan_new': (c_int, [c_int, vp, vp, vp, vp, c_int, ctypes.POINTER(vp)]),

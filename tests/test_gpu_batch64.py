@@ -148,3 +148,44 @@ def test_batch64_one_launch_per_phase(batch64, fp64_builds):
             assert np.array_equal(dF[k].to_host(), F) and Eb[0] == E and Eb[1] == Eg, (it, k)
     print('fp64, one launch per phase: 64 builds = single builds, eager and 3 graph replays')
     del batch, plans
+
+
+def test_batch64_orbital_form(batch64):
+    # the orbital form over 4 streams (P = 2 C C^T from the same seeded orbitals): the batch
+    # equals the plans' own orbital-form builds bit for bit, eagerly and from three graph
+    # replays, for FP64 and int8
+    import napmo_amd
+    from napmo_amd import workloads
+    from napmo_amd.device import DeviceArray
+    from napmo_amd.system import cext
+    mols, grids, bases, Ps = batch64
+    for prec in ('fp64', 'int8'):
+        plans = [napmo_amd.GridFock(g, b, m['lmax'], prec) for g, b, m in zip(grids, bases, mols)]
+        rng = np.random.default_rng(5)
+        Cs = [workloads.occupied_orbitals(rng, f.overlap(), m['nocc']) for f, m in zip(plans, mols)]
+        assert all(np.allclose(2.0 * C @ C.T, P, atol=1e-12) for C, P in zip(Cs, Ps))
+        single = [f.build_orbitals(C) for f, C in zip(plans, Cs)]
+        dC = [DeviceArray.from_host(C) for C in Cs]
+        dF = [DeviceArray(P.shape) for P in Ps]
+        dE = [DeviceArray((2,)) for _ in Ps]
+        nocc = [C.shape[1] for C in Cs]
+        batch = napmo_amd.BatchFock(plans, nlanes=4)
+        batch.build_orbitals(_ptr(dC), nocc, 2.0, _ptr(dF), _ptr(dE))
+        batch.synchronize()
+        for k, (F, E, Eg) in enumerate(single):
+            Eb = dE[k].to_host()
+            assert np.array_equal(dF[k].to_host(), F) and Eb[0] == E and Eb[1] == Eg, (prec, k)
+        batch.capture_orbitals(_ptr(dC), nocc, 2.0, _ptr(dF), _ptr(dE))
+        for it in range(3):
+            for a, P in zip(dF, Ps):  # outputs cleared, so every replay rewrites them
+                Z = np.zeros_like(P)
+                cext.check(cext.napmo_memcpy(a.ptr, Z.ctypes.data, Z.nbytes, 1))
+            batch.replay()
+            batch.synchronize()
+            for k, (F, E, Eg) in enumerate(single):
+                Eb = dE[k].to_host()
+                assert np.array_equal(dF[k].to_host(), F) and Eb[0] == E and Eb[1] == Eg, (prec, it, k)
+        print('%s orbital form: 64 single builds = eager batch = 3 graph replays' % prec)
+        with pytest.raises(cext.NapmoError):
+            napmo_amd.BatchFock(plans, nlanes=0).build_orbitals(_ptr(dC), nocc, 2.0, _ptr(dF), _ptr(dE))
+        del batch, plans
