@@ -37,11 +37,16 @@ GATE_EH = 1e-6   # |dE_trace| of a mixed mode against the FP64 mode, in Hartree
 METRIC = 'grid Fock builds/s + % roofline, mixed vs FP64 on 1 MI355X at |dE|<=1uEh'
 
 
-MODES = {  # name: (plan precision, density input)
+# int8s: the int8 plan with screening (NAPMO_TUNE_INT8_SCREEN): in each chunk of points the
+# 16 x 16 blocks of F bounded by 2^-SCREEN_S in every element are skipped
+SCREEN_S = 50
+PLAN_PREC = {'int8s': 'int8'}
+MODES = {  # name: (plan, density input)
     'fp64': ('fp64', 'P'),
     'fp64-orb': ('fp64', 'C'),
     'int8-orb': ('int8', 'C'),
     'int8': ('int8', 'P'),
+    'int8s-orb': ('int8s', 'C'),
     'bf16x3-orb': ('bf16x3', 'C'),
     'bf16x3': ('bf16x3', 'P'),
 }
@@ -65,7 +70,9 @@ def gpu_leg(args, log):
         if prec not in plans:
             t0 = time.perf_counter()
             try:
-                plans[prec] = napmo_amd.GridFock(grid, basis, lmax, prec)
+                plans[prec] = napmo_amd.GridFock(grid, basis, lmax, PLAN_PREC.get(prec, prec))
+                if prec == 'int8s':
+                    plans[prec].set_screen(SCREEN_S)
             except cext.NapmoError as e:
                 log('precision %s unavailable: %s' % (prec, e))
                 continue
@@ -88,7 +95,7 @@ def gpu_leg(args, log):
                 setting, src = autotune.tune(plan, dP.ptr, dF.ptr, dE.ptr)
             else:
                 setting, src = autotune.tune(
-                    plan, dP.ptr, dF.ptr, dE.ptr, tag='orb',
+                    plan, dP.ptr, dF.ptr, dE.ptr, tag='orb' if prec != 'int8s' else 'orb-s%d' % SCREEN_S,
                     build=lambda plan=plan: plan.build_orbitals_device(dC.ptr, C.shape[1], 2.0, dF.ptr, dE.ptr))
             log('autotune %s: %s (%s, %.1fs)' % (prec, setting, src, time.perf_counter() - t0))
     results = {}
@@ -120,6 +127,10 @@ def gpu_leg(args, log):
         F = dF.to_host()
         results[m] = {'ms_per_build': dt * 1e3, 'builds_per_s': 1.0 / dt, 'E': float(E[0]),
                       'E_grid': float(E[1]), 'phases_ms': phases, 'F': F, 'form': form, 'prec': prec}
+        if PLAN_PREC.get(prec, prec) == 'int8':
+            kept, total = plan.screen_stats()
+            results[m]['blocks_kept'] = kept / total
+            log('%s: %d of %d (block, chunk) products computed (%.1f%%)' % (m, kept, total, 100.0 * kept / total))
         log('%s: %.3f ms/build  E_trace=%.12f E_grid=%.12f phases=%s' % (
             m, dt * 1e3, E[0], E[1], {k: round(v, 4) for k, v in phases.items()}))
     N = grid.size
@@ -566,7 +577,8 @@ def batch_leg(args, log):
 
 
 PHASE_KERNELS = {'density': 'k_density_mfma',
-                 'fock': {'bf16x3': 'k_fock_bf16x3_pass', 'fp64': 'k_fock_f64', 'int8': 'k_ozaki_fock'},
+                 'fock': {'bf16x3': 'k_fock_bf16x3_pass', 'fp64': 'k_fock_f64', 'int8': 'k_ozaki_fock',
+                          'int8s': 'k_ozaki_fock [screened 2^-%d]' % SCREEN_S},
                  'synth': 'k_synth2n<', 'project': 'k_project', 'radial': 'k_radial',
                  'fock_reduce': 'k_fock_reduce'}
 
@@ -624,8 +636,10 @@ def phase_work(res, name, N, n, L, M):
         k = n if res['form'] == 'P' else res['nocc']
         return 2.0 * N * n * k, PEAK['fp64_mfma'], 'TFLOP/s', 'mfma'
     if name == 'fock':
-        if res['prec'] == 'int8':
-            return INT8_PRODUCTS * 2.0 * N * n * n, PEAK['int8_mfma'], 'TOP/s', 'mfma'
+        if res['prec'] in ('int8', 'int8s'):
+            # screened: the (block, chunk) products the kernel computed
+            return (INT8_PRODUCTS * 2.0 * N * n * n * res.get('blocks_kept', 1.0), PEAK['int8_mfma'],
+                    'TOP/s', 'mfma')
         if res['prec'] == 'bf16x3':
             return 3 * 2.0 * N * n * n, PEAK['bf16_mfma'], 'TFLOP/s', 'mfma'
         return 2.0 * N * n * n, PEAK['fp64_mfma'], 'TFLOP/s', 'mfma'
@@ -645,6 +659,9 @@ def roofline(res, N, n, L, M):
     scale = 1e9 if unit == 'GB/s' else 1e12
     roof = {'bound': bound, 'achieved': work / (ms * 1e-3) / scale, 'peak': peak / scale,
             'unit': unit, 'frac': work / (ms * 1e-3) / peak, 'traffic': measured_traffic(res, name)}
+    if name == 'fock' and res['prec'] == 'int8s':
+        roof['blocks_kept'] = res['blocks_kept']
+        roof['achieved_dense_equivalent'] = roof['achieved'] / res['blocks_kept']
     if name == 'synth':
         roof['issued_flops_per_term'] = SYNTH_FLOPS_ISSUED
         roof['achieved_issued'] = SYNTH_FLOPS_ISSUED * M * N * L / (ms * 1e-3) / 1e12
@@ -660,7 +677,7 @@ def build_roofline(res, N, n, L, M):
     for name in ('density', 'fock', 'project', 'synth'):
         work, peak, _, _ = phase_work(res, name, N, n, L, M)
         t += work / peak
-    phi_bytes = {'fp64': 8.0, 'int8': 4.0, 'bf16x3': 4.0}[res['prec']]
+    phi_bytes = {'fp64': 8.0, 'int8': 4.0, 'int8s': 4.0, 'bf16x3': 4.0}[res['prec']]
     nbytes = N * n * 8.0 + N * n * phi_bytes + 40.0 * N + 16.0 * n * n
     t = max(t, nbytes / PEAK['hbm'])
     return {'t_roof_ms': t * 1e3, 't_ms': res['ms_per_build'], 'frac': t * 1e3 / res['ms_per_build']}
@@ -752,7 +769,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--modes', default='fp64,fp64-orb,int8-orb,bf16x3-orb',
+    ap.add_argument('--modes', default='fp64,fp64-orb,int8-orb,int8s-orb,bf16x3-orb',
                     help='comma list of modes (%s); fp64 is always timed' % ', '.join(MODES))
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-tune', action='store_true', help='keep the default launch shapes')
@@ -830,11 +847,13 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': {'fp64': 'f64', 'int8': 'int8+f64', 'bf16x3': 'bf16x3+f64'}[head['prec']],
+        'dtype': {'fp64': 'f64', 'int8': 'int8+f64', 'int8s': 'int8+f64', 'bf16x3': 'bf16x3+f64'}[head['prec']],
         'data': 'synthetic (seeded geometry, basis and density; no reference data files)',
         'config': {'workload': 'grid Fock build, 16 atoms, 50x590 Becke grid (472,000 points), '
                                '300 s/p/d Gaussians, lmax 12, P resident in HBM',
-                   'npoint': N, 'nbasis': n, 'ncenter': M, 'lmax': lmax, 'precision': best},
+                   'npoint': N, 'nbasis': n, 'ncenter': M, 'lmax': lmax, 'precision': best,
+                   'screening': ('F blocks bounded by 2^-%d per element and chunk of points skipped'
+                                 % SCREEN_S if head['prec'] == 'int8s' else 'none')},
         'roofline': dict(roof, kernel=kname),
         'build_roofline': build_roofline(head, N, n, L, M),
         'cpu_baseline': cpu,
@@ -845,6 +864,7 @@ def main():
                       'E_grid': r['E_grid'], 'E_trace': r['E'], 'dE_grid_vs_fp64': r.get('dE_grid', 0.0),
                       'dE_trace_vs_fp64': r.get('dE_trace', 0.0), 'dF_max_vs_fp64': r.get('dF_max', 0.0),
                       'meets_gate': abs(r.get('dE_trace', 0.0)) <= GATE_EH,
+                      'blocks_kept': r.get('blocks_kept'),
                       'phases_ms': r['phases_ms'],
                       'build_roofline': build_roofline(r, N, n, L, M)}
                   for m, r in results.items()},

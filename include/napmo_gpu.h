@@ -450,14 +450,23 @@ enum {
                                    of at least this many, run on FP64 MFMA, the others one at a
                                    time; 1..16, 17 = one at a time only; 0 = 10 (a property of
                                    the grid) */
-  NAPMO_TUNE_BACKWARD_POINTS = 10 /* energy derivatives (dE/dP, dE/dR): 0 = from the stored
+  NAPMO_TUNE_BACKWARD_POINTS = 10, /* energy derivatives (dE/dP, dE/dR): 0 = from the stored
                                      Phi with Phi P for all points at once; k > 0 = Phi
                                      recollocated in chunks of about k points (rounded up to
                                      a multiple of the contraction chunks), neither the stored
                                      Phi nor a full Phi P read: the same results bit for bit */
+  NAPMO_TUNE_INT8_SCREEN = 11 /* int8 mode: S > 0 skips, per chunk of points, the 16 x 16 blocks
+                                 of F whose chunk sum is bounded by 2^-S in every element (from
+                                 the chunk maxima of |sqrt(w bw) Phi| and |v|), so each element
+                                 moves by at most (chunks) 2^-S; 0 = off (default). Not a
+                                 launch shape: results change within that bound */
 };
 int napmo_fock_plan_set_tuning(void *plan, int key, long value);
 int napmo_fock_plan_get_tuning(void *plan, int key, long *value);
+/* int8 plan: the upper 16 x 16 blocks x chunks of points that the last contraction computed
+ * under NAPMO_TUNE_INT8_SCREEN (*kept) and all of them (*total); equal when screening is off.
+ * Synchronises the plan's stream. No reference counterpart (measurement). */
+int napmo_fock_plan_int8_screen_stats(void *plan, long *kept, long *total);
 /* The grid-level keys (NAPMO_TUNE_SYNTH_NP, NAPMO_TUNE_SYNTH_MFMA_MIN) on a BeckeGrid handle. */
 int napmo_grid_set_tuning(void *grid, int key, long value);
 int napmo_grid_get_tuning(void *grid, int key, long *value);

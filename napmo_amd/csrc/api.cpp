@@ -1015,6 +1015,13 @@ int napmo_fock_plan_set_tuning(void *plan, int key, long value) {
 int napmo_fock_plan_get_tuning(void *plan, int key, long *value) {
   return guarded([&] { *value = ((FockPlan *)plan)->get_tuning(key); });
 }
+int napmo_fock_plan_int8_screen_stats(void *plan, long *kept, long *total) {
+  return guarded([&] {
+    NAPMO_REQUIRE(kept && total, "int8 screening statistics: null output");
+    const auto st = ((FockPlan *)plan)->int8_screen_stats();
+    *kept = st.first, *total = st.second;
+  });
+}
 
 int napmo_grid_set_tuning(void *grid, int key, long value) {
   return guarded([&] {

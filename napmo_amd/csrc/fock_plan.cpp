@@ -118,7 +118,10 @@ void FockPlan::set_tuning(int key, long value) {
       break;
     case NAPMO_TUNE_INT8_CHUNK:
     case NAPMO_TUNE_INT8_WAVE_BLOCKS:
+    case NAPMO_TUNE_INT8_SCREEN:
       NAPMO_REQUIRE(prec == NAPMO_PREC_INT8, "int8 tuning keys need an NAPMO_PREC_INT8 plan");
+      NAPMO_REQUIRE(key != NAPMO_TUNE_INT8_SCREEN || (value >= 0 && value <= 1000),
+                    "int8 screening threshold must be 0 (off) to 1000");
       NAPMO_REQUIRE(key != NAPMO_TUNE_INT8_CHUNK || value == 0 ||
                         (value % OZ_STAGE == 0 && value <= (long)OZ_MAX_STAGES * OZ_STAGE),
                     "int8 chunk must be a multiple of 64 points, at most 4032");
@@ -127,8 +130,10 @@ void FockPlan::set_tuning(int key, long value) {
       NAPMO_HIP(hipStreamSynchronize(stream));
       if (key == NAPMO_TUNE_INT8_CHUNK)
         oz_chunk = value;
-      else
+      else if (key == NAPMO_TUNE_INT8_WAVE_BLOCKS)
         oz_wave_blocks = value ? (int)value : OZ_MAXB;
+      else
+        oz_screen = (int)value;
       ozaki_setup();
       break;
     default: throw Error("unknown tuning key " + std::to_string(key));
@@ -150,6 +155,7 @@ long FockPlan::get_tuning(int key) const {
     case NAPMO_TUNE_DENSITY_STREAM: return density_stream;
     case NAPMO_TUNE_INT8_CHUNK: return prec == NAPMO_PREC_INT8 ? oz.chunk : 0;
     case NAPMO_TUNE_INT8_WAVE_BLOCKS: return prec == NAPMO_PREC_INT8 ? oz_wave_blocks : 0;
+    case NAPMO_TUNE_INT8_SCREEN: return prec == NAPMO_PREC_INT8 ? oz_screen : 0;
     default: throw Error("unknown tuning key " + std::to_string(key));
   }
 }
@@ -279,8 +285,30 @@ void FockPlan::ozaki_setup() {
   std::vector<int> tab = oz.tasks;
   tab.insert(tab.end(), oz.row0.begin(), oz.row0.end());
   oz_tasks.upload(tab, st);
-  launch_ozaki_setup(oz, phi64.get(), grid.wbw.get(), oz_planes.get(), oz_eA.get(), oz_maxd1.get(), st);
+  oz_perm.resize(0);
+  std::vector<int> order;  // kept until the stream is synchronised below
+  if (oz_screen > 0) {     // the compact functions first, from their chunk exponents
+    launch_ozaki_scales(oz, phi64.get(), grid.wbw.get(), nullptr, oz_eA.get(), st);
+    std::vector<int> eA((size_t)ld * oz.nchunk);
+    NAPMO_HIP(hipMemcpyAsync(eA.data(), oz_eA.get(), eA.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    NAPMO_HIP(hipStreamSynchronize(st));
+    order = ozaki_compact_order(oz, eA);
+    oz_perm.upload(order, st);
+  }
+  launch_ozaki_setup(oz, phi64.get(), grid.wbw.get(), oz_perm.get(), oz_planes.get(), oz_eA.get(),
+                     oz_maxd1.get(), st);
   NAPMO_HIP(hipStreamSynchronize(st));
+}
+
+std::pair<long, long> FockPlan::int8_screen_stats() {
+  NAPMO_REQUIRE(prec == NAPMO_PREC_INT8, "int8 screening statistics need an NAPMO_PREC_INT8 plan");
+  std::vector<int> eA((size_t)ld * oz.nchunk);
+  std::vector<double> vs(oz.Nq);
+  NAPMO_HIP(hipStreamSynchronize(stream));
+  oz_eA.download(eA.data(), eA.size(), stream);
+  oz_vs.download(vs.data(), vs.size(), stream);
+  NAPMO_HIP(hipStreamSynchronize(stream));
+  return {ozaki_kept_blocks(oz, eA, vs, oz_screen), (long)oz.nchunk * oz.nblk};
 }
 
 void FockPlan::ozaki_contract(const double *veff, double *dF, int slot) {
@@ -288,9 +316,9 @@ void FockPlan::ozaki_contract(const double *veff, double *dF, int slot) {
   launch_ozaki_vs(veff, grid.wbw.get(), grid.N, oz.Nq, oz_vs.get(), st);
   mark(slot, 7);
   launch_ozaki_fock(oz, oz_tasks.get(), oz_planes.get(), oz_eA.get(), oz_maxd1.get(), oz_vs.get(),
-                    oz_part.get(), st);
+                    oz_screen, oz_part.get(), st);
   mark(slot, 8);
-  launch_ozaki_reduce(oz, oz_part.get(), dF, st);
+  launch_ozaki_reduce(oz, oz_part.get(), oz_perm.get(), dF, st);
 }
 
 void FockPlan::contract(int slot, const double *dP, const double *veff, double *dF, double *dE,

@@ -41,9 +41,13 @@ struct FockPlan {
   DeviceBuffer<unsigned char> oz_maxd1;
   DeviceBuffer<double> oz_vs, oz_part;
   long oz_chunk = 0;       // NAPMO_TUNE_INT8_CHUNK (0: default)
+  int oz_screen = 0;       // NAPMO_TUNE_INT8_SCREEN (0: off)
+  DeviceBuffer<int> oz_perm;  // screening order of the functions (empty: identity)
   int oz_wave_blocks = OZ_MAXB;  // NAPMO_TUNE_INT8_WAVE_BLOCKS
   void ozaki_setup();
   void ozaki_contract(const double *veff, double *dF, int slot);
+  // int8 mode: (upper blocks x chunks computed by the last contraction, all of them)
+  std::pair<long, long> int8_screen_stats();
   int use_pass = 0;  // 0: per-tile kernels, 1: one-pass BF16x3, 2: symmetric one-pass BF16x3
   long fock_wg = 0, pass_wg = 0;  // workgroup targets of the two shapes (0 = default)
   int density_ks = 16;            // K per LDS stage of the density kernels (16 or 32)

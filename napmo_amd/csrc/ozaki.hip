@@ -63,22 +63,25 @@ __device__ inline size_t seg_off(int nf, int plane, int fl, int q) {
 }
 
 // ------------------------------------------------------------------ per geometry
-// eA[f][c] = exponent of the chunk maximum of |sqrt|wbw| phi| (one workgroup per chunk).
+// eA[f][c] = exponent of the chunk maximum of |sqrt|wbw| phi| (one workgroup per chunk;
+// OZ_ZERO_EXP for a function that is zero in the chunk, whose digits are all zero either way).
 __global__ void __launch_bounds__(256) k_oz_scales(const double *__restrict__ phi64, int ld,
-                                                   const double *__restrict__ wbw, long N, long chunk,
-                                                   int nchunk, int *__restrict__ eA) {
+                                                   const double *__restrict__ wbw, const int *__restrict__ perm,
+                                                   long N, long chunk, int nchunk, int *__restrict__ eA) {
   const int c = blockIdx.x;
   const long p0 = (long)c * chunk, p1 = min(N, p0 + chunk);
   for (int f = threadIdx.x; f < ld; f += blockDim.x) {
+    const int col = perm ? perm[f] : f;
     double m = 0.0;
-    for (long p = p0; p < p1; ++p) m = fmax(m, fabs(sqrt(fabs(wbw[p])) * phi64[p * ld + f]));
-    eA[(size_t)f * nchunk + c] = pow2_ceil_exp(m);
+    for (long p = p0; p < p1; ++p) m = fmax(m, fabs(sqrt(fabs(wbw[p])) * phi64[p * ld + col]));
+    eA[(size_t)f * nchunk + c] = m > 0.0 ? pow2_ceil_exp(m) : OZ_ZERO_EXP;
   }
 }
 
 // Digit planes and per-stage max |d1|: one workgroup per (stage of 64 points, 64 functions).
 __global__ void __launch_bounds__(256) k_oz_digits(const double *__restrict__ phi64, int ld,
-                                                   const double *__restrict__ wbw, long N, long Nq,
+                                                   const double *__restrict__ wbw,
+                                                   const int *__restrict__ perm, long N, long Nq,
                                                    long chunk, int nchunk, const int *__restrict__ eA,
                                                    signed char *__restrict__ planes,
                                                    unsigned char *__restrict__ maxd1) {
@@ -89,7 +92,8 @@ __global__ void __launch_bounds__(256) k_oz_digits(const double *__restrict__ ph
   for (int e = tid; e < OZ_STAGE * 64; e += 256) {
     const int pr = e / 64, fc = e % 64;
     const long p = p0 + pr;
-    tile[pr][fc] = p < N ? sqrt(fabs(wbw[p])) * phi64[p * ld + f0 + fc] : 0.0;
+    const int col = perm ? perm[f0 + fc] : f0 + fc;
+    tile[pr][fc] = p < N ? sqrt(fabs(wbw[p])) * phi64[p * ld + col] : 0.0;
   }
   __syncthreads();
   const int fl = tid >> 2, q = tid & 3, f = f0 + fl;
@@ -139,6 +143,7 @@ struct OzArgs {
   double *partial;
   long Nq, chunk, pstride;
   int ld, nb, nchunk, ngroups, nstage_total;
+  int screen, lgchunk;  // skip blocks bounded by 2^-screen (0: none); 2^lgchunk >= chunk
 };
 
 __device__ inline v4i mfma8(v4i a, v4i b, v4i c) {
@@ -163,13 +168,13 @@ __device__ inline unsigned pack_bytes(int b0, int b1, int b2, int b3) {
   return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
 }
 
-// B digits of the group's row functions for one stage into an LDS buffer: one (function,
-// 8 points) item per thread and pass. JB = rint(JA vs 2^-(eV + eB)) with
-// JA = 2^14 (2^14 d1 + 2^7 d2 + d3) + 2^7 d4 + d5 read from the digit planes.
-__device__ inline void stage_b_digits(const OzArgs &a, int r0, int nB, long pn, const double *scB,
-                                      char *buf, int tid) {
-  for (int it = tid; it < nB * 8; it += 64 * OZ_WAVES) {
-    const int fl = it >> 3, s8 = it & 7;
+// B digits of the group's listed row blocks (rowl[0..nrow), local indices) for one stage into
+// an LDS buffer: one (function, 8 points) item per thread and pass. JB = rint(JA vs 2^-(eV + eB))
+// with JA = 2^14 (2^14 d1 + 2^7 d2 + d3) + 2^7 d4 + d5 read from the digit planes.
+__device__ inline void stage_b_digits(const OzArgs &a, int r0, int nB, const int *rowl, int nrow, long pn,
+                                      const double *scB, char *buf, int tid) {
+  for (int it = tid; it < nrow * 128; it += 64 * OZ_WAVES) {
+    const int fl = 16 * rowl[it >> 7] + ((it >> 3) & 15), s8 = it & 7;
     const int f = 16 * r0 + fl;
     const long p = pn + 8 * s8;
     uint2 dg[OZ_DIGITS];
@@ -224,6 +229,9 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
   int *eRow = (int *)(scB + nB);                           // [nB]
   double *maxv = (double *)(eRow + nB + (nB & 1));          // [OZ_MAX_STAGES]
   int *s_eV_ptr = (int *)(maxv + OZ_MAX_STAGES);             // [1] (all LDS is dynamic)
+  int *s_nrow = s_eV_ptr + 1;                                // [1]
+  int *eBlk = s_nrow + 1;                                    // [nb]: block exponents, this chunk
+  int *rowl = eBlk + a.nb;                                   // [r1 - r0]: row blocks with work
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ns = (int)(a.chunk / OZ_STAGE);
   const long pc = (long)c * a.chunk;
@@ -244,6 +252,31 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
   }
   __syncthreads();
   const int eV = *s_eV_ptr;
+  // screening: the largest chunk exponent of each column block's functions, then the row
+  // blocks with at least one block left (every row block when screening is off)
+  auto skipped = [&](int I, int J) { return eBlk[I] + eBlk[J] + eV + a.lgchunk <= -a.screen; };
+  if (a.screen > 0) {
+    for (int b = r0 + wave; b < a.nb; b += OZ_WAVES) {
+      int e = lane < 16 ? a.eA[(size_t)(16 * b + lane) * a.nchunk + c] : OZ_ZERO_EXP;
+      for (int o = 8; o > 0; o >>= 1) e = max(e, __shfl_xor(e, o));
+      if (lane == 0) eBlk[b] = e;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int nr = 0;
+      for (int I = r0; I < r1; ++I) {
+        bool need = false;
+        for (int J = I; J < a.nb && !need; ++J) need = !skipped(I, J);
+        if (need) rowl[nr++] = I - r0;
+      }
+      *s_nrow = nr;
+    }
+  } else if (tid == 0) {
+    for (int I = r0; I < r1; ++I) rowl[I - r0] = I - r0;
+    *s_nrow = r1 - r0;
+  }
+  __syncthreads();
+  const int nrow = *s_nrow;
   for (int fl = tid; fl < nB; fl += 64 * OZ_WAVES) {
     const int f = 16 * r0 + fl;
     const unsigned char *md = a.maxd1 + (size_t)f * a.nstage_total + stage0;
@@ -254,8 +287,6 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
     scB[fl] = ldexp(1.0, -eV - eB);
     eRow[fl] = a.eA[(size_t)f * a.nchunk + c] + eV + eB;
   }
-  __syncthreads();
-  stage_b_digits(a, r0, nB, pc, scB, ldsB, tid);
   __syncthreads();
 
   v4i acc[OZ_MAXB][5];
@@ -274,6 +305,19 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
   }
 #pragma unroll
   for (int cc = 0; cc < OZ_MAXCOL; ++cc) cJ[cc] = __builtin_amdgcn_readfirstlane(task[3 * OZ_MAXB + cc]);
+  // slots whose block is skipped keep zero accumulators (their partial is written as 0), and
+  // columns that no remaining slot uses are not loaded
+  bool live[OZ_MAXB];
+#pragma unroll
+  for (int k = 0; k < OZ_MAXB; ++k)
+    live[k] = tI[k] >= 0 && !(a.screen > 0 && __builtin_amdgcn_readfirstlane((int)skipped(tI[k], tJ[k])));
+#pragma unroll
+  for (int cc = 0; cc < OZ_MAXCOL; ++cc) {
+    bool used = false;
+#pragma unroll
+    for (int k = 0; k < OZ_MAXB; ++k) used |= live[k] && tC[k] == cc;
+    if (!used) cJ[cc] = -1;
+  }
   const int r16 = lane & 15, q = lane >> 4;
   // waves 0-3 and 4-7 (one of each per SIMD) run the MFMAs of a stage and the digit formation
   // of the next in opposite order; both load their column digits before the formation, whose
@@ -294,7 +338,7 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
 #pragma unroll
     for (int k = 0; k < OZ_MAXB; ++k) {
       const int I = tI[k], ci = tC[k];
-      if (I < 0) continue;
+      if (!live[k]) continue;
       v4i ra[OZ_DIGITS];
       const int fl = 16 * (I - r0) + r16;
 #pragma unroll
@@ -307,24 +351,28 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
         block_products(ra, cf[2], acc[k]);
     }
   };
-  if (mfma_first) load_columns(pc);
-  for (int t = 0; t < ns; ++t) {
-    const long p0 = pc + (long)t * OZ_STAGE;
-    const char *bufB = ldsB + (t & 1) * bufsz;
-    char *bufN = ldsB + ((t + 1) & 1) * bufsz;
-    const bool more = t + 1 < ns;
-    if (mfma_first) {
-      mfma_phase(bufB);
-      if (more) {
-        load_columns(p0 + OZ_STAGE);
-        stage_b_digits(a, r0, nB, p0 + OZ_STAGE, scB, bufN, tid);
-      }
-    } else {
-      load_columns(p0);
-      if (more) stage_b_digits(a, r0, nB, p0 + OZ_STAGE, scB, bufN, tid);
-      mfma_phase(bufB);
-    }
+  if (nrow > 0) {  // (the same for the whole workgroup)
+    stage_b_digits(a, r0, nB, rowl, nrow, pc, scB, ldsB, tid);
     __syncthreads();
+    if (mfma_first) load_columns(pc);
+    for (int t = 0; t < ns; ++t) {
+      const long p0 = pc + (long)t * OZ_STAGE;
+      const char *bufB = ldsB + (t & 1) * bufsz;
+      char *bufN = ldsB + ((t + 1) & 1) * bufsz;
+      const bool more = t + 1 < ns;
+      if (mfma_first) {
+        mfma_phase(bufB);
+        if (more) {
+          load_columns(p0 + OZ_STAGE);
+          stage_b_digits(a, r0, nB, rowl, nrow, p0 + OZ_STAGE, scB, bufN, tid);
+        }
+      } else {
+        load_columns(p0);
+        if (more) stage_b_digits(a, r0, nB, rowl, nrow, p0 + OZ_STAGE, scB, bufN, tid);
+        mfma_phase(bufB);
+      }
+      __syncthreads();
+    }
   }
   // ---- epilogue: scaled FP64 partial of every block of this wave
   double *part = a.partial + (size_t)c * a.pstride;
@@ -347,7 +395,7 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
 
 // F[mu][nu] = F[nu][mu] = sum over chunks (in order) of the block element, mu <= nu < n.
 __global__ void k_ozaki_reduce(const double *__restrict__ partial, long pstride, int nchunk, int nb,
-                               int n, double *__restrict__ F) {
+                               int n, const int *__restrict__ perm, double *__restrict__ F) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long)n * n) return;
   const int mu = (int)(e / n), nu = (int)(e % n);
@@ -365,8 +413,9 @@ __global__ void k_ozaki_reduce(const double *__restrict__ partial, long pstride,
     for (int j = 0; j < 8; ++j) s += x[j];
   }
   for (; c < nchunk; ++c) s += partial[(long)c * pstride + off];
-  F[(long)mu * n + nu] = s;
-  F[(long)nu * n + mu] = s;
+  const int x = perm ? perm[mu] : mu, y = perm ? perm[nu] : nu;
+  F[(long)x * n + y] = s;
+  F[(long)y * n + x] = s;
 }
 
 }  // namespace
@@ -404,9 +453,9 @@ static bool pack_waves(int r0, int r1, int nb, int wave_blocks, std::vector<int>
   return it == blocks.size();
 }
 
-static size_t group_lds(int r0, int r1) {
+static size_t group_lds(int r0, int r1, int nb) {
   const size_t nB = 16 * (size_t)(r1 - r0);
-  return 2 * OZ_DIGITS * nB * 64 + nB * 12 + 8 + OZ_MAX_STAGES * 8 + 16;
+  return 2 * OZ_DIGITS * nB * 64 + nB * 12 + 8 + OZ_MAX_STAGES * 8 + 16 + 8 * (size_t)nb;
 }
 
 OzakiLayout ozaki_layout(long N, int n, int ld, int wave_blocks, size_t max_lds, long chunk_target) {
@@ -431,7 +480,7 @@ OzakiLayout ozaki_layout(long N, int n, int ld, int wave_blocks, size_t max_lds,
   while (r0 < nb) {
     int best = -1;
     for (int r1 = r0 + 1; r1 <= nb; ++r1) {
-      if (group_lds(r0, r1) > max_lds || !pack_waves(r0, r1, nb, wave_blocks, tab)) break;
+      if (group_lds(r0, r1, nb) > max_lds || !pack_waves(r0, r1, nb, wave_blocks, tab)) break;
       best = r1;
     }
     if (best < 0)
@@ -439,7 +488,7 @@ OzakiLayout ozaki_layout(long N, int n, int ld, int wave_blocks, size_t max_lds,
                   " functions does not fit the wave slots (use the FP64 mode)");
     pack_waves(r0, best, nb, wave_blocks, tab);
     tasks.insert(tasks.end(), tab.begin(), tab.end());
-    lds = std::max(lds, group_lds(r0, best));
+    lds = std::max(lds, group_lds(r0, best, nb));
     row0.push_back(best);
     r0 = best;
   }
@@ -450,13 +499,54 @@ OzakiLayout ozaki_layout(long N, int n, int ld, int wave_blocks, size_t max_lds,
   return L;
 }
 
-void launch_ozaki_setup(const OzakiLayout &L, const double *phi64, const double *wbw,
+void launch_ozaki_scales(const OzakiLayout &L, const double *phi64, const double *wbw, const int *perm,
+                         int *eA, hipStream_t s) {
+  k_oz_scales<<<L.nchunk, 256, 0, s>>>(phi64, L.ld, wbw, perm, L.N, L.chunk, L.nchunk, eA);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_ozaki_setup(const OzakiLayout &L, const double *phi64, const double *wbw, const int *perm,
                         signed char *planes, int *eA, unsigned char *maxd1, hipStream_t s) {
-  k_oz_scales<<<L.nchunk, 256, 0, s>>>(phi64, L.ld, wbw, L.N, L.chunk, L.nchunk, eA);
-  NAPMO_HIP(hipGetLastError());
+  launch_ozaki_scales(L, phi64, wbw, perm, eA, s);
   k_oz_digits<<<dim3((unsigned)(L.Nq / OZ_STAGE), L.ld / 64), 256, 0, s>>>(
-      phi64, L.ld, wbw, L.N, L.Nq, L.chunk, L.nchunk, eA, planes, maxd1);
+      phi64, L.ld, wbw, perm, L.N, L.Nq, L.chunk, L.nchunk, eA, planes, maxd1);
   NAPMO_HIP(hipGetLastError());
+}
+
+long ozaki_kept_blocks(const OzakiLayout &L, const std::vector<int> &eA, const std::vector<double> &vs,
+                       int screen) {
+  NAPMO_REQUIRE(eA.size() == (size_t)L.ld * L.nchunk && vs.size() == (size_t)L.Nq,
+                "int8 screening count: table sizes");
+  if (screen <= 0) return (long)L.nchunk * L.nblk;
+  int lg = 0;
+  while ((1L << lg) < L.chunk) ++lg;
+  long kept = 0;
+  std::vector<int> eb(L.nb);
+  for (int c = 0; c < L.nchunk; ++c) {
+    double m = 0.0;
+    for (long p = (long)c * L.chunk; p < (long)(c + 1) * L.chunk; ++p) m = std::max(m, std::fabs(vs[p]));
+    const int eV = pow2_ceil_exp(m);
+    for (int b = 0; b < L.nb; ++b) {
+      eb[b] = OZ_ZERO_EXP;
+      for (int i = 0; i < 16; ++i) eb[b] = std::max(eb[b], eA[(size_t)(16 * b + i) * L.nchunk + c]);
+    }
+    for (int I = 0; I < L.nb; ++I)
+      for (int J = I; J < L.nb; ++J) kept += eb[I] + eb[J] + eV + lg > -screen;
+  }
+  return kept;
+}
+
+std::vector<int> ozaki_compact_order(const OzakiLayout &L, const std::vector<int> &eA) {
+  NAPMO_REQUIRE(eA.size() == (size_t)L.ld * L.nchunk, "int8 screening order: exponent table size");
+  int top = OZ_ZERO_EXP;
+  for (int f = 0; f < L.n; ++f)
+    for (int c = 0; c < L.nchunk; ++c) top = std::max(top, eA[(size_t)f * L.nchunk + c]);
+  std::vector<int> reach(L.n, 0), order(L.ld);
+  for (int f = 0; f < L.n; ++f)
+    for (int c = 0; c < L.nchunk; ++c) reach[f] += eA[(size_t)f * L.nchunk + c] >= top - 30;
+  for (int f = 0; f < L.ld; ++f) order[f] = f;
+  std::stable_sort(order.begin(), order.begin() + L.n, [&](int x, int y) { return reach[x] < reach[y]; });
+  return order;
 }
 
 void launch_ozaki_vs(const double *v, const double *wbw, long N, long Nq, double *vs, hipStream_t s) {
@@ -465,7 +555,7 @@ void launch_ozaki_vs(const double *v, const double *wbw, long N, long Nq, double
 }
 
 void launch_ozaki_fock(const OzakiLayout &L, const int *d_tasks, const signed char *planes,
-                       const int *eA, const unsigned char *maxd1, const double *vs,
+                       const int *eA, const unsigned char *maxd1, const double *vs, int screen,
                        double *partial, hipStream_t s) {
   static bool attr = false;
   if (!attr) {
@@ -482,14 +572,19 @@ void launch_ozaki_fock(const OzakiLayout &L, const int *d_tasks, const signed ch
   a.Nq = L.Nq, a.chunk = L.chunk, a.pstride = L.partial_stride();
   a.ld = L.ld, a.nb = L.nb, a.nchunk = L.nchunk, a.ngroups = L.ngroups;
   a.nstage_total = (int)(L.Nq / OZ_STAGE);
+  NAPMO_REQUIRE(screen >= 0 && screen <= 1000, "int8 screening threshold out of range");
+  a.screen = screen;
+  a.lgchunk = 0;
+  while ((1L << a.lgchunk) < L.chunk) ++a.lgchunk;
   const int nc8 = (int)round_up(L.nchunk, 8);
   k_ozaki_fock<<<nc8 * L.ngroups, 64 * OZ_WAVES, L.lds_bytes, s>>>(a);
   NAPMO_HIP(hipGetLastError());
 }
 
-void launch_ozaki_reduce(const OzakiLayout &L, const double *partial, double *F, hipStream_t s) {
+void launch_ozaki_reduce(const OzakiLayout &L, const double *partial, const int *perm, double *F,
+                         hipStream_t s) {
   const long nn = (long)L.n * L.n;
-  k_ozaki_reduce<<<div_up(nn, 256), 256, 0, s>>>(partial, L.partial_stride(), L.nchunk, L.nb, L.n, F);
+  k_ozaki_reduce<<<div_up(nn, 256), 256, 0, s>>>(partial, L.partial_stride(), L.nchunk, L.nb, L.n, perm, F);
   NAPMO_HIP(hipGetLastError());
 }
 

@@ -14,7 +14,17 @@
 // both operands and the dropped products of level >= 7 (below 2^-35 of the leading one); the
 // energy of the returned F stays within 1e-6 Eh of FP64 on molecules whose near-nucleus shells
 // make 28-bit digits miss it by up to 7e-5 Eh.
+//
+// Screening (optional, threshold 2^-S): Gaussian basis functions are local, so in most chunks of
+// points most 16 x 16 blocks of F get nothing. With every |At| of a block's functions in a chunk
+// below 2^eI (the largest of their chunk exponents), |vs| below 2^eV and at most 2^lg points in
+// the chunk, every element of block (I, J)'s chunk sum is below 2^(eI + eJ + eV + lg); blocks
+// with eI + eJ + eV + lg <= -S are skipped in that chunk, so each element of F moves by at most
+// nchunk 2^-S. The functions are then ordered by how many chunks they reach (the most compact
+// first), so the blocks of compact functions are empty far from their centres.
 #pragma once
+
+#include <vector>
 
 #include "common.hpp"
 
@@ -50,19 +60,35 @@ struct OzakiLayout {
 // no grouping fits the LDS or the wave slots.
 OzakiLayout ozaki_layout(long N, int n, int ld, int wave_blocks, size_t max_lds, long chunk_target = 0);
 
+constexpr int OZ_ZERO_EXP = -2000;  // chunk exponent of a function that is zero in the chunk
+
 // Per-geometry operand: digits of At = sqrt|wbw| Phi. phi64 [Npad x ld] point-major (Npad >= N),
-// wbw [N]; out: planes [OZ_DIGITS][ld][Nq] int8 function-major, eA [ld][nchunk] int, maxd1
-// [ld][Nq / OZ_STAGE] uint8 (largest |d1| per stage).
-void launch_ozaki_setup(const OzakiLayout &L, const double *phi64, const double *wbw,
+// wbw [N]; perm [ld] (NULL: identity): operand function f is column perm[f] of phi64; out:
+// planes [OZ_DIGITS][ld][Nq] int8 function-major, eA [ld][nchunk] int (OZ_ZERO_EXP where the
+// function is zero in the chunk), maxd1 [ld][Nq / OZ_STAGE] uint8 (largest |d1| per stage).
+void launch_ozaki_scales(const OzakiLayout &L, const double *phi64, const double *wbw, const int *perm,
+                         int *eA, hipStream_t s);
+void launch_ozaki_setup(const OzakiLayout &L, const double *phi64, const double *wbw, const int *perm,
                         signed char *planes, int *eA, unsigned char *maxd1, hipStream_t s);
+// The screening order: functions by the number of chunks where their chunk exponent reaches
+// the largest one less 30 (fewest first, ties by index), from eA [ld][nchunk] of the
+// identity order; functions n..ld-1 stay last.
+std::vector<int> ozaki_compact_order(const OzakiLayout &L, const std::vector<int> &eA);
+// Upper 16 x 16 blocks x chunks that launch_ozaki_fock computes at threshold 2^-screen for the
+// potential vs [Nq] (all of them when screen == 0), from eA [ld][nchunk] in operand order.
+long ozaki_kept_blocks(const OzakiLayout &L, const std::vector<int> &eA, const std::vector<double> &vs,
+                       int screen);
 // vs [Nq] = sign(wbw) v for p < N, 0 beyond.
 void launch_ozaki_vs(const double *v, const double *wbw, long N, long Nq, double *vs, hipStream_t s);
 // Per build: partial [nchunk][partial_stride] FP64 (the 16 x 16 values of every upper block,
-// row-major, blocks in upper-triangle row order); d_tasks = L.tasks followed by L.row0.
+// row-major, blocks in upper-triangle row order); d_tasks = L.tasks followed by L.row0;
+// screen = S > 0 skips the blocks bounded by 2^-S (0: none).
 void launch_ozaki_fock(const OzakiLayout &L, const int *d_tasks, const signed char *planes,
-                       const int *eA, const unsigned char *maxd1, const double *vs,
+                       const int *eA, const unsigned char *maxd1, const double *vs, int screen,
                        double *partial, hipStream_t s);
-// F [n x n] (symmetric) = the fixed-order sum over chunks of the partials.
-void launch_ozaki_reduce(const OzakiLayout &L, const double *partial, double *F, hipStream_t s);
+// F [n x n] (symmetric) = the fixed-order sum over chunks of the partials; element (mu, nu) of
+// the operand order is F[perm[mu]][perm[nu]] (perm NULL: identity).
+void launch_ozaki_reduce(const OzakiLayout &L, const double *partial, const int *perm, double *F,
+                         hipStream_t s);
 
 }  // namespace napmo

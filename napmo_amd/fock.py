@@ -14,6 +14,8 @@ import numpy as np
 
 from napmo_amd.system import cext
 
+SCREEN_KEY = 11   # NAPMO_TUNE_INT8_SCREEN
+
 PRECISIONS = {'fp64': 0, 'fp32': 1, 'bf16x3': 2, 'int8': 3}
 
 
@@ -125,6 +127,17 @@ class GridFock(object):
 
     def gradient_device(self, dP, dG):
         cext.check(cext.napmo_grid_fock_gradient_device(self._this, dP, dG))
+
+    def set_screen(self, S):
+        """int8 plans: skip, per chunk of points, the 16 x 16 blocks of F bounded by 2^-S in
+        every element (0: off), so each element moves by at most (chunks) 2^-S."""
+        cext.check(cext.napmo_fock_plan_set_tuning(self._this, SCREEN_KEY, int(S)))
+
+    def screen_stats(self):
+        """int8 plans: (blocks x chunks computed by the last contraction, all of them)."""
+        kept, total = ctypes.c_long(), ctypes.c_long()
+        cext.check(cext.napmo_fock_plan_int8_screen_stats(self._this, ctypes.byref(kept), ctypes.byref(total)))
+        return kept.value, total.value
 
     def profile(self, max_builds):
         cext.check(cext.napmo_fock_plan_profile(self._this, max_builds))

@@ -208,6 +208,7 @@ _SIGNATURES = {
     'napmo_grid_poisson_adjoint': (c_int, [vp, vp, vp]),
     'napmo_fock_plan_set_tuning': (c_int, [vp, c_int, c_long]),
     'napmo_fock_plan_get_tuning': (c_int, [vp, c_int, ctypes.POINTER(c_long)]),
+    'napmo_fock_plan_int8_screen_stats': (c_int, [vp, ctypes.POINTER(c_long), ctypes.POINTER(c_long)]),
     'napmo_grid_set_tuning': (c_int, [vp, c_int, c_long]),
     'napmo_grid_get_tuning': (c_int, [vp, c_int, ctypes.POINTER(c_long)]),
     'napmo_batch_plan_new': (c_int, [c_int, vp, c_int, ctypes.POINTER(vp)]),

@@ -1370,7 +1370,25 @@ def _split5(J):
     return [d1, d2, d3, d4, d5]
 
 
-def int8_fock(phi, wbw, v, chunk, stage=64):
+OZ_ZERO_EXP = -2000
+
+
+def int8_compact_order(phi, wbw, chunk):
+    """The screening order of the int8 contraction (napmo_amd/csrc/ozaki.hip,
+    ozaki_compact_order): functions by the number of chunks in which the exponent of their
+    chunk maximum of |sqrt|wbw| phi| reaches the largest such exponent less 30, fewest first,
+    ties by index."""
+    N, n = phi.shape
+    nchunk = -(-N // chunk)
+    G = np.zeros((nchunk * chunk, n))
+    G[:N] = np.sqrt(np.abs(wbw))[:, None] * phi
+    m = np.abs(G).reshape(nchunk, chunk, n).max(axis=1)            # [nchunk, n]
+    e = np.where(m > 0.0, _pow2_ceil_exp(m), OZ_ZERO_EXP)
+    reach = (e >= e.max() - 30).sum(axis=0)
+    return np.argsort(reach, kind='stable')
+
+
+def int8_fock(phi, wbw, v, chunk, stage=64, screen=0, order=None, stats=None):
     """F = Phi^T diag(wbw v) Phi as the int8 MFMA kernel computes it (napmo_amd/csrc/ozaki.hip).
 
     At = sqrt|wbw| Phi is written per (function, chunk) as a 35-bit integer at the chunk's
@@ -1381,8 +1399,20 @@ def int8_fock(phi, wbw, v, chunk, stage=64):
     exactly over the chunk, one integer S_L per level (|S_L| < 2^27 for up to 63 stages), and the
     chunk's block is 2^(e_row + e_col) ((((S2 2^-12 + S3 2^-19) + S4 2^-26) + S5 2^-33) + S6 2^-40);
     chunks are summed in order. Products of level >= 7 (below 2^-35 of the leading one) are the
-    only ones dropped. phi [N, n]."""
+    only ones dropped. phi [N, n].
+
+    Screening (screen = S > 0, with ``order`` the operand order of the functions): in each
+    chunk the 16 x 16 block (I, J), I <= J in that order, is left out when
+    eI + eJ + eV + lg <= -S, eI the largest chunk exponent eA of the block's functions (functions
+    that are zero in the chunk count as OZ_ZERO_EXP) and 2^lg >= chunk; every element of the
+    left-out chunk sum is below 2^-S. ``stats`` (a dict) receives the counts of blocks kept and
+    left out."""
+    if order is not None:
+        order = np.asarray(order)
+        phi = phi[:, order]
     N, n = phi.shape
+    nb = -(-n // 16)
+    lg = int(np.ceil(np.log2(chunk)))
     nchunk = -(-N // chunk)
     Nq = nchunk * chunk
     G = np.zeros((Nq, n))
@@ -1419,6 +1449,23 @@ def int8_fock(phi, wbw, v, chunk, stage=64):
                 T = sum(r[i].T @ a[L - 2 - i] for i in range(L - 1))
                 S[L] += np.rint(T).astype(np.int64)       # exact integers
         val = (((S[2] * 2.0 ** -12 + S[3] * 2.0 ** -19) + S[4] * 2.0 ** -26) + S[5] * 2.0 ** -33) + S[6] * 2.0 ** -40
-        F = F + np.ldexp(val, (eA + eV + eB)[:, None] + eA[None, :])
+        blockF = np.ldexp(val, (eA + eV + eB)[:, None] + eA[None, :])
+        if screen > 0:
+            ez = np.full(16 * nb, OZ_ZERO_EXP)
+            ez[:n] = np.where(np.abs(Gc).max(axis=0) > 0.0, eA, OZ_ZERO_EXP)
+            eblk = ez.reshape(nb, 16).max(axis=1)
+            keep = np.repeat(np.repeat(eblk[:, None] + eblk[None, :] + eV + lg > -screen, 16, 0), 16, 1)[:n, :n]
+            if stats is not None:
+                upper = np.triu(np.ones((nb, nb), dtype=bool))
+                kb = (eblk[:, None] + eblk[None, :] + eV + lg > -screen) & upper
+                stats['kept'] = stats.get('kept', 0) + int(kb.sum())
+                stats['skipped'] = stats.get('skipped', 0) + int((upper & ~kb).sum())
+            blockF = np.where(keep, blockF, 0.0)
+        F = F + blockF
     U = np.triu(F)
-    return U + np.triu(U, 1).T
+    F = U + np.triu(U, 1).T
+    if order is not None:
+        out = np.empty_like(F)
+        out[np.ix_(order, order)] = F
+        F = out
+    return F

@@ -4,7 +4,10 @@
     stages, row groups and a grid with negative Lebedev weights;
   * its accuracy against the FP64 build: |dE_trace| = |1/2 tr(P (F_int8 - F_fp64))| <= 1e-6 Eh
     on the 16-atom 472,000-point system (the stated mixed-precision tolerance), and far below
-    it on the small systems."""
+    it on the small systems;
+  * screening (NAPMO_TUNE_INT8_SCREEN): bit for bit against the oracle's restatement with the
+    same threshold and function order, and on the 16-atom system within the stated element
+    bound of the unscreened contraction and the 1e-6 Eh tolerance of the FP64 build."""
 import numpy as np
 import pytest
 
@@ -14,6 +17,12 @@ from tests import systems
 pytestmark = pytest.mark.gpu
 
 TUNE_INT8_CHUNK = 7
+TUNE_INT8_SCREEN = 11
+
+
+def _set_screen(fock, S):
+    from napmo_amd.system import cext
+    cext.check(cext.napmo_fock_plan_set_tuning(fock._this, TUNE_INT8_SCREEN, S))
 
 
 def _plan_chunk(fock, chunk):
@@ -98,3 +107,62 @@ def test_int8_build_meets_the_stated_tolerance_on_the_16_atom_system():
     assert abs(E3 - E1) <= 1e-6
     assert np.abs(F2 - F1).max() <= 1e-7 * np.abs(F1).max()
     assert np.array_equal(F2, F2.T)
+
+
+def test_int8_screening_is_bit_exact_with_the_oracle_restatement():
+    # five atoms (70 functions, 5 blocks, 42 chunks of 320 points): at 2^-30 and 2^-50 the
+    # oracle leaves out 147 and 120 of the 630 chunk blocks; 2^-1000 leaves out none but the
+    # function order changes
+    import napmo_amd
+    rng = np.random.default_rng(71)
+    atoms = systems.random_molecule(rng, 5, 5.0, min_sep=1.2)
+    shells = systems.random_shells(rng, 5, 7)
+    g, b = systems.gpu_system(atoms, shells, 24, 110)
+    fock = napmo_amd.GridFock(g, b, 8, 'int8')
+    chunk = _plan_chunk(fock, 320)
+    phi = b.compute(g.points)
+    wbw = g.weights * g.becke_weights
+    v = 1.0 / (0.5 + ((g.points - g.origin[0]) ** 2).sum(1)) - 0.3 * np.exp(-((g.points - g.origin[1]) ** 2).sum(1))
+    F_plain = fock.contract(v)
+    order = core.int8_compact_order(phi, wbw, chunk)
+    nchunk = -(-g.size // chunk)
+    for S in (1000, 50, 30):
+        _set_screen(fock, S)
+        F = fock.contract(v)
+        st = {}
+        Fo = core.int8_fock(phi, wbw, v, chunk, screen=S, order=order, stats=st)
+        d = np.abs(F - F_plain).max()
+        print('screen 2^-%d: %d of %d chunk blocks left out; max|F - oracle| %.3e, max|F - unscreened| %.3e' % (
+            S, st['skipped'], st['kept'] + st['skipped'], np.abs(F - Fo).max(), d))
+        assert np.array_equal(F, Fo)
+        assert d <= nchunk * 2.0 ** -min(S, 1000) + 1e-9 * np.abs(F_plain).max()
+    _set_screen(fock, 0)
+    assert np.array_equal(fock.contract(v), F_plain)
+
+
+def test_int8_screening_on_the_16_atom_system():
+    # 16 atoms, 472,000 points, 300 functions: the screened build (2^-50 per element and chunk)
+    # within the element bound of the unscreened one and within 1e-6 Eh of the FP64 build
+    import napmo_amd
+    atoms, shells, nr, na, lmax = systems.chon16_system()
+    g, b = systems.gpu_system(atoms, shells, nr, na)
+    f64 = napmo_amd.GridFock(g, b, lmax, 'fp64')
+    C = systems.occupied_orbitals(np.random.default_rng(2), f64.overlap(), 40)
+    F1, E1, Eg1 = f64.build_orbitals(C, 2.0)
+    del f64
+    i8 = napmo_amd.GridFock(g, b, lmax, 'int8')
+    F2, E2, Eg2 = i8.build_orbitals(C, 2.0)
+    _set_screen(i8, 50)
+    F3, E3, Eg3 = i8.build_orbitals(C, 2.0)
+    import ctypes
+    from napmo_amd.system import cext
+    chunk = ctypes.c_long()
+    cext.check(cext.napmo_fock_plan_get_tuning(i8._this, TUNE_INT8_CHUNK, ctypes.byref(chunk)))
+    nchunk = -(-g.size // chunk.value)
+    print('chon16 int8 screened 2^-50: dE_trace %.3e (unscreened %.3e), max|F - unscreened| %.3e, '
+          'max|F - fp64| %.3e' % (E3 - E1, E2 - E1, np.abs(F3 - F2).max(), np.abs(F3 - F1).max()))
+    assert Eg3 == Eg2
+    assert abs(E3 - E1) <= 1e-6
+    assert np.abs(F3 - F2).max() <= nchunk * 2.0 ** -50 + 1e-9 * np.abs(F2).max()
+    assert np.abs(F3 - F1).max() <= 1e-7 * np.abs(F1).max()
+    assert np.array_equal(F3, F3.T)
