@@ -578,7 +578,7 @@ def batch_leg(args, log):
 
 PHASE_KERNELS = {'density': 'k_density_mfma',
                  'fock': {'bf16x3': 'k_fock_bf16x3_pass', 'fp64': 'k_fock_f64', 'int8': 'k_ozaki_fock',
-                          'int8s': 'k_ozaki_fock [screened 2^-%d]' % SCREEN_S},
+                          'int8s': 'k_ozaki_fock'},
                  'synth': 'k_synth2n<', 'project': 'k_project', 'radial': 'k_radial',
                  'fock_reduce': 'k_fock_reduce'}
 
@@ -604,8 +604,10 @@ def table_traffic(kern, want):
 
 def measured_traffic(res, phase):
     """HBM bytes per launch (read + write) of the phase's kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_chon16.json), or None."""
-    kern = traffic_table('traffic_chon16.json')
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_chon16.json; the screened int8
+    mode's contraction from profiles/traffic_chon16_int8_screened.json), or None."""
+    kern = traffic_table('traffic_chon16_int8_screened.json' if res['prec'] == 'int8s' and phase == 'fock'
+                         else 'traffic_chon16.json')
     if not kern:
         return None
     want = PHASE_KERNELS.get(phase)
