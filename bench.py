@@ -164,8 +164,11 @@ def apmo_leg(args, log):
     dF = [DeviceArray(P.shape) for P in Ps]
     dE = DeviceArray((8,))
     out = {}
-    for prec in ('fp64', 'fp64-one', 'int8', 'bf16x3'):
-        plan = napmo_amd.APMOFock(species, prec.split('-')[0], one_launch=prec.endswith('-one'))
+    for prec in ('fp64', 'fp64-one', 'int8', 'int8s', 'bf16x3'):
+        base = prec.split('-')[0]
+        plan = napmo_amd.APMOFock(species, PLAN_PREC.get(base, base), one_launch=prec.endswith('-one'))
+        if prec == 'int8s':
+            plan.set_screen(SCREEN_S)
         step = lambda: plan.build_device([a.ptr for a in dP], [a.ptr for a in dF], dE.ptr)  # noqa: E731
         for _ in range(args.warmup):
             step()
@@ -185,7 +188,7 @@ def apmo_leg(args, log):
     log('apmo fp64 one launch per phase vs per species: max|dF| %.3e, max dE %.3e' %
         (d1, max(abs(one['E'][s][k] - ref['E'][s][k]) for s in range(2) for k in range(4))))
     best = 'fp64' if ref['builds_per_s'] >= one['builds_per_s'] else 'fp64-one'
-    for prec in ('int8', 'bf16x3'):
+    for prec in ('int8', 'int8s', 'bf16x3'):
         mix = out[prec]
         mix['dE_grid'] = max(abs(mix['E'][s][1] - ref['E'][s][1]) for s in range(2))
         mix['dE_trace'] = max(abs(mix['E'][s][0] - ref['E'][s][0]) for s in range(2))
@@ -200,11 +203,14 @@ def apmo_leg(args, log):
         'metric': METRIC, 'value': out[best]['builds_per_s'], 'unit': 'builds/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': out[best]['ms_per_build'],
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': {'fp64': 'f64', 'fp64-one': 'f64', 'int8': 'int8+f64', 'bf16x3': 'bf16x3+f64'}[best],
+        'dtype': {'fp64': 'f64', 'fp64-one': 'f64', 'int8': 'int8+f64', 'int8s': 'int8+f64',
+                  'bf16x3': 'bf16x3+f64'}[best],
         'data': 'synthetic (seeded geometry, basis and densities)',
         'config': {'workload': 'multi-species Coulomb build: electrons on 8 x 100 x 302 points (120 '
                                'functions) + 4 quantum protons on 4 x 60 x 194 points (320 functions), '
-                               'cross-species Poisson coupling', 'precision': best},
+                               'cross-species Poisson coupling', 'precision': best,
+                   'screening': ('F blocks bounded by 2^-%d per element and chunk of points skipped'
+                                 % SCREEN_S if best == 'int8s' else 'none')},
         'modes': {p: {'builds_per_s': r['builds_per_s'], 'ms_per_build': r['ms_per_build'], 'E': r['E'],
                       'dE_trace_vs_fp64': r.get('dE_trace', 0.0), 'dF_max_vs_fp64': r.get('dF', 0.0)}
                   for p, r in out.items()},
@@ -479,12 +485,15 @@ def batch_leg(args, log):
                                                            time.perf_counter() - t0))
     dP, dC, dF, dE, Cs = [], [], [], [], []
     out = {}
-    for mode in ('fp64', 'fp64-group', 'int8', 'fp64-orb', 'int8-orb'):
+    for mode in ('fp64', 'fp64-group', 'int8', 'fp64-orb', 'int8-orb', 'int8s-orb'):
         prec = mode.split('-')[0]
         lanes = 0 if mode.endswith('-group') else args.lanes
         orb = mode.endswith('-orb')
         t0 = time.perf_counter()
-        plans = [napmo_amd.GridFock(g, b, m['lmax'], prec) for g, b, m in zip(grids, bases, mols)]
+        plans = [napmo_amd.GridFock(g, b, m['lmax'], PLAN_PREC.get(prec, prec)) for g, b, m in zip(grids, bases, mols)]
+        if prec == 'int8s':
+            for f in plans:
+                f.set_screen(SCREEN_S)
         if not Ps:
             rng = np.random.default_rng(5)
             for m, f in zip(mols, plans):
@@ -499,7 +508,8 @@ def batch_leg(args, log):
         sources = {}
         for f, a, c_, b_, e_, k in zip(plans, dP, dC, dF, dE, nocc):
             if orb:
-                _, src = autotune.tune(f, a.ptr, b_.ptr, e_.ptr, tag='orb',
+                _, src = autotune.tune(f, a.ptr, b_.ptr, e_.ptr,
+                                       tag='orb' if prec != 'int8s' else 'orb-s%d' % SCREEN_S,
                                        build=lambda f=f, c_=c_, b_=b_, e_=e_, k=k:
                                        f.build_orbitals_device(c_.ptr, k, 2.0, b_.ptr, e_.ptr))
             else:
@@ -547,7 +557,7 @@ def batch_leg(args, log):
         raise RuntimeError('one-launch-per-phase batch differs from the per-plan builds')
     ref = out['fp64']
     gate = {}
-    for mode in ('int8', 'fp64-orb', 'int8-orb'):
+    for mode in ('int8', 'fp64-orb', 'int8-orb', 'int8s-orb'):
         r = out[mode]
         r['dEg'] = np.abs(r['E'][:, 1] - ref['E'][:, 1])
         r['dEt'] = np.abs(r['E'][:, 0] - ref['E'][:, 0])
@@ -567,7 +577,9 @@ def batch_leg(args, log):
                                '(%d points), 20-200 functions, HIP-graph replay (%s)'
                                % (sum(g.size for g in grids),
                                   'one launch per phase' if best.endswith('-group')
-                                  else '%d streams' % args.lanes), 'precision': best},
+                                  else '%d streams' % args.lanes), 'precision': best,
+                   'screening': ('F blocks bounded by 2^-%d per element and chunk of points skipped'
+                                 % SCREEN_S if best.startswith('int8s') else 'none')},
         'modes': {p: {'ms_per_batch_eager': r['ms_eager'], 'ms_per_batch_graph': r['ms_graph']}
                   for p, r in out.items()},
         'per_molecule_vs_fp64': {m: {'dE_grid': out[m]['dEg'].tolist(), 'dE_trace': out[m]['dEt'].tolist(),

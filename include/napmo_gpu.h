@@ -522,6 +522,9 @@ int napmo_apmo_sync(void *plan);
  * reduction and trace kernels. Equal to the per-species build to rounding. on = 0: back
  * to one stream per species. */
 int napmo_apmo_plan_set_one_launch(void *plan, int on);
+/* napmo_fock_plan_set_tuning on species s's Fock plan (no reference counterpart), e.g.
+ * NAPMO_TUNE_INT8_SCREEN; not in one-launch mode. */
+int napmo_apmo_plan_set_tuning(void *plan, int species, int key, long value);
 /* One-launch mode only: record one build on fixed device pointers into a HIP graph;
  * napmo_apmo_replay re-runs it (write the next P into the same buffers between replays);
  * napmo_apmo_sync waits. */

@@ -42,6 +42,16 @@ class APMOFock(object):
         """Every phase of a build as one launch for all species (FP64 plans)."""
         cext.check(cext.napmo_apmo_plan_set_one_launch(self._this, int(bool(on))))
 
+    def set_tuning(self, species, key, value):
+        """A ``NAPMO_TUNE_*`` key of one species' Fock plan (not in one-launch mode)."""
+        cext.check(cext.napmo_apmo_plan_set_tuning(self._this, int(species), int(key), int(value)))
+
+    def set_screen(self, S):
+        """int8 plans: the block screening at 2^-S of every species' contraction (0: off)."""
+        from napmo_amd.fock import SCREEN_KEY
+        for s in range(len(self.species)):
+            self.set_tuning(s, SCREEN_KEY, S)
+
     def __del__(self):
         if getattr(self, '_this', None):
             cext.napmo_apmo_plan_del(self._this)

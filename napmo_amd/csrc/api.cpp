@@ -1108,6 +1108,15 @@ int napmo_apmo_sync(void *plan) {
 int napmo_apmo_plan_set_one_launch(void *plan, int on) {
   return guarded([&] { ((ApmoPlan *)plan)->set_one_launch(on != 0); });
 }
+int napmo_apmo_plan_set_tuning(void *plan, int species, int key, long value) {
+  return guarded([&] {
+    ApmoPlan *a = (ApmoPlan *)plan;
+    NAPMO_REQUIRE(species >= 0 && species < a->nspecies(), "apmo tuning: species index out of range");
+    NAPMO_REQUIRE(!a->group, "apmo tuning: turn the one-launch mode off first");
+    a->synchronize();
+    a->sp[species].plan->set_tuning(key, value);
+  });
+}
 int napmo_apmo_capture(void *plan, const double **dP, double **dF, double *dE) {
   return guarded([&] { ((ApmoPlan *)plan)->capture(dP, dF, dE); });
 }

@@ -225,6 +225,7 @@ _SIGNATURES = {
     'napmo_apmo_build_device': (c_int, [vp, vp, vp, vp]),
     'napmo_apmo_sync': (c_int, [vp]),
     'napmo_apmo_plan_set_one_launch': (c_int, [vp, c_int]),
+    'napmo_apmo_plan_set_tuning': (c_int, [vp, c_int, c_int, c_long]),
     'napmo_apmo_capture': (c_int, [vp, vp, vp, vp]),
     'napmo_apmo_replay': (c_int, [vp]),
     'napmo_grid_fock_build_device': (c_int, [vp, vp, vp, vp]),

@@ -124,16 +124,25 @@ def test_apmo_full_size_properties():
     assert rel < 1e-2
     # mixed modes against FP64, per species, on the energy of the F each mode returns
     # (E[s, 0] = 1/2 tr(P F)); E[s, 1] comes from the FP64 rho and v in every mode
-    for prec in ('int8', 'bf16x3'):
-        mix = napmo_amd.APMOFock(species, prec)
+    F8 = None
+    for prec in ('int8', 'int8s', 'bf16x3'):
+        mix = napmo_amd.APMOFock(species, prec[:4] if prec == 'int8s' else prec)
+        if prec == 'int8s':
+            mix.set_screen(50)    # blocks bounded by 2^-50 per element and chunk skipped
         Fm, Em = mix.build([Pe, Pn])
+        if prec == 'int8':
+            F8 = Fm
         for s in range(2):
             dF = np.max(np.abs(Fm[s] - Fs[s]))
             dEt = Em[s, 0] - E[s, 0]
             print('apmo %s species %d max|dF| %.3e dE_grid %.3e dE_trace %.3e' %
                   (prec, s, dF, Em[s, 1] - E[s, 1], dEt))
             assert abs(Em[s, 1] - E[s, 1]) < 1e-9 * abs(E[s, 1])
-            if prec == 'int8':
+            if prec == 'int8s':
+                # within the stated bound (chunks x 2^-50 per element) of the unscreened
+                # contraction, up to the rounding of the reordered operands
+                assert np.max(np.abs(Fm[s] - F8[s])) <= 1e-9 * np.abs(F8[s]).max()
+            if prec.startswith('int8'):
                 # exact int32 accumulation of 28-bit digit slices: within 1e-6 Eh
                 assert abs(dEt) <= 1e-6
                 assert dF < 1e-7 * np.abs(Fs[s]).max()

@@ -1,7 +1,8 @@
 """The 64-molecule batch at its full size (64 seeded molecules, 2-12 atoms, uneven 35-110 x
 110-590 grids, 20-200 functions): batch = single builds bit for bit, HIP-graph replays =
 eager batch, FP64 parity with the oracle on the smallest molecules, and the int8 batch's
-per-molecule energy error against the FP64 build within 1e-6 Eh."""
+per-molecule energy error against the FP64 build within 1e-6 Eh (also with the int8
+contraction's block screening at 2^-50)."""
 import numpy as np
 import pytest
 
@@ -150,21 +151,29 @@ def test_batch64_one_launch_per_phase(batch64, fp64_builds):
     del batch, plans
 
 
-def test_batch64_orbital_form(batch64):
+def test_batch64_orbital_form(batch64, fp64_builds):
     # the orbital form over 4 streams (P = 2 C C^T from the same seeded orbitals): the batch
     # equals the plans' own orbital-form builds bit for bit, eagerly and from three graph
-    # replays, for FP64 and int8
+    # replays, for FP64, int8 and int8 with block screening at 2^-50 (every molecule's
+    # 1/2 tr(P F) within 1e-6 Eh of the FP64 P-form build)
     import napmo_amd
     from napmo_amd import workloads
     from napmo_amd.device import DeviceArray
     from napmo_amd.system import cext
     mols, grids, bases, Ps = batch64
-    for prec in ('fp64', 'int8'):
-        plans = [napmo_amd.GridFock(g, b, m['lmax'], prec) for g, b, m in zip(grids, bases, mols)]
+    for prec in ('fp64', 'int8', 'int8s'):
+        plans = [napmo_amd.GridFock(g, b, m['lmax'], prec[:4] if prec == 'int8s' else prec)
+                 for g, b, m in zip(grids, bases, mols)]
+        if prec == 'int8s':
+            for f in plans:
+                f.set_screen(50)
         rng = np.random.default_rng(5)
         Cs = [workloads.occupied_orbitals(rng, f.overlap(), m['nocc']) for f, m in zip(plans, mols)]
         assert all(np.allclose(2.0 * C @ C.T, P, atol=1e-12) for C, P in zip(Cs, Ps))
         single = [f.build_orbitals(C) for f, C in zip(plans, Cs)]
+        dEt = np.array([abs(x[1] - r[1]) for x, r in zip(single, fp64_builds)])
+        print('%s orbital form: max |dE_trace| vs fp64 %.3e Eh' % (prec, dEt.max()))
+        assert dEt.max() <= GATE_EH
         dC = [DeviceArray.from_host(C) for C in Cs]
         dF = [DeviceArray(P.shape) for P in Ps]
         dE = [DeviceArray((2,)) for _ in Ps]
