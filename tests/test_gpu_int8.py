@@ -135,9 +135,12 @@ def test_int8_screening_is_bit_exact_with_the_oracle_restatement():
         print('screen 2^-%d: %d of %d chunk blocks left out; max|F - oracle| %.3e, max|F - unscreened| %.3e' % (
             S, st['skipped'], st['kept'] + st['skipped'], np.abs(F - Fo).max(), d))
         assert np.array_equal(F, Fo)
+        assert fock.screen_stats() == (st['kept'], st['kept'] + st['skipped'])
         assert d <= nchunk * 2.0 ** -min(S, 1000) + 1e-9 * np.abs(F_plain).max()
     _set_screen(fock, 0)
     assert np.array_equal(fock.contract(v), F_plain)
+    kept, total = fock.screen_stats()
+    assert kept == total
 
 
 def test_int8_screening_on_the_16_atom_system():
