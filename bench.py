@@ -37,9 +37,13 @@ GATE_EH = 1e-6   # |dE_trace| of a mixed mode against the FP64 mode, in Hartree
 METRIC = 'grid Fock builds/s + % roofline, mixed vs FP64 on 1 MI355X at |dE|<=1uEh'
 
 
-# int8s: the int8 plan with screening (NAPMO_TUNE_INT8_SCREEN): in each chunk of points the
-# 16 x 16 blocks of F bounded by 2^-SCREEN_S in every element are skipped
+# int8s: the int8 plan with screening: in each chunk of points the 16 x 16 blocks of F bounded
+# by 2^-SCREEN_S in every element are skipped (NAPMO_TUNE_INT8_SCREEN), and in the orbital-form
+# density each 16-point tile's 16-function groups whose terms of X = Phi C are bounded by
+# 2^-SCREEN_S (NAPMO_TUNE_DENSITY_SCREEN)
 SCREEN_S = 50
+SCREENING = ('F blocks bounded by 2^-%d per element and chunk of points skipped; density terms of '
+             'X = Phi C bounded by 2^-%d per 16-point tile and 16-function group skipped')
 PLAN_PREC = {'int8s': 'int8'}
 MODES = {  # name: (plan, density input)
     'fp64': ('fp64', 'P'),
@@ -73,6 +77,7 @@ def gpu_leg(args, log):
                 plans[prec] = napmo_amd.GridFock(grid, basis, lmax, PLAN_PREC.get(prec, prec))
                 if prec == 'int8s':
                     plans[prec].set_screen(SCREEN_S)
+                    plans[prec].set_density_screen(SCREEN_S)
             except cext.NapmoError as e:
                 log('precision %s unavailable: %s' % (prec, e))
                 continue
@@ -209,8 +214,8 @@ def apmo_leg(args, log):
         'config': {'workload': 'multi-species Coulomb build: electrons on 8 x 100 x 302 points (120 '
                                'functions) + 4 quantum protons on 4 x 60 x 194 points (320 functions), '
                                'cross-species Poisson coupling', 'precision': best,
-                   'screening': ('F blocks bounded by 2^-%d per element and chunk of points skipped'
-                                 % SCREEN_S if best == 'int8s' else 'none')},
+                   'screening': ('F blocks bounded by 2^-%d per element and chunk of points skipped (the '
+                                 'P-form density is not screened)' % SCREEN_S if best == 'int8s' else 'none')},
         'modes': {p: {'builds_per_s': r['builds_per_s'], 'ms_per_build': r['ms_per_build'], 'E': r['E'],
                       'dE_trace_vs_fp64': r.get('dE_trace', 0.0), 'dF_max_vs_fp64': r.get('dF', 0.0)}
                   for p, r in out.items()},
@@ -494,6 +499,7 @@ def batch_leg(args, log):
         if prec == 'int8s':
             for f in plans:
                 f.set_screen(SCREEN_S)
+                f.set_density_screen(SCREEN_S)
         if not Ps:
             rng = np.random.default_rng(5)
             for m, f in zip(mols, plans):
@@ -578,8 +584,7 @@ def batch_leg(args, log):
                                % (sum(g.size for g in grids),
                                   'one launch per phase' if best.endswith('-group')
                                   else '%d streams' % args.lanes), 'precision': best,
-                   'screening': ('F blocks bounded by 2^-%d per element and chunk of points skipped'
-                                 % SCREEN_S if best.startswith('int8s') else 'none')},
+                   'screening': (SCREENING % (SCREEN_S, SCREEN_S) if best.startswith('int8s') else 'none')},
         'modes': {p: {'ms_per_batch_eager': r['ms_eager'], 'ms_per_batch_graph': r['ms_graph']}
                   for p, r in out.items()},
         'per_molecule_vs_fp64': {m: {'dE_grid': out[m]['dEg'].tolist(), 'dE_trace': out[m]['dEt'].tolist(),
@@ -866,8 +871,7 @@ def main():
         'config': {'workload': 'grid Fock build, 16 atoms, 50x590 Becke grid (472,000 points), '
                                '300 s/p/d Gaussians, lmax 12, P resident in HBM',
                    'npoint': N, 'nbasis': n, 'ncenter': M, 'lmax': lmax, 'precision': best,
-                   'screening': ('F blocks bounded by 2^-%d per element and chunk of points skipped'
-                                 % SCREEN_S if head['prec'] == 'int8s' else 'none')},
+                   'screening': (SCREENING % (SCREEN_S, SCREEN_S) if head['prec'] == 'int8s' else 'none')},
         'roofline': dict(roof, kernel=kname),
         'build_roofline': build_roofline(head, N, n, L, M),
         'cpu_baseline': cpu,

@@ -455,11 +455,18 @@ enum {
                                      recollocated in chunks of about k points (rounded up to
                                      a multiple of the contraction chunks), neither the stored
                                      Phi nor a full Phi P read: the same results bit for bit */
-  NAPMO_TUNE_INT8_SCREEN = 11 /* int8 mode: S > 0 skips, per chunk of points, the 16 x 16 blocks
+  NAPMO_TUNE_INT8_SCREEN = 11, /* int8 mode: S > 0 skips, per chunk of points, the 16 x 16 blocks
                                  of F whose chunk sum is bounded by 2^-S in every element (from
                                  the chunk maxima of |sqrt(w bw) Phi| and |v|), so each element
                                  moves by at most (chunks) 2^-S; 0 = off (default). Not a
                                  launch shape: results change within that bound */
+  NAPMO_TUNE_DENSITY_SCREEN = 12 /* orbital-form density on the streamed kernel (nocc <= 64):
+                                    S > 0 skips, per 16-point tile, the 16-function groups whose
+                                    terms of X = Phi C are bounded by 2^-S (from the tile maxima
+                                    of |Phi| and the group's sums of max |C|), so each X_pc moves
+                                    by at most (ld / 16) 2^-S; keeps a copy of Phi with its
+                                    columns reordered (compact functions first); 0 = off
+                                    (default). Not a launch shape: rho changes within that bound */
 };
 int napmo_fock_plan_set_tuning(void *plan, int key, long value);
 int napmo_fock_plan_get_tuning(void *plan, int key, long *value);

@@ -17,6 +17,13 @@
 namespace napmo {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+
+// Smallest e with 2^e >= m (m > 0).
+__device__ inline int dens_pow2_ceil_exp(double m) {
+  int ex;
+  const double mant = frexp(m, &ex);
+  return mant == 0.5 ? ex - 1 : ex;
+}
 typedef double d2 __attribute__((ext_vector_type(2)));
 
 namespace {
@@ -193,22 +200,49 @@ __global__ void __launch_bounds__(256) k_density_mfma_group(const DensityMember 
 // group multiplies element s of it by Cf[g][j][lane][s] (the k order inside a group is
 // permuted identically on both operands). A wave owns 16 RB rows at a time and walks the
 // row tiles with a stride of all waves in the grid; two groups are in flight per wave.
-template <int NB, int RB, int WAVES, int PF>
+//
+// SCR (screening, sc.S > 0): phi is the plan's Phi with its columns in the order sc.perm
+// (row k of C is C[sc.perm[k]]), sc.eT [Npad / 16][ld / 16] bounds every |Phi| of a 16-row,
+// 16-function block by 2^eT, and each workgroup bounds sum over a group's functions of
+// max_c |C| by 2^eC; the products of a 16-row tile and group with eT + eC <= -S (each of
+// their terms of X below 2^-S) are neither loaded nor multiplied.
+template <int NB, int RB, int WAVES, int PF, bool SCR>
 __global__ void __launch_bounds__(64 * WAVES) k_density_orb_stream(
     const double *__restrict__ phi, int ld, long Npad, const double *__restrict__ C, int nrow,
-    int ncol, int kend, double scale, long nvalid, long ntiles, double *__restrict__ out) {
-  extern __shared__ d4 Cf[];  // [G][NB][64] vectors of 4 steps
+    int ncol, int kend, double scale, long nvalid, long ntiles, double *__restrict__ out,
+    DensityScreen sc) {
+  extern __shared__ d4 Cf[];  // [G][NB][64] vectors of 4 steps (then, SCR: eC [G] int)
   const int G = kend / 16;
   // C [nrow x ncol] row-major as given; entries past it are the zero padding
   for (int e = threadIdx.x; e < G * NB * 256; e += 64 * WAVES) {
     const int st = e & 3, ln = (e >> 2) & 63, gj = e >> 8;
     const int j = gj % NB, g = gj / NB;
     const int k = 16 * g + 4 * (ln >> 4) + st, c = 16 * j + (ln & 15);
-    ((double *)Cf)[e] = (k < nrow && c < ncol) ? C[(long)k * ncol + c] : 0.0;
+    const int kk = SCR && k < nrow ? sc.perm[k] : k;
+    ((double *)Cf)[e] = (k < nrow && c < ncol) ? C[(long)kk * ncol + c] : 0.0;
+  }
+  int *eC = (int *)(Cf + G * NB * 64);
+  if constexpr (SCR) {
+    // eC[g]: exponent bounding sum over the group's 16 functions of max_c |C| (16 lanes each)
+    for (int e = threadIdx.x; e < G * 16; e += 64 * WAVES) {
+      const int k = e;  // 16 g + t
+      double m = 0.0;
+      if (k < nrow) {
+        const double *row = C + (long)sc.perm[k] * ncol;
+        for (int c = 0; c < ncol; ++c) m = fmax(m, fabs(row[c]));
+      }
+      m += __shfl_xor(m, 8);
+      m += __shfl_xor(m, 4);
+      m += __shfl_xor(m, 2);
+      m += __shfl_xor(m, 1);
+      if ((e & 15) == 0) eC[e >> 4] = m > 0.0 ? dens_pow2_ceil_exp(m) : -4000;
+    }
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, kr = lane >> 4, mi = lane & 15;
   const long nw = (long)gridDim.x * WAVES;
+  const long ntile16 = Npad / 16;
+  const int ldg = ld / 16;
   for (long t = (long)blockIdx.x * WAVES + wave; t < ntiles; t += nw) {
     const long r0 = t * 16 * RB;
     const double *ap[RB];
@@ -217,6 +251,17 @@ __global__ void __launch_bounds__(64 * WAVES) k_density_orb_stream(
       const long row = min(r0 + rb * 16 + mi, Npad - 1);  // rows past Npad: read, never written
       ap[rb] = phi + row * ld + 4 * kr;
     }
+    // SCR: bit g of live[rb] = the (16-row tile, group g) products are kept
+    unsigned long long live[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+      live[rb] = ~0ull;
+      if constexpr (SCR) {
+        const long t16 = min(r0 / 16 + rb, ntile16 - 1);
+        const bool keep = lane < G && (int)sc.eT[t16 * ldg + lane] + eC[lane] > -sc.S;
+        live[rb] = __ballot(keep);
+      }
+    }
     d4 acc[RB][NB];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
@@ -224,19 +269,32 @@ __global__ void __launch_bounds__(64 * WAVES) k_density_orb_stream(
       for (int j = 0; j < NB; ++j) acc[rb][j] = (d4){0, 0, 0, 0};
     auto load = [&](d4 (&a)[RB], int g) {
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb) a[rb] = *(const d4 *)(ap[rb] + 16 * g);
+      for (int rb = 0; rb < RB; ++rb)
+        if (!SCR || ((live[rb] >> g) & 1)) a[rb] = *(const d4 *)(ap[rb] + 16 * g);
     };
     auto compute = [&](const d4 (&a)[RB], int g) {
       d4 b[NB];
 #pragma unroll
       for (int j = 0; j < NB; ++j) b[j] = Cf[(g * NB + j) * 64 + lane];
+      if constexpr (!SCR) {
 #pragma unroll
-      for (int st = 0; st < 4; ++st)
+        for (int st = 0; st < 4; ++st)
+#pragma unroll
+          for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+              acc[rb][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb][st], b[j][st], acc[rb][j], 0, 0, 0);
+      } else {
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb)
+          if ((live[rb] >> g) & 1) {
 #pragma unroll
-          for (int j = 0; j < NB; ++j)
-            acc[rb][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb][st], b[j][st], acc[rb][j], 0, 0, 0);
+            for (int st = 0; st < 4; ++st)
+#pragma unroll
+              for (int j = 0; j < NB; ++j)
+                acc[rb][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb][st], b[j][st], acc[rb][j], 0, 0, 0);
+          }
+      }
     };
     // a ring of PF groups in flight: slot i holds group g0 + i
     d4 a[PF][RB];
@@ -271,13 +329,14 @@ __global__ void __launch_bounds__(64 * WAVES) k_density_orb_stream(
   }
 }
 
-template <int NB, int RB, int WAVES, int PF>
-void launch_orb_stream_nb(const double *phi, int ld, long Npad, const double *C, int nrow, int ncol,
-                          int kend, double scale, long nvalid, double *out, hipStream_t s) {
+template <int NB, int RB, int WAVES, int PF, bool SCR>
+void launch_orb_stream_scr(const double *phi, int ld, long Npad, const double *C, int nrow, int ncol,
+                           int kend, double scale, long nvalid, double *out, const DensityScreen &sc,
+                           hipStream_t s) {
   constexpr int STREAM_RB = RB, STREAM_WAVES = WAVES;
-  const size_t bytes = (size_t)kend * 16 * NB * sizeof(double);
+  const size_t bytes = (size_t)kend * 16 * NB * sizeof(double) + (SCR ? 4 * (size_t)round_up(kend / 16, 4) : 0);
   const long ntiles = div_up(Npad, 16L * STREAM_RB);
-  auto kern = k_density_orb_stream<NB, RB, WAVES, PF>;
+  auto kern = k_density_orb_stream<NB, RB, WAVES, PF, SCR>;
   static thread_local int configured = -1;  // per instantiation: dynamic LDS above 64 KB
   int dev = 0;
   NAPMO_HIP(hipGetDevice(&dev));
@@ -292,8 +351,18 @@ void launch_orb_stream_nb(const double *phi, int ld, long Npad, const double *C,
   NAPMO_REQUIRE(occ >= 1, "density_orb_stream: no occupancy");
   const long blocks = std::min<long>((long)occ * cus, div_up(ntiles, STREAM_WAVES));
   kern<<<dim3((unsigned)blocks), dim3(64 * STREAM_WAVES), bytes, s>>>(phi, ld, Npad, C, nrow, ncol,
-                                                                       kend, scale, nvalid, ntiles, out);
+                                                                       kend, scale, nvalid, ntiles, out, sc);
   NAPMO_HIP(hipGetLastError());
+}
+
+template <int NB, int RB, int WAVES, int PF>
+void launch_orb_stream_nb(const double *phi, int ld, long Npad, const double *C, int nrow, int ncol,
+                          int kend, double scale, long nvalid, double *out, const DensityScreen &sc,
+                          hipStream_t s) {
+  if (sc.S > 0)
+    launch_orb_stream_scr<NB, RB, WAVES, PF, true>(phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, sc, s);
+  else
+    launch_orb_stream_scr<NB, RB, WAVES, PF, false>(phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, sc, s);
 }
 
 template <int MODE, int DKS, int PF>
@@ -398,31 +467,66 @@ bool density_orb_stream_supported(int ld, int ldc, int nreal) {
 template <int RB, int WAVES, int PF>
 static void orb_stream_shape(int nb, const double *phi, int ld, long Npad, const double *C, int nrow,
                              int ncol, int kend, double scale, long nvalid, double *out,
-                             hipStream_t s) {
+                             const DensityScreen &sc, hipStream_t s) {
   switch (nb) {
-    case 1: launch_orb_stream_nb<1, RB, WAVES, PF>(phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, s); break;
-    case 2: launch_orb_stream_nb<2, RB, WAVES, PF>(phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, s); break;
-    case 3: launch_orb_stream_nb<3, RB, WAVES, PF>(phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, s); break;
-    default: launch_orb_stream_nb<4, RB, WAVES, PF>(phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, s); break;
+    case 1: launch_orb_stream_nb<1, RB, WAVES, PF>(phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, sc, s); break;
+    case 2: launch_orb_stream_nb<2, RB, WAVES, PF>(phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, sc, s); break;
+    case 3: launch_orb_stream_nb<3, RB, WAVES, PF>(phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, sc, s); break;
+    default: launch_orb_stream_nb<4, RB, WAVES, PF>(phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, sc, s); break;
   }
 }
 
 void launch_density_orb_stream(const double *phi, int ld, long Npad, const double *C, int nrow,
                                int ncol, double *out, hipStream_t s, double scale, long nvalid,
-                               int shape) {
+                               int shape, const DensityScreen &sc) {
   const int ldc = density_orb_cols(ncol);
   NAPMO_REQUIRE(nrow >= 1 && nrow <= ld && ncol >= 1, "density_orb_stream: bad C shape");
   NAPMO_REQUIRE(density_orb_stream_supported(ld, ldc, nrow) && Npad % 16 == 0,
                 "density_orb_stream: C does not fit in LDS or bad padding");
+  NAPMO_REQUIRE(sc.S <= 0 || (sc.perm && sc.eT && ld <= 1024),
+                "density_orb_stream: screening needs the column order and block exponents");
   const int kend = (int)std::min<long>(ld, round_up(nrow, 16));
   if (nvalid < 0) nvalid = Npad;
   const int nb = ldc / 16;
   // shapes: rows per wave (16 RB), waves per workgroup, K groups in flight
   switch (shape) {
-    case 2: orb_stream_shape<3, 8, 2>(nb, phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, s); break;
-    case 3: orb_stream_shape<3, 8, 3>(nb, phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, s); break;
-    default: orb_stream_shape<2, 12, 3>(nb, phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, s); break;
+    case 2: orb_stream_shape<3, 8, 2>(nb, phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, sc, s); break;
+    case 3: orb_stream_shape<3, 8, 3>(nb, phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, sc, s); break;
+    default: orb_stream_shape<2, 12, 3>(nb, phi, ld, Npad, C, nrow, ncol, kend, scale, nvalid, out, sc, s); break;
   }
+}
+
+__global__ void k_phi_tile_exp(const double *__restrict__ phi, int ld, long ntile16,
+                               signed char *__restrict__ out) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ntile16 * ld) return;
+  const long t = e / ld;
+  const int f = (int)(e % ld);
+  double m = 0.0;
+  for (int r = 0; r < 16; ++r) m = fmax(m, fabs(phi[(16 * t + r) * ld + f]));
+  out[e] = m > 0.0 ? (signed char)max(-127, min(127, dens_pow2_ceil_exp(m))) : (signed char)-128;
+}
+
+__global__ void k_permute_cols(const double *__restrict__ phi, int ld, long Npad, const int *__restrict__ perm,
+                               double *__restrict__ out) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= Npad * ld) return;
+  const long p = e / ld;
+  const int k = (int)(e % ld);
+  out[e] = phi[p * ld + perm[k]];
+}
+
+void launch_phi_tile_exp(const double *phi, int ld, long Npad, signed char *out, hipStream_t s) {
+  NAPMO_REQUIRE(Npad % 16 == 0, "phi tile exponents: rows not a multiple of 16");
+  const long ne = (Npad / 16) * ld;
+  k_phi_tile_exp<<<div_up(ne, 256), 256, 0, s>>>(phi, ld, Npad / 16, out);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_permute_cols(const double *phi, int ld, long Npad, const int *perm, double *out, hipStream_t s) {
+  const long ne = Npad * ld;
+  k_permute_cols<<<div_up(ne, 256), 256, 0, s>>>(phi, ld, Npad, perm, out);
+  NAPMO_HIP(hipGetLastError());
 }
 
 void launch_phi_times(const double *phi, int ld, long Npad, const double *B, int ldb, double *X,

@@ -116,6 +116,13 @@ void FockPlan::set_tuning(int key, long value) {
       NAPMO_REQUIRE(value >= 0 && value <= 3, "density streaming choice must be 0 to 3");
       density_stream = (int)value;
       break;
+    case NAPMO_TUNE_DENSITY_SCREEN:
+      NAPMO_REQUIRE(value >= 0 && value <= 1000, "density screening threshold must be 0 (off) to 1000");
+      NAPMO_REQUIRE(value == 0 || ld <= 1024, "density screening needs at most 1024 functions");
+      NAPMO_HIP(hipStreamSynchronize(stream));
+      density_screen = (int)value;
+      density_screen_setup();
+      break;
     case NAPMO_TUNE_INT8_CHUNK:
     case NAPMO_TUNE_INT8_WAVE_BLOCKS:
     case NAPMO_TUNE_INT8_SCREEN:
@@ -153,6 +160,7 @@ long FockPlan::get_tuning(int key) const {
     case NAPMO_TUNE_SYNTH_MFMA_MIN: return grid.synth_mfma_min;
     case NAPMO_TUNE_BACKWARD_POINTS: return bwd_points ? backward_chunk() : 0;
     case NAPMO_TUNE_DENSITY_STREAM: return density_stream;
+    case NAPMO_TUNE_DENSITY_SCREEN: return density_screen;
     case NAPMO_TUNE_INT8_CHUNK: return prec == NAPMO_PREC_INT8 ? oz.chunk : 0;
     case NAPMO_TUNE_INT8_WAVE_BLOCKS: return prec == NAPMO_PREC_INT8 ? oz_wave_blocks : 0;
     case NAPMO_TUNE_INT8_SCREEN: return prec == NAPMO_PREC_INT8 ? oz_screen : 0;
@@ -211,6 +219,45 @@ void FockPlan::density_P(int slot, const double *dP) {
   launch_rho_reduce(rho_part.get(), ld / TILE, Npad, grid.N, rho.get(), st);
 }
 
+void FockPlan::density_screen_setup() {
+  hipStream_t st = stream;
+  if (density_screen <= 0) {
+    phi_scr.resize(0), dens_perm.resize(0), dens_eT.resize(0);
+    return;
+  }
+  // per (16-row tile, function) exponents of max |Phi|; the compact functions first (by the
+  // number of tiles where they reach the largest exponent less 30), then the block exponents
+  // of Phi in that order
+  const long nt = Npad / 16;
+  const int ng = ld / 16;
+  DeviceBuffer<signed char> e16((size_t)nt * ld);
+  launch_phi_tile_exp(phi64.get(), ld, Npad, e16.get(), st);
+  std::vector<signed char> e((size_t)nt * ld);
+  e16.download(e.data(), e.size(), st);
+  NAPMO_HIP(hipStreamSynchronize(st));
+  int top = -128;
+  for (long t = 0; t < nt; ++t)
+    for (int f = 0; f < n; ++f) top = std::max(top, (int)e[(size_t)t * ld + f]);
+  std::vector<long> reach(n, 0);
+  for (long t = 0; t < nt; ++t)
+    for (int f = 0; f < n; ++f) reach[f] += e[(size_t)t * ld + f] != -128 && e[(size_t)t * ld + f] >= top - 30;
+  std::vector<int> order(ld);
+  for (int f = 0; f < ld; ++f) order[f] = f;
+  std::stable_sort(order.begin(), order.begin() + n, [&](int x, int y) { return reach[x] < reach[y]; });
+  std::vector<signed char> eT((size_t)nt * ng);
+  for (long t = 0; t < nt; ++t)
+    for (int g = 0; g < ng; ++g) {
+      int m = -128;
+      for (int i = 0; i < 16; ++i) m = std::max(m, (int)e[(size_t)t * ld + order[16 * g + i]]);
+      eT[(size_t)t * ng + g] = (signed char)m;
+    }
+  dens_perm.upload(order, st);
+  dens_eT.upload(eT, st);
+  phi_scr.resize((size_t)Npad * ld);
+  launch_permute_cols(phi64.get(), ld, Npad, dens_perm.get(), phi_scr.get(), st);
+  NAPMO_HIP(hipStreamSynchronize(st));
+}
+
 void FockPlan::density_C(int slot, const double *dC, int nocc, double eta) {
   NAPMO_REQUIRE(nocc > 0 && nocc <= n, "orbital count out of range");
   hipStream_t st = stream;
@@ -221,8 +268,10 @@ void FockPlan::density_C(int slot, const double *dC, int nocc, double eta) {
     // rho = eta sum_i X_pi^2 itself. P = eta C C^T feeds the energy trace only.
     launch_gram(dC, n, nocc, eta, Pfull.get(), st);
     mark(slot, 0);
-    launch_density_orb_stream(phi64.get(), ld, Npad, dC, n, nocc, rho.get(), st, eta, grid.N,
-                              density_stream);
+    DensityScreen sc;
+    if (density_screen > 0) sc.perm = dens_perm.get(), sc.eT = dens_eT.get(), sc.S = density_screen;
+    launch_density_orb_stream(density_screen > 0 ? phi_scr.get() : phi64.get(), ld, Npad, dC, n, nocc,
+                              rho.get(), st, eta, grid.N, density_stream, sc);
     mark(slot, 1);
     return;
   }

@@ -54,6 +54,11 @@ struct FockPlan {
   int density_pf = 1;             // stages in flight in the density kernels (1 or 2)
   int density_stream = 1;  // orbital form: C resident in LDS, Phi streamed (when it fits), in
                            // launch shape 1..3; 0: the LDS-staged kernels
+  int density_screen = 0;  // NAPMO_TUNE_DENSITY_SCREEN (0: off)
+  DeviceBuffer<double> phi_scr;        // Phi with its columns in the order dens_perm
+  DeviceBuffer<int> dens_perm;         // [ld]
+  DeviceBuffer<signed char> dens_eT;   // [Npad / 16][ld / 16] block exponents of phi_scr
+  void density_screen_setup();
   hipStream_t stream = nullptr;   // every build launch goes here (default: the grid's)
   void reshape();
   // Launch-shape knobs (NAPMO_TUNE_*), for the autotuner; not while a build is running.

@@ -323,9 +323,24 @@ void launch_density_orb_f64(const double *phi, int ld, long Npad, const double *
 // out[p] = scale sum_c X_pc^2 for p < nvalid (nvalid < 0: Npad). shape 1..3: (rows per
 // wave, waves per workgroup, K groups in flight) = (32, 12, 3), (48, 8, 2), (48, 8, 3).
 bool density_orb_stream_supported(int ld, int ldc, int nreal);
+// Screening of the streamed orbital-form density (S > 0): phi is then the plan's Phi with its
+// columns in the order perm (row k of C taken as C[perm[k]]), eT [Npad / 16][ld / 16] int8 with
+// every |phi| of a 16-row, 16-column block <= 2^eT; the (16-row tile, 16-function group)
+// products with eT + eC <= -S, 2^eC >= sum over the group of max_c |C|, are skipped, so each
+// X_pc moves by at most (ld / 16) 2^-S.
+struct DensityScreen {
+  const int *perm = nullptr;
+  const signed char *eT = nullptr;
+  int S = 0;
+};
 void launch_density_orb_stream(const double *phi, int ld, long Npad, const double *C, int nrow,
                                int ncol, double *out, hipStream_t s, double scale, long nvalid,
-                               int shape = 1);
+                               int shape = 1, const DensityScreen &sc = DensityScreen());
+// Per (16-row tile, column) exponent of max |phi| over the 16 rows: out [Npad / 16][ld] int8,
+// clamped to [-127, 127], -128 for an all-zero tile column.
+void launch_phi_tile_exp(const double *phi, int ld, long Npad, signed char *out, hipStream_t s);
+// out [Npad x ld] = phi with its columns in the order perm (out[:, k] = phi[:, perm[k]]).
+void launch_permute_cols(const double *phi, int ld, long Npad, const int *perm, double *out, hipStream_t s);
 void launch_gram(const double *C, int n, int nocc, double eta, double *P, hipStream_t s);
 void launch_pad2d(const double *src, int rows, int cols, double *dst, int drows, int dcols,
                   hipStream_t s);

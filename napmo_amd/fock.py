@@ -14,7 +14,8 @@ import numpy as np
 
 from napmo_amd.system import cext
 
-SCREEN_KEY = 11   # NAPMO_TUNE_INT8_SCREEN
+SCREEN_KEY = 11          # NAPMO_TUNE_INT8_SCREEN
+DENSITY_SCREEN_KEY = 12  # NAPMO_TUNE_DENSITY_SCREEN
 
 PRECISIONS = {'fp64': 0, 'fp32': 1, 'bf16x3': 2, 'int8': 3}
 
@@ -132,6 +133,12 @@ class GridFock(object):
         """int8 plans: skip, per chunk of points, the 16 x 16 blocks of F bounded by 2^-S in
         every element (0: off), so each element moves by at most (chunks) 2^-S."""
         cext.check(cext.napmo_fock_plan_set_tuning(self._this, SCREEN_KEY, int(S)))
+
+    def set_density_screen(self, S):
+        """Orbital-form density on the streamed kernel: skip, per 16-point tile, the
+        16-function groups whose terms of X = Phi C are bounded by 2^-S (0: off), so each
+        X_pc moves by at most (ld / 16) 2^-S."""
+        cext.check(cext.napmo_fock_plan_set_tuning(self._this, DENSITY_SCREEN_KEY, int(S)))
 
     def screen_stats(self):
         """int8 plans: (blocks x chunks computed by the last contraction, all of them)."""

@@ -1149,6 +1149,53 @@ def density_from_dm(phi, P):
     return np.einsum('pm,mn,pn->p', phi, P, phi)
 
 
+def _butterfly16(v):
+    """Sum of 16 values in the order of a 16-lane xor butterfly (offsets 8, 4, 2, 1) read at
+    lane 0: the order in which the GPU forms a group's sum."""
+    v = list(v)
+    for o in (8, 4, 2, 1):
+        v = [v[i] + v[i ^ o] for i in range(16)]
+    return v[0]
+
+
+def density_orbitals_screened(phi, C, eta, S, ld=None):
+    """rho = eta sum_c X_pc^2, X = Phi C, with the screening of the streamed orbital-form
+    density (napmo_amd/csrc/density.hip, k_density_orb_stream with SCR; the order and block
+    exponents of FockPlan::density_screen_setup): per 16-row tile t and function f the
+    exponent e[t][f] of max |Phi| over the tile (clamped to [-127, 127], -128 if zero); the
+    functions ordered by the number of tiles where e reaches the largest one less 30 (fewest
+    first, ties by index); per 16-function group g of that order eT[t][g] = max e and
+    eC[g] = exponent of the butterfly sum over the group of max_c |C|; the (t, g) products
+    with eT + eC <= -S are left out. Returns (rho, order, kept fraction). phi [N, n], C [n, nocc]."""
+    N, n = phi.shape
+    ld = ld or -(-n // 64) * 64
+    nt = -(-N // 16)
+    P = np.zeros((nt * 16, ld))
+    P[:N, :n] = phi
+    m = np.abs(P).reshape(nt, 16, ld).max(axis=1)
+    e = np.where(m > 0.0, np.clip(_pow2_ceil_exp(m), -127, 127), -128)
+    en = e[:, :n]
+    top = en.max()
+    reach = ((en != -128) & (en >= top - 30)).sum(axis=0)
+    order = np.concatenate([np.argsort(reach, kind='stable'), np.arange(n, ld)])
+    G = -(-n // 16)
+    eT = e[:, order].reshape(nt, ld // 16, 16).max(axis=2)[:, :G]
+    cmax = np.zeros(16 * G)
+    cmax[:n] = np.abs(C[order[:n]]).max(axis=1)
+    csum = np.array([_butterfly16(cmax[16 * g:16 * g + 16]) for g in range(G)])
+    eC = np.where(csum > 0.0, _pow2_ceil_exp(csum), -4000)
+    keep = eT + eC[None, :] > -S                                     # [nt, G]
+    Pp = P[:, order[:16 * G]] if 16 * G <= ld else None
+    Cp = np.zeros((16 * G, C.shape[1]))
+    Cp[:n] = C[order[:n]]
+    X = np.zeros((nt * 16, C.shape[1]))
+    for g in range(G):
+        rows = np.repeat(keep[:, g], 16)
+        X[rows] += Pp[rows, 16 * g:16 * g + 16] @ Cp[16 * g:16 * g + 16]
+    rho = eta * (X[:N] ** 2).sum(axis=1)
+    return rho, order, float(keep.mean())
+
+
 def density_from_dm_screened(phi, dm, epsilon, dmmaxrow, out):
     """nwavefunction_compute_density_from_dm (src/wavefunction/nwavefunction.cpp:109-159)
     with its sequential screening: one running epsilon over the points in order; at the
