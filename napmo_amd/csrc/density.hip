@@ -243,6 +243,16 @@ __global__ void __launch_bounds__(64 * WAVES) k_density_orb_stream(
   const long nw = (long)gridDim.x * WAVES;
   const long ntile16 = Npad / 16;
   const int ldg = ld / 16;
+  // SCR: the block exponents of a wave's next tile are loaded one tile ahead
+  auto tile_exp = [&](long t, int rb) -> int {
+    const long t16 = min(t * RB + rb, ntile16 - 1);
+    return lane < G ? (int)sc.eT[t16 * ldg + lane] : 0;
+  };
+  int eTn[RB];
+  if constexpr (SCR) {
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) eTn[rb] = tile_exp((long)blockIdx.x * WAVES + wave, rb);
+  }
   for (long t = (long)blockIdx.x * WAVES + wave; t < ntiles; t += nw) {
     const long r0 = t * 16 * RB;
     const double *ap[RB];
@@ -257,9 +267,9 @@ __global__ void __launch_bounds__(64 * WAVES) k_density_orb_stream(
     for (int rb = 0; rb < RB; ++rb) {
       live[rb] = ~0ull;
       if constexpr (SCR) {
-        const long t16 = min(r0 / 16 + rb, ntile16 - 1);
-        const bool keep = lane < G && (int)sc.eT[t16 * ldg + lane] + eC[lane] > -sc.S;
+        const bool keep = lane < G && eTn[rb] + eC[lane] > -sc.S;
         live[rb] = __ballot(keep);
+        eTn[rb] = tile_exp(t + nw, rb);
       }
     }
     d4 acc[RB][NB];
@@ -296,18 +306,44 @@ __global__ void __launch_bounds__(64 * WAVES) k_density_orb_stream(
           }
       }
     };
-    // a ring of PF groups in flight: slot i holds group g0 + i
     d4 a[PF][RB];
+    if constexpr (!SCR) {
+      // a ring of PF groups in flight: slot i holds group g0 + i
 #pragma unroll
-    for (int i = 0; i < PF; ++i)
-      if (i < G) load(a[i], i);
-    for (int g0 = 0; g0 < G; g0 += PF) {
+      for (int i = 0; i < PF; ++i)
+        if (i < G) load(a[i], i);
+      for (int g0 = 0; g0 < G; g0 += PF) {
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+          const int g = g0 + i;
+          if (g < G) {
+            compute(a[i], g);
+            if (g + PF < G) load(a[i], g + PF);
+          }
+        }
+      }
+    } else {
+      // the ring walks only the groups some 16-row tile of this wave keeps, in increasing
+      // order (each accumulator adds its kept groups in the same order as the full ring)
+      unsigned long long rest = 0;
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) rest |= live[rb];
+      if (G < 64) rest &= (1ull << G) - 1;
+      int gl[PF];
 #pragma unroll
       for (int i = 0; i < PF; ++i) {
-        const int g = g0 + i;
-        if (g < G) {
-          compute(a[i], g);
-          if (g + PF < G) load(a[i], g + PF);
+        gl[i] = rest ? (int)__builtin_ctzll(rest) : -1;
+        rest &= rest - 1;
+        if (gl[i] >= 0) load(a[i], gl[i]);
+      }
+      while (gl[0] >= 0) {
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+          if (gl[i] < 0) break;
+          compute(a[i], gl[i]);
+          gl[i] = rest ? (int)__builtin_ctzll(rest) : -1;
+          rest &= rest - 1;
+          if (gl[i] >= 0) load(a[i], gl[i]);
         }
       }
     }
