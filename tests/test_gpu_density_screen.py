@@ -64,3 +64,35 @@ def test_both_screenings_on_the_16_atom_system():
           (E2 - E1, Eg2 - Eg1, np.abs(F2 - F1).max()))
     assert abs(E2 - E1) <= 1e-6 and abs(Eg2 - Eg1) <= 1e-6
     assert np.abs(F2 - F1).max() <= 1e-7 * np.abs(F1).max()
+
+
+def test_screening_edge_cases_small_basis_one_orbital_aggressive_threshold():
+    # H2 with a cc-pVDZ-size basis (10 functions: one partial block / group), one orbital and
+    # the most aggressive thresholds: both screened paths still equal their oracle
+    # restatements, and the contraction's left-out blocks are counted as the oracle counts them
+    import napmo_amd
+    g, b = systems.gpu_system(systems.H2_ATOMS, systems.named_shells(systems.H2_ATOMS, 'CC-PVDZ'), 40, 110)
+    phi = b.compute(g.points)
+    wbw = g.weights * g.becke_weights
+    C = np.ascontiguousarray(np.linalg.qr(np.random.default_rng(8).normal(size=(b['nbasis'], 1)))[0]) * 0.7
+    f = napmo_amd.GridFock(g, b, 6, 'int8')
+    for S in (1, 8):
+        f.set_density_screen(S)
+        _, _, _, rho, _ = f.build_orbitals(C, 2.0, want_grid=True)
+        ro, _, kept = core.density_orbitals_screened(phi, C, 2.0, S)
+        print('H2 density S %d: kept %.3f, max|rho - oracle| %.3e' % (S, kept, np.abs(rho - ro).max()))
+        assert np.abs(rho - ro).max() <= 1e-12 * max(ro.max(), 1e-300)
+    f.set_density_screen(0)
+    v = 1.0 / (0.5 + (g.points ** 2).sum(1))
+    chunk = 256
+    from napmo_amd.system import cext
+    cext.check(cext.napmo_fock_plan_set_tuning(f._this, 7, chunk))
+    order = core.int8_compact_order(phi, wbw, chunk)
+    for S in (1, 8, 30):
+        f.set_screen(S)
+        F = f.contract(v)
+        st = {}
+        Fo = core.int8_fock(phi, wbw, v, chunk, screen=S, order=order, stats=st)
+        print('H2 int8 S %d: %d of %d chunk blocks left out' % (S, st['skipped'], st['kept'] + st['skipped']))
+        assert np.array_equal(F, Fo)
+        assert f.screen_stats() == (st['kept'], st['kept'] + st['skipped'])
