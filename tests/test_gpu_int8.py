@@ -169,3 +169,30 @@ def test_int8_screening_on_the_16_atom_system():
     assert np.abs(F3 - F2).max() <= nchunk * 2.0 ** -50 + 1e-9 * np.abs(F2).max()
     assert np.abs(F3 - F1).max() <= 1e-7 * np.abs(F1).max()
     assert np.array_equal(F3, F3.T)
+
+
+@pytest.mark.parametrize('seed,natom,ncycle', [(11, 2, 4), (12, 4, 8), (13, 6, 12)])
+def test_int8_screening_bit_exact_on_random_molecules(seed, natom, ncycle):
+    # random molecules of 20-120 functions in a 9 bohr box at thresholds 2^-20 and 2^-50:
+    # the screened contraction equals the oracle's restatement bit for bit, and its count of
+    # computed blocks equals the oracle's
+    import napmo_amd
+    rng = np.random.default_rng(seed)
+    atoms = systems.random_molecule(rng, natom, 9.0, min_sep=1.4)
+    shells = systems.random_shells(rng, natom, ncycle)
+    g, b = systems.gpu_system(atoms, shells, 20, 110)
+    fock = napmo_amd.GridFock(g, b, 6, 'int8')
+    chunk = _plan_chunk(fock, 384)
+    phi = b.compute(g.points)
+    wbw = g.weights * g.becke_weights
+    v = 2.0 / (0.3 + ((g.points - g.origin[0]) ** 2).sum(1)) + 0.1
+    order = core.int8_compact_order(phi, wbw, chunk)
+    for S in (20, 50):
+        fock.set_screen(S)
+        F = fock.contract(v)
+        st = {}
+        Fo = core.int8_fock(phi, wbw, v, chunk, screen=S, order=order, stats=st)
+        print('seed %d n %d S %d: %d of %d chunk blocks left out' % (
+            seed, b['nbasis'], S, st['skipped'], st['kept'] + st['skipped']))
+        assert np.array_equal(F, Fo)
+        assert fock.screen_stats() == (st['kept'], st['kept'] + st['skipped'])
