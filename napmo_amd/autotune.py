@@ -45,9 +45,11 @@ def candidates(fock):
     """Launch shapes of the Fock contraction, each with both density stage depths."""
     if fock.precision == 'int8':
         # points per chunk (about N / 128 .. N / 512 chunks; every chunk is one workgroup per
-        # row group and one FP64 partial of F) and the blocks per wave of the row-group packing
+        # row group and one FP64 partial of F; with screening, shorter chunks keep fewer
+        # products but add partials to reduce) and the blocks per wave of the row-group packing
         N = fock.npoint
-        chunks = sorted({min(4032, max(64, -(-N // k) + 63) // 64 * 64) for k in (128, 192, 256, 384, 512)})
+        chunks = sorted({min(4032, max(64, -(-N // k) + 63) // 64 * 64)
+                         for k in (128, 192, 256, 320, 384, 448, 512)})
         out = [{'int8_chunk': c, 'int8_wave_blocks': b} for c in chunks for b in (3, 4, 5)]
     elif fock.precision == 'bf16x3':
         out = [{'use_pass': 1, 'pass_wg': w} for w in (128, 256, 512, 1024)]
