@@ -460,13 +460,18 @@ enum {
                                  the chunk maxima of |sqrt(w bw) Phi| and |v|), so each element
                                  moves by at most (chunks) 2^-S; 0 = off (default). Not a
                                  launch shape: results change within that bound */
-  NAPMO_TUNE_DENSITY_SCREEN = 12 /* orbital-form density on the streamed kernel (nocc <= 64):
+  NAPMO_TUNE_DENSITY_SCREEN = 12, /* orbital-form density on the streamed kernel (nocc <= 64):
                                     S > 0 skips, per 16-point tile, the 16-function groups whose
                                     terms of X = Phi C are bounded by 2^-S (from the tile maxima
                                     of |Phi| and the group's sums of max |C|), so each X_pc moves
                                     by at most (ld / 16) 2^-S; keeps a copy of Phi with its
                                     columns reordered (compact functions first); 0 = off
                                     (default). Not a launch shape: rho changes within that bound */
+  NAPMO_TUNE_INT8_PACK = 13 /* int8 mode: 1 = per build, the blocks each chunk keeps (all, or
+                               those NAPMO_TUNE_INT8_SCREEN keeps) are packed into as few
+                               workgroup tasks as the LDS and wave slots allow, taken in turn by
+                               one workgroup per compute unit; 0 = fixed row groups (default).
+                               A launch shape: the same results bit for bit */
 };
 int napmo_fock_plan_set_tuning(void *plan, int key, long value);
 int napmo_fock_plan_get_tuning(void *plan, int key, long *value);

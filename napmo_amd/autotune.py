@@ -21,11 +21,12 @@ from napmo_amd.system import cext
 
 (TUNE_FOCK_WG, TUNE_PASS_WG, TUNE_USE_PASS, TUNE_DENSITY_KS, TUNE_DENSITY_PF, TUNE_SYNTH_NP,
  TUNE_DENSITY_STREAM, TUNE_INT8_CHUNK, TUNE_INT8_WAVE_BLOCKS) = range(9)
+TUNE_INT8_PACK = 13
 _KEYS = {'fock_wg': TUNE_FOCK_WG, 'pass_wg': TUNE_PASS_WG, 'use_pass': TUNE_USE_PASS,
          'density_ks': TUNE_DENSITY_KS, 'density_pf': TUNE_DENSITY_PF, 'synth_np': TUNE_SYNTH_NP,
          'density_stream': TUNE_DENSITY_STREAM, 'int8_chunk': TUNE_INT8_CHUNK,
-         'int8_wave_blocks': TUNE_INT8_WAVE_BLOCKS}
-_INT8_KEYS = ('int8_chunk', 'int8_wave_blocks')   # only int8 plans have them
+         'int8_wave_blocks': TUNE_INT8_WAVE_BLOCKS, 'int8_pack': TUNE_INT8_PACK}
+_INT8_KEYS = ('int8_chunk', 'int8_wave_blocks', 'int8_pack')   # only int8 plans have them
 DEFAULTS = {'fock_wg': 0, 'pass_wg': 0, 'density_ks': 16, 'density_pf': 1,
             'synth_np': 2, 'density_stream': 1}   # 0: the library's default shape
 
@@ -50,7 +51,9 @@ def candidates(fock):
         N = fock.npoint
         chunks = sorted({min(4032, max(64, -(-N // k) + 63) // 64 * 64)
                          for k in (128, 192, 256, 320, 384, 448, 512)})
-        out = [{'int8_chunk': c, 'int8_wave_blocks': b} for c in chunks for b in (3, 4, 5)]
+        out = [{'int8_chunk': c, 'int8_wave_blocks': b, 'int8_pack': 0} for c in chunks for b in (3, 4, 5)]
+        # the kept blocks packed per chunk into tasks (the row-group packing then unused)
+        out += [{'int8_chunk': c, 'int8_wave_blocks': 5, 'int8_pack': 1} for c in chunks]
     elif fock.precision == 'bf16x3':
         out = [{'use_pass': 1, 'pass_wg': w} for w in (128, 256, 512, 1024)]
         # the symmetric form only with short chunks (its FP32 chunk sums drift otherwise)

@@ -123,6 +123,13 @@ void FockPlan::set_tuning(int key, long value) {
       density_screen = (int)value;
       density_screen_setup();
       break;
+    case NAPMO_TUNE_INT8_PACK:
+      NAPMO_REQUIRE(prec == NAPMO_PREC_INT8, "int8 tuning keys need an NAPMO_PREC_INT8 plan");
+      NAPMO_REQUIRE(value == 0 || value == 1, "int8 packing must be 0 or 1");
+      NAPMO_REQUIRE(value == 0 || (oz.pack_rows >= 1 && oz.nb <= OZ_MAXROWS),
+                    "int8 packing: the basis does not fit a packed task");
+      oz_pack = (int)value;
+      break;
     case NAPMO_TUNE_INT8_CHUNK:
     case NAPMO_TUNE_INT8_WAVE_BLOCKS:
     case NAPMO_TUNE_INT8_SCREEN:
@@ -164,6 +171,7 @@ long FockPlan::get_tuning(int key) const {
     case NAPMO_TUNE_INT8_CHUNK: return prec == NAPMO_PREC_INT8 ? oz.chunk : 0;
     case NAPMO_TUNE_INT8_WAVE_BLOCKS: return prec == NAPMO_PREC_INT8 ? oz_wave_blocks : 0;
     case NAPMO_TUNE_INT8_SCREEN: return prec == NAPMO_PREC_INT8 ? oz_screen : 0;
+    case NAPMO_TUNE_INT8_PACK: return prec == NAPMO_PREC_INT8 ? oz_pack : 0;
     default: throw Error("unknown tuning key " + std::to_string(key));
   }
 }
@@ -331,6 +339,8 @@ void FockPlan::ozaki_setup() {
   oz_maxd1.resize((size_t)ld * (oz.Nq / OZ_STAGE));
   oz_vs.resize(oz.Nq);
   oz_part.resize((size_t)oz.nchunk * oz.partial_stride());
+  oz_ptasks.resize((size_t)oz.nchunk * oz.nb * OZ_PTASK);
+  oz_pctr.resize(2);
   std::vector<int> tab = oz.tasks;
   tab.insert(tab.end(), oz.row0.begin(), oz.row0.end());
   oz_tasks.upload(tab, st);
@@ -364,8 +374,12 @@ void FockPlan::ozaki_contract(const double *veff, double *dF, int slot) {
   hipStream_t st = stream;
   launch_ozaki_vs(veff, grid.wbw.get(), grid.N, oz.Nq, oz_vs.get(), st);
   mark(slot, 7);
-  launch_ozaki_fock(oz, oz_tasks.get(), oz_planes.get(), oz_eA.get(), oz_maxd1.get(), oz_vs.get(),
-                    oz_screen, oz_part.get(), st);
+  if (oz_pack)
+    launch_ozaki_fock_packed(oz, oz_planes.get(), oz_eA.get(), oz_maxd1.get(), oz_vs.get(), oz_screen,
+                             oz_part.get(), oz_ptasks.get(), oz_pctr.get(), st);
+  else
+    launch_ozaki_fock(oz, oz_tasks.get(), oz_planes.get(), oz_eA.get(), oz_maxd1.get(), oz_vs.get(),
+                      oz_screen, oz_part.get(), st);
   mark(slot, 8);
   launch_ozaki_reduce(oz, oz_part.get(), oz_perm.get(), dF, st);
 }

@@ -144,6 +144,11 @@ struct OzArgs {
   long Nq, chunk, pstride;
   int ld, nb, nchunk, ngroups, nstage_total;
   int screen, lgchunk;  // skip blocks bounded by 2^-screen (0: none); 2^lgchunk >= chunk
+  // packed mode (ptasks != NULL): workgroups take the packer's tasks [*pntask][OZ_PTASK] in
+  // turn (*pnext: the next task to take)
+  const int *ptasks;
+  const int *pntask;
+  int *pnext;
 };
 
 __device__ inline v4i mfma8(v4i a, v4i b, v4i c) {
@@ -171,11 +176,11 @@ __device__ inline unsigned pack_bytes(int b0, int b1, int b2, int b3) {
 // B digits of the group's listed row blocks (rowl[0..nrow), local indices) for one stage into
 // an LDS buffer: one (function, 8 points) item per thread and pass. JB = rint(JA vs 2^-(eV + eB))
 // with JA = 2^14 (2^14 d1 + 2^7 d2 + d3) + 2^7 d4 + d5 read from the digit planes.
-__device__ inline void stage_b_digits(const OzArgs &a, int r0, int nB, const int *rowl, int nrow, long pn,
-                                      const double *scB, char *buf, int tid) {
+__device__ inline void stage_b_digits(const OzArgs &a, const int *rowlist, int nB, const int *rowl, int nrow,
+                                      long pn, const double *scB, char *buf, int tid) {
   for (int it = tid; it < nrow * 128; it += 64 * OZ_WAVES) {
     const int fl = 16 * rowl[it >> 7] + ((it >> 3) & 15), s8 = it & 7;
-    const int f = 16 * r0 + fl;
+    const int f = 16 * rowlist[fl >> 4] + (fl & 15);
     const long p = pn + 8 * s8;
     uint2 dg[OZ_DIGITS];
 #pragma unroll
@@ -208,21 +213,18 @@ __device__ inline void stage_b_digits(const OzArgs &a, int r0, int nB, const int
   }
 }
 
-// Workgroup = (row group g, chunk c): block rows [r0, r1) against every column block >= r0.
-// The B digits of the group's rows are formed per stage into a double-buffered LDS region;
-// each wave reads the A digits of its (at most OZ_MAXCOL) column blocks straight from the
-// planes into registers and owns at most OZ_MAXB blocks. The two halves of the workgroup (one
-// wave of each per SIMD) run the stage's MFMAs and the next stage's digit formation in
-// opposite order, so matrix and vector work of a SIMD overlap; one barrier per stage.
-__global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
-  extern __shared__ __align__(16) char lds[];
-  // chunk c on XCD c % 8 for every group (workgroups are dealt to the 8 XCDs in turn)
-  const int id = blockIdx.x;
-  const int c8 = id % 8, rest = id / 8;
-  const int g = rest % a.ngroups, c = (rest / a.ngroups) * 8 + c8;
-  if (c >= a.nchunk) return;
-  const int r0 = a.row0[g], r1 = a.row0[g + 1];
-  const int nB = 16 * (r1 - r0);
+// One (chunk c, set of block rows) task of the contraction: the block rows rows[0..nrows) (NULL:
+// the contiguous rows r0, r0 + 1, ...) against the columns of the waves' tasks wtask (OZ_TASK
+// ints per wave). The B digits of the rows are formed per stage into a double-buffered LDS
+// region; each wave reads the A digits of its (at most OZ_MAXCOL) column blocks straight from
+// the planes into registers and owns at most OZ_MAXB blocks. The two halves of the workgroup
+// (one wave of each per SIMD) run the stage's MFMAs and the next stage's digit formation in
+// opposite order, so matrix and vector work of a SIMD overlap; one barrier per stage. With
+// screen (static groups only) the blocks bounded by 2^-screen are skipped here; packed tasks
+// hold kept blocks only.
+__device__ inline void ozaki_task(const OzArgs &a, char *lds, int c, int nrows, const int *rows, int r0,
+                                  const int *wtask, bool screen) {
+  const int nB = 16 * nrows;
   const size_t bufsz = (size_t)OZ_DIGITS * nB * 64;
   char *ldsB = lds;                                         // [2][OZ_DIGITS][nB][64]
   double *scB = (double *)(lds + 2 * bufsz);                // [nB]
@@ -231,11 +233,19 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
   int *s_eV_ptr = (int *)(maxv + OZ_MAX_STAGES);             // [1] (all LDS is dynamic)
   int *s_nrow = s_eV_ptr + 1;                                // [1]
   int *eBlk = s_nrow + 1;                                    // [nb]: block exponents, this chunk
-  int *rowl = eBlk + a.nb;                                   // [r1 - r0]: row blocks with work
+  int *rowl = eBlk + a.nb;                                   // [nrows]: local row blocks with work
+  int *rowlist = rowl + OZ_MAXROWS;                          // [nrows]: local -> block row
+  int *rowslot = rowlist + OZ_MAXROWS;                       // [nb]: block row -> local
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ns = (int)(a.chunk / OZ_STAGE);
   const long pc = (long)c * a.chunk;
   const int stage0 = (int)(pc / OZ_STAGE);
+  const int rfirst = rows ? rows[0] : r0;
+  for (int s = tid; s < nrows; s += 64 * OZ_WAVES) {
+    const int I = rows ? rows[s] : r0 + s;
+    rowlist[s] = I;
+    rowslot[I] = s;
+  }
 
   // ---- prologue: stage maxima of |vs|, the chunk exponent eV, and per B function the
   // scale of its digits: bound = 2^-6 max_t (maxv_t 2^-eV) (maxd1 + 1) >= max |vs At| 2^-(eA+eV)
@@ -255,8 +265,8 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
   // screening: the largest chunk exponent of each column block's functions, then the row
   // blocks with at least one block left (every row block when screening is off)
   auto skipped = [&](int I, int J) { return eBlk[I] + eBlk[J] + eV + a.lgchunk <= -a.screen; };
-  if (a.screen > 0) {
-    for (int b = r0 + wave; b < a.nb; b += OZ_WAVES) {
+  if (screen) {
+    for (int b = rfirst + wave; b < a.nb; b += OZ_WAVES) {
       int e = lane < 16 ? a.eA[(size_t)(16 * b + lane) * a.nchunk + c] : OZ_ZERO_EXP;
       for (int o = 8; o > 0; o >>= 1) e = max(e, __shfl_xor(e, o));
       if (lane == 0) eBlk[b] = e;
@@ -264,21 +274,22 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
     __syncthreads();
     if (tid == 0) {
       int nr = 0;
-      for (int I = r0; I < r1; ++I) {
+      for (int s = 0; s < nrows; ++s) {
+        const int I = rows ? rows[s] : r0 + s;
         bool need = false;
         for (int J = I; J < a.nb && !need; ++J) need = !skipped(I, J);
-        if (need) rowl[nr++] = I - r0;
+        if (need) rowl[nr++] = s;
       }
       *s_nrow = nr;
     }
   } else if (tid == 0) {
-    for (int I = r0; I < r1; ++I) rowl[I - r0] = I - r0;
-    *s_nrow = r1 - r0;
+    for (int s = 0; s < nrows; ++s) rowl[s] = s;
+    *s_nrow = nrows;
   }
   __syncthreads();
   const int nrow = *s_nrow;
   for (int fl = tid; fl < nB; fl += 64 * OZ_WAVES) {
-    const int f = 16 * r0 + fl;
+    const int f = 16 * rowlist[fl >> 4] + (fl & 15);
     const unsigned char *md = a.maxd1 + (size_t)f * a.nstage_total + stage0;
     double bound = 0.0;
     for (int t = 0; t < ns; ++t) bound = fmax(bound, ldexp(maxv[t], -eV) * (double)(md[t] + 1));
@@ -295,7 +306,7 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
 #pragma unroll
     for (int L = 0; L < 5; ++L) acc[k][L] = (v4i){0, 0, 0, 0};
   // this wave's task, read once into scalar registers
-  const int *task = a.tasks + (size_t)(g * OZ_WAVES + wave) * OZ_TASK;
+  const int *task = wtask + (size_t)wave * OZ_TASK;
   int tI[OZ_MAXB], tJ[OZ_MAXB], tC[OZ_MAXB], cJ[OZ_MAXCOL];
 #pragma unroll
   for (int k = 0; k < OZ_MAXB; ++k) {
@@ -310,7 +321,7 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
   bool live[OZ_MAXB];
 #pragma unroll
   for (int k = 0; k < OZ_MAXB; ++k)
-    live[k] = tI[k] >= 0 && !(a.screen > 0 && __builtin_amdgcn_readfirstlane((int)skipped(tI[k], tJ[k])));
+    live[k] = tI[k] >= 0 && !(screen && __builtin_amdgcn_readfirstlane((int)skipped(tI[k], tJ[k])));
 #pragma unroll
   for (int cc = 0; cc < OZ_MAXCOL; ++cc) {
     bool used = false;
@@ -340,7 +351,7 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
       const int I = tI[k], ci = tC[k];
       if (!live[k]) continue;
       v4i ra[OZ_DIGITS];
-      const int fl = 16 * (I - r0) + r16;
+      const int fl = 16 * rowslot[I] + r16;
 #pragma unroll
       for (int d = 0; d < OZ_DIGITS; ++d) ra[d] = *(const v4i *)&bufB[seg_off(nB, d, fl, q)];
       if (ci == 0)
@@ -352,7 +363,7 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
     }
   };
   if (nrow > 0) {  // (the same for the whole workgroup)
-    stage_b_digits(a, r0, nB, rowl, nrow, pc, scB, ldsB, tid);
+    stage_b_digits(a, rowlist, nB, rowl, nrow, pc, scB, ldsB, tid);
     __syncthreads();
     if (mfma_first) load_columns(pc);
     for (int t = 0; t < ns; ++t) {
@@ -364,11 +375,11 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
         mfma_phase(bufB);
         if (more) {
           load_columns(p0 + OZ_STAGE);
-          stage_b_digits(a, r0, nB, rowl, nrow, p0 + OZ_STAGE, scB, bufN, tid);
+          stage_b_digits(a, rowlist, nB, rowl, nrow, p0 + OZ_STAGE, scB, bufN, tid);
         }
       } else {
         load_columns(p0);
-        if (more) stage_b_digits(a, r0, nB, rowl, nrow, p0 + OZ_STAGE, scB, bufN, tid);
+        if (more) stage_b_digits(a, rowlist, nB, rowl, nrow, p0 + OZ_STAGE, scB, bufN, tid);
         mfma_phase(bufB);
       }
       __syncthreads();
@@ -388,9 +399,164 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
       const int m = 4 * q + r;
       const double val = (((acc[k][0][r] * 0x1p-12 + acc[k][1][r] * 0x1p-19) + acc[k][2][r] * 0x1p-26) +
                           acc[k][3][r] * 0x1p-33) + acc[k][4][r] * 0x1p-40;
-      part[(size_t)blk * 256 + m * 16 + r16] = ldexp(val, eRow[16 * (I - r0) + m] + ec);
+      part[(size_t)blk * 256 + m * 16 + r16] = ldexp(val, eRow[16 * rowslot[I] + m] + ec);
     }
   }
+}
+
+// Workgroup = (row group g, chunk c): block rows [r0, r1) against every column block >= r0.
+__global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock(OzArgs a) {
+  extern __shared__ __align__(16) char lds[];
+  // chunk c on XCD c % 8 for every group (workgroups are dealt to the 8 XCDs in turn)
+  const int id = blockIdx.x;
+  const int c8 = id % 8, rest = id / 8;
+  const int g = rest % a.ngroups, c = (rest / a.ngroups) * 8 + c8;
+  if (c >= a.nchunk) return;
+  const int r0 = a.row0[g], r1 = a.row0[g + 1];
+  ozaki_task(a, lds, c, r1 - r0, nullptr, r0, a.tasks + (size_t)g * OZ_WAVES * OZ_TASK, a.screen > 0);
+}
+
+// Packed tasks: every workgroup takes the next task until none is left (the task count was
+// written by the packing pass before this launch).
+__global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock_packed(OzArgs a) {
+  extern __shared__ __align__(16) char lds[];
+  __shared__ int s_task;
+  const int total = *a.pntask;
+  for (;;) {
+    if (threadIdx.x == 0) s_task = atomicAdd(a.pnext, 1);
+    __syncthreads();
+    const int t = s_task;
+    __syncthreads();  // s_task read by all; the previous task's LDS use is over
+    if (t >= total) break;
+    const int *rec = a.ptasks + (size_t)t * OZ_PTASK;
+    ozaki_task(a, lds, rec[0], rec[1], rec + 2, 0, rec + 2 + OZ_MAXROWS, false);
+  }
+}
+
+__global__ void k_oz_pack_reset(int *ctr) {
+  if (threadIdx.x == 0) ctr[0] = 0, ctr[1] = 0;
+}
+
+struct OzPackArgs {
+  const int *eA;
+  const double *vs;
+  double *partial;
+  int *ptasks, *ctr;
+  long chunk, pstride;
+  int nchunk, nb, screen, lgchunk, rmax;
+};
+
+// Packing pass, one wave per chunk: the chunk exponent of |vs| and the block exponents (as the
+// contraction forms them), the kept blocks (all of them when screen == 0), zeros for the
+// skipped blocks' partials, then (the whole wave, uniformly) the kept block rows in order, grouped greedily into
+// tasks of at most rmax rows whose kept blocks, column by column, fit the waves' slots as
+// pack_waves fills them; each task is appended to ptasks at an index taken from ctr[0].
+__global__ void __launch_bounds__(64) k_oz_pack(OzPackArgs p) {
+  __shared__ int eb[OZ_MAXROWS];
+  __shared__ int s_eV;
+  const int c = blockIdx.x, lane = threadIdx.x, nb = p.nb;
+  const long pc = (long)c * p.chunk;
+  double m = 0.0;
+  // the chunk's stages 8 at a time (independent loads in flight; chunk = 64 * stages)
+  const int ns = (int)(p.chunk / OZ_STAGE);
+  int t0 = 0;
+  for (; t0 + 8 <= ns; t0 += 8) {
+    double x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = p.vs[pc + (long)(t0 + k) * OZ_STAGE + lane];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) m = fmax(m, fabs(x[k]));
+  }
+  for (; t0 < ns; ++t0) m = fmax(m, fabs(p.vs[pc + (long)t0 * OZ_STAGE + lane]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if (lane == 0) s_eV = pow2_ceil_exp(m);
+  if (lane < nb) {
+    int e = OZ_ZERO_EXP;
+    for (int i = 0; i < 16; ++i) e = max(e, p.eA[(size_t)(16 * lane + i) * p.nchunk + c]);
+    eb[lane] = e;
+  }
+  __syncthreads();
+  const int eV = s_eV;
+  // live (lane I): bit J = block (I, J) kept; colm (lane J): bit I = block (I, J) kept. The
+  // packing below runs on the whole wave with uniform control flow and reads them per lane
+  // with readlane, so its sequential steps are register operations.
+  unsigned live_r = 0;
+  if (lane < nb)
+    for (int J = lane; J < nb; ++J)
+      if (p.screen <= 0 || eb[lane] + eb[J] + eV + p.lgchunk > -p.screen) live_r |= 1u << J;
+  unsigned colm_r = 0;
+  for (int I = 0; I < nb; ++I) {
+    const unsigned li = (unsigned)__builtin_amdgcn_readlane((int)live_r, I);
+    if (I <= lane) colm_r |= ((li >> lane) & 1u) << I;
+  }
+  auto liveI = [&](int I) { return (unsigned)__builtin_amdgcn_readlane((int)live_r, I); };
+  auto colmJ = [&](int J) { return (unsigned)__builtin_amdgcn_readlane((int)colm_r, J); };
+  double *part = p.partial + (size_t)c * p.pstride;
+  for (int I = 0; I < nb; ++I) {
+    const unsigned li = liveI(I);
+    for (int J = I; J < nb; ++J)
+      if (!((li >> J) & 1)) {
+        const int blk = I * nb - I * (I - 1) / 2 + (J - I);
+        for (int e = lane; e < 256; e += 64) part[(size_t)blk * 256 + e] = 0.0;
+      }
+  }
+  // the kept blocks of the rows rm, column by column, rows in order, into the waves' slots
+  // as pack_waves fills them (out: OZ_WAVES * OZ_TASK ints written by lane 0, or NULL to
+  // test the fit)
+  auto fits = [&](unsigned rm, int *out) {
+    if (out && lane == 0)
+      for (int e = 0; e < OZ_WAVES * OZ_TASK; ++e) out[e] = -1;
+    int total = 0;
+    for (int J = 0; J < nb; ++J) total += __popc(colmJ(J) & rm);
+    int it = 0, J = 0;
+    unsigned mj = colmJ(0) & rm;
+    for (int w = 0; w < OZ_WAVES && it < total; ++w) {
+      const int left = OZ_WAVES - w;
+      const int target = min(OZ_MAXB, (total - it + left - 1) / left);
+      int ncol = 0, k = 0, last = -1;
+      while (it < total && k < target) {
+        while (mj == 0) mj = colmJ(++J) & rm;  // it < total: a kept block remains
+        if (J != last) {
+          if (ncol == OZ_MAXCOL) break;
+          if (out && lane == 0) out[w * OZ_TASK + 3 * OZ_MAXB + ncol] = J;
+          last = J, ++ncol;
+        }
+        const int I = __ffs(mj) - 1;
+        mj &= mj - 1;
+        if (out && lane == 0)
+          out[w * OZ_TASK + 3 * k] = I, out[w * OZ_TASK + 3 * k + 1] = J, out[w * OZ_TASK + 3 * k + 2] = ncol - 1;
+        ++k, ++it;
+      }
+    }
+    return it == total;
+  };
+  auto emit = [&](unsigned rm) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(&p.ctr[0], 1);
+    t = __builtin_amdgcn_readfirstlane(t);
+    int *rec = p.ptasks + (size_t)t * OZ_PTASK;
+    if (lane == 0) {
+      rec[0] = c;
+      int nr = 0;
+      for (unsigned b = rm; b; b &= b - 1) rec[2 + nr++] = __ffs(b) - 1;
+      rec[1] = nr;
+      for (int s = nr; s < OZ_MAXROWS; ++s) rec[2 + s] = -1;
+    }
+    fits(rm, rec + 2 + OZ_MAXROWS);
+  };
+  unsigned cur = 0;
+  int ncur = 0;
+  for (int I = 0; I < nb; ++I) {
+    if (!liveI(I)) continue;
+    const unsigned cand = cur | (1u << I);
+    if (ncur > 0 && (ncur + 1 > p.rmax || !fits(cand, nullptr))) {
+      emit(cur);
+      cur = 1u << I, ncur = 1;
+    } else {
+      cur = cand, ++ncur;
+    }
+  }
+  if (ncur > 0) emit(cur);
 }
 
 // F[mu][nu] = F[nu][mu] = sum over chunks (in order) of the block element, mu <= nu < n.
@@ -455,7 +621,7 @@ static bool pack_waves(int r0, int r1, int nb, int wave_blocks, std::vector<int>
 
 static size_t group_lds(int r0, int r1, int nb) {
   const size_t nB = 16 * (size_t)(r1 - r0);
-  return 2 * OZ_DIGITS * nB * 64 + nB * 12 + 8 + OZ_MAX_STAGES * 8 + 16 + 8 * (size_t)nb;
+  return 2 * OZ_DIGITS * nB * 64 + nB * 12 + 8 + OZ_MAX_STAGES * 8 + 16 + 8 * (size_t)nb + 8 * OZ_MAXROWS;
 }
 
 OzakiLayout ozaki_layout(long N, int n, int ld, int wave_blocks, size_t max_lds, long chunk_target) {
@@ -496,6 +662,13 @@ OzakiLayout ozaki_layout(long N, int n, int ld, int wave_blocks, size_t max_lds,
   L.row0 = row0;
   L.tasks = tasks;
   L.lds_bytes = lds;
+  // packed tasks: as many block rows as the LDS takes (16 bytes left for the task index)
+  for (int R = std::min(nb, OZ_MAXROWS); R >= 1; --R)
+    if (group_lds(0, R, nb) + 16 <= max_lds) {
+      L.pack_rows = R;
+      L.pack_lds = group_lds(0, R, nb);
+      break;
+    }
   return L;
 }
 
@@ -576,8 +749,48 @@ void launch_ozaki_fock(const OzakiLayout &L, const int *d_tasks, const signed ch
   a.screen = screen;
   a.lgchunk = 0;
   while ((1L << a.lgchunk) < L.chunk) ++a.lgchunk;
+  a.ptasks = nullptr, a.pntask = nullptr, a.pnext = nullptr;
   const int nc8 = (int)round_up(L.nchunk, 8);
   k_ozaki_fock<<<nc8 * L.ngroups, 64 * OZ_WAVES, L.lds_bytes, s>>>(a);
+  NAPMO_HIP(hipGetLastError());
+}
+
+void launch_ozaki_fock_packed(const OzakiLayout &L, const signed char *planes, const int *eA,
+                              const unsigned char *maxd1, const double *vs, int screen, double *partial,
+                              int *ptasks, int *ctr, hipStream_t s) {
+  static bool attr = false;
+  if (!attr) {
+    // the kernel's own 4-byte task slot is static LDS: leave 16 bytes for it
+    NAPMO_HIP(hipFuncSetAttribute((const void *)k_ozaki_fock_packed, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)max_lds_per_block() - 16));
+    attr = true;
+  }
+  NAPMO_REQUIRE(L.pack_rows >= 1 && L.nb <= OZ_MAXROWS && L.pack_lds + 16 <= max_lds_per_block(),
+                "int8 Fock contraction: packed tasks do not fit");
+  NAPMO_REQUIRE(screen >= 0 && screen <= 1000, "int8 screening threshold out of range");
+  int lg = 0;
+  while ((1L << lg) < L.chunk) ++lg;
+  k_oz_pack_reset<<<1, 64, 0, s>>>(ctr);
+  NAPMO_HIP(hipGetLastError());
+  OzPackArgs pa;
+  pa.eA = eA, pa.vs = vs, pa.partial = partial, pa.ptasks = ptasks, pa.ctr = ctr;
+  pa.chunk = L.chunk, pa.pstride = L.partial_stride();
+  pa.nchunk = L.nchunk, pa.nb = L.nb, pa.screen = screen, pa.lgchunk = lg, pa.rmax = L.pack_rows;
+  k_oz_pack<<<L.nchunk, 64, 0, s>>>(pa);
+  NAPMO_HIP(hipGetLastError());
+  OzArgs a;
+  a.planes = planes, a.eA = eA, a.maxd1 = maxd1, a.vs = vs;
+  a.tasks = nullptr, a.row0 = nullptr;
+  a.partial = partial;
+  a.Nq = L.Nq, a.chunk = L.chunk, a.pstride = L.partial_stride();
+  a.ld = L.ld, a.nb = L.nb, a.nchunk = L.nchunk, a.ngroups = L.ngroups;
+  a.nstage_total = (int)(L.Nq / OZ_STAGE);
+  a.screen = screen, a.lgchunk = lg;
+  a.ptasks = ptasks, a.pntask = ctr, a.pnext = ctr + 1;
+  int dev = 0, cus = 0;
+  NAPMO_HIP(hipGetDevice(&dev));
+  NAPMO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  k_ozaki_fock_packed<<<cus, 64 * OZ_WAVES, L.pack_lds, s>>>(a);
   NAPMO_HIP(hipGetLastError());
 }
 

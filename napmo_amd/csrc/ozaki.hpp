@@ -37,6 +37,8 @@ constexpr int OZ_MAXCOL = 3;      // column blocks per wave (their A digits held
 constexpr int OZ_TASK = 3 * OZ_MAXB + OZ_MAXCOL;  // ints per wave task
 constexpr int OZ_WAVES = 8;       // waves per workgroup
 constexpr int OZ_MAX_STAGES = 63; // stages per chunk (the level sums stay far inside int32)
+constexpr int OZ_MAXROWS = 24;    // block rows of one task (n <= 384)
+constexpr int OZ_PTASK = 2 + OZ_MAXROWS + OZ_WAVES * OZ_TASK;  // ints per packed task
 
 // Per-plan layout: chunks of points, 16-function blocks, groups of block rows.
 struct OzakiLayout {
@@ -52,6 +54,8 @@ struct OzakiLayout {
   // wave's column list; I = -1: empty), then the wave's column list (-1: none)
   std::vector<int> tasks;
   size_t lds_bytes = 0;    // largest dynamic LDS of a group
+  int pack_rows = 0;       // packed tasks: at most this many block rows (their LDS fits)
+  size_t pack_lds = 0;     // dynamic LDS of a packed task
   long partial_stride() const { return (long)nblk * 256 + 32; }
 };
 
@@ -86,6 +90,14 @@ void launch_ozaki_vs(const double *v, const double *wbw, long N, long Nq, double
 void launch_ozaki_fock(const OzakiLayout &L, const int *d_tasks, const signed char *planes,
                        const int *eA, const unsigned char *maxd1, const double *vs, int screen,
                        double *partial, hipStream_t s);
+// The same contraction with the blocks each chunk keeps packed into as few tasks as the LDS and
+// wave slots allow (screen > 0): a packing pass per chunk forms the kept set (the rule above)
+// and writes zeros for the skipped blocks' partials, then a grid of one workgroup per compute
+// unit takes the tasks in turn. ptasks [nchunk * nb][OZ_PTASK] and ctr [2] are scratch. Equal
+// to launch_ozaki_fock at the same threshold, bit for bit.
+void launch_ozaki_fock_packed(const OzakiLayout &L, const signed char *planes, const int *eA,
+                              const unsigned char *maxd1, const double *vs, int screen, double *partial,
+                              int *ptasks, int *ctr, hipStream_t s);
 // F [n x n] (symmetric) = the fixed-order sum over chunks of the partials; element (mu, nu) of
 // the operand order is F[perm[mu]][perm[nu]] (perm NULL: identity).
 void launch_ozaki_reduce(const OzakiLayout &L, const double *partial, const int *perm, double *F,

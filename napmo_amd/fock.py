@@ -16,6 +16,7 @@ from napmo_amd.system import cext
 
 SCREEN_KEY = 11          # NAPMO_TUNE_INT8_SCREEN
 DENSITY_SCREEN_KEY = 12  # NAPMO_TUNE_DENSITY_SCREEN
+INT8_PACK_KEY = 13       # NAPMO_TUNE_INT8_PACK
 
 PRECISIONS = {'fp64': 0, 'fp32': 1, 'bf16x3': 2, 'int8': 3}
 
@@ -139,6 +140,11 @@ class GridFock(object):
         16-function groups whose terms of X = Phi C are bounded by 2^-S (0: off), so each
         X_pc moves by at most (ld / 16) 2^-S."""
         cext.check(cext.napmo_fock_plan_set_tuning(self._this, DENSITY_SCREEN_KEY, int(S)))
+
+    def set_int8_pack(self, on):
+        """int8 plans: pack each chunk's kept blocks into as few workgroup tasks as fit (the
+        same results bit for bit)."""
+        cext.check(cext.napmo_fock_plan_set_tuning(self._this, INT8_PACK_KEY, int(bool(on))))
 
     def screen_stats(self):
         """int8 plans: (blocks x chunks computed by the last contraction, all of them)."""

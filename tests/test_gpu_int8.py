@@ -141,6 +141,9 @@ def test_int8_screening_is_bit_exact_with_the_oracle_restatement():
     assert np.array_equal(fock.contract(v), F_plain)
     kept, total = fock.screen_stats()
     assert kept == total
+    fock.set_int8_pack(True)    # every block packed per chunk: the unscreened contraction
+    assert np.array_equal(fock.contract(v), F_plain)
+    fock.set_int8_pack(False)
 
 
 def test_int8_screening_on_the_16_atom_system():
@@ -157,6 +160,10 @@ def test_int8_screening_on_the_16_atom_system():
     F2, E2, Eg2 = i8.build_orbitals(C, 2.0)
     _set_screen(i8, 50)
     F3, E3, Eg3 = i8.build_orbitals(C, 2.0)
+    i8.set_int8_pack(True)     # packed tasks: the same F bit for bit
+    F4, E4, Eg4 = i8.build_orbitals(C, 2.0)
+    assert np.array_equal(F4, F3) and E4 == E3
+    i8.set_int8_pack(False)
     import ctypes
     from napmo_amd.system import cext
     chunk = ctypes.c_long()
@@ -196,3 +203,7 @@ def test_int8_screening_bit_exact_on_random_molecules(seed, natom, ncycle):
             seed, b['nbasis'], S, st['skipped'], st['kept'] + st['skipped']))
         assert np.array_equal(F, Fo)
         assert fock.screen_stats() == (st['kept'], st['kept'] + st['skipped'])
+        # the kept blocks packed per chunk into tasks: the same contraction bit for bit
+        fock.set_int8_pack(True)
+        assert np.array_equal(fock.contract(v), Fo)
+        fock.set_int8_pack(False)
