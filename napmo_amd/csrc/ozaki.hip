@@ -530,10 +530,7 @@ __global__ void __launch_bounds__(64) k_oz_pack(OzPackArgs p) {
     }
     return it == total;
   };
-  auto emit = [&](unsigned rm) {
-    int t = 0;
-    if (lane == 0) t = atomicAdd(&p.ctr[0], 1);
-    t = __builtin_amdgcn_readfirstlane(t);
+  auto emit = [&](unsigned rm, int t) {
     int *rec = p.ptasks + (size_t)t * OZ_PTASK;
     if (lane == 0) {
       rec[0] = c;
@@ -544,19 +541,31 @@ __global__ void __launch_bounds__(64) k_oz_pack(OzPackArgs p) {
     }
     fits(rm, rec + 2 + OZ_MAXROWS);
   };
+  // up to 3 blocks per wave always fit (a wave then never needs a fourth column), so the
+  // wave filling is simulated only past OZ_WAVES * OZ_MAXCOL kept blocks
+  // the row groups first, then one atomic for the chunk's task slots
+  unsigned grp[OZ_MAXROWS];
+  int ngrp = 0;
   unsigned cur = 0;
-  int ncur = 0;
+  int ncur = 0, tcur = 0;
   for (int I = 0; I < nb; ++I) {
-    if (!liveI(I)) continue;
+    const unsigned li = liveI(I);
+    if (!li) continue;
     const unsigned cand = cur | (1u << I);
-    if (ncur > 0 && (ncur + 1 > p.rmax || !fits(cand, nullptr))) {
-      emit(cur);
-      cur = 1u << I, ncur = 1;
+    const int tc = tcur + __popc(li);
+    if (ncur > 0 && (ncur + 1 > p.rmax || (tc > OZ_WAVES * OZ_MAXCOL && !fits(cand, nullptr)))) {
+      grp[ngrp++] = cur;
+      cur = 1u << I, ncur = 1, tcur = __popc(li);
     } else {
-      cur = cand, ++ncur;
+      cur = cand, ++ncur, tcur = tc;
     }
   }
-  if (ncur > 0) emit(cur);
+  if (ncur > 0) grp[ngrp++] = cur;
+  if (ngrp == 0) return;
+  int tfirst = 0;
+  if (lane == 0) tfirst = atomicAdd(&p.ctr[0], ngrp);
+  tfirst = __builtin_amdgcn_readfirstlane(tfirst);
+  for (int k = 0; k < ngrp; ++k) emit(grp[k], tfirst + k);
 }
 
 // F[mu][nu] = F[nu][mu] = sum over chunks (in order) of the block element, mu <= nu < n.
