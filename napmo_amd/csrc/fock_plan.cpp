@@ -341,6 +341,10 @@ void FockPlan::ozaki_setup() {
   oz_part.resize((size_t)oz.nchunk * oz.partial_stride());
   oz_ptasks.resize((size_t)oz.nchunk * oz.nb * OZ_PTASK);
   oz_pctr.resize(2);
+  oz_pcache.resize((size_t)oz.nchunk * oz.nb * OZ_PTASK);
+  oz_pccount.resize(oz.nchunk);
+  const std::vector<int> no_cache(oz.nchunk, OZ_NO_CACHE);  // kept until the synchronisation below
+  oz_pcev.upload(no_cache, st);
   std::vector<int> tab = oz.tasks;
   tab.insert(tab.end(), oz.row0.begin(), oz.row0.end());
   oz_tasks.upload(tab, st);
@@ -376,7 +380,8 @@ void FockPlan::ozaki_contract(const double *veff, double *dF, int slot) {
   mark(slot, 7);
   if (oz_pack)
     launch_ozaki_fock_packed(oz, oz_planes.get(), oz_eA.get(), oz_maxd1.get(), oz_vs.get(), oz_screen,
-                             oz_part.get(), oz_ptasks.get(), oz_pctr.get(), st);
+                             oz_part.get(), oz_ptasks.get(), oz_pctr.get(), oz_pcache.get(),
+                             oz_pccount.get(), oz_pcev.get(), st);
   else
     launch_ozaki_fock(oz, oz_tasks.get(), oz_planes.get(), oz_eA.get(), oz_maxd1.get(), oz_vs.get(),
                       oz_screen, oz_part.get(), st);

@@ -44,6 +44,7 @@ struct FockPlan {
   int oz_screen = 0;       // NAPMO_TUNE_INT8_SCREEN (0: off)
   int oz_pack = 0;         // NAPMO_TUNE_INT8_PACK: kept blocks packed per chunk
   DeviceBuffer<int> oz_ptasks, oz_pctr;  // packed tasks and their two counters
+  DeviceBuffer<int> oz_pcache, oz_pccount, oz_pcev;  // per-chunk packing kept across builds
   DeviceBuffer<int> oz_perm;  // screening order of the functions (empty: identity)
   int oz_wave_blocks = OZ_MAXB;  // NAPMO_TUNE_INT8_WAVE_BLOCKS
   void ozaki_setup();

@@ -429,7 +429,7 @@ __global__ void __launch_bounds__(64 * OZ_WAVES) k_ozaki_fock_packed(OzArgs a) {
     __syncthreads();  // s_task read by all; the previous task's LDS use is over
     if (t >= total) break;
     const int *rec = a.ptasks + (size_t)t * OZ_PTASK;
-    ozaki_task(a, lds, rec[0], rec[1], rec + 2, 0, rec + 2 + OZ_MAXROWS, false);
+    ozaki_task(a, lds, rec[0], rec[1], rec + 2, 0, rec + 2 + OZ_MAXROWS, a.screen > 0);
   }
 }
 
@@ -442,6 +442,7 @@ struct OzPackArgs {
   const double *vs;
   double *partial;
   int *ptasks, *ctr;
+  int *cache, *ccount, *cev;  // per chunk: its last tasks [nb][OZ_PTASK], their count, their eV
   long chunk, pstride;
   int nchunk, nb, screen, lgchunk, rmax;
 };
@@ -501,11 +502,11 @@ __global__ void __launch_bounds__(64) k_oz_pack(OzPackArgs p) {
       }
   }
   // the kept blocks of the rows rm, column by column, rows in order, into the waves' slots
-  // as pack_waves fills them (out: OZ_WAVES * OZ_TASK ints written by lane 0, or NULL to
-  // test the fit)
-  auto fits = [&](unsigned rm, int *out) {
+  // as pack_waves fills them (out and out2: two copies of the OZ_WAVES * OZ_TASK ints, written
+  // by lane 0, or NULL to test the fit)
+  auto fits = [&](unsigned rm, int *out, int *out2) {
     if (out && lane == 0)
-      for (int e = 0; e < OZ_WAVES * OZ_TASK; ++e) out[e] = -1;
+      for (int e = 0; e < OZ_WAVES * OZ_TASK; ++e) out[e] = -1, out2[e] = -1;
     int total = 0;
     for (int J = 0; J < nb; ++J) total += __popc(colmJ(J) & rm);
     int it = 0, J = 0;
@@ -518,28 +519,45 @@ __global__ void __launch_bounds__(64) k_oz_pack(OzPackArgs p) {
         while (mj == 0) mj = colmJ(++J) & rm;  // it < total: a kept block remains
         if (J != last) {
           if (ncol == OZ_MAXCOL) break;
-          if (out && lane == 0) out[w * OZ_TASK + 3 * OZ_MAXB + ncol] = J;
+          if (out && lane == 0) out[w * OZ_TASK + 3 * OZ_MAXB + ncol] = J, out2[w * OZ_TASK + 3 * OZ_MAXB + ncol] = J;
           last = J, ++ncol;
         }
         const int I = __ffs(mj) - 1;
         mj &= mj - 1;
-        if (out && lane == 0)
+        if (out && lane == 0) {
           out[w * OZ_TASK + 3 * k] = I, out[w * OZ_TASK + 3 * k + 1] = J, out[w * OZ_TASK + 3 * k + 2] = ncol - 1;
+          out2[w * OZ_TASK + 3 * k] = I, out2[w * OZ_TASK + 3 * k + 1] = J, out2[w * OZ_TASK + 3 * k + 2] = ncol - 1;
+        }
         ++k, ++it;
       }
     }
     return it == total;
   };
-  auto emit = [&](unsigned rm, int t) {
-    int *rec = p.ptasks + (size_t)t * OZ_PTASK;
-    if (lane == 0) {
-      rec[0] = c;
-      int nr = 0;
-      for (unsigned b = rm; b; b &= b - 1) rec[2 + nr++] = __ffs(b) - 1;
-      rec[1] = nr;
-      for (int s = nr; s < OZ_MAXROWS; ++s) rec[2 + s] = -1;
+  // a chunk whose |v| exponent has not grown since its tasks were formed reuses them: their
+  // blocks include every block kept now, and the contraction's rule skips the others
+  {
+    const int cev = p.cev[c];
+    if (cev != OZ_NO_CACHE && eV <= cev) {
+      const int ng = p.ccount[c];
+      if (ng == 0) return;
+      int tf = 0;
+      if (lane == 0) tf = atomicAdd(&p.ctr[0], ng);
+      tf = __builtin_amdgcn_readfirstlane(tf);
+      const int *src = p.cache + (size_t)c * nb * OZ_PTASK;
+      int *dst = p.ptasks + (size_t)tf * OZ_PTASK;
+      for (int e = lane; e < ng * OZ_PTASK; e += 64) dst[e] = src[e];
+      return;
     }
-    fits(rm, rec + 2 + OZ_MAXROWS);
+  }
+  auto emit = [&](unsigned rm, int *rec, int *rec2) {  // the task into both records
+    if (lane == 0) {
+      rec[0] = rec2[0] = c;
+      int nr = 0;
+      for (unsigned b = rm; b; b &= b - 1, ++nr) rec[2 + nr] = rec2[2 + nr] = __ffs(b) - 1;
+      rec[1] = rec2[1] = nr;
+      for (int s = nr; s < OZ_MAXROWS; ++s) rec[2 + s] = rec2[2 + s] = -1;
+    }
+    fits(rm, rec + 2 + OZ_MAXROWS, rec2 + 2 + OZ_MAXROWS);
   };
   // up to 3 blocks per wave always fit (a wave then never needs a fourth column), so the
   // wave filling is simulated only past OZ_WAVES * OZ_MAXCOL kept blocks
@@ -553,7 +571,7 @@ __global__ void __launch_bounds__(64) k_oz_pack(OzPackArgs p) {
     if (!li) continue;
     const unsigned cand = cur | (1u << I);
     const int tc = tcur + __popc(li);
-    if (ncur > 0 && (ncur + 1 > p.rmax || (tc > OZ_WAVES * OZ_MAXCOL && !fits(cand, nullptr)))) {
+    if (ncur > 0 && (ncur + 1 > p.rmax || (tc > OZ_WAVES * OZ_MAXCOL && !fits(cand, nullptr, nullptr)))) {
       grp[ngrp++] = cur;
       cur = 1u << I, ncur = 1, tcur = __popc(li);
     } else {
@@ -561,11 +579,14 @@ __global__ void __launch_bounds__(64) k_oz_pack(OzPackArgs p) {
     }
   }
   if (ncur > 0) grp[ngrp++] = cur;
+  // the tasks into the build's list and into this chunk's cache slots
+  if (lane == 0) p.ccount[c] = ngrp, p.cev[c] = eV;
   if (ngrp == 0) return;
   int tfirst = 0;
   if (lane == 0) tfirst = atomicAdd(&p.ctr[0], ngrp);
   tfirst = __builtin_amdgcn_readfirstlane(tfirst);
-  for (int k = 0; k < ngrp; ++k) emit(grp[k], tfirst + k);
+  for (int k = 0; k < ngrp; ++k)
+    emit(grp[k], p.ptasks + (size_t)(tfirst + k) * OZ_PTASK, p.cache + ((size_t)c * nb + k) * OZ_PTASK);
 }
 
 // F[mu][nu] = F[nu][mu] = sum over chunks (in order) of the block element, mu <= nu < n.
@@ -766,7 +787,7 @@ void launch_ozaki_fock(const OzakiLayout &L, const int *d_tasks, const signed ch
 
 void launch_ozaki_fock_packed(const OzakiLayout &L, const signed char *planes, const int *eA,
                               const unsigned char *maxd1, const double *vs, int screen, double *partial,
-                              int *ptasks, int *ctr, hipStream_t s) {
+                              int *ptasks, int *ctr, int *cache, int *ccount, int *cev, hipStream_t s) {
   static bool attr = false;
   if (!attr) {
     // the kernel's own 4-byte task slot is static LDS: leave 16 bytes for it
@@ -783,6 +804,7 @@ void launch_ozaki_fock_packed(const OzakiLayout &L, const signed char *planes, c
   NAPMO_HIP(hipGetLastError());
   OzPackArgs pa;
   pa.eA = eA, pa.vs = vs, pa.partial = partial, pa.ptasks = ptasks, pa.ctr = ctr;
+  pa.cache = cache, pa.ccount = ccount, pa.cev = cev;
   pa.chunk = L.chunk, pa.pstride = L.partial_stride();
   pa.nchunk = L.nchunk, pa.nb = L.nb, pa.screen = screen, pa.lgchunk = lg, pa.rmax = L.pack_rows;
   k_oz_pack<<<L.nchunk, 64, 0, s>>>(pa);

@@ -65,6 +65,7 @@ struct OzakiLayout {
 OzakiLayout ozaki_layout(long N, int n, int ld, int wave_blocks, size_t max_lds, long chunk_target = 0);
 
 constexpr int OZ_ZERO_EXP = -2000;  // chunk exponent of a function that is zero in the chunk
+constexpr int OZ_NO_CACHE = -1000000;  // packing cache: no tasks formed for the chunk yet
 
 // Per-geometry operand: digits of At = sqrt|wbw| Phi. phi64 [Npad x ld] point-major (Npad >= N),
 // wbw [N]; perm [ld] (NULL: identity): operand function f is column perm[f] of phi64; out:
@@ -93,11 +94,13 @@ void launch_ozaki_fock(const OzakiLayout &L, const int *d_tasks, const signed ch
 // The same contraction with the blocks each chunk keeps packed into as few tasks as the LDS and
 // wave slots allow (screen > 0): a packing pass per chunk forms the kept set (the rule above)
 // and writes zeros for the skipped blocks' partials, then a grid of one workgroup per compute
-// unit takes the tasks in turn. ptasks [nchunk * nb][OZ_PTASK] and ctr [2] are scratch. Equal
-// to launch_ozaki_fock at the same threshold, bit for bit.
+// unit takes the tasks in turn. ptasks [nchunk * nb][OZ_PTASK] and ctr [2] are scratch; cache
+// [nchunk * nb][OZ_PTASK], ccount [nchunk] and cev [nchunk] (OZ_NO_CACHE at first) keep each
+// chunk's tasks across builds, reused while the chunk's |v| exponent has not grown. Equal to
+// launch_ozaki_fock at the same threshold, bit for bit.
 void launch_ozaki_fock_packed(const OzakiLayout &L, const signed char *planes, const int *eA,
                               const unsigned char *maxd1, const double *vs, int screen, double *partial,
-                              int *ptasks, int *ctr, hipStream_t s);
+                              int *ptasks, int *ctr, int *cache, int *ccount, int *cev, hipStream_t s);
 // F [n x n] (symmetric) = the fixed-order sum over chunks of the partials; element (mu, nu) of
 // the operand order is F[perm[mu]][perm[nu]] (perm NULL: identity).
 void launch_ozaki_reduce(const OzakiLayout &L, const double *partial, const int *perm, double *F,

@@ -203,7 +203,13 @@ def test_int8_screening_bit_exact_on_random_molecules(seed, natom, ncycle):
             seed, b['nbasis'], S, st['skipped'], st['kept'] + st['skipped']))
         assert np.array_equal(F, Fo)
         assert fock.screen_stats() == (st['kept'], st['kept'] + st['skipped'])
-        # the kept blocks packed per chunk into tasks: the same contraction bit for bit
+        # the kept blocks packed per chunk into tasks: the same contraction bit for bit, on
+        # the first build, on a repeat (tasks reused), after the potential grows (tasks formed
+        # again) and after it shrinks back (the grown tasks reused, the extra blocks skipped)
+        F_big = fock.contract(8.0 * v)
         fock.set_int8_pack(True)
+        assert np.array_equal(fock.contract(v), Fo)
+        assert np.array_equal(fock.contract(v), Fo)
+        assert np.array_equal(fock.contract(8.0 * v), F_big)
         assert np.array_equal(fock.contract(v), Fo)
         fock.set_int8_pack(False)
