@@ -154,8 +154,9 @@ def test_batch64_one_launch_per_phase(batch64, fp64_builds):
 def test_batch64_orbital_form(batch64, fp64_builds):
     # the orbital form over 4 streams (P = 2 C C^T from the same seeded orbitals): the batch
     # equals the plans' own orbital-form builds bit for bit, eagerly and from three graph
-    # replays, for FP64, int8 and int8 with its contraction and density screened at 2^-50
-    # (every molecule's 1/2 tr(P F) within 1e-6 Eh of the FP64 P-form build)
+    # replays, for FP64, int8 and int8 with its contraction and density screened at 2^-50 and
+    # the contraction's kept blocks packed into tasks (every molecule's 1/2 tr(P F) within
+    # 1e-6 Eh of the FP64 P-form build)
     import napmo_amd
     from napmo_amd import workloads
     from napmo_amd.device import DeviceArray
@@ -168,6 +169,7 @@ def test_batch64_orbital_form(batch64, fp64_builds):
             for f in plans:
                 f.set_screen(50)
                 f.set_density_screen(50)
+                f.set_int8_pack(True)   # packed tasks, kept across builds and graph replays
         rng = np.random.default_rng(5)
         Cs = [workloads.occupied_orbitals(rng, f.overlap(), m['nocc']) for f, m in zip(plans, mols)]
         assert all(np.allclose(2.0 * C @ C.T, P, atol=1e-12) for C, P in zip(Cs, Ps))
