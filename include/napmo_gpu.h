@@ -467,11 +467,12 @@ enum {
                                     by at most (ld / 16) 2^-S; keeps a copy of Phi with its
                                     columns reordered (compact functions first); 0 = off
                                     (default). Not a launch shape: rho changes within that bound */
-  NAPMO_TUNE_INT8_PACK = 13 /* int8 mode: 1 = per build, the blocks each chunk keeps (all, or
-                               those NAPMO_TUNE_INT8_SCREEN keeps) are packed into as few
-                               workgroup tasks as the LDS and wave slots allow, taken in turn by
-                               one workgroup per compute unit; 0 = fixed row groups (default).
-                               A launch shape: the same results bit for bit */
+  NAPMO_TUNE_INT8_PACK = 13 /* int8 mode: 1 = the blocks each chunk keeps (all, or those
+                               NAPMO_TUNE_INT8_SCREEN keeps) are packed into as few workgroup
+                               tasks as the LDS and wave slots allow, taken in turn by one
+                               workgroup per compute unit; a chunk's tasks are formed again
+                               only when its |v| exponent grows; 0 = fixed row groups
+                               (default). A launch shape: the same results bit for bit */
 };
 int napmo_fock_plan_set_tuning(void *plan, int key, long value);
 int napmo_fock_plan_get_tuning(void *plan, int key, long *value);
